@@ -1,0 +1,1057 @@
+// ORACLE — test infrastructure, not product code.
+//
+// CPU FP64 restatement of the reference's moving-mesh hot path.  Only the
+// tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this
+// library, and only as the checker.  Built with -ffp-contract=off so every
+// FP64 expression rounds exactly as the reference's non-FMA x86-64 build.
+//
+// How each part is pinned (tests/test_oracle_*.py):
+//   or_ngrid, or_grid_sort ........ golden vectors written by the reference
+//                                   harness (N_grid, sorted points, permutation);
+//                                   closed form: stable sort by (cell, index).
+//   or_periodic_ghosts ............ golden ghost_ids (reference periodic mesh);
+//                                   invariant: every ghost is a +-1 shift of its
+//                                   source, regions match the buff tests.
+//   or_knn ........................ golden knearest (reference knn::solve) and
+//                                   brute force on small inputs.
+//   clipping / extraction ......... golden meshes (topology bit-exact, volumes,
+//                                   normals, areas); closed forms on BCC and
+//                                   hexagonal lattices; sum of volumes = box;
+//                                   sum_f A_f n_f = 0; face (i,j) <=> (j,i).
+//   restart ....................... equals the direct build at the larger k.
+//   or_dt_cfl, or_hydro_step(0) ... golden post-step states from the reference
+//                                   hydro loop.
+//   or_hydro_step(1), walls ....... two-cell known answer, F(U,U)=f(U),
+//                                   zero mass flux through a wall, uniform
+//                                   state stays uniform (parity against the
+//                                   reference does not exist for them).
+//   centroids ..................... closed forms on lattices and the
+//                                   divergence identity (no reference output).
+#include "oracle.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+enum Status {
+    triangle_overflow = 0,
+    vertex_overflow = 1,
+    inconsistent_boundary = 2,
+    security_radius_not_reached = 3,
+    success = 4,
+    needs_exact_predicates = 5
+};
+
+struct P4 {
+    double x, y, z, w;
+};
+
+inline P4 sub4(const P4& a, const P4& b) { return {a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w}; }
+inline P4 add4(const P4& a, const P4& b) { return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+inline double dot3(const P4& a, const P4& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline P4 cross3(const P4& a, const P4& b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x, 0};
+}
+// allvars.h:212-239 — the evaluation order of these determinants is part of
+// the bit pattern the reference produces.
+inline double d2(double a11, double a12, double a21, double a22) { return a11 * a22 - a12 * a21; }
+inline double d3(double a11, double a12, double a13, double a21, double a22, double a23, double a31,
+                 double a32, double a33) {
+    return a11 * d2(a22, a23, a32, a33) - a21 * d2(a12, a13, a32, a33) + a31 * d2(a12, a13, a22, a23);
+}
+inline double d4(double a11, double a12, double a13, double a14, double a21, double a22, double a23,
+                 double a24, double a31, double a32, double a33, double a34, double a41, double a42,
+                 double a43, double a44) {
+    double m12 = a21 * a12 - a11 * a22;
+    double m13 = a31 * a12 - a11 * a32;
+    double m14 = a41 * a12 - a11 * a42;
+    double m23 = a31 * a22 - a21 * a32;
+    double m24 = a41 * a22 - a21 * a42;
+    double m34 = a41 * a32 - a31 * a42;
+    double m123 = m23 * a13 - m13 * a23 + m12 * a33;
+    double m124 = m24 * a13 - m14 * a23 + m12 * a43;
+    double m134 = m34 * a13 - m14 * a33 + m13 * a43;
+    double m234 = m34 * a23 - m24 * a33 + m23 * a43;
+    return m234 * a14 - m134 * a24 + m124 * a34 - m123 * a44;
+}
+inline double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
+inline double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
+
+const uint8_t END = 255;
+
+// ------------------------------------------------------------------ grid
+// knn.cpp:287-301
+inline int cell_of(int dim, int N, const double* p) {
+    int i = (int)std::floor(p[0] * (double)N);
+    int j = (int)std::floor(p[1] * (double)N);
+    i = std::max(0, std::min(i, N - 1));
+    j = std::max(0, std::min(j, N - 1));
+    if (dim == 2) return i + j * N;
+    int k = (int)std::floor(p[2] * (double)N);
+    k = std::max(0, std::min(k, N - 1));
+    return i + j * N + k * N * N;
+}
+
+inline double dist2(int dim, const double* a, const double* b) {
+    double dx = a[0] - b[0], dy = a[1] - b[1];
+    if (dim == 2) return dx * dx + dy * dy;
+    double dz = a[2] - b[2];
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// ------------------------------------------------------------------ cell
+// cell.cpp:42-411: the convex cell as planes + vertices (plane triples).
+struct Cell {
+    int dim, max_p, max_t;
+    const double* pts;  // sorted points [n][dim]
+    int seed_id;
+    P4 seed;
+    int status;
+    int nb_v, nb_t, nb_r;
+    uint8_t first_b;
+    std::vector<P4> plane;
+    std::vector<int> pvid;
+    std::vector<uint8_t> tri;  // 3 bytes per vertex (third unused in 2D)
+    std::vector<uint8_t> bnext;
+
+    Cell(int dim_, int maxp, int maxt, const double* p, int sid)
+        : dim(dim_), max_p(maxp), max_t(maxt), pts(p), seed_id(sid) {
+        plane.assign(max_p, P4{0, 0, 0, 0});
+        pvid.assign(max_p, -1);
+        tri.assign(3 * (size_t)max_t, 0);
+        bnext.assign(max_p, END);
+        first_b = END;
+        status = success;
+        seed = point(sid);
+        const double eps = 1e-14;
+        const double lo = -eps, hi = 1.0 + eps;
+        plane[0] = {1.0, 0.0, 0.0, -lo};
+        plane[1] = {-1.0, 0.0, 0.0, hi};
+        plane[2] = {0.0, 1.0, 0.0, -lo};
+        plane[3] = {0.0, -1.0, 0.0, hi};
+        if (dim == 2) {
+            const uint8_t t0[4][2] = {{2, 0}, {1, 2}, {3, 1}, {0, 3}};
+            for (int t = 0; t < 4; t++) set_tri(t, t0[t][0], t0[t][1], 0);
+            nb_v = 4;
+            nb_t = 4;
+        } else {
+            plane[4] = {0.0, 0.0, 1.0, -lo};
+            plane[5] = {0.0, 0.0, -1.0, hi};
+            const uint8_t t0[8][3] = {{2, 5, 0}, {5, 3, 0}, {1, 5, 2}, {5, 1, 3},
+                                      {4, 2, 0}, {4, 0, 3}, {2, 4, 1}, {4, 3, 1}};
+            for (int t = 0; t < 8; t++) set_tri(t, t0[t][0], t0[t][1], t0[t][2]);
+            nb_v = 6;
+            nb_t = 8;
+        }
+        nb_r = 0;
+    }
+
+    P4 point(int id) const {
+        const double* f = pts + (size_t)dim * id;
+        return {f[0], f[1], dim == 3 ? f[2] : 0.0, 1.0};
+    }
+    uint8_t& tp(int t, int e) { return tri[3 * (size_t)t + e]; }
+    void set_tri(int t, uint8_t a, uint8_t b, uint8_t c) {
+        tp(t, 0) = a;
+        tp(t, 1) = b;
+        tp(t, 2) = c;
+    }
+    void swap_tri(int a, int b) {
+        for (int e = 0; e < 3; e++) std::swap(tp(a, e), tp(b, e));
+    }
+    bool refs(int t, int p) {
+        for (int d = 0; d < dim; d++)
+            if (tp(t, d) == p) return true;
+        return false;
+    }
+
+    // cell.cpp:212-226
+    int new_halfplane(int vid) {
+        if (nb_v >= max_p) {
+            status = vertex_overflow;
+            return -1;
+        }
+        P4 B = point(vid);
+        P4 dir = sub4(seed, B);
+        P4 ave2 = add4(seed, B);
+        double dot = dot3(ave2, dir);
+        plane[nb_v] = {dir.x, dir.y, dir.z, -dot / 2.0};
+        pvid[nb_v] = vid;
+        nb_v++;
+        return nb_v - 1;
+    }
+
+    // cell.cpp:229-275
+    bool conflict(int t, const P4& eqn) {
+        const P4& p1 = plane[tp(t, 0)];
+        const P4& p2 = plane[tp(t, 1)];
+        double det, eps;
+        if (dim == 2) {
+            det = d3(p1.x, p2.x, eqn.x, p1.y, p2.y, eqn.y, p1.w, p2.w, eqn.w);
+            double mx = std::max({std::fabs(p1.x), std::fabs(p2.x), std::fabs(eqn.x)});
+            double my = std::max({std::fabs(p1.y), std::fabs(p2.y), std::fabs(eqn.y)});
+            double mw = std::max({std::fabs(p1.w), std::fabs(p2.w), std::fabs(eqn.w)});
+            double mm = std::max({mx, my, mw});
+            eps = 1e-14 * mx * my * mw;
+            eps *= mm;
+        } else {
+            const P4& p3 = plane[tp(t, 2)];
+            det = d4(p1.x, p2.x, p3.x, eqn.x, p1.y, p2.y, p3.y, eqn.y, p1.z, p2.z, p3.z, eqn.z, p1.w,
+                     p2.w, p3.w, eqn.w);
+            double mx = std::max({std::fabs(p1.x), std::fabs(p2.x), std::fabs(p3.x), std::fabs(eqn.x)});
+            double my = std::max({std::fabs(p1.y), std::fabs(p2.y), std::fabs(p3.y), std::fabs(eqn.y)});
+            double mz = std::max({std::fabs(p1.z), std::fabs(p2.z), std::fabs(p3.z), std::fabs(eqn.z)});
+            eps = 1e-14 * mx * my * mz;
+            double mm = std::max(std::max(mx, my), mz);
+            eps *= (mm * mm);
+        }
+        if (std::fabs(det) < eps) status = needs_exact_predicates;
+        return det > 0.0;
+    }
+
+    // cell.cpp:278-397
+    void compute_boundary() {
+        for (int i = 0; i < max_p; i++) bnext[i] = END;
+        first_b = END;
+        if (dim == 2) {
+            std::vector<int> cnt(max_p, 0);
+            for (int r = 0; r < nb_r; r++) {
+                cnt[tp(nb_t + r, 0)]++;
+                cnt[tp(nb_t + r, 1)]++;
+            }
+            uint8_t bl[2] = {0, 0};
+            int nbl = 0;
+            for (int p = 0; p < nb_v; p++)
+                if (cnt[p] == 1 && nbl < 2) bl[nbl++] = (uint8_t)p;
+            if (nbl != 2) {
+                status = inconsistent_boundary;
+                return;
+            }
+            first_b = bl[0];
+            bnext[bl[0]] = bl[1];
+            bnext[bl[1]] = bl[0];
+            return;
+        }
+        int nb_iter = 0;
+        int t = nb_t;
+        while (nb_r > 0) {
+            if (nb_iter++ > 100) {
+                status = inconsistent_boundary;
+                return;
+            }
+            bool in_border[3], opp[3];
+            for (int e = 0; e < 3; e++) in_border[e] = (bnext[tp(t, e)] != END);
+            for (int e = 0; e < 3; e++) opp[e] = (bnext[tp(t, (e + 1) % 3)] == tp(t, e));
+            bool simple = true;
+            for (int e = 0; e < 3; e++)
+                if (!opp[e] && !opp[(e + 1) % 3] && in_border[(e + 1) % 3]) simple = false;
+            if (!opp[0] && !opp[1] && !opp[2]) {
+                if (first_b == END) {
+                    for (int e = 0; e < 3; e++) bnext[tp(t, e)] = tp(t, (e + 1) % 3);
+                    first_b = tp(t, 0);
+                } else {
+                    simple = false;
+                }
+            }
+            if (!simple) {
+                t++;
+                if (t == nb_t + nb_r) t = nb_t;
+                continue;
+            }
+            for (int e = 0; e < 3; e++)
+                if (!opp[e]) bnext[tp(t, e)] = tp(t, (e + 1) % 3);
+            for (int e = 0; e < 3; e++) {
+                if (opp[e] && opp[(e + 1) % 3]) {
+                    uint8_t q = tp(t, (e + 1) % 3);
+                    if (first_b == q) first_b = bnext[q];
+                    bnext[q] = END;
+                }
+            }
+            swap_tri(t, nb_t + nb_r - 1);
+            t = nb_t;
+            nb_r--;
+        }
+    }
+
+    // cell.cpp:400-418
+    void new_vertex(uint8_t i, uint8_t j, uint8_t k) {
+        if (nb_t + 1 >= max_t) {
+            status = triangle_overflow;
+            return;
+        }
+        if (dim == 2) {
+            double rw = d2(plane[i].x, plane[i].y, plane[j].x, plane[j].y);
+            if (rw > 0)
+                set_tri(nb_t, j, i, 0);
+            else
+                set_tri(nb_t, i, j, 0);
+        } else {
+            set_tri(nb_t, i, j, k);
+        }
+        nb_t++;
+    }
+
+    // cell.cpp:164-208
+    void clip(int vid) {
+        int cur = new_halfplane(vid);
+        if (status == vertex_overflow) return;
+        P4 eqn = plane[cur];
+        nb_r = 0;
+        int i = 0;
+        while (i < nb_t) {
+            if (conflict(i, eqn)) {
+                nb_t--;
+                swap_tri(i, nb_t);
+                nb_r++;
+            } else {
+                i++;
+            }
+        }
+        if (status == needs_exact_predicates) return;
+        if (nb_r == 0) {
+            nb_v--;
+            return;
+        }
+        compute_boundary();
+        if (status != success) return;
+        if (first_b == END) return;
+        uint8_t cir = first_b;
+        do {
+            new_vertex((uint8_t)cur, cir, dim == 3 ? bnext[cir] : 0);
+            if (status != success) return;
+            cir = bnext[cir];
+        } while (cir != first_b);
+    }
+
+    // cell.cpp:442-465
+    P4 vertex(int t, bool divide = true) {
+        const P4& a = plane[tp(t, 0)];
+        const P4& b = plane[tp(t, 1)];
+        P4 r;
+        if (dim == 2) {
+            r.x = -d2(a.w, a.y, b.w, b.y);
+            r.y = -d2(a.x, a.w, b.x, b.w);
+            r.z = 0;
+            r.w = d2(a.x, a.y, b.x, b.y);
+            if (divide) return {r.x / r.w, r.y / r.w, 0, 1};
+        } else {
+            const P4& c = plane[tp(t, 2)];
+            r.x = -d3(a.w, a.y, a.z, b.w, b.y, b.z, c.w, c.y, c.z);
+            r.y = -d3(a.x, a.w, a.z, b.x, b.w, b.z, c.x, c.w, c.z);
+            r.z = -d3(a.x, a.y, a.w, b.x, b.y, b.w, c.x, c.y, c.w);
+            r.w = d3(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z);
+            if (divide) return {r.x / r.w, r.y / r.w, r.z / r.w, 1};
+        }
+        return r;
+    }
+
+    // cell.cpp:423-438
+    bool radius_reached(const P4& last) {
+        double vd = 0;
+        for (int i = 0; i < nb_t; i++) {
+            P4 pc = vertex(i);
+            P4 df = sub4(pc, seed);
+            vd = dmax(dot3(df, df), vd);
+        }
+        P4 df = sub4(last, seed);
+        return dot3(df, df) > 4 * vd;
+    }
+};
+
+// Extracted geometry of one cell, faces in plane order (cell.cpp:470-524).
+struct CellOut {
+    double volume = 0;
+    double centroid[3] = {0, 0, 0};
+    std::vector<int> nbr;
+    std::vector<double> normal, area, fcen;
+};
+
+// cell.cpp:529-552 — 2D area from atan2-sorted vertices.
+double area2d(const std::vector<P4>& v, int nt) {
+    double cx = 0, cy = 0;
+    for (int i = 0; i < nt; i++) {
+        cx += v[i].x;
+        cy += v[i].y;
+    }
+    cx /= nt;
+    cy /= nt;
+    std::vector<int> order(nt);
+    for (int i = 0; i < nt; i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+        return std::atan2(v[a].y - cy, v[a].x - cx) < std::atan2(v[b].y - cy, v[b].x - cx);
+    });
+    double a = 0;
+    for (int i = 0; i < nt; i++) {
+        int j = (i + 1) % nt;
+        a += v[order[i]].x * v[order[j]].y;
+        a -= v[order[j]].x * v[order[i]].y;
+    }
+    return std::fabs(a) / 2.0;
+}
+
+// cell.cpp:470-685 plus centroids (no reference counterpart: the cell
+// centroid is the volume-weighted mean of the cone/triangle centroids of the
+// fan, the face centroid the area-weighted mean of the fan triangles).
+void extract(Cell& c, CellOut& out) {
+    const int dim = c.dim;
+    std::vector<P4> vtx(c.nb_t);
+    for (int i = 0; i < c.nb_t; i++) vtx[i] = c.vertex(i, true);
+    double vol = 0;
+    double cx = 0, cy = 0, cz = 0;
+    if (dim == 2) out.volume = area2d(vtx, c.nb_t);
+    for (int p = 0; p < c.nb_v; p++) {
+        std::vector<int> idx;
+        for (int i = 0; i < c.nb_t; i++)
+            if (c.refs(i, p)) idx.push_back(i);
+        if ((int)idx.size() < dim) continue;
+        std::vector<P4> fv;
+        if (dim == 2) {
+            for (int i : idx) fv.push_back(vtx[i]);
+        } else {
+            // cell.cpp:584-617 — order by adjacency
+            std::vector<int> ord{idx[0]};
+            std::vector<bool> used(idx.size(), false);
+            used[0] = true;
+            for (size_t step = 1; step < idx.size(); step++) {
+                int last = ord.back();
+                uint8_t oth[2];
+                int cnt = 0;
+                for (int d = 0; d < 3; d++) {
+                    uint8_t pl = c.tp(last, d);
+                    if (pl != (uint8_t)p) oth[cnt++] = pl;
+                }
+                bool found = false;
+                for (size_t j = 0; j < idx.size() && !found; j++) {
+                    if (used[j]) continue;
+                    for (int o = 0; o < 2; o++) {
+                        if (c.refs(idx[j], oth[o])) {
+                            ord.push_back(idx[j]);
+                            used[j] = true;
+                            found = true;
+                            break;
+                        }
+                    }
+                }
+                if (!found) break;
+            }
+            if ((int)ord.size() < 3) continue;
+            for (int i : ord) fv.push_back(vtx[i]);
+        }
+        const P4& pe = c.plane[p];
+        double nl = std::sqrt(dot3(pe, pe));
+        double n3[3] = {pe.x / nl, pe.y / nl, pe.z / nl};
+        double meas = 0;
+        double fc[3] = {0, 0, 0};
+        if (dim == 2) {
+            double dx = fv[1].x - fv[0].x, dy = fv[1].y - fv[0].y;
+            meas = std::sqrt(dx * dx + dy * dy);
+            fc[0] = 0.5 * (fv[0].x + fv[1].x);
+            fc[1] = 0.5 * (fv[0].y + fv[1].y);
+            // triangle (seed, v0, v1): area-weighted centroid contribution
+            double ta = 0.5 * std::fabs((fv[0].x - c.seed.x) * (fv[1].y - c.seed.y) -
+                                        (fv[1].x - c.seed.x) * (fv[0].y - c.seed.y));
+            cx += ta * (c.seed.x + fv[0].x + fv[1].x) / 3.0;
+            cy += ta * (c.seed.y + fv[0].y + fv[1].y) / 3.0;
+        } else {
+            {
+                P4 e1 = sub4(fv[1], fv[0]);
+                P4 e2 = sub4(fv[2], fv[0]);
+                P4 cr = cross3(e1, e2);
+                P4 cen{0, 0, 0, 0};
+                for (const P4& q : fv) {
+                    cen.x += q.x;
+                    cen.y += q.y;
+                    cen.z += q.z;
+                }
+                cen.x /= fv.size();
+                cen.y /= fv.size();
+                cen.z /= fv.size();
+                P4 outw = sub4(cen, c.seed);
+                if (dot3(cr, outw) < 0) std::reverse(fv.begin(), fv.end());
+            }
+            P4 v0 = fv[0];
+            for (size_t i = 1; i + 1 < fv.size(); i++) {
+                P4 cr = cross3(sub4(fv[i], v0), sub4(fv[i + 1], v0));
+                double ta = 0.5 * std::sqrt(cr.x * cr.x + cr.y * cr.y + cr.z * cr.z);
+                meas += ta;
+                fc[0] += ta * (v0.x + fv[i].x + fv[i + 1].x) / 3.0;
+                fc[1] += ta * (v0.y + fv[i].y + fv[i + 1].y) / 3.0;
+                fc[2] += ta * (v0.z + fv[i].z + fv[i + 1].z) / 3.0;
+            }
+            for (size_t i = 1; i + 1 < fv.size(); i++) {
+                P4 a = sub4(fv[0], c.seed);
+                P4 b = sub4(fv[i], c.seed);
+                P4 cc = sub4(fv[i + 1], c.seed);
+                double tv = dot3(a, cross3(b, cc)) / 6.0;
+                vol += tv;
+                // tetra (seed, v0, vi, vi+1) centroid
+                cx += tv * (c.seed.x + fv[0].x + fv[i].x + fv[i + 1].x) / 4.0;
+                cy += tv * (c.seed.y + fv[0].y + fv[i].y + fv[i + 1].y) / 4.0;
+                cz += tv * (c.seed.z + fv[0].z + fv[i].z + fv[i + 1].z) / 4.0;
+            }
+            if (meas > 0) {
+                fc[0] /= meas;
+                fc[1] /= meas;
+                fc[2] /= meas;
+            }
+        }
+        out.nbr.push_back(c.pvid[p]);
+        out.normal.insert(out.normal.end(), n3, n3 + 3);
+        out.area.push_back(meas);
+        out.fcen.insert(out.fcen.end(), fc, fc + 3);
+    }
+    if (dim == 3) out.volume = std::fabs(vol);
+    double wv = (dim == 3) ? vol : out.volume;
+    if (wv != 0) {
+        out.centroid[0] = cx / wv;
+        out.centroid[1] = cy / wv;
+        out.centroid[2] = (dim == 3) ? cz / wv : 0.0;
+    }
+}
+
+// ------------------------------------------------------------------ mesh
+struct Mesh {
+    int dim = 3, k = 0, N_grid = 0;
+    int64_t n_seeds = 0, n_hydro = 0;
+    std::vector<double> sorted;
+    std::vector<uint32_t> perm, knn;
+    // reference VMesh layout, original order
+    std::vector<double> seeds, volumes, normal, area, ccen, fcen;
+    std::vector<int64_t> fcount, fptr, ghost_ids;
+    std::vector<int32_t> nbr, status, kcert;
+    int64_t n_failed = 0;
+};
+
+// knn.cpp:16-117 + 119-204
+void grid_sort(int dim, const double* pts, int64_t n, int N, double* sorted, uint32_t* perm,
+               int32_t* counters, int32_t* ptrs) {
+    int64_t ncell = (dim == 2) ? (int64_t)N * N : (int64_t)N * N * N;
+    std::vector<int32_t> cnt(ncell, 0), start(ncell, 0);
+    std::vector<int> cell(n);
+    for (int64_t i = 0; i < n; i++) {
+        cell[i] = cell_of(dim, N, pts + dim * i);
+        cnt[cell[i]]++;
+    }
+    int32_t run = 0;
+    for (int64_t c = 0; c < ncell; c++) {
+        start[c] = cnt[c] > 0 ? run : 0;  // knn.cpp:176-180 leaves empty cells at 0
+        run += cnt[c];
+    }
+    std::vector<int32_t> fill(ncell, 0);
+    for (int64_t i = 0; i < n; i++) {
+        int c = cell[i];
+        int32_t pos = start[c] + fill[c]++;
+        std::memcpy(sorted + dim * (int64_t)pos, pts + dim * i, sizeof(double) * dim);
+        perm[pos] = (uint32_t)i;
+    }
+    if (counters) std::memcpy(counters, cnt.data(), sizeof(int32_t) * ncell);
+    if (ptrs) std::memcpy(ptrs, start.data(), sizeof(int32_t) * ncell);
+}
+
+// knn.cpp:237-347: ring search, heap keyed by (dist^2, original index).
+void knn_search(int dim, const double* sorted, int64_t n, int N, const int32_t* cnt,
+                const int32_t* start, const uint32_t* perm, int k, uint32_t* out) {
+#pragma omp parallel
+    {
+        std::vector<double> hd(k);
+        std::vector<uint32_t> hi(k), ho(k);
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t q = 0; q < n; q++) {
+            const double* p = sorted + dim * q;
+            int c = cell_of(dim, N, p);
+            int ci = c % N, cj = (c / N) % N, ck = (dim == 3) ? c / (N * N) : 0;
+            int size = 0;
+            auto less = [&](double da, uint32_t oa, double db, uint32_t ob) {
+                return da < db || (da == db && oa < ob);
+            };
+            auto sift = [&](int j) {
+                for (;;) {
+                    int l = 2 * j + 1, r = l + 1, m = j;
+                    if (l < size && less(hd[m], ho[m], hd[l], ho[l])) m = l;
+                    if (r < size && less(hd[m], ho[m], hd[r], ho[r])) m = r;
+                    if (m == j) return;
+                    std::swap(hd[j], hd[m]);
+                    std::swap(hi[j], hi[m]);
+                    std::swap(ho[j], ho[m]);
+                    j = m;
+                }
+            };
+            auto push = [&](double d, uint32_t idx) {
+                uint32_t o = perm[idx];
+                if (size < k) {
+                    // sift-up insert
+                    int j = size++;
+                    hd[j] = d;
+                    hi[j] = idx;
+                    ho[j] = o;
+                    while (j > 0) {
+                        int par = (j - 1) / 2;
+                        if (!less(hd[par], ho[par], hd[j], ho[j])) break;
+                        std::swap(hd[j], hd[par]);
+                        std::swap(hi[j], hi[par]);
+                        std::swap(ho[j], ho[par]);
+                        j = par;
+                    }
+                } else if (less(d, o, hd[0], ho[0])) {
+                    hd[0] = d;
+                    hi[0] = idx;
+                    ho[0] = o;
+                    sift(0);
+                }
+            };
+            int maxring = N;  // all rings: the grid is covered once ring >= N
+            for (int ring = 0; ring <= maxring; ring++) {
+                if (ring >= 1 && size == k) {
+                    double md = (double)(ring - 1) / (double)N;
+                    if (hd[0] < md * md) break;
+                }
+                int kl = (dim == 3) ? -ring : 0, kh = (dim == 3) ? ring : 0;
+                for (int dk = kl; dk <= kh; dk++)
+                    for (int dj = -ring; dj <= ring; dj++)
+                        for (int di = -ring; di <= ring; di++) {
+                            int cheb = std::max(std::abs(di), std::max(std::abs(dj), std::abs(dk)));
+                            if (cheb != ring) continue;
+                            int ii = ci + di, jj = cj + dj, kk = ck + dk;
+                            if (ii < 0 || jj < 0 || kk < 0 || ii >= N || jj >= N || (dim == 3 && kk >= N))
+                                continue;
+                            int cc = ii + jj * N + kk * N * N;
+                            for (int32_t s = start[cc]; s < start[cc] + cnt[cc]; s++) {
+                                if (s == q) continue;
+                                push(dist2(dim, p, sorted + dim * (int64_t)s), (uint32_t)s);
+                            }
+                        }
+            }
+            // ascending order
+            std::vector<int> ord(size);
+            for (int i = 0; i < size; i++) ord[i] = i;
+            std::sort(ord.begin(), ord.end(),
+                      [&](int a, int b) { return less(hd[a], ho[a], hd[b], ho[b]); });
+            for (int i = 0; i < k; i++) out[q * k + i] = i < size ? hi[ord[i]] : UINT32_MAX;
+        }
+    }
+}
+
+int ngrid(int dim, int64_t n) {
+    return std::max(1, (int)std::round(std::pow((float)n / 3.1f, 1.0f / (float)dim)));
+}
+
+// periodic_mesh.cpp:63-126.  Region order of the reference: 'A' = (0,1),
+// 'L' = (0,buff) shifted by +1, 'H' = (1-buff,1) shifted by -1.
+const char* REG2[8] = {"LA", "AH", "HA", "AL", "LL", "LH", "HH", "HL"};
+const char* REG3[26] = {"AAL", "LAA", "AHA", "HAA", "ALA", "AAH", "ALL", "LAL", "AHL",
+                        "HAL", "LLA", "LHA", "HHA", "HLA", "LAH", "AHH", "HAH", "ALH",
+                        "LLL", "LHL", "HHL", "HLL", "LLH", "LHH", "HHH", "HLH"};
+
+int64_t periodic_ghosts(int dim, const double* pts, int64_t n, double buff, double* out,
+                        int64_t* gid) {
+    const char** reg = dim == 2 ? REG2 : REG3;
+    int nreg = dim == 2 ? 8 : 26;
+    int64_t ng = 0;
+    std::vector<double> tmp;  // ghosts, unscaled
+    for (int64_t i = 0; i < n; i++) {
+        const double* p = pts + dim * i;
+        for (int r = 0; r < nreg; r++) {
+            bool in = true;
+            double sh[3] = {0, 0, 0};
+            for (int d = 0; d < dim && in; d++) {
+                char c = reg[r][d];
+                double lo = c == 'H' ? 1. - buff : 0.;
+                double hi = c == 'L' ? buff : 1.;
+                in = (p[d] > lo && p[d] < hi);
+                sh[d] = c == 'L' ? 1. : (c == 'H' ? -1. : 0.);
+            }
+            if (!in) continue;
+            for (int d = 0; d < dim; d++) tmp.push_back(p[d] + sh[d]);
+            gid[ng++] = i;
+        }
+    }
+    double scale = 1. / (1. + (2 * buff));
+    for (int64_t i = 0; i < n; i++)
+        for (int d = 0; d < dim; d++) out[dim * i + d] = scale * (pts[dim * i + d] - 0.5) + 0.5;
+    for (int64_t g = 0; g < ng; g++)
+        for (int d = 0; d < dim; d++) out[dim * (n + g) + d] = scale * (tmp[dim * g + d] - 0.5) + 0.5;
+    return n + ng;
+}
+
+// voronoi.cpp:115-156 for one seed with its neighbour row; returns kcert.
+int run_cell(int dim, int maxp, int maxt, const double* sorted, const uint32_t* row, int k, int sid,
+             CellOut* out, int* status) {
+    Cell c(dim, maxp, maxt, sorted, sid);
+    int kc = 0;
+    for (int v = 0; v < k; v++) {
+        uint32_t z = row[v];
+        c.clip((int)z);
+        if (c.radius_reached(c.point((int)z))) {
+            kc = v + 1;
+            break;
+        }
+        if (c.status != success) break;
+    }
+    if (!c.radius_reached(c.point((int)row[k - 1]))) c.status = security_radius_not_reached;
+    if (c.status != success) kc = 0;
+    *status = c.status;
+    if (c.status == success && out) extract(c, *out);
+    return kc;
+}
+
+Mesh* build(int dim, const double* in_pts, int64_t n, int boundary, double buff, int k, int maxp,
+            int maxt, int restart) {
+    Mesh* m = new Mesh();
+    m->dim = dim;
+    std::vector<double> pts;
+    int64_t ntot = n;
+    if (boundary == 0) {
+        int64_t cap = n;
+        for (int d = 0; d < dim; d++) cap *= 3;
+        pts.resize(cap * dim);
+        m->ghost_ids.resize(cap);
+        ntot = periodic_ghosts(dim, in_pts, n, buff, pts.data(), m->ghost_ids.data());
+        pts.resize(ntot * dim);
+        m->ghost_ids.resize(ntot - n);
+        m->n_hydro = n;
+    } else {
+        pts.assign(in_pts, in_pts + n * dim);
+        m->n_hydro = (boundary == 1) ? n : 0;
+    }
+    m->n_seeds = ntot;
+    int N = ngrid(dim, ntot);
+    m->N_grid = N;
+    m->sorted.resize(ntot * dim);
+    m->perm.resize(ntot);
+    int64_t ncell = (dim == 2) ? (int64_t)N * N : (int64_t)N * N * N;
+    std::vector<int32_t> cnt(ncell), start(ncell);
+    grid_sort(dim, pts.data(), ntot, N, m->sorted.data(), m->perm.data(), cnt.data(), start.data());
+    int kmax = k;
+    if (restart) {
+        const int sched[] = {16, 24, 32, 48, 64, 96, 128, 192, 256};
+        for (int s : sched)
+            if (s > kmax) kmax = s;
+        if (kmax > ntot - 1) kmax = (int)(ntot - 1);
+        if (kmax < k) kmax = k;
+    }
+    m->k = kmax;
+    m->knn.resize((size_t)ntot * kmax);
+    knn_search(dim, m->sorted.data(), ntot, N, cnt.data(), start.data(), m->perm.data(), kmax,
+               m->knn.data());
+    std::vector<CellOut> cells(ntot);
+    std::vector<int> st(ntot), kc(ntot);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t s = 0; s < ntot; s++) {
+        // Without restart the neighbour list is cut at k (reference semantics).
+        // With restart, certification at a prefix of the longer list gives the
+        // same cell as a restart with that longer list.
+        int kk = restart ? kmax : k;
+        std::vector<uint32_t> row(m->knn.begin() + s * kmax, m->knn.begin() + s * kmax + kk);
+        kc[s] = run_cell(dim, maxp, maxt, m->sorted.data(), row.data(), kk, (int)s, &cells[s], &st[s]);
+    }
+    // unpermute (voronoi.cpp:212-333) and, for periodic, unscale (periodic_mesh.cpp:170-190)
+    std::vector<int64_t> o2s(ntot);
+    for (int64_t s = 0; s < ntot; s++) o2s[m->perm[s]] = s;
+    m->seeds.assign(ntot * 3, 0.0);
+    m->volumes.assign(ntot, 0.0);
+    m->ccen.assign(ntot * 3, 0.0);
+    m->fcount.assign(ntot, 0);
+    m->fptr.assign(ntot, 0);
+    m->status.assign(ntot, 0);
+    m->kcert.assign(ntot, 0);
+    int64_t cursor = 0;
+    for (int64_t o = 0; o < ntot; o++) {
+        int64_t s = o2s[o];
+        const CellOut& c = cells[s];
+        m->status[o] = st[s];
+        m->kcert[o] = kc[s];
+        if (st[s] != success) m->n_failed++;
+        m->fptr[o] = cursor;
+        m->fcount[o] = (int64_t)c.nbr.size();
+        m->volumes[o] = c.volume;
+        if (st[s] == success)
+            for (int d = 0; d < dim; d++) m->seeds[3 * o + d] = m->sorted[dim * s + d];
+        for (int d = 0; d < 3; d++) m->ccen[3 * o + d] = c.centroid[d];
+        for (size_t f = 0; f < c.nbr.size(); f++) {
+            int nb = c.nbr[f];
+            m->nbr.push_back(nb >= 0 ? (int32_t)m->perm[nb] : nb);
+            for (int d = 0; d < 3; d++) m->normal.push_back(c.normal[3 * f + d]);
+            m->area.push_back(c.area[f]);
+            for (int d = 0; d < 3; d++) m->fcen.push_back(c.fcen[3 * f + d]);
+        }
+        cursor += (int64_t)c.nbr.size();
+    }
+    if (boundary == 0) {
+        double scale = 1. + (2 * buff);
+        double vs = dim == 2 ? scale * scale : scale * scale * scale;
+        double as = dim == 2 ? scale : scale * scale;
+        for (int64_t i = 0; i < ntot; i++) {
+            for (int d = 0; d < dim; d++) {
+                m->seeds[3 * i + d] = (m->seeds[3 * i + d] - 0.5) * scale + 0.5;
+                m->ccen[3 * i + d] = (m->ccen[3 * i + d] - 0.5) * scale + 0.5;
+            }
+            m->volumes[i] = vs * m->volumes[i];
+        }
+        for (size_t f = 0; f < m->area.size(); f++) {
+            m->area[f] = as * m->area[f];
+            for (int d = 0; d < dim; d++) m->fcen[3 * f + d] = (m->fcen[3 * f + d] - 0.5) * scale + 0.5;
+        }
+    }
+    return m;
+}
+
+// ------------------------------------------------------------------ hydro
+struct Prim {
+    double rho = 0;
+    double v[3] = {0, 0, 0};
+    double E = 0;
+};
+struct Geom {
+    double n[3], m[3], p[3];
+};
+
+// riemann.cpp:108-130
+Geom geom_of(const double* nr) {
+    Geom g;
+    double nn = std::sqrt(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
+    g.n[0] = nr[0] / nn;
+    g.n[1] = nr[1] / nn;
+    g.n[2] = nr[2] / nn;
+    if (g.n[0] != 0.0 || g.n[1] != 0.0) {
+        g.m[0] = -g.n[1];
+        g.m[1] = g.n[0];
+        g.m[2] = 0.0;
+    } else {
+        g.m[0] = 1.0;
+        g.m[1] = 0.0;
+        g.m[2] = 0.0;
+    }
+    double mm = std::sqrt(g.m[0] * g.m[0] + g.m[1] * g.m[1] + g.m[2] * g.m[2]);
+    g.m[0] /= mm;
+    g.m[1] /= mm;
+    g.m[2] /= mm;
+    g.p[0] = g.n[1] * g.m[2] - g.n[2] * g.m[1];
+    g.p[1] = g.n[2] * g.m[0] - g.n[0] * g.m[2];
+    g.p[2] = g.n[0] * g.m[1] - g.n[1] * g.m[0];
+    return g;
+}
+
+// riemann.cpp:80-106
+void to_face(int dim, Prim& s, const Geom& g) {
+    double vx = s.v[0], vy = s.v[1];
+    if (dim == 2) {
+        s.v[0] = vx * g.n[0] + vy * g.n[1];
+        s.v[1] = vx * g.m[0] + vy * g.m[1];
+    } else {
+        double vz = s.v[2];
+        s.v[0] = vx * g.n[0] + vy * g.n[1] + vz * g.n[2];
+        s.v[1] = vx * g.m[0] + vy * g.m[1] + vz * g.m[2];
+        s.v[2] = vx * g.p[0] + vy * g.p[1] + vz * g.p[2];
+    }
+}
+void from_face(int dim, Prim& s, const Geom& g) {
+    double vx = s.v[0], vy = s.v[1];
+    if (dim == 2) {
+        s.v[0] = vx * g.n[0] + vy * g.m[0];
+        s.v[1] = vx * g.n[1] + vy * g.m[1];
+    } else {
+        double vz = s.v[2];
+        s.v[0] = vx * g.n[0] + vy * g.m[0] + vz * g.p[0];
+        s.v[1] = vx * g.n[1] + vy * g.m[1] + vz * g.p[1];
+        s.v[2] = vx * g.n[2] + vy * g.m[2] + vz * g.p[2];
+    }
+}
+// riemann.cpp:151-157
+double pressure(int dim, double gam, const Prim& s) {
+    if (dim == 2) return (gam - 1) * (s.E - (0.5 * s.rho * (s.v[0] * s.v[0] + s.v[1] * s.v[1])));
+    return (gam - 1) * (s.E - (0.5 * s.rho * (s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2])));
+}
+// riemann.cpp:132-149
+Prim euler_flux(int dim, double gam, const Prim& s) {
+    double P = pressure(dim, gam, s);
+    Prim f;
+    f.rho = s.rho * s.v[0];
+    f.v[0] = s.rho * s.v[0] * s.v[0] + P;
+    f.v[1] = s.rho * s.v[0] * s.v[1];
+    if (dim == 3) f.v[2] = s.rho * s.v[0] * s.v[2];
+    f.E = (s.E + P) * s.v[0];
+    return f;
+}
+
+// riemann.cpp:7-45.  wall: the right state is the mirror of the left one in
+// the face frame (normal velocity negated), so SL == -SR bit for bit.
+Prim hll(int dim, double gam, Prim L, Prim R, const double* nrm, bool wall) {
+    Geom g = geom_of(nrm);
+    to_face(dim, L, g);
+    if (wall) {
+        R = L;
+        R.v[0] = -L.v[0];
+    } else {
+        to_face(dim, R, g);
+    }
+    Prim fl = euler_flux(dim, gam, L), fr = euler_flux(dim, gam, R);
+    double cl = std::sqrt((gam * pressure(dim, gam, L)) / L.rho);
+    double cr = std::sqrt((gam * pressure(dim, gam, R)) / R.rho);
+    double SL = dmin(L.v[0] - cl, R.v[0] - cr);
+    double SR = dmax(L.v[0] + cl, R.v[0] + cr);
+    Prim F;
+    if (SL >= 0) {
+        F = fl;
+    } else if (SL < 0 && SR > 0) {
+        F.rho = (SR * fl.rho - SL * fr.rho + SL * SR * (R.rho - L.rho)) / (SR - SL);
+        for (int d = 0; d < dim; d++)
+            F.v[d] = (SR * fl.v[d] - SL * fr.v[d] + SL * SR * (R.rho * R.v[d] - L.rho * L.v[d])) / (SR - SL);
+        F.E = (SR * fl.E - SL * fr.E + SL * SR * (R.E - L.E)) / (SR - SL);
+    } else if (SR <= 0) {
+        F = fr;
+    }
+    from_face(dim, F, g);
+    return F;
+}
+
+}  // namespace
+
+// ======================================================================= C-ABI
+extern "C" {
+
+int or_ngrid(int dim, int64_t n) { return ngrid(dim, n); }
+
+int64_t or_periodic_ghosts(int dim, const double* pts, int64_t n, double buff, double* out_pts,
+                           int64_t* ghost_ids) {
+    return periodic_ghosts(dim, pts, n, buff, out_pts, ghost_ids);
+}
+
+int or_grid_sort(int dim, const double* pts, int64_t n, int N_grid, double* sorted, uint32_t* perm,
+                 int32_t* counters, int32_t* ptrs) {
+    grid_sort(dim, pts, n, N_grid, sorted, perm, counters, ptrs);
+    return 0;
+}
+
+int or_knn(int dim, const double* sorted, int64_t n, int N_grid, const int32_t* counters,
+           const int32_t* ptrs, const uint32_t* perm, int k, uint32_t* out) {
+    knn_search(dim, sorted, n, N_grid, counters, ptrs, perm, k, out);
+    return 0;
+}
+
+void* or_mesh_build(int dim, const double* pts, int64_t n, int boundary, double buff, int k,
+                    int max_p, int max_t, int restart) {
+    if ((dim != 2 && dim != 3) || n < 2 || k < 1 || k > n - 1 || max_p > 255 || max_t > 255)
+        return nullptr;
+    return build(dim, pts, n, boundary, buff, k, max_p, max_t, restart);
+}
+
+void or_mesh_free(void* h) { delete (Mesh*)h; }
+
+void or_mesh_sizes(void* h, int64_t* s) {
+    Mesh* m = (Mesh*)h;
+    s[0] = m->n_seeds;
+    s[1] = m->n_hydro;
+    s[2] = (int64_t)m->nbr.size();
+    s[3] = m->n_failed;
+    s[4] = m->N_grid;
+    s[5] = m->k;
+}
+
+int or_mesh_get(void* h, const char* name, void* dst) {
+    Mesh* m = (Mesh*)h;
+    auto cp = [&](const auto& v) {
+        if (!v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+        return 0;
+    };
+    std::string s(name);
+    if (s == "seeds") return cp(m->seeds);
+    if (s == "volumes") return cp(m->volumes);
+    if (s == "face_counts") return cp(m->fcount);
+    if (s == "face_ptr") return cp(m->fptr);
+    if (s == "neighbor_cell") return cp(m->nbr);
+    if (s == "face_normal") return cp(m->normal);
+    if (s == "face_area") return cp(m->area);
+    if (s == "ghost_ids") return cp(m->ghost_ids);
+    if (s == "status") return cp(m->status);
+    if (s == "kcert") return cp(m->kcert);
+    if (s == "cell_centroid") return cp(m->ccen);
+    if (s == "face_centroid") return cp(m->fcen);
+    if (s == "sorted_points") return cp(m->sorted);
+    if (s == "permutation") return cp(m->perm);
+    if (s == "knearest") return cp(m->knn);
+    return -1;
+}
+
+double or_dt_cfl(void* h, double gam, double cfl, const double* rho, const double* vel, const double* E) {
+    Mesh* m = (Mesh*)h;
+    const int dim = m->dim;
+    double mn = 1e100;
+    for (int64_t i = 0; i < m->n_hydro; i++) {
+        Prim s;
+        s.rho = rho[i];
+        s.E = E[i];
+        for (int d = 0; d < dim; d++) s.v[d] = vel[dim * i + d];
+        double P = pressure(dim, gam, s);
+        double c = std::sqrt(gam * P / s.rho);
+        double R, va;
+        if (dim == 2) {
+            R = std::sqrt(m->volumes[i] / M_PI);
+            va = std::sqrt(s.v[0] * s.v[0] + s.v[1] * s.v[1]);
+        } else {
+            R = std::cbrt(3.0 * m->volumes[i] / (4.0 * M_PI));
+            va = std::sqrt(s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]);
+        }
+        double dti = cfl * (R / (c + va));
+        if (dti < mn) mn = dti;
+    }
+    return mn;
+}
+
+int or_hydro_step(void* h, double gam, double dt, int flux_mode, double* rho, double* vel, double* E) {
+    Mesh* m = (Mesh*)h;
+    const int dim = m->dim;
+    const int64_t nh = m->n_hydro;
+    std::vector<double> nr(nh), nv(nh * dim), nE(nh);
+    int bad = 0;
+#pragma omp parallel for reduction(+ : bad)
+    for (int64_t i = 0; i < nh; i++) {
+        Prim si;
+        si.rho = rho[i];
+        si.E = E[i];
+        for (int d = 0; d < dim; d++) si.v[d] = vel[dim * i + d];
+        Prim tot;
+        for (int64_t j = 0; j < m->fcount[i]; j++) {
+            int64_t f = m->fptr[i] + j;
+            int nb = m->nbr[f];
+            bool wall = nb < 0;
+            Prim sj;
+            if (!wall) {
+                int64_t idx = nb;
+                if (idx >= nh) idx = m->ghost_ids[idx - nh];
+                sj.rho = rho[idx];
+                sj.E = E[idx];
+                for (int d = 0; d < dim; d++) sj.v[d] = vel[dim * idx + d];
+            }
+            double n3[3] = {m->normal[3 * f], m->normal[3 * f + 1], m->normal[3 * f + 2]};
+            if (flux_mode == 1) {
+                n3[0] = -n3[0];
+                n3[1] = -n3[1];
+                n3[2] = -n3[2];
+            }
+            Prim F = hll(dim, gam, si, sj, n3, wall);
+            double a = m->area[f];
+            tot.rho += F.rho * a;
+            for (int d = 0; d < dim; d++) tot.v[d] += F.v[d] * a;
+            tot.E += F.E * a;
+        }
+        if (m->fcount[i] == 0) bad++;
+        double frac = dt / m->volumes[i];
+        nr[i] = si.rho - frac * tot.rho;
+        double inv = 1.0 / nr[i];
+        for (int d = 0; d < dim; d++) nv[dim * i + d] = (si.rho * si.v[d] - frac * tot.v[d]) * inv;
+        nE[i] = si.E - frac * tot.E;
+    }
+    std::memcpy(rho, nr.data(), sizeof(double) * nh);
+    std::memcpy(vel, nv.data(), sizeof(double) * nh * dim);
+    std::memcpy(E, nE.data(), sizeof(double) * nh);
+    return bad;
+}
+
+}  // extern "C"

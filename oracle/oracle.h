@@ -1,0 +1,77 @@
+/* ORACLE — test infrastructure, not product code.
+ *
+ * A from-scratch CPU FP64 restatement of the reference's per-timestep hot path
+ * (grid KNN, meshless Voronoi clipping with security-radius certificate, HLL
+ * finite-volume update and CFL reduce).  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it, and only as the checker.
+ *
+ * Every function cites the reference file:line it restates.  Arrays are
+ * caller-owned; points are row-major [n][dim] doubles.
+ *
+ * Pinning (what checks each function): see the header of oracle.cpp.
+ */
+#ifndef PGPU_ORACLE_H
+#define PGPU_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* knn.cpp:101 — grid resolution, float arithmetic as in the reference. */
+int or_ngrid(int dim, int64_t n);
+
+/* periodic_mesh.cpp:63-126 — ghost points (point-major, region order of the
+ * reference) then scale into [0,1]^dim.  out_pts capacity: n*3^dim rows;
+ * ghost_ids capacity n*(3^dim-1).  Returns n_total (hydro + ghosts). */
+int64_t or_periodic_ghosts(int dim, const double* pts, int64_t n, double buff,
+                           double* out_pts, int64_t* ghost_ids);
+
+/* knn.cpp:119-222 — stable counting sort into the uniform grid. */
+int or_grid_sort(int dim, const double* pts, int64_t n, int N_grid, double* sorted,
+                 uint32_t* perm, int32_t* counters, int32_t* ptrs);
+
+/* knn.cpp:237-347 — k nearest neighbours (sorted-index space), ascending by
+ * (FP64 dist^2, original index).  out: n*k. */
+int or_knn(int dim, const double* sorted, int64_t n, int N_grid, const int32_t* counters,
+           const int32_t* ptrs, const uint32_t* perm, int k, uint32_t* out);
+
+/* Mesh build.  boundary: 0 = periodic ghosts (periodic_mesh.cpp:44), 1 = the
+ * unit box as reflective walls (no ghosts, no scaling), 2 = plain
+ * compute_mesh on the given points (voronoi.cpp:17).  k: neighbours per seed.
+ * restart != 0: seeds that fail the certificate are recomputed from a longer
+ * neighbour list (k schedule 16,24,32,48,64,96,128,192,256 above k).
+ * Returns an opaque handle or NULL. */
+void* or_mesh_build(int dim, const double* pts, int64_t n, int boundary, double buff, int k,
+                    int max_p, int max_t, int restart);
+void or_mesh_free(void* h);
+
+/* sizes: [0]=n_seeds [1]=n_hydro [2]=num_faces [3]=n_failed [4]=N_grid [5]=k_used */
+void or_mesh_sizes(void* h, int64_t* sizes);
+
+/* Copy one array out of a mesh handle (reference VMesh layout, original
+ * order, voronoi.cpp:212-333):
+ *   "seeds" f8[n_seeds*3]  "volumes" f8[n_seeds]  "face_counts" i8[n_seeds]
+ *   "face_ptr" i8[n_seeds] "neighbor_cell" i4[F] "face_normal" f8[F*3]
+ *   "face_area" f8[F] "ghost_ids" i8[n_seeds-n_hydro] "status" i4[n_seeds]
+ *   "kcert" i4[n_seeds] (1 + index of the certifying neighbour, 0 if none)
+ *   "cell_centroid" f8[n_seeds*3] "face_centroid" f8[F*3]
+ *   "sorted_points" f8[n_seeds*dim] "permutation" u4[n_seeds] "knearest" u4[n_seeds*k]
+ * Returns 0 on success, -1 for an unknown name. */
+int or_mesh_get(void* h, const char* name, void* dst);
+
+/* finite_volume_solver.cpp:151-186 — CFL time step over hydro cells. */
+double or_dt_cfl(void* h, double gamma, double cfl, const double* rho, const double* vel,
+                 const double* E);
+
+/* finite_volume_solver.cpp:43-123 + riemann.cpp:7-124 — one static-mesh HLL
+ * step, in place.  flux_mode 0 = the reference's face orientation (stored
+ * inward normal, cell i as left state); 1 = outward normal (physical).
+ * Box faces (neighbour -1) are reflective walls. */
+int or_hydro_step(void* h, double gamma, double dt, int flux_mode, double* rho, double* vel,
+                  double* E);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,129 @@
+"""ORACLE (test infrastructure): ctypes binding of liboracle.so.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker.  The library is the CPU FP64
+restatement of the reference path (see oracle.cpp for what pins each part).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
+        L = C.CDLL(path)
+        dp, i64p, u32p, i32p = (np.ctypeslib.ndpointer(dtype=t, flags="C") for t in
+                                (np.float64, np.int64, np.uint32, np.int32))
+        L.or_ngrid.argtypes = [C.c_int, C.c_int64]
+        L.or_ngrid.restype = C.c_int
+        L.or_periodic_ghosts.argtypes = [C.c_int, dp, C.c_int64, C.c_double, dp, i64p]
+        L.or_periodic_ghosts.restype = C.c_int64
+        L.or_grid_sort.argtypes = [C.c_int, dp, C.c_int64, C.c_int, dp, u32p, i32p, i32p]
+        L.or_knn.argtypes = [C.c_int, dp, C.c_int64, C.c_int, i32p, i32p, u32p, C.c_int, u32p]
+        L.or_mesh_build.argtypes = [C.c_int, dp, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_int,
+                                    C.c_int, C.c_int]
+        L.or_mesh_build.restype = C.c_void_p
+        L.or_mesh_free.argtypes = [C.c_void_p]
+        L.or_mesh_sizes.argtypes = [C.c_void_p, i64p]
+        L.or_mesh_get.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
+        L.or_dt_cfl.argtypes = [C.c_void_p, C.c_double, C.c_double, dp, dp, dp]
+        L.or_dt_cfl.restype = C.c_double
+        L.or_hydro_step.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, dp, dp, dp]
+        _LIB = L
+    return _LIB
+
+
+def ngrid(dim, n):
+    return lib().or_ngrid(dim, n)
+
+
+def periodic_ghosts(pts, buff):
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    n, dim = pts.shape
+    cap = n * 3 ** dim
+    out = np.empty((cap, dim))
+    gid = np.empty(cap, dtype=np.int64)
+    ntot = lib().or_periodic_ghosts(dim, pts, n, buff, out, gid)
+    return out[:ntot].copy(), gid[: ntot - n].copy()
+
+
+def grid_sort(pts, N):
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    n, dim = pts.shape
+    sorted_ = np.empty_like(pts)
+    perm = np.empty(n, dtype=np.uint32)
+    ncell = N ** dim
+    cnt = np.empty(ncell, dtype=np.int32)
+    start = np.empty(ncell, dtype=np.int32)
+    lib().or_grid_sort(dim, pts, n, N, sorted_, perm, cnt, start)
+    return sorted_, perm, cnt, start
+
+
+def knn(pts, k):
+    """Reference knn::init + knn::solve on pts: (N, sorted, perm, knearest[n,k])."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    n, dim = pts.shape
+    N = ngrid(dim, n)
+    s, perm, cnt, start = grid_sort(pts, N)
+    out = np.empty(n * k, dtype=np.uint32)
+    lib().or_knn(dim, s, n, N, cnt, start, perm, k, out)
+    return N, s, perm, out.reshape(n, k)
+
+
+_I8 = ("face_counts", "face_ptr", "ghost_ids")
+_I4 = ("neighbor_cell", "status", "kcert")
+
+
+class Mesh:
+    """Oracle mesh in the reference VMesh layout (original order)."""
+
+    BOUNDARY = {"periodic": 0, "reflective": 1, "none": 2}
+
+    def __init__(self, pts, boundary="periodic", buff=0.04, k=32, max_p=64, max_t=96, restart=False):
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        n, dim = pts.shape
+        self.dim = dim
+        self.h = lib().or_mesh_build(dim, pts, n, self.BOUNDARY[boundary], buff, k, max_p, max_t,
+                                     int(restart))
+        if not self.h:
+            raise ValueError("oracle mesh build rejected its arguments")
+        s = np.zeros(6, dtype=np.int64)
+        lib().or_mesh_sizes(self.h, s)
+        self.n_seeds, self.n_hydro, self.num_faces, self.n_failed, self.N_grid, self.k = map(int, s)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_mesh_free(self.h)
+            self.h = None
+
+    def get(self, name):
+        n, F, d = self.n_seeds, self.num_faces, self.dim
+        shapes = {"seeds": (n, 3), "volumes": (n,), "face_counts": (n,), "face_ptr": (n,),
+                  "neighbor_cell": (F,), "face_normal": (F, 3), "face_area": (F,),
+                  "ghost_ids": (n - self.n_hydro if self.n_hydro else 0,), "status": (n,),
+                  "kcert": (n,), "cell_centroid": (n, 3), "face_centroid": (F, 3),
+                  "sorted_points": (n, d), "permutation": (n,), "knearest": (n, self.k)}
+        dt = (np.int64 if name in _I8 else np.int32 if name in _I4 else
+              np.uint32 if name in ("permutation", "knearest") else np.float64)
+        out = np.zeros(shapes[name], dtype=dt)
+        if lib().or_mesh_get(self.h, name.encode(), out.ctypes.data) != 0:
+            raise KeyError(name)
+        return out
+
+    def dt_cfl(self, gamma, cfl, rho, vel, E):
+        return lib().or_dt_cfl(self.h, gamma, cfl, np.ascontiguousarray(rho), np.ascontiguousarray(vel),
+                               np.ascontiguousarray(E))
+
+    def hydro_step(self, gamma, dt, rho, vel, E, flux_mode=0):
+        """In-place static-mesh HLL step; returns the number of faceless hydro cells."""
+        return lib().or_hydro_step(self.h, gamma, dt, flux_mode, rho, vel, E)
