@@ -1,0 +1,182 @@
+"""Python view of the C ABI: context, KNN problem, Voronoi mesh, hydro state.
+
+Names and argument meaning follow the reference's C++ interfaces
+(src/knn/knn.h:36-62, src/voronoi/voronoi.h:64-81, src/voronoi/periodic_mesh.h:22,
+src/hydro/finite_volume_solver.h:20-27); every call goes through libpgpu.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._native import BOUNDARY, FLUX, MeshInfo, MeshParams, VMeshView, check, lib
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().pg_device_count(C.byref(n)))
+    return n.value
+
+
+class Context:
+    """One GPU and one stream (replaces the reference's gpu* shim, allvars.h:92-129)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().pg_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def sync(self):
+        check(lib().pg_ctx_sync(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class KnnProblem:
+    """knn::init / knn::solve / get_points / get_knearest / get_permutation."""
+
+    def __init__(self, ctx: Context, pts: np.ndarray):
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        self.ctx, self.n, self.dim = ctx, pts.shape[0], pts.shape[1]
+        h = C.c_void_p()
+        check(lib().pg_knn_init(ctx.h, self.dim, _ptr(pts), self.n, C.byref(h)))
+        self.h = h
+        self.k = 0
+        n64, N, k = C.c_int64(), C.c_int(), C.c_int()
+        check(lib().pg_knn_info(h, C.byref(n64), C.byref(N), C.byref(k)))
+        self.N_grid = N.value
+
+    def solve(self, k: int):
+        check(lib().pg_knn_solve(self.ctx.h, self.h, k))
+        self.k = k
+
+    def get_points(self):
+        out = np.empty((self.n, self.dim))
+        check(lib().pg_knn_get_points(self.ctx.h, self.h, _ptr(out)))
+        return out
+
+    def get_knearest(self):
+        out = np.empty((self.n, self.k), dtype=np.uint32)
+        check(lib().pg_knn_get_knearest(self.ctx.h, self.h, _ptr(out)))
+        return out
+
+    def get_permutation(self):
+        out = np.empty(self.n, dtype=np.uint32)
+        check(lib().pg_knn_get_permutation(self.ctx.h, self.h, _ptr(out)))
+        return out
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().pg_knn_free(C.byref(self.h))
+            self.h = None
+
+    def __del__(self):
+        self.free()
+
+
+def mesh_params(dim=3, boundary="periodic", buff=0.04, k=32, max_p=64, max_t=96, restart=True,
+                k_max=256) -> MeshParams:
+    p = MeshParams()
+    lib().pg_mesh_params_default(C.byref(p))
+    p.dim, p.boundary, p.buff, p.k = dim, BOUNDARY[boundary], buff, k
+    p.max_p, p.max_t, p.restart, p.k_max = max_p, max_t, int(restart), k_max
+    return p
+
+
+class Mesh:
+    """voronoi::compute_mesh / compute_periodic_mesh and the VMesh copy-out."""
+
+    def __init__(self, ctx: Context, pts: np.ndarray, boundary="periodic", buff=0.04, k=32, max_p=64,
+                 max_t=96, restart=True, k_max=256):
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        self.ctx, self.dim = ctx, pts.shape[1]
+        prm = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max)
+        h = C.c_void_p()
+        check(lib().pg_mesh_compute(ctx.h, C.byref(prm), _ptr(pts), pts.shape[0], C.byref(h)))
+        self.h = h
+        info = MeshInfo()
+        check(lib().pg_mesh_info(h, C.byref(info)))
+        self.info = info
+        self.n_seeds, self.n_hydro, self.num_faces = info.n_seeds, info.n_hydro, info.num_faces
+        self.n_failed, self.n_restarted, self.N_grid, self.k_used = (
+            info.n_failed, info.n_restarted, info.N_grid, info.k_used)
+
+    def copy_out(self):
+        """Reference VMesh arrays in original order (dict of numpy arrays)."""
+        n, F = self.n_seeds, self.num_faces
+        ng = n - self.n_hydro if self.n_hydro else 0
+        out = {
+            "cell_ids": np.empty(n, np.int64), "seeds": np.empty((n, 3)), "volumes": np.empty(n),
+            "face_counts": np.empty(n, np.int64), "face_ptr": np.empty(n, np.int64),
+            "neighbor_cell": np.empty(F, np.int32), "face_normal": np.empty((F, 3)),
+            "face_area": np.empty(F), "ghost_ids": np.empty(max(ng, 0), np.int64),
+            "status": np.empty(n, np.int32), "kcert": np.empty(n, np.int32),
+            "cell_centroid": np.empty((n, 3)), "face_centroid": np.empty((F, 3)),
+        }
+        v = VMeshView()
+        for name, arr in out.items():
+            setattr(v, name, arr.ctypes.data if arr.size else None)
+        check(lib().pg_mesh_copy_out(self.ctx.h, self.h, C.byref(v)))
+        return out
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().pg_mesh_free(C.byref(self.h))
+            self.h = None
+
+    def __del__(self):
+        self.free()
+
+
+class HydroState:
+    """hydro::init / dt_CFL / hydro_step / free_prim on a static mesh."""
+
+    def __init__(self, ctx: Context, mesh: Mesh, rho, vel, E):
+        self.ctx, self.mesh = ctx, mesh
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        self.n, self.dim = rho.shape[0], mesh.dim
+        h = C.c_void_p()
+        check(lib().pg_state_create(ctx.h, mesh.h, _ptr(rho), _ptr(vel), _ptr(E), C.byref(h)))
+        self.h = h
+
+    def dt_cfl(self, gamma, cfl):
+        dt = C.c_double()
+        check(lib().pg_dt_cfl(self.ctx.h, self.mesh.h, self.h, gamma, cfl, C.byref(dt)))
+        return dt.value
+
+    def step(self, gamma, dt, flux="reference"):
+        check(lib().pg_hydro_step(self.ctx.h, self.mesh.h, self.h, gamma, dt, FLUX[flux]))
+
+    def copy_out(self):
+        rho = np.empty(self.n)
+        vel = np.empty((self.n, self.dim))
+        E = np.empty(self.n)
+        check(lib().pg_state_copy_out(self.ctx.h, self.h, _ptr(rho), _ptr(vel), _ptr(E)))
+        return rho, vel, E
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().pg_state_free(C.byref(self.h))
+            self.h = None
+
+    def __del__(self):
+        self.free()

@@ -1,0 +1,672 @@
+// C ABI of libpgpu (include/pgpu.h): device memory, the rebuild sequence and
+// the restart loop.  Host code only; every kernel lives in the .hip files.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pgpu.h"
+#include "common.h"
+#include "hydro.h"
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+void hipchk(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error{PG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)};
+}
+#define HIPCHK(x) hipchk((x), #x)
+
+void argchk(bool ok, const char* what) {
+    if (!ok) throw Error{PG_ERR_ARG, what};
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return PG_OK;
+    } catch (const Error& e) {
+        g_err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return PG_ERR_NOMEM;
+    }
+}
+
+// Grow-only device buffer.
+struct DBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <class T>
+    T* get(size_t count) {
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        if (bytes > cap) {
+            if (p) HIPCHK(hipFree(p));
+            p = nullptr;
+            cap = 0;
+            hipError_t e = hipMalloc(&p, bytes);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                throw Error{PG_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+            }
+            cap = bytes;
+        }
+        return (T*)p;
+    }
+    template <class T>
+    T* as() const {
+        return (T*)p;
+    }
+};
+
+// knn.cpp:101, float arithmetic as the reference.
+int ngrid(int dim, int64_t n) {
+    return std::max(1, (int)std::round(std::pow((float)n / 3.1f, 1.0f / (float)dim)));
+}
+
+const int KSCHED[] = {16, 24, 32, 48, 64, 96, 128, 192, 256};
+constexpr int MAXP = 64;
+
+}  // namespace
+
+struct pg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+void use(pg_ctx* c) {
+    argchk(c != nullptr, "null context");
+    HIPCHK(hipSetDevice(c->device));
+}
+
+// Uniform grid + counting sort (knn::init).
+struct Grid {
+    int dim = 3;
+    int64_t n = 0;
+    int N = 0;
+    int64_t ncell = 0;
+    DBuf pts, sorted, perm, cell, counts, starts, fill, tmp, total;
+
+    void build(hipStream_t s) {
+        ncell = (int64_t)N * N * (dim == 3 ? N : 1);
+        int32_t* cnt = counts.get<int32_t>(ncell);
+        int32_t* st = starts.get<int32_t>(ncell);
+        int32_t* fl = fill.get<int32_t>(ncell);
+        int32_t* tp = tmp.get<int32_t>(pg::scan_tmp_elems(std::max(ncell, n)));
+        HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * ncell, s));
+        HIPCHK(hipMemsetAsync(fl, 0, sizeof(int32_t) * ncell, s));
+        pg::launch_cell_count(dim, pts.as<double>(), n, N, cell.get<int32_t>(n), cnt, s);
+        pg::scan_exclusive_i32(cnt, st, ncell, tp, total.get<int32_t>(1), s);
+        uint32_t* pm = perm.get<uint32_t>(n);
+        pg::launch_cell_scatter(n, cell.as<int32_t>(), st, fl, pm, s);
+        pg::launch_cell_sort(ncell, st, cnt, pm, s);
+        pg::launch_gather_points(dim, n, pm, pts.as<double>(), sorted.get<double>((size_t)n * dim), s);
+        HIPCHK(hipGetLastError());
+    }
+};
+
+}  // namespace
+
+struct pg_knn {
+    Grid g;
+    DBuf knn;
+    int k = 0;
+};
+
+struct pg_mesh {
+    pg_mesh_params prm{};
+    int64_t n_in = 0, n_seeds = 0, n_hydro = 0;
+    Grid g;
+    DBuf in_pts, gcnt, goff, gsrc, gtot;
+    DBuf knn, rows, list, flags, offs, cnt1;
+    // faces (padded, face-slot major) and cells, sorted order
+    DBuf nbr, area, nrm, cen, count, vol, ccen, status, kcert;
+    // hydro tables
+    DBuf hid, area_u, vol_u, o2s, bad;
+    bool finalized = false;
+    pg_mesh_info_t info{};
+
+    pg::FaceArrays faces() {
+        pg::FaceArrays fa;
+        fa.stride = n_seeds;
+        fa.nbr = nbr.as<int32_t>();
+        fa.area = area.as<double>();
+        fa.nrm = nrm.as<double>();
+        fa.cen = cen.as<double>();
+        fa.count = count.as<uint8_t>();
+        fa.vol = vol.as<double>();
+        fa.ccen = ccen.as<double>();
+        fa.status = status.as<int32_t>();
+        fa.kcert = kcert.as<int32_t>();
+        return fa;
+    }
+    double scale() const { return prm.boundary == PG_BOUNDARY_PERIODIC ? 1. + (2 * prm.buff) : 1.0; }
+};
+
+struct pg_state {
+    int dim = 3;
+    int64_t n = 0;
+    DBuf rho[2], vel[2], E[2];
+    int cur = 0;
+};
+
+namespace {
+
+int32_t read_i32(const int32_t* d, hipStream_t s) {
+    int32_t h = 0;
+    HIPCHK(hipMemcpyAsync(&h, d, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return h;
+}
+
+// Collect the sorted ids whose status equals code; returns the count.
+int64_t collect(pg_mesh* m, int code, uint32_t* list, hipStream_t s) {
+    int64_t n = m->n_seeds;
+    int32_t* fl = m->flags.get<int32_t>(n);
+    int32_t* of = m->offs.get<int32_t>(n);
+    int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(std::max(n, m->g.ncell)));
+    int32_t* c = m->cnt1.get<int32_t>(1);
+    pg::launch_collect_status(n, nullptr, m->status.as<int32_t>(), code, list, fl, of, tp, c, s);
+    HIPCHK(hipGetLastError());
+    return read_i32(c, s);
+}
+
+void run_list(pg_mesh* m, const uint32_t* list, int64_t nq, int k, int big, hipStream_t s) {
+    uint32_t* rows = m->rows.get<uint32_t>((size_t)nq * k);
+    pg::launch_knn(m->prm.dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), m->n_seeds, m->g.N,
+                   m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), list, nq, k, rows, s);
+    pg::launch_voronoi(m->prm.dim, m->g.sorted.as<double>(), m->n_seeds, list, nq, rows, k,
+                       m->prm.max_p, m->prm.max_t, big, m->faces(), s);
+    HIPCHK(hipGetLastError());
+}
+
+void mesh_build(pg_ctx* c, pg_mesh* m, const double* pts_host, int64_t n) {
+    hipStream_t s = c->stream;
+    const pg_mesh_params& p = m->prm;
+    const int dim = p.dim;
+    m->n_in = n;
+    double* din = m->in_pts.get<double>((size_t)n * dim);
+    HIPCHK(hipMemcpyAsync(din, pts_host, sizeof(double) * n * dim, hipMemcpyHostToDevice, s));
+    int64_t nseeds = n;
+    if (p.boundary == PG_BOUNDARY_PERIODIC) {
+        int32_t* gc = m->gcnt.get<int32_t>(n);
+        int32_t* go = m->goff.get<int32_t>(n);
+        int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(n));
+        int32_t* gt = m->gtot.get<int32_t>(1);
+        pg::launch_ghost_count(dim, din, n, p.buff, gc, s);
+        pg::scan_exclusive_i32(gc, go, n, tp, gt, s);
+        int64_t ng = read_i32(gt, s);
+        nseeds = n + ng;
+        double* gp = m->g.pts.get<double>((size_t)nseeds * dim);
+        pg::launch_ghost_write(dim, din, n, p.buff, go, gp, m->gsrc.get<int64_t>(std::max<int64_t>(ng, 1)), s);
+        m->n_hydro = n;
+    } else {
+        double* gp = m->g.pts.get<double>((size_t)n * dim);
+        HIPCHK(hipMemcpyAsync(gp, din, sizeof(double) * n * dim, hipMemcpyDeviceToDevice, s));
+        m->n_hydro = p.boundary == PG_BOUNDARY_REFLECTIVE ? n : 0;
+    }
+    HIPCHK(hipGetLastError());
+    m->n_seeds = nseeds;
+    argchk(nseeds - 1 >= p.k, "k must be below the number of seeds");
+    m->g.dim = dim;
+    m->g.n = nseeds;
+    m->g.N = ngrid(dim, nseeds);
+    if (p.boundary != PG_BOUNDARY_REFLECTIVE && m->g.N < 16)
+        throw Error{PG_ERR_GRID, "KNN: N_grid < 16 (the reference's minimum mesh size)"};
+    m->g.build(s);
+
+    // face and cell outputs
+    const size_t ns = (size_t)nseeds;
+    m->nbr.get<int32_t>(ns * MAXP);
+    m->area.get<double>(ns * MAXP);
+    m->nrm.get<double>(3 * ns * MAXP);
+    m->cen.get<double>(3 * ns * MAXP);
+    m->count.get<uint8_t>(ns);
+    m->vol.get<double>(ns);
+    m->ccen.get<double>(3 * ns);
+    m->status.get<int32_t>(ns);
+    m->kcert.get<int32_t>(ns);
+    m->finalized = false;
+
+    // round 0: every seed at k
+    int k = p.k;
+    uint32_t* rows = m->knn.get<uint32_t>(ns * k);
+    pg::launch_knn(dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), nseeds, m->g.N,
+                   m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), nullptr, nseeds, k, rows, s);
+    pg::launch_voronoi(dim, m->g.sorted.as<double>(), nseeds, nullptr, nseeds, rows, k, p.max_p, p.max_t,
+                       0, m->faces(), s);
+    HIPCHK(hipGetLastError());
+
+    // restarts: uncertified seeds at the next k; seeds that outgrew the fast
+    // kernel's 64 vertex slots in the 128-slot kernel
+    std::vector<int> sched{k};
+    for (int v : KSCHED)
+        if (v > k && v <= p.k_max && v <= nseeds - 1) sched.push_back(v);
+    size_t ki = 0;
+    int64_t restarted = 0;
+    uint32_t* lst = m->list.get<uint32_t>(ns);
+    for (int round = 0; round < 64; round++) {
+        bool work = false;
+        int64_t nso = collect(m, pg::SLOT_OVERFLOW, lst, s);
+        if (nso > 0) {
+            run_list(m, lst, nso, sched[ki], 1, s);
+            restarted += nso;
+            work = true;
+        }
+        if (p.restart && ki + 1 < sched.size()) {
+            int64_t nsr = collect(m, PG_SECURITY_RADIUS_NOT_REACHED, lst, s);
+            if (nsr > 0) {
+                ki++;
+                run_list(m, lst, nsr, sched[ki], 0, s);
+                restarted += nsr;
+                work = true;
+            }
+        }
+        if (!work) break;
+    }
+    m->info.n_seeds = nseeds;
+    m->info.n_hydro = m->n_hydro;
+    m->info.N_grid = m->g.N;
+    m->info.k_used = sched[ki];
+    m->info.n_restarted = restarted;
+    m->info.n_failed = collect(m, -1, lst, s);
+    // total faces
+    int32_t* co = m->flags.get<int32_t>(ns);
+    int32_t* po = m->offs.get<int32_t>(ns);
+    int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(std::max<int64_t>(nseeds, m->g.ncell)));
+    pg::launch_counts_orig(nseeds, m->g.perm.as<uint32_t>(), m->count.as<uint8_t>(), co, s);
+    pg::scan_exclusive_i32(co, po, nseeds, tp, m->cnt1.get<int32_t>(1), s);
+    m->info.num_faces = read_i32(m->cnt1.as<int32_t>(), s);
+}
+
+void mesh_finalize(pg_ctx* c, pg_mesh* m) {
+    if (m->finalized) return;
+    hipStream_t s = c->stream;
+    const size_t ns = (size_t)m->n_seeds;
+    const double sc = m->scale();
+    const double as = m->prm.dim == 2 ? sc : sc * sc;
+    const double vs = m->prm.dim == 2 ? sc * sc : sc * sc * sc;
+    int32_t* bad = m->bad.get<int32_t>(1);
+    HIPCHK(hipMemsetAsync(bad, 0, sizeof(int32_t), s));
+    pg::launch_finalize(m->n_seeds, m->n_hydro, m->n_seeds, m->g.perm.as<uint32_t>(), m->gsrc.as<int64_t>(),
+                        m->count.as<uint8_t>(), m->nbr.as<int32_t>(), m->area.as<double>(),
+                        m->vol.as<double>(), as, vs, m->hid.get<int32_t>(ns * MAXP),
+                        m->area_u.get<double>(ns * MAXP), m->vol_u.get<double>(ns),
+                        m->o2s.get<int32_t>(std::max<int64_t>(m->n_hydro, 1)), bad, s);
+    HIPCHK(hipGetLastError());
+    if (read_i32(bad, s) != 0)
+        throw Error{PG_ERR_CELLS, "hydro cells without faces (uncertified cells)"};
+    m->finalized = true;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+const char* pg_version(void) { return "pgpu 0.1 (gfx950)"; }
+const char* pg_last_error(void) { return g_err.c_str(); }
+
+int pg_device_count(int* count) {
+    return guarded([&] {
+        argchk(count != nullptr, "null count");
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            n = 0;
+        }
+        *count = n;
+    });
+}
+
+int pg_ctx_create(int device, pg_ctx** out) {
+    return guarded([&] {
+        argchk(out != nullptr, "null out");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+            (void)hipGetLastError();
+            throw Error{PG_ERR_NODEV, "no GPU device visible"};
+        }
+        argchk(device >= 0 && device < n, "device index out of range");
+        HIPCHK(hipSetDevice(device));
+        pg_ctx* c = new pg_ctx();
+        c->device = device;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            hipchk(e, "hipStreamCreate");
+        }
+        *out = c;
+    });
+}
+
+int pg_ctx_destroy(pg_ctx* c) {
+    return guarded([&] {
+        if (!c) return;
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+    });
+}
+
+int pg_ctx_sync(pg_ctx* c) {
+    return guarded([&] {
+        use(c);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+// ------------------------------------------------------------------ KNN
+int pg_knn_init(pg_ctx* c, int dim, const double* pts, int64_t n, pg_knn** out) {
+    return guarded([&] {
+        use(c);
+        argchk(out && pts, "null argument");
+        argchk(dim == 2 || dim == 3, "dim must be 2 or 3");
+        argchk(n >= 2 && n < (int64_t)1 << 31, "n out of range");
+        int N = ngrid(dim, n);
+        if (N < 16) throw Error{PG_ERR_GRID, "KNN: N_grid < 16 (the reference's minimum mesh size)"};
+        pg_knn* k = new pg_knn();
+        try {
+            k->g.dim = dim;
+            k->g.n = n;
+            k->g.N = N;
+            double* d = k->g.pts.get<double>((size_t)n * dim);
+            HIPCHK(hipMemcpyAsync(d, pts, sizeof(double) * n * dim, hipMemcpyHostToDevice, c->stream));
+            k->g.build(c->stream);
+            HIPCHK(hipStreamSynchronize(c->stream));
+        } catch (...) {
+            delete k;
+            throw;
+        }
+        *out = k;
+    });
+}
+
+int pg_knn_solve(pg_ctx* c, pg_knn* k, int kk) {
+    return guarded([&] {
+        use(c);
+        argchk(k != nullptr, "null knn");
+        argchk(kk >= 1 && kk <= 256, "k must be in [1, 256]");
+        uint32_t* o = k->knn.get<uint32_t>((size_t)k->g.n * kk);
+        pg::launch_knn(k->g.dim, k->g.sorted.as<double>(), k->g.perm.as<uint32_t>(), k->g.n, k->g.N,
+                       k->g.starts.as<int32_t>(), k->g.counts.as<int32_t>(), nullptr, k->g.n, kk, o,
+                       c->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        k->k = kk;
+    });
+}
+
+int pg_knn_info(const pg_knn* k, int64_t* n, int* N, int* kk) {
+    return guarded([&] {
+        argchk(k != nullptr, "null knn");
+        if (n) *n = k->g.n;
+        if (N) *N = k->g.N;
+        if (kk) *kk = k->k;
+    });
+}
+
+int pg_knn_get_points(pg_ctx* c, const pg_knn* k, double* out) {
+    return guarded([&] {
+        use(c);
+        argchk(k && out, "null argument");
+        HIPCHK(hipMemcpyAsync(out, k->g.sorted.p, sizeof(double) * k->g.n * k->g.dim, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pg_knn_get_knearest(pg_ctx* c, const pg_knn* k, uint32_t* out) {
+    return guarded([&] {
+        use(c);
+        argchk(k && out, "null argument");
+        argchk(k->k > 0, "knn not solved");
+        HIPCHK(hipMemcpyAsync(out, k->knn.p, sizeof(uint32_t) * k->g.n * k->k, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pg_knn_get_permutation(pg_ctx* c, const pg_knn* k, uint32_t* out) {
+    return guarded([&] {
+        use(c);
+        argchk(k && out, "null argument");
+        HIPCHK(hipMemcpyAsync(out, k->g.perm.p, sizeof(uint32_t) * k->g.n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pg_knn_free(pg_knn** k) {
+    return guarded([&] {
+        if (!k || !*k) return;
+        delete *k;
+        *k = nullptr;
+    });
+}
+
+// -------------------------------------------------------------- Voronoi
+void pg_mesh_params_default(pg_mesh_params* p) {
+    if (!p) return;
+    p->dim = 3;
+    p->boundary = PG_BOUNDARY_PERIODIC;
+    p->buff = 0.04;
+    p->k = 32;
+    p->max_p = 64;
+    p->max_t = 96;
+    p->restart = 1;
+    p->k_max = 256;
+}
+
+int pg_mesh_compute(pg_ctx* c, const pg_mesh_params* prm, const double* pts, int64_t n, pg_mesh** out) {
+    return guarded([&] {
+        use(c);
+        argchk(prm && pts && out, "null argument");
+        argchk(prm->dim == 2 || prm->dim == 3, "dim must be 2 or 3");
+        argchk(prm->boundary >= 0 && prm->boundary <= 2, "unknown boundary");
+        argchk(prm->k >= 1 && prm->k <= 256, "k must be in [1, 256]");
+        argchk(prm->max_p >= 2 * prm->dim + 1 && prm->max_p <= MAXP, "max_p must be in [2*dim+1, 64]");
+        argchk(prm->max_t >= 8 && prm->max_t <= 128, "max_t must be in [8, 128]");
+        argchk(prm->boundary != PG_BOUNDARY_PERIODIC || (prm->buff > 0 && prm->buff < 0.5),
+               "buff must be in (0, 0.5)");
+        argchk(n >= 2 && n < (int64_t)1 << 29, "n out of range");
+        pg_mesh* m = new pg_mesh();
+        try {
+            m->prm = *prm;
+            if (m->prm.k_max < m->prm.k) m->prm.k_max = m->prm.k;
+            mesh_build(c, m, pts, n);
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int pg_mesh_info(const pg_mesh* m, pg_mesh_info_t* info) {
+    return guarded([&] {
+        argchk(m && info, "null argument");
+        *info = m->info;
+    });
+}
+
+int pg_mesh_copy_out(pg_ctx* c, const pg_mesh* mc, pg_vmesh_view* o) {
+    return guarded([&] {
+        use(c);
+        argchk(mc && o, "null argument");
+        pg_mesh* m = const_cast<pg_mesh*>(mc);
+        hipStream_t s = c->stream;
+        const int64_t n = m->n_seeds, F = m->info.num_faces;
+        const int dim = m->prm.dim;
+        const double sc = m->scale();
+        const double as = dim == 2 ? sc : sc * sc;
+        const double vs = dim == 2 ? sc * sc : sc * sc * sc;
+        // face pointers in original order
+        int32_t* co = m->flags.get<int32_t>(n);
+        int32_t* po = m->offs.get<int32_t>(n);
+        int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(std::max<int64_t>(n, m->g.ncell)));
+        pg::launch_counts_orig(n, m->g.perm.as<uint32_t>(), m->count.as<uint8_t>(), co, s);
+        pg::scan_exclusive_i32(co, po, n, tp, m->cnt1.get<int32_t>(1), s);
+        DBuf onbr, onrm, oarea, ocen, oseeds, ovol, ost, okc, occ;
+        int32_t* dnbr = onbr.get<int32_t>(F);
+        double* dnrm = onrm.get<double>(3 * F);
+        double* dar = oarea.get<double>(F);
+        double* dcen = ocen.get<double>(3 * F);
+        pg::launch_pack_faces(n, n, n * MAXP, dim, m->g.perm.as<uint32_t>(), m->count.as<uint8_t>(), po,
+                              m->nbr.as<int32_t>(), m->nrm.as<double>(), m->area.as<double>(),
+                              m->cen.as<double>(), as, sc, dnbr, dnrm, dar, dcen, s);
+        double* dse = oseeds.get<double>(3 * n);
+        double* dvo = ovol.get<double>(n);
+        int32_t* dst = ost.get<int32_t>(n);
+        int32_t* dkc = okc.get<int32_t>(n);
+        double* dcc = occ.get<double>(3 * n);
+        pg::launch_cells_orig(n, n, dim, m->g.perm.as<uint32_t>(), m->g.sorted.as<double>(),
+                              m->status.as<int32_t>(), m->kcert.as<int32_t>(), m->vol.as<double>(),
+                              m->ccen.as<double>(), sc, vs, dse, dvo, dst, dkc, dcc, s);
+        HIPCHK(hipGetLastError());
+        std::vector<int32_t> hcnt(n), hptr(n);
+        HIPCHK(hipMemcpyAsync(hcnt.data(), co, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(hptr.data(), po, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+        if (o->neighbor_cell) HIPCHK(hipMemcpyAsync(o->neighbor_cell, dnbr, sizeof(int32_t) * F, hipMemcpyDeviceToHost, s));
+        if (o->face_normal) HIPCHK(hipMemcpyAsync(o->face_normal, dnrm, sizeof(double) * 3 * F, hipMemcpyDeviceToHost, s));
+        if (o->face_area) HIPCHK(hipMemcpyAsync(o->face_area, dar, sizeof(double) * F, hipMemcpyDeviceToHost, s));
+        if (o->face_centroid) HIPCHK(hipMemcpyAsync(o->face_centroid, dcen, sizeof(double) * 3 * F, hipMemcpyDeviceToHost, s));
+        if (o->seeds) HIPCHK(hipMemcpyAsync(o->seeds, dse, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+        if (o->volumes) HIPCHK(hipMemcpyAsync(o->volumes, dvo, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        if (o->status) HIPCHK(hipMemcpyAsync(o->status, dst, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+        if (o->kcert) HIPCHK(hipMemcpyAsync(o->kcert, dkc, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+        if (o->cell_centroid) HIPCHK(hipMemcpyAsync(o->cell_centroid, dcc, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+        const int64_t ng = n - m->n_in;
+        if (o->ghost_ids && m->prm.boundary == PG_BOUNDARY_PERIODIC && ng > 0)
+            HIPCHK(hipMemcpyAsync(o->ghost_ids, m->gsrc.p, sizeof(int64_t) * ng, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n; i++) {
+            if (o->cell_ids) o->cell_ids[i] = i;
+            if (o->face_counts) o->face_counts[i] = hcnt[i];
+            if (o->face_ptr) o->face_ptr[i] = hptr[i];
+        }
+    });
+}
+
+int pg_mesh_free(pg_mesh** m) {
+    return guarded([&] {
+        if (!m || !*m) return;
+        delete *m;
+        *m = nullptr;
+    });
+}
+
+// ---------------------------------------------------------------- hydro
+int pg_state_create(pg_ctx* c, const pg_mesh* m, const double* rho, const double* vel, const double* E,
+                    pg_state** out) {
+    return guarded([&] {
+        use(c);
+        argchk(m && rho && vel && E && out, "null argument");
+        argchk(m->n_hydro > 0, "mesh has no hydro cells");
+        pg_state* st = new pg_state();
+        try {
+            st->dim = m->prm.dim;
+            st->n = m->n_hydro;
+            const size_t n = (size_t)st->n, d = (size_t)st->dim;
+            for (int b = 0; b < 2; b++) {
+                st->rho[b].get<double>(n);
+                st->vel[b].get<double>(n * d);
+                st->E[b].get<double>(n);
+            }
+            HIPCHK(hipMemcpyAsync(st->rho[0].p, rho, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(st->vel[0].p, vel, sizeof(double) * n * d, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(st->E[0].p, E, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        } catch (...) {
+            delete st;
+            throw;
+        }
+        *out = st;
+    });
+}
+
+int pg_dt_cfl(pg_ctx* c, const pg_mesh* mc, const pg_state* st, double gamma, double cfl, double* dt) {
+    return guarded([&] {
+        use(c);
+        argchk(mc && st && dt, "null argument");
+        pg_mesh* m = const_cast<pg_mesh*>(mc);
+        argchk(st->n == m->n_hydro && st->dim == m->prm.dim, "state does not match mesh");
+        mesh_finalize(c, m);
+        DBuf out;
+        unsigned long long* d = out.get<unsigned long long>(1);
+        double init = 1e100;  // finite_volume_solver.cpp:153
+        HIPCHK(hipMemcpyAsync(d, &init, sizeof(double), hipMemcpyHostToDevice, c->stream));
+        const int b = st->cur;
+        pg::launch_dt_cfl(st->dim, st->n, m->o2s.as<int32_t>(), m->vol_u.as<double>(), gamma, cfl,
+                          st->rho[b].as<double>(), st->vel[b].as<double>(), st->E[b].as<double>(), d,
+                          c->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(dt, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pg_hydro_step(pg_ctx* c, const pg_mesh* mc, pg_state* st, double gamma, double dt, int flux_mode) {
+    return guarded([&] {
+        use(c);
+        argchk(mc && st, "null argument");
+        argchk(flux_mode == PG_FLUX_REFERENCE || flux_mode == PG_FLUX_PHYSICAL, "unknown flux mode");
+        pg_mesh* m = const_cast<pg_mesh*>(mc);
+        argchk(st->n == m->n_hydro && st->dim == m->prm.dim, "state does not match mesh");
+        mesh_finalize(c, m);
+        const int b = st->cur, nb = 1 - b;
+        pg::launch_hll_static(st->dim, m->n_seeds, m->n_hydro, m->n_seeds, m->n_seeds * MAXP,
+                              m->g.perm.as<uint32_t>(), m->count.as<uint8_t>(), m->hid.as<int32_t>(),
+                              m->nrm.as<double>(), m->area_u.as<double>(), m->vol_u.as<double>(), gamma, dt,
+                              flux_mode, st->rho[b].as<double>(), st->vel[b].as<double>(),
+                              st->E[b].as<double>(), st->rho[nb].as<double>(), st->vel[nb].as<double>(),
+                              st->E[nb].as<double>(), c->stream);
+        HIPCHK(hipGetLastError());
+        st->cur = nb;
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pg_state_copy_out(pg_ctx* c, const pg_state* st, double* rho, double* vel, double* E) {
+    return guarded([&] {
+        use(c);
+        argchk(st != nullptr, "null state");
+        const int b = st->cur;
+        const size_t n = (size_t)st->n;
+        if (rho) HIPCHK(hipMemcpyAsync(rho, st->rho[b].p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        if (vel) HIPCHK(hipMemcpyAsync(vel, st->vel[b].p, sizeof(double) * n * st->dim, hipMemcpyDeviceToHost, c->stream));
+        if (E) HIPCHK(hipMemcpyAsync(E, st->E[b].p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pg_state_free(pg_state** st) {
+    return guarded([&] {
+        if (!st || !*st) return;
+        delete *st;
+        *st = nullptr;
+    });
+}
+
+}  // extern "C"

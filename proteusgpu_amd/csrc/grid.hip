@@ -1,0 +1,207 @@
+// Periodic ghosts and the uniform-grid counting sort (knn::init).
+//
+// Ghosts (periodic_mesh.cpp:63-126): a point inside the buff-wide layer of a
+// face, edge or corner region of the unit box gets a copy shifted by +-1
+// along those axes.  Ghosts are appended point-major in the reference's
+// region order, then every point is scaled into [0,1]^d by 1/(1+2 buff).
+//
+// Grid (knn.cpp:119-222): N_grid^d cells of width 1/N_grid; per-cell counts,
+// an exclusive scan of the counts in cell-id order gives each cell's range,
+// and within a range the points are ordered by input index -- exactly the
+// order of the reference's serial store loop.  The scatter uses atomics and
+// the per-cell sort restores that order deterministically.
+#include "common.h"
+#include "kernels.h"
+
+namespace pg {
+namespace {
+
+constexpr int TB = 256;
+
+// Region codes per axis: 0 = whole (0,1), 1 = low layer (0,buff) shifted +1,
+// 2 = high layer (1-buff,1) shifted -1.  The order is the reference's
+// (periodic_mesh.cpp:70-79 in 2D, :83-111 in 3D); it fixes ghost numbering.
+__constant__ uint8_t REG2[8][2] = {{1, 0}, {0, 2}, {2, 0}, {0, 1}, {1, 1}, {1, 2}, {2, 2}, {2, 1}};
+__constant__ uint8_t REG3[26][3] = {
+    {0, 0, 1}, {1, 0, 0}, {0, 2, 0}, {2, 0, 0}, {0, 1, 0}, {0, 0, 2},                        // faces
+    {0, 1, 1}, {1, 0, 1}, {0, 2, 1}, {2, 0, 1}, {1, 1, 0}, {1, 2, 0},                        // edges
+    {2, 2, 0}, {2, 1, 0}, {1, 0, 2}, {0, 2, 2}, {2, 0, 2}, {0, 1, 2},                        //
+    {1, 1, 1}, {1, 2, 1}, {2, 2, 1}, {2, 1, 1}, {1, 1, 2}, {1, 2, 2}, {2, 2, 2}, {2, 1, 2}}; // corners
+
+template <int DIM>
+PG_DEV bool in_region(const double* p, const uint8_t* code, double buff) {
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        double lo = code[d] == 2 ? 1. - buff : 0.;
+        double hi = code[d] == 1 ? buff : 1.;
+        in = in && (p[d] > lo && p[d] < hi);  // periodic_mesh.cpp:16-24, strict
+    }
+    return in;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(TB) k_ghost_count(const double* pts, int64_t n, double buff,
+                                                    int32_t* cnt) {
+    int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    double p[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) p[d] = pts[DIM * i + d];
+    int c = 0;
+    constexpr int NR = DIM == 2 ? 8 : 26;
+    for (int r = 0; r < NR; r++) c += in_region<DIM>(p, DIM == 2 ? REG2[r] : REG3[r], buff) ? 1 : 0;
+    cnt[i] = c;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(TB) k_ghost_write(const double* pts, int64_t n, double buff,
+                                                    const int32_t* off, double* out,
+                                                    int64_t* gsrc) {
+    int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    const double scale = 1. / (1. + (2 * buff));
+    double p[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        p[d] = pts[DIM * i + d];
+        out[DIM * i + d] = scale * (p[d] - 0.5) + 0.5;
+    }
+    int64_t g = off ? off[i] : 0;
+    constexpr int NR = DIM == 2 ? 8 : 26;
+    for (int r = 0; r < NR; r++) {
+        const uint8_t* code = DIM == 2 ? REG2[r] : REG3[r];
+        if (!in_region<DIM>(p, code, buff)) continue;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            double sh = code[d] == 1 ? 1. : (code[d] == 2 ? -1. : 0.);
+            double q = p[d] + sh;
+            out[DIM * (n + g) + d] = scale * (q - 0.5) + 0.5;
+        }
+        gsrc[g] = i;
+        g++;
+    }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(TB) k_cell_count(const double* pts, int64_t n, int N,
+                                                   int32_t* cell, int32_t* counts) {
+    int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    double p[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) p[d] = pts[DIM * i + d];
+    int c = grid_cell<DIM>(p, N);
+    cell[i] = c;
+    atomicAdd(&counts[c], 1);
+}
+
+__global__ void __launch_bounds__(TB) k_cell_scatter(int64_t n, const int32_t* cell,
+                                                     const int32_t* starts, int32_t* fill,
+                                                     uint32_t* idx) {
+    int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    int c = cell[i];
+    int pos = starts[c] + atomicAdd(&fill[c], 1);
+    idx[pos] = (uint32_t)i;
+}
+
+// Insertion sort of each cell's index range.  Uniform inputs hold ~3 points
+// per cell; the register buffer covers ranges up to 32 and longer ranges
+// are sorted in place in global memory.
+__global__ void __launch_bounds__(TB) k_cell_sort(int64_t ncell, const int32_t* starts,
+                                                  const int32_t* counts, uint32_t* idx) {
+    int64_t c = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (c >= ncell) return;
+    int len = counts[c];
+    if (len < 2) return;
+    uint32_t* a = idx + starts[c];
+    if (len <= 32) {
+        uint32_t v[32];
+        for (int i = 0; i < len; i++) v[i] = a[i];
+        for (int i = 1; i < len; i++) {
+            uint32_t x = v[i];
+            int j = i - 1;
+            while (j >= 0 && v[j] > x) {
+                v[j + 1] = v[j];
+                j--;
+            }
+            v[j + 1] = x;
+        }
+        for (int i = 0; i < len; i++) a[i] = v[i];
+        return;
+    }
+    for (int i = 1; i < len; i++) {
+        uint32_t x = a[i];
+        int j = i - 1;
+        while (j >= 0 && a[j] > x) {
+            a[j + 1] = a[j];
+            j--;
+        }
+        a[j + 1] = x;
+    }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(TB) k_gather_points(int64_t n, const uint32_t* perm,
+                                                      const double* pts, double* sorted) {
+    int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (s >= n) return;
+    uint32_t o = perm[s];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) sorted[DIM * s + d] = pts[DIM * (int64_t)o + d];
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + TB - 1) / TB); }
+
+}  // namespace
+
+void launch_ghost_count(int dim, const double* pts, int64_t n, double buff, int32_t* cnt,
+                        hipStream_t s) {
+    if (n <= 0) return;
+    if (dim == 2)
+        k_ghost_count<2><<<nblk(n), TB, 0, s>>>(pts, n, buff, cnt);
+    else
+        k_ghost_count<3><<<nblk(n), TB, 0, s>>>(pts, n, buff, cnt);
+}
+
+void launch_ghost_write(int dim, const double* pts, int64_t n, double buff, const int32_t* off,
+                        double* out, int64_t* gsrc, hipStream_t s) {
+    if (n <= 0) return;
+    if (dim == 2)
+        k_ghost_write<2><<<nblk(n), TB, 0, s>>>(pts, n, buff, off, out, gsrc);
+    else
+        k_ghost_write<3><<<nblk(n), TB, 0, s>>>(pts, n, buff, off, out, gsrc);
+}
+
+void launch_cell_count(int dim, const double* pts, int64_t n, int N, int32_t* cell,
+                       int32_t* counts, hipStream_t s) {
+    if (n <= 0) return;
+    if (dim == 2)
+        k_cell_count<2><<<nblk(n), TB, 0, s>>>(pts, n, N, cell, counts);
+    else
+        k_cell_count<3><<<nblk(n), TB, 0, s>>>(pts, n, N, cell, counts);
+}
+
+void launch_cell_scatter(int64_t n, const int32_t* cell, const int32_t* starts, int32_t* fill,
+                         uint32_t* idx, hipStream_t s) {
+    if (n <= 0) return;
+    k_cell_scatter<<<nblk(n), TB, 0, s>>>(n, cell, starts, fill, idx);
+}
+
+void launch_cell_sort(int64_t ncell, const int32_t* starts, const int32_t* counts, uint32_t* idx,
+                      hipStream_t s) {
+    if (ncell <= 0) return;
+    k_cell_sort<<<nblk(ncell), TB, 0, s>>>(ncell, starts, counts, idx);
+}
+
+void launch_gather_points(int dim, int64_t n, const uint32_t* perm, const double* pts,
+                          double* sorted, hipStream_t s) {
+    if (n <= 0) return;
+    if (dim == 2)
+        k_gather_points<2><<<nblk(n), TB, 0, s>>>(n, perm, pts, sorted);
+    else
+        k_gather_points<3><<<nblk(n), TB, 0, s>>>(n, perm, pts, sorted);
+}
+
+}  // namespace pg

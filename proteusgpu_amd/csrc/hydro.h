@@ -1,0 +1,40 @@
+// Host-side launchers of the hydro and copy-out kernels (internal interface).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pg {
+
+void launch_finalize(int64_t n, int64_t n_hydro, int64_t stride, const uint32_t* perm,
+                     const int64_t* gsrc, const uint8_t* count, const int32_t* nbr,
+                     const double* area, const double* vol, double ascale, double vscale,
+                     int32_t* hid, double* area_u, double* vol_u, int32_t* o2s, int32_t* bad,
+                     hipStream_t s);
+
+void launch_hll_static(int dim, int64_t n, int64_t n_hydro, int64_t stride, int64_t nplane,
+                       const uint32_t* perm, const uint8_t* count, const int32_t* hid,
+                       const double* nrm, const double* area_u, const double* vol_u, double gam,
+                       double dt, int flux_mode, const double* rho, const double* vel,
+                       const double* E, double* rho2, double* vel2, double* E2, hipStream_t s);
+
+void launch_dt_cfl(int dim, int64_t n_hydro, const int32_t* o2s, const double* vol_u, double gam,
+                   double cfl, const double* rho, const double* vel, const double* E,
+                   unsigned long long* out, hipStream_t s);
+
+void launch_counts_orig(int64_t n, const uint32_t* perm, const uint8_t* count, int32_t* co,
+                        hipStream_t s);
+
+void launch_pack_faces(int64_t n, int64_t stride, int64_t nplane, int dim, const uint32_t* perm,
+                       const uint8_t* count, const int32_t* ptr_orig, const int32_t* nbr,
+                       const double* nrm, const double* area, const double* cen, double ascale,
+                       double cscale, int32_t* o_nbr, double* o_nrm, double* o_area, double* o_cen,
+                       hipStream_t s);
+
+void launch_cells_orig(int64_t n, int64_t stride, int dim, const uint32_t* perm,
+                       const double* sorted, const int32_t* status, const int32_t* kcert,
+                       const double* vol, const double* ccen, double scale, double vscale,
+                       double* o_seeds, double* o_vol, int32_t* o_status, int32_t* o_kcert,
+                       double* o_ccen, hipStream_t s);
+
+}  // namespace pg

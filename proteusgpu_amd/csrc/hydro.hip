@@ -1,0 +1,373 @@
+// Hydro on the Voronoi mesh: mesh finalisation, the HLL update
+// (finite_volume_solver.cpp:43-148, riemann.cpp:7-157), the CFL reduce
+// (finite_volume_solver.cpp:151-186) and the reference-layout copy-out of
+// the mesh (voronoi.cpp:212-333 unpermute, periodic_mesh.cpp:170-190 unscale).
+//
+// Face data come from the clip kernel in face-slot-major padded storage
+// (entry (f, sorted cell s) at f*stride + s), so threads of a wave that handle
+// consecutive sorted cells read consecutive addresses for every face slot.
+#include "common.h"
+#include "hydro.h"
+
+namespace pg {
+namespace {
+
+constexpr int TB = 256;
+inline unsigned nblk(int64_t n) { return (unsigned)((n + TB - 1) / TB); }
+
+// Per sorted cell: hydro id of every face's neighbour (-1 = box plane,
+// which is a reflective wall), unscaled area and cell volume.
+__global__ void __launch_bounds__(TB) k_finalize(int64_t n, int64_t n_hydro, int64_t stride,
+                                                 const uint32_t* perm, const int64_t* gsrc,
+                                                 const uint8_t* count, const int32_t* nbr,
+                                                 const double* area, const double* vol, double ascale,
+                                                 double vscale, int32_t* hid, double* area_u,
+                                                 double* vol_u, int32_t* o2s, int32_t* bad) {
+    int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t o = perm[s];
+    if (o >= n_hydro) return;
+    o2s[o] = (int32_t)s;
+    vol_u[s] = vscale * vol[s];
+    const int c = count[s];
+    if (c == 0) atomicAdd(bad, 1);
+    for (int f = 0; f < c; f++) {
+        const int64_t at = (int64_t)f * stride + s;
+        const int32_t j = nbr[at];
+        int32_t h = -1;
+        if (j >= 0) {
+            const uint32_t oj = perm[j];
+            h = oj < n_hydro ? (int32_t)oj : (int32_t)gsrc[oj - n_hydro];
+        }
+        hid[at] = h;
+        area_u[at] = ascale * area[at];
+    }
+}
+
+struct Prim {
+    double rho, v[3], E;
+};
+
+template <int DIM>
+PG_DEV double pressure(double gam, const Prim& s) {
+    if (DIM == 2) return (gam - 1) * (s.E - (0.5 * s.rho * (s.v[0] * s.v[0] + s.v[1] * s.v[1])));
+    return (gam - 1) *
+           (s.E - (0.5 * s.rho * (s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2])));
+}
+
+// riemann.cpp:108-130
+struct Geom {
+    double n[3], m[3], p[3];
+};
+PG_DEV Geom geom_of(double n0, double n1, double n2) {
+    Geom g;
+    const double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    g.n[0] = n0 / nn;
+    g.n[1] = n1 / nn;
+    g.n[2] = n2 / nn;
+    if (g.n[0] != 0.0 || g.n[1] != 0.0) {
+        g.m[0] = -g.n[1];
+        g.m[1] = g.n[0];
+        g.m[2] = 0.0;
+    } else {
+        g.m[0] = 1.0;
+        g.m[1] = 0.0;
+        g.m[2] = 0.0;
+    }
+    const double mm = sqrt(g.m[0] * g.m[0] + g.m[1] * g.m[1] + g.m[2] * g.m[2]);
+    g.m[0] = g.m[0] / mm;
+    g.m[1] = g.m[1] / mm;
+    g.m[2] = g.m[2] / mm;
+    g.p[0] = g.n[1] * g.m[2] - g.n[2] * g.m[1];
+    g.p[1] = g.n[2] * g.m[0] - g.n[0] * g.m[2];
+    g.p[2] = g.n[0] * g.m[1] - g.n[1] * g.m[0];
+    return g;
+}
+template <int DIM>
+PG_DEV void to_face(Prim& s, const Geom& g) {
+    const double vx = s.v[0], vy = s.v[1];
+    if (DIM == 2) {
+        s.v[0] = vx * g.n[0] + vy * g.n[1];
+        s.v[1] = vx * g.m[0] + vy * g.m[1];
+    } else {
+        const double vz = s.v[2];
+        s.v[0] = vx * g.n[0] + vy * g.n[1] + vz * g.n[2];
+        s.v[1] = vx * g.m[0] + vy * g.m[1] + vz * g.m[2];
+        s.v[2] = vx * g.p[0] + vy * g.p[1] + vz * g.p[2];
+    }
+}
+template <int DIM>
+PG_DEV void from_face(Prim& s, const Geom& g) {
+    const double vx = s.v[0], vy = s.v[1];
+    if (DIM == 2) {
+        s.v[0] = vx * g.n[0] + vy * g.m[0];
+        s.v[1] = vx * g.n[1] + vy * g.m[1];
+    } else {
+        const double vz = s.v[2];
+        s.v[0] = vx * g.n[0] + vy * g.m[0] + vz * g.p[0];
+        s.v[1] = vx * g.n[1] + vy * g.m[1] + vz * g.p[1];
+        s.v[2] = vx * g.n[2] + vy * g.m[2] + vz * g.p[2];
+    }
+}
+template <int DIM>
+PG_DEV Prim euler_flux(double gam, const Prim& s) {
+    const double P = pressure<DIM>(gam, s);
+    Prim f;
+    f.rho = s.rho * s.v[0];
+    f.v[0] = s.rho * s.v[0] * s.v[0] + P;
+    f.v[1] = s.rho * s.v[0] * s.v[1];
+    f.v[2] = DIM == 3 ? s.rho * s.v[0] * s.v[2] : 0.0;
+    f.E = (s.E + P) * s.v[0];
+    return f;
+}
+
+// riemann.cpp:7-45 with the reference's operation order.  wall: the right
+// state is the left one mirrored in the face frame, so SL == -SR exactly and
+// the mass flux through the wall is exactly zero.
+template <int DIM>
+PG_DEV Prim hll(double gam, Prim L, Prim R, const Geom& g, bool wall) {
+    to_face<DIM>(L, g);
+    if (wall) {
+        R = L;
+        R.v[0] = -L.v[0];
+    } else {
+        to_face<DIM>(R, g);
+    }
+    const Prim fl = euler_flux<DIM>(gam, L), fr = euler_flux<DIM>(gam, R);
+    const double cl = sqrt((gam * pressure<DIM>(gam, L)) / L.rho);
+    const double cr = sqrt((gam * pressure<DIM>(gam, R)) / R.rho);
+    const double SL = smin(L.v[0] - cl, R.v[0] - cr);
+    const double SR = smax(L.v[0] + cl, R.v[0] + cr);
+    Prim F = {0.0, {0.0, 0.0, 0.0}, 0.0};
+    if (SL >= 0) {
+        F = fl;
+    } else if (SL < 0 && SR > 0) {
+        F.rho = (SR * fl.rho - SL * fr.rho + SL * SR * (R.rho - L.rho)) / (SR - SL);
+#pragma unroll
+        for (int d = 0; d < DIM; d++)
+            F.v[d] = (SR * fl.v[d] - SL * fr.v[d] + SL * SR * (R.rho * R.v[d] - L.rho * L.v[d])) /
+                     (SR - SL);
+        F.E = (SR * fl.E - SL * fr.E + SL * SR * (R.E - L.E)) / (SR - SL);
+    } else if (SR <= 0) {
+        F = fr;
+    }
+    from_face<DIM>(F, g);
+    return F;
+}
+
+// One static-mesh HLL update per hydro cell (finite_volume_solver.cpp:53-106):
+// every cell sums the fluxes through its own faces, exactly as the
+// reference, and writes the new primitives to the second buffer.
+template <int DIM>
+__global__ void __launch_bounds__(TB)
+    k_hll_static(int64_t n, int64_t n_hydro, int64_t stride, int64_t nplane,
+                 const uint32_t* __restrict__ perm, const uint8_t* __restrict__ count,
+                 const int32_t* __restrict__ hid, const double* __restrict__ nrm,
+                 const double* __restrict__ area_u, const double* __restrict__ vol_u, double gam,
+                 double dt, int flux_mode, const double* __restrict__ rho,
+                 const double* __restrict__ vel, const double* __restrict__ E,
+                 double* __restrict__ rho2, double* __restrict__ vel2, double* __restrict__ E2) {
+    int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t i = perm[s];
+    if (i >= n_hydro) return;
+    Prim si;
+    si.rho = rho[i];
+    si.E = E[i];
+    si.v[2] = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) si.v[d] = vel[(size_t)DIM * i + d];
+    Prim tot = {0.0, {0.0, 0.0, 0.0}, 0.0};
+    const int c = count[s];
+    const double sgn = flux_mode == 1 ? -1.0 : 1.0;
+    for (int f = 0; f < c; f++) {
+        const int64_t at = (int64_t)f * stride + s;
+        const int32_t j = hid[at];
+        const bool wall = j < 0;
+        Prim sj = si;
+        if (!wall) {
+            sj.rho = rho[j];
+            sj.E = E[j];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) sj.v[d] = vel[(size_t)DIM * j + d];
+        }
+        // flux_mode 1 negates the stored (inward) normal exactly
+        const Geom g = geom_of(sgn * nrm[at], sgn * nrm[nplane + at], sgn * nrm[2 * nplane + at]);
+        const Prim F = hll<DIM>(gam, si, sj, g, wall);
+        const double a = area_u[at];
+        tot.rho += F.rho * a;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) tot.v[d] += F.v[d] * a;
+        tot.E += F.E * a;
+    }
+    const double frac = dt / vol_u[s];
+    const double nr = si.rho - frac * tot.rho;
+    rho2[i] = nr;
+    const double inv = 1.0 / nr;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) vel2[(size_t)DIM * i + d] = (si.rho * si.v[d] - frac * tot.v[d]) * inv;
+    E2[i] = si.E - frac * tot.E;
+}
+
+// finite_volume_solver.cpp:151-186: per-cell CFL time step, min over cells.
+template <int DIM>
+__global__ void __launch_bounds__(TB)
+    k_dt_cfl(int64_t n_hydro, const int32_t* __restrict__ o2s, const double* __restrict__ vol_u,
+             double gam, double cfl, const double* __restrict__ rho, const double* __restrict__ vel,
+             const double* __restrict__ E, unsigned long long* out) {
+    int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    double dti = 1e100;
+    if (i < n_hydro) {
+        Prim s;
+        s.rho = rho[i];
+        s.E = E[i];
+        s.v[2] = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) s.v[d] = vel[(size_t)DIM * i + d];
+        const double P = pressure<DIM>(gam, s);
+        const double c = sqrt(gam * P / s.rho);
+        const double V = vol_u[o2s[i]];
+        double R, va;
+        if (DIM == 2) {
+            R = sqrt(V / M_PI);
+            va = sqrt(s.v[0] * s.v[0] + s.v[1] * s.v[1]);
+        } else {
+            R = cbrt(3.0 * V / (4.0 * M_PI));
+            va = sqrt(s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]);
+        }
+        dti = cfl * (R / (c + va));
+    }
+    // wave min, then one atomic per wave (non-negative doubles order as bits)
+    for (int d = 32; d > 0; d >>= 1) dti = smin(__shfl_xor(dti, d, 64), dti);
+    if ((threadIdx.x & 63) == 0) {
+        if (dti >= 0.0) atomic_min_pos_double(out, dti);
+        else atomicMin(out, 0ull);  // a negative step (bad state) reports 0
+    }
+}
+
+// Copy-out: face counts in original order.
+__global__ void __launch_bounds__(TB) k_counts_orig(int64_t n, const uint32_t* perm,
+                                                    const uint8_t* count, int32_t* co) {
+    int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (s >= n) return;
+    co[perm[s]] = count[s];
+}
+
+// Copy-out: packed reference-layout faces of every cell in original order.
+__global__ void __launch_bounds__(TB)
+    k_pack_faces(int64_t n, int64_t stride, int64_t nplane, int dim, const uint32_t* perm,
+                 const uint8_t* count, const int32_t* ptr_orig, const int32_t* nbr,
+                 const double* nrm, const double* area, const double* cen, double ascale, double cscale,
+                 int32_t* o_nbr, double* o_nrm, double* o_area, double* o_cen) {
+    int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t o = perm[s];
+    const int c = count[s];
+    const int64_t base = ptr_orig[o];
+    for (int f = 0; f < c; f++) {
+        const int64_t at = (int64_t)f * stride + s;
+        const int64_t w = base + f;
+        const int32_t j = nbr[at];
+        o_nbr[w] = j >= 0 ? (int32_t)perm[j] : j;
+        o_nrm[3 * w] = nrm[at];
+        o_nrm[3 * w + 1] = nrm[nplane + at];
+        o_nrm[3 * w + 2] = nrm[2 * nplane + at];
+        o_area[w] = ascale * area[at];
+        for (int d = 0; d < 3; d++) {
+            double x = cen[(int64_t)d * nplane + at];
+            o_cen[3 * w + d] = (d < dim && cscale != 1.0) ? (x - 0.5) * cscale + 0.5 : x;
+        }
+    }
+}
+
+// Copy-out: per-cell arrays in original order (seeds unscaled as the
+// reference, periodic_mesh.cpp:179-186; failed cells keep zero seeds).
+__global__ void __launch_bounds__(TB)
+    k_cells_orig(int64_t n, int64_t stride, int dim, const uint32_t* perm, const double* sorted,
+                 const int32_t* status, const int32_t* kcert, const double* vol, const double* ccen,
+                 double scale, double vscale, double* o_seeds, double* o_vol, int32_t* o_status,
+                 int32_t* o_kcert, double* o_ccen) {
+    int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t o = perm[s];
+    const bool ok = status[s] == SUCCESS;
+    for (int d = 0; d < 3; d++) {
+        double x = (ok && d < dim) ? sorted[(size_t)dim * s + d] : 0.0;
+        double cx = ccen[(int64_t)d * stride + s];
+        if (scale != 1.0 && d < dim) {
+            x = (x - 0.5) * scale + 0.5;
+            cx = (cx - 0.5) * scale + 0.5;
+        }
+        o_seeds[3 * (int64_t)o + d] = x;
+        o_ccen[3 * (int64_t)o + d] = cx;
+    }
+    o_vol[o] = vscale * vol[s];
+    o_status[o] = status[s];
+    o_kcert[o] = kcert[s];
+}
+
+}  // namespace
+
+void launch_finalize(int64_t n, int64_t n_hydro, int64_t stride, const uint32_t* perm,
+                     const int64_t* gsrc, const uint8_t* count, const int32_t* nbr,
+                     const double* area, const double* vol, double ascale, double vscale,
+                     int32_t* hid, double* area_u, double* vol_u, int32_t* o2s, int32_t* bad,
+                     hipStream_t s) {
+    if (n <= 0) return;
+    k_finalize<<<nblk(n), TB, 0, s>>>(n, n_hydro, stride, perm, gsrc, count, nbr, area, vol, ascale,
+                                      vscale, hid, area_u, vol_u, o2s, bad);
+}
+
+void launch_hll_static(int dim, int64_t n, int64_t n_hydro, int64_t stride, int64_t nplane,
+                       const uint32_t* perm, const uint8_t* count, const int32_t* hid,
+                       const double* nrm, const double* area_u, const double* vol_u, double gam,
+                       double dt, int flux_mode, const double* rho, const double* vel,
+                       const double* E, double* rho2, double* vel2, double* E2, hipStream_t s) {
+    if (n <= 0) return;
+    if (dim == 2)
+        k_hll_static<2><<<nblk(n), TB, 0, s>>>(n, n_hydro, stride, nplane, perm, count, hid, nrm,
+                                               area_u, vol_u, gam, dt, flux_mode, rho, vel, E, rho2,
+                                               vel2, E2);
+    else
+        k_hll_static<3><<<nblk(n), TB, 0, s>>>(n, n_hydro, stride, nplane, perm, count, hid, nrm,
+                                               area_u, vol_u, gam, dt, flux_mode, rho, vel, E, rho2,
+                                               vel2, E2);
+}
+
+void launch_dt_cfl(int dim, int64_t n_hydro, const int32_t* o2s, const double* vol_u, double gam,
+                   double cfl, const double* rho, const double* vel, const double* E,
+                   unsigned long long* out, hipStream_t s) {
+    if (n_hydro <= 0) return;
+    if (dim == 2)
+        k_dt_cfl<2><<<nblk(n_hydro), TB, 0, s>>>(n_hydro, o2s, vol_u, gam, cfl, rho, vel, E, out);
+    else
+        k_dt_cfl<3><<<nblk(n_hydro), TB, 0, s>>>(n_hydro, o2s, vol_u, gam, cfl, rho, vel, E, out);
+}
+
+void launch_counts_orig(int64_t n, const uint32_t* perm, const uint8_t* count, int32_t* co,
+                        hipStream_t s) {
+    if (n <= 0) return;
+    k_counts_orig<<<nblk(n), TB, 0, s>>>(n, perm, count, co);
+}
+
+void launch_pack_faces(int64_t n, int64_t stride, int64_t nplane, int dim, const uint32_t* perm,
+                       const uint8_t* count, const int32_t* ptr_orig, const int32_t* nbr,
+                       const double* nrm, const double* area, const double* cen, double ascale,
+                       double cscale, int32_t* o_nbr, double* o_nrm, double* o_area, double* o_cen,
+                       hipStream_t s) {
+    if (n <= 0) return;
+    k_pack_faces<<<nblk(n), TB, 0, s>>>(n, stride, nplane, dim, perm, count, ptr_orig, nbr, nrm, area,
+                                        cen, ascale, cscale, o_nbr, o_nrm, o_area, o_cen);
+}
+
+void launch_cells_orig(int64_t n, int64_t stride, int dim, const uint32_t* perm,
+                       const double* sorted, const int32_t* status, const int32_t* kcert,
+                       const double* vol, const double* ccen, double scale, double vscale,
+                       double* o_seeds, double* o_vol, int32_t* o_status, int32_t* o_kcert,
+                       double* o_ccen, hipStream_t s) {
+    if (n <= 0) return;
+    k_cells_orig<<<nblk(n), TB, 0, s>>>(n, stride, dim, perm, sorted, status, kcert, vol, ccen, scale,
+                                        vscale, o_seeds, o_vol, o_status, o_kcert, o_ccen);
+}
+
+}  // namespace pg

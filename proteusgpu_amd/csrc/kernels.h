@@ -1,0 +1,75 @@
+// Host-side launchers of the libpgpu kernels (internal interface).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pg {
+
+// ---------------------------------------------------------------- scan.hip
+// Exclusive prefix sum of n int32 values; writes total (device int32) when
+// total != nullptr.  tmp must hold scan_tmp_elems(n) int32.
+int64_t scan_tmp_elems(int64_t n);
+void scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* tmp, int32_t* total,
+                        hipStream_t s);
+
+// ---------------------------------------------------------------- grid.hip
+// periodic_mesh.cpp:63-112: number of ghost regions each point falls in.
+void launch_ghost_count(int dim, const double* pts, int64_t n, double buff, int32_t* cnt,
+                        hipStream_t s);
+// periodic_mesh.cpp:50-65,119-126: write scaled points, hydro first then the
+// ghosts in (point, region) order; ghost_src[g] = source hydro index.
+void launch_ghost_write(int dim, const double* pts, int64_t n, double buff, const int32_t* off,
+                        double* out, int64_t* ghost_src, hipStream_t s);
+// knn.cpp:119-222: grid cell of every point and per-cell counts.
+void launch_cell_count(int dim, const double* pts, int64_t n, int N, int32_t* cell,
+                       int32_t* counts, hipStream_t s);
+// Scatter point indices into their cells' ranges (any order within a cell).
+void launch_cell_scatter(int64_t n, const int32_t* cell, const int32_t* starts, int32_t* fill,
+                         uint32_t* idx, hipStream_t s);
+// Sort every cell's range ascending: the order of the reference's serial
+// store loop (knn.cpp:187-204), i.e. stable by input index.
+void launch_cell_sort(int64_t ncell, const int32_t* starts, const int32_t* counts, uint32_t* idx,
+                      hipStream_t s);
+// sorted[s] = pts[perm[s]]
+void launch_gather_points(int dim, int64_t n, const uint32_t* perm, const double* pts,
+                          double* sorted, hipStream_t s);
+
+// ----------------------------------------------------------------- knn.hip
+// k nearest neighbours of queries (sorted ids; nullptr = all n points).
+// Rows ascending by (FP64 dist^2, original index); out row stride k.
+void launch_knn(int dim, const double* sorted, const uint32_t* perm, int64_t n, int N,
+                const int32_t* starts, const int32_t* counts, const uint32_t* queries, int64_t nq,
+                int k, uint32_t* out, hipStream_t s);
+
+// ------------------------------------------------------------- voronoi.hip
+struct FaceArrays {
+    // face-slot-major padded storage: entry (f, cell s) at f * stride + s
+    int64_t stride;
+    int32_t* nbr;      // sorted neighbour id, -1 = box plane
+    double* area;
+    double* nrm;       // 3 planes of stride*maxf doubles: x | y | z
+    double* cen;       // face centroid, same layout
+    uint8_t* count;    // faces per cell
+    double* vol;       // cell volume
+    double* ccen;      // cell centroid, 3 planes of stride doubles
+    int32_t* status;   // PG status per cell
+    int32_t* kcert;    // 1 + index of the certifying neighbour, 0 if none
+};
+
+// Clip + certify + extract the seeds listed in `seeds` (sorted ids; nullptr
+// = all n) with their neighbour rows knn[row q] (row q belongs to seeds[q]).
+// big != 0 selects the 128-slot kernel.
+void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq,
+                    const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa,
+                    hipStream_t s);
+
+// Collect the sorted ids whose status is not SUCCESS (order: ascending id).
+// nfail (device int32) receives the count; tmp as for scan.
+void launch_collect_failed(int64_t n, const int32_t* status, uint32_t* list, int32_t* flags,
+                           int32_t* offs, int32_t* tmp, int32_t* nfail, hipStream_t s);
+void launch_collect_status(int64_t nq, const uint32_t* seeds, const int32_t* status, int code,
+                           uint32_t* list, int32_t* flags, int32_t* offs, int32_t* tmp,
+                           int32_t* nout, hipStream_t s);
+
+}  // namespace pg

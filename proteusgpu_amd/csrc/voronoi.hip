@@ -1,0 +1,83 @@
+// Kernel entry points for the Voronoi clip and the restart bookkeeping.
+#include "voronoi_kernel.h"
+
+namespace pg {
+namespace {
+
+// Fast kernel: 16 lanes x 4 slots = 64 vertex slots per seed, 4 seeds per
+// wave.  Large kernel: 16 lanes x 8 slots = 128 slots, for seeds whose cell
+// outgrows 64 vertices before max_t (rare).
+template <int DIM, int G, int S>
+__global__ void __launch_bounds__(64) k_voronoi(const double* __restrict__ pts,
+                                                const uint32_t* __restrict__ seeds, int64_t nq,
+                                                const uint32_t* __restrict__ knn, int k, int max_p,
+                                                int max_t, FaceArrays fa) {
+    using C = VoroCfg<DIM, G, S>;
+    __shared__ SeedSmem<C::NS, C::MAXP> sm[C::SPW];
+    voronoi_group<DIM, G, S>(pts, seeds, nq, knn, k, max_p, max_t, fa, sm);
+}
+
+template <int DIM, int G, int S>
+void run(const double* pts, const uint32_t* seeds, int64_t nq, const uint32_t* knn, int k,
+         int max_p, int max_t, FaceArrays fa, hipStream_t s) {
+    using C = VoroCfg<DIM, G, S>;
+    int64_t nb = (nq + C::SPW - 1) / C::SPW;
+    k_voronoi<DIM, G, S><<<dim3((unsigned)nb), dim3(64), 0, s>>>(pts, seeds, nq, knn, k, max_p,
+                                                                  max_t, fa);
+}
+
+constexpr int TB = 256;
+
+__global__ void __launch_bounds__(TB) k_flag_status(int64_t nq, const uint32_t* seeds,
+                                                    const int32_t* status, int code, int32_t* flags) {
+    int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i >= nq) return;
+    uint32_t s = seeds ? seeds[i] : (uint32_t)i;
+    int st = status[s];
+    flags[i] = (code < 0) ? (st != SUCCESS) : (st == code);
+}
+
+__global__ void __launch_bounds__(TB) k_compact(int64_t nq, const uint32_t* seeds,
+                                                const int32_t* flags, const int32_t* offs,
+                                                uint32_t* list) {
+    int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i >= nq || !flags[i]) return;
+    list[offs[i]] = seeds ? seeds[i] : (uint32_t)i;
+}
+
+}  // namespace
+
+void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq,
+                    const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa,
+                    hipStream_t s) {
+    (void)n;
+    if (nq <= 0) return;
+    if (dim == 2) {
+        if (big)
+            run<2, 16, 8>(sorted, seeds, nq, knn, k, max_p, max_t, fa, s);
+        else
+            run<2, 16, 4>(sorted, seeds, nq, knn, k, max_p, max_t, fa, s);
+    } else {
+        if (big)
+            run<3, 16, 8>(sorted, seeds, nq, knn, k, max_p, max_t, fa, s);
+        else
+            run<3, 16, 4>(sorted, seeds, nq, knn, k, max_p, max_t, fa, s);
+    }
+}
+
+void launch_collect_status(int64_t nq, const uint32_t* seeds, const int32_t* status, int code,
+                           uint32_t* list, int32_t* flags, int32_t* offs, int32_t* tmp,
+                           int32_t* nout, hipStream_t s) {
+    if (nq <= 0) return;
+    unsigned nb = (unsigned)((nq + TB - 1) / TB);
+    k_flag_status<<<nb, TB, 0, s>>>(nq, seeds, status, code, flags);
+    scan_exclusive_i32(flags, offs, nq, tmp, nout, s);
+    k_compact<<<nb, TB, 0, s>>>(nq, seeds, flags, offs, list);
+}
+
+void launch_collect_failed(int64_t n, const int32_t* status, uint32_t* list, int32_t* flags,
+                           int32_t* offs, int32_t* tmp, int32_t* nfail, hipStream_t s) {
+    launch_collect_status(n, nullptr, status, -1, list, flags, offs, tmp, nfail, s);
+}
+
+}  // namespace pg

@@ -1,0 +1,750 @@
+// Meshless Voronoi clipping on CDNA4: one 16-lane group per seed.
+//
+// Algorithm (cell.cpp:42-411, voronoi.cpp:115-156, after Ray et al. 2018):
+// a seed's cell starts as the box [-1e-14, 1+1e-14]^d and is clipped by the
+// bisector planes of its neighbours in KNN order; after every clip the cell
+// is certified once the just-clipped neighbour is farther than twice the
+// largest seed-vertex distance.  Vertices are plane triples (pairs in 2D).
+//
+// Layout on the GPU (differs from the reference, results do not):
+//   * a wave64 holds SPW = 64/G seeds, G lanes each; a lane owns S vertex
+//     slots, slot id gs = s*G + lane.  Per slot the lane keeps in registers
+//     the vertex position (px,py,pz), the Cramer denominator W, the squared
+//     seed distance d2 and two float bounds (q, m) used by the filter below.
+//   * planes (FP64), plane->neighbour ids, vertex plane triples and the
+//     vertex adjacency (slot across each edge) live in LDS.
+//   * a clip tests every live vertex in parallel.  The reference decides
+//     with a 4x4 (3x3 in 2D) FP64 determinant per vertex; here
+//     det ~= W * (e . p + e.w) is evaluated from the cached vertex and used
+//     only when |det| exceeds a proven bound (48u |e|_1 q + eps bound, see
+//     conflict()); otherwise the reference determinant and its eps test are
+//     evaluated exactly as written.  Decisions and needs_exact_predicates
+//     flags are therefore identical to the reference.
+//   * the cavity boundary is found from the adjacency (an edge of a removed
+//     vertex whose neighbour is kept), in parallel over removed vertices,
+//     instead of the reference's serial walk; new vertices are the same
+//     plane triples (cur, a, b) with the same orientation, so vertex
+//     positions and the certificate are bit-identical to the reference.
+//   * extraction walks each face's vertex cycle through the adjacency; face
+//     order is plane order as in the reference (box planes, then KNN order).
+#pragma once
+
+#include "common.h"
+#include "kernels.h"
+#include "simt.h"
+
+namespace pg {
+
+struct Plane {
+    double x, y, z, w;
+};
+
+template <int DIM, int G, int S>
+struct VoroCfg {
+    static constexpr int NS = G * S;               // vertex slots per seed
+    static constexpr int SPW = 64 / G;             // seeds per wave (= block)
+    static constexpr int MAXP = 64;                // plane slots per seed
+    static constexpr int NW = (NS + 63) / 64;      // 64-bit words of a slot mask
+    static constexpr int PPL = MAXP / G;           // planes per lane at extraction
+    static_assert(NS <= 128, "slot ids are bytes below 255");
+    static_assert(G <= 64 && 64 % G == 0, "groups tile the wave");
+};
+
+template <int NS, int MAXP>
+struct SeedSmem {
+    union {
+        Plane plane[MAXP];      // planes while clipping
+        double vpos[3 * NS];    // vertex positions during extraction
+    };
+    float pn1[MAXP];        // |x|+|y|+|z|+|w| rounded up
+    float pmx[MAXP];        // largest coefficient entering the eps test, rounded up
+    int32_t pvid[MAXP];     // neighbour (sorted id) of each plane, -1 = box
+    uint32_t tri[NS];       // planes of each vertex: a | b << 8 | c << 16
+    uint8_t adj[3][NS];     // neighbour slot across edge/line e
+    uint32_t edge[NS];      // boundary edges of the current clip
+    uint8_t newslot[NS];    // slot of the j-th new vertex
+    uint8_t by_start[MAXP]; // 3D: boundary edge starting at a plane
+    uint8_t by_end[MAXP];   // 3D: boundary edge ending at a plane
+    uint32_t first[MAXP];   // extraction: lowest vertex slot on each plane
+};
+
+// Round a non-negative double up to float (next float above on inexact).
+PG_DEV float f2f_up(double d) {
+    float f = (float)d;
+    if ((double)f < d) f = __int_as_float(__float_as_int(f) + 1);
+    return f;
+}
+PG_DEV int tri_p(uint32_t t, int e) { return (int)((t >> (8 * e)) & 0xffu); }
+
+template <int NW>
+PG_DEV bool mask_has(const uint64_t (&m)[NW], int gs) {
+    if (gs >= 64 * NW) return false;
+    return (m[gs >> 6] >> (gs & 63)) & 1ull;
+}
+template <int NW>
+PG_DEV int mask_rank(const uint64_t (&m)[NW], int gs) {
+    int r = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        if (w < (gs >> 6)) r += popc64(m[w]);
+        else if (w == (gs >> 6)) r += popc64(m[w] & ((1ull << (gs & 63)) - 1ull));
+    }
+    return r;
+}
+
+// Vertex of three planes (two lines in 2D) by the reference's Cramer rule
+// (cell.cpp:442-465), homogeneous part (X, Y, Z, W).
+template <int DIM>
+PG_DEV void cramer(const Plane& a, const Plane& b, const Plane& c, double& X, double& Y, double& Z,
+                   double& W) {
+    if (DIM == 2) {
+        X = -det2(a.w, a.y, b.w, b.y);
+        Y = -det2(a.x, a.w, b.x, b.w);
+        Z = 0.0;
+        W = det2(a.x, a.y, b.x, b.y);
+    } else {
+        X = -det3(a.w, a.y, a.z, b.w, b.y, b.z, c.w, c.y, c.z);
+        Y = -det3(a.x, a.w, a.z, b.x, b.w, b.z, c.x, c.w, c.z);
+        Z = -det3(a.x, a.y, a.w, b.x, b.y, b.w, c.x, c.y, c.w);
+        W = det3(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z);
+    }
+}
+
+// The reference conflict predicate (cell.cpp:229-275), evaluated exactly.
+template <int DIM>
+PG_DEV bool conflict_exact(const Plane& p1, const Plane& p2, const Plane& p3, const Plane& e,
+                           bool& flag) {
+    double det, eps;
+    if (DIM == 2) {
+        det = det3(p1.x, p2.x, e.x, p1.y, p2.y, e.y, p1.w, p2.w, e.w);
+        double mx = fmax(fmax(fabs(p1.x), fabs(p2.x)), fabs(e.x));
+        double my = fmax(fmax(fabs(p1.y), fabs(p2.y)), fabs(e.y));
+        double mw = fmax(fmax(fabs(p1.w), fabs(p2.w)), fabs(e.w));
+        double mm = fmax(fmax(mx, my), mw);
+        eps = 1e-14 * mx * my * mw;
+        eps *= mm;
+    } else {
+        det = det4(p1.x, p2.x, p3.x, e.x, p1.y, p2.y, p3.y, e.y, p1.z, p2.z, p3.z, e.z, p1.w, p2.w,
+                   p3.w, e.w);
+        double mx = fmax(fmax(fmax(fabs(p1.x), fabs(p2.x)), fabs(p3.x)), fabs(e.x));
+        double my = fmax(fmax(fmax(fabs(p1.y), fabs(p2.y)), fabs(p3.y)), fabs(e.y));
+        double mz = fmax(fmax(fmax(fabs(p1.z), fabs(p2.z)), fabs(p3.z)), fabs(e.z));
+        eps = 1e-14 * mx * my * mz;
+        double mm = fmax(fmax(mx, my), mz);
+        eps *= (mm * mm);
+    }
+    if (fabs(det) < eps) flag = true;
+    return det > 0.0;
+}
+
+template <int DIM>
+PG_DEV void plane_bounds(const Plane& p, float& n1, float& mx) {
+    n1 = f2f_up(fabs(p.x) + fabs(p.y) + fabs(p.z) + fabs(p.w));
+    double m = DIM == 2 ? fmax(fmax(fabs(p.x), fabs(p.y)), fabs(p.w))
+                        : fmax(fmax(fabs(p.x), fabs(p.y)), fabs(p.z));
+    mx = f2f_up(m);
+}
+
+// Per-seed register state of one lane.
+template <int S>
+struct SlotRegs {
+    double px[S], py[S], pz[S], pw[S], d2[S];
+    float q[S], m[S];
+    uint32_t live;  // bit s = slot s holds a vertex
+};
+
+template <int DIM, int S>
+PG_DEV void slot_write(SlotRegs<S>& r, int s, double px, double py, double pz, double pw, double d2,
+                       float q, float m) {
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+        if (i == s) {
+            r.px[i] = px;
+            r.py[i] = py;
+            r.pz[i] = pz;
+            r.pw[i] = pw;
+            r.d2[i] = d2;
+            r.q[i] = q;
+            r.m[i] = m;
+        }
+    }
+    r.live |= 1u << s;
+}
+
+// Build vertex numerics from its planes (reference arithmetic for position
+// and squared seed distance, cell.cpp:423-465).
+template <int DIM>
+PG_DEV void vertex_numerics(const Plane& a, const Plane& b, const Plane& c, double sx, double sy,
+                            double sz, double& px, double& py, double& pz, double& pw,
+                            double& d2) {
+    double X, Y, Z, W;
+    cramer<DIM>(a, b, c, X, Y, Z, W);
+    px = X / W;
+    py = Y / W;
+    pz = DIM == 3 ? Z / W : 0.0;
+    pw = W;
+    double dx = px - sx, dy = py - sy, dz = pz - sz;
+    d2 = dx * dx + dy * dy + dz * dz;
+}
+
+template <int DIM, int G, int S>
+__device__ void voronoi_group(const double* __restrict__ pts, const uint32_t* __restrict__ seeds,
+                              int64_t nq, const uint32_t* __restrict__ knn, int k, int max_p,
+                              int max_t, FaceArrays fa,
+                              SeedSmem<VoroCfg<DIM, G, S>::NS, VoroCfg<DIM, G, S>::MAXP>* smem) {
+    using C = VoroCfg<DIM, G, S>;
+    constexpr int NS = C::NS, NW = C::NW, MAXP = C::MAXP;
+    constexpr int D = DIM;
+    const int tid = (int)threadIdx.x;
+    const int grp = tid / G, gl = tid % G;
+    const int shift = grp * G;
+    const uint64_t gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+    const int64_t qi = (int64_t)blockIdx.x * C::SPW + grp;
+    const bool valid = qi < nq;
+    SeedSmem<NS, MAXP>& sm = smem[grp];
+    if (max_p > MAXP) max_p = MAXP;
+
+    const uint32_t sid = valid ? (seeds ? seeds[qi] : (uint32_t)qi) : 0u;
+    const uint32_t* row = knn + (valid ? (size_t)qi * (size_t)k : 0);
+    const double sx = pts[(size_t)D * sid], sy = pts[(size_t)D * sid + 1];
+    const double sz = D == 3 ? pts[(size_t)D * sid + 2] : 0.0;
+
+    // ---------------------------------------------------------------- init
+    // Box planes (cell.cpp:96-136): eps = 1e-14 margin.
+    if (gl < 2 * D) {
+        const double eps = 1e-14, lo = -eps, hi = 1.0 + eps;
+        Plane p;
+        switch (gl) {
+            case 0: p = {1.0, 0.0, 0.0, -lo}; break;
+            case 1: p = {-1.0, 0.0, 0.0, hi}; break;
+            case 2: p = {0.0, 1.0, 0.0, -lo}; break;
+            case 3: p = {0.0, -1.0, 0.0, hi}; break;
+            case 4: p = {0.0, 0.0, 1.0, -lo}; break;
+            default: p = {0.0, 0.0, -1.0, hi}; break;
+        }
+        sm.plane[gl] = p;
+        plane_bounds<D>(p, sm.pn1[gl], sm.pmx[gl]);
+        sm.pvid[gl] = -1;
+    }
+    // Initial vertices (cell.cpp:139-157).
+    constexpr int NV0 = D == 3 ? 8 : 4;
+    if (gl < NV0) {
+        uint32_t t;
+        if (D == 3) {
+            const uint32_t T3[8] = {2 | 5 << 8 | 0 << 16, 5 | 3 << 8 | 0 << 16, 1 | 5 << 8 | 2 << 16,
+                                    5 | 1 << 8 | 3 << 16, 4 | 2 << 8 | 0 << 16, 4 | 0 << 8 | 3 << 16,
+                                    2 | 4 << 8 | 1 << 16, 4 | 3 << 8 | 1 << 16};
+            t = T3[gl];
+        } else {
+            const uint32_t T2[4] = {2 | 0 << 8, 1 | 2 << 8, 3 | 1 << 8, 0 | 3 << 8};
+            t = T2[gl];
+        }
+        sm.tri[gl] = t;
+    }
+    block_sync();
+    SlotRegs<S> R;
+    R.live = 0;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        R.px[s] = R.py[s] = R.pz[s] = R.pw[s] = R.d2[s] = 0.0;
+        R.q[s] = R.m[s] = 0.f;
+    }
+    // NV0 <= G: initial vertex j lives in slot 0 of lane j.
+    if (gl < NV0) {
+        uint32_t t = sm.tri[gl];
+        const Plane& a = sm.plane[tri_p(t, 0)];
+        const Plane& b = sm.plane[tri_p(t, 1)];
+        const Plane& c = sm.plane[D == 3 ? tri_p(t, 2) : tri_p(t, 1)];
+        double px, py, pz, pw, d2;
+        vertex_numerics<D>(a, b, c, sx, sy, sz, px, py, pz, pw, d2);
+        float q = D == 3 ? sm.pn1[tri_p(t, 0)] * sm.pn1[tri_p(t, 1)] * sm.pn1[tri_p(t, 2)]
+                         : sm.pn1[tri_p(t, 0)] * sm.pn1[tri_p(t, 1)];
+        float m = fmaxf(sm.pmx[tri_p(t, 0)], sm.pmx[tri_p(t, 1)]);
+        if (D == 3) m = fmaxf(m, sm.pmx[tri_p(t, 2)]);
+        slot_write<D, S>(R, 0, px, py, pz, pw, d2, q * 1.000001f, m);
+        // adjacency of the initial polytope
+        for (int e = 0; e < D; e++) {
+            uint8_t nb = END;
+            if (D == 3) {
+                int a0 = tri_p(t, e), b0 = tri_p(t, (e + 1) % 3);
+                for (int j = 0; j < NV0; j++) {
+                    if (j == gl) continue;
+                    uint32_t u = sm.tri[j];
+                    for (int f = 0; f < 3; f++)
+                        if (tri_p(u, f) == b0 && tri_p(u, (f + 1) % 3) == a0) nb = (uint8_t)j;
+                }
+            } else {
+                int L = tri_p(t, e);
+                for (int j = 0; j < NV0; j++) {
+                    if (j == gl) continue;
+                    uint32_t u = sm.tri[j];
+                    if (tri_p(u, 0) == L || tri_p(u, 1) == L) nb = (uint8_t)j;
+                }
+            }
+            sm.adj[e][gl] = nb;
+        }
+    }
+    block_sync();
+
+    int nb_v = 2 * D;
+    int nlive = NV0;
+    int status = SUCCESS;
+    int kc = 0;
+    bool done = !valid;
+    double R2 = 0.0;
+
+    // --------------------------------------------------------- clip loop
+    for (int v = 0; v < k; v++) {
+        if (wave_ballot(!done) == 0ull) break;
+        const bool act = !done;
+        // -- new half plane (cell.cpp:212-226)
+        uint32_t z = 0;
+        double bx = 0, by = 0, bz = 0;
+        Plane e = {0, 0, 0, 0};
+        bool plane_ok = false;
+        int cur = 0;
+        if (act) {
+            z = row[v];
+            bx = pts[(size_t)D * z];
+            by = pts[(size_t)D * z + 1];
+            bz = D == 3 ? pts[(size_t)D * z + 2] : 0.0;
+            if (nb_v >= max_p) {
+                status = VERTEX_OVERFLOW;
+            } else {
+                double dx = sx - bx, dy = sy - by, dz = sz - bz;
+                double ax = sx + bx, ay = sy + by, az = sz + bz;
+                double dot = ax * dx + ay * dy + az * dz;
+                e = {dx, dy, dz, -dot / 2.0};
+                cur = nb_v;
+                plane_ok = true;
+                if (gl == 0) {
+                    sm.plane[cur] = e;
+                    plane_bounds<D>(e, sm.pn1[cur], sm.pmx[cur]);
+                    sm.pvid[cur] = (int32_t)z;
+                }
+                nb_v++;
+            }
+        }
+        // -- conflict test of every live vertex
+        uint32_t rem = 0;
+        bool flag = false;
+        if (act && plane_ok) {
+            const double en1 = (fabs(e.x) + fabs(e.y) + fabs(e.z) + fabs(e.w)) * (1.0 + 1e-15);
+            const double emx = D == 2 ? fmax(fmax(fabs(e.x), fabs(e.y)), fabs(e.w))
+                                      : fmax(fmax(fabs(e.x), fabs(e.y)), fabs(e.z));
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                if (!((R.live >> s) & 1u)) continue;
+                double t;
+                if (D == 3)
+                    t = R.pw[s] * fma(e.x, R.px[s], fma(e.y, R.py[s], fma(e.z, R.pz[s], e.w)));
+                else
+                    t = R.pw[s] * fma(e.x, R.px[s], fma(e.y, R.py[s], e.w));
+                double M = fmax((double)R.m[s], emx);
+                double M2 = M * M;
+                double epsu = D == 3 ? 1.0000001e-14 * M2 * M2 * M : 1.0000001e-14 * M2 * M2;
+                double bnd = 5.4e-15 * en1 * (double)R.q[s] + epsu;
+                bool c;
+                if (fabs(t) > bnd) {
+                    c = t > 0.0;
+                } else {
+                    uint32_t tt = sm.tri[s * G + gl];
+                    const Plane p1 = sm.plane[tri_p(tt, 0)];
+                    const Plane p2 = sm.plane[tri_p(tt, 1)];
+                    const Plane p3 = sm.plane[D == 3 ? tri_p(tt, 2) : tri_p(tt, 1)];
+                    c = conflict_exact<D>(p1, p2, p3, e, flag);
+                }
+                if (c) rem |= 1u << s;
+            }
+        }
+        // -- group masks
+        uint64_t Rm[NW], Lm[NW];
+#pragma unroll
+        for (int w = 0; w < NW; w++) Rm[w] = Lm[w] = 0ull;
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            uint64_t b = (wave_ballot((rem >> s) & 1u) >> shift) & gmask;
+            int pos = s * G;
+            Rm[pos >> 6] |= b << (pos & 63);
+        }
+        const bool gflag = ((wave_ballot(flag) >> shift) & gmask) != 0ull;
+        int nb_r = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) nb_r += popc64(Rm[w]);
+        bool doclip = false;
+        if (act && plane_ok) {
+            R.live &= ~rem;  // removed vertices leave the cell (also on failure)
+            if (gflag) {
+                status = NEEDS_EXACT_PREDICATES;
+            } else if (nb_r == 0) {
+                nb_v--;  // the plane clipped nothing (cell.cpp:187-190)
+            } else {
+                doclip = true;
+            }
+        }
+        // -- cavity boundary: edges of removed vertices whose neighbour is kept
+        int mycnt = 0;
+        if (doclip) {
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                if (!((rem >> s) & 1u)) continue;
+                const int gs = s * G + gl;
+#pragma unroll
+                for (int ee = 0; ee < D; ee++)
+                    if (!mask_has<NW>(Rm, sm.adj[ee][gs])) mycnt++;
+            }
+        }
+        int incl = mycnt;
+#pragma unroll
+        for (int d = 1; d < G; d <<= 1) {
+            int t = shfl_up(incl, d, G);
+            if (gl >= d) incl += t;
+        }
+        const int Bt = shfl(incl, G - 1, G);
+        int j = incl - mycnt;
+        if (doclip) {
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                if (!((rem >> s) & 1u)) continue;
+                const int gs = s * G + gl;
+                const uint32_t t = sm.tri[gs];
+#pragma unroll
+                for (int ee = 0; ee < D; ee++) {
+                    const int K = sm.adj[ee][gs];
+                    if (mask_has<NW>(Rm, K)) continue;
+                    if (j >= NS) {  // more boundary edges than slots: overflow below
+                        j++;
+                        continue;
+                    }
+                    if (D == 3) {
+                        const int a = tri_p(t, ee), b = tri_p(t, (ee + 1) % 3);
+                        sm.edge[j] = (uint32_t)a | (uint32_t)b << 8 | (uint32_t)K << 16;
+                        sm.by_start[a] = (uint8_t)j;
+                        sm.by_end[b] = (uint8_t)j;
+                    } else {
+                        const int L = tri_p(t, ee);
+                        sm.edge[j] = (uint32_t)L | (uint32_t)K << 16;
+                    }
+                    j++;
+                }
+            }
+        }
+        // live mask of the group (after removal)
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            uint64_t b = (wave_ballot((R.live >> s) & 1u) >> shift) & gmask;
+            int pos = s * G;
+            Lm[pos >> 6] |= b << (pos & 63);
+        }
+        block_sync();
+        // -- consistency and capacity
+        bool ok = doclip;
+        bool bad = false;
+        if (doclip) {
+            if (D == 3) {
+                for (int jj = gl; jj < Bt; jj += G) {
+                    uint32_t ed = sm.edge[jj];
+                    int a = ed & 0xff, b = (ed >> 8) & 0xff;
+                    if (sm.by_start[a] != jj || sm.by_end[b] != jj) bad = true;
+                }
+                if (gl == 0 && !bad) {
+                    // the boundary must be one cycle through all Bt edges
+                    int jj = 0, steps = 0;
+                    do {
+                        int b = (sm.edge[jj] >> 8) & 0xff;
+                        jj = sm.by_start[b];
+                        steps++;
+                    } while (jj != 0 && steps <= Bt);
+                    if (steps != Bt) bad = true;
+                }
+            } else {
+                bad = (Bt != 2);
+            }
+        }
+        bad = ((wave_ballot(bad) >> shift) & gmask) != 0ull;
+        if (doclip) {
+            const int nl_after = nlive - nb_r + Bt;
+            if (bad) {
+                status = INCONSISTENT_BOUNDARY;
+                ok = false;
+            } else if (nl_after >= max_t) {
+                status = TRIANGLE_OVERFLOW;  // cell.cpp:401-404
+                ok = false;
+            } else if (nl_after > NS) {
+                status = SLOT_OVERFLOW;
+                ok = false;
+            }
+            nlive -= nb_r;
+        }
+        // -- slots of the new vertices: the j-th boundary edge takes the
+        //    j-th free slot (slot-id order)
+        uint32_t asg = 0;
+        if (ok) {
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const int gs = s * G + gl;
+                if ((R.live >> s) & 1u) continue;
+                // free-slot rank = gs - (live slots below gs)
+                int r = gs - mask_rank<NW>(Lm, gs);
+                if (r < Bt) {
+                    sm.newslot[r] = (uint8_t)gs;
+                    asg |= 1u << s;
+                }
+            }
+        }
+        block_sync();
+        // -- new vertices (cell.cpp:346-364 orientation, :442-465 position)
+        if (ok) {
+            float cn1, cmx;
+            plane_bounds<D>(e, cn1, cmx);
+            while (asg) {
+                const int s = ffs64(asg);
+                asg &= asg - 1;
+                const int gs = s * G + gl;
+                const int r = gs - mask_rank<NW>(Lm, gs);
+                const uint32_t ed = sm.edge[r];
+                uint32_t t;
+                Plane pa, pb;
+                float q, m;
+                if (D == 3) {
+                    const int a = ed & 0xff, b = (ed >> 8) & 0xff, K = (ed >> 16) & 0xff;
+                    t = (uint32_t)cur | (uint32_t)a << 8 | (uint32_t)b << 16;
+                    pa = sm.plane[a];
+                    pb = sm.plane[b];
+                    q = cn1 * sm.pn1[a] * sm.pn1[b] * 1.000001f;
+                    m = fmaxf(cmx, fmaxf(sm.pmx[a], sm.pmx[b]));
+                    sm.tri[gs] = t;
+                    sm.adj[0][gs] = sm.newslot[sm.by_end[a]];
+                    sm.adj[1][gs] = (uint8_t)K;
+                    sm.adj[2][gs] = sm.newslot[sm.by_start[b]];
+                    const uint32_t tk = sm.tri[K];
+                    for (int f = 0; f < 3; f++)
+                        if (tri_p(tk, f) == b && tri_p(tk, (f + 1) % 3) == a) sm.adj[f][K] = (uint8_t)gs;
+                    double px, py, pz, pw, d2;
+                    vertex_numerics<D>(e, pa, pb, sx, sy, sz, px, py, pz, pw, d2);
+                    slot_write<D, S>(R, s, px, py, pz, pw, d2, q, m);
+                } else {
+                    const int L = ed & 0xff, K = (ed >> 16) & 0xff;
+                    pa = sm.plane[L];
+                    const double rw = det2(e.x, e.y, pa.x, pa.y);
+                    const bool swap = rw > 0;
+                    t = swap ? ((uint32_t)L | (uint32_t)cur << 8) : ((uint32_t)cur | (uint32_t)L << 8);
+                    const uint8_t other = sm.newslot[1 - r];
+                    sm.tri[gs] = t;
+                    sm.adj[0][gs] = swap ? (uint8_t)K : other;
+                    sm.adj[1][gs] = swap ? other : (uint8_t)K;
+                    const uint32_t tk = sm.tri[K];
+                    for (int f = 0; f < 2; f++)
+                        if (tri_p(tk, f) == L) sm.adj[f][K] = (uint8_t)gs;
+                    q = cn1 * sm.pn1[L] * 1.000001f;
+                    m = fmaxf(cmx, sm.pmx[L]);
+                    double px, py, pz, pw, d2;
+                    if (swap)
+                        vertex_numerics<D>(pa, e, e, sx, sy, sz, px, py, pz, pw, d2);
+                    else
+                        vertex_numerics<D>(e, pa, pa, sx, sy, sz, px, py, pz, pw, d2);
+                    slot_write<D, S>(R, s, px, py, pz, pw, d2, q, m);
+                }
+            }
+            nlive += Bt;
+        }
+        block_sync();
+        // -- certificate (cell.cpp:423-438) with the just-clipped neighbour
+        double mx = 0.0;
+#pragma unroll
+        for (int s = 0; s < S; s++)
+            if ((R.live >> s) & 1u) mx = smax(R.d2[s], mx);
+#pragma unroll
+        for (int d = 1; d < G; d <<= 1) mx = smax(shfl_xor(mx, d, G), mx);
+        if (act) {
+            R2 = mx;
+            const double dx = bx - sx, dy = by - sy, dz = bz - sz;
+            const double dz2 = dx * dx + dy * dy + dz * dz;
+            if (dz2 > 4 * R2) {
+                kc = v + 1;
+                done = true;
+            }
+            if (status != SUCCESS) done = true;
+        }
+    }
+
+    // final certificate with the k-th neighbour (voronoi.cpp:145-147)
+    if (valid) {
+        const uint32_t zl = row[k - 1];
+        const double dx = pts[(size_t)D * zl] - sx, dy = pts[(size_t)D * zl + 1] - sy;
+        const double dz = D == 3 ? pts[(size_t)D * zl + 2] - sz : 0.0;
+        if (!(dx * dx + dy * dy + dz * dz > 4 * R2)) status = SECURITY_RADIUS_NOT_REACHED;
+        if (status != SUCCESS) kc = 0;
+    }
+
+    // ---------------------------------------------------------- extraction
+    const bool ext = valid && status == SUCCESS;
+    // normals and neighbour ids of my planes (cell.cpp:629-633)
+    double nx[C::PPL], ny[C::PPL], nz[C::PPL];
+    int32_t pv[C::PPL];
+#pragma unroll
+    for (int i = 0; i < C::PPL; i++) {
+        const int p = gl + G * i;
+        nx[i] = ny[i] = nz[i] = 0.0;
+        pv[i] = -1;
+        if (ext && p < nb_v) {
+            const Plane pe = sm.plane[p];
+            const double nl = sqrt(pe.x * pe.x + pe.y * pe.y + pe.z * pe.z);
+            nx[i] = pe.x / nl;
+            ny[i] = pe.y / nl;
+            nz[i] = pe.z / nl;
+            pv[i] = sm.pvid[p];
+        }
+        if (p < MAXP) sm.first[p] = 0xffffffffu;
+    }
+    block_sync();
+    // vertex positions to LDS (over the planes), lowest slot per plane
+    if (ext) {
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            if (!((R.live >> s) & 1u)) continue;
+            const int gs = s * G + gl;
+            const uint32_t t = sm.tri[gs];
+#pragma unroll
+            for (int ee = 0; ee < D; ee++) atomicMin(&sm.first[tri_p(t, ee)], (uint32_t)gs);
+        }
+    }
+    block_sync();
+    if (ext) {
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            if (!((R.live >> s) & 1u)) continue;
+            const int gs = s * G + gl;
+            sm.vpos[3 * gs] = R.px[s];
+            sm.vpos[3 * gs + 1] = R.py[s];
+            sm.vpos[3 * gs + 2] = R.pz[s];
+        }
+    }
+    block_sync();
+    // faces: walk each plane's vertex cycle through the adjacency
+    double vol = 0.0, cxs = 0.0, cys = 0.0, czs = 0.0;
+    bool isface[C::PPL];
+    double farea[C::PPL], fcx[C::PPL], fcy[C::PPL], fcz[C::PPL];
+    const double* vpos = sm.vpos;
+#pragma unroll
+    for (int i = 0; i < C::PPL; i++) {
+        const int p = gl + G * i;
+        isface[i] = false;
+        farea[i] = fcx[i] = fcy[i] = fcz[i] = 0.0;
+        if (!(ext && p < nb_v)) continue;
+        const uint32_t v0 = sm.first[p];
+        if (v0 == 0xffffffffu) continue;
+        const double x0 = vpos[3 * v0], y0 = vpos[3 * v0 + 1], z0 = vpos[3 * v0 + 2];
+        if (D == 2) {
+            const uint32_t t0 = sm.tri[v0];
+            const int ee = tri_p(t0, 0) == p ? 0 : 1;
+            const int v1 = sm.adj[ee][v0];
+            const double x1 = vpos[3 * v1], y1 = vpos[3 * v1 + 1];
+            const double dx = x1 - x0, dy = y1 - y0;
+            farea[i] = sqrt(dx * dx + dy * dy);
+            fcx[i] = 0.5 * (x0 + x1);
+            fcy[i] = 0.5 * (y0 + y1);
+            const double ta = 0.5 * fabs((x0 - sx) * (y1 - sy) - (x1 - sx) * (y0 - sy));
+            vol += ta;
+            cxs += ta * (sx + x0 + x1) / 3.0;
+            cys += ta * (sy + y0 + y1) / 3.0;
+            isface[i] = true;
+        } else {
+            int cnt = 0;
+            uint32_t cv = v0;
+            double ax = 0, ay = 0, az = 0;  // area-weighted centroid sums
+            double area = 0.0;
+            double px = 0, py = 0, pz = 0;
+            bool have_prev = false;
+            do {
+                const double x = vpos[3 * cv], y = vpos[3 * cv + 1], zz = vpos[3 * cv + 2];
+                if (have_prev && cnt >= 2) {
+                    const double e1x = px - x0, e1y = py - y0, e1z = pz - z0;
+                    const double e2x = x - x0, e2y = y - y0, e2z = zz - z0;
+                    const double crx = e1y * e2z - e1z * e2y;
+                    const double cry = e1z * e2x - e1x * e2z;
+                    const double crz = e1x * e2y - e1y * e2x;
+                    const double ta = 0.5 * sqrt(crx * crx + cry * cry + crz * crz);
+                    area += ta;
+                    ax += ta * (x0 + px + x) / 3.0;
+                    ay += ta * (y0 + py + y) / 3.0;
+                    az += ta * (z0 + pz + zz) / 3.0;
+                    // tetra (seed, v0, prev, cur): seed inside => same sign on a face
+                    const double a1 = x0 - sx, a2 = y0 - sy, a3 = z0 - sz;
+                    const double b1 = px - sx, b2 = py - sy, b3 = pz - sz;
+                    const double c1 = x - sx, c2 = y - sy, c3 = zz - sz;
+                    const double tv =
+                        fabs(a1 * (b2 * c3 - b3 * c2) + a2 * (b3 * c1 - b1 * c3) + a3 * (b1 * c2 - b2 * c1)) /
+                        6.0;
+                    vol += tv;
+                    cxs += tv * (sx + x0 + px + x) / 4.0;
+                    cys += tv * (sy + y0 + py + y) / 4.0;
+                    czs += tv * (sz + z0 + pz + zz) / 4.0;
+                }
+                px = x;
+                py = y;
+                pz = zz;
+                have_prev = true;
+                cnt++;
+                const uint32_t t = sm.tri[cv];
+                const int ee = tri_p(t, 0) == p ? 0 : (tri_p(t, 1) == p ? 1 : 2);
+                cv = sm.adj[ee][cv];
+            } while (cv != v0 && cnt <= NS);
+            if (cnt >= 3) {
+                isface[i] = true;
+                farea[i] = area;
+                if (area > 0) {
+                    fcx[i] = ax / area;
+                    fcy[i] = ay / area;
+                    fcz[i] = az / area;
+                }
+            }
+        }
+    }
+    // face order = plane order among planes that form a face
+    uint64_t fm = 0;
+#pragma unroll
+    for (int i = 0; i < C::PPL; i++) {
+        uint64_t b = (wave_ballot(isface[i]) >> shift) & gmask;
+        fm |= b << (G * i);
+    }
+    // reduce cell volume and centroid sums over the group
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+        vol += shfl_xor(vol, d, G);
+        cxs += shfl_xor(cxs, d, G);
+        cys += shfl_xor(cys, d, G);
+        czs += shfl_xor(czs, d, G);
+    }
+    if (valid) {
+        const int64_t st = fa.stride;
+#pragma unroll
+        for (int i = 0; i < C::PPL; i++) {
+            const int p = gl + G * i;
+            if (!isface[i]) continue;
+            const int fi = popc64(fm & ((1ull << p) - 1ull));
+            const int64_t at = (int64_t)fi * st + sid;
+            fa.nbr[at] = pv[i];
+            fa.area[at] = farea[i];
+            const int64_t plane_stride = st * MAXP;
+            fa.nrm[at] = nx[i];
+            fa.nrm[plane_stride + at] = ny[i];
+            fa.nrm[2 * plane_stride + at] = nz[i];
+            fa.cen[at] = fcx[i];
+            fa.cen[plane_stride + at] = fcy[i];
+            fa.cen[2 * plane_stride + at] = D == 3 ? fcz[i] : 0.0;
+        }
+        if (gl == 0) {
+            fa.count[sid] = ext ? (uint8_t)popc64(fm) : (uint8_t)0;
+            fa.vol[sid] = ext ? vol : 0.0;
+            const double iv = vol != 0.0 ? 1.0 / vol : 0.0;
+            fa.ccen[sid] = ext ? cxs * iv : 0.0;
+            fa.ccen[st + sid] = ext ? cys * iv : 0.0;
+            fa.ccen[2 * st + sid] = ext ? czs * iv : 0.0;
+            fa.status[sid] = status;
+            fa.kcert[sid] = kc;
+        }
+    }
+}
+
+}  // namespace pg

@@ -1,0 +1,128 @@
+"""GPU parity: libpgpu (through the C ABI) against the FP64 oracle.
+
+Bar: KNN lists, sorted points, permutation and Voronoi topology (face counts,
+neighbour ids, statuses, certified k) bit-exact; normals bit-exact (same
+arithmetic on the same planes); volumes and areas within rtol 1e-12 (the GPU
+sums fan triangles in its own vertex order); hydro states within rtol 1e-11
+after several steps (geometry round-off propagated; dt within rtol 1e-13
+because cbrt is not pinned across libms).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from proteusgpu_amd import api
+from proteusgpu_amd.generators import sod_state, uniform_points
+
+pytestmark = pytest.mark.gpu
+
+
+def cubic_lattice(m):
+    g = (np.arange(m) + 0.5) / m
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    return np.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+
+
+@pytest.mark.parametrize("dim,n,k,seed", [(2, 2000, 16, 1), (3, 13000, 32, 2), (2, 5000, 48, 3)])
+def test_knn_bitexact(ctx, dim, n, k, seed):
+    print(f"knn dim={dim} n={n} k={k} seed={seed}", flush=True)
+    pts = uniform_points(n, dim, seed)
+    kp = api.KnnProblem(ctx, pts)
+    kp.solve(k)
+    N, s, perm, kn = O.knn(pts, k)
+    assert kp.N_grid == N
+    assert np.array_equal(kp.get_points(), s)
+    assert np.array_equal(kp.get_permutation(), perm)
+    assert np.array_equal(kp.get_knearest(), kn)
+
+
+def test_knn_ties_lattice(ctx):
+    """Cubic lattice: every distance is tied many times; order is (dist^2, original index)."""
+    pts = cubic_lattice(24)
+    print("knn lattice n=", len(pts), flush=True)
+    kp = api.KnnProblem(ctx, pts)
+    kp.solve(26)
+    _, _, _, kn = O.knn(pts, 26)
+    assert np.array_equal(kp.get_knearest(), kn)
+
+
+def _cmp_mesh(g, ref, geom_rtol=1e-12):
+    # 2D volumes: the reference's shoelace (cell.cpp:529-552) works in absolute
+    # coordinates, so its round-off is absolute (~u |x|^2), not relative.
+    vol_atol = 1e-15 if ref.dim == 2 else 0.0
+    assert np.array_equal(g["status"], ref.get("status"))
+    assert np.array_equal(g["kcert"], ref.get("kcert"))
+    assert np.array_equal(g["face_counts"], ref.get("face_counts"))
+    assert np.array_equal(g["face_ptr"], ref.get("face_ptr"))
+    assert np.array_equal(g["neighbor_cell"], ref.get("neighbor_cell"))
+    assert np.array_equal(g["face_normal"], ref.get("face_normal"))
+    np.testing.assert_allclose(g["volumes"], ref.get("volumes"), rtol=geom_rtol, atol=vol_atol)
+    np.testing.assert_allclose(g["face_area"], ref.get("face_area"), rtol=geom_rtol, atol=1e-300)
+    np.testing.assert_array_equal(g["seeds"], ref.get("seeds"))
+
+
+@pytest.mark.parametrize("dim,n,k,seed", [(2, 2000, 32, 4), (3, 13000, 64, 5), (3, 13000, 32, 6)])
+def test_mesh_reference_semantics(ctx, dim, n, k, seed):
+    """voronoi::compute_mesh at fixed k, no restart: statuses (incl. failures) match."""
+    print(f"mesh none dim={dim} n={n} k={k}", flush=True)
+    pts = uniform_points(n, dim, seed)
+    m = api.Mesh(ctx, pts, boundary="none", k=k, restart=False)
+    ref = O.Mesh(pts, "none", k=k, restart=False)
+    assert m.n_failed == ref.n_failed
+    _cmp_mesh(m.copy_out(), ref)
+
+
+@pytest.mark.parametrize("dim,n,k,buff", [(2, 10000, 16, 0.04), (3, 20000, 32, 0.12)])
+def test_periodic_restart(ctx, dim, n, k, buff):
+    """Periodic ghosts + certificate restarts: every cell certified, identical to the oracle."""
+    print(f"periodic restart dim={dim} n={n} k={k}", flush=True)
+    pts = uniform_points(n, dim, 7)
+    m = api.Mesh(ctx, pts, boundary="periodic", buff=buff, k=k, restart=True)
+    ref = O.Mesh(pts, "periodic", buff=buff, k=k, restart=True)
+    assert m.n_failed == 0 and ref.n_failed == 0
+    g = m.copy_out()
+    _cmp_mesh(g, ref)
+    assert np.array_equal(g["ghost_ids"], ref.get("ghost_ids"))
+    vol = g["volumes"][: m.n_hydro].sum()
+    assert abs(vol - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("flux", ["reference", "physical"])
+def test_hydro_steps(ctx, flux):
+    dim, n, k, gamma = 2, 10000, 48, 1.4
+    print(f"hydro flux={flux}", flush=True)
+    pts = uniform_points(n, dim, 8)
+    rho, vel, E = sod_state(pts, gamma)
+    m = api.Mesh(ctx, pts, boundary="periodic", k=k, restart=True)
+    ref = O.Mesh(pts, "periodic", k=k, restart=True)
+    st = api.HydroState(ctx, m, rho, vel, E)
+    r, v, e = rho.copy(), vel.copy(), E.copy()
+    mode = 0 if flux == "reference" else 1
+    for _ in range(3):
+        dt_g = st.dt_cfl(gamma, 0.3)
+        dt_o = ref.dt_cfl(gamma, 0.3, r, v, e)
+        assert abs(dt_g - dt_o) <= 1e-13 * dt_o
+        st.step(gamma, dt_o, flux)
+        ref.hydro_step(gamma, dt_o, r, v, e, flux_mode=mode)
+    rg, vg, eg = st.copy_out()
+    np.testing.assert_allclose(rg, r, rtol=1e-11)
+    np.testing.assert_allclose(eg, e, rtol=1e-11)
+    np.testing.assert_allclose(vg, v, rtol=1e-9, atol=1e-12)
+
+
+def test_reflective_box(ctx):
+    """Box planes are walls: no ghosts, unit volume, no mass through walls."""
+    dim, n, k, gamma = 3, 20000, 32, 1.4
+    pts = uniform_points(n, dim, 9)
+    m = api.Mesh(ctx, pts, boundary="reflective", k=k, restart=True)
+    ref = O.Mesh(pts, "reflective", k=k, restart=True)
+    g = m.copy_out()
+    _cmp_mesh(g, ref)
+    assert abs(g["volumes"].sum() - 1.0) < 1e-12
+    rho, vel, E = sod_state(pts, gamma)
+    st = api.HydroState(ctx, m, rho, vel, E)
+    mass0 = np.sum(rho * g["volumes"])
+    for _ in range(3):
+        st.step(gamma, st.dt_cfl(gamma, 0.3), "physical")
+    rg, _, _ = st.copy_out()
+    assert abs(np.sum(rg * g["volumes"]) - mass0) < 1e-12 * mass0
