@@ -148,6 +148,49 @@ int pg_state_copy_out(pg_ctx* ctx, const pg_state* st, double* rho, double* vel,
 /* hydro::free_prim (finite_volume_solver.h:21). */
 int pg_state_free(pg_state** st);
 
+/* --------------------------------------------------- moving-mesh simulation */
+/* The per-timestep loop of main.cpp:50-63 kept resident in HBM: generators,
+ * conserved state (FP64 mass, momentum, energy) and the mesh never leave the
+ * device between steps.  One pg_sim_step = primitives from the conserved
+ * state, CFL dt, HLL update, generator drift (move_mesh) and one full
+ * Voronoi rebuild (ghosts, grid, KNN, clip, restarts).  With move_mesh = 0
+ * the mesh is static as in the reference driver (main.cpp:41). */
+typedef struct {
+    pg_mesh_params mesh;
+    double gamma;   /* adiabatic index (allvars.cpp:14 hard-codes 5/3)           */
+    double cfl;     /* CFL number (main.cpp:47 hard-codes 0.4)                   */
+    int flux_mode;  /* PG_FLUX_*                                                 */
+    int move_mesh;  /* 1: generators move with the fluid, ALE face fluxes        */
+} pg_sim_params;
+
+/* Stage times of the last pg_sim_step, milliseconds (HIP events). */
+typedef struct {
+    double hydro_ms;   /* primitives + dt + HLL update + drift                 */
+    double ghosts_ms;  /* ghost generation (periodic)                          */
+    double grid_ms;    /* grid counting sort                                   */
+    double knn_ms;     /* KNN of every seed at the configured k                */
+    double clip_ms;    /* first clip pass                                      */
+    double restart_ms; /* certificate restarts                                 */
+    double final_ms;   /* hydro face tables                                    */
+    double total_ms;
+    int64_t n_restarted;
+    int k_used;
+} pg_sim_times;
+
+int pg_sim_create(pg_ctx* ctx, const pg_sim_params* params, const double* pts, const double* rho,
+                  const double* vel, const double* E, int64_t n, pg_sim** out);
+/* dt_max > 0 caps the CFL step (main.cpp:54-55 clamps to t_end). */
+int pg_sim_step(pg_ctx* ctx, pg_sim* sim, double dt_max, double* dt_used);
+/* nsteps static-mesh HLL updates on the current mesh (no rebuild). */
+int pg_sim_hydro_steps(pg_ctx* ctx, pg_sim* sim, int nsteps, double dt_max, double* t_advanced);
+int pg_sim_times_last(const pg_sim* sim, pg_sim_times* t);
+int pg_sim_mesh(const pg_sim* sim, const pg_mesh** mesh);
+/* Primitives (original order) and current generator positions. */
+int pg_sim_get(pg_ctx* ctx, const pg_sim* sim, double* pts, double* rho, double* vel, double* E);
+/* FP64 totals: [mass, px, py, pz, energy]. */
+int pg_sim_totals(pg_ctx* ctx, pg_sim* sim, double* tot5);
+int pg_sim_free(pg_sim** sim);
+
 #ifdef __cplusplus
 }
 #endif

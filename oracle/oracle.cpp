@@ -681,8 +681,10 @@ int64_t periodic_ghosts(int dim, const double* pts, int64_t n, double buff, doub
 }
 
 // voronoi.cpp:115-156 for one seed with its neighbour row; returns kcert.
+// A row that holds every other seed (k == n - 1) certifies the cell by
+// exhaustion; the reference would report security_radius_not_reached there.
 int run_cell(int dim, int maxp, int maxt, const double* sorted, const uint32_t* row, int k, int sid,
-             CellOut* out, int* status) {
+             CellOut* out, int* status, bool exhaustive) {
     Cell c(dim, maxp, maxt, sorted, sid);
     int kc = 0;
     for (int v = 0; v < k; v++) {
@@ -694,7 +696,12 @@ int run_cell(int dim, int maxp, int maxt, const double* sorted, const uint32_t* 
         }
         if (c.status != success) break;
     }
-    if (!c.radius_reached(c.point((int)row[k - 1]))) c.status = security_radius_not_reached;
+    if (!c.radius_reached(c.point((int)row[k - 1]))) {
+        if (exhaustive && c.status == success)
+            kc = k;
+        else
+            c.status = security_radius_not_reached;
+    }
     if (c.status != success) kc = 0;
     *status = c.status;
     if (c.status == success && out) extract(c, *out);
@@ -749,7 +756,8 @@ Mesh* build(int dim, const double* in_pts, int64_t n, int boundary, double buff,
         // same cell as a restart with that longer list.
         int kk = restart ? kmax : k;
         std::vector<uint32_t> row(m->knn.begin() + s * kmax, m->knn.begin() + s * kmax + kk);
-        kc[s] = run_cell(dim, maxp, maxt, m->sorted.data(), row.data(), kk, (int)s, &cells[s], &st[s]);
+        kc[s] = run_cell(dim, maxp, maxt, m->sorted.data(), row.data(), kk, (int)s, &cells[s], &st[s],
+                         kk >= ntot - 1);
     }
     // unpermute (voronoi.cpp:212-333) and, for periodic, unscale (periodic_mesh.cpp:170-190)
     std::vector<int64_t> o2s(ntot);
@@ -1052,6 +1060,22 @@ int or_hydro_step(void* h, double gam, double dt, int flux_mode, double* rho, do
     std::memcpy(vel, nv.data(), sizeof(double) * nh * dim);
     std::memcpy(E, nE.data(), sizeof(double) * nh);
     return bad;
+}
+
+void or_hll(int dim, double gam, const double* UL, const double* UR, const double* n, int wall, double* out) {
+    Prim L, R;
+    L.rho = UL[0];
+    R.rho = UR[0];
+    for (int d = 0; d < 3; d++) {
+        L.v[d] = d < dim ? UL[1 + d] : 0.0;
+        R.v[d] = d < dim ? UR[1 + d] : 0.0;
+    }
+    L.E = UL[4];
+    R.E = UR[4];
+    Prim F = hll(dim, gam, L, R, n, wall != 0);
+    out[0] = F.rho;
+    for (int d = 0; d < 3; d++) out[1 + d] = F.v[d];
+    out[4] = F.E;
 }
 
 }  // extern "C"

@@ -71,6 +71,12 @@ double or_dt_cfl(void* h, double gamma, double cfl, const double* rho, const dou
 int or_hydro_step(void* h, double gamma, double dt, int flux_mode, double* rho, double* vel,
                   double* E);
 
+/* riemann.cpp:7-45 — HLL flux for states (rho, vx, vy, vz, E) across a face
+ * with normal n (the frame's +x axis), lab frame result in out[5].
+ * wall != 0: the right state is the mirror of the left one. */
+void or_hll(int dim, double gamma, const double* UL, const double* UR, const double* n, int wall,
+            double* out);
+
 #ifdef __cplusplus
 }
 #endif

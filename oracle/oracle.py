@@ -39,6 +39,8 @@ def lib():
         L.or_dt_cfl.argtypes = [C.c_void_p, C.c_double, C.c_double, dp, dp, dp]
         L.or_dt_cfl.restype = C.c_double
         L.or_hydro_step.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, dp, dp, dp]
+        L.or_hll.argtypes = [C.c_int, C.c_double, dp, dp, dp, C.c_int, dp]
+        L.or_hll.restype = None
         _LIB = L
     return _LIB
 
@@ -78,6 +80,14 @@ def knn(pts, k):
     out = np.empty(n * k, dtype=np.uint32)
     lib().or_knn(dim, s, n, N, cnt, start, perm, k, out)
     return N, s, perm, out.reshape(n, k)
+
+
+def hll(dim, gamma, UL, UR, n, wall=False):
+    """HLL flux (riemann.cpp:7-45) of states (rho, vx, vy, vz, E) across normal n."""
+    out = np.zeros(5)
+    lib().or_hll(dim, gamma, np.asarray(UL, dtype=np.float64), np.asarray(UR, dtype=np.float64),
+                 np.asarray(n, dtype=np.float64), int(wall), out)
+    return out
 
 
 _I8 = ("face_counts", "face_ptr", "ghost_ids")

@@ -49,9 +49,13 @@ class SimParams(C.Structure):
 
 
 class SimTimes(C.Structure):
-    _fields_ = [("ghosts_ms", C.c_double), ("grid_ms", C.c_double), ("knn_ms", C.c_double),
-                ("clip_ms", C.c_double), ("restart_ms", C.c_double), ("hydro_ms", C.c_double),
-                ("total_ms", C.c_double)]
+    _fields_ = [("hydro_ms", C.c_double), ("ghosts_ms", C.c_double), ("grid_ms", C.c_double),
+                ("knn_ms", C.c_double), ("clip_ms", C.c_double), ("restart_ms", C.c_double),
+                ("final_ms", C.c_double), ("total_ms", C.c_double), ("n_restarted", C.c_int64),
+                ("k_used", C.c_int)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 _lib = None
@@ -83,6 +87,14 @@ def lib():
         "pg_dt_cfl": ([vp, vp, vp, dbl, dbl, C.POINTER(C.c_double)], i32),
         "pg_hydro_step": ([vp, vp, vp, dbl, dbl, i32], i32),
         "pg_state_copy_out": ([vp, vp, vp, vp, vp], i32), "pg_state_free": ([pp], i32),
+        "pg_sim_create": ([vp, C.POINTER(SimParams), vp, vp, vp, vp, i64, pp], i32),
+        "pg_sim_step": ([vp, vp, dbl, C.POINTER(C.c_double)], i32),
+        "pg_sim_hydro_steps": ([vp, vp, i32, dbl, C.POINTER(C.c_double)], i32),
+        "pg_sim_times_last": ([vp, C.POINTER(SimTimes)], i32),
+        "pg_sim_mesh": ([vp, pp], i32),
+        "pg_sim_get": ([vp, vp, vp, vp, vp, vp], i32),
+        "pg_sim_totals": ([vp, vp, vp], i32),
+        "pg_sim_free": ([pp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -103,4 +115,6 @@ def exported_symbols():
             "pg_ctx_sync", "pg_knn_init", "pg_knn_solve", "pg_knn_info", "pg_knn_get_points",
             "pg_knn_get_knearest", "pg_knn_get_permutation", "pg_knn_free", "pg_mesh_params_default",
             "pg_mesh_compute", "pg_mesh_info", "pg_mesh_copy_out", "pg_mesh_free", "pg_state_create",
-            "pg_dt_cfl", "pg_hydro_step", "pg_state_copy_out", "pg_state_free"]
+            "pg_dt_cfl", "pg_hydro_step", "pg_state_copy_out", "pg_state_free", "pg_sim_create",
+            "pg_sim_step", "pg_sim_hydro_steps", "pg_sim_times_last", "pg_sim_mesh", "pg_sim_get",
+            "pg_sim_totals", "pg_sim_free"]

@@ -180,3 +180,77 @@ class HydroState:
 
     def __del__(self):
         self.free()
+
+
+class Simulation:
+    """Device-resident moving-mesh loop (main.cpp:50-63 kept in HBM).
+
+    step(): primitives, CFL dt, HLL update, drift (move_mesh) and one full
+    Voronoi rebuild.  hydro_steps(n): n static-mesh HLL updates.
+    """
+
+    def __init__(self, ctx: Context, pts, rho, vel, E, boundary="reflective", buff=0.04, k=32,
+                 gamma=5.0 / 3.0, cfl=0.4, flux="physical", move_mesh=True, max_p=64, max_t=96,
+                 restart=True, k_max=256):
+        from ._native import SimParams
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        self.ctx, self.n, self.dim = ctx, pts.shape[0], pts.shape[1]
+        p = SimParams()
+        p.mesh = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max)
+        p.gamma, p.cfl, p.flux_mode, p.move_mesh = gamma, cfl, FLUX[flux], int(move_mesh)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, vel, E)]
+        h = C.c_void_p()
+        check(lib().pg_sim_create(ctx.h, C.byref(p), _ptr(pts), _ptr(arrs[0]), _ptr(arrs[1]),
+                                  _ptr(arrs[2]), self.n, C.byref(h)))
+        self.h = h
+
+    def step(self, dt_max: float = 0.0) -> float:
+        dt = C.c_double()
+        check(lib().pg_sim_step(self.ctx.h, self.h, dt_max, C.byref(dt)))
+        return dt.value
+
+    def hydro_steps(self, n: int, dt_max: float = 0.0) -> float:
+        t = C.c_double()
+        check(lib().pg_sim_hydro_steps(self.ctx.h, self.h, n, dt_max, C.byref(t)))
+        return t.value
+
+    def times(self) -> dict:
+        from ._native import SimTimes
+        t = SimTimes()
+        check(lib().pg_sim_times_last(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def get(self):
+        pts = np.empty((self.n, self.dim))
+        rho = np.empty(self.n)
+        vel = np.empty((self.n, self.dim))
+        E = np.empty(self.n)
+        check(lib().pg_sim_get(self.ctx.h, self.h, _ptr(pts), _ptr(rho), _ptr(vel), _ptr(E)))
+        return pts, rho, vel, E
+
+    def totals(self):
+        t = np.zeros(5)
+        check(lib().pg_sim_totals(self.ctx.h, self.h, _ptr(t)))
+        return t
+
+    def mesh_copy_out(self):
+        """Reference-layout VMesh of the current mesh."""
+        hm = C.c_void_p()
+        check(lib().pg_sim_mesh(self.h, C.byref(hm)))
+        m = Mesh.__new__(Mesh)
+        m.ctx, m.dim, m.h = self.ctx, self.dim, hm
+        info = MeshInfo()
+        check(lib().pg_mesh_info(hm, C.byref(info)))
+        m.n_seeds, m.n_hydro, m.num_faces = info.n_seeds, info.n_hydro, info.num_faces
+        try:
+            return m.copy_out()
+        finally:
+            m.h = None  # owned by the simulation
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().pg_sim_free(C.byref(self.h))
+            self.h = None
+
+    def __del__(self):
+        self.free()

@@ -200,13 +200,31 @@ void run_list(pg_mesh* m, const uint32_t* list, int64_t nq, int k, int big, hipS
     HIPCHK(hipGetLastError());
 }
 
-void mesh_build(pg_ctx* c, pg_mesh* m, const double* pts_host, int64_t n) {
+void count_faces(pg_mesh* m, hipStream_t s);
+
+// Stage markers of one rebuild (HIP events on the context stream).
+struct Marks {
+    hipEvent_t ev[12] = {};
+    bool on = false;
+    void mark(int i, hipStream_t s) {
+        if (on) HIPCHK(hipEventRecord(ev[i], s));
+    }
+};
+enum { MK_START, MK_GHOSTS, MK_GRID, MK_KNN, MK_CLIP, MK_RESTART, MK_FINAL, MK_HYDRO, MK_END, MK_N };
+
+void mesh_build(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool on_device = false,
+                Marks* mk = nullptr) {
     hipStream_t s = c->stream;
     const pg_mesh_params& p = m->prm;
     const int dim = p.dim;
     m->n_in = n;
-    double* din = m->in_pts.get<double>((size_t)n * dim);
-    HIPCHK(hipMemcpyAsync(din, pts_host, sizeof(double) * n * dim, hipMemcpyHostToDevice, s));
+    double* din;
+    if (on_device) {
+        din = const_cast<double*>(pts);
+    } else {
+        din = m->in_pts.get<double>((size_t)n * dim);
+        HIPCHK(hipMemcpyAsync(din, pts, sizeof(double) * n * dim, hipMemcpyHostToDevice, s));
+    }
     int64_t nseeds = n;
     if (p.boundary == PG_BOUNDARY_PERIODIC) {
         int32_t* gc = m->gcnt.get<int32_t>(n);
@@ -233,7 +251,9 @@ void mesh_build(pg_ctx* c, pg_mesh* m, const double* pts_host, int64_t n) {
     m->g.N = ngrid(dim, nseeds);
     if (p.boundary != PG_BOUNDARY_REFLECTIVE && m->g.N < 16)
         throw Error{PG_ERR_GRID, "KNN: N_grid < 16 (the reference's minimum mesh size)"};
+    if (mk) mk->mark(MK_GHOSTS, s);
     m->g.build(s);
+    if (mk) mk->mark(MK_GRID, s);
 
     // face and cell outputs
     const size_t ns = (size_t)nseeds;
@@ -253,9 +273,11 @@ void mesh_build(pg_ctx* c, pg_mesh* m, const double* pts_host, int64_t n) {
     uint32_t* rows = m->knn.get<uint32_t>(ns * k);
     pg::launch_knn(dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), nseeds, m->g.N,
                    m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), nullptr, nseeds, k, rows, s);
+    if (mk) mk->mark(MK_KNN, s);
     pg::launch_voronoi(dim, m->g.sorted.as<double>(), nseeds, nullptr, nseeds, rows, k, p.max_p, p.max_t,
                        0, m->faces(), s);
     HIPCHK(hipGetLastError());
+    if (mk) mk->mark(MK_CLIP, s);
 
     // restarts: uncertified seeds at the next k; seeds that outgrew the fast
     // kernel's 64 vertex slots in the 128-slot kernel
@@ -284,13 +306,20 @@ void mesh_build(pg_ctx* c, pg_mesh* m, const double* pts_host, int64_t n) {
         }
         if (!work) break;
     }
+    if (mk) mk->mark(MK_RESTART, s);
     m->info.n_seeds = nseeds;
     m->info.n_hydro = m->n_hydro;
     m->info.N_grid = m->g.N;
     m->info.k_used = sched[ki];
     m->info.n_restarted = restarted;
     m->info.n_failed = collect(m, -1, lst, s);
-    // total faces
+    if (!on_device) count_faces(m, s);
+}
+
+// Total faces over all cells (VMesh.num_faces).
+void count_faces(pg_mesh* m, hipStream_t s) {
+    const size_t ns = (size_t)m->n_seeds;
+    const int64_t nseeds = m->n_seeds;
     int32_t* co = m->flags.get<int32_t>(ns);
     int32_t* po = m->offs.get<int32_t>(ns);
     int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(std::max<int64_t>(nseeds, m->g.ncell)));
@@ -306,16 +335,23 @@ void mesh_finalize(pg_ctx* c, pg_mesh* m) {
     const double sc = m->scale();
     const double as = m->prm.dim == 2 ? sc : sc * sc;
     const double vs = m->prm.dim == 2 ? sc * sc : sc * sc * sc;
-    int32_t* bad = m->bad.get<int32_t>(1);
-    HIPCHK(hipMemsetAsync(bad, 0, sizeof(int32_t), s));
-    pg::launch_finalize(m->n_seeds, m->n_hydro, m->n_seeds, m->g.perm.as<uint32_t>(), m->gsrc.as<int64_t>(),
+    int32_t* bad = m->bad.get<int32_t>(2);
+    HIPCHK(hipMemsetAsync(bad, 0, 2 * sizeof(int32_t), s));
+    const int64_t* gsrc = m->prm.boundary == PG_BOUNDARY_PERIODIC ? m->gsrc.as<int64_t>() : nullptr;
+    pg::launch_finalize(m->n_seeds, m->n_hydro, m->n_seeds, m->g.perm.as<uint32_t>(), gsrc,
                         m->count.as<uint8_t>(), m->nbr.as<int32_t>(), m->area.as<double>(),
                         m->vol.as<double>(), as, vs, m->hid.get<int32_t>(ns * MAXP),
                         m->area_u.get<double>(ns * MAXP), m->vol_u.get<double>(ns),
                         m->o2s.get<int32_t>(std::max<int64_t>(m->n_hydro, 1)), bad, s);
     HIPCHK(hipGetLastError());
-    if (read_i32(bad, s) != 0)
-        throw Error{PG_ERR_CELLS, "hydro cells without faces (uncertified cells)"};
+    int32_t hb[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(hb, bad, sizeof(hb), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hb[0] != 0) throw Error{PG_ERR_CELLS, "hydro cells without faces (uncertified cells)"};
+    if (hb[1] != 0)
+        throw Error{PG_ERR_CELLS, std::to_string(hb[1]) +
+                                      " hydro faces reach the box plane: the periodic ghost layer (buff) is "
+                                      "thinner than the cells; increase buff"};
     m->finalized = true;
 }
 
@@ -666,6 +702,228 @@ int pg_state_free(pg_state** st) {
         if (!st || !*st) return;
         delete *st;
         *st = nullptr;
+    });
+}
+
+}  // extern "C"
+
+// ================================================================ simulation
+struct pg_sim {
+    pg_sim_params prm{};
+    pg_mesh mesh;
+    int64_t n = 0;
+    DBuf x, mass, mom, en, rho, vel, E, dt, part;
+    Marks mk;
+    pg_sim_times last{};
+    ~pg_sim() {
+        for (hipEvent_t& e : mk.ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    pg::SimView view() {
+        return pg::SimView{n, mass.as<double>(), mom.as<double>(), en.as<double>(),
+                           rho.as<double>(), vel.as<double>(), E.as<double>()};
+    }
+};
+
+namespace {
+
+void sim_rebuild(pg_ctx* c, pg_sim* S, Marks* mk) {
+    mesh_build(c, &S->mesh, S->x.as<double>(), S->n, true, mk);
+    if (S->mesh.info.n_failed)
+        throw Error{PG_ERR_CELLS, std::to_string(S->mesh.info.n_failed) + " cells left uncertified"};
+    S->mesh.finalized = false;
+    mesh_finalize(c, &S->mesh);
+    if (mk) mk->mark(MK_FINAL, c->stream);
+}
+
+// primitives, CFL dt (device scalar), HLL update; drift when the mesh moves
+void sim_hydro(pg_ctx* c, pg_sim* S, double dt_max, bool moving) {
+    hipStream_t s = c->stream;
+    pg_mesh& m = S->mesh;
+    const int dim = S->prm.mesh.dim;
+    const double sc = m.scale();
+    pg::launch_prim_from_q(dim, S->view(), m.o2s.as<int32_t>(), m.vol_u.as<double>(), s);
+    unsigned long long* d = S->dt.get<unsigned long long>(1);
+    const double init = 1e100;
+    HIPCHK(hipMemcpyAsync(d, &init, sizeof(double), hipMemcpyHostToDevice, s));
+    pg::launch_dt_cfl(dim, S->n, m.o2s.as<int32_t>(), m.vol_u.as<double>(), S->prm.gamma, S->prm.cfl,
+                      S->rho.as<double>(), S->vel.as<double>(), S->E.as<double>(), d, s);
+    pg::launch_hll_ale(dim, m.n_seeds, m.n_hydro, m.n_seeds, m.n_seeds * MAXP, m.g.perm.as<uint32_t>(),
+                       m.count.as<uint8_t>(), m.hid.as<int32_t>(), m.nbr.as<int32_t>(), m.nrm.as<double>(),
+                       m.area_u.as<double>(), m.cen.as<double>(), m.g.sorted.as<double>(), sc, S->prm.gamma,
+                       (const double*)d, dt_max, S->prm.flux_mode, moving ? 1 : 0, S->view(), s);
+    if (moving)
+        pg::launch_drift(dim, S->n, S->x.as<double>(), S->vel.as<double>(), (const double*)d, dt_max,
+                         S->prm.mesh.boundary == PG_BOUNDARY_PERIODIC ? 1 : 0, s);
+    HIPCHK(hipGetLastError());
+}
+
+double sim_read_dt(pg_ctx* c, pg_sim* S, double dt_max) {
+    double dt = 0;
+    HIPCHK(hipMemcpyAsync(&dt, S->dt.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (dt_max > 0 && dt > dt_max) dt = dt_max;
+    return dt;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_sim_create(pg_ctx* c, const pg_sim_params* prm, const double* pts, const double* rho,
+                  const double* vel, const double* E, int64_t n, pg_sim** out) {
+    return guarded([&] {
+        use(c);
+        argchk(prm && pts && rho && vel && E && out, "null argument");
+        const pg_mesh_params& mp = prm->mesh;
+        argchk(mp.dim == 2 || mp.dim == 3, "dim must be 2 or 3");
+        argchk(mp.boundary == PG_BOUNDARY_PERIODIC || mp.boundary == PG_BOUNDARY_REFLECTIVE,
+               "a simulation needs a periodic or reflective box");
+        argchk(mp.k >= 1 && mp.k <= 256, "k must be in [1, 256]");
+        argchk(mp.max_p >= 2 * mp.dim + 1 && mp.max_p <= MAXP, "max_p must be in [2*dim+1, 64]");
+        argchk(mp.max_t >= 8 && mp.max_t <= 128, "max_t must be in [8, 128]");
+        argchk(mp.boundary != PG_BOUNDARY_PERIODIC || (mp.buff > 0 && mp.buff < 0.5), "buff must be in (0, 0.5)");
+        argchk(prm->gamma > 1.0 && prm->cfl > 0.0, "gamma > 1 and cfl > 0 required");
+        argchk(prm->flux_mode == PG_FLUX_REFERENCE || prm->flux_mode == PG_FLUX_PHYSICAL, "unknown flux mode");
+        argchk(n >= 2 && n < (int64_t)1 << 29, "n out of range");
+        pg_sim* S = new pg_sim();
+        try {
+            S->prm = *prm;
+            S->mesh.prm = prm->mesh;
+            if (S->mesh.prm.k_max < S->mesh.prm.k) S->mesh.prm.k_max = S->mesh.prm.k;
+            S->n = n;
+            const int dim = mp.dim;
+            const size_t nn = (size_t)n;
+            for (hipEvent_t& e : S->mk.ev) HIPCHK(hipEventCreate(&e));
+            S->mk.on = true;
+            HIPCHK(hipMemcpyAsync(S->x.get<double>(nn * dim), pts, sizeof(double) * nn * dim,
+                                  hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(S->rho.get<double>(nn), rho, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(S->vel.get<double>(nn * dim), vel, sizeof(double) * nn * dim,
+                                  hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(S->E.get<double>(nn), E, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+            S->mass.get<double>(nn);
+            S->mom.get<double>(nn * dim);
+            S->en.get<double>(nn);
+            sim_rebuild(c, S, nullptr);
+            pg::launch_q_from_prim(dim, S->view(), S->mesh.o2s.as<int32_t>(), S->mesh.vol_u.as<double>(), c->stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(c->stream));
+        } catch (...) {
+            delete S;
+            throw;
+        }
+        *out = S;
+    });
+}
+
+int pg_sim_step(pg_ctx* c, pg_sim* S, double dt_max, double* dt_used) {
+    return guarded([&] {
+        use(c);
+        argchk(S != nullptr, "null sim");
+        hipStream_t s = c->stream;
+        Marks& mk = S->mk;
+        mk.mark(MK_START, s);
+        sim_hydro(c, S, dt_max, S->prm.move_mesh != 0);
+        mk.mark(MK_HYDRO, s);
+        if (S->prm.move_mesh) {
+            sim_rebuild(c, S, &mk);
+        } else {
+            // static mesh: rebuild anyway (same generators) so every step has
+            // the same cost model; the result equals the current mesh
+            sim_rebuild(c, S, &mk);
+        }
+        mk.mark(MK_END, s);
+        HIPCHK(hipStreamSynchronize(s));
+        auto el = [&](int a, int b) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, mk.ev[a], mk.ev[b]));
+            return (double)ms;
+        };
+        pg_sim_times& t = S->last;
+        t.hydro_ms = el(MK_START, MK_HYDRO);
+        t.ghosts_ms = el(MK_HYDRO, MK_GHOSTS);
+        t.grid_ms = el(MK_GHOSTS, MK_GRID);
+        t.knn_ms = el(MK_GRID, MK_KNN);
+        t.clip_ms = el(MK_KNN, MK_CLIP);
+        t.restart_ms = el(MK_CLIP, MK_RESTART);
+        t.final_ms = el(MK_RESTART, MK_FINAL);
+        t.total_ms = el(MK_START, MK_END);
+        t.n_restarted = S->mesh.info.n_restarted;
+        t.k_used = S->mesh.info.k_used;
+        double dt = sim_read_dt(c, S, dt_max);
+        if (dt_used) *dt_used = dt;
+    });
+}
+
+int pg_sim_hydro_steps(pg_ctx* c, pg_sim* S, int nsteps, double dt_max, double* t_adv) {
+    return guarded([&] {
+        use(c);
+        argchk(S != nullptr && nsteps >= 0, "bad argument");
+        double t = 0;
+        for (int i = 0; i < nsteps; i++) {
+            sim_hydro(c, S, dt_max, false);
+            t += sim_read_dt(c, S, dt_max);
+        }
+        if (t_adv) *t_adv = t;
+    });
+}
+
+int pg_sim_times_last(const pg_sim* S, pg_sim_times* t) {
+    return guarded([&] {
+        argchk(S && t, "null argument");
+        *t = S->last;
+    });
+}
+
+int pg_sim_mesh(const pg_sim* S, const pg_mesh** m) {
+    return guarded([&] {
+        argchk(S && m, "null argument");
+        *m = &S->mesh;
+    });
+}
+
+int pg_sim_get(pg_ctx* c, const pg_sim* Sc, double* pts, double* rho, double* vel, double* E) {
+    return guarded([&] {
+        use(c);
+        argchk(Sc != nullptr, "null sim");
+        pg_sim* S = const_cast<pg_sim*>(Sc);
+        hipStream_t s = c->stream;
+        const int dim = S->prm.mesh.dim;
+        const size_t n = (size_t)S->n;
+        pg::launch_prim_from_q(dim, S->view(), S->mesh.o2s.as<int32_t>(), S->mesh.vol_u.as<double>(), s);
+        HIPCHK(hipGetLastError());
+        if (pts) HIPCHK(hipMemcpyAsync(pts, S->x.p, sizeof(double) * n * dim, hipMemcpyDeviceToHost, s));
+        if (rho) HIPCHK(hipMemcpyAsync(rho, S->rho.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        if (vel) HIPCHK(hipMemcpyAsync(vel, S->vel.p, sizeof(double) * n * dim, hipMemcpyDeviceToHost, s));
+        if (E) HIPCHK(hipMemcpyAsync(E, S->E.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+int pg_sim_totals(pg_ctx* c, pg_sim* S, double* tot) {
+    return guarded([&] {
+        use(c);
+        argchk(S && tot, "null argument");
+        const int dim = S->prm.mesh.dim;
+        const int64_t nb = pg::totals_blocks(S->n);
+        double* part = S->part.get<double>((size_t)5 * nb);
+        pg::launch_totals(dim, S->view(), part, c->stream);
+        HIPCHK(hipGetLastError());
+        std::vector<double> h((size_t)5 * nb);
+        HIPCHK(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int q = 0; q < 5; q++) tot[q] = 0;
+        for (int64_t b = 0; b < nb; b++)
+            for (int q = 0; q < 5; q++) tot[q] += h[(size_t)5 * b + q];
+    });
+}
+
+int pg_sim_free(pg_sim** S) {
+    return guarded([&] {
+        if (!S || !*S) return;
+        delete *S;
+        *S = nullptr;
     });
 }
 

@@ -6,6 +6,29 @@
 
 namespace pg {
 
+// Conserved, extensive state per hydro cell (original order): mass,
+// momentum, total energy (FP64), and the primitives derived from it.
+struct SimView {
+    int64_t n_hydro;
+    double* mass;
+    double* mom;  // [n][dim]
+    double* en;
+    double* rho;  // primitives
+    double* vel;  // [n][dim]
+    double* E;    // energy per volume
+};
+
+void launch_q_from_prim(int dim, SimView sv, const int32_t* o2s, const double* vol_u, hipStream_t s);
+void launch_prim_from_q(int dim, SimView sv, const int32_t* o2s, const double* vol_u, hipStream_t s);
+void launch_hll_ale(int dim, int64_t n, int64_t n_hydro, int64_t stride, int64_t nplane, const uint32_t* perm,
+                    const uint8_t* count, const int32_t* hid, const int32_t* nbr, const double* nrm,
+                    const double* area_u, const double* cen, const double* sorted, double scale, double gam,
+                    const double* dt_dev, double dt_max, int flux_mode, int move, SimView sv, hipStream_t s);
+void launch_drift(int dim, int64_t n, double* x, const double* vel, const double* dt_dev, double dt_max,
+                  int periodic, hipStream_t s);
+int64_t totals_blocks(int64_t n);
+void launch_totals(int dim, SimView sv, double* part, hipStream_t s);
+
 void launch_finalize(int64_t n, int64_t n_hydro, int64_t stride, const uint32_t* perm,
                      const int64_t* gsrc, const uint8_t* count, const int32_t* nbr,
                      const double* area, const double* vol, double ascale, double vscale,

@@ -38,6 +38,11 @@ __global__ void __launch_bounds__(TB) k_finalize(int64_t n, int64_t n_hydro, int
         if (j >= 0) {
             const uint32_t oj = perm[j];
             h = oj < n_hydro ? (int32_t)oj : (int32_t)gsrc[oj - n_hydro];
+        } else if (gsrc) {
+            // periodic mesh: a hydro cell reaching the box plane means the
+            // ghost layer is thinner than the cell (finite_volume_solver.cpp:130-136
+            // would read out of bounds here)
+            atomicAdd(bad + 1, 1);
         }
         hid[at] = h;
         area_u[at] = ascale * area[at];
@@ -245,6 +250,189 @@ __global__ void __launch_bounds__(TB)
     }
 }
 
+// ---------------------------------------------------------------- simulation
+// Conserved, extensive state per hydro cell (original order): mass, momentum,
+// total energy, all FP64.  Primitives are derived from it and the current
+// cell volume before every update.
+
+
+template <int DIM>
+__global__ void __launch_bounds__(TB) k_q_from_prim(SimView sv, const int32_t* o2s, const double* vol_u) {
+    int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i >= sv.n_hydro) return;
+    const double V = vol_u[o2s[i]];
+    const double r = sv.rho[i];
+    sv.mass[i] = r * V;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) sv.mom[(size_t)DIM * i + d] = r * sv.vel[(size_t)DIM * i + d] * V;
+    sv.en[i] = sv.E[i] * V;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(TB) k_prim_from_q(SimView sv, const int32_t* o2s, const double* vol_u) {
+    int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i >= sv.n_hydro) return;
+    const double V = vol_u[o2s[i]];
+    const double m = sv.mass[i];
+    sv.rho[i] = m / V;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) sv.vel[(size_t)DIM * i + d] = sv.mom[(size_t)DIM * i + d] / m;
+    sv.E[i] = sv.en[i] / V;
+}
+
+// Moving-mesh (ALE) HLL update of the conserved state.  The face moves with
+// w_f = (w_i + w_j)/2 plus the standard correction for the offset of the face
+// centroid from the generators' midpoint; the Riemann problem is solved in
+// the face frame and transformed back (mass flux unchanged, momentum
+// + F_m w_f, energy + F_p.w_f + |w_f|^2 F_m / 2).  With move == 0 this is the
+// static flux of k_hll_static applied to extensive variables.  Walls do not
+// move (w_f = 0).  No reference counterpart: the reference never moves its mesh.
+template <int DIM>
+__global__ void __launch_bounds__(TB)
+    k_hll_ale(int64_t n, int64_t n_hydro, int64_t stride, int64_t nplane, const uint32_t* __restrict__ perm,
+              const uint8_t* __restrict__ count, const int32_t* __restrict__ hid,
+              const int32_t* __restrict__ nbr, const double* __restrict__ nrm,
+              const double* __restrict__ area_u, const double* __restrict__ cen,
+              const double* __restrict__ sorted, double scale, double gam, const double* dt_dev,
+              double dt_max, int flux_mode, int move, SimView sv) {
+    int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t i = perm[s];
+    if (i >= n_hydro) return;
+    double dt = *dt_dev;
+    if (dt_max > 0 && dt > dt_max) dt = dt_max;
+    Prim si;
+    si.rho = sv.rho[i];
+    si.E = sv.E[i];
+    si.v[2] = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) si.v[d] = sv.vel[(size_t)DIM * i + d];
+    double ri[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; d++) ri[d] = (sorted[(size_t)DIM * s + d] - 0.5) * scale + 0.5;
+    double fm = 0.0, fp[3] = {0, 0, 0}, fe = 0.0;
+    const int c = count[s];
+    const double sgn = flux_mode == 1 ? -1.0 : 1.0;
+    for (int f = 0; f < c; f++) {
+        const int64_t at = (int64_t)f * stride + s;
+        const int32_t j = hid[at];
+        const bool wall = j < 0;
+        Prim sj = si;
+        double w[3] = {0.0, 0.0, 0.0};
+        if (!wall) {
+            sj.rho = sv.rho[j];
+            sj.E = sv.E[j];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) sj.v[d] = sv.vel[(size_t)DIM * j + d];
+            if (move) {
+                const int32_t js = nbr[at];
+                double rj[3] = {0, 0, 0}, dr[3], mid[3];
+                double dd = 0.0, proj = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    rj[d] = (sorted[(size_t)DIM * js + d] - 0.5) * scale + 0.5;
+                    dr[d] = rj[d] - ri[d];
+                    mid[d] = 0.5 * (ri[d] + rj[d]);
+                    dd += dr[d] * dr[d];
+                    const double fc = (cen[(int64_t)d * nplane + at] - 0.5) * scale + 0.5;
+                    proj += (si.v[d] - sj.v[d]) * (fc - mid[d]);
+                }
+#pragma unroll
+                for (int d = 0; d < DIM; d++) w[d] = 0.5 * (si.v[d] + sj.v[d]) + proj * dr[d] / dd;
+            }
+        }
+        // total energy per volume is frame dependent: boost it with the velocity
+        Prim L = si, R = sj;
+        if (move && !wall) {
+            double ke_l = 0, ke_r = 0, ke_l2 = 0, ke_r2 = 0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                ke_l += L.v[d] * L.v[d];
+                ke_r += R.v[d] * R.v[d];
+                L.v[d] -= w[d];
+                R.v[d] -= w[d];
+                ke_l2 += L.v[d] * L.v[d];
+                ke_r2 += R.v[d] * R.v[d];
+            }
+            L.E += 0.5 * L.rho * (ke_l2 - ke_l);
+            R.E += 0.5 * R.rho * (ke_r2 - ke_r);
+        }
+        const Geom g = geom_of(sgn * nrm[at], sgn * nrm[nplane + at], sgn * nrm[2 * nplane + at]);
+        const Prim F = hll<DIM>(gam, L, R, g, wall);
+        double Fm = F.rho, Fp[3] = {F.v[0], F.v[1], F.v[2]}, Fe = F.E;
+        if (move && !wall) {
+            double wp = 0.0, ww = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                wp += Fp[d] * w[d];
+                ww += w[d] * w[d];
+            }
+            Fe += wp + 0.5 * ww * Fm;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) Fp[d] += Fm * w[d];
+        }
+        const double a = area_u[at];
+        fm += Fm * a;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) fp[d] += Fp[d] * a;
+        fe += Fe * a;
+    }
+    sv.mass[i] -= dt * fm;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) sv.mom[(size_t)DIM * i + d] -= dt * fp[d];
+    sv.en[i] -= dt * fe;
+}
+
+// Generators move with the fluid: x += v dt, wrapped (periodic) or mirrored
+// at the walls (reflective box).
+template <int DIM>
+__global__ void __launch_bounds__(TB) k_drift(int64_t n, double* x, const double* vel, const double* dt_dev,
+                                              double dt_max, int periodic) {
+    int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    double dt = *dt_dev;
+    if (dt_max > 0 && dt > dt_max) dt = dt_max;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        double p = x[(size_t)DIM * i + d] + vel[(size_t)DIM * i + d] * dt;
+        if (periodic) {
+            p -= floor(p);
+            if (p >= 1.0) p = 0.0;
+        } else {
+            if (p < 0.0) p = -p;
+            if (p >= 1.0) p = 2.0 - p;
+            p = fmin(fmax(p, 0.0), 0.9999999999999999);
+        }
+        x[(size_t)DIM * i + d] = p;
+    }
+}
+
+// FP64 totals of the conserved variables (one atomic-free partial per block).
+template <int DIM>
+__global__ void __launch_bounds__(TB) k_totals(SimView sv, double* part) {
+    __shared__ double red[5][TB];
+    int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    double v[5] = {0, 0, 0, 0, 0};
+    if (i < sv.n_hydro) {
+        v[0] = sv.mass[i];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) v[1 + d] = sv.mom[(size_t)DIM * i + d];
+        v[4] = sv.en[i];
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) red[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int w = TB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int q = 0; q < 5; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 5; q++) part[(size_t)5 * blockIdx.x + q] = red[q][0];
+}
+
 // Copy-out: face counts in original order.
 __global__ void __launch_bounds__(TB) k_counts_orig(int64_t n, const uint32_t* perm,
                                                     const uint8_t* count, int32_t* co) {
@@ -342,6 +530,46 @@ void launch_dt_cfl(int dim, int64_t n_hydro, const int32_t* o2s, const double* v
         k_dt_cfl<2><<<nblk(n_hydro), TB, 0, s>>>(n_hydro, o2s, vol_u, gam, cfl, rho, vel, E, out);
     else
         k_dt_cfl<3><<<nblk(n_hydro), TB, 0, s>>>(n_hydro, o2s, vol_u, gam, cfl, rho, vel, E, out);
+}
+
+void launch_q_from_prim(int dim, SimView sv, const int32_t* o2s, const double* vol_u, hipStream_t s) {
+    if (sv.n_hydro <= 0) return;
+    if (dim == 2) k_q_from_prim<2><<<nblk(sv.n_hydro), TB, 0, s>>>(sv, o2s, vol_u);
+    else k_q_from_prim<3><<<nblk(sv.n_hydro), TB, 0, s>>>(sv, o2s, vol_u);
+}
+
+void launch_prim_from_q(int dim, SimView sv, const int32_t* o2s, const double* vol_u, hipStream_t s) {
+    if (sv.n_hydro <= 0) return;
+    if (dim == 2) k_prim_from_q<2><<<nblk(sv.n_hydro), TB, 0, s>>>(sv, o2s, vol_u);
+    else k_prim_from_q<3><<<nblk(sv.n_hydro), TB, 0, s>>>(sv, o2s, vol_u);
+}
+
+void launch_hll_ale(int dim, int64_t n, int64_t n_hydro, int64_t stride, int64_t nplane, const uint32_t* perm,
+                    const uint8_t* count, const int32_t* hid, const int32_t* nbr, const double* nrm,
+                    const double* area_u, const double* cen, const double* sorted, double scale, double gam,
+                    const double* dt_dev, double dt_max, int flux_mode, int move, SimView sv, hipStream_t s) {
+    if (n <= 0) return;
+    if (dim == 2)
+        k_hll_ale<2><<<nblk(n), TB, 0, s>>>(n, n_hydro, stride, nplane, perm, count, hid, nbr, nrm, area_u, cen,
+                                            sorted, scale, gam, dt_dev, dt_max, flux_mode, move, sv);
+    else
+        k_hll_ale<3><<<nblk(n), TB, 0, s>>>(n, n_hydro, stride, nplane, perm, count, hid, nbr, nrm, area_u, cen,
+                                            sorted, scale, gam, dt_dev, dt_max, flux_mode, move, sv);
+}
+
+void launch_drift(int dim, int64_t n, double* x, const double* vel, const double* dt_dev, double dt_max,
+                  int periodic, hipStream_t s) {
+    if (n <= 0) return;
+    if (dim == 2) k_drift<2><<<nblk(n), TB, 0, s>>>(n, x, vel, dt_dev, dt_max, periodic);
+    else k_drift<3><<<nblk(n), TB, 0, s>>>(n, x, vel, dt_dev, dt_max, periodic);
+}
+
+int64_t totals_blocks(int64_t n) { return (int64_t)nblk(n); }
+
+void launch_totals(int dim, SimView sv, double* part, hipStream_t s) {
+    if (sv.n_hydro <= 0) return;
+    if (dim == 2) k_totals<2><<<nblk(sv.n_hydro), TB, 0, s>>>(sv, part);
+    else k_totals<3><<<nblk(sv.n_hydro), TB, 0, s>>>(sv, part);
 }
 
 void launch_counts_orig(int64_t n, const uint32_t* perm, const uint8_t* count, int32_t* co,

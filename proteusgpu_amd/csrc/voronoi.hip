@@ -8,21 +8,21 @@ namespace {
 // wave.  Large kernel: 16 lanes x 8 slots = 128 slots, for seeds whose cell
 // outgrows 64 vertices before max_t (rare).
 template <int DIM, int G, int S>
-__global__ void __launch_bounds__(64) k_voronoi(const double* __restrict__ pts,
+__global__ void __launch_bounds__(64) k_voronoi(const double* __restrict__ pts, int64_t n_total,
                                                 const uint32_t* __restrict__ seeds, int64_t nq,
                                                 const uint32_t* __restrict__ knn, int k, int max_p,
                                                 int max_t, FaceArrays fa) {
     using C = VoroCfg<DIM, G, S>;
     __shared__ SeedSmem<C::NS, C::MAXP> sm[C::SPW];
-    voronoi_group<DIM, G, S>(pts, seeds, nq, knn, k, max_p, max_t, fa, sm);
+    voronoi_group<DIM, G, S>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm);
 }
 
 template <int DIM, int G, int S>
-void run(const double* pts, const uint32_t* seeds, int64_t nq, const uint32_t* knn, int k,
+void run(const double* pts, int64_t n_total, const uint32_t* seeds, int64_t nq, const uint32_t* knn, int k,
          int max_p, int max_t, FaceArrays fa, hipStream_t s) {
     using C = VoroCfg<DIM, G, S>;
     int64_t nb = (nq + C::SPW - 1) / C::SPW;
-    k_voronoi<DIM, G, S><<<dim3((unsigned)nb), dim3(64), 0, s>>>(pts, seeds, nq, knn, k, max_p,
+    k_voronoi<DIM, G, S><<<dim3((unsigned)nb), dim3(64), 0, s>>>(pts, n_total, seeds, nq, knn, k, max_p,
                                                                   max_t, fa);
 }
 
@@ -50,18 +50,17 @@ __global__ void __launch_bounds__(TB) k_compact(int64_t nq, const uint32_t* seed
 void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq,
                     const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa,
                     hipStream_t s) {
-    (void)n;
     if (nq <= 0) return;
     if (dim == 2) {
         if (big)
-            run<2, 16, 8>(sorted, seeds, nq, knn, k, max_p, max_t, fa, s);
+            run<2, 16, 8>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, s);
         else
-            run<2, 16, 4>(sorted, seeds, nq, knn, k, max_p, max_t, fa, s);
+            run<2, 16, 4>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, s);
     } else {
         if (big)
-            run<3, 16, 8>(sorted, seeds, nq, knn, k, max_p, max_t, fa, s);
+            run<3, 16, 8>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, s);
         else
-            run<3, 16, 4>(sorted, seeds, nq, knn, k, max_p, max_t, fa, s);
+            run<3, 16, 4>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, s);
     }
 }
 

@@ -188,7 +188,7 @@ PG_DEV void vertex_numerics(const Plane& a, const Plane& b, const Plane& c, doub
 }
 
 template <int DIM, int G, int S>
-__device__ void voronoi_group(const double* __restrict__ pts, const uint32_t* __restrict__ seeds,
+__device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, const uint32_t* __restrict__ seeds,
                               int64_t nq, const uint32_t* __restrict__ knn, int k, int max_p,
                               int max_t, FaceArrays fa,
                               SeedSmem<VoroCfg<DIM, G, S>::NS, VoroCfg<DIM, G, S>::MAXP>* smem) {
@@ -568,12 +568,18 @@ __device__ void voronoi_group(const double* __restrict__ pts, const uint32_t* __
         }
     }
 
-    // final certificate with the k-th neighbour (voronoi.cpp:145-147)
+    // final certificate with the k-th neighbour (voronoi.cpp:145-147); a row
+    // holding every other seed certifies the cell by exhaustion
     if (valid) {
         const uint32_t zl = row[k - 1];
         const double dx = pts[(size_t)D * zl] - sx, dy = pts[(size_t)D * zl + 1] - sy;
         const double dz = D == 3 ? pts[(size_t)D * zl + 2] - sz : 0.0;
-        if (!(dx * dx + dy * dy + dz * dz > 4 * R2)) status = SECURITY_RADIUS_NOT_REACHED;
+        if (!(dx * dx + dy * dy + dz * dz > 4 * R2)) {
+            if ((int64_t)k >= n_total - 1 && status == SUCCESS)
+                kc = k;
+            else
+                status = SECURITY_RADIUS_NOT_REACHED;
+        }
         if (status != SUCCESS) kc = 0;
     }
 

@@ -1,0 +1,239 @@
+"""Pins of the oracle beyond the reference's golden vectors.
+
+Closed forms (hexagonal and BCC lattices), independent libraries (SciPy
+cKDTree, Qhull Voronoi), brute force, invariants of a Voronoi tessellation
+and known answers of the HLL flux -- each chosen so that a plausible slip
+(a dropped term, a sign, an index) fails it.
+"""
+import numpy as np
+import pytest
+from scipy.spatial import Voronoi, cKDTree
+
+from oracle import oracle as O
+from proteusgpu_amd.generators import uniform_points
+
+
+# ------------------------------------------------------------------- KNN
+@pytest.mark.parametrize("dim,n,k", [(2, 1500, 16), (3, 1200, 24)])
+def test_knn_brute_force(dim, n, k):
+    pts = uniform_points(n, dim, 31)
+    N, s, perm, kn = O.knn(pts, k)
+    d2 = ((s[:, None, :] - s[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(d2, np.inf)
+    order = np.lexsort((np.broadcast_to(perm, d2.shape), d2), axis=1)[:, :k]
+    assert np.array_equal(kn, order)
+
+
+def test_knn_against_ckdtree():
+    pts = uniform_points(20000, 3, 32)
+    N, s, perm, kn = O.knn(pts, 32)
+    d, idx = cKDTree(s).query(s, k=33)
+    got = np.sqrt(((s[kn] - s[:, None, :]) ** 2).sum(-1))
+    np.testing.assert_allclose(got, d[:, 1:], rtol=1e-14, atol=0)
+
+
+def test_grid_sort_is_stable_counting_sort():
+    pts = uniform_points(5000, 2, 33)
+    N = O.ngrid(2, len(pts))
+    s, perm, cnt, start = O.grid_sort(pts, N)
+    cell = np.clip(np.floor(pts * N).astype(int), 0, N - 1)
+    cid = cell[:, 0] + cell[:, 1] * N
+    assert np.array_equal(perm, np.argsort(cid, kind="stable"))
+    assert np.array_equal(cnt, np.bincount(cid, minlength=N * N))
+
+
+# ------------------------------------------------------------ tessellation
+def hex_lattice(m):
+    a = 1.0 / m
+    pts = []
+    for r in range(int(1 / (a * np.sqrt(3) / 2)) + 1):
+        y = 0.03 + r * a * np.sqrt(3) / 2
+        x0 = 0.013 + (a / 2 if r % 2 else 0.0)
+        for c in range(m + 1):
+            x = x0 + c * a
+            if x < 1 and y < 1:
+                pts.append((x, y))
+    return np.array(pts), a
+
+
+def test_hexagonal_lattice_closed_form():
+    pts, a = hex_lattice(40)
+    m = O.Mesh(pts, "none", k=24)
+    st, fc, vol = m.get("status"), m.get("face_counts"), m.get("volumes")
+    area, fp = m.get("face_area"), m.get("face_ptr")
+    interior = np.all((pts > 3 * a) & (pts < 1 - 3 * a), axis=1)
+    assert interior.sum() > 500
+    assert np.all(st[interior] == 4)
+    assert np.all(fc[interior] == 6)
+    np.testing.assert_allclose(vol[interior], np.sqrt(3) / 2 * a * a, rtol=1e-10)
+    for i in np.where(interior)[0][:200]:
+        np.testing.assert_allclose(area[fp[i]:fp[i] + 6], a / np.sqrt(3), rtol=1e-10)
+        np.testing.assert_allclose(m.get("cell_centroid")[i, :2], pts[i], atol=1e-12)
+
+
+def bcc_lattice(m, off=0.0131, jitter=0.0):
+    a = 1.0 / m
+    g = off + a * np.arange(m)
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    c = np.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+    pts = np.concatenate([c, c + a / 2])
+    if jitter:
+        pts = pts + jitter * a * (uniform_points(len(pts), 3, 77) - 0.5)
+    return pts, a
+
+
+def test_bcc_exact_lattice_is_flagged():
+    """The exact lattice has vertices on later bisector planes: the
+    reference's eps test flags those cells (cell.cpp:270-272)."""
+    pts, a = bcc_lattice(20)
+    m = O.Mesh(pts, "reflective", k=32, restart=True)
+    st = m.get("status")
+    assert np.any(st == 5)
+
+
+def test_bcc_truncated_octahedron():
+    """BCC lattice jittered by 1e-7 a: truncated octahedra of volume a^3/2,
+    6 squares of a^2/8 and 8 hexagons of 3 sqrt(3) a^2/16, to ~1e-6."""
+    pts, a = bcc_lattice(20, jitter=1e-7)
+    m = O.Mesh(pts, "reflective", k=32, restart=True)
+    st, fc, vol = m.get("status"), m.get("face_counts"), m.get("volumes")
+    fp, area = m.get("face_ptr"), m.get("face_area")
+    interior = np.all((pts > 2.5 * a) & (pts < 1 - 2.5 * a), axis=1)
+    assert interior.sum() > 1000
+    assert np.all(st == 4)
+    # jitter turns each degree-4 lattice vertex into two degree-3 vertices
+    # joined by a tiny edge, but the 14 faces remain
+    assert np.all(fc[interior] == 14)
+    np.testing.assert_allclose(vol[interior], a ** 3 / 2, rtol=2e-6)
+    sq, hx = a * a / 8, 3 * np.sqrt(3) * a * a / 16
+    for i in np.where(interior)[0][:100]:
+        fa = np.sort(area[fp[i]:fp[i] + 14])
+        np.testing.assert_allclose(fa[:6], sq, rtol=2e-6)
+        np.testing.assert_allclose(fa[6:], hx, rtol=2e-6)
+    np.testing.assert_allclose(m.get("cell_centroid")[interior], pts[interior], atol=1e-8 * a)
+    assert abs(vol.sum() - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("dim,n,k,boundary,buff", [(2, 4000, 16, "periodic", 0.06),
+                                                   (3, 12000, 32, "periodic", 0.12),
+                                                   (3, 12000, 32, "reflective", 0.0)])
+def test_tessellation_invariants(dim, n, k, boundary, buff):
+    pts = uniform_points(n, dim, 34)
+    m = O.Mesh(pts, boundary, buff=buff if buff else 0.04, k=k, restart=True)
+    assert m.n_failed == 0
+    nh = m.n_hydro
+    vol, fc, fp = m.get("volumes"), m.get("face_counts"), m.get("face_ptr")
+    nb, nrm, area, fcen = m.get("neighbor_cell"), m.get("face_normal"), m.get("face_area"), m.get("face_centroid")
+    seeds, ccen = m.get("seeds"), m.get("cell_centroid")
+    box = (1 + 2 * buff) ** dim if boundary == "periodic" else 1.0
+    # hydro cells tile the domain
+    target = 1.0 if boundary == "periodic" else (1 + 2e-14) ** dim
+    assert abs(vol[:nh].sum() - target) < 1e-12
+    assert box > 0
+    gid = m.get("ghost_ids") if boundary == "periodic" else None
+    face_of = {}
+    for i in range(nh):
+        sl = slice(fp[i], fp[i] + fc[i])
+        A, N = area[sl], nrm[sl]
+        # closed surface: sum_f A_f n_f = 0
+        assert np.linalg.norm((A[:, None] * N).sum(0)) < 1e-12 * A.sum()
+        # divergence theorem with face centroids (n stored inward)
+        d = (fcen[sl] - seeds[i])[:, :dim]
+        hv = -(d * N[:, :dim]).sum(1)
+        assert abs((A * hv).sum() / dim - vol[i]) < 1e-12 * vol[i] + 1e-16
+        # cell centroid: pyramids from the seed, base centroid weighted 3/4 (2D: 2/3)
+        w = A * hv / dim
+        frac = 0.75 if dim == 3 else 2.0 / 3.0
+        cpred = (w[:, None] * (seeds[i] + frac * (fcen[sl] - seeds[i]))).sum(0) / w.sum()
+        np.testing.assert_allclose(cpred[:dim], ccen[i, :dim], atol=1e-12)
+        for f in range(fp[i], fp[i] + fc[i]):
+            j = nb[f]
+            if j >= 0:
+                face_of[(i, j)] = f
+    # every face (i,j) between hydro cells has its twin (j,i): equal area, opposite normal
+    for (i, j), f in face_of.items():
+        if j < nh:
+            g = face_of[(j, i)]
+            # each side clips its own cell: vertex round-off is absolute
+            # (~u |x|^2 / h), so tiny faces agree to an absolute bound
+            assert abs(area[f] - area[g]) < 1e-10 * area[f] + 1e-14
+            assert np.array_equal(nrm[f], -nrm[g])
+
+
+def test_topology_against_qhull():
+    """Neighbour sets of periodic hydro cells equal Qhull's (SciPy Voronoi) on
+    the same ghost-augmented points."""
+    pts = uniform_points(3000, 2, 35)
+    m = O.Mesh(pts, "periodic", buff=0.06, k=16, restart=True)
+    aug, gid = O.periodic_ghosts(pts, 0.06)
+    assert np.array_equal(gid, m.get("ghost_ids"))
+    vor = Voronoi(aug)
+    qn = [set() for _ in range(len(aug))]
+    for a, b in vor.ridge_points:
+        qn[a].add(b)
+        qn[b].add(a)
+    fc, fp, nb = m.get("face_counts"), m.get("face_ptr"), m.get("neighbor_cell")
+    for i in range(len(pts)):
+        assert set(nb[fp[i]:fp[i] + fc[i]].tolist()) == qn[i]
+
+
+def test_restart_equals_direct_larger_k():
+    pts = uniform_points(12000, 3, 36)
+    a = O.Mesh(pts, "periodic", buff=0.12, k=24, restart=True)
+    b = O.Mesh(pts, "periodic", buff=0.12, k=256, restart=False)
+    assert a.n_failed == 0 and b.n_failed == 0
+    for name in ("face_counts", "neighbor_cell", "volumes", "face_area", "face_normal", "kcert"):
+        assert np.array_equal(a.get(name), b.get(name)), name
+
+
+# ---------------------------------------------------------------------- HLL
+def flux_exact(U, n, gamma, dim):
+    rho, v, E = U[0], np.array(U[1:4]), U[4]
+    P = (gamma - 1) * (E - 0.5 * rho * (v[:dim] @ v[:dim]))
+    vn = v @ n
+    return np.concatenate([[rho * vn], rho * vn * v + P * n, [(E + P) * vn]])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_hll_consistency(dim):
+    g = 1.4
+    n = np.array([0.6, 0.8, 0.0]) if dim == 2 else np.array([2.0, -1.0, 2.0]) / 3.0
+    U = np.array([1.2, 0.3, -0.2, 0.1 if dim == 3 else 0.0, 3.0])
+    np.testing.assert_allclose(O.hll(dim, g, U, U, n), flux_exact(U, n, g, dim), rtol=1e-14, atol=1e-15)
+
+
+def test_hll_supersonic_upwind_and_antisymmetry():
+    g, n = 1.4, np.array([1.0, 0.0, 0.0])
+    UL = np.array([1.0, 5.0, 0.0, 0.0, 1.0 / 0.4 + 12.5])
+    UR = np.array([0.5, 5.0, 0.0, 0.0, 0.5 / 0.4 + 6.25])
+    np.testing.assert_allclose(O.hll(3, g, UL, UR, n), flux_exact(UL, n, g, 3), rtol=1e-14)
+    UL = np.array([1.0, 0.1, 0.2, -0.3, 2.5])
+    UR = np.array([0.3, -0.2, 0.0, 0.1, 1.0])
+    nn = np.array([0.36, 0.48, 0.8])
+    np.testing.assert_allclose(O.hll(3, g, UL, UR, nn), -O.hll(3, g, UR, UL, -nn), rtol=1e-12, atol=1e-15)
+
+
+def test_hll_wall_has_zero_mass_flux():
+    U = np.array([0.9, 0.37, -0.21, 0.05, 2.2])
+    F = O.hll(3, 5.0 / 3.0, U, U, np.array([0.0, 0.6, 0.8]), wall=True)
+    assert F[0] == 0.0 and F[4] == 0.0
+
+
+def two_cell_momentum(flux_mode):
+    pts = np.array([[0.25, 0.5], [0.75, 0.5]])
+    m = O.Mesh(pts, "reflective", k=1)
+    gamma = 1.4
+    rho = np.array([1.0, 1.0])
+    vel = np.zeros((2, 2))
+    E = np.array([1.0, 0.1]) / (gamma - 1)
+    m.hydro_step(gamma, 1e-3, rho, vel, E, flux_mode=flux_mode)
+    return vel[0, 0]
+
+
+def test_two_cell_known_answer():
+    """Left cell at P=1, right at P=0.1, both at rest: physically the left
+    cell is pushed toward the right one.  The reference's face orientation
+    (stored inward normal with cell i as left state, riemann.cpp:10-13) flips
+    the central flux and pushes it the other way."""
+    assert two_cell_momentum(1) > 0
+    assert two_cell_momentum(0) < 0
