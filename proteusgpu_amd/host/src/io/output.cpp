@@ -1,0 +1,93 @@
+#include "output.h"
+
+#include <sys/stat.h>
+
+#include <iostream>
+#include <vector>
+
+OutputHandler output;
+
+bool OutputHandler::initialize() {
+    struct stat st;
+    if (stat(outputDirectory.c_str(), &st) == 0) return true;
+    if (mkdir(outputDirectory.c_str(), 0755) != 0) {
+        std::cerr << "OUTPUT: Error! Could not create output directory: " << outputDirectory << std::endl;
+        return false;
+    }
+    return true;
+}
+
+#ifdef USE_HDF5
+namespace {
+template <class T>
+void write(hid_t group, const char* name, hid_t type, const T* data, hsize_t n0, hsize_t n1 = 0) {
+    hsize_t dims[2] = {n0, n1};
+    hid_t sp = H5Screate_simple(n1 ? 2 : 1, dims, nullptr);
+    hid_t ds = H5Dcreate(group, name, type, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    if (n0) H5Dwrite(ds, type, H5S_ALL, H5S_ALL, H5P_DEFAULT, data);
+    H5Dclose(ds);
+    H5Sclose(sp);
+}
+template <class T>
+void attr(hid_t g, const char* name, hid_t type, T v) {
+    hid_t sp = H5Screate(H5S_SCALAR);
+    hid_t a = H5Acreate(g, name, type, sp, H5P_DEFAULT, H5P_DEFAULT);
+    H5Awrite(a, type, &v);
+    H5Aclose(a);
+    H5Sclose(sp);
+}
+}  // namespace
+
+bool OutputHandler::writeMeshFile(const std::string& filename, const VMesh* m, const primvars* p, int n_hydro) {
+    std::string path = outputDirectory + "/" + filename;
+    hid_t f = H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    if (f < 0) {
+        std::cerr << "OUTPUT: Error! Could not create file: " << path << std::endl;
+        return false;
+    }
+    const int n = (int)m->n_hydro;  // vmesh_to_meshdata writes n_hydro cells (voronoi.cpp:341)
+    hid_t h = H5Gcreate(f, "header", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    attr(h, "dimension", H5T_NATIVE_INT, (int)DIMENSION);
+    attr(h, "extent", H5T_NATIVE_DOUBLE, 1.0);
+    attr(h, "n", H5T_NATIVE_INT, n);
+    attr(h, "k", H5T_NATIVE_INT, (int)_K_);
+    attr(h, "nmax", H5T_NATIVE_INT, (int)_MAX_P_);
+    attr(h, "seed", H5T_NATIVE_INT, 0);
+    hbool_t store = 0;
+    attr(h, "store_edge_coords", H5T_NATIVE_HBOOL, store);
+    H5Gclose(h);
+    hid_t cells = H5Gcreate(f, "cells", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    std::vector<int> ids(n), fc(n);
+    std::vector<double> seeds((size_t)n * DIMENSION);
+    for (int i = 0; i < n; i++) {
+        ids[i] = (int)m->cell_ids[i];
+        fc[i] = (int)m->face_counts[i];
+        const double* s = (const double*)&m->seeds[i];
+        for (int d = 0; d < DIMENSION; d++) seeds[(size_t)i * DIMENSION + d] = s[d];
+    }
+    write(cells, "cell_ids", H5T_NATIVE_INT, ids.data(), n);
+    write(cells, "seeds", H5T_NATIVE_DOUBLE, seeds.data(), n, DIMENSION);
+    write(cells, "volumes", H5T_NATIVE_DOUBLE, m->volumes, n);
+    write(cells, "face_counts", H5T_NATIVE_INT, fc.data(), n);
+    hid_t faces = H5Gcreate(cells, "faces", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    const hsize_t nf = m->num_faces;
+    std::vector<double> nrm(nf * DIMENSION);
+    for (hsize_t i = 0; i < nf; i++) {
+        const double* v = (const double*)&m->face_normal[i];
+        for (int d = 0; d < DIMENSION; d++) nrm[i * DIMENSION + d] = v[d];
+    }
+    write(faces, "neighbor_cell", H5T_NATIVE_INT, m->neighbor_cell, nf);
+    write(faces, "normal", H5T_NATIVE_DOUBLE, nrm.data(), nf, DIMENSION);
+    write(faces, "area", H5T_NATIVE_DOUBLE, m->face_area, nf);
+    H5Gclose(faces);
+    H5Gclose(cells);
+    hid_t hy = H5Gcreate(f, "hydro", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    write(hy, "rho", H5T_NATIVE_DOUBLE, p->rho, n_hydro);
+    write(hy, "vel", H5T_NATIVE_DOUBLE, (const double*)p->v, n_hydro, DIMENSION);
+    write(hy, "Energy", H5T_NATIVE_DOUBLE, p->E, n_hydro);
+    H5Gclose(hy);
+    H5Fclose(f);
+    std::cout << "OUTPUT: wrote " << path << std::endl;
+    return true;
+}
+#endif

@@ -1,0 +1,28 @@
+// Snapshot writer with the reference's HDF5 layout (src/io/output.cpp:28-303):
+// /header@{dimension,extent,n,k,nmax,seed,store_edge_coords},
+// /cells/{cell_ids,seeds,volumes,face_counts}, /cells/faces/{neighbor_cell,normal,area},
+// /hydro/{rho,vel,Energy}.  Cell arrays cover n_hydro cells while the face
+// arrays cover every cell, ghosts included (voronoi.cpp:341,387).
+#ifndef PGPU_HOST_OUTPUT_H
+#define PGPU_HOST_OUTPUT_H
+
+#include <string>
+
+#include "../global/allvars.h"
+#include "../voronoi/voronoi.h"
+
+class OutputHandler {
+   public:
+    explicit OutputHandler(const std::string& dir = "./output/") : outputDirectory(dir) {}
+    bool initialize();
+    std::string getOutputDirectory() const { return outputDirectory; }
+#ifdef USE_HDF5
+    bool writeMeshFile(const std::string& filename, const VMesh* mesh, const primvars* primvar, int n_hydro);
+#endif
+   private:
+    std::string outputDirectory;
+};
+
+extern OutputHandler output;
+
+#endif

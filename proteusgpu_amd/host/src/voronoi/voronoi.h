@@ -1,0 +1,39 @@
+// voronoi:: interface of the reference (src/voronoi/voronoi.h:22-81,
+// periodic_mesh.h:22) on libpgpu.  The VMesh is the reference's struct of
+// arrays in original input order; the device mesh behind it stays in HBM and
+// is what hydro::hydro_step runs on.
+#ifndef PGPU_HOST_VORONOI_H
+#define PGPU_HOST_VORONOI_H
+
+#include "../global/allvars.h"
+
+struct VMesh {
+    hsize_t* cell_ids;      // cell ids
+    double3* seeds;         // seed points (z = 0 in 2D)
+    hsize_t n_seeds;        // number of cells
+    double* volumes;        // area in 2D, volume in 3D
+    hsize_t* face_counts;   // faces per cell
+    hsize_t* face_ptr;      // start of each cell's faces
+    int* neighbor_cell;     // original id of the neighbour, -1 = box plane
+    double3* face_normal;   // unit (seed_i - seed_j)
+    double* face_area;      // edge length in 2D, area in 3D
+    hsize_t num_faces;
+    hsize_t n_hydro;        // active cells (n_seeds - n_hydro ghosts)
+    hsize_t* ghost_ids;     // original cell of each ghost
+    // no reference counterpart: per-cell status and minimal certified k
+    int* status;
+    int* kcert;
+    pg_mesh* dev;           // the device mesh
+};
+
+namespace voronoi {
+
+void free_vmesh(VMesh* mesh);                                          // voronoi.h:65
+VMesh* compute_mesh(POINT_TYPE* pts_data, int num_points);             // voronoi.h:68
+VMesh* compute_periodic_mesh(POINT_TYPE* pts_data, hsize_t num_points);  // periodic_mesh.h:22
+// no reference counterpart: the unit box as reflective walls
+VMesh* compute_reflective_mesh(POINT_TYPE* pts_data, hsize_t num_points);
+
+}  // namespace voronoi
+
+#endif
