@@ -3,15 +3,21 @@
 
 Workload (the metric's configuration): 3D, 1,000,000 generators uniform in
 the unit cube, reflective box, Sod along x (rho,P = 1,1 | 0.125,0.1, v=0,
-gamma=1.4), fixed k=32 with certificate restarts, CFL 0.3, generators moving
-with the fluid.  A step is one moving-mesh step: primitives + CFL dt + HLL
-update + drift + one full Voronoi rebuild (grid, KNN, clip, restarts, face
-tables), all resident in HBM.  value = hydro cells / step time.
+gamma=1.4), CFL 0.3, generators moving with the fluid.  A step is one
+moving-mesh step: primitives + CFL dt + HLL update + drift + one full Voronoi
+rebuild (grid, KNN, clip, restarts, face tables), all resident in HBM.
+
+Two runs of the same steps from the same initial state: fixed k=32 with
+certificate restarts (the plain method), and learned k (a classifier trained
+on the GPU, in this run, on the first mesh, choosing each seed's first k;
+the certificate restarts what it underestimates).  value = hydro cells /
+step time of the learned run; the fixed-k run, the speedup and a check that
+both runs end in bit-identical states are reported beside it.
 
 Prints one JSON line.  Extra fields: per-stage times, the "1 rebuild + 10 HLL
-steps" schedule of the same configuration, the roofline of the dominant
-kernel (the Voronoi clip) and a CPU baseline (the FP64 oracle on a bounded
-sample of the same workload, timed on this host).
+steps" schedule, the roofline of the dominant kernel (the first clip pass)
+and a CPU baseline (the FP64 oracle on a bounded sample of the same
+workload, timed on this host).
 """
 import argparse
 import json
@@ -24,11 +30,13 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-# Algorithmic HBM bytes per cell of the clip kernel (3D, k=32, Poisson-Voronoi
+# Algorithmic HBM bytes per cell of the clip kernel (3D, Poisson-Voronoi
 # F~15.54 faces, F/2 owned): read the k-row (4k) and the seed (12), write the
 # cell record (volume 8, centroid 12, range 8) and the owned face records
-# (ids + area 12, normal 12, centroid 12) -> 128 + 12 + 28 + 7.77 * 36.
-CLIP_BYTES_PER_CELL = 128 + 12 + 28 + 7.77 * 36
+# (ids + area 12, normal 12, centroid 12): 4k + 12 + 28 + 7.77 * 36.
+CLIP_FIXED_BYTES_PER_CELL = 12 + 28 + 7.77 * 36
+# Whole step (grid 42.6 + KNN 140 + clip 448 + HLL 500 + drift 60) at k = 32.
+PATH_BYTES_PER_CELL = 1190.0
 HBM_PEAK_GBS = 8000.0
 
 
@@ -41,7 +49,24 @@ def parse():
     ap.add_argument("--k", type=int, default=32)
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe-bytes", type=int, default=1 << 30,
+                    help="size of the streaming-copy probe (measured HBM bandwidth)")
+    ap.add_argument("--train-steps", type=int, default=400, help="Adam steps of the k classifier")
     return ap.parse_args()
+
+
+def clip_traffic(launches):
+    """HBM bytes per launch of the first clip pass at this workload: the
+    FETCH_SIZE + WRITE_SIZE of its launches in one step (rocprofv3 --pmc,
+    scaled by a copy kernel of known size), from the committed summary in
+    profiles/, divided by the launches per step; None without a summary."""
+    path = os.path.join(ROOT, "profiles", "k_voronoi_1m_traffic.json")
+    if not os.path.exists(path) or launches <= 0:
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    per_step = d.get("learned_first_pass_bytes_per_step")
+    return None if per_step is None else round(per_step / launches, 1)
 
 
 def cpu_baseline(n_sample, k):
@@ -63,6 +88,33 @@ def cpu_baseline(n_sample, k):
                       f"1 HLL step ({el:.2f} s)"}
 
 
+def run_steps(ctx, sim, warmup, steps):
+    for _ in range(warmup):
+        sim.step()
+    ctx.sync()
+    stages = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+        stages.append(sim.times())
+    ctx.sync()
+    el = time.perf_counter() - t0
+    mean = {k: float(np.mean([s[k] for s in stages])) for k in stages[0] if k.endswith("_ms")}
+    mean["n_restarted"] = float(np.mean([s["n_restarted"] for s in stages]))
+    mean["row_entries"] = float(np.mean([s["row_entries"] for s in stages]))
+    return el * 1e3 / steps, mean
+
+
+def schedule_ms(ctx, sim):
+    """One full rebuild plus 10 static-mesh HLL steps."""
+    ctx.sync()
+    t1 = time.perf_counter()
+    sim.step()
+    sim.hydro_steps(10)
+    ctx.sync()
+    return (time.perf_counter() - t1) * 1e3
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -75,58 +127,88 @@ def main():
     pts = uniform_points(a.n, 3, 42)
     rho, vel, E = sod_state(pts, gamma)
     ctx = api.Context(0)
-    sim = api.Simulation(ctx, pts, rho, vel, E, boundary="reflective", k=a.k, gamma=gamma, cfl=cfl,
-                         flux="physical", move_mesh=True)
-    for _ in range(a.warmup):
-        sim.step()
+    mk = lambda: api.Simulation(ctx, pts, rho, vel, E, boundary="reflective", k=a.k, gamma=gamma, cfl=cfl,
+                                flux="physical", move_mesh=True)
+
+    # fixed k (the plain method): k=32 and certificate restarts
+    plain = mk()
+    ms_plain, st_plain = run_steps(ctx, plain, a.warmup, a.steps)
+    sched_plain = schedule_ms(ctx, plain)
+
+    # learned k: train on the initial mesh of the same workload (on the GPU),
+    # then run the same steps from the same initial state
+    learned = mk()
+    km = api.KModel(ctx, seed=1)
     ctx.sync()
-    stages = []
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        sim.step()
-        stages.append(sim.times())
+    loss = km.train_mesh(learned, steps=a.train_steps, batch=4096, lr=3e-3)
     ctx.sync()
-    el = time.perf_counter() - t0
-    ms_step = el * 1e3 / a.steps
-    value = a.n / (el / a.steps)
-    mean = {k: float(np.mean([s[k] for s in stages])) for k in stages[0] if k.endswith("_ms")}
-    restarted = float(np.mean([s["n_restarted"] for s in stages]))
+    train_ms = (time.perf_counter() - t0) * 1e3
+    learned.set_kmodel(km)
+    ms_learned, st_learned = run_steps(ctx, learned, a.warmup, a.steps)
+    sched_learned = schedule_ms(ctx, learned)
+    info = learned.mesh_info()
 
-    # "one full rebuild plus 10 HLL steps" of the same configuration
-    ctx.sync()
-    t1 = time.perf_counter()
-    sim.step()
-    sim.hydro_steps(10)
-    ctx.sync()
-    sched_ms = (time.perf_counter() - t1) * 1e3
+    # both runs took the same steps from the same state: identical results
+    pp, rp, vp, ep = plain.get()
+    pl, rl, vl, el_ = learned.get()
+    identical = bool(np.array_equal(pp, pl) and np.array_equal(rp, rl) and np.array_equal(vp, vl)
+                     and np.array_equal(ep, el_))
 
-    clip_ms = mean["clip_ms"]
-    achieved = CLIP_BYTES_PER_CELL * a.n / (clip_ms * 1e-3) / 1e9
+    # roofline of the dominant kernel: the first clip pass (k_voronoi<3,16,4>,
+    # one launch per k bucket) of the learned step
+    n_seeds = info["n_seeds"]
+    rows_first = float(sum(int(c) * b for c, b in zip(info["n_class"], (16, 24, 32, 48, 64, 96))))
+    launches = int(sum(1 for c in info["n_class"] if c > 0))
+    clip_bytes = 4.0 * rows_first + n_seeds * CLIP_FIXED_BYTES_PER_CELL
+    clip_ms = st_learned["clip_ms"]
+    achieved = clip_bytes / (clip_ms * 1e-3) / 1e9
+    copy_gbs = ctx.stream_copy_probe(a.probe_bytes, 10)
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-            "kernel": "k_voronoi<3,16,4> (first clip pass, all seeds)",
-            "bytes_per_cell": round(CLIP_BYTES_PER_CELL, 1), "kernel_ms": round(clip_ms, 3)}
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": clip_traffic(launches),
+            "kernel": "k_voronoi<3,16,4> (first clip pass, one launch per k bucket)",
+            "launches_per_step": launches, "bytes_per_launch": round(clip_bytes / max(launches, 1), 1),
+            "avg_launch_ms": round(clip_ms / max(launches, 1), 4),
+            "bytes_per_cell": round(clip_bytes / n_seeds, 1),
+            "measured_copy_GBs": round(copy_gbs, 1), "frac_of_measured_copy": round(achieved / copy_gbs, 5)}
+    path_gbs = PATH_BYTES_PER_CELL * a.n / (ms_learned * 1e-3) / 1e9
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(a.cpu_sample, a.k)
-    tot = sim.totals()
+    tot = learned.totals()
     out = {
         "metric": "cells/s per Voronoi rebuild+HLL step, 1M 3D seeds; learned-k vs fixed-k speedup",
-        "value": round(value, 1), "unit": "cells/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
+        "value": round(a.n / (ms_learned * 1e-3), 1), "unit": "cells/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms_learned, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"3D {a.n} uniform generators, reflective box, Sod along x (gamma 1.4), "
-                               f"fixed k={a.k} + certificate restarts, CFL 0.3, moving mesh",
-                   "n_cells": a.n, "k": a.k},
+                               f"CFL 0.3, moving mesh, one full Voronoi rebuild per step; k per seed from "
+                               f"the learned classifier (trained in this run on the first mesh) with "
+                               f"certificate restarts; fixed k={a.k} + restarts timed beside",
+                   "n_cells": a.n, "k_fixed": a.k},
         "roofline": roof, "cpu_baseline": cpu,
-        "stages_ms": {k: round(v, 3) for k, v in mean.items()},
-        "restarted_seeds_per_step": restarted,
-        "schedule_1rebuild_10hll_ms": round(sched_ms, 3),
-        "learned_k_speedup": None,
+        "fixed_k": {"ms_per_step": round(ms_plain, 3), "cells_per_s": round(a.n / (ms_plain * 1e-3), 1),
+                    "stages_ms": {k: round(v, 3) for k, v in st_plain.items() if k.endswith("_ms")},
+                    "restarted_seeds_per_step": st_plain["n_restarted"],
+                    "row_entries_per_step": st_plain["row_entries"],
+                    "schedule_1rebuild_10hll_ms": round(sched_plain, 3)},
+        "learned_k": {"ms_per_step": round(ms_learned, 3),
+                      "stages_ms": {k: round(v, 3) for k, v in st_learned.items() if k.endswith("_ms")},
+                      "restarted_seeds_per_step": st_learned["n_restarted"],
+                      "row_entries_per_step": st_learned["row_entries"],
+                      "seeds_per_bucket": [int(c) for c in info["n_class"]],
+                      "schedule_1rebuild_10hll_ms": round(sched_learned, 3),
+                      "train_ms": round(train_ms, 2), "train_steps": a.train_steps, "train_loss": round(loss, 4)},
+        "learned_k_speedup": round(ms_plain / ms_learned, 4),
+        "learned_equals_fixed_state": identical,
+        "path_roofline": {"bytes_per_cell": PATH_BYTES_PER_CELL, "achieved_GBs": round(path_gbs, 2),
+                          "frac": round(path_gbs / HBM_PEAK_GBS, 5)},
         "conserved_totals": [float(x) for x in tot],
     }
     print(json.dumps(out))
-    sim.free()
+    plain.free()
+    learned.free()
+    km.free()
     ctx.close()
 
 

@@ -52,6 +52,7 @@ typedef struct pg_knn pg_knn;
 typedef struct pg_mesh pg_mesh;
 typedef struct pg_state pg_state;
 typedef struct pg_sim pg_sim;
+typedef struct pg_kmodel pg_kmodel;
 
 /* ------------------------------------------------------------------ context */
 const char* pg_version(void);
@@ -61,6 +62,10 @@ int pg_device_count(int* count);
 int pg_ctx_create(int device, pg_ctx** out);
 int pg_ctx_destroy(pg_ctx* ctx);
 int pg_ctx_sync(pg_ctx* ctx);
+/* No reference counterpart: streams `bytes` device-to-device `reps` times and
+ * returns the mean kernel time (HIP events); measured HBM bandwidth =
+ * 2*bytes/ms.  Also the calibration kernel for HBM traffic counters. */
+int pg_stream_copy_probe(pg_ctx* ctx, int64_t bytes, int reps, double* ms_per_copy);
 
 /* ---------------------------------------------------------------------- KNN */
 /* knn::init (knn.h:36, knn.cpp:16-117): copy points, N_grid, stable
@@ -99,6 +104,9 @@ typedef struct {
     int64_t n_restarted;/* seeds recomputed at a larger k                        */
     int N_grid;
     int k_used;         /* largest k any seed needed                             */
+    int64_t row_entries;/* neighbour-row entries computed (sum of rows x k over   *
+                         * the first pass and every restart)                     */
+    int64_t n_class[6]; /* learned build: seeds predicted per k bucket            */
 } pg_mesh_info_t;
 
 /* voronoi::compute_mesh (voronoi.h:68, voronoi.cpp:17-65) for boundary NONE,
@@ -175,6 +183,8 @@ typedef struct {
     double total_ms;
     int64_t n_restarted;
     int k_used;
+    double predict_ms; /* learned k: features + classifier + bucketing         */
+    int64_t row_entries;
 } pg_sim_times;
 
 int pg_sim_create(pg_ctx* ctx, const pg_sim_params* params, const double* pts, const double* rho,
@@ -190,6 +200,55 @@ int pg_sim_get(pg_ctx* ctx, const pg_sim* sim, double* pts, double* rho, double*
 /* FP64 totals: [mass, px, py, pz, energy]. */
 int pg_sim_totals(pg_ctx* ctx, pg_sim* sim, double* tot5);
 int pg_sim_free(pg_sim** sim);
+/* Learned k for every following rebuild of the simulation (NULL: the fixed
+ * k of the parameters).  The model must outlive its use. */
+int pg_sim_set_kmodel(pg_sim* sim, pg_kmodel* model);
+
+/* ---------------------------------------------------------------- learned k */
+/* No counterpart in the reference (its k is the compile-time _K_, knn.h:20).
+ * A classifier 16 features -> 64 -> 64 -> 6 (ReLU; bfloat16 operands, FP32
+ * accumulation on MFMA) picks each seed's k among the buckets
+ * 16, 24, 32, 48, 64, 96; the security-radius certificate judges every
+ * cell and restarts the ones it rejects, so the mesh equals the fixed-k
+ * mesh bit for bit.  Parameters: one float32 vector of PG_KMODEL_NPARAM,
+ * W1[16][64] b1[64] W2[64][64] b2[64] W3[64][6] b3[6] (z = x W + b). */
+#define PG_KMODEL_NFEAT 16
+#define PG_KMODEL_NCLASS 6
+#define PG_KMODEL_NPARAM 5638
+
+/* Glorot-uniform weights from a SplitMix64 stream of `seed`, zero biases. */
+int pg_kmodel_create(pg_ctx* ctx, uint64_t seed, pg_kmodel** out);
+int pg_kmodel_set_params(pg_ctx* ctx, pg_kmodel* model, const float* params);
+int pg_kmodel_get_params(pg_ctx* ctx, const pg_kmodel* model, float* params);
+/* Cost model of the bucket choice: a seed costs a0 + k per pass and a
+ * rejected seed repeats at 2k (then 4k, ...); the bucket whose chain has the
+ * least expected cost under the predicted distribution wins (default
+ * a0 = 32). */
+int pg_kmodel_set_cost(pg_kmodel* model, double a0);
+/* Host features (n x 16) -> bucket index per row (and the 6 logits per row
+ * when logits != NULL). */
+int pg_kmodel_predict(pg_ctx* ctx, const pg_kmodel* model, const float* feat, int64_t n, int32_t* cls,
+                      float* logits);
+/* Mean cross-entropy of the rows and its gradient (PG_KMODEL_NPARAM floats). */
+int pg_kmodel_grad(pg_ctx* ctx, const pg_kmodel* model, const float* feat, const int32_t* cls, int64_t n,
+                   float* grad, double* loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the rows as one batch. */
+int pg_kmodel_train_step(pg_ctx* ctx, pg_kmodel* model, const float* feat, const int32_t* cls, int64_t n,
+                         double lr, double* loss);
+/* `steps` Adam steps on a built mesh: features and labels (the bucket of
+ * each seed's minimal certified k) stay on the device; batches of `batch`
+ * seeds are drawn by a hash of (seed, step, i).  loss: mean loss of the
+ * last step. */
+int pg_kmodel_train_mesh(pg_ctx* ctx, pg_kmodel* model, const pg_mesh* mesh, int steps, int64_t batch,
+                         double lr, uint64_t seed, double* loss);
+int pg_kmodel_free(pg_kmodel** model);
+
+/* Features of every seed of a built mesh, original order (n_seeds x 16). */
+int pg_mesh_features(pg_ctx* ctx, const pg_mesh* mesh, float* out);
+/* pg_mesh_compute with each seed's first k chosen by `model` (NULL: the
+ * fixed k of the parameters). */
+int pg_mesh_compute_k(pg_ctx* ctx, const pg_mesh_params* params, const pg_kmodel* model, const double* pts,
+                      int64_t n, pg_mesh** out);
 
 #ifdef __cplusplus
 }

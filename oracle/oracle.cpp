@@ -919,6 +919,103 @@ Prim hll(int dim, double gam, Prim L, Prim R, const double* nrm, bool wall) {
     return F;
 }
 
+// ------------------------------------------------------------------ learned k
+// Features of one seed for the k classifier (no reference counterpart: the
+// reference has a compile-time _K_).  Sorted-space seed s, grid N, mean
+// occupancy lam = n / N^dim.  The 3^dim cube around the seed's cell is split
+// by the number m of nonzero offsets (own cell, faces, edges, corners); the
+// 8 smallest FP64 squared distances to points in the cube are kept.
+//   f0..f3  mean count per in-grid cell of class m=0..3, / lam (0 if none)
+//   f4      in-grid cells / 3^dim
+//   f5..f7  |2u-1| per axis, u = position inside the own cell (0 in 2D z)
+//   f8,f9   sqrt(d2 of the 4th, 8th nearest in the cube) * N (2 if absent)
+//   f10     sqrt(d2 of the nearest in the cube) * N (2 if absent)
+//   f11     log2(1 + cube total / (in-grid cells * lam))
+//   f12     min(distance to the unit box's faces * N, 3) / 3
+//   f13     (max - min of the in-grid neighbour counts) / lam
+//   f14     |sum of count * offset| / (3^dim * lam)
+//   f15     f9 / f8
+void kfeatures(int dim, const double* sorted, int64_t n, int N, const int32_t* cnt,
+               const int32_t* start, int64_t s, float* f) {
+    const double* p = sorted + (size_t)dim * s;
+    const int c = cell_of(dim, N, p);
+    const int ci = c % N, cj = (c / N) % N, ck = dim == 3 ? c / (N * N) : 0;
+    const double ncell = (double)N * N * (dim == 3 ? N : 1);
+    const double lam = (double)n / ncell;
+    double best[8];
+    int nb = 0;
+    double csum[4] = {0, 0, 0, 0};
+    int cin[4] = {0, 0, 0, 0};
+    int inside = 0;
+    double total = 0, gx = 0, gy = 0, gz = 0;
+    int nmin = INT_MAX, nmax = -1;
+    const int kr = dim == 3 ? 1 : 0;
+    for (int dk = -kr; dk <= kr; dk++)
+        for (int dj = -1; dj <= 1; dj++)
+            for (int di = -1; di <= 1; di++) {
+                const int ii = ci + di, jj = cj + dj, kk = ck + dk;
+                if (ii < 0 || jj < 0 || kk < 0 || ii >= N || jj >= N || kk >= N) continue;
+                const int cc = ii + jj * N + kk * N * N;
+                const int m = (di != 0) + (dj != 0) + (dk != 0);
+                const int ct = cnt[cc];
+                inside++;
+                csum[m] += ct;
+                cin[m]++;
+                total += ct;
+                gx += (double)ct * di;
+                gy += (double)ct * dj;
+                gz += (double)ct * dk;
+                if (m > 0) {
+                    nmin = std::min(nmin, ct);
+                    nmax = std::max(nmax, ct);
+                }
+                for (int32_t q = start[cc]; q < start[cc] + ct; q++) {
+                    if (q == s) continue;
+                    const double d = dist2(dim, p, sorted + (size_t)dim * q);
+                    if (nb < 8) {
+                        int j = nb++;
+                        while (j > 0 && best[j - 1] > d) {
+                            best[j] = best[j - 1];
+                            j--;
+                        }
+                        best[j] = d;
+                    } else if (d < best[7]) {
+                        int j = 7;
+                        while (j > 0 && best[j - 1] > d) {
+                            best[j] = best[j - 1];
+                            j--;
+                        }
+                        best[j] = d;
+                    }
+                }
+            }
+    const int cube = dim == 3 ? 27 : 9;
+    for (int m = 0; m < 4; m++) f[m] = cin[m] ? (float)(csum[m] / cin[m] / lam) : 0.0f;
+    f[4] = (float)((double)inside / cube);
+    const int cidx[3] = {ci, cj, ck};
+    for (int d = 0; d < 3; d++) {
+        if (d < dim) {
+            const double u = p[d] * (double)N - (double)cidx[d];
+            f[5 + d] = (float)std::fabs(2.0 * u - 1.0);
+        } else {
+            f[5 + d] = 0.0f;
+        }
+    }
+    const double r4 = nb >= 4 ? std::sqrt(best[3]) * N : 2.0;
+    const double r8 = nb >= 8 ? std::sqrt(best[7]) * N : 2.0;
+    const double r1 = nb >= 1 ? std::sqrt(best[0]) * N : 2.0;
+    f[8] = (float)r4;
+    f[9] = (float)r8;
+    f[10] = (float)r1;
+    f[11] = (float)std::log2(1.0 + total / (inside * lam));
+    double wall = 1e300;
+    for (int d = 0; d < dim; d++) wall = std::min(wall, std::min(p[d], 1.0 - p[d]));
+    f[12] = (float)(std::min(std::max(wall, 0.0) * N, 3.0) / 3.0);
+    f[13] = nmax >= 0 ? (float)((double)(nmax - nmin) / lam) : 0.0f;
+    f[14] = (float)(std::sqrt(gx * gx + gy * gy + gz * gz) / (cube * lam));
+    f[15] = (float)(r8 / r4);
+}
+
 }  // namespace
 
 // ======================================================================= C-ABI
@@ -934,6 +1031,13 @@ int64_t or_periodic_ghosts(int dim, const double* pts, int64_t n, double buff, d
 int or_grid_sort(int dim, const double* pts, int64_t n, int N_grid, double* sorted, uint32_t* perm,
                  int32_t* counters, int32_t* ptrs) {
     grid_sort(dim, pts, n, N_grid, sorted, perm, counters, ptrs);
+    return 0;
+}
+
+int or_kfeatures(int dim, const double* sorted, int64_t n, int N_grid, const int32_t* counters,
+                 const int32_t* ptrs, float* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < n; s++) kfeatures(dim, sorted, n, N_grid, counters, ptrs, s, out + 16 * s);
     return 0;
 }
 

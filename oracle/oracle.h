@@ -36,6 +36,11 @@ int or_grid_sort(int dim, const double* pts, int64_t n, int N_grid, double* sort
 int or_knn(int dim, const double* sorted, int64_t n, int N_grid, const int32_t* counters,
            const int32_t* ptrs, const uint32_t* perm, int k, uint32_t* out);
 
+/* Features of every seed for the learned-k classifier, sorted space
+ * (no reference counterpart).  out: n*16 floats; definition in oracle.cpp. */
+int or_kfeatures(int dim, const double* sorted, int64_t n, int N_grid, const int32_t* counters,
+                 const int32_t* ptrs, float* out);
+
 /* Mesh build.  boundary: 0 = periodic ghosts (periodic_mesh.cpp:44), 1 = the
  * unit box as reflective walls (no ghosts, no scaling), 2 = plain
  * compute_mesh on the given points (voronoi.cpp:17).  k: neighbours per seed.

@@ -30,6 +30,8 @@ def lib():
         L.or_periodic_ghosts.restype = C.c_int64
         L.or_grid_sort.argtypes = [C.c_int, dp, C.c_int64, C.c_int, dp, u32p, i32p, i32p]
         L.or_knn.argtypes = [C.c_int, dp, C.c_int64, C.c_int, i32p, i32p, u32p, C.c_int, u32p]
+        L.or_kfeatures.argtypes = [C.c_int, dp, C.c_int64, C.c_int, i32p, i32p,
+                                   np.ctypeslib.ndpointer(dtype=np.float32, flags="C")]
         L.or_mesh_build.argtypes = [C.c_int, dp, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_int,
                                     C.c_int, C.c_int]
         L.or_mesh_build.restype = C.c_void_p
@@ -80,6 +82,19 @@ def knn(pts, k):
     out = np.empty(n * k, dtype=np.uint32)
     lib().or_knn(dim, s, n, N, cnt, start, perm, k, out)
     return N, s, perm, out.reshape(n, k)
+
+
+def kfeatures(sorted_pts, N):
+    """Learned-k features (n, 16) float32 of every seed of a sorted point set
+    (rows in grid order, as returned by grid_sort / Mesh.get('sorted_points'))."""
+    s = np.ascontiguousarray(sorted_pts, dtype=np.float64)
+    n, dim = s.shape
+    # the grid tables of an already sorted set: sorting again keeps the order
+    s2, perm, cnt, start = grid_sort(s, N)
+    assert np.array_equal(s2, s), "points are not in grid order"
+    out = np.empty((n, 16), dtype=np.float32)
+    lib().or_kfeatures(dim, s, n, N, cnt, start, out)
+    return out
 
 
 def hll(dim, gamma, UL, UR, n, wall=False):

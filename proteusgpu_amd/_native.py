@@ -34,7 +34,7 @@ class MeshParams(C.Structure):
 class MeshInfo(C.Structure):
     _fields_ = [("n_seeds", C.c_int64), ("n_hydro", C.c_int64), ("num_faces", C.c_int64),
                 ("n_failed", C.c_int64), ("n_restarted", C.c_int64), ("N_grid", C.c_int),
-                ("k_used", C.c_int)]
+                ("k_used", C.c_int), ("row_entries", C.c_int64), ("n_class", C.c_int64 * 6)]
 
 
 class VMeshView(C.Structure):
@@ -52,10 +52,14 @@ class SimTimes(C.Structure):
     _fields_ = [("hydro_ms", C.c_double), ("ghosts_ms", C.c_double), ("grid_ms", C.c_double),
                 ("knn_ms", C.c_double), ("clip_ms", C.c_double), ("restart_ms", C.c_double),
                 ("final_ms", C.c_double), ("total_ms", C.c_double), ("n_restarted", C.c_int64),
-                ("k_used", C.c_int)]
+                ("k_used", C.c_int), ("predict_ms", C.c_double), ("row_entries", C.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+KMODEL_NFEAT, KMODEL_NCLASS, KMODEL_NPARAM = 16, 6, 5638
+KMODEL_BUCKETS = (16, 24, 32, 48, 64, 96)
 
 
 _lib = None
@@ -75,6 +79,7 @@ def lib():
         "pg_version": ([], C.c_char_p), "pg_last_error": ([], C.c_char_p),
         "pg_device_count": ([C.POINTER(C.c_int)], i32),
         "pg_ctx_create": ([i32, pp], i32), "pg_ctx_destroy": ([vp], i32), "pg_ctx_sync": ([vp], i32),
+        "pg_stream_copy_probe": ([vp, i64, i32, C.POINTER(C.c_double)], i32),
         "pg_knn_init": ([vp, i32, vp, i64, pp], i32), "pg_knn_solve": ([vp, vp, i32], i32),
         "pg_knn_info": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int)], i32),
         "pg_knn_get_points": ([vp, vp, vp], i32), "pg_knn_get_knearest": ([vp, vp, vp], i32),
@@ -95,6 +100,17 @@ def lib():
         "pg_sim_get": ([vp, vp, vp, vp, vp, vp], i32),
         "pg_sim_totals": ([vp, vp, vp], i32),
         "pg_sim_free": ([pp], i32),
+        "pg_sim_set_kmodel": ([vp, vp], i32),
+        "pg_kmodel_create": ([vp, C.c_uint64, pp], i32),
+        "pg_kmodel_set_params": ([vp, vp, vp], i32), "pg_kmodel_get_params": ([vp, vp, vp], i32),
+        "pg_kmodel_set_cost": ([vp, dbl], i32),
+        "pg_kmodel_predict": ([vp, vp, vp, i64, vp, vp], i32),
+        "pg_kmodel_grad": ([vp, vp, vp, vp, i64, vp, C.POINTER(C.c_double)], i32),
+        "pg_kmodel_train_step": ([vp, vp, vp, vp, i64, dbl, C.POINTER(C.c_double)], i32),
+        "pg_kmodel_train_mesh": ([vp, vp, vp, i32, i64, dbl, C.c_uint64, C.POINTER(C.c_double)], i32),
+        "pg_kmodel_free": ([pp], i32),
+        "pg_mesh_features": ([vp, vp, vp], i32),
+        "pg_mesh_compute_k": ([vp, C.POINTER(MeshParams), vp, vp, i64, pp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -112,9 +128,12 @@ def check(rc):
 def exported_symbols():
     """Names of the C ABI entry points this binding knows (for the export test)."""
     return ["pg_version", "pg_last_error", "pg_device_count", "pg_ctx_create", "pg_ctx_destroy",
-            "pg_ctx_sync", "pg_knn_init", "pg_knn_solve", "pg_knn_info", "pg_knn_get_points",
+            "pg_ctx_sync", "pg_stream_copy_probe", "pg_knn_init", "pg_knn_solve", "pg_knn_info", "pg_knn_get_points",
             "pg_knn_get_knearest", "pg_knn_get_permutation", "pg_knn_free", "pg_mesh_params_default",
             "pg_mesh_compute", "pg_mesh_info", "pg_mesh_copy_out", "pg_mesh_free", "pg_state_create",
             "pg_dt_cfl", "pg_hydro_step", "pg_state_copy_out", "pg_state_free", "pg_sim_create",
             "pg_sim_step", "pg_sim_hydro_steps", "pg_sim_times_last", "pg_sim_mesh", "pg_sim_get",
-            "pg_sim_totals", "pg_sim_free"]
+            "pg_sim_totals", "pg_sim_free", "pg_sim_set_kmodel", "pg_kmodel_create", "pg_kmodel_set_params",
+            "pg_kmodel_get_params", "pg_kmodel_set_cost", "pg_kmodel_predict", "pg_kmodel_grad",
+            "pg_kmodel_train_step", "pg_kmodel_train_mesh", "pg_kmodel_free", "pg_mesh_features",
+            "pg_mesh_compute_k"]

@@ -10,7 +10,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._native import BOUNDARY, FLUX, MeshInfo, MeshParams, VMeshView, check, lib
+from ._native import (BOUNDARY, FLUX, KMODEL_NCLASS, KMODEL_NFEAT, KMODEL_NPARAM, MeshInfo, MeshParams,
+                      VMeshView, check, lib)
 
 
 def _ptr(a):
@@ -33,6 +34,12 @@ class Context:
 
     def sync(self):
         check(lib().pg_ctx_sync(self.h))
+
+    def stream_copy_probe(self, nbytes: int = 1 << 30, reps: int = 10) -> float:
+        """Measured device copy bandwidth in GB/s (reads + writes)."""
+        ms = C.c_double()
+        check(lib().pg_stream_copy_probe(self.h, nbytes, reps, C.byref(ms)))
+        return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
     def close(self):
         if getattr(self, "h", None):
@@ -100,23 +107,103 @@ def mesh_params(dim=3, boundary="periodic", buff=0.04, k=32, max_p=64, max_t=96,
     return p
 
 
+class KModel:
+    """Learned neighbour count (no reference counterpart): a 16-64-64-6
+    classifier of each seed's k bucket, trained and evaluated on the GPU."""
+
+    def __init__(self, ctx: Context, seed: int = 1):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib().pg_kmodel_create(ctx.h, seed, C.byref(h)))
+        self.h = h
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.empty(KMODEL_NPARAM, dtype=np.float32)
+        check(lib().pg_kmodel_get_params(self.ctx.h, self.h, _ptr(out)))
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.shape == (KMODEL_NPARAM,)
+        check(lib().pg_kmodel_set_params(self.ctx.h, self.h, _ptr(p)))
+
+    def set_cost(self, a0: float):
+        check(lib().pg_kmodel_set_cost(self.h, a0))
+
+    def predict(self, feat, logits=False):
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        n = feat.shape[0]
+        cls = np.empty(n, dtype=np.int32)
+        lg = np.empty((n, KMODEL_NCLASS), dtype=np.float32) if logits else None
+        check(lib().pg_kmodel_predict(self.ctx.h, self.h, _ptr(feat), n, _ptr(cls), _ptr(lg)))
+        return (cls, lg) if logits else cls
+
+    def grad(self, feat, cls):
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        cls = np.ascontiguousarray(cls, dtype=np.int32)
+        g = np.empty(KMODEL_NPARAM, dtype=np.float32)
+        loss = C.c_double()
+        check(lib().pg_kmodel_grad(self.ctx.h, self.h, _ptr(feat), _ptr(cls), feat.shape[0], _ptr(g),
+                                   C.byref(loss)))
+        return loss.value, g
+
+    def train_step(self, feat, cls, lr):
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        cls = np.ascontiguousarray(cls, dtype=np.int32)
+        loss = C.c_double()
+        check(lib().pg_kmodel_train_step(self.ctx.h, self.h, _ptr(feat), _ptr(cls), feat.shape[0], lr,
+                                         C.byref(loss)))
+        return loss.value
+
+    def train_mesh(self, mesh, steps=400, batch=4096, lr=3e-3, seed=7):
+        """Adam steps on the seeds of a built mesh (Mesh or Simulation);
+        returns the last step's mean loss."""
+        h = mesh.mesh_handle() if hasattr(mesh, "mesh_handle") else mesh.h
+        loss = C.c_double()
+        check(lib().pg_kmodel_train_mesh(self.ctx.h, self.h, h, steps, batch, lr, seed, C.byref(loss)))
+        return loss.value
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().pg_kmodel_free(C.byref(self.h))
+            self.h = None
+
+    def __del__(self):
+        self.free()
+
+
 class Mesh:
-    """voronoi::compute_mesh / compute_periodic_mesh and the VMesh copy-out."""
+    """voronoi::compute_mesh / compute_periodic_mesh and the VMesh copy-out.
+    kmodel: a trained KModel picks each seed's first k (learned k)."""
 
     def __init__(self, ctx: Context, pts: np.ndarray, boundary="periodic", buff=0.04, k=32, max_p=64,
-                 max_t=96, restart=True, k_max=256):
+                 max_t=96, restart=True, k_max=256, kmodel: KModel | None = None):
         pts = np.ascontiguousarray(pts, dtype=np.float64)
         self.ctx, self.dim = ctx, pts.shape[1]
         prm = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max)
         h = C.c_void_p()
-        check(lib().pg_mesh_compute(ctx.h, C.byref(prm), _ptr(pts), pts.shape[0], C.byref(h)))
+        km = kmodel.h if kmodel is not None else None
+        check(lib().pg_mesh_compute_k(ctx.h, C.byref(prm), km, _ptr(pts), pts.shape[0], C.byref(h)))
         self.h = h
+        self._load_info()
+
+    def _load_info(self):
         info = MeshInfo()
-        check(lib().pg_mesh_info(h, C.byref(info)))
+        check(lib().pg_mesh_info(self.h, C.byref(info)))
         self.info = info
         self.n_seeds, self.n_hydro, self.num_faces = info.n_seeds, info.n_hydro, info.num_faces
         self.n_failed, self.n_restarted, self.N_grid, self.k_used = (
             info.n_failed, info.n_restarted, info.N_grid, info.k_used)
+        self.row_entries = info.row_entries
+        self.n_class = np.array(info.n_class[:], dtype=np.int64)
+
+    def features(self):
+        """Learned-k features of every seed, original order (n_seeds, 16)."""
+        out = np.empty((self.n_seeds, KMODEL_NFEAT), dtype=np.float32)
+        check(lib().pg_mesh_features(self.ctx.h, self.h, _ptr(out)))
+        return out
 
     def copy_out(self):
         """Reference VMesh arrays in original order (dict of numpy arrays)."""
@@ -232,6 +319,24 @@ class Simulation:
         t = np.zeros(5)
         check(lib().pg_sim_totals(self.ctx.h, self.h, _ptr(t)))
         return t
+
+    def set_kmodel(self, km: KModel | None):
+        """Learned k for the following rebuilds (None: fixed k)."""
+        check(lib().pg_sim_set_kmodel(self.h, km.h if km is not None else None))
+        self._km = km
+
+    def mesh_info(self) -> dict:
+        """Counters of the current mesh (pg_mesh_info)."""
+        info = MeshInfo()
+        check(lib().pg_mesh_info(self.mesh_handle(), C.byref(info)))
+        d = {name: getattr(info, name) for name, _ in MeshInfo._fields_}
+        d["n_class"] = [int(x) for x in info.n_class]
+        return d
+
+    def mesh_handle(self):
+        hm = C.c_void_p()
+        check(lib().pg_sim_mesh(self.h, C.byref(hm)))
+        return hm
 
     def mesh_copy_out(self):
         """Reference-layout VMesh of the current mesh."""

@@ -138,7 +138,8 @@ struct pg_mesh {
     int64_t n_in = 0, n_seeds = 0, n_hydro = 0;
     Grid g;
     DBuf in_pts, gcnt, goff, gsrc, gtot;
-    DBuf knn, rows, list, flags, offs, cnt1;
+    DBuf knn, rows, list, list2, list3, flags, offs, cnt1;
+    DBuf feat, kcls, klist;  // learned k: features, bucket per seed, seeds by bucket
     // faces (padded, face-slot major) and cells, sorted order
     DBuf nbr, area, nrm, cen, count, vol, ccen, status, kcert;
     // hydro tables
@@ -163,6 +164,13 @@ struct pg_mesh {
     double scale() const { return prm.boundary == PG_BOUNDARY_PERIODIC ? 1. + (2 * prm.buff) : 1.0; }
 };
 
+struct pg_kmodel {
+    int device = 0;
+    float a0 = 32.f;
+    int64_t step = 0;  // Adam steps taken
+    DBuf P, m, v, part, ploss, loss, grad, X, y, cls, logits;
+};
+
 struct pg_state {
     int dim = 3;
     int64_t n = 0;
@@ -179,14 +187,14 @@ int32_t read_i32(const int32_t* d, hipStream_t s) {
     return h;
 }
 
-// Collect the sorted ids whose status equals code; returns the count.
-int64_t collect(pg_mesh* m, int code, uint32_t* list, hipStream_t s) {
-    int64_t n = m->n_seeds;
-    int32_t* fl = m->flags.get<int32_t>(n);
-    int32_t* of = m->offs.get<int32_t>(n);
-    int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(std::max(n, m->g.ncell)));
+// Seeds of `seeds` (nullptr: all) whose status equals code, into out.
+int64_t collect_list(pg_mesh* m, const uint32_t* seeds, int64_t nq, int code, uint32_t* out,
+                     hipStream_t s) {
+    int32_t* fl = m->flags.get<int32_t>(m->n_seeds);
+    int32_t* of = m->offs.get<int32_t>(m->n_seeds);
+    int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(std::max(m->n_seeds, m->g.ncell)));
     int32_t* c = m->cnt1.get<int32_t>(1);
-    pg::launch_collect_status(n, nullptr, m->status.as<int32_t>(), code, list, fl, of, tp, c, s);
+    pg::launch_collect_status(nq, seeds, m->status.as<int32_t>(), code, out, fl, of, tp, c, s);
     HIPCHK(hipGetLastError());
     return read_i32(c, s);
 }
@@ -196,8 +204,58 @@ void run_list(pg_mesh* m, const uint32_t* list, int64_t nq, int k, int big, hipS
     pg::launch_knn(m->prm.dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), m->n_seeds, m->g.N,
                    m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), list, nq, k, rows, s);
     pg::launch_voronoi(m->prm.dim, m->g.sorted.as<double>(), m->n_seeds, list, nq, rows, k,
-                       m->prm.max_p, m->prm.max_t, big, m->faces(), s);
+                       m->prm.max_p, m->prm.max_t, big, m->faces(), s, true);
     HIPCHK(hipGetLastError());
+    m->info.row_entries += nq * k;
+}
+
+// Next row length of a restart: the first bucket at least twice k (a
+// restarted seed repeats its clips from the start, so k grows
+// geometrically: few rounds, and a longer KNN row costs far less than a
+// repeated failed clip), else the cap.  -1 when k is at the cap.  The
+// result does not depend on the schedule: a longer row extends a shorter
+// one and certification happens at the same neighbour.
+int next_k(int k, int cap) {
+    for (int v : KSCHED) {
+        if (v <= k || v > cap) continue;
+        if (v >= 2 * k || v == cap || v == KSCHED[sizeof(KSCHED) / sizeof(int) - 1]) return v;
+    }
+    return k < cap ? cap : -1;
+}
+
+// Seeds cur[0..nq) (nullptr: all) were just clipped with rows of length k.
+// Seeds that outgrew the fast kernel's vertex slots are recomputed by the
+// 128-slot kernel at the same k; uncertified seeds at the next k, until all
+// pass or k reaches its cap.  Returns the largest k used.
+int restart_chain(pg_mesh* m, const uint32_t* cur, int64_t nq, int k, hipStream_t s) {
+    const pg_mesh_params& p = m->prm;
+    const size_t ns = (size_t)m->n_seeds;
+    uint32_t* buf[2] = {m->list.get<uint32_t>(ns), m->list2.get<uint32_t>(ns)};
+    uint32_t* ov = m->list3.get<uint32_t>(ns);
+    const int cap = (int)std::min<int64_t>(std::max(p.k_max, k), m->n_seeds - 1);
+    const uint32_t* L = cur;
+    int64_t nL = nq;
+    int kmax = k;
+    while (nL > 0) {
+        const int64_t no = collect_list(m, L, nL, pg::SLOT_OVERFLOW, ov, s);
+        if (no > 0) {
+            run_list(m, ov, no, k, 1, s);
+            m->info.n_restarted += no;
+        }
+        if (!p.restart) break;
+        const int kn = next_k(k, cap);
+        if (kn < 0) break;
+        uint32_t* dst = (L == buf[0]) ? buf[1] : buf[0];
+        const int64_t nr = collect_list(m, L, nL, PG_SECURITY_RADIUS_NOT_REACHED, dst, s);
+        if (nr == 0) break;
+        k = kn;
+        run_list(m, dst, nr, k, 0, s);
+        m->info.n_restarted += nr;
+        kmax = std::max(kmax, k);
+        L = dst;
+        nL = nr;
+    }
+    return kmax;
 }
 
 void count_faces(pg_mesh* m, hipStream_t s);
@@ -210,10 +268,10 @@ struct Marks {
         if (on) HIPCHK(hipEventRecord(ev[i], s));
     }
 };
-enum { MK_START, MK_GHOSTS, MK_GRID, MK_KNN, MK_CLIP, MK_RESTART, MK_FINAL, MK_HYDRO, MK_END, MK_N };
+enum { MK_START, MK_GHOSTS, MK_GRID, MK_PREDICT, MK_KNN, MK_CLIP, MK_RESTART, MK_FINAL, MK_HYDRO, MK_END, MK_N };
 
 void mesh_build(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool on_device = false,
-                Marks* mk = nullptr) {
+                Marks* mk = nullptr, const pg_kmodel* km = nullptr) {
     hipStream_t s = c->stream;
     const pg_mesh_params& p = m->prm;
     const int dim = p.dim;
@@ -268,51 +326,87 @@ void mesh_build(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool on_dev
     m->kcert.get<int32_t>(ns);
     m->finalized = false;
 
-    // round 0: every seed at k
-    int k = p.k;
-    uint32_t* rows = m->knn.get<uint32_t>(ns * k);
-    pg::launch_knn(dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), nseeds, m->g.N,
-                   m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), nullptr, nseeds, k, rows, s);
-    if (mk) mk->mark(MK_KNN, s);
-    pg::launch_voronoi(dim, m->g.sorted.as<double>(), nseeds, nullptr, nseeds, rows, k, p.max_p, p.max_t,
-                       0, m->faces(), s);
-    HIPCHK(hipGetLastError());
-    if (mk) mk->mark(MK_CLIP, s);
-
-    // restarts: uncertified seeds at the next k; seeds that outgrew the fast
-    // kernel's 64 vertex slots in the 128-slot kernel
-    std::vector<int> sched{k};
-    for (int v : KSCHED)
-        if (v > k && v <= p.k_max && v <= nseeds - 1) sched.push_back(v);
-    size_t ki = 0;
-    int64_t restarted = 0;
-    uint32_t* lst = m->list.get<uint32_t>(ns);
-    for (int round = 0; round < 64; round++) {
-        bool work = false;
-        int64_t nso = collect(m, pg::SLOT_OVERFLOW, lst, s);
-        if (nso > 0) {
-            run_list(m, lst, nso, sched[ki], 1, s);
-            restarted += nso;
-            work = true;
-        }
-        if (p.restart && ki + 1 < sched.size()) {
-            int64_t nsr = collect(m, PG_SECURITY_RADIUS_NOT_REACHED, lst, s);
-            if (nsr > 0) {
-                ki++;
-                run_list(m, lst, nsr, sched[ki], 0, s);
-                restarted += nsr;
-                work = true;
+    m->info = pg_mesh_info_t{};
+    int k_used = p.k;
+    if (!km) {
+        // first pass: every seed at the configured k
+        const int k = p.k;
+        if (mk) mk->mark(MK_PREDICT, s);
+        uint32_t* rows = m->knn.get<uint32_t>(ns * k);
+        pg::launch_knn(dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), nseeds, m->g.N,
+                       m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), nullptr, nseeds, k, rows, s);
+        if (mk) mk->mark(MK_KNN, s);
+        pg::launch_voronoi(dim, m->g.sorted.as<double>(), nseeds, nullptr, nseeds, rows, k, p.max_p, p.max_t,
+                           0, m->faces(), s);
+        HIPCHK(hipGetLastError());
+        m->info.row_entries += nseeds * k;
+        if (mk) mk->mark(MK_CLIP, s);
+        k_used = restart_chain(m, nullptr, nseeds, k, s);
+    } else {
+        // learned first pass: features -> bucket per seed -> seeds grouped by
+        // bucket (ascending id inside a bucket) -> KNN + clip per bucket
+        float* feat = m->feat.get<float>(ns * pg::LK_NFEAT);
+        int32_t* cls = m->kcls.get<int32_t>(ns);
+        uint32_t* kl = m->klist.get<uint32_t>(ns);
+        pg::launch_kfeatures(dim, m->g.sorted.as<double>(), nseeds, m->g.N, m->g.counts.as<int32_t>(),
+                             m->g.starts.as<int32_t>(), nullptr, feat, s);
+        pg::launch_kpredict(feat, nseeds, km->P.as<float>(), km->a0, nullptr, cls, nullptr, s);
+        HIPCHK(hipGetLastError());
+        int64_t off[pg::LK_NCLASS + 1] = {0};
+        int bk[pg::LK_NCLASS];
+        {
+            int32_t* fl = m->flags.get<int32_t>(ns);
+            int32_t* of = m->offs.get<int32_t>(ns);
+            int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(std::max(nseeds, m->g.ncell)));
+            int32_t* cnt = m->cnt1.get<int32_t>(pg::LK_NCLASS);
+            for (int c = 0; c < pg::LK_NCLASS; c++)
+                pg::launch_collect_status(nseeds, nullptr, cls, c, kl, fl, of, tp, cnt + c, s);
+            // each collect overwrote kl from its start: run them again into
+            // their final segments once the counts are known
+            int32_t hc[pg::LK_NCLASS];
+            HIPCHK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (int c = 0; c < pg::LK_NCLASS; c++) {
+                off[c + 1] = off[c] + hc[c];
+                m->info.n_class[c] = hc[c];
             }
+            for (int c = 0; c < pg::LK_NCLASS; c++)
+                if (hc[c] > 0)
+                    pg::launch_collect_status(nseeds, nullptr, cls, c, kl + off[c], fl, of, tp, cnt + c, s);
+            HIPCHK(hipGetLastError());
         }
-        if (!work) break;
+        if (mk) mk->mark(MK_PREDICT, s);
+        int64_t roff[pg::LK_NCLASS + 1] = {0};
+        for (int c = 0; c < pg::LK_NCLASS; c++) {
+            bk[c] = (int)std::min<int64_t>(pg::lk_bucket(c), nseeds - 1);
+            roff[c + 1] = roff[c] + (off[c + 1] - off[c]) * bk[c];
+        }
+        uint32_t* rows = m->knn.get<uint32_t>((size_t)std::max<int64_t>(roff[pg::LK_NCLASS], 1));
+        for (int c = 0; c < pg::LK_NCLASS; c++)
+            if (off[c + 1] > off[c])
+                pg::launch_knn(dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), nseeds, m->g.N,
+                               m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), kl + off[c],
+                               off[c + 1] - off[c], bk[c], rows + roff[c], s);
+        if (mk) mk->mark(MK_KNN, s);
+        for (int c = 0; c < pg::LK_NCLASS; c++)
+            if (off[c + 1] > off[c]) {
+                pg::launch_voronoi(dim, m->g.sorted.as<double>(), nseeds, kl + off[c], off[c + 1] - off[c],
+                                   rows + roff[c], bk[c], p.max_p, p.max_t, 0, m->faces(), s);
+                m->info.row_entries += (off[c + 1] - off[c]) * bk[c];
+            }
+        HIPCHK(hipGetLastError());
+        if (mk) mk->mark(MK_CLIP, s);
+        k_used = 0;
+        for (int c = 0; c < pg::LK_NCLASS; c++)
+            if (off[c + 1] > off[c])
+                k_used = std::max(k_used, restart_chain(m, kl + off[c], off[c + 1] - off[c], bk[c], s));
     }
     if (mk) mk->mark(MK_RESTART, s);
     m->info.n_seeds = nseeds;
     m->info.n_hydro = m->n_hydro;
     m->info.N_grid = m->g.N;
-    m->info.k_used = sched[ki];
-    m->info.n_restarted = restarted;
-    m->info.n_failed = collect(m, -1, lst, s);
+    m->info.k_used = k_used;
+    m->info.n_failed = collect_list(m, nullptr, nseeds, -1, m->list.get<uint32_t>(ns), s);
     if (!on_device) count_faces(m, s);
 }
 
@@ -411,6 +505,31 @@ int pg_ctx_sync(pg_ctx* c) {
     return guarded([&] {
         use(c);
         HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pg_stream_copy_probe(pg_ctx* c, int64_t bytes, int reps, double* ms) {
+    return guarded([&] {
+        use(c);
+        argchk(ms != nullptr && bytes >= 32 && reps >= 1, "bad argument");
+        bytes &= ~(int64_t)31;
+        DBuf a, b;
+        void* pa = a.get<char>((size_t)bytes);
+        void* pb = b.get<char>((size_t)bytes);
+        HIPCHK(hipMemsetAsync(pa, 1, (size_t)bytes, c->stream));
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        pg::launch_stream_copy(pa, pb, bytes, c->stream);  // warm-up
+        HIPCHK(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps; r++) pg::launch_stream_copy(pa, pb, bytes, c->stream);
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = (double)t / reps;
     });
 }
 
@@ -515,9 +634,11 @@ void pg_mesh_params_default(pg_mesh_params* p) {
     p->k_max = 256;
 }
 
-int pg_mesh_compute(pg_ctx* c, const pg_mesh_params* prm, const double* pts, int64_t n, pg_mesh** out) {
+int pg_mesh_compute_k(pg_ctx* c, const pg_mesh_params* prm, const pg_kmodel* km, const double* pts, int64_t n,
+                      pg_mesh** out) {
     return guarded([&] {
         use(c);
+        argchk(!km || km->device == c->device, "model lives on another device");
         argchk(prm && pts && out, "null argument");
         argchk(prm->dim == 2 || prm->dim == 3, "dim must be 2 or 3");
         argchk(prm->boundary >= 0 && prm->boundary <= 2, "unknown boundary");
@@ -531,13 +652,17 @@ int pg_mesh_compute(pg_ctx* c, const pg_mesh_params* prm, const double* pts, int
         try {
             m->prm = *prm;
             if (m->prm.k_max < m->prm.k) m->prm.k_max = m->prm.k;
-            mesh_build(c, m, pts, n);
+            mesh_build(c, m, pts, n, false, nullptr, km);
         } catch (...) {
             delete m;
             throw;
         }
         *out = m;
     });
+}
+
+int pg_mesh_compute(pg_ctx* c, const pg_mesh_params* prm, const double* pts, int64_t n, pg_mesh** out) {
+    return pg_mesh_compute_k(c, prm, nullptr, pts, n, out);
 }
 
 int pg_mesh_info(const pg_mesh* m, pg_mesh_info_t* info) {
@@ -710,6 +835,7 @@ int pg_state_free(pg_state** st) {
 // ================================================================ simulation
 struct pg_sim {
     pg_sim_params prm{};
+    const pg_kmodel* km = nullptr;
     pg_mesh mesh;
     int64_t n = 0;
     DBuf x, mass, mom, en, rho, vel, E, dt, part;
@@ -728,7 +854,7 @@ struct pg_sim {
 namespace {
 
 void sim_rebuild(pg_ctx* c, pg_sim* S, Marks* mk) {
-    mesh_build(c, &S->mesh, S->x.as<double>(), S->n, true, mk);
+    mesh_build(c, &S->mesh, S->x.as<double>(), S->n, true, mk, S->km);
     if (S->mesh.info.n_failed)
         throw Error{PG_ERR_CELLS, std::to_string(S->mesh.info.n_failed) + " cells left uncertified"};
     S->mesh.finalized = false;
@@ -844,7 +970,9 @@ int pg_sim_step(pg_ctx* c, pg_sim* S, double dt_max, double* dt_used) {
         t.hydro_ms = el(MK_START, MK_HYDRO);
         t.ghosts_ms = el(MK_HYDRO, MK_GHOSTS);
         t.grid_ms = el(MK_GHOSTS, MK_GRID);
-        t.knn_ms = el(MK_GRID, MK_KNN);
+        t.predict_ms = el(MK_GRID, MK_PREDICT);
+        t.knn_ms = el(MK_PREDICT, MK_KNN);
+        t.row_entries = S->mesh.info.row_entries;
         t.clip_ms = el(MK_KNN, MK_CLIP);
         t.restart_ms = el(MK_CLIP, MK_RESTART);
         t.final_ms = el(MK_RESTART, MK_FINAL);
@@ -924,6 +1052,220 @@ int pg_sim_free(pg_sim** S) {
         if (!S || !*S) return;
         delete *S;
         *S = nullptr;
+    });
+}
+
+int pg_sim_set_kmodel(pg_sim* S, pg_kmodel* km) {
+    return guarded([&] {
+        argchk(S != nullptr, "null sim");
+        S->km = km;
+    });
+}
+
+// ---------------------------------------------------------------- learned k
+namespace {
+
+uint64_t splitmix64(uint64_t& x) {
+    uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+void km_use(pg_ctx* c, const pg_kmodel* km) {
+    use(c);
+    argchk(km != nullptr, "null model");
+    argchk(km->device == c->device, "model lives on another device");
+}
+
+// Forward + backward of rows X[0..B) (device) on the stream; the summed
+// gradient lands in km->grad and the loss in km->loss.  lr > 0 also takes
+// one Adam step.
+void km_grad(pg_ctx* c, pg_kmodel* km, const float* X, const int32_t* y, int64_t n, int64_t B,
+             uint64_t seed, float lr) {
+    const int64_t nblk = pg::lk_grad_blocks(B);
+    float* part = km->part.get<float>((size_t)nblk * pg::LK_NPARAM);
+    float* pl = km->ploss.get<float>((size_t)nblk);
+    pg::launch_kgrad(X, y, n, B, seed, km->step, km->P.as<float>(), part, pl, c->stream);
+    pg::launch_kadam(part, pl, (int)nblk, km->P.as<float>(), km->m.as<float>(), km->v.as<float>(),
+                     km->grad.get<float>(pg::LK_NPARAM), km->loss.get<float>(1), km->step, lr, c->stream);
+    HIPCHK(hipGetLastError());
+    if (lr > 0.f) km->step++;
+}
+
+void km_upload_batch(pg_ctx* c, pg_kmodel* km, const float* feat, const int32_t* cls, int64_t n) {
+    HIPCHK(hipMemcpyAsync(km->X.get<float>((size_t)n * pg::LK_NFEAT), feat, sizeof(float) * n * pg::LK_NFEAT,
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(km->y.get<int32_t>((size_t)n), cls, sizeof(int32_t) * n, hipMemcpyHostToDevice,
+                          c->stream));
+}
+
+double km_read_loss(pg_ctx* c, pg_kmodel* km) {
+    float l = 0.f;
+    HIPCHK(hipMemcpyAsync(&l, km->loss.p, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return (double)l;
+}
+
+void check_classes(const int32_t* cls, int64_t n) {
+    for (int64_t i = 0; i < n; i++) argchk(cls[i] >= 0 && cls[i] < pg::LK_NCLASS, "class out of range");
+}
+
+}  // namespace
+
+int pg_kmodel_create(pg_ctx* c, uint64_t seed, pg_kmodel** out) {
+    return guarded([&] {
+        use(c);
+        argchk(out != nullptr, "null out");
+        static_assert(PG_KMODEL_NPARAM == pg::LK_NPARAM, "parameter count");
+        static_assert(PG_KMODEL_NFEAT == pg::LK_NFEAT && PG_KMODEL_NCLASS == pg::LK_NCLASS, "model shape");
+        pg_kmodel* km = new pg_kmodel();
+        try {
+            km->device = c->device;
+            std::vector<float> h(pg::LK_NPARAM, 0.f);
+            const int fan[3][2] = {{16, 64}, {64, 64}, {64, 6}};
+            const int off[3] = {0, 16 * 64 + 64, 16 * 64 + 64 + 64 * 64 + 64};
+            uint64_t st = seed;
+            for (int L = 0; L < 3; L++) {
+                const double a = std::sqrt(6.0 / (fan[L][0] + fan[L][1]));
+                for (int i = 0; i < fan[L][0] * fan[L][1]; i++) {
+                    const double u = (double)(splitmix64(st) >> 11) * (1.0 / 9007199254740992.0);
+                    h[off[L] + i] = (float)((2.0 * u - 1.0) * a);
+                }
+            }
+            HIPCHK(hipMemcpyAsync(km->P.get<float>(pg::LK_NPARAM), h.data(), sizeof(float) * pg::LK_NPARAM,
+                                  hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemsetAsync(km->m.get<float>(pg::LK_NPARAM), 0, sizeof(float) * pg::LK_NPARAM, c->stream));
+            HIPCHK(hipMemsetAsync(km->v.get<float>(pg::LK_NPARAM), 0, sizeof(float) * pg::LK_NPARAM, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        } catch (...) {
+            delete km;
+            throw;
+        }
+        *out = km;
+    });
+}
+
+int pg_kmodel_set_params(pg_ctx* c, pg_kmodel* km, const float* params) {
+    return guarded([&] {
+        km_use(c, km);
+        argchk(params != nullptr, "null params");
+        HIPCHK(hipMemcpyAsync(km->P.p, params, sizeof(float) * pg::LK_NPARAM, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemsetAsync(km->m.p, 0, sizeof(float) * pg::LK_NPARAM, c->stream));
+        HIPCHK(hipMemsetAsync(km->v.p, 0, sizeof(float) * pg::LK_NPARAM, c->stream));
+        km->step = 0;
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pg_kmodel_get_params(pg_ctx* c, const pg_kmodel* km, float* params) {
+    return guarded([&] {
+        km_use(c, km);
+        argchk(params != nullptr, "null params");
+        HIPCHK(hipMemcpyAsync(params, km->P.p, sizeof(float) * pg::LK_NPARAM, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pg_kmodel_set_cost(pg_kmodel* km, double a0) {
+    return guarded([&] {
+        argchk(km != nullptr, "null model");
+        argchk(a0 >= 0.0 && a0 < 1e6, "a0 out of range");
+        km->a0 = (float)a0;
+    });
+}
+
+int pg_kmodel_predict(pg_ctx* c, const pg_kmodel* kmc, const float* feat, int64_t n, int32_t* cls,
+                      float* logits) {
+    return guarded([&] {
+        km_use(c, kmc);
+        argchk(feat && cls && n >= 1 && n < (int64_t)1 << 31, "bad argument");
+        pg_kmodel* km = const_cast<pg_kmodel*>(kmc);
+        float* X = km->X.get<float>((size_t)n * pg::LK_NFEAT);
+        HIPCHK(hipMemcpyAsync(X, feat, sizeof(float) * n * pg::LK_NFEAT, hipMemcpyHostToDevice, c->stream));
+        int32_t* dc = km->cls.get<int32_t>((size_t)n);
+        float* dl = logits ? km->logits.get<float>((size_t)n * pg::LK_NCLASS) : nullptr;
+        pg::launch_kpredict(X, n, km->P.as<float>(), km->a0, nullptr, dc, dl, c->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(cls, dc, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+        if (logits)
+            HIPCHK(hipMemcpyAsync(logits, dl, sizeof(float) * n * pg::LK_NCLASS, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pg_kmodel_grad(pg_ctx* c, const pg_kmodel* kmc, const float* feat, const int32_t* cls, int64_t n,
+                   float* grad, double* loss) {
+    return guarded([&] {
+        km_use(c, kmc);
+        argchk(feat && cls && grad && n >= 1 && n < (int64_t)1 << 31, "bad argument");
+        check_classes(cls, n);
+        pg_kmodel* km = const_cast<pg_kmodel*>(kmc);
+        km_upload_batch(c, km, feat, cls, n);
+        km_grad(c, km, km->X.as<float>(), km->y.as<int32_t>(), n, n, 0, 0.f);
+        HIPCHK(hipMemcpyAsync(grad, km->grad.p, sizeof(float) * pg::LK_NPARAM, hipMemcpyDeviceToHost, c->stream));
+        const double l = km_read_loss(c, km);
+        if (loss) *loss = l;
+    });
+}
+
+int pg_kmodel_train_step(pg_ctx* c, pg_kmodel* km, const float* feat, const int32_t* cls, int64_t n, double lr,
+                         double* loss) {
+    return guarded([&] {
+        km_use(c, km);
+        argchk(feat && cls && n >= 1 && n < (int64_t)1 << 31 && lr > 0.0, "bad argument");
+        check_classes(cls, n);
+        km_upload_batch(c, km, feat, cls, n);
+        km_grad(c, km, km->X.as<float>(), km->y.as<int32_t>(), n, n, 0, (float)lr);
+        const double l = km_read_loss(c, km);
+        if (loss) *loss = l;
+    });
+}
+
+int pg_kmodel_train_mesh(pg_ctx* c, pg_kmodel* km, const pg_mesh* mc, int steps, int64_t batch, double lr,
+                         uint64_t seed, double* loss) {
+    return guarded([&] {
+        km_use(c, km);
+        argchk(mc != nullptr, "null mesh");
+        argchk(steps >= 1 && batch >= 16 && batch <= (1 << 22) && lr > 0.0, "bad training argument");
+        pg_mesh* m = const_cast<pg_mesh*>(mc);
+        const int64_t n = m->n_seeds;
+        argchk(n >= 1 && m->status.p, "mesh is not built");
+        float* X = km->X.get<float>((size_t)n * pg::LK_NFEAT);
+        int32_t* y = km->y.get<int32_t>((size_t)n);
+        pg::launch_kfeatures(m->prm.dim, m->g.sorted.as<double>(), n, m->g.N, m->g.counts.as<int32_t>(),
+                             m->g.starts.as<int32_t>(), nullptr, X, c->stream);
+        pg::launch_klabels(m->kcert.as<int32_t>(), n, y, c->stream);
+        HIPCHK(hipGetLastError());
+        const uint64_t sd = seed ? seed : 1;
+        for (int t = 0; t < steps; t++) km_grad(c, km, X, y, n, batch, sd, (float)lr);
+        const double l = km_read_loss(c, km);
+        if (loss) *loss = l;
+    });
+}
+
+int pg_kmodel_free(pg_kmodel** km) {
+    return guarded([&] {
+        if (!km || !*km) return;
+        delete *km;
+        *km = nullptr;
+    });
+}
+
+int pg_mesh_features(pg_ctx* c, const pg_mesh* mc, float* out) {
+    return guarded([&] {
+        use(c);
+        argchk(mc && out, "null argument");
+        pg_mesh* m = const_cast<pg_mesh*>(mc);
+        const int64_t n = m->n_seeds;
+        argchk(n >= 1 && m->g.sorted.p, "mesh is not built");
+        DBuf tmp;
+        float* d = tmp.get<float>((size_t)n * pg::LK_NFEAT);
+        pg::launch_kfeatures(m->prm.dim, m->g.sorted.as<double>(), n, m->g.N, m->g.counts.as<int32_t>(),
+                             m->g.starts.as<int32_t>(), m->g.perm.as<uint32_t>(), d, c->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, d, sizeof(float) * n * pg::LK_NFEAT, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
     });
 }
 

@@ -59,10 +59,11 @@ struct FaceArrays {
 
 // Clip + certify + extract the seeds listed in `seeds` (sorted ids; nullptr
 // = all n) with their neighbour rows knn[row q] (row q belongs to seeds[q]).
-// big != 0 selects the 128-slot kernel.
+// big != 0 selects the 128-slot kernel; restart launches the fast kernel
+// under the name k_voronoi_restart (profiles tell the passes apart).
 void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq,
                     const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa,
-                    hipStream_t s);
+                    hipStream_t s, bool restart = false);
 
 // Collect the sorted ids whose status is not SUCCESS (order: ascending id).
 // nfail (device int32) receives the count; tmp as for scan.
@@ -71,5 +72,42 @@ void launch_collect_failed(int64_t n, const int32_t* status, uint32_t* list, int
 void launch_collect_status(int64_t nq, const uint32_t* seeds, const int32_t* status, int code,
                            uint32_t* list, int32_t* flags, int32_t* offs, int32_t* tmp,
                            int32_t* nout, hipStream_t s);
+
+// ---------------------------------------------------------- learned_k.hip
+// Classifier of the neighbour count: 16 features -> 64 -> 64 -> 6 buckets.
+constexpr int LK_NFEAT = 16, LK_NCLASS = 6;
+constexpr int LK_NPARAM = 16 * 64 + 64 + 64 * 64 + 64 + 64 * 6 + 6;
+int lk_bucket(int c);  // 16, 24, 32, 48, 64, 96
+// Features of every seed (sorted space) into out[16 * (out_index ? out_index[s] : s)]:
+//   f0..f3  mean count per in-grid cell of the own cell / face / edge /
+//           corner neighbours of the 3^dim cube, over the mean occupancy
+//   f4      in-grid fraction of the cube
+//   f5..f7  |2u - 1|, u = position inside the own cell per axis (0 in 2D z)
+//   f8,f9   distance to the 4th / 8th nearest point of the cube, in cell
+//           widths (2 if absent);  f10 the nearest;  f15 = f9 / f8
+//   f11     log2(1 + cube density / mean density)
+//   f12     min(distance to the unit box faces in cell widths, 3) / 3
+//   f13     (max - min neighbour count) / mean occupancy
+//   f14     |sum of count * offset| / (3^dim * mean occupancy)
+void launch_kfeatures(int dim, const double* sorted, int64_t n, int N, const int32_t* counts,
+                      const int32_t* starts, const uint32_t* out_index, float* out, hipStream_t s);
+// cls[s] = bucket with the least expected cost of its restart chain
+// (k, 2k, 4k, ...; a pass costs a0 + k); logits
+// (n x 6, optional) receives the network outputs.
+void launch_kpredict(const float* X, int64_t n, const float* params, float a0, const uint32_t* out_index,
+                     int32_t* cls, float* logits, hipStream_t s);
+int64_t lk_grad_blocks(int64_t B);
+// Forward + backward of a batch of B rows (seed != 0: rows drawn from n by a
+// hash of (seed, step, i)); part: lk_grad_blocks(B) x LK_NPARAM partial sums.
+void launch_kgrad(const float* X, const int32_t* y, int64_t n, int64_t B, uint64_t seed, int64_t step,
+                  const float* params, float* part, float* part_loss, hipStream_t s);
+// Reduce the partials in block order; Adam step when lr > 0.
+void launch_kadam(const float* part, const float* part_loss, int nblk, float* params, float* m, float* v,
+                  float* grad_out, float* loss_out, int64_t step, float lr, hipStream_t s);
+// Training label (bucket index) of every seed from its certified k.
+void launch_klabels(const int32_t* kcert, int64_t n, int32_t* y, hipStream_t s);
+// ---------------------------------------------------------------- probe.hip
+// Streaming device copy of `bytes` (multiple of 32).
+void launch_stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
 
 }  // namespace pg

@@ -12,6 +12,7 @@
 // neighbour set is the same.
 #include "common.h"
 #include "kernels.h"
+#include "knn_wave.h"
 
 namespace pg {
 namespace {
@@ -146,11 +147,31 @@ __global__ void __launch_bounds__(BLOCK)
 #undef HI
 }
 
+template <int DIM, int M>
+void run_wave(const double* sorted, const uint32_t* perm, int N, const int32_t* starts,
+              const int32_t* counts, const uint32_t* queries, int64_t nq, int k, uint32_t* out,
+              hipStream_t s) {
+    k_knn_wave<DIM, M><<<(unsigned)nq, 64, 0, s>>>(sorted, perm, N, starts, counts, queries, nq, k, out);
+}
+
+template <int DIM>
+void dispatch_wave(const double* sorted, const uint32_t* perm, int N, const int32_t* starts,
+                   const int32_t* counts, const uint32_t* queries, int64_t nq, int k, uint32_t* out,
+                   hipStream_t s) {
+    if (k <= 64) run_wave<DIM, 1>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
+    else if (k <= 128) run_wave<DIM, 2>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
+    else if (k <= 192) run_wave<DIM, 3>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
+    else run_wave<DIM, 4>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
+}
+
 template <int DIM, int BLOCK>
 void run(const double* sorted, const uint32_t* perm, int N, const int32_t* starts,
          const int32_t* counts, const uint32_t* queries, int64_t nq, int k, uint32_t* out,
          hipStream_t s) {
     size_t smem = (size_t)k * BLOCK * (sizeof(double) + sizeof(uint32_t));
+    if (smem > 64 * 1024)
+        (void)hipFuncSetAttribute((const void*)k_knn<DIM, BLOCK>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)smem);
     unsigned nb = (unsigned)((nq + BLOCK - 1) / BLOCK);
     k_knn<DIM, BLOCK><<<nb, BLOCK, smem, s>>>(sorted, perm, N, starts, counts, queries, nq, k, out);
 }
@@ -162,16 +183,26 @@ void launch_knn(int dim, const double* sorted, const uint32_t* perm, int64_t n, 
                 int k, uint32_t* out, hipStream_t s) {
     (void)n;
     if (nq <= 0) return;
-    // LDS per block = 12 B * k * BLOCK, kept <= 48 KB
-    if (k <= 64) {
+    // Long rows (and short query lists) go to the wave-per-query kernel; the
+    // threshold can be moved with PGPU_KNN_WAVE_MIN_K for tuning runs.
+    static const int wave_min_k = [] {
+        const char* e = getenv("PGPU_KNN_WAVE_MIN_K");
+        return e ? atoi(e) : 33;
+    }();
+    if (k >= wave_min_k) {
+        if (dim == 2) dispatch_wave<2>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
+        else dispatch_wave<3>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
+        return;
+    }
+    // LDS per block = 12 B * k * BLOCK.  gfx950 gives a workgroup up to
+    // 160 KB, so full 64-lane waves are used up to k = 208; beyond that a
+    // 32-lane block (a half-filled wave) keeps the heap within 160 KB.
+    if (k <= 208) {
         if (dim == 2) run<2, 64>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
         else run<3, 64>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
-    } else if (k <= 128) {
+    } else {
         if (dim == 2) run<2, 32>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
         else run<3, 32>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
-    } else {
-        if (dim == 2) run<2, 16>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
-        else run<3, 16>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
     }
 }
 

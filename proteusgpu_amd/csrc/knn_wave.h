@@ -1,0 +1,195 @@
+// Wave-per-query k-nearest-neighbour kernel (device code; launched from knn.hip).
+#pragma once
+
+#include "common.h"
+
+namespace pg {
+
+// ------------------------------------------------------------------------
+// Wave-per-query variant.  The 64 lanes of a wave cooperate on one query:
+// the k best (dist^2, original id, sorted id) are kept sorted in registers,
+// slot-major (rank r at slot r / 64 of lane r % 64, M = ceil(k/64) slots).
+// Ring cells are enumerated 64 at a time, their point ranges prefix-summed
+// across the wave, and candidates processed 64 at a time: a candidate is
+// kept if it beats the current k-th entry (broadcast), and every kept
+// candidate is inserted by one cross-lane shift of the ranks above it.
+// Same pruning rule and same total order as k_knn, no LDS heap, full waves:
+// used for long rows (certificate restarts), where the per-thread heap would
+// leave the CUs nearly empty.
+struct Cand {
+    double d;
+    uint32_t o, s;  // original id (ties), sorted id (output)
+};
+
+PG_DEV bool cand_less(double da, uint32_t oa, double db, uint32_t ob) {
+    return da < db || (da == db && oa < ob);
+}
+
+template <int DIM, int M>
+__global__ void __launch_bounds__(64)
+    k_knn_wave(const double* __restrict__ sorted, const uint32_t* __restrict__ perm, int N,
+               const int32_t* __restrict__ starts, const int32_t* __restrict__ counts,
+               const uint32_t* __restrict__ queries, int64_t nq, int k, uint32_t* __restrict__ out) {
+    __shared__ int32_t c_start[64], c_pfx[65];
+    const int l = threadIdx.x;
+    const int64_t qi = blockIdx.x;
+    if (qi >= nq) return;  // whole block
+    const uint32_t q = queries ? queries[qi] : (uint32_t)qi;
+    double p[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; d++) p[d] = sorted[(size_t)DIM * q + d];
+    const int c = grid_cell<DIM>(p, N);
+    const int ci = c % N, cj = (c / N) % N, ck = DIM == 3 ? c / (N * N) : 0;
+
+    double kd[M];
+    uint32_t ko[M], ks[M];
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        kd[j] = __builtin_inf();
+        ko[j] = 0xffffffffu;
+        ks[j] = 0xffffffffu;
+    }
+    int size = 0;
+    const int kl = (k - 1) & 63, kj = (k - 1) >> 6;  // lane and slot of rank k-1
+
+    for (int ring = 0; ring < N; ring++) {
+        if (ring >= 1 && size == k) {
+            double kth = 0;
+#pragma unroll
+            for (int j = 0; j < M; j++)
+                if (j == kj) kth = __shfl(kd[j], kl, 64);
+            const double md = (double)(ring - 1) / (double)N;
+            if (kth < md * md) break;
+        }
+        const int side = 2 * ring + 1;
+        const int ncube = DIM == 3 ? side * side * side : side * side;
+        for (int base = 0; base < ncube; base += 64) {
+            // this lane's cell of the cube, if it lies on the ring and in the grid
+            const int t = base + l;
+            int cnt = 0, st = 0;
+            if (t < ncube) {
+                const int di = t % side - ring, dj = (t / side) % side - ring;
+                const int dk = DIM == 3 ? t / (side * side) - ring : 0;
+                const int cheb = max(abs(di), max(abs(dj), abs(dk)));
+                const int ii = ci + di, jj = cj + dj, kk = ck + dk;
+                if (cheb == ring && ii >= 0 && jj >= 0 && kk >= 0 && ii < N && jj < N && (DIM == 2 || kk < N)) {
+                    const int cc = ii + jj * N + kk * N * N;
+                    st = starts[cc];
+                    cnt = counts[cc];
+                }
+            }
+            int incl = cnt;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                int v = __shfl_up(incl, d, 64);
+                if (l >= d) incl += v;
+            }
+            const int total = __shfl(incl, 63, 64);
+            if (total == 0) continue;
+            c_start[l] = st;
+            c_pfx[l + 1] = incl;
+            if (l == 0) c_pfx[0] = 0;
+            __syncthreads();
+            for (int cb = 0; cb < total; cb += 64) {
+                const int j = cb + l;
+                bool take = false;
+                double dd = 0;
+                uint32_t co = 0, cs = 0;
+                if (j < total) {
+                    // cell holding candidate j: largest a with c_pfx[a] <= j
+                    int lo = 0, hi = 63;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (c_pfx[mid] <= j) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    cs = (uint32_t)(c_start[lo] + (j - c_pfx[lo]));
+                    if (cs != q) {
+                        double pc[3] = {0, 0, 0};
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) pc[d] = sorted[(size_t)DIM * cs + d];
+                        dd = dist2<DIM>(p, pc);
+                        co = perm[cs];
+                        take = true;
+                    }
+                }
+                if (size == k) {
+                    double kth = 0;
+                    uint32_t kto = 0;
+#pragma unroll
+                    for (int jj = 0; jj < M; jj++)
+                        if (jj == kj) {
+                            kth = __shfl(kd[jj], kl, 64);
+                            kto = __shfl(ko[jj], kl, 64);
+                        }
+                    take = take && cand_less(dd, co, kth, kto);
+                }
+                uint64_t surv = (uint64_t)__ballot(take);
+                while (surv) {
+                    const int src = __ffsll((unsigned long long)surv) - 1;
+                    surv &= surv - 1;
+                    const double xd = __shfl(dd, src, 64);
+                    const uint32_t xo = __shfl(co, src, 64), xs = __shfl(cs, src, 64);
+                    if (size == k) {
+                        // it may no longer beat the k-th entry after earlier inserts
+                        double kth = 0;
+                        uint32_t kto = 0;
+#pragma unroll
+                        for (int jj = 0; jj < M; jj++)
+                            if (jj == kj) {
+                                kth = __shfl(kd[jj], kl, 64);
+                                kto = __shfl(ko[jj], kl, 64);
+                            }
+                        if (!cand_less(xd, xo, kth, kto)) continue;
+                    }
+                    // insertion rank = number of entries below the candidate
+                    int pos = 0;
+#pragma unroll
+                    for (int jj = 0; jj < M; jj++) {
+                        const bool below = (jj * 64 + l < size) && cand_less(kd[jj], ko[jj], xd, xo);
+                        pos += __popcll((unsigned long long)__ballot(below));
+                    }
+                    // shift ranks >= pos up by one (slot-major), insert at pos
+                    double carry_d = 0;
+                    uint32_t carry_o = 0, carry_s = 0;
+#pragma unroll
+                    for (int jj = 0; jj < M; jj++) {
+                        const double up_d = __shfl_up(kd[jj], 1, 64);
+                        const uint32_t up_o = __shfl_up(ko[jj], 1, 64), up_s = __shfl_up(ks[jj], 1, 64);
+                        const double top_d = __shfl(kd[jj], 63, 64);
+                        const uint32_t top_o = __shfl(ko[jj], 63, 64), top_s = __shfl(ks[jj], 63, 64);
+                        const int r = jj * 64 + l;
+                        double nd = kd[jj];
+                        uint32_t no = ko[jj], ns = ks[jj];
+                        if (r == pos) {
+                            nd = xd;
+                            no = xo;
+                            ns = xs;
+                        } else if (r > pos) {
+                            nd = l > 0 ? up_d : carry_d;
+                            no = l > 0 ? up_o : carry_o;
+                            ns = l > 0 ? up_s : carry_s;
+                        }
+                        carry_d = top_d;
+                        carry_o = top_o;
+                        carry_s = top_s;
+                        kd[jj] = nd;
+                        ko[jj] = no;
+                        ks[jj] = ns;
+                    }
+                    if (size < k) size++;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    uint32_t* o = out + (size_t)qi * k;
+#pragma unroll
+    for (int jj = 0; jj < M; jj++) {
+        const int r = jj * 64 + l;
+        if (r < k) o[r] = r < size ? ks[jj] : 0xffffffffu;
+    }
+}
+
+
+}  // namespace pg

@@ -17,13 +17,29 @@ __global__ void __launch_bounds__(64) k_voronoi(const double* __restrict__ pts, 
     voronoi_group<DIM, G, S>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm);
 }
 
+// The same kernel under its own name for certificate restarts, so that
+// profiles separate the first pass from the restarts.
+template <int DIM, int G, int S>
+__global__ void __launch_bounds__(64) k_voronoi_restart(const double* __restrict__ pts, int64_t n_total,
+                                                        const uint32_t* __restrict__ seeds, int64_t nq,
+                                                        const uint32_t* __restrict__ knn, int k, int max_p,
+                                                        int max_t, FaceArrays fa) {
+    using C = VoroCfg<DIM, G, S>;
+    __shared__ SeedSmem<C::NS, C::MAXP> sm[C::SPW];
+    voronoi_group<DIM, G, S>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm);
+}
+
 template <int DIM, int G, int S>
 void run(const double* pts, int64_t n_total, const uint32_t* seeds, int64_t nq, const uint32_t* knn, int k,
-         int max_p, int max_t, FaceArrays fa, hipStream_t s) {
+         int max_p, int max_t, FaceArrays fa, bool restart, hipStream_t s) {
     using C = VoroCfg<DIM, G, S>;
     int64_t nb = (nq + C::SPW - 1) / C::SPW;
-    k_voronoi<DIM, G, S><<<dim3((unsigned)nb), dim3(64), 0, s>>>(pts, n_total, seeds, nq, knn, k, max_p,
-                                                                  max_t, fa);
+    if (restart)
+        k_voronoi_restart<DIM, G, S><<<dim3((unsigned)nb), dim3(64), 0, s>>>(pts, n_total, seeds, nq, knn, k,
+                                                                              max_p, max_t, fa);
+    else
+        k_voronoi<DIM, G, S><<<dim3((unsigned)nb), dim3(64), 0, s>>>(pts, n_total, seeds, nq, knn, k, max_p,
+                                                                      max_t, fa);
 }
 
 constexpr int TB = 256;
@@ -49,18 +65,18 @@ __global__ void __launch_bounds__(TB) k_compact(int64_t nq, const uint32_t* seed
 
 void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq,
                     const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa,
-                    hipStream_t s) {
+                    hipStream_t s, bool restart) {
     if (nq <= 0) return;
     if (dim == 2) {
         if (big)
-            run<2, 16, 8>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, s);
+            run<2, 16, 8>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, true, s);
         else
-            run<2, 16, 4>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, s);
+            run<2, 16, 4>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
     } else {
         if (big)
-            run<3, 16, 8>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, s);
+            run<3, 16, 8>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, true, s);
         else
-            run<3, 16, 4>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, s);
+            run<3, 16, 4>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
     }
 }
 

@@ -66,7 +66,10 @@ struct SeedSmem {
     uint8_t by_start[MAXP]; // 3D: boundary edge starting at a plane
     uint8_t by_end[MAXP];   // 3D: boundary edge ending at a plane
     uint32_t first[MAXP];   // extraction: lowest vertex slot on each plane
+    uint32_t nbi[32];       // next chunk of the neighbour row (prefetched)
+    double nbp[32][3];      // and the neighbours' positions
 };
+constexpr int NB_CHUNK = 32;
 
 // Round a non-negative double up to float (next float above on inexact).
 PG_DEV float f2f_up(double d) {
@@ -297,6 +300,19 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
     for (int v = 0; v < k; v++) {
         if (wave_ballot(!done) == 0ull) break;
         const bool act = !done;
+        // the group prefetches the next NB_CHUNK neighbours and their
+        // positions into LDS, so the clip loop issues no dependent global loads
+        if ((v & (NB_CHUNK - 1)) == 0) {
+            for (int j = gl; j < NB_CHUNK; j += G) {
+                if (act && v + j < k) {
+                    const uint32_t zz = row[v + j];
+                    sm.nbi[j] = zz;
+#pragma unroll
+                    for (int d = 0; d < D; d++) sm.nbp[j][d] = pts[(size_t)D * zz + d];
+                }
+            }
+            block_sync();
+        }
         // -- new half plane (cell.cpp:212-226)
         uint32_t z = 0;
         double bx = 0, by = 0, bz = 0;
@@ -304,10 +320,11 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
         bool plane_ok = false;
         int cur = 0;
         if (act) {
-            z = row[v];
-            bx = pts[(size_t)D * z];
-            by = pts[(size_t)D * z + 1];
-            bz = D == 3 ? pts[(size_t)D * z + 2] : 0.0;
+            const int jj = v & (NB_CHUNK - 1);
+            z = sm.nbi[jj];
+            bx = sm.nbp[jj][0];
+            by = sm.nbp[jj][1];
+            bz = D == 3 ? sm.nbp[jj][2] : 0.0;
             if (nb_v >= max_p) {
                 status = VERTEX_OVERFLOW;
             } else {
