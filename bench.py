@@ -115,6 +115,18 @@ def schedule_ms(ctx, sim):
     return (time.perf_counter() - t1) * 1e3
 
 
+def run_batch(ctx, B, warmup, steps):
+    for _ in range(warmup):
+        B.step()
+    ctx.sync()
+    gpu_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gpu_ms.append(B.step())
+    ctx.sync()
+    return (time.perf_counter() - t0) * 1e3 / steps, float(np.mean(gpu_ms))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -127,41 +139,56 @@ def main():
     pts = uniform_points(a.n, 3, 42)
     rho, vel, E = sod_state(pts, gamma)
     ctx = api.Context(0)
-    mk = lambda: api.Simulation(ctx, pts, rho, vel, E, boundary="reflective", k=a.k, gamma=gamma, cfl=cfl,
-                                flux="physical", move_mesh=True)
+    kw = dict(boundary="reflective", k=a.k, gamma=gamma, cfl=cfl, flux="physical", move_mesh=True)
 
-    # fixed k (the plain method): k=32 and certificate restarts
-    plain = mk()
+    # 1. the plain method: fixed k=32 + certificate restarts, host-driven step
+    plain = api.Simulation(ctx, pts, rho, vel, E, **kw)
     ms_plain, st_plain = run_steps(ctx, plain, a.warmup, a.steps)
     sched_plain = schedule_ms(ctx, plain)
 
-    # learned k: train on the initial mesh of the same workload (on the GPU),
-    # then run the same steps from the same initial state
-    learned = mk()
+    # 2. learned k: the classifier is trained on the GPU, in this run, on the
+    #    first mesh of the workload (certified k of every seed as labels)
     km = api.KModel(ctx, seed=1)
+    trainer = api.Simulation(ctx, pts, rho, vel, E, **kw)
     ctx.sync()
     t0 = time.perf_counter()
-    loss = km.train_mesh(learned, steps=a.train_steps, batch=4096, lr=3e-3)
+    loss = km.train_mesh(trainer, steps=a.train_steps, batch=4096, lr=3e-3)
     ctx.sync()
     train_ms = (time.perf_counter() - t0) * 1e3
-    learned.set_kmodel(km)
-    ms_learned, st_learned = run_steps(ctx, learned, a.warmup, a.steps)
-    sched_learned = schedule_ms(ctx, learned)
-    info = learned.mesh_info()
+    # learned k in the host-driven step (stage times)
+    trainer.set_kmodel(km)
+    ms_lsim, st_lsim = run_steps(ctx, trainer, a.warmup, a.steps)
+    info_l = trainer.mesh_info()
+    trainer.free()
 
-    # both runs took the same steps from the same state: identical results
-    pp, rp, vp, ep = plain.get()
-    pl, rl, vl, el_ = learned.get()
-    identical = bool(np.array_equal(pp, pl) and np.array_equal(rp, rl) and np.array_equal(vp, vl)
-                     and np.array_equal(ep, el_))
+    # 3. the batched step: one box, the whole step in one hipGraph launch,
+    #    fixed k and learned k
+    sizes = np.array([a.n])
+    bfix = api.Batch(ctx, sizes, pts, rho, vel, E, **kw)
+    ms_bfix, gpu_bfix = run_batch(ctx, bfix, a.warmup, a.steps)
+    blearn = api.Batch(ctx, sizes, pts, rho, vel, E, **kw)
+    blearn.set_kmodel(km)
+    ms_blearn, gpu_blearn = run_batch(ctx, blearn, a.warmup, a.steps)
+
+    # every run took the same steps from the same state: identical results
+    # (the batched runs did warmup + steps; a fresh host-driven run is
+    # brought to the same step count)
+    same = True
+    chk = api.Simulation(ctx, pts, rho, vel, E, **kw)
+    for _ in range(a.warmup + a.steps):
+        chk.step()
+    ref = chk.get()
+    for got in (bfix.get(), blearn.get()):
+        same = same and all(np.array_equal(x, y) for x, y in zip(got, ref))
+    chk.free()
 
     # roofline of the dominant kernel: the first clip pass (k_voronoi<3,16,4>,
     # one launch per k bucket) of the learned step
-    n_seeds = info["n_seeds"]
-    rows_first = float(sum(int(c) * b for c, b in zip(info["n_class"], (16, 24, 32, 48, 64, 96))))
-    launches = int(sum(1 for c in info["n_class"] if c > 0))
+    n_seeds = info_l["n_seeds"]
+    rows_first = float(sum(int(c) * b for c, b in zip(info_l["n_class"], (16, 24, 32, 48, 64, 96))))
+    launches = int(sum(1 for c in info_l["n_class"] if c > 0))
     clip_bytes = 4.0 * rows_first + n_seeds * CLIP_FIXED_BYTES_PER_CELL
-    clip_ms = st_learned["clip_ms"]
+    clip_ms = st_lsim["clip_ms"]
     achieved = clip_bytes / (clip_ms * 1e-3) / 1e9
     copy_gbs = ctx.stream_copy_probe(a.probe_bytes, 10)
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -171,44 +198,47 @@ def main():
             "avg_launch_ms": round(clip_ms / max(launches, 1), 4),
             "bytes_per_cell": round(clip_bytes / n_seeds, 1),
             "measured_copy_GBs": round(copy_gbs, 1), "frac_of_measured_copy": round(achieved / copy_gbs, 5)}
-    path_gbs = PATH_BYTES_PER_CELL * a.n / (ms_learned * 1e-3) / 1e9
+    best = ms_blearn
+    path_gbs = PATH_BYTES_PER_CELL * a.n / (best * 1e-3) / 1e9
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(a.cpu_sample, a.k)
-    tot = learned.totals()
+    tot = blearn.totals()[0]
     out = {
         "metric": "cells/s per Voronoi rebuild+HLL step, 1M 3D seeds; learned-k vs fixed-k speedup",
-        "value": round(a.n / (ms_learned * 1e-3), 1), "unit": "cells/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": round(ms_learned, 3), "higher_is_better": True, "scaling": "weak",
+        "value": round(a.n / (best * 1e-3), 1), "unit": "cells/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(best, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"3D {a.n} uniform generators, reflective box, Sod along x (gamma 1.4), "
-                               f"CFL 0.3, moving mesh, one full Voronoi rebuild per step; k per seed from "
-                               f"the learned classifier (trained in this run on the first mesh) with "
-                               f"certificate restarts; fixed k={a.k} + restarts timed beside",
+                               f"CFL 0.3, moving mesh, one full Voronoi rebuild per step; batched step (one "
+                               f"hipGraph launch) with k per seed from the learned classifier (trained in this "
+                               f"run on the first mesh) and certificate restarts; the plain fixed k={a.k} step "
+                               f"timed beside",
                    "n_cells": a.n, "k_fixed": a.k},
         "roofline": roof, "cpu_baseline": cpu,
-        "fixed_k": {"ms_per_step": round(ms_plain, 3), "cells_per_s": round(a.n / (ms_plain * 1e-3), 1),
-                    "stages_ms": {k: round(v, 3) for k, v in st_plain.items() if k.endswith("_ms")},
-                    "restarted_seeds_per_step": st_plain["n_restarted"],
-                    "row_entries_per_step": st_plain["row_entries"],
-                    "schedule_1rebuild_10hll_ms": round(sched_plain, 3)},
-        "learned_k": {"ms_per_step": round(ms_learned, 3),
-                      "stages_ms": {k: round(v, 3) for k, v in st_learned.items() if k.endswith("_ms")},
-                      "restarted_seeds_per_step": st_learned["n_restarted"],
-                      "row_entries_per_step": st_learned["row_entries"],
-                      "seeds_per_bucket": [int(c) for c in info["n_class"]],
-                      "schedule_1rebuild_10hll_ms": round(sched_learned, 3),
-                      "train_ms": round(train_ms, 2), "train_steps": a.train_steps, "train_loss": round(loss, 4)},
-        "learned_k_speedup": round(ms_plain / ms_learned, 4),
-        "learned_equals_fixed_state": identical,
+        "learned_k_speedup": round(ms_plain / best, 4),
+        "results_identical": same,
+        "plain_fixed_k": {"ms_per_step": round(ms_plain, 3), "cells_per_s": round(a.n / (ms_plain * 1e-3), 1),
+                          "stages_ms": {k: round(v, 3) for k, v in st_plain.items() if k.endswith("_ms")},
+                          "restarted_seeds_per_step": st_plain["n_restarted"],
+                          "row_entries_per_step": st_plain["row_entries"],
+                          "schedule_1rebuild_10hll_ms": round(sched_plain, 3)},
+        "learned_k_host_step": {"ms_per_step": round(ms_lsim, 3),
+                                "stages_ms": {k: round(v, 3) for k, v in st_lsim.items() if k.endswith("_ms")},
+                                "restarted_seeds_per_step": st_lsim["n_restarted"],
+                                "row_entries_per_step": st_lsim["row_entries"],
+                                "seeds_per_bucket": [int(c) for c in info_l["n_class"]]},
+        "batched_fixed_k": {"ms_per_step": round(ms_bfix, 3), "gpu_ms_per_step": round(gpu_bfix, 3)},
+        "batched_learned_k": {"ms_per_step": round(ms_blearn, 3), "gpu_ms_per_step": round(gpu_blearn, 3)},
+        "training": {"ms": round(train_ms, 2), "adam_steps": a.train_steps, "batch": 4096,
+                     "final_loss": round(loss, 4)},
         "path_roofline": {"bytes_per_cell": PATH_BYTES_PER_CELL, "achieved_GBs": round(path_gbs, 2),
                           "frac": round(path_gbs / HBM_PEAK_GBS, 5)},
         "conserved_totals": [float(x) for x in tot],
     }
     print(json.dumps(out))
-    plain.free()
-    learned.free()
-    km.free()
+    for o in (plain, bfix, blearn, km):
+        o.free()
     ctx.close()
 
 

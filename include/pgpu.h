@@ -53,6 +53,7 @@ typedef struct pg_mesh pg_mesh;
 typedef struct pg_state pg_state;
 typedef struct pg_sim pg_sim;
 typedef struct pg_kmodel pg_kmodel;
+typedef struct pg_batch pg_batch;
 
 /* ------------------------------------------------------------------ context */
 const char* pg_version(void);
@@ -220,11 +221,13 @@ int pg_sim_set_kmodel(pg_sim* sim, pg_kmodel* model);
 int pg_kmodel_create(pg_ctx* ctx, uint64_t seed, pg_kmodel** out);
 int pg_kmodel_set_params(pg_ctx* ctx, pg_kmodel* model, const float* params);
 int pg_kmodel_get_params(pg_ctx* ctx, const pg_kmodel* model, float* params);
-/* Cost model of the bucket choice: a seed costs a0 + k per pass and a
- * rejected seed repeats at 2k (then 4k, ...); the bucket whose chain has the
- * least expected cost under the predicted distribution wins (default
- * a0 = 32). */
-int pg_kmodel_set_cost(pg_kmodel* model, double a0);
+/* Cost model of the bucket choice: a pass at row length k costs k (KNN row)
+ * + r m (m clip steps: the certified k, or k when the certificate fails);
+ * a rejected seed repeats at 2k (then 4k, ...).  The bucket whose chain has
+ * the least expected cost under the predicted distribution wins.
+ * r = clip cost per step / KNN cost per row entry (default 2.8, measured on
+ * MI355X at 1M 3D seeds). */
+int pg_kmodel_set_cost(pg_kmodel* model, double r);
 /* Host features (n x 16) -> bucket index per row (and the 6 logits per row
  * when logits != NULL). */
 int pg_kmodel_predict(pg_ctx* ctx, const pg_kmodel* model, const float* feat, int64_t n, int32_t* cls,
@@ -249,6 +252,53 @@ int pg_mesh_features(pg_ctx* ctx, const pg_mesh* mesh, float* out);
  * fixed k of the parameters). */
 int pg_mesh_compute_k(pg_ctx* ctx, const pg_mesh_params* params, const pg_kmodel* model, const double* pts,
                       int64_t n, pg_mesh** out);
+
+/* ------------------------------------------------------------------ batches */
+/* No counterpart in the reference (one box per run).  Many independent boxes
+ * of uneven size advanced together: each pg_batch_step is one hipGraph
+ * launch (captured at the first step) holding the moving-mesh step of
+ * every box -- primitives, per-box CFL step, HLL update, drift, ghosts,
+ * per-box grids, KNN, clip and the restart chain -- with all per-step
+ * counts kept on the device.  Every box is bit-identical to the same box
+ * run alone through pg_sim_step.  Boxes are concatenated box-major in the
+ * point and state arrays; coordinates of every box lie in its own unit box.
+ * Every box needs more than max(k, k_max) + 1 points. */
+typedef struct {
+    int clip_slots;      /* vertex slots per seed of the fast clip kernel: 48 or 64 */
+    int knn_wave_min_k;  /* rows at least this long use the wave-per-query KNN      */
+    int blocks_per_cu;   /* persistent grid size of the KNN and clip kernels         */
+    int use_graph;       /* 1: capture the step in a hipGraph, 0: plain launches     */
+} pg_batch_tuning;
+
+typedef struct {
+    int nbox;
+    int64_t n_hydro;        /* cells over all boxes                              */
+    int64_t n_seeds;        /* cells + ghosts of the current mesh                */
+    int64_t seed_capacity;  /* seeds reserved (ghost layers included)            */
+    double last_step_ms;
+} pg_batch_info_t;
+
+int pg_batch_create(pg_ctx* ctx, const pg_sim_params* params, int nbox, const int64_t* n_per_box,
+                    const double* pts, const double* rho, const double* vel, const double* E, pg_batch** out);
+/* One step of every box; ms (optional) = GPU time of the step. */
+int pg_batch_step(pg_ctx* ctx, pg_batch* batch, double* ms);
+int pg_batch_set_tuning(pg_batch* batch, const pg_batch_tuning* t);
+/* Learned k for the following steps (NULL: the fixed k of the parameters);
+ * the model must outlive its use.  Results do not change. */
+int pg_batch_set_kmodel(pg_batch* batch, const pg_kmodel* model);
+int pg_batch_get_tuning(const pg_batch* batch, pg_batch_tuning* t);
+/* Time every tuning candidate (clip slots x KNN kernel choice x grid size)
+ * on `reps` steps from the current state, restore the state and keep the
+ * fastest.  Results never depend on the tuning. */
+int pg_batch_autotune(pg_ctx* ctx, pg_batch* batch, int reps, pg_batch_tuning* best, double* best_ms);
+/* Primitives and generator positions, box-major. */
+int pg_batch_get(pg_ctx* ctx, pg_batch* batch, double* pts, double* rho, double* vel, double* E);
+/* CFL step each box took in the last step (nbox doubles). */
+int pg_batch_dt(pg_ctx* ctx, const pg_batch* batch, double* dt);
+/* FP64 totals per box: [mass, px, py, pz, energy] x nbox. */
+int pg_batch_totals(pg_ctx* ctx, pg_batch* batch, double* tot);
+int pg_batch_info(pg_ctx* ctx, const pg_batch* batch, pg_batch_info_t* info);
+int pg_batch_free(pg_batch** batch);
 
 #ifdef __cplusplus
 }

@@ -125,29 +125,30 @@ def loss_grad(params, X, y, bf16_ops=True):
     return loss, pack({"W1": dW1, "b1": db1, "W2": dW2, "b2": db2, "W3": dW3, "b3": db3})
 
 
-def decide(logits, a0):
-    """Bucket with the least expected cost of its restart chain.
+# representative certified k of each bucket's interval (midpoints; 80 for the
+# open last bucket), used by the cost model of the decision
+KREP = (12.0, 20.0, 28.0, 40.0, 56.0, 80.0)
 
-    A pass at row length k costs a0 + k per seed (fixed work plus work
-    proportional to the row); a seed the certificate rejects repeats at the
-    next length of the restart schedule (at least 2k).  Starting at bucket b
-    the chain runs k_0 = k_b, k_1 = 2 k_0, ... and stage j is needed with
-    probability S(k_{j-1}) = P(kcert > k_{j-1}) = sum of p_c over buckets
-    k_c > k_{j-1}; the chain ends when no bucket lies above."""
+
+def chain_cost(k0, K, r):
+    """Cost, in units of one KNN row entry, of a seed with certified k = K
+    started at row length k0: each pass costs k (KNN row) + r * m (clip
+    steps: m = K once certified, else the whole row), and a rejected pass
+    repeats at twice the row length (capped at 256)."""
+    k, cost = float(k0), 0.0
+    while True:
+        if K <= k:
+            return cost + k + r * K
+        cost += k + r * k
+        k = min(2.0 * k, 256.0)
+
+
+def decide(logits, r):
+    """Bucket with the least expected chain cost under the softmax of the
+    logits (r = clip cost per step over KNN cost per row entry)."""
     p = softmax(np.asarray(logits, dtype=np.float64))
-    kb = np.asarray(BUCKETS, dtype=np.float64)
-    cost = np.zeros_like(p)
-    for b in range(NC):
-        surv = np.ones(p.shape[0])
-        k = kb[b]
-        while True:
-            cost[:, b] += surv * (a0 + k)
-            above = kb > k
-            if not above.any():
-                break
-            surv = p[:, above].sum(axis=1)
-            k = 2.0 * k
-    return np.argmin(cost, axis=1).astype(np.int32)
+    C = np.array([[chain_cost(BUCKETS[b], KREP[c], r) for c in range(NC)] for b in range(NC)])
+    return np.argmin(p @ C.T, axis=1).astype(np.int32)
 
 
 def adam(params, grad, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):

@@ -58,6 +58,16 @@ class SimTimes(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("clip_slots", C.c_int), ("knn_wave_min_k", C.c_int), ("blocks_per_cu", C.c_int),
+                ("use_graph", C.c_int)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("nbox", C.c_int), ("n_hydro", C.c_int64), ("n_seeds", C.c_int64), ("seed_capacity", C.c_int64),
+                ("last_step_ms", C.c_double)]
+
+
 KMODEL_NFEAT, KMODEL_NCLASS, KMODEL_NPARAM = 16, 6, 5638
 KMODEL_BUCKETS = (16, 24, 32, 48, 64, 96)
 
@@ -111,6 +121,17 @@ def lib():
         "pg_kmodel_free": ([pp], i32),
         "pg_mesh_features": ([vp, vp, vp], i32),
         "pg_mesh_compute_k": ([vp, C.POINTER(MeshParams), vp, vp, i64, pp], i32),
+        "pg_batch_create": ([vp, C.POINTER(SimParams), i32, vp, vp, vp, vp, vp, pp], i32),
+        "pg_batch_step": ([vp, vp, C.POINTER(C.c_double)], i32),
+        "pg_batch_set_tuning": ([vp, C.POINTER(BatchTuning)], i32),
+        "pg_batch_set_kmodel": ([vp, vp], i32),
+        "pg_batch_get_tuning": ([vp, C.POINTER(BatchTuning)], i32),
+        "pg_batch_autotune": ([vp, vp, i32, C.POINTER(BatchTuning), C.POINTER(C.c_double)], i32),
+        "pg_batch_get": ([vp, vp, vp, vp, vp, vp], i32),
+        "pg_batch_dt": ([vp, vp, vp], i32),
+        "pg_batch_totals": ([vp, vp, vp], i32),
+        "pg_batch_info": ([vp, vp, C.POINTER(BatchInfo)], i32),
+        "pg_batch_free": ([pp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -136,4 +157,6 @@ def exported_symbols():
             "pg_sim_totals", "pg_sim_free", "pg_sim_set_kmodel", "pg_kmodel_create", "pg_kmodel_set_params",
             "pg_kmodel_get_params", "pg_kmodel_set_cost", "pg_kmodel_predict", "pg_kmodel_grad",
             "pg_kmodel_train_step", "pg_kmodel_train_mesh", "pg_kmodel_free", "pg_mesh_features",
-            "pg_mesh_compute_k"]
+            "pg_mesh_compute_k", "pg_batch_create", "pg_batch_step", "pg_batch_set_tuning", "pg_batch_set_kmodel",
+            "pg_batch_get_tuning", "pg_batch_autotune", "pg_batch_get", "pg_batch_dt", "pg_batch_totals",
+            "pg_batch_info", "pg_batch_free"]

@@ -359,3 +359,89 @@ class Simulation:
 
     def __del__(self):
         self.free()
+
+
+class Batch:
+    """Many independent boxes advanced by one captured step (pg_batch_*).
+
+    sizes: points per box; pts/rho/vel/E concatenated box-major, each box in
+    its own unit box.  step() advances every box by its own CFL step."""
+
+    def __init__(self, ctx: Context, sizes, pts, rho, vel, E, boundary="periodic", buff=0.04, k=16,
+                 gamma=5.0 / 3.0, cfl=0.3, flux="physical", move_mesh=True, max_p=64, max_t=96, restart=True,
+                 k_max=256):
+        from ._native import SimParams
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        self.ctx, self.dim = ctx, pts.shape[1]
+        self.sizes = np.ascontiguousarray(sizes, dtype=np.int64)
+        self.n = int(self.sizes.sum())
+        assert pts.shape[0] == self.n
+        p = SimParams()
+        p.mesh = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max)
+        p.gamma, p.cfl, p.flux_mode, p.move_mesh = gamma, cfl, FLUX[flux], int(move_mesh)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, vel, E)]
+        h = C.c_void_p()
+        check(lib().pg_batch_create(ctx.h, C.byref(p), len(self.sizes), _ptr(self.sizes), _ptr(pts),
+                                    _ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]), C.byref(h)))
+        self.h = h
+
+    def step(self) -> float:
+        """One step of every box; returns its GPU time in ms."""
+        ms = C.c_double()
+        check(lib().pg_batch_step(self.ctx.h, self.h, C.byref(ms)))
+        return ms.value
+
+    @property
+    def tuning(self) -> dict:
+        from ._native import BatchTuning
+        t = BatchTuning()
+        check(lib().pg_batch_get_tuning(self.h, C.byref(t)))
+        return {name: getattr(t, name) for name, _ in t._fields_}
+
+    def set_tuning(self, clip_slots=64, knn_wave_min_k=33, blocks_per_cu=8, use_graph=True):
+        from ._native import BatchTuning
+        t = BatchTuning(clip_slots, knn_wave_min_k, blocks_per_cu, int(use_graph))
+        check(lib().pg_batch_set_tuning(self.h, C.byref(t)))
+
+    def set_kmodel(self, km: "KModel | None"):
+        """Learned k for the following steps (None: fixed k)."""
+        check(lib().pg_batch_set_kmodel(self.h, km.h if km is not None else None))
+        self._km = km
+
+    def autotune(self, reps=2):
+        from ._native import BatchTuning
+        t, ms = BatchTuning(), C.c_double()
+        check(lib().pg_batch_autotune(self.ctx.h, self.h, reps, C.byref(t), C.byref(ms)))
+        return {name: getattr(t, name) for name, _ in t._fields_}, ms.value
+
+    def get(self):
+        pts = np.empty((self.n, self.dim))
+        rho = np.empty(self.n)
+        vel = np.empty((self.n, self.dim))
+        E = np.empty(self.n)
+        check(lib().pg_batch_get(self.ctx.h, self.h, _ptr(pts), _ptr(rho), _ptr(vel), _ptr(E)))
+        return pts, rho, vel, E
+
+    def dt(self):
+        out = np.empty(len(self.sizes))
+        check(lib().pg_batch_dt(self.ctx.h, self.h, _ptr(out)))
+        return out
+
+    def totals(self):
+        out = np.empty((len(self.sizes), 5))
+        check(lib().pg_batch_totals(self.ctx.h, self.h, _ptr(out)))
+        return out
+
+    def info(self) -> dict:
+        from ._native import BatchInfo
+        i = BatchInfo()
+        check(lib().pg_batch_info(self.ctx.h, self.h, C.byref(i)))
+        return {name: getattr(i, name) for name, _ in i._fields_}
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().pg_batch_free(C.byref(self.h))
+            self.h = None
+
+    def __del__(self):
+        self.free()

@@ -77,10 +77,7 @@ struct DBuf {
     }
 };
 
-// knn.cpp:101, float arithmetic as the reference.
-int ngrid(int dim, int64_t n) {
-    return std::max(1, (int)std::round(std::pow((float)n / 3.1f, 1.0f / (float)dim)));
-}
+int ngrid(int dim, int64_t n) { return pg::ngrid_of(dim, n); }
 
 const int KSCHED[] = {16, 24, 32, 48, 64, 96, 128, 192, 256};
 constexpr int MAXP = 64;
@@ -139,7 +136,7 @@ struct pg_mesh {
     Grid g;
     DBuf in_pts, gcnt, goff, gsrc, gtot;
     DBuf knn, rows, list, list2, list3, flags, offs, cnt1;
-    DBuf feat, kcls, klist;  // learned k: features, bucket per seed, seeds by bucket
+    DBuf feat, kcls, klist, kcnt;  // learned k: features, bucket per seed, seeds by bucket, bucket sizes
     // faces (padded, face-slot major) and cells, sorted order
     DBuf nbr, area, nrm, cen, count, vol, ccen, status, kcert;
     // hydro tables
@@ -166,7 +163,7 @@ struct pg_mesh {
 
 struct pg_kmodel {
     int device = 0;
-    float a0 = 32.f;
+    float a0 = 2.8f;  // clip cost per step over KNN cost per row entry
     int64_t step = 0;  // Adam steps taken
     DBuf P, m, v, part, ploss, loss, grad, X, y, cls, logits;
 };
@@ -347,59 +344,52 @@ void mesh_build(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool on_dev
         // bucket (ascending id inside a bucket) -> KNN + clip per bucket
         float* feat = m->feat.get<float>(ns * pg::LK_NFEAT);
         int32_t* cls = m->kcls.get<int32_t>(ns);
-        uint32_t* kl = m->klist.get<uint32_t>(ns);
         pg::launch_kfeatures(dim, m->g.sorted.as<double>(), nseeds, m->g.N, m->g.counts.as<int32_t>(),
                              m->g.starts.as<int32_t>(), nullptr, feat, s);
         pg::launch_kpredict(feat, nseeds, km->P.as<float>(), km->a0, nullptr, cls, nullptr, s);
         HIPCHK(hipGetLastError());
-        int64_t off[pg::LK_NCLASS + 1] = {0};
-        int bk[pg::LK_NCLASS];
+        int64_t hc[pg::LK_NCLASS];
         {
-            int32_t* fl = m->flags.get<int32_t>(ns);
-            int32_t* of = m->offs.get<int32_t>(ns);
-            int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(std::max(nseeds, m->g.ncell)));
-            int32_t* cnt = m->cnt1.get<int32_t>(pg::LK_NCLASS);
-            for (int c = 0; c < pg::LK_NCLASS; c++)
-                pg::launch_collect_status(nseeds, nullptr, cls, c, kl, fl, of, tp, cnt + c, s);
-            // each collect overwrote kl from its start: run them again into
-            // their final segments once the counts are known
-            int32_t hc[pg::LK_NCLASS];
-            HIPCHK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            for (int c = 0; c < pg::LK_NCLASS; c++) {
-                off[c + 1] = off[c] + hc[c];
-                m->info.n_class[c] = hc[c];
-            }
-            for (int c = 0; c < pg::LK_NCLASS; c++)
-                if (hc[c] > 0)
-                    pg::launch_collect_status(nseeds, nullptr, cls, c, kl + off[c], fl, of, tp, cnt + c, s);
+            const int64_t nbk = pg::lk_bucket_blocks(nseeds) * pg::LK_NCLASS;
+            int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(std::max({nseeds, m->g.ncell, nbk})));
+            int64_t* dcnt = m->kcnt.get<int64_t>(pg::LK_NCLASS);
+            pg::launch_kbucket(cls, nullptr, nseeds, m->flags.get<int32_t>(std::max<size_t>(ns, nbk)),
+                               m->offs.get<int32_t>(std::max<size_t>(ns, nbk)), tp, m->cnt1.get<int32_t>(1), ns,
+                               m->klist.get<uint32_t>(ns * pg::LK_NCLASS), dcnt, s);
             HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(hc, dcnt, sizeof(hc), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
         }
-        if (mk) mk->mark(MK_PREDICT, s);
+        uint32_t* kl = m->klist.as<uint32_t>();
+        int bk[pg::LK_NCLASS];
         int64_t roff[pg::LK_NCLASS + 1] = {0};
         for (int c = 0; c < pg::LK_NCLASS; c++) {
+            m->info.n_class[c] = hc[c];
             bk[c] = (int)std::min<int64_t>(pg::lk_bucket(c), nseeds - 1);
-            roff[c + 1] = roff[c] + (off[c + 1] - off[c]) * bk[c];
+            roff[c + 1] = roff[c] + hc[c] * bk[c];
         }
+        if (mk) mk->mark(MK_PREDICT, s);
         uint32_t* rows = m->knn.get<uint32_t>((size_t)std::max<int64_t>(roff[pg::LK_NCLASS], 1));
         for (int c = 0; c < pg::LK_NCLASS; c++)
-            if (off[c + 1] > off[c])
+            if (hc[c] > 0)
                 pg::launch_knn(dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), nseeds, m->g.N,
-                               m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), kl + off[c],
-                               off[c + 1] - off[c], bk[c], rows + roff[c], s);
+                               m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), kl + (size_t)c * ns, hc[c],
+                               bk[c], rows + roff[c], s);
         if (mk) mk->mark(MK_KNN, s);
         for (int c = 0; c < pg::LK_NCLASS; c++)
-            if (off[c + 1] > off[c]) {
-                pg::launch_voronoi(dim, m->g.sorted.as<double>(), nseeds, kl + off[c], off[c + 1] - off[c],
-                                   rows + roff[c], bk[c], p.max_p, p.max_t, 0, m->faces(), s);
-                m->info.row_entries += (off[c + 1] - off[c]) * bk[c];
+            if (hc[c] > 0) {
+                pg::launch_voronoi(dim, m->g.sorted.as<double>(), nseeds, kl + (size_t)c * ns, hc[c], rows + roff[c],
+                                   bk[c], p.max_p, p.max_t, 0, m->faces(), s);
+                m->info.row_entries += hc[c] * bk[c];
             }
         HIPCHK(hipGetLastError());
         if (mk) mk->mark(MK_CLIP, s);
-        k_used = 0;
-        for (int c = 0; c < pg::LK_NCLASS; c++)
-            if (off[c + 1] > off[c])
-                k_used = std::max(k_used, restart_chain(m, kl + off[c], off[c + 1] - off[c], bk[c], s));
+        // every rejected seed (any bucket) continues in one restart chain:
+        // a seed recomputed at a row length at least its certified k gives
+        // the same cell whatever its first bucket, so the chain may start at
+        // 64 (overflowed seeds at 64, uncertified ones at 128, then 256)
+        k_used = std::max(restart_chain(m, nullptr, nseeds, std::min<int64_t>(64, nseeds - 1), s),
+                          *std::max_element(bk, bk + pg::LK_NCLASS));
     }
     if (mk) mk->mark(MK_RESTART, s);
     m->info.n_seeds = nseeds;
@@ -1266,6 +1256,527 @@ int pg_mesh_features(pg_ctx* c, const pg_mesh* mc, float* out) {
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(out, d, sizeof(float) * n * pg::LK_NFEAT, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+}  // extern "C"
+
+// ================================================================== batches
+struct pg_batch {
+    pg_sim_params prm{};
+    int nbox = 0, dim = 3;
+    bool periodic = true;
+    int64_t n_h = 0, cap = 0, cell_cap = 0;
+    std::vector<int64_t> hoff_h;
+    std::vector<int> ksched, ksched_learned;
+    // tuning (timing only; results do not depend on it)
+    int slots = 64, wave_min_k = 33, bpc = 8;
+    bool use_graph = true;
+    const pg_kmodel* km = nullptr;  // learned k (nullptr: fixed k)
+    // static
+    DBuf hoff, hbox, x, xs;
+    DBuf feat, kcls, klists, kcnt;  // learned k
+    // conserved state and primitives (hydro order)
+    DBuf mass, mom, en, rho, vel, E, dt_box;
+    DBuf mass_s, mom_s, en_s;  // autotuner save area
+    // per step
+    DBuf gcnt, goff, gtot, soff, N, coff, stat, spts, sbox, orig_h;
+    DBuf cell, counts, starts, fill, tmp, total, perm, sorted, sorted_box;
+    DBuf knn, lists, ncount, flags, offs;
+    DBuf nbr, area, nrm, cen, count, vol, ccen, status, kcert;
+    DBuf hid, area_u, vol_u, o2s, hsorted, tot;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float last_ms = 0.f;
+    ~pg_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+    pg::SimView view() {
+        return pg::SimView{n_h, mass.as<double>(), mom.as<double>(), en.as<double>(), rho.as<double>(),
+                           vel.as<double>(), E.as<double>()};
+    }
+    pg::FaceArrays faces() {
+        pg::FaceArrays fa;
+        fa.stride = cap;
+        fa.nbr = nbr.as<int32_t>();
+        fa.area = area.as<double>();
+        fa.nrm = nrm.as<double>();
+        fa.cen = cen.as<double>();
+        fa.count = count.as<uint8_t>();
+        fa.vol = vol.as<double>();
+        fa.ccen = ccen.as<double>();
+        fa.status = status.as<int32_t>();
+        fa.kcert = kcert.as<int32_t>();
+        return fa;
+    }
+    double scale() const { return periodic ? 1. + (2 * prm.mesh.buff) : 1.0; }
+    int64_t* n_dev() { return soff.as<int64_t>() + nbox; }
+    // status block: [0] capacity error bits, [1] hydro cells without faces,
+    // [2] periodic hydro faces on the box plane, [3] uncertified seeds
+    int32_t* st() { return stat.as<int32_t>(); }
+};
+
+namespace {
+
+// The rebuild of every box (ghosts, grids, KNN, clip, restarts, hydro
+// tables), enqueued on the stream with device-side counts only.
+void batch_rebuild(pg_batch* B, hipStream_t s) {
+    const int dim = B->dim;
+    const pg_mesh_params& p = B->prm.mesh;
+    const int64_t cap = B->cap;
+    pg::launch_fill_i32(B->st(), 4, 0, s);
+    const int32_t* goff = nullptr;
+    if (B->periodic) {
+        pg::launch_ghost_count(dim, B->x.as<double>(), B->n_h, p.buff, B->gcnt.as<int32_t>(), s);
+        pg::scan_exclusive_i32(B->gcnt.as<int32_t>(), B->goff.as<int32_t>(), B->n_h, B->tmp.as<int32_t>(),
+                               B->gtot.as<int32_t>(), s);
+        goff = B->goff.as<int32_t>();
+    }
+    pg::launch_box_setup(B->nbox, dim, B->hoff.as<int64_t>(), goff, B->gtot.as<int32_t>(), cap, B->cell_cap,
+                         B->soff.as<int64_t>(), B->N.as<int32_t>(), B->coff.as<int64_t>(), B->st(), s);
+    pg::launch_bseeds(dim, B->x.as<double>(), B->n_h, p.buff, B->periodic ? 1 : 0, B->hbox.as<int32_t>(),
+                      B->hoff.as<int64_t>(), goff, B->soff.as<int64_t>(), B->st(), B->spts.as<double>(),
+                      B->sbox.as<int32_t>(), B->orig_h.as<int32_t>(), s);
+    const int64_t* n_dev = B->n_dev();
+    pg::launch_bgrid(dim, B->spts.as<double>(), n_dev, cap, B->cell_cap, B->sbox.as<int32_t>(), B->N.as<int32_t>(),
+                     B->coff.as<int64_t>(), B->cell.as<int32_t>(), B->counts.as<int32_t>(), B->starts.as<int32_t>(),
+                     B->fill.as<int32_t>(), B->tmp.as<int32_t>(), B->total.as<int32_t>(), B->perm.as<uint32_t>(),
+                     B->sorted.as<double>(), B->sorted_box.as<int32_t>(), s);
+    auto knn = [&](const uint32_t* q, const int64_t* nq, int k) {
+        pg::launch_bknn(dim, B->sorted.as<double>(), B->perm.as<uint32_t>(), B->sorted_box.as<int32_t>(),
+                        B->N.as<int32_t>(), B->coff.as<int64_t>(), B->starts.as<int32_t>(), B->counts.as<int32_t>(),
+                        q, nq, cap, k, B->knn.as<uint32_t>(), B->wave_min_k, B->bpc, s);
+    };
+    auto clip = [&](const uint32_t* q, const int64_t* nq, int k, int big) {
+        pg::launch_bvoronoi(dim, B->sorted.as<double>(), q, nq, cap, B->knn.as<uint32_t>(), k, p.max_p, p.max_t, big,
+                            B->slots, B->faces(), B->bpc, s);
+    };
+    auto collect = [&](const uint32_t* L, const int64_t* nL, int code, uint32_t* out, int64_t* nout) {
+        pg::launch_bcollect(nL, cap, L, B->status.as<int32_t>(), code, out, B->flags.as<int32_t>(),
+                            B->offs.as<int32_t>(), B->tmp.as<int32_t>(), B->total.as<int32_t>(), nout, s);
+    };
+    // first pass: every seed at the fixed k, or every bucket of the learned
+    // k at its own k; then the restart chain of the single-box build
+    const std::vector<int>& sched = B->km ? B->ksched_learned : B->ksched;
+    int k = sched[0];
+    if (!B->km) {
+        knn(nullptr, n_dev, k);
+        clip(nullptr, n_dev, k, 0);
+    } else {
+        float* feat = B->feat.as<float>();
+        int32_t* cls = B->kcls.as<int32_t>();
+        pg::launch_bkfeatures(dim, B->sorted.as<double>(), n_dev, cap, B->sorted_box.as<int32_t>(),
+                              B->soff.as<int64_t>(), B->N.as<int32_t>(), B->coff.as<int64_t>(), B->counts.as<int32_t>(),
+                              B->starts.as<int32_t>(), feat, s);
+        pg::launch_kpredict(feat, cap, B->km->P.as<float>(), B->km->a0, nullptr, cls, nullptr, s, n_dev);
+        pg::launch_kbucket(cls, n_dev, cap, B->flags.as<int32_t>(), B->offs.as<int32_t>(), B->tmp.as<int32_t>(),
+                           B->total.as<int32_t>(), cap, B->klists.as<uint32_t>(), B->kcnt.as<int64_t>(), s);
+        for (int c = 0; c < pg::LK_NCLASS; c++) {
+            const uint32_t* lc = B->klists.as<uint32_t>() + (size_t)c * cap;
+            knn(lc, B->kcnt.as<int64_t>() + c, pg::lk_bucket(c));
+            clip(lc, B->kcnt.as<int64_t>() + c, pg::lk_bucket(c), 0);
+        }
+    }
+    uint32_t* lst[3] = {B->lists.as<uint32_t>(), B->lists.as<uint32_t>() + cap, B->lists.as<uint32_t>() + 2 * cap};
+    int64_t* cnt[3] = {B->ncount.as<int64_t>(), B->ncount.as<int64_t>() + 1, B->ncount.as<int64_t>() + 2};
+    const uint32_t* L = nullptr;
+    const int64_t* nL = n_dev;
+    for (size_t r = 1; r <= sched.size(); r++) {
+        collect(L, nL, pg::SLOT_OVERFLOW, lst[2], cnt[2]);
+        knn(lst[2], cnt[2], k);
+        clip(lst[2], cnt[2], k, 1);
+        if (r == sched.size() || !p.restart) break;
+        const int d = (L == lst[0]) ? 1 : 0;
+        collect(L, nL, PG_SECURITY_RADIUS_NOT_REACHED, lst[d], cnt[d]);
+        k = sched[r];
+        knn(lst[d], cnt[d], k);
+        clip(lst[d], cnt[d], k, 0);
+        L = lst[d];
+        nL = cnt[d];
+    }
+    const double sc = B->scale();
+    const double as = dim == 2 ? sc : sc * sc, vs = dim == 2 ? sc * sc : sc * sc * sc;
+    pg::launch_bfinalize(n_dev, cap, cap, B->perm.as<uint32_t>(), B->orig_h.as<int32_t>(), B->count.as<uint8_t>(),
+                         B->nbr.as<int32_t>(), B->area.as<double>(), B->vol.as<double>(), as, vs, B->periodic ? 1 : 0,
+                         B->hid.as<int32_t>(), B->area_u.as<double>(), B->vol_u.as<double>(), B->o2s.as<int32_t>(),
+                         B->hsorted.as<int32_t>(), B->st() + 1, s);
+    pg::launch_bcount_failed(n_dev, cap, B->status.as<int32_t>(), B->st() + 3, s);
+}
+
+// primitives, per-box CFL step, ALE HLL update, drift
+void batch_hydro(pg_batch* B, hipStream_t s) {
+    const int dim = B->dim;
+    const bool move = B->prm.move_mesh != 0;
+    pg::launch_prim_from_q(dim, B->view(), B->o2s.as<int32_t>(), B->vol_u.as<double>(), s);
+    pg::launch_bdt(dim, B->n_h, B->nbox, B->hbox.as<int32_t>(), B->o2s.as<int32_t>(), B->vol_u.as<double>(),
+                   B->prm.gamma, B->prm.cfl, B->rho.as<double>(), B->vel.as<double>(), B->E.as<double>(),
+                   B->dt_box.as<double>(), s);
+    pg::launch_bhll(dim, B->n_dev(), B->cap, B->cap, B->cap * MAXP, B->hsorted.as<int32_t>(), B->hbox.as<int32_t>(),
+                    B->count.as<uint8_t>(), B->hid.as<int32_t>(), B->nbr.as<int32_t>(), B->nrm.as<double>(),
+                    B->area_u.as<double>(), B->cen.as<double>(), B->sorted.as<double>(), B->scale(), B->prm.gamma,
+                    B->dt_box.as<double>(), B->prm.flux_mode, move ? 1 : 0, B->view(), s);
+    if (move)
+        pg::launch_bdrift(dim, B->n_h, B->hbox.as<int32_t>(), B->x.as<double>(), B->vel.as<double>(),
+                          B->dt_box.as<double>(), B->periodic ? 1 : 0, s);
+}
+
+void batch_check(pg_ctx* c, pg_batch* B) {
+    int32_t h[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(h, B->st(), sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (h[0]) throw Error{PG_ERR_CELLS, "batch: seeds or grid cells exceed the capacity reserved at creation"};
+    if (h[3]) throw Error{PG_ERR_CELLS, std::to_string(h[3]) + " cells left uncertified"};
+    if (h[1]) throw Error{PG_ERR_CELLS, "hydro cells without faces (uncertified cells)"};
+    if (h[2])
+        throw Error{PG_ERR_CELLS, std::to_string(h[2]) +
+                                      " hydro faces reach the box plane: the periodic ghost layer (buff) is "
+                                      "thinner than the cells; increase buff"};
+}
+
+void batch_step_eager(pg_ctx* c, pg_batch* B) {
+    batch_hydro(B, c->stream);
+    batch_rebuild(B, c->stream);
+    HIPCHK(hipGetLastError());
+}
+
+void batch_capture(pg_ctx* c, pg_batch* B) {
+    if (B->exec) {
+        HIPCHK(hipGraphExecDestroy(B->exec));
+        B->exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    try {
+        batch_hydro(B, c->stream);
+        batch_rebuild(B, c->stream);
+    } catch (...) {
+        // end the capture so that the stream is usable again
+        if (hipStreamEndCapture(c->stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        throw;
+    }
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        hipchk(e, "hipStreamEndCapture");
+    }
+    e = hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    hipchk(e, "hipGraphInstantiate");
+}
+
+void batch_step(pg_ctx* c, pg_batch* B) {
+    HIPCHK(hipEventRecord(B->e0, c->stream));
+    if (B->use_graph) {
+        if (!B->exec) batch_capture(c, B);
+        HIPCHK(hipGraphLaunch(B->exec, c->stream));
+    } else {
+        batch_step_eager(c, B);
+    }
+    HIPCHK(hipEventRecord(B->e1, c->stream));
+    batch_check(c, B);
+    HIPCHK(hipEventElapsedTime(&B->last_ms, B->e0, B->e1));
+}
+
+void batch_copy_state(pg_batch* B, bool save, hipStream_t s) {
+    const size_t n = (size_t)B->n_h, d = (size_t)B->dim;
+    auto cp = [&](DBuf& a, DBuf& b, size_t bytes) {
+        if (save) HIPCHK(hipMemcpyAsync(b.p, a.p, bytes, hipMemcpyDeviceToDevice, s));
+        else HIPCHK(hipMemcpyAsync(a.p, b.p, bytes, hipMemcpyDeviceToDevice, s));
+    };
+    cp(B->x, B->xs, sizeof(double) * n * d);
+    cp(B->mass, B->mass_s, sizeof(double) * n);
+    cp(B->mom, B->mom_s, sizeof(double) * n * d);
+    cp(B->en, B->en_s, sizeof(double) * n);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_batch_create(pg_ctx* c, const pg_sim_params* prm, int nbox, const int64_t* n_per_box, const double* pts,
+                    const double* rho, const double* vel, const double* E, pg_batch** out) {
+    return guarded([&] {
+        use(c);
+        argchk(prm && n_per_box && pts && rho && vel && E && out, "null argument");
+        const pg_mesh_params& mp = prm->mesh;
+        argchk(mp.dim == 2 || mp.dim == 3, "dim must be 2 or 3");
+        argchk(mp.boundary == PG_BOUNDARY_PERIODIC || mp.boundary == PG_BOUNDARY_REFLECTIVE,
+               "a batch needs periodic or reflective boxes");
+        argchk(mp.k >= 1 && mp.k <= 256 && mp.k_max <= 256, "k and k_max must be in [1, 256]");
+        argchk(mp.max_p >= 2 * mp.dim + 1 && mp.max_p <= MAXP, "max_p must be in [2*dim+1, 64]");
+        argchk(mp.max_t >= 8 && mp.max_t <= 128, "max_t must be in [8, 128]");
+        argchk(mp.boundary != PG_BOUNDARY_PERIODIC || (mp.buff > 0 && mp.buff < 0.5), "buff must be in (0, 0.5)");
+        argchk(prm->gamma > 1.0 && prm->cfl > 0.0, "gamma > 1 and cfl > 0 required");
+        argchk(prm->flux_mode == PG_FLUX_REFERENCE || prm->flux_mode == PG_FLUX_PHYSICAL, "unknown flux mode");
+        argchk(nbox >= 1 && nbox <= 4096, "nbox must be in [1, 4096]");
+        pg_batch* B = new pg_batch();
+        try {
+            B->prm = *prm;
+            if (B->prm.mesh.k_max < B->prm.mesh.k) B->prm.mesh.k_max = B->prm.mesh.k;
+            B->nbox = nbox;
+            B->dim = mp.dim;
+            B->periodic = mp.boundary == PG_BOUNDARY_PERIODIC;
+            const int kneed = std::max(B->prm.mesh.k, B->prm.mesh.k_max);
+            B->hoff_h.assign(nbox + 1, 0);
+            std::vector<int32_t> hb;
+            int64_t cap = 0, ccap = 0;
+            const double gfrac = B->periodic ? std::pow(1.0 + 2.0 * mp.buff, mp.dim) - 1.0 : 0.0;
+            for (int b = 0; b < nbox; b++) {
+                const int64_t nb = n_per_box[b];
+                argchk(nb >= kneed + 2 && nb < (int64_t)1 << 26, "every box needs more than k_max + 1 points");
+                B->hoff_h[b + 1] = B->hoff_h[b] + nb;
+                const int64_t cb = nb + (int64_t)std::ceil(2.0 * gfrac * (double)nb) + 64;
+                cap += cb;
+                const int64_t Nb = ngrid(mp.dim, cb) + 1;
+                ccap += mp.dim == 2 ? Nb * Nb : Nb * Nb * Nb;
+                if (!B->periodic) argchk(ngrid(mp.dim, nb) >= 1, "box too small");
+            }
+            argchk(cap < (int64_t)1 << 30, "batch too large");
+            B->n_h = B->hoff_h[nbox];
+            B->cap = cap;
+            B->cell_cap = ccap;
+            hb.resize(B->n_h);
+            for (int b = 0; b < nbox; b++)
+                for (int64_t i = B->hoff_h[b]; i < B->hoff_h[b + 1]; i++) hb[i] = b;
+            // restart schedule of the single-box build (first k, then next_k)
+            B->ksched.push_back(B->prm.mesh.k);
+            if (B->prm.mesh.restart)
+                for (int kk = next_k(B->prm.mesh.k, B->prm.mesh.k_max); kk > 0; kk = next_k(kk, B->prm.mesh.k_max))
+                    B->ksched.push_back(kk);
+            // learned k: after the bucket pass every rejected seed continues
+            // from 64 (a recomputation at any row length at least its
+            // certified k gives the same cell)
+            B->ksched_learned.push_back(std::min(64, B->prm.mesh.k_max));
+            if (B->prm.mesh.restart)
+                for (int kk = next_k(B->ksched_learned[0], B->prm.mesh.k_max); kk > 0;
+                     kk = next_k(kk, B->prm.mesh.k_max))
+                    B->ksched_learned.push_back(kk);
+            const int kmax = std::max(96, *std::max_element(B->ksched.begin(), B->ksched.end()));
+            const size_t n = (size_t)B->n_h, d = (size_t)mp.dim, cs = (size_t)cap;
+            hipStream_t s = c->stream;
+            HIPCHK(hipMemcpyAsync(B->hoff.get<int64_t>(nbox + 1), B->hoff_h.data(), sizeof(int64_t) * (nbox + 1),
+                                  hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(B->hbox.get<int32_t>(n), hb.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(B->x.get<double>(n * d), pts, sizeof(double) * n * d, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(B->rho.get<double>(n), rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(B->vel.get<double>(n * d), vel, sizeof(double) * n * d, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(B->E.get<double>(n), E, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            B->xs.get<double>(n * d);
+            B->mass.get<double>(n);
+            B->mom.get<double>(n * d);
+            B->en.get<double>(n);
+            B->mass_s.get<double>(n);
+            B->mom_s.get<double>(n * d);
+            B->en_s.get<double>(n);
+            B->dt_box.get<double>(nbox);
+            B->gcnt.get<int32_t>(n);
+            B->goff.get<int32_t>(n);
+            B->gtot.get<int32_t>(1);
+            B->soff.get<int64_t>(nbox + 1);
+            B->N.get<int32_t>(nbox);
+            B->coff.get<int64_t>(nbox + 1);
+            B->stat.get<int32_t>(4);
+            B->spts.get<double>(cs * d);
+            B->sbox.get<int32_t>(cs);
+            B->orig_h.get<int32_t>(cs);
+            B->cell.get<int32_t>(cs);
+            B->counts.get<int32_t>(ccap);
+            B->starts.get<int32_t>(ccap);
+            B->fill.get<int32_t>(ccap);
+            B->tmp.get<int32_t>(pg::scan_tmp_elems(
+                std::max<int64_t>({ccap, cap, B->n_h, pg::lk_bucket_blocks(cap) * pg::LK_NCLASS})));
+            B->total.get<int32_t>(1);
+            B->perm.get<uint32_t>(cs);
+            B->sorted.get<double>(cs * d);
+            B->sorted_box.get<int32_t>(cs);
+            B->knn.get<uint32_t>(cs * kmax);
+            B->lists.get<uint32_t>(3 * cs);
+            B->ncount.get<int64_t>(3);
+            B->flags.get<int32_t>(cs);
+            B->offs.get<int32_t>(cs);
+            B->nbr.get<int32_t>(cs * MAXP);
+            B->area.get<double>(cs * MAXP);
+            B->nrm.get<double>(3 * cs * MAXP);
+            B->cen.get<double>(3 * cs * MAXP);
+            B->count.get<uint8_t>(cs);
+            B->vol.get<double>(cs);
+            B->ccen.get<double>(3 * cs);
+            B->status.get<int32_t>(cs);
+            B->kcert.get<int32_t>(cs);
+            B->hid.get<int32_t>(cs * MAXP);
+            B->area_u.get<double>(cs * MAXP);
+            B->vol_u.get<double>(cs);
+            B->o2s.get<int32_t>(n);
+            B->hsorted.get<int32_t>(cs);
+            B->tot.get<double>(5 * (size_t)nbox);
+            B->feat.get<float>(cs * pg::LK_NFEAT);
+            B->kcls.get<int32_t>(cs);
+            B->klists.get<uint32_t>(cs * pg::LK_NCLASS);
+            B->kcnt.get<int64_t>(pg::LK_NCLASS);
+            HIPCHK(hipEventCreate(&B->e0));
+            HIPCHK(hipEventCreate(&B->e1));
+            B->slots = pg::clip_slots();
+            batch_rebuild(B, s);
+            HIPCHK(hipGetLastError());
+            batch_check(c, B);
+            pg::launch_q_from_prim(mp.dim, B->view(), B->o2s.as<int32_t>(), B->vol_u.as<double>(), s);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(s));
+        } catch (...) {
+            delete B;
+            throw;
+        }
+        *out = B;
+    });
+}
+
+int pg_batch_step(pg_ctx* c, pg_batch* B, double* ms) {
+    return guarded([&] {
+        use(c);
+        argchk(B != nullptr, "null batch");
+        batch_step(c, B);
+        if (ms) *ms = B->last_ms;
+    });
+}
+
+int pg_batch_set_kmodel(pg_batch* B, const pg_kmodel* km) {
+    return guarded([&] {
+        argchk(B != nullptr, "null batch");
+        if (B->km != km && B->exec) {
+            (void)hipGraphExecDestroy(B->exec);
+            B->exec = nullptr;
+        }
+        B->km = km;
+    });
+}
+
+int pg_batch_set_tuning(pg_batch* B, const pg_batch_tuning* t) {
+    return guarded([&] {
+        argchk(B && t, "null argument");
+        argchk(t->clip_slots == 48 || t->clip_slots == 64, "clip_slots must be 48 or 64");
+        argchk(t->knn_wave_min_k >= 1 && t->blocks_per_cu >= 1 && t->blocks_per_cu <= 64, "bad tuning");
+        const bool changed = B->slots != t->clip_slots || B->wave_min_k != t->knn_wave_min_k ||
+                             B->bpc != t->blocks_per_cu || B->use_graph != (t->use_graph != 0);
+        B->slots = t->clip_slots;
+        B->wave_min_k = t->knn_wave_min_k;
+        B->bpc = t->blocks_per_cu;
+        B->use_graph = t->use_graph != 0;
+        if (changed && B->exec) {
+            (void)hipGraphExecDestroy(B->exec);
+            B->exec = nullptr;
+        }
+    });
+}
+
+int pg_batch_get_tuning(const pg_batch* B, pg_batch_tuning* t) {
+    return guarded([&] {
+        argchk(B && t, "null argument");
+        t->clip_slots = B->slots;
+        t->knn_wave_min_k = B->wave_min_k;
+        t->blocks_per_cu = B->bpc;
+        t->use_graph = B->use_graph ? 1 : 0;
+    });
+}
+
+int pg_batch_autotune(pg_ctx* c, pg_batch* B, int reps, pg_batch_tuning* best, double* best_ms) {
+    return guarded([&] {
+        use(c);
+        argchk(B != nullptr && reps >= 1 && reps <= 16, "bad argument");
+        hipStream_t s = c->stream;
+        batch_copy_state(B, true, s);
+        const pg_batch_tuning start{B->slots, B->wave_min_k, B->bpc, B->use_graph ? 1 : 0};
+        pg_batch_tuning winner = start;
+        double wms = 1e300;
+        const int slot_c[2] = {64, 48};
+        const int wave_c[2] = {33, 1};
+        const int bpc_c[3] = {4, 8, 16};
+        for (int a = 0; a < 2; a++)
+            for (int w = 0; w < 2; w++)
+                for (int q = 0; q < 3; q++) {
+                    const pg_batch_tuning t{slot_c[a], wave_c[w], bpc_c[q], 1};
+                    if (pg_batch_set_tuning(B, &t) != PG_OK) throw Error{PG_ERR_ARG, g_err};
+                    double tot = 0;
+                    for (int r = 0; r < reps; r++) {
+                        batch_copy_state(B, false, s);
+                        batch_rebuild(B, s);  // the mesh of the saved state
+                        batch_step(c, B);
+                        tot += B->last_ms;
+                    }
+                    if (tot / reps < wms) {
+                        wms = tot / reps;
+                        winner = t;
+                    }
+                }
+        if (pg_batch_set_tuning(B, &winner) != PG_OK) throw Error{PG_ERR_ARG, g_err};
+        batch_copy_state(B, false, s);
+        batch_rebuild(B, s);
+        HIPCHK(hipGetLastError());
+        batch_check(c, B);
+        if (best) *best = winner;
+        if (best_ms) *best_ms = wms;
+    });
+}
+
+int pg_batch_get(pg_ctx* c, pg_batch* B, double* pts, double* rho, double* vel, double* E) {
+    return guarded([&] {
+        use(c);
+        argchk(B != nullptr, "null batch");
+        hipStream_t s = c->stream;
+        const size_t n = (size_t)B->n_h, d = (size_t)B->dim;
+        pg::launch_prim_from_q(B->dim, B->view(), B->o2s.as<int32_t>(), B->vol_u.as<double>(), s);
+        HIPCHK(hipGetLastError());
+        if (pts) HIPCHK(hipMemcpyAsync(pts, B->x.p, sizeof(double) * n * d, hipMemcpyDeviceToHost, s));
+        if (rho) HIPCHK(hipMemcpyAsync(rho, B->rho.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        if (vel) HIPCHK(hipMemcpyAsync(vel, B->vel.p, sizeof(double) * n * d, hipMemcpyDeviceToHost, s));
+        if (E) HIPCHK(hipMemcpyAsync(E, B->E.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+int pg_batch_dt(pg_ctx* c, const pg_batch* B, double* dt) {
+    return guarded([&] {
+        use(c);
+        argchk(B && dt, "null argument");
+        HIPCHK(hipMemcpyAsync(dt, B->dt_box.p, sizeof(double) * B->nbox, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pg_batch_totals(pg_ctx* c, pg_batch* B, double* tot) {
+    return guarded([&] {
+        use(c);
+        argchk(B && tot, "null argument");
+        pg::launch_btotals(B->dim, B->nbox, B->hoff.as<int64_t>(), B->view(), B->tot.as<double>(), c->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(tot, B->tot.p, sizeof(double) * 5 * B->nbox, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pg_batch_info(pg_ctx* c, const pg_batch* B, pg_batch_info_t* info) {
+    return guarded([&] {
+        use(c);
+        argchk(B && info, "null argument");
+        int64_t ns = 0;
+        HIPCHK(hipMemcpyAsync(&ns, B->soff.as<int64_t>() + B->nbox, sizeof(int64_t), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        info->nbox = B->nbox;
+        info->n_hydro = B->n_h;
+        info->n_seeds = ns;
+        info->seed_capacity = B->cap;
+        info->last_step_ms = B->last_ms;
+    });
+}
+
+int pg_batch_free(pg_batch** B) {
+    return guarded([&] {
+        if (!B || !*B) return;
+        delete *B;
+        *B = nullptr;
     });
 }
 

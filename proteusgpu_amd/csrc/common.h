@@ -68,6 +68,17 @@ PG_DEV int grid_cell(const double* p, int N) {
     return i + j * N + k * N * N;
 }
 
+// knn.cpp:22: N_grid = round(pow(n / 3.1f, 1.0f / dim)), at least 1.  The
+// power is evaluated in FP64 and rounded to float (the correctly rounded
+// float power), identically on host and device, so that a box's grid can be
+// sized on the GPU exactly as the host sizes it.
+__host__ __device__ inline int ngrid_of(int dim, int64_t n) {
+    const float x = (float)n / 3.1f;
+    const float p = (float)pow((double)x, (double)(1.0f / (float)dim));
+    const int N = (int)roundf(p);
+    return N < 1 ? 1 : N;
+}
+
 // knn.h:64-75
 template <int DIM>
 PG_DEV double dist2(const double* a, const double* b) {

@@ -60,4 +60,20 @@ void launch_cells_orig(int64_t n, int64_t stride, int dim, const uint32_t* perm,
                        double* o_seeds, double* o_vol, int32_t* o_status, int32_t* o_kcert,
                        double* o_ccen, hipStream_t s);
 
+// batched boxes (batch.hip)
+void launch_bfinalize(const int64_t* n_dev, int64_t cap, int64_t stride, const uint32_t* perm, const int32_t* orig_h,
+                      const uint8_t* count, const int32_t* nbr, const double* area, const double* vol, double ascale,
+                      double vscale, int periodic, int32_t* hid, double* area_u, double* vol_u, int32_t* o2s,
+                      int32_t* hsorted, int32_t* bad, hipStream_t s);
+void launch_bdt(int dim, int64_t n_h, int nbox, const int32_t* hbox, const int32_t* o2s, const double* vol_u,
+                double gam, double cfl, const double* rho, const double* vel, const double* E, double* dt_box,
+                hipStream_t s);
+void launch_bhll(int dim, const int64_t* n_dev, int64_t cap, int64_t stride, int64_t nplane, const int32_t* hsorted,
+                 const int32_t* hbox, const uint8_t* count, const int32_t* hid, const int32_t* nbr, const double* nrm,
+                 const double* area_u, const double* cen, const double* sorted, double scale, double gam,
+                 const double* dt_box, int flux_mode, int move, SimView sv, hipStream_t s);
+void launch_bdrift(int dim, int64_t n_h, const int32_t* hbox, double* x, const double* vel, const double* dt_box,
+                   int periodic, hipStream_t s);
+void launch_btotals(int dim, int nbox, const int64_t* hoff, SimView sv, double* out, hipStream_t s);
+
 }  // namespace pg

@@ -8,6 +8,7 @@
 // consecutive sorted cells read consecutive addresses for every face slot.
 #include "common.h"
 #include "hydro.h"
+#include "hydro_core.h"
 
 namespace pg {
 namespace {
@@ -47,117 +48,6 @@ __global__ void __launch_bounds__(TB) k_finalize(int64_t n, int64_t n_hydro, int
         hid[at] = h;
         area_u[at] = ascale * area[at];
     }
-}
-
-struct Prim {
-    double rho, v[3], E;
-};
-
-template <int DIM>
-PG_DEV double pressure(double gam, const Prim& s) {
-    if (DIM == 2) return (gam - 1) * (s.E - (0.5 * s.rho * (s.v[0] * s.v[0] + s.v[1] * s.v[1])));
-    return (gam - 1) *
-           (s.E - (0.5 * s.rho * (s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2])));
-}
-
-// riemann.cpp:108-130
-struct Geom {
-    double n[3], m[3], p[3];
-};
-PG_DEV Geom geom_of(double n0, double n1, double n2) {
-    Geom g;
-    const double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
-    g.n[0] = n0 / nn;
-    g.n[1] = n1 / nn;
-    g.n[2] = n2 / nn;
-    if (g.n[0] != 0.0 || g.n[1] != 0.0) {
-        g.m[0] = -g.n[1];
-        g.m[1] = g.n[0];
-        g.m[2] = 0.0;
-    } else {
-        g.m[0] = 1.0;
-        g.m[1] = 0.0;
-        g.m[2] = 0.0;
-    }
-    const double mm = sqrt(g.m[0] * g.m[0] + g.m[1] * g.m[1] + g.m[2] * g.m[2]);
-    g.m[0] = g.m[0] / mm;
-    g.m[1] = g.m[1] / mm;
-    g.m[2] = g.m[2] / mm;
-    g.p[0] = g.n[1] * g.m[2] - g.n[2] * g.m[1];
-    g.p[1] = g.n[2] * g.m[0] - g.n[0] * g.m[2];
-    g.p[2] = g.n[0] * g.m[1] - g.n[1] * g.m[0];
-    return g;
-}
-template <int DIM>
-PG_DEV void to_face(Prim& s, const Geom& g) {
-    const double vx = s.v[0], vy = s.v[1];
-    if (DIM == 2) {
-        s.v[0] = vx * g.n[0] + vy * g.n[1];
-        s.v[1] = vx * g.m[0] + vy * g.m[1];
-    } else {
-        const double vz = s.v[2];
-        s.v[0] = vx * g.n[0] + vy * g.n[1] + vz * g.n[2];
-        s.v[1] = vx * g.m[0] + vy * g.m[1] + vz * g.m[2];
-        s.v[2] = vx * g.p[0] + vy * g.p[1] + vz * g.p[2];
-    }
-}
-template <int DIM>
-PG_DEV void from_face(Prim& s, const Geom& g) {
-    const double vx = s.v[0], vy = s.v[1];
-    if (DIM == 2) {
-        s.v[0] = vx * g.n[0] + vy * g.m[0];
-        s.v[1] = vx * g.n[1] + vy * g.m[1];
-    } else {
-        const double vz = s.v[2];
-        s.v[0] = vx * g.n[0] + vy * g.m[0] + vz * g.p[0];
-        s.v[1] = vx * g.n[1] + vy * g.m[1] + vz * g.p[1];
-        s.v[2] = vx * g.n[2] + vy * g.m[2] + vz * g.p[2];
-    }
-}
-template <int DIM>
-PG_DEV Prim euler_flux(double gam, const Prim& s) {
-    const double P = pressure<DIM>(gam, s);
-    Prim f;
-    f.rho = s.rho * s.v[0];
-    f.v[0] = s.rho * s.v[0] * s.v[0] + P;
-    f.v[1] = s.rho * s.v[0] * s.v[1];
-    f.v[2] = DIM == 3 ? s.rho * s.v[0] * s.v[2] : 0.0;
-    f.E = (s.E + P) * s.v[0];
-    return f;
-}
-
-// riemann.cpp:7-45 with the reference's operation order.  wall: the right
-// state is the left one mirrored in the face frame, so SL == -SR exactly and
-// the mass flux through the wall is exactly zero.
-template <int DIM>
-PG_DEV Prim hll(double gam, Prim L, Prim R, const Geom& g, bool wall) {
-    to_face<DIM>(L, g);
-    if (wall) {
-        R = L;
-        R.v[0] = -L.v[0];
-    } else {
-        to_face<DIM>(R, g);
-    }
-    const Prim fl = euler_flux<DIM>(gam, L), fr = euler_flux<DIM>(gam, R);
-    const double cl = sqrt((gam * pressure<DIM>(gam, L)) / L.rho);
-    const double cr = sqrt((gam * pressure<DIM>(gam, R)) / R.rho);
-    const double SL = smin(L.v[0] - cl, R.v[0] - cr);
-    const double SR = smax(L.v[0] + cl, R.v[0] + cr);
-    Prim F = {0.0, {0.0, 0.0, 0.0}, 0.0};
-    if (SL >= 0) {
-        F = fl;
-    } else if (SL < 0 && SR > 0) {
-        F.rho = (SR * fl.rho - SL * fr.rho + SL * SR * (R.rho - L.rho)) / (SR - SL);
-#pragma unroll
-        for (int d = 0; d < DIM; d++)
-            F.v[d] = (SR * fl.v[d] - SL * fr.v[d] + SL * SR * (R.rho * R.v[d] - L.rho * L.v[d])) /
-                     (SR - SL);
-        F.E = (SR * fl.E - SL * fr.E + SL * SR * (R.E - L.E)) / (SR - SL);
-    } else if (SR <= 0) {
-        F = fr;
-    }
-    from_face<DIM>(F, g);
-    return F;
 }
 
 // One static-mesh HLL update per hydro cell (finite_volume_solver.cpp:53-106):
@@ -301,82 +191,9 @@ __global__ void __launch_bounds__(TB)
     if (i >= n_hydro) return;
     double dt = *dt_dev;
     if (dt_max > 0 && dt > dt_max) dt = dt_max;
-    Prim si;
-    si.rho = sv.rho[i];
-    si.E = sv.E[i];
-    si.v[2] = 0.0;
-#pragma unroll
-    for (int d = 0; d < DIM; d++) si.v[d] = sv.vel[(size_t)DIM * i + d];
-    double ri[3] = {0, 0, 0};
-#pragma unroll
-    for (int d = 0; d < DIM; d++) ri[d] = (sorted[(size_t)DIM * s + d] - 0.5) * scale + 0.5;
-    double fm = 0.0, fp[3] = {0, 0, 0}, fe = 0.0;
-    const int c = count[s];
-    const double sgn = flux_mode == 1 ? -1.0 : 1.0;
-    for (int f = 0; f < c; f++) {
-        const int64_t at = (int64_t)f * stride + s;
-        const int32_t j = hid[at];
-        const bool wall = j < 0;
-        Prim sj = si;
-        double w[3] = {0.0, 0.0, 0.0};
-        if (!wall) {
-            sj.rho = sv.rho[j];
-            sj.E = sv.E[j];
-#pragma unroll
-            for (int d = 0; d < DIM; d++) sj.v[d] = sv.vel[(size_t)DIM * j + d];
-            if (move) {
-                const int32_t js = nbr[at];
-                double rj[3] = {0, 0, 0}, dr[3], mid[3];
-                double dd = 0.0, proj = 0.0;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    rj[d] = (sorted[(size_t)DIM * js + d] - 0.5) * scale + 0.5;
-                    dr[d] = rj[d] - ri[d];
-                    mid[d] = 0.5 * (ri[d] + rj[d]);
-                    dd += dr[d] * dr[d];
-                    const double fc = (cen[(int64_t)d * nplane + at] - 0.5) * scale + 0.5;
-                    proj += (si.v[d] - sj.v[d]) * (fc - mid[d]);
-                }
-#pragma unroll
-                for (int d = 0; d < DIM; d++) w[d] = 0.5 * (si.v[d] + sj.v[d]) + proj * dr[d] / dd;
-            }
-        }
-        // total energy per volume is frame dependent: boost it with the velocity
-        Prim L = si, R = sj;
-        if (move && !wall) {
-            double ke_l = 0, ke_r = 0, ke_l2 = 0, ke_r2 = 0;
-#pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                ke_l += L.v[d] * L.v[d];
-                ke_r += R.v[d] * R.v[d];
-                L.v[d] -= w[d];
-                R.v[d] -= w[d];
-                ke_l2 += L.v[d] * L.v[d];
-                ke_r2 += R.v[d] * R.v[d];
-            }
-            L.E += 0.5 * L.rho * (ke_l2 - ke_l);
-            R.E += 0.5 * R.rho * (ke_r2 - ke_r);
-        }
-        const Geom g = geom_of(sgn * nrm[at], sgn * nrm[nplane + at], sgn * nrm[2 * nplane + at]);
-        const Prim F = hll<DIM>(gam, L, R, g, wall);
-        double Fm = F.rho, Fp[3] = {F.v[0], F.v[1], F.v[2]}, Fe = F.E;
-        if (move && !wall) {
-            double wp = 0.0, ww = 0.0;
-#pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                wp += Fp[d] * w[d];
-                ww += w[d] * w[d];
-            }
-            Fe += wp + 0.5 * ww * Fm;
-#pragma unroll
-            for (int d = 0; d < DIM; d++) Fp[d] += Fm * w[d];
-        }
-        const double a = area_u[at];
-        fm += Fm * a;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) fp[d] += Fp[d] * a;
-        fe += Fe * a;
-    }
+    double fm, fp[3], fe;
+    ale_cell_flux<DIM>(s, i, stride, nplane, count, hid, nbr, nrm, area_u, cen, sorted, scale, gam, flux_mode,
+                       move, sv, fm, fp, fe);
     sv.mass[i] -= dt * fm;
 #pragma unroll
     for (int d = 0; d < DIM; d++) sv.mom[(size_t)DIM * i + d] -= dt * fp[d];

@@ -65,6 +65,9 @@ void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* se
                     const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa,
                     hipStream_t s, bool restart = false);
 
+// Vertex slots per seed of the fast clip kernel (48 or 64).
+void set_clip_slots(int slots);
+int clip_slots();
 // Collect the sorted ids whose status is not SUCCESS (order: ascending id).
 // nfail (device int32) receives the count; tmp as for scan.
 void launch_collect_failed(int64_t n, const int32_t* status, uint32_t* list, int32_t* flags,
@@ -91,11 +94,16 @@ int lk_bucket(int c);  // 16, 24, 32, 48, 64, 96
 //   f14     |sum of count * offset| / (3^dim * mean occupancy)
 void launch_kfeatures(int dim, const double* sorted, int64_t n, int N, const int32_t* counts,
                       const int32_t* starts, const uint32_t* out_index, float* out, hipStream_t s);
-// cls[s] = bucket with the least expected cost of its restart chain
-// (k, 2k, 4k, ...; a pass costs a0 + k); logits
+// cls[s] = bucket with the least expected cost of its restart chain (a pass
+// at row length k costs k + r m, m = clip steps, r = a0); logits
 // (n x 6, optional) receives the network outputs.
+// n_dev (optional): seed count on the device, n then is the capacity.
 void launch_kpredict(const float* X, int64_t n, const float* params, float a0, const uint32_t* out_index,
-                     int32_t* cls, float* logits, hipStream_t s);
+                     int32_t* cls, float* logits, hipStream_t s, const int64_t* n_dev = nullptr);
+// Features of batched boxes (sorted seeds of every box, box-major).
+void launch_bkfeatures(int dim, const double* sorted, const int64_t* n_dev, int64_t cap, const int32_t* sbox,
+                       const int64_t* soff, const int32_t* N, const int64_t* coff, const int32_t* counts,
+                       const int32_t* starts, float* out, hipStream_t s);
 int64_t lk_grad_blocks(int64_t B);
 // Forward + backward of a batch of B rows (seed != 0: rows drawn from n by a
 // hash of (seed, step, i)); part: lk_grad_blocks(B) x LK_NPARAM partial sums.
@@ -104,8 +112,42 @@ void launch_kgrad(const float* X, const int32_t* y, int64_t n, int64_t B, uint64
 // Reduce the partials in block order; Adam step when lr > 0.
 void launch_kadam(const float* part, const float* part_loss, int nblk, float* params, float* m, float* v,
                   float* grad_out, float* loss_out, int64_t step, float lr, hipStream_t s);
+// Stable split of seeds 0..n-1 (n = *n_dev when n_dev != nullptr, else
+// n_cap) by class: class c into lists + c * list_cap (ascending seed id),
+// its size into counts[c] (device int64).  cnt, off: NC * lk_bucket_blocks
+// int32; tmp as for scan of that size.
+int64_t lk_bucket_blocks(int64_t n_cap);
+void launch_kbucket(const int32_t* cls, const int64_t* n_dev, int64_t n_cap, int32_t* cnt, int32_t* off,
+                    int32_t* tmp, int32_t* total, int64_t list_cap, uint32_t* lists, int64_t* counts,
+                    hipStream_t s);
 // Training label (bucket index) of every seed from its certified k.
 void launch_klabels(const int32_t* kcert, int64_t n, int32_t* y, hipStream_t s);
+// -------------------------------------------------------------- batch.hip
+// Batched boxes, box-major arrays; counts that change per step are device
+// scalars (n_dev, nq_dev) and launches depend only on capacities (graph
+// capturable).  See batch.hip.
+void launch_box_setup(int nbox, int dim, const int64_t* hoff, const int32_t* goff, const int32_t* gtot,
+                      int64_t seed_cap, int64_t cell_cap, int64_t* soff, int32_t* N, int64_t* coff, int32_t* err,
+                      hipStream_t s);
+void launch_bseeds(int dim, const double* pts, int64_t n_h, double buff, int periodic, const int32_t* hbox,
+                   const int64_t* hoff, const int32_t* goff, const int64_t* soff, const int32_t* err, double* out,
+                   int32_t* sbox, int32_t* orig_h, hipStream_t s);
+void launch_bgrid(int dim, const double* pts, const int64_t* n_dev, int64_t cap, int64_t cell_cap,
+                  const int32_t* sbox, const int32_t* N, const int64_t* coff, int32_t* cell, int32_t* counts,
+                  int32_t* starts, int32_t* fill, int32_t* tmp, int32_t* total, uint32_t* perm, double* sorted,
+                  int32_t* sorted_box, hipStream_t s);
+void launch_bknn(int dim, const double* sorted, const uint32_t* perm, const int32_t* sbox, const int32_t* N,
+                 const int64_t* coff, const int32_t* starts, const int32_t* counts, const uint32_t* queries,
+                 const int64_t* nq_dev, int64_t cap, int k, uint32_t* out, int wave_min_k, int blocks_per_cu,
+                 hipStream_t s);
+void launch_bvoronoi(int dim, const double* sorted, const uint32_t* seeds, const int64_t* nq_dev, int64_t cap,
+                     const uint32_t* knn, int k, int max_p, int max_t, int big, int slots, FaceArrays fa,
+                     int blocks_per_cu, hipStream_t s);
+void launch_bcollect(const int64_t* nq_dev, int64_t cap, const uint32_t* seeds, const int32_t* status, int code,
+                     uint32_t* list, int32_t* flags, int32_t* offs, int32_t* tmp, int32_t* total, int64_t* nout,
+                     hipStream_t s);
+void launch_bcount_failed(const int64_t* n_dev, int64_t cap, const int32_t* status, int32_t* nfail, hipStream_t s);
+void launch_fill_i32(int32_t* a, int64_t n, int32_t v, hipStream_t s);
 // ---------------------------------------------------------------- probe.hip
 // Streaming device copy of `bytes` (multiple of 32).
 void launch_stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);

@@ -1,9 +1,136 @@
-// Wave-per-query k-nearest-neighbour kernel (device code; launched from knn.hip).
+// k-nearest-neighbour search of one query point (device code shared by the
+// single-box and batched kernels of knn.hip and batch.hip).
 #pragma once
 
 #include "common.h"
 
 namespace pg {
+
+// One query of the per-thread heap search; the heap lives in LDS at
+// hd/hi[entry * BLOCK + threadIdx.x].  Grid cells of this query's box are
+// cbase + i + j N + k N^2.
+template <int DIM, int BLOCK>
+PG_DEV void knn_heap_query(const double* __restrict__ sorted, const uint32_t* __restrict__ perm, int N,
+                           int64_t cbase, const int32_t* __restrict__ starts,
+                           const int32_t* __restrict__ counts, uint32_t q, int k, uint32_t* __restrict__ o,
+                           double* hd, uint32_t* hi) {
+    const int t = threadIdx.x;
+    double p[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; d++) p[d] = sorted[(size_t)DIM * q + d];
+    const int c = grid_cell<DIM>(p, N);
+    const int ci = c % N, cj = (c / N) % N, ck = DIM == 3 ? c / (N * N) : 0;
+
+#define HD(e) hd[(size_t)(e) * BLOCK + t]
+#define HI(e) hi[(size_t)(e) * BLOCK + t]
+    auto less = [&](double da, uint32_t ia, double db, uint32_t ib) -> bool {
+        return da < db || (da == db && perm[ia] < perm[ib]);
+    };
+    int size = 0;
+    auto push = [&](double d, uint32_t id) {
+        if (size < k) {
+            int j = size++;
+            while (j > 0) {
+                int par = (j - 1) >> 1;
+                double pd = HD(par);
+                uint32_t pi = HI(par);
+                if (!less(pd, pi, d, id)) break;
+                HD(j) = pd;
+                HI(j) = pi;
+                j = par;
+            }
+            HD(j) = d;
+            HI(j) = id;
+        } else if (less(d, id, HD(0), HI(0))) {
+            int j = 0;
+            for (;;) {
+                int l = 2 * j + 1, r = l + 1, m = -1;
+                double md = d;
+                uint32_t mi = id;
+                if (l < k && less(md, mi, HD(l), HI(l))) {
+                    m = l;
+                    md = HD(l);
+                    mi = HI(l);
+                }
+                if (r < k && less(md, mi, HD(r), HI(r))) {
+                    m = r;
+                    md = HD(r);
+                    mi = HI(r);
+                }
+                if (m < 0) break;
+                HD(j) = md;
+                HI(j) = mi;
+                j = m;
+            }
+            HD(j) = d;
+            HI(j) = id;
+        }
+    };
+    auto scan_cell = [&](int ii, int jj, int kk) {
+        const int64_t cc = cbase + ii + jj * N + (int64_t)kk * N * N;
+        const int32_t b = starts[cc], e = b + counts[cc];
+        for (int32_t s = b; s < e; s++) {
+            if ((uint32_t)s == q) continue;
+            double pc[3] = {0, 0, 0};
+#pragma unroll
+            for (int d = 0; d < DIM; d++) pc[d] = sorted[(size_t)DIM * s + d];
+            push(dist2<DIM>(p, pc), (uint32_t)s);
+        }
+    };
+    for (int ring = 0; ring < N; ring++) {
+        if (ring >= 1 && size == k) {
+            double md = (double)(ring - 1) / (double)N;
+            if (HD(0) < md * md) break;
+        }
+        const int klo = DIM == 3 ? max(ck - ring, 0) : 0, khi = DIM == 3 ? min(ck + ring, N - 1) : 0;
+        const int jlo = max(cj - ring, 0), jhi = min(cj + ring, N - 1);
+        for (int kk = klo; kk <= khi; kk++) {
+            const bool kedge = DIM == 3 && (kk == ck - ring || kk == ck + ring);
+            for (int jj = jlo; jj <= jhi; jj++) {
+                const bool jedge = (jj == cj - ring || jj == cj + ring);
+                if (kedge || jedge || ring == 0) {
+                    const int ilo = max(ci - ring, 0), ihi = min(ci + ring, N - 1);
+                    for (int ii = ilo; ii <= ihi; ii++) scan_cell(ii, jj, kk);
+                } else {
+                    if (ci - ring >= 0) scan_cell(ci - ring, jj, kk);
+                    if (ci + ring < N) scan_cell(ci + ring, jj, kk);
+                }
+            }
+        }
+    }
+    // heapsort to ascending order (knn.cpp:420-426)
+    for (int end = size - 1; end > 0; end--) {
+        double d = HD(end);
+        uint32_t id = HI(end);
+        HD(end) = HD(0);
+        HI(end) = HI(0);
+        int j = 0;
+        for (;;) {
+            int l = 2 * j + 1, r = l + 1, m = -1;
+            double md = d;
+            uint32_t mi = id;
+            if (l < end && less(md, mi, HD(l), HI(l))) {
+                m = l;
+                md = HD(l);
+                mi = HI(l);
+            }
+            if (r < end && less(md, mi, HD(r), HI(r))) {
+                m = r;
+                md = HD(r);
+                mi = HI(r);
+            }
+            if (m < 0) break;
+            HD(j) = md;
+            HI(j) = mi;
+            j = m;
+        }
+        HD(j) = d;
+        HI(j) = id;
+    }
+    for (int e = 0; e < k; e++) o[e] = e < size ? HI(e) : 0xffffffffu;
+#undef HD
+#undef HI
+}
 
 // ------------------------------------------------------------------------
 // Wave-per-query variant.  The 64 lanes of a wave cooperate on one query:
@@ -25,16 +152,15 @@ PG_DEV bool cand_less(double da, uint32_t oa, double db, uint32_t ob) {
     return da < db || (da == db && oa < ob);
 }
 
+// One query by the 64 lanes of a one-wave block; c_start / c_pfx are the
+// block's LDS tables (64 and 65 entries).  Grid cells of the query's box are
+// cbase + i + j N + k N^2.
 template <int DIM, int M>
-__global__ void __launch_bounds__(64)
-    k_knn_wave(const double* __restrict__ sorted, const uint32_t* __restrict__ perm, int N,
-               const int32_t* __restrict__ starts, const int32_t* __restrict__ counts,
-               const uint32_t* __restrict__ queries, int64_t nq, int k, uint32_t* __restrict__ out) {
-    __shared__ int32_t c_start[64], c_pfx[65];
+PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __restrict__ perm, int N,
+                           int64_t cbase, const int32_t* __restrict__ starts,
+                           const int32_t* __restrict__ counts, uint32_t q, int k, uint32_t* __restrict__ o,
+                           int32_t* c_start, int32_t* c_pfx) {
     const int l = threadIdx.x;
-    const int64_t qi = blockIdx.x;
-    if (qi >= nq) return;  // whole block
-    const uint32_t q = queries ? queries[qi] : (uint32_t)qi;
     double p[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < DIM; d++) p[d] = sorted[(size_t)DIM * q + d];
@@ -73,7 +199,7 @@ __global__ void __launch_bounds__(64)
                 const int cheb = max(abs(di), max(abs(dj), abs(dk)));
                 const int ii = ci + di, jj = cj + dj, kk = ck + dk;
                 if (cheb == ring && ii >= 0 && jj >= 0 && kk >= 0 && ii < N && jj < N && (DIM == 2 || kk < N)) {
-                    const int cc = ii + jj * N + kk * N * N;
+                    const int64_t cc = cbase + ii + jj * N + (int64_t)kk * N * N;
                     st = starts[cc];
                     cnt = counts[cc];
                 }
@@ -183,13 +309,11 @@ __global__ void __launch_bounds__(64)
             __syncthreads();
         }
     }
-    uint32_t* o = out + (size_t)qi * k;
 #pragma unroll
     for (int jj = 0; jj < M; jj++) {
         const int r = jj * 64 + l;
         if (r < k) o[r] = r < size ? ks[jj] : 0xffffffffu;
     }
 }
-
 
 }  // namespace pg

@@ -30,6 +30,22 @@ constexpr int OW1 = 0, OB1 = OW1 + NF * NH, OW2 = OB1 + NH, OB2 = OW2 + NH * NH,
 static_assert(OB3 + NC == LK_NPARAM, "parameter layout");
 
 __constant__ int c_buckets[NC] = {16, 24, 32, 48, 64, 96};
+// representative certified k of each bucket's interval (midpoints; 80 for
+// the open last bucket)
+__constant__ float c_krep[NC] = {12.f, 20.f, 28.f, 40.f, 56.f, 80.f};
+
+// Cost, in KNN row entries, of a seed with certified k = K started at row
+// length k0: a pass costs k (row) + r m (clip steps; m = K once certified,
+// else k); a rejected pass repeats at 2k (at most 256).
+PG_DEV float chain_cost(float k0, float K, float r) {
+    float k = k0, cost = 0.f;
+    for (int stage = 0; stage < 6; stage++) {
+        if (K <= k) return cost + k + r * K;
+        cost += k + r * k;
+        k = fminf(2.f * k, 256.f);
+    }
+    return cost;
+}
 
 PG_DEV short f2bf(float x) {
     uint32_t u = __float_as_uint(x);
@@ -43,14 +59,11 @@ PG_DEV f4 mfma(bf4 a, bf4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf
 PG_DEV f4 zero4() { return f4{0.f, 0.f, 0.f, 0.f}; }
 
 // ------------------------------------------------------------ features
-// One thread per seed (sorted space); see the feature list in kernels.h.
+// Features of sorted seed s of a box of n seeds with grid N whose cells are
+// cbase + i + j N + k N^2; see the feature list in kernels.h.
 template <int DIM>
-__global__ void __launch_bounds__(256)
-    k_kfeatures(const double* __restrict__ sorted, int64_t n, int N, const int32_t* __restrict__ cnt,
-                const int32_t* __restrict__ start, const uint32_t* __restrict__ out_index,
-                float* __restrict__ out) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
+PG_DEV void kfeatures_one(const double* __restrict__ sorted, int64_t s, int64_t n, int N, int64_t cbase,
+                          const int32_t* __restrict__ cnt, const int32_t* __restrict__ start, float (&f)[16]) {
     double p[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < DIM; d++) p[d] = sorted[(size_t)DIM * s + d];
@@ -71,7 +84,7 @@ __global__ void __launch_bounds__(256)
             for (int di = -1; di <= 1; di++) {
                 const int ii = ci + di, jj = cj + dj, kk = ck + dk;
                 if (ii < 0 || jj < 0 || kk < 0 || ii >= N || jj >= N || kk >= N) continue;
-                const int cc = ii + jj * N + kk * N * N;
+                const int64_t cc = cbase + ii + jj * N + (int64_t)kk * N * N;
                 const int m = (di != 0) + (dj != 0) + (dk != 0);
                 const int ct = cnt[cc];
                 inside++;
@@ -103,7 +116,6 @@ __global__ void __launch_bounds__(256)
                 }
             }
     const int cube = DIM == 3 ? 27 : 9;
-    float f[16];
 #pragma unroll
     for (int m = 0; m < 4; m++) f[m] = cin[m] ? (float)(csum[m] / cin[m] / lam) : 0.0f;
     f[4] = (float)((double)inside / cube);
@@ -131,10 +143,40 @@ __global__ void __launch_bounds__(256)
     f[13] = nmax >= 0 ? (float)((double)(nmax - nmin) / lam) : 0.0f;
     f[14] = (float)(sqrt(gx * gx + gy * gy + gz * gz) / (cube * lam));
     f[15] = (float)(r8 / r4);
-    const int64_t o = out_index ? (int64_t)out_index[s] : s;
+}
+
+PG_DEV void store_features(float* out, int64_t o, const float (&f)[16]) {
     float4* dst = (float4*)(out + 16 * o);
 #pragma unroll
     for (int v = 0; v < 4; v++) dst[v] = make_float4(f[4 * v], f[4 * v + 1], f[4 * v + 2], f[4 * v + 3]);
+}
+
+// One thread per seed (sorted space).
+template <int DIM>
+__global__ void __launch_bounds__(256)
+    k_kfeatures(const double* __restrict__ sorted, int64_t n, int N, const int32_t* __restrict__ cnt,
+                const int32_t* __restrict__ start, const uint32_t* __restrict__ out_index,
+                float* __restrict__ out) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    float f[16];
+    kfeatures_one<DIM>(sorted, s, n, N, 0, cnt, start, f);
+    store_features(out, out_index ? (int64_t)out_index[s] : s, f);
+}
+
+// Batched boxes: the box of each sorted seed gives its grid and seed count.
+template <int DIM>
+__global__ void __launch_bounds__(256)
+    k_bkfeatures(const double* __restrict__ sorted, const int64_t* __restrict__ n_dev, const int32_t* __restrict__ sbox,
+                 const int64_t* __restrict__ soff, const int32_t* __restrict__ N, const int64_t* __restrict__ coff,
+                 const int32_t* __restrict__ cnt, const int32_t* __restrict__ start, float* __restrict__ out) {
+    const int64_t n = *n_dev;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x) {
+        const int b = sbox[s];
+        float f[16];
+        kfeatures_one<DIM>(sorted, s, soff[b + 1] - soff[b], N[b], coff[b], cnt, start, f);
+        store_features(out, s, f);
+    }
 }
 
 // ------------------------------------------------------------ weights
@@ -242,13 +284,13 @@ PG_DEV f4 softmax16(f4 z, int q) {
 
 // ------------------------------------------------------------ inference
 // grid-stride over groups of 16 seeds, one wave per group.  cls[s] = the
-// bucket whose restart chain (k_b, 2 k_b, 4 k_b, ...; a pass at k costs
-// a0 + k, stage j runs with probability P(kcert > k_{j-1})) has the least
-// expected cost under the softmax of the logits.
+// bucket with the least expected restart-chain cost (chain_cost, r = a0)
+// under the softmax of the logits.
 __global__ void __launch_bounds__(256)
-    k_kpredict(const float* __restrict__ X, int64_t n, const float* __restrict__ P, float a0,
-               const uint32_t* __restrict__ out_index, int32_t* __restrict__ cls,
-               float* __restrict__ logits) {
+    k_kpredict(const float* __restrict__ X, int64_t n_host, const int64_t* __restrict__ n_dev,
+               const float* __restrict__ P, float a0, const uint32_t* __restrict__ out_index,
+               int32_t* __restrict__ cls, float* __restrict__ logits) {
+    const int64_t n = n_dev ? *n_dev : n_host;
     const int l = threadIdx.x & 63, q = l >> 4, r = l & 15;
     FwdWeights w;
     load_fwd(P, q, r, w);
@@ -273,23 +315,9 @@ __global__ void __launch_bounds__(256)
             int bi = 0;
 #pragma unroll
             for (int b = 0; b < NC; b++) {
-                // restart chain from bucket b: k, 2k, 4k, ... ; stage j runs
-                // with probability P(kcert > k_{j-1})
-                float cost = 0.f, surv = 1.f, k = (float)c_buckets[b];
-                for (int stage = 0; stage < 8; stage++) {
-                    cost += surv * (a0 + k);
-                    float s = 0.f;
-                    bool any = false;
+                float cost = 0.f;
 #pragma unroll
-                    for (int c = 0; c < NC; c++)
-                        if ((float)c_buckets[c] > k) {
-                            s += pc[c];
-                            any = true;
-                        }
-                    if (!any) break;
-                    surv = s;
-                    k *= 2.f;
-                }
+                for (int c = 0; c < NC; c++) cost += pc[c] * chain_cost((float)c_buckets[b], c_krep[c], a0);
                 if (cost < best) {
                     best = cost;
                     bi = b;
@@ -529,7 +557,84 @@ __global__ void k_klabels(const int32_t* __restrict__ kcert, int64_t n, int32_t*
     y[s] = c;
 }
 
+// ------------------------------------------------------------ bucketing
+// Stable split of seeds 0..n-1 by class into per-class lists (class c at
+// lists + c * list_cap, ascending seed id).  Pass 1: per block of 256 seeds
+// the count of each class, stored class-major (cnt[c * nblk + blk]); an
+// exclusive scan of that array; pass 2: each seed's rank among the seeds of
+// its class before it.  n from n_dev when given (capacity-shaped launch).
+constexpr int BTB = 256;
+
+PG_DEV int64_t seed_count(const int64_t* n_dev, int64_t n) { return n_dev ? *n_dev : n; }
+
+__global__ void __launch_bounds__(BTB)
+    k_bucket_hist(const int32_t* __restrict__ cls, const int64_t* __restrict__ n_dev, int64_t n_host, int nblk,
+                  int32_t* __restrict__ cnt) {
+    __shared__ int32_t wc[BTB / 64][NC];
+    const int64_t n = seed_count(n_dev, n_host);
+    const int64_t i = (int64_t)blockIdx.x * BTB + threadIdx.x;
+    const int c = i < n ? cls[i] : -1;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        const uint64_t m = (uint64_t)__ballot(c == k);
+        if (l == 0) wc[w][k] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < NC) {
+        int t = 0;
+#pragma unroll
+        for (int ww = 0; ww < BTB / 64; ww++) t += wc[ww][threadIdx.x];
+        cnt[threadIdx.x * nblk + blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(BTB)
+    k_bucket_scatter(const int32_t* __restrict__ cls, const int64_t* __restrict__ n_dev, int64_t n_host, int nblk,
+                     const int32_t* __restrict__ off, int64_t list_cap, uint32_t* __restrict__ lists) {
+    __shared__ int32_t wc[BTB / 64][NC];
+    const int64_t n = seed_count(n_dev, n_host);
+    const int64_t i = (int64_t)blockIdx.x * BTB + threadIdx.x;
+    const int c = i < n ? cls[i] : -1;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    int rank = 0;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        const uint64_t m = (uint64_t)__ballot(c == k);
+        if (l == 0) wc[w][k] = __popcll(m);
+        if (c == k) rank = __popcll(m & ((1ull << l) - 1ull));
+    }
+    __syncthreads();
+    if (c < 0) return;
+    for (int ww = 0; ww < w; ww++) rank += wc[ww][c];
+    const int64_t pos = (int64_t)off[c * nblk + blockIdx.x] - off[c * nblk] + rank;
+    lists[(int64_t)c * list_cap + pos] = (uint32_t)i;
+}
+
+// counts[c] = seeds of class c (int64, device)
+__global__ void k_bucket_counts(const int32_t* __restrict__ off, const int32_t* __restrict__ total, int nblk,
+                                int64_t* __restrict__ counts) {
+    const int c = threadIdx.x;
+    if (c >= NC) return;
+    const int64_t a = off[c * nblk];
+    const int64_t b = c + 1 < NC ? (int64_t)off[(c + 1) * nblk] : (int64_t)*total;
+    counts[c] = b - a;
+}
+
 }  // namespace
+
+int64_t lk_bucket_blocks(int64_t n_cap) { return (n_cap + BTB - 1) / BTB; }
+
+void launch_kbucket(const int32_t* cls, const int64_t* n_dev, int64_t n_cap, int32_t* cnt, int32_t* off,
+                    int32_t* tmp, int32_t* total, int64_t list_cap, uint32_t* lists, int64_t* counts,
+                    hipStream_t s) {
+    const int nblk = (int)lk_bucket_blocks(n_cap);
+    if (nblk <= 0) return;
+    k_bucket_hist<<<nblk, BTB, 0, s>>>(cls, n_dev, n_cap, nblk, cnt);
+    scan_exclusive_i32(cnt, off, (int64_t)NC * nblk, tmp, total, s);
+    k_bucket_scatter<<<nblk, BTB, 0, s>>>(cls, n_dev, n_cap, nblk, off, list_cap, lists);
+    k_bucket_counts<<<1, 64, 0, s>>>(off, total, nblk, counts);
+}
 
 int lk_bucket(int c) {
     static const int b[LK_NCLASS] = {16, 24, 32, 48, 64, 96};
@@ -545,11 +650,19 @@ void launch_kfeatures(int dim, const double* sorted, int64_t n, int N, const int
 }
 
 void launch_kpredict(const float* X, int64_t n, const float* params, float a0, const uint32_t* out_index,
-                     int32_t* cls, float* logits, hipStream_t s) {
+                     int32_t* cls, float* logits, hipStream_t s, const int64_t* n_dev) {
     if (n <= 0) return;
     const int64_t groups = (n + 15) / 16;
     const int64_t blocks = std::min<int64_t>((groups + 3) / 4, 4096);
-    k_kpredict<<<(unsigned)blocks, 256, 0, s>>>(X, n, params, a0, out_index, cls, logits);
+    k_kpredict<<<(unsigned)blocks, 256, 0, s>>>(X, n, n_dev, params, a0, out_index, cls, logits);
+}
+
+void launch_bkfeatures(int dim, const double* sorted, const int64_t* n_dev, int64_t cap, const int32_t* sbox,
+                       const int64_t* soff, const int32_t* N, const int64_t* coff, const int32_t* counts,
+                       const int32_t* starts, float* out, hipStream_t s) {
+    const unsigned nb = (unsigned)std::min<int64_t>(std::max<int64_t>((cap + 255) / 256, 1), 4096);
+    if (dim == 2) k_bkfeatures<2><<<nb, 256, 0, s>>>(sorted, n_dev, sbox, soff, N, coff, counts, starts, out);
+    else k_bkfeatures<3><<<nb, 256, 0, s>>>(sorted, n_dev, sbox, soff, N, coff, counts, starts, out);
 }
 
 int64_t lk_grad_blocks(int64_t B) { return (B + 16 * GROUPS_PER_BLOCK - 1) / (16 * GROUPS_PER_BLOCK); }

@@ -1,4 +1,6 @@
 // Kernel entry points for the Voronoi clip and the restart bookkeeping.
+#include <cstdlib>
+
 #include "voronoi_kernel.h"
 
 namespace pg {
@@ -14,7 +16,7 @@ __global__ void __launch_bounds__(64) k_voronoi(const double* __restrict__ pts, 
                                                 int max_t, FaceArrays fa) {
     using C = VoroCfg<DIM, G, S>;
     __shared__ SeedSmem<C::NS, C::MAXP> sm[C::SPW];
-    voronoi_group<DIM, G, S>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm);
+    voronoi_group<DIM, G, S>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm, blockIdx.x);
 }
 
 // The same kernel under its own name for certificate restarts, so that
@@ -26,7 +28,7 @@ __global__ void __launch_bounds__(64) k_voronoi_restart(const double* __restrict
                                                         int max_t, FaceArrays fa) {
     using C = VoroCfg<DIM, G, S>;
     __shared__ SeedSmem<C::NS, C::MAXP> sm[C::SPW];
-    voronoi_group<DIM, G, S>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm);
+    voronoi_group<DIM, G, S>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm, blockIdx.x);
 }
 
 template <int DIM, int G, int S>
@@ -63,6 +65,16 @@ __global__ void __launch_bounds__(TB) k_compact(int64_t nq, const uint32_t* seed
 
 }  // namespace
 
+// Vertex slots of the fast kernel per seed: 16 lanes x 4 (64 slots) or
+// 16 lanes x 3 (48 slots; less work per clip, more seeds overflow into the
+// 128-slot kernel).  Chosen by the autotuner or PGPU_CLIP_SLOTS.
+static int g_clip_slots = [] {
+    const char* e = getenv("PGPU_CLIP_SLOTS");
+    return (e && atoi(e) == 48) ? 3 : 4;
+}();
+void set_clip_slots(int slots) { g_clip_slots = slots == 48 ? 3 : 4; }
+int clip_slots() { return g_clip_slots * 16; }
+
 void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq,
                     const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa,
                     hipStream_t s, bool restart) {
@@ -70,11 +82,15 @@ void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* se
     if (dim == 2) {
         if (big)
             run<2, 16, 8>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, true, s);
+        else if (g_clip_slots == 3)
+            run<2, 16, 3>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
         else
             run<2, 16, 4>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
     } else {
         if (big)
             run<3, 16, 8>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, true, s);
+        else if (g_clip_slots == 3)
+            run<3, 16, 3>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
         else
             run<3, 16, 4>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
     }

@@ -194,7 +194,8 @@ template <int DIM, int G, int S>
 __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, const uint32_t* __restrict__ seeds,
                               int64_t nq, const uint32_t* __restrict__ knn, int k, int max_p,
                               int max_t, FaceArrays fa,
-                              SeedSmem<VoroCfg<DIM, G, S>::NS, VoroCfg<DIM, G, S>::MAXP>* smem) {
+                              SeedSmem<VoroCfg<DIM, G, S>::NS, VoroCfg<DIM, G, S>::MAXP>* smem,
+                              int64_t vblock) {
     using C = VoroCfg<DIM, G, S>;
     constexpr int NS = C::NS, NW = C::NW, MAXP = C::MAXP;
     constexpr int D = DIM;
@@ -202,7 +203,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
     const int grp = tid / G, gl = tid % G;
     const int shift = grp * G;
     const uint64_t gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
-    const int64_t qi = (int64_t)blockIdx.x * C::SPW + grp;
+    const int64_t qi = vblock * C::SPW + grp;
     const bool valid = qi < nq;
     SeedSmem<NS, MAXP>& sm = smem[grp];
     if (max_p > MAXP) max_p = MAXP;

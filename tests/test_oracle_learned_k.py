@@ -87,11 +87,14 @@ def test_labels_and_decision():
     for c in range(6):
         z = np.full((1, 6), -big)
         z[0, c] = big
-        assert L.decide(z, 32.0)[0] == c  # a certain class is chosen as is
+        for r in (0.5, 2.8, 10.0):
+            assert L.decide(z, r)[0] == c  # a certain class is chosen as is
     # uncertain between 32 and 64 at even odds: the restart penalty favours 64
     z = np.full((1, 6), -big)
     z[0, 2] = z[0, 4] = 0.0
-    assert L.decide(z, 32.0)[0] == 4
+    assert L.decide(z, 2.8)[0] == 4
+    # chain costs by hand: start 16, certified k 56 -> passes 16, 32 fail, 64 passes
+    assert L.chain_cost(16, 56, 2.0) == (16 + 32) + (32 + 64) + (64 + 112)
 
 
 def test_adam_first_step_is_signed_lr():
