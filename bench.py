@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--probe-bytes", type=int, default=1 << 30,
                     help="size of the streaming-copy probe (measured HBM bandwidth)")
     ap.add_argument("--train-steps", type=int, default=400, help="Adam steps of the k classifier")
+    ap.add_argument("--only", choices=["plain", "learned", "batched"], default=None,
+                    help="run one variant only (for profiling): the fixed-k host step, the learned-k host "
+                         "step, or the batched fixed- and learned-k steps")
     return ap.parse_args()
 
 
@@ -127,6 +130,38 @@ def run_batch(ctx, B, warmup, steps):
     return (time.perf_counter() - t0) * 1e3 / steps, float(np.mean(gpu_ms))
 
 
+def profile_variant(a, ctx, api, pts, rho, vel, E, kw):
+    """One variant alone, so that a profiler sees only its kernels."""
+    out = {"variant": a.only, "n": a.n, "steps": a.steps}
+    km = None
+    if a.only in ("learned", "batched"):
+        km = api.KModel(ctx, seed=1)
+        t = api.Simulation(ctx, pts, rho, vel, E, **kw)
+        out["train_loss"] = km.train_mesh(t, steps=a.train_steps, batch=4096, lr=3e-3)
+        t.free()
+    if a.only in ("plain", "learned"):
+        sim = api.Simulation(ctx, pts, rho, vel, E, **kw)
+        if km is not None:
+            sim.set_kmodel(km)
+        ms, st = run_steps(ctx, sim, a.warmup, a.steps)
+        out.update(ms_per_step=round(ms, 3), stages_ms={k: round(v, 3) for k, v in st.items() if k.endswith("_ms")},
+                   mesh=sim.mesh_info())
+        sim.free()
+    else:
+        for name, model in (("batched_fixed_k", None), ("batched_learned_k", km)):
+            B = api.Batch(ctx, np.array([a.n]), pts, rho, vel, E, **kw)
+            if model is not None:
+                B.set_kmodel(model)
+            ms, gpu = run_batch(ctx, B, a.warmup, a.steps)
+            out[name] = {"ms_per_step": round(ms, 3), "gpu_ms_per_step": round(gpu, 3)}
+            B.free()
+    out["measured_copy_GBs"] = round(ctx.stream_copy_probe(a.probe_bytes, 10), 1)
+    print(json.dumps(out))
+    if km is not None:
+        km.free()
+    ctx.close()
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -140,6 +175,9 @@ def main():
     rho, vel, E = sod_state(pts, gamma)
     ctx = api.Context(0)
     kw = dict(boundary="reflective", k=a.k, gamma=gamma, cfl=cfl, flux="physical", move_mesh=True)
+
+    if a.only:
+        return profile_variant(a, ctx, api, pts, rho, vel, E, kw)
 
     # 1. the plain method: fixed k=32 + certificate restarts, host-driven step
     plain = api.Simulation(ctx, pts, rho, vel, E, **kw)

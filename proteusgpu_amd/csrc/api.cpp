@@ -102,7 +102,7 @@ struct Grid {
     int64_t n = 0;
     int N = 0;
     int64_t ncell = 0;
-    DBuf pts, sorted, perm, cell, counts, starts, fill, tmp, total;
+    DBuf pts, sorted, perm, cell, counts, starts, fill, tmp, total, big;
 
     void build(hipStream_t s) {
         ncell = (int64_t)N * N * (dim == 3 ? N : 1);
@@ -116,7 +116,8 @@ struct Grid {
         pg::scan_exclusive_i32(cnt, st, ncell, tp, total.get<int32_t>(1), s);
         uint32_t* pm = perm.get<uint32_t>(n);
         pg::launch_cell_scatter(n, cell.as<int32_t>(), st, fl, pm, s);
-        pg::launch_cell_sort(ncell, st, cnt, pm, s);
+        int32_t* bg = big.get<int32_t>(ncell + 1);
+        pg::launch_cell_sort(ncell, st, cnt, pm, bg + 1, bg, s);
         pg::launch_gather_points(dim, n, pm, pts.as<double>(), sorted.get<double>((size_t)n * dim), s);
         HIPCHK(hipGetLastError());
     }
@@ -428,9 +429,19 @@ void mesh_finalize(pg_ctx* c, pg_mesh* m) {
                         m->area_u.get<double>(ns * MAXP), m->vol_u.get<double>(ns),
                         m->o2s.get<int32_t>(std::max<int64_t>(m->n_hydro, 1)), bad, s);
     HIPCHK(hipGetLastError());
-    int32_t hb[2] = {0, 0};
+    // faces over all cells (VMesh.num_faces), read with the error counters:
+    // meshes built on the device (simulations) learn their face count here
+    int32_t* co = m->flags.get<int32_t>(ns);
+    int32_t* po = m->offs.get<int32_t>(ns);
+    int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(std::max<int64_t>(m->n_seeds, m->g.ncell)));
+    pg::launch_counts_orig(m->n_seeds, m->g.perm.as<uint32_t>(), m->count.as<uint8_t>(), co, s);
+    pg::scan_exclusive_i32(co, po, m->n_seeds, tp, m->cnt1.get<int32_t>(1), s);
+    HIPCHK(hipGetLastError());
+    int32_t hb[2] = {0, 0}, nf = 0;
     HIPCHK(hipMemcpyAsync(hb, bad, sizeof(hb), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&nf, m->cnt1.as<int32_t>(), sizeof(nf), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    m->info.num_faces = nf;
     if (hb[0] != 0) throw Error{PG_ERR_CELLS, "hydro cells without faces (uncertified cells)"};
     if (hb[1] != 0)
         throw Error{PG_ERR_CELLS, std::to_string(hb[1]) +
@@ -679,6 +690,14 @@ int pg_mesh_copy_out(pg_ctx* c, const pg_mesh* mc, pg_vmesh_view* o) {
         int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(std::max<int64_t>(n, m->g.ncell)));
         pg::launch_counts_orig(n, m->g.perm.as<uint32_t>(), m->count.as<uint8_t>(), co, s);
         pg::scan_exclusive_i32(co, po, n, tp, m->cnt1.get<int32_t>(1), s);
+        // the face arrays below are sized by the caller from pg_mesh_info:
+        // refuse a mesh whose face count differs from the one reported
+        int32_t nf = 0;
+        HIPCHK(hipMemcpyAsync(&nf, m->cnt1.as<int32_t>(), sizeof(nf), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if ((int64_t)nf != F)
+            throw Error{PG_ERR_ARG, "mesh has " + std::to_string(nf) + " faces, pg_mesh_info reported " +
+                                        std::to_string(F) + "; query pg_mesh_info again"};
         DBuf onbr, onrm, oarea, ocen, oseeds, ovol, ost, okc, occ;
         int32_t* dnbr = onbr.get<int32_t>(F);
         double* dnrm = onrm.get<double>(3 * F);
@@ -1281,7 +1300,7 @@ struct pg_batch {
     DBuf mass_s, mom_s, en_s;  // autotuner save area
     // per step
     DBuf gcnt, goff, gtot, soff, N, coff, stat, spts, sbox, orig_h;
-    DBuf cell, counts, starts, fill, tmp, total, perm, sorted, sorted_box;
+    DBuf cell, counts, starts, fill, tmp, total, perm, sorted, sorted_box, big;
     DBuf knn, lists, ncount, flags, offs;
     DBuf nbr, area, nrm, cen, count, vol, ccen, status, kcert;
     DBuf hid, area_u, vol_u, o2s, hsorted, tot;
@@ -1343,7 +1362,7 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
     pg::launch_bgrid(dim, B->spts.as<double>(), n_dev, cap, B->cell_cap, B->sbox.as<int32_t>(), B->N.as<int32_t>(),
                      B->coff.as<int64_t>(), B->cell.as<int32_t>(), B->counts.as<int32_t>(), B->starts.as<int32_t>(),
                      B->fill.as<int32_t>(), B->tmp.as<int32_t>(), B->total.as<int32_t>(), B->perm.as<uint32_t>(),
-                     B->sorted.as<double>(), B->sorted_box.as<int32_t>(), s);
+                     B->sorted.as<double>(), B->sorted_box.as<int32_t>(), B->big.as<int32_t>(), s);
     auto knn = [&](const uint32_t* q, const int64_t* nq, int k) {
         pg::launch_bknn(dim, B->sorted.as<double>(), B->perm.as<uint32_t>(), B->sorted_box.as<int32_t>(),
                         B->N.as<int32_t>(), B->coff.as<int64_t>(), B->starts.as<int32_t>(), B->counts.as<int32_t>(),
@@ -1586,6 +1605,7 @@ int pg_batch_create(pg_ctx* c, const pg_sim_params* prm, int nbox, const int64_t
             B->counts.get<int32_t>(ccap);
             B->starts.get<int32_t>(ccap);
             B->fill.get<int32_t>(ccap);
+            B->big.get<int32_t>(ccap + 1);
             B->tmp.get<int32_t>(pg::scan_tmp_elems(
                 std::max<int64_t>({ccap, cap, B->n_h, pg::lk_bucket_blocks(cap) * pg::LK_NCLASS})));
             B->total.get<int32_t>(1);

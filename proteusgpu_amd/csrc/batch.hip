@@ -417,7 +417,7 @@ void launch_bseeds(int dim, const double* pts, int64_t n_h, double buff, int per
 void launch_bgrid(int dim, const double* pts, const int64_t* n_dev, int64_t cap, int64_t cell_cap,
                   const int32_t* sbox, const int32_t* N, const int64_t* coff, int32_t* cell, int32_t* counts,
                   int32_t* starts, int32_t* fill, int32_t* tmp, int32_t* total, uint32_t* perm, double* sorted,
-                  int32_t* sorted_box, hipStream_t s) {
+                  int32_t* sorted_box, int32_t* big, hipStream_t s) {
     const unsigned gb = std::min<unsigned>(nblk(cap), 2048);
     const unsigned gc = std::min<unsigned>(nblk(cell_cap), 2048);
     k_fill<int32_t><<<gc, TB, 0, s>>>(counts, cell_cap, 0);
@@ -426,7 +426,7 @@ void launch_bgrid(int dim, const double* pts, const int64_t* n_dev, int64_t cap,
     else k_bcell_count<3><<<gb, TB, 0, s>>>(pts, n_dev, cap, sbox, N, coff, cell, counts);
     scan_exclusive_i32(counts, starts, cell_cap, tmp, total, s);
     k_bcell_scatter<<<gb, TB, 0, s>>>(n_dev, cap, cell, starts, fill, perm);
-    launch_cell_sort(cell_cap, starts, counts, perm, s);
+    launch_cell_sort(cell_cap, starts, counts, perm, big + 1, big, s);
     if (dim == 2) k_bgather<2><<<gb, TB, 0, s>>>(n_dev, cap, perm, pts, sbox, sorted, sorted_box);
     else k_bgather<3><<<gb, TB, 0, s>>>(n_dev, cap, perm, pts, sbox, sorted, sorted_box);
 }

@@ -10,6 +10,8 @@
 // and within a range the points are ordered by input index -- exactly the
 // order of the reference's serial store loop.  The scatter uses atomics and
 // the per-cell sort restores that order deterministically.
+#include <algorithm>
+
 #include "common.h"
 #include "ghosts.h"
 #include "kernels.h"
@@ -85,39 +87,69 @@ __global__ void __launch_bounds__(TB) k_cell_scatter(int64_t n, const int32_t* c
     idx[pos] = (uint32_t)i;
 }
 
-// Insertion sort of each cell's index range.  Uniform inputs hold ~3 points
-// per cell; the register buffer covers ranges up to 32 and longer ranges
-// are sorted in place in global memory.
+// Sort each cell's index range ascending.  Uniform inputs hold ~3 points
+// per cell: ranges up to 32 are insertion-sorted in registers by one
+// thread; longer ranges (clustered inputs) are listed and sorted by a whole
+// block (k_cell_sort_big).
 __global__ void __launch_bounds__(TB) k_cell_sort(int64_t ncell, const int32_t* starts,
-                                                  const int32_t* counts, uint32_t* idx) {
+                                                  const int32_t* counts, uint32_t* idx, int32_t* big,
+                                                  int32_t* nbig) {
     int64_t c = (int64_t)blockIdx.x * TB + threadIdx.x;
     if (c >= ncell) return;
     int len = counts[c];
     if (len < 2) return;
-    uint32_t* a = idx + starts[c];
-    if (len <= 32) {
-        uint32_t v[32];
-        for (int i = 0; i < len; i++) v[i] = a[i];
-        for (int i = 1; i < len; i++) {
-            uint32_t x = v[i];
-            int j = i - 1;
-            while (j >= 0 && v[j] > x) {
-                v[j + 1] = v[j];
-                j--;
-            }
-            v[j + 1] = x;
-        }
-        for (int i = 0; i < len; i++) a[i] = v[i];
+    if (len > 32) {
+        big[atomicAdd(nbig, 1)] = (int32_t)c;
         return;
     }
+    uint32_t* a = idx + starts[c];
+    uint32_t v[32];
+    for (int i = 0; i < len; i++) v[i] = a[i];
     for (int i = 1; i < len; i++) {
-        uint32_t x = a[i];
+        uint32_t x = v[i];
         int j = i - 1;
-        while (j >= 0 && a[j] > x) {
-            a[j + 1] = a[j];
+        while (j >= 0 && v[j] > x) {
+            v[j + 1] = v[j];
             j--;
         }
-        a[j + 1] = x;
+        v[j + 1] = x;
+    }
+    for (int i = 0; i < len; i++) a[i] = v[i];
+}
+
+// One block per listed cell (grid-stride over the list): bitonic sort of
+// the range in place, in the form whose comparators all point the same way
+// (the first step of every merge compares mirror positions), so a range of
+// any length sorts as if padded with +inf: comparators whose partner lies
+// beyond the range are skipped.  The values are distinct point indices.
+__global__ void __launch_bounds__(TB) k_cell_sort_big(const int32_t* __restrict__ big, const int32_t* __restrict__ nbig,
+                                                      const int32_t* __restrict__ starts,
+                                                      const int32_t* __restrict__ counts, uint32_t* idx) {
+    const int nb = *nbig;
+    for (int bi = blockIdx.x; bi < nb; bi += gridDim.x) {
+        const int c = big[bi];
+        const int64_t len = counts[c];
+        uint32_t* a = idx + starts[c];
+        int64_t P = 1;
+        while (P < len) P <<= 1;
+        for (int64_t k = 2; k <= P; k <<= 1) {
+            for (int64_t j = k >> 1; j > 0; j >>= 1) {
+                for (int64_t t = threadIdx.x; t < P / 2; t += TB) {
+                    // comparator t of this substep: position i in the lower half
+                    // of its group of 2j, partner l
+                    const int64_t i = (t / j) * 2 * j + (t % j);
+                    const int64_t l = (j == (k >> 1)) ? (i / k) * k + (k - 1 - (i % k)) : i + j;
+                    if (l < len && i < len) {
+                        const uint32_t x = a[i], y = a[l];
+                        if (x > y) {
+                            a[i] = y;
+                            a[l] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
     }
 }
 
@@ -132,6 +164,10 @@ __global__ void __launch_bounds__(TB) k_gather_points(int64_t n, const uint32_t*
 }
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + TB - 1) / TB); }
+
+__global__ void k_fill_one(int32_t* p) {
+    if (threadIdx.x == 0) *p = 0;
+}
 
 }  // namespace
 
@@ -169,9 +205,11 @@ void launch_cell_scatter(int64_t n, const int32_t* cell, const int32_t* starts, 
 }
 
 void launch_cell_sort(int64_t ncell, const int32_t* starts, const int32_t* counts, uint32_t* idx,
-                      hipStream_t s) {
+                      int32_t* big, int32_t* nbig, hipStream_t s) {
     if (ncell <= 0) return;
-    k_cell_sort<<<nblk(ncell), TB, 0, s>>>(ncell, starts, counts, idx);
+    k_fill_one<<<1, 64, 0, s>>>(nbig);
+    k_cell_sort<<<nblk(ncell), TB, 0, s>>>(ncell, starts, counts, idx, big, nbig);
+    k_cell_sort_big<<<(unsigned)std::min<int64_t>(ncell, 1024), TB, 0, s>>>(big, nbig, starts, counts, idx);
 }
 
 void launch_gather_points(int dim, int64_t n, const uint32_t* perm, const double* pts,

@@ -28,9 +28,10 @@ void launch_cell_count(int dim, const double* pts, int64_t n, int N, int32_t* ce
 void launch_cell_scatter(int64_t n, const int32_t* cell, const int32_t* starts, int32_t* fill,
                          uint32_t* idx, hipStream_t s);
 // Sort every cell's range ascending: the order of the reference's serial
-// store loop (knn.cpp:187-204), i.e. stable by input index.
+// store loop (knn.cpp:187-204), i.e. stable by input index.  big (ncell
+// int32) and nbig (one int32) are scratch for ranges longer than 32.
 void launch_cell_sort(int64_t ncell, const int32_t* starts, const int32_t* counts, uint32_t* idx,
-                      hipStream_t s);
+                      int32_t* big, int32_t* nbig, hipStream_t s);
 // sorted[s] = pts[perm[s]]
 void launch_gather_points(int dim, int64_t n, const uint32_t* perm, const double* pts,
                           double* sorted, hipStream_t s);
@@ -135,7 +136,7 @@ void launch_bseeds(int dim, const double* pts, int64_t n_h, double buff, int per
 void launch_bgrid(int dim, const double* pts, const int64_t* n_dev, int64_t cap, int64_t cell_cap,
                   const int32_t* sbox, const int32_t* N, const int64_t* coff, int32_t* cell, int32_t* counts,
                   int32_t* starts, int32_t* fill, int32_t* tmp, int32_t* total, uint32_t* perm, double* sorted,
-                  int32_t* sorted_box, hipStream_t s);
+                  int32_t* sorted_box, int32_t* big, hipStream_t s);
 void launch_bknn(int dim, const double* sorted, const uint32_t* perm, const int32_t* sbox, const int32_t* N,
                  const int64_t* coff, const int32_t* starts, const int32_t* counts, const uint32_t* queries,
                  const int64_t* nq_dev, int64_t cap, int k, uint32_t* out, int wave_min_k, int blocks_per_cu,

@@ -7,8 +7,12 @@
 // cfl (0.4), gamma (5/3), buff (0.04), boundary (periodic|reflective),
 // move_mesh (false: the mesh is built once as in the reference; true: the
 // generators move with the fluid and the mesh is rebuilt every step, all
-// resident in HBM through pg_sim_*).
+// resident in HBM through pg_sim_*).  Learned k (resident runs): learned_k
+// (false), mlp_weights (float32 parameter file to load; when it does not
+// exist the classifier is trained on the GPU on the first mesh and, with
+// mlp_weights_out, saved), mlp_train_steps (400).
 #include <cstdio>
+#include <fstream>
 #include <iostream>
 #include <string>
 #include <vector>
@@ -48,6 +52,32 @@ int run_resident(int n, double t_end, double cfl, int boundary, bool move) {
     pgpu_host::check(pg_sim_create(pgpu_host::ctx(), &sp, icData.seedpos.data(), icData.rho.data(),
                                    icData.vel.data(), icData.Energy.data(), n, &sim),
                      "pg_sim_create");
+    pg_kmodel* km = nullptr;
+    if (input.getBool("learned_k", false)) {
+        pgpu_host::check(pg_kmodel_create(pgpu_host::ctx(), 1, &km), "pg_kmodel_create");
+        const std::string wf = input.getString("mlp_weights", "");
+        std::vector<float> w(PG_KMODEL_NPARAM);
+        std::ifstream in(wf, std::ios::binary);
+        if (!wf.empty() && in.read((char*)w.data(), sizeof(float) * w.size())) {
+            pgpu_host::check(pg_kmodel_set_params(pgpu_host::ctx(), km, w.data()), "pg_kmodel_set_params");
+            std::cout << "Learned k: weights from " << wf << std::endl;
+        } else {
+            const pg_mesh* m0 = nullptr;
+            pgpu_host::check(pg_sim_mesh(sim, &m0), "pg_sim_mesh");
+            double loss = 0;
+            pgpu_host::check(pg_kmodel_train_mesh(pgpu_host::ctx(), km, m0, input.getInt("mlp_train_steps", 400), 4096,
+                                                  3e-3, 7, &loss),
+                             "pg_kmodel_train_mesh");
+            std::cout << "Learned k: trained on the first mesh, loss " << loss << std::endl;
+            const std::string wo = input.getString("mlp_weights_out", "");
+            if (!wo.empty()) {
+                pgpu_host::check(pg_kmodel_get_params(pgpu_host::ctx(), km, w.data()), "pg_kmodel_get_params");
+                std::ofstream out(wo, std::ios::binary);
+                out.write((const char*)w.data(), sizeof(float) * w.size());
+            }
+        }
+        pgpu_host::check(pg_sim_set_kmodel(sim, km), "pg_sim_set_kmodel");
+    }
     double t = 0;
     int step = 0;
     while (t < t_end) {
@@ -61,6 +91,7 @@ int run_resident(int n, double t_end, double cfl, int boundary, bool move) {
     std::vector<double> pts((size_t)n * DIMENSION), rho(n), vel((size_t)n * DIMENSION), E(n);
     pgpu_host::check(pg_sim_get(pgpu_host::ctx(), sim, pts.data(), rho.data(), vel.data(), E.data()), "pg_sim_get");
     pg_sim_free(&sim);
+    if (km) pg_kmodel_free(&km);
 #ifdef USE_HDF5
     // snapshot of the final generators' mesh with the final state
     VMesh* mesh = boundary == PG_BOUNDARY_PERIODIC ? voronoi::compute_periodic_mesh((POINT_TYPE*)pts.data(), n)

@@ -30,7 +30,9 @@ def test_kh_boxes_equal_single_runs(ctx, graph):
     print(f"kh batch sizes={sizes.tolist()} graph={graph}", flush=True)
     pts = batch_points(sizes, 2, 100)
     rho, vel, E = kelvin_helmholtz_state(pts)
-    kw = dict(boundary="periodic", buff=0.04, k=16, gamma=5.0 / 3.0, cfl=0.3, flux="physical", move_mesh=True)
+    # boxes of 1000 cells have cells ~0.03 wide: the ghost layer must be
+    # wider than the reference's 0.04 for no hydro face to reach the box plane
+    kw = dict(boundary="periodic", buff=0.1, k=16, gamma=5.0 / 3.0, cfl=0.3, flux="physical", move_mesh=True)
     B = api.Batch(ctx, sizes, pts, rho, vel, E, **kw)
     B.set_tuning(use_graph=graph)
     tot0 = B.totals()
@@ -73,7 +75,7 @@ def test_tunings_and_autotune_do_not_change_results(ctx):
     sizes = box_sizes(12, 1000, 6000, 7)
     pts = batch_points(sizes, 2, 300)
     rho, vel, E = kelvin_helmholtz_state(pts)
-    kw = dict(boundary="periodic", k=16, gamma=5.0 / 3.0, cfl=0.3)
+    kw = dict(boundary="periodic", buff=0.1, k=16, gamma=5.0 / 3.0, cfl=0.3)
     ref = None
     for slots, wmk, bpc in [(64, 33, 8), (48, 1, 4), (64, 1, 16)]:
         B = api.Batch(ctx, sizes, pts, rho, vel, E, **kw)
@@ -107,9 +109,9 @@ def test_learned_k_batch_equals_fixed(ctx):
     sizes = np.array([3000, 5200, 4100])
     pts = batch_points(sizes, 2, 400)
     rho, vel, E = kelvin_helmholtz_state(pts)
-    kw = dict(boundary="periodic", k=16, gamma=5.0 / 3.0, cfl=0.3)
+    kw = dict(boundary="periodic", buff=0.1, k=16, gamma=5.0 / 3.0, cfl=0.3)
     km = api.KModel(ctx, seed=2)
-    km.train_mesh(api.Mesh(ctx, pts[:3000], boundary="periodic", k=16), steps=100, batch=2048, lr=3e-3)
+    km.train_mesh(api.Mesh(ctx, pts[:3000], boundary="periodic", buff=0.1, k=16), steps=100, batch=2048, lr=3e-3)
     bad = api.KModel(ctx)
     P = np.zeros(L.NPARAM, dtype=np.float32)
     P[5632] = 30.0  # every seed in the k = 16 bucket

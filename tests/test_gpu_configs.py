@@ -1,0 +1,132 @@
+"""The workload configurations at full size, checked through size-independent
+properties (the oracle cannot run them in seconds):
+
+* clustered 3D (4M points, 64 Gaussians with sigma log-uniform in
+  [0.002, 0.05] plus 20% uniform, periodic): the learned-k mesh equals the
+  fixed-k mesh array for array, every cell is certified, and the hydro
+  volumes tile the unit box;
+* 256 Kelvin-Helmholtz boxes (sizes log-uniform in [1000, 65536]), 50 batched
+  steps: mass and energy of every box conserved, and the smallest boxes
+  bit-identical to running them alone.
+"""
+import numpy as np
+import pytest
+
+from proteusgpu_amd import api
+from proteusgpu_amd.generators import batch_points, box_sizes, gaussian_mixture, kelvin_helmholtz_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _arrays(m, keys=("status", "kcert", "face_counts", "volumes", "neighbor_cell")):
+    n, F = m.n_seeds, m.num_faces
+    import ctypes as C
+    from proteusgpu_amd._native import VMeshView, check, lib
+    out = {"status": np.empty(n, np.int32), "kcert": np.empty(n, np.int32), "face_counts": np.empty(n, np.int64),
+           "volumes": np.empty(n), "neighbor_cell": np.empty(F if "neighbor_cell" in keys else 0, np.int32)}
+    out = {k: v for k, v in out.items() if k in keys}
+    v = VMeshView()
+    for name, arr in out.items():
+        setattr(v, name, arr.ctypes.data)
+    check(lib().pg_mesh_copy_out(m.ctx.h, m.h, C.byref(v)))
+    return out
+
+
+def test_clustered_4m_learned_equals_fixed(ctx):
+    n = 4_000_000
+    print("clustered: generating", flush=True)
+    pts = gaussian_mixture(n, 64, 0.002, 0.05, 0.2, 3)
+    km = api.KModel(ctx, seed=9)
+    sub = pts[:: 10]
+    km.train_mesh(api.Mesh(ctx, sub, boundary="periodic", k=32), steps=300, batch=4096, lr=3e-3)
+    print("clustered: fixed k", flush=True)
+    plain = api.Mesh(ctx, pts, boundary="periodic", k=32)
+    a = _arrays(plain)
+    print(f"clustered: fixed k done, {plain.n_seeds} seeds, restarted {plain.n_restarted}, "
+          f"k_used {plain.k_used}", flush=True)
+    plain.free()
+    learned = api.Mesh(ctx, pts, boundary="periodic", k=32, kmodel=km)
+    b = _arrays(learned)
+    print(f"clustered: learned k classes {learned.n_class}, restarted {learned.n_restarted}", flush=True)
+    assert learned.n_failed == 0
+    for key in a:
+        assert np.array_equal(a[key], b[key]), key
+    assert abs(a["volumes"][:n].sum() - 1.0) < 1e-9
+    assert np.all(a["status"] == 4)
+
+
+def test_kh_256_boxes_50_steps(ctx):
+    sizes = box_sizes(256, 1000, 65536, 11)
+    print(f"kh: {len(sizes)} boxes, {sizes.sum()} cells", flush=True)
+    pts = batch_points(sizes, 2, 500)
+    rho, vel, E = kelvin_helmholtz_state(pts)
+    # buff 0.1: the smallest boxes (1000 cells) have cells ~0.03 wide
+    kw = dict(boundary="periodic", buff=0.1, k=16, gamma=5.0 / 3.0, cfl=0.3, flux="physical", move_mesh=True)
+    B = api.Batch(ctx, sizes, pts, rho, vel, E, **kw)
+    tot0 = B.totals()
+    ms = [B.step() for _ in range(50)]
+    tot = B.totals()
+    print(f"kh: 50 steps, {np.mean(ms[5:]):.3f} ms per step (GPU)", flush=True)
+    np.testing.assert_allclose(tot[:, 0], tot0[:, 0], rtol=1e-12)
+    np.testing.assert_allclose(tot[:, 4], tot0[:, 4], rtol=1e-12)
+    bx, br, bv, bE = B.get()
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    for b in np.argsort(sizes, kind="stable")[:3]:
+        sl = slice(off[b], off[b + 1])
+        sim = api.Simulation(ctx, pts[sl], rho[sl], vel[sl], E[sl], **kw)
+        for _ in range(50):
+            sim.step()
+        ax, ar, av, aE = sim.get()
+        sim.free()
+        assert np.array_equal(bx[sl], ax) and np.array_equal(br[sl], ar), f"box {b}"
+        assert np.array_equal(bv[sl], av) and np.array_equal(bE[sl], aE), f"box {b}"
+    B.free()
+
+
+def test_sedov_8m_100_steps(ctx):
+    """200^3 jittered lattice, rho=1, P=1e-5, gamma=5/3, E=1 in the 32 cells
+    nearest the centre; 100 moving-mesh steps of the batched step with
+    learned k.  Mass and energy conserved; the blast radius (density peak of
+    the radial profile) against R = xi0 (E t^2 / rho)^(1/5)."""
+    from oracle import analytic as A
+    from proteusgpu_amd.generators import jittered_lattice
+    g, m = 5.0 / 3.0, 200
+    pts = jittered_lattice(m, 3, 23)
+    n = len(pts)
+    d2 = np.sum((pts - 0.5) ** 2, axis=1)
+    hot = np.argsort(d2, kind="stable")[:32]
+    mesh = api.Mesh(ctx, pts, boundary="reflective", k=32)
+    V = _arrays(mesh, ("volumes",))["volumes"]
+    km = api.KModel(ctx, seed=4)
+    km.train_mesh(mesh, steps=200, batch=4096, lr=3e-3)
+    mesh.free()
+    rho = np.ones(n)
+    vel = np.zeros((n, 3))
+    E = np.full(n, 1e-5 / (g - 1.0))
+    E[hot] += (1.0 / 32.0) / V[hot]
+    B = api.Batch(ctx, np.array([n]), pts, rho, vel, E, boundary="reflective", k=32, gamma=g, cfl=0.3,
+                  flux="physical", move_mesh=True)
+    B.set_kmodel(km)
+    tot0 = B.totals()[0]
+    t = 0.0
+    ms = []
+    for _ in range(100):
+        ms.append(B.step())
+        t += float(B.dt()[0])
+    tot = B.totals()[0]
+    x, r, v, e = B.get()
+    rad = np.sqrt(np.sum((x - 0.5) ** 2, axis=1))
+    R = A.sedov_xi0(g) * t ** 0.4
+    bins = np.linspace(0.0, 3.0 * R, 61)
+    idx = np.digitize(rad, bins) - 1
+    prof = np.array([r[idx == b].mean() if np.any(idx == b) else 0.0 for b in range(60)])
+    r_shock = 0.5 * (bins[np.argmax(prof)] + bins[np.argmax(prof) + 1])
+    print(f"sedov 8M: t={t:.3e} R_num={r_shock:.4f} R_exact={R:.4f} ({R * m:.1f} cells) peak rho={prof.max():.2f} "
+          f"{np.mean(ms[5:]):.1f} ms/step; mass drift {(tot[0] - tot0[0]) / tot0[0]:.2e}, "
+          f"energy drift {(tot[4] - tot0[4]) / tot0[4]:.2e}", flush=True)
+    assert abs(tot[0] - tot0[0]) <= 1e-12 * tot0[0]
+    assert abs(tot[4] - tot0[4]) <= 1e-11 * tot0[4]
+    # after 100 steps the blast spans only a few lattice cells: a first-order
+    # scheme places the density peak within two cells of the analytic radius
+    assert abs(r_shock - R) <= 2.0 / m + 0.1 * R
+    B.free()
