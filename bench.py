@@ -91,6 +91,11 @@ def cpu_baseline(n_sample, k):
                       f"1 HLL step ({el:.2f} s)"}
 
 
+def phase(name):
+    """Progress line on stderr (the JSON result line stays alone on stdout)."""
+    print(f"[bench] {name}", file=sys.stderr, flush=True)
+
+
 def run_steps(ctx, sim, warmup, steps):
     for _ in range(warmup):
         sim.step()
@@ -180,12 +185,14 @@ def main():
         return profile_variant(a, ctx, api, pts, rho, vel, E, kw)
 
     # 1. the plain method: fixed k=32 + certificate restarts, host-driven step
+    phase("fixed k, host-driven step")
     plain = api.Simulation(ctx, pts, rho, vel, E, **kw)
     ms_plain, st_plain = run_steps(ctx, plain, a.warmup, a.steps)
     sched_plain = schedule_ms(ctx, plain)
 
     # 2. learned k: the classifier is trained on the GPU, in this run, on the
     #    first mesh of the workload (certified k of every seed as labels)
+    phase("training the k classifier")
     km = api.KModel(ctx, seed=1)
     trainer = api.Simulation(ctx, pts, rho, vel, E, **kw)
     ctx.sync()
@@ -194,6 +201,7 @@ def main():
     ctx.sync()
     train_ms = (time.perf_counter() - t0) * 1e3
     # learned k in the host-driven step (stage times)
+    phase("learned k, host-driven step")
     trainer.set_kmodel(km)
     ms_lsim, st_lsim = run_steps(ctx, trainer, a.warmup, a.steps)
     info_l = trainer.mesh_info()
@@ -202,16 +210,21 @@ def main():
     # 3. the batched step: one box, the whole step in one hipGraph launch,
     #    fixed k and learned k
     sizes = np.array([a.n])
+    phase("batched step, fixed k")
     bfix = api.Batch(ctx, sizes, pts, rho, vel, E, **kw)
+    tune_fix, _ = bfix.autotune(reps=1)
     ms_bfix, gpu_bfix = run_batch(ctx, bfix, a.warmup, a.steps)
+    phase("batched step, learned k")
     blearn = api.Batch(ctx, sizes, pts, rho, vel, E, **kw)
     blearn.set_kmodel(km)
+    tune_learn, _ = blearn.autotune(reps=1)
     ms_blearn, gpu_blearn = run_batch(ctx, blearn, a.warmup, a.steps)
 
     # every run took the same steps from the same state: identical results
     # (the batched runs did warmup + steps; a fresh host-driven run is
     # brought to the same step count)
     same = True
+    phase("identity check")
     chk = api.Simulation(ctx, pts, rho, vel, E, **kw)
     for _ in range(a.warmup + a.steps):
         chk.step()
@@ -240,6 +253,7 @@ def main():
     path_gbs = PATH_BYTES_PER_CELL * a.n / (best * 1e-3) / 1e9
     cpu = None
     if not a.no_cpu:
+        phase("cpu baseline")
         cpu = cpu_baseline(a.cpu_sample, a.k)
     tot = blearn.totals()[0]
     out = {
@@ -266,8 +280,10 @@ def main():
                                 "restarted_seeds_per_step": st_lsim["n_restarted"],
                                 "row_entries_per_step": st_lsim["row_entries"],
                                 "seeds_per_bucket": [int(c) for c in info_l["n_class"]]},
-        "batched_fixed_k": {"ms_per_step": round(ms_bfix, 3), "gpu_ms_per_step": round(gpu_bfix, 3)},
-        "batched_learned_k": {"ms_per_step": round(ms_blearn, 3), "gpu_ms_per_step": round(gpu_blearn, 3)},
+        "batched_fixed_k": {"ms_per_step": round(ms_bfix, 3), "gpu_ms_per_step": round(gpu_bfix, 3),
+                            "tuning": tune_fix},
+        "batched_learned_k": {"ms_per_step": round(ms_blearn, 3), "gpu_ms_per_step": round(gpu_blearn, 3),
+                              "tuning": tune_learn},
         "training": {"ms": round(train_ms, 2), "adam_steps": a.train_steps, "batch": 4096,
                      "final_loss": round(loss, 4)},
         "path_roofline": {"bytes_per_cell": PATH_BYTES_PER_CELL, "achieved_GBs": round(path_gbs, 2),

@@ -83,6 +83,13 @@ int pg_knn_get_permutation(pg_ctx* ctx, const pg_knn* knn, uint32_t* out); /* n 
 /* knn::knn_free (knn.h:54): frees and nulls the handle. */
 int pg_knn_free(pg_knn** knn);
 
+/* Shape of the fast clip kernel for every following build of this process
+ * (no reference counterpart; results never depend on it): lanes per seed
+ * (8 or 16) and vertex slots per seed (48 or 64; seeds that outgrow them
+ * are recomputed by a 128-slot kernel).  Default 16 x 64, or PGPU_CLIP_CFG. */
+int pg_set_clip_config(int lanes, int slots);
+int pg_get_clip_config(int* lanes, int* slots);
+
 /* ------------------------------------------------------------------ Voronoi */
 typedef struct {
     int dim;       /* 2 or 3 (Config.sh dim_2D / dim_3D)                          */
@@ -169,8 +176,15 @@ typedef struct {
     double gamma;   /* adiabatic index (allvars.cpp:14 hard-codes 5/3)           */
     double cfl;     /* CFL number (main.cpp:47 hard-codes 0.4)                   */
     int flux_mode;  /* PG_FLUX_*                                                 */
-    int move_mesh;  /* 1: generators move with the fluid, ALE face fluxes        */
+    int move_mesh;  /* PG_MOVE_*                                                 */
 } pg_sim_params;
+
+/* Generator motion of a simulation (pg_sim_params.move_mesh). */
+#define PG_MOVE_STATIC 0      /* static mesh (main.cpp:41)                     */
+#define PG_MOVE_FLUID 1       /* generators move with the fluid velocity       */
+#define PG_MOVE_REGULARIZED 2 /* fluid velocity plus a drift towards the cell
+                               * centroid (Springel 2010, MNRAS 401, 791, eq. 63:
+                               * eta = 0.25, chi = 1), keeps cells round        */
 
 /* Stage times of the last pg_sim_step, milliseconds (HIP events). */
 typedef struct {
@@ -264,6 +278,7 @@ int pg_mesh_compute_k(pg_ctx* ctx, const pg_mesh_params* params, const pg_kmodel
  * point and state arrays; coordinates of every box lie in its own unit box.
  * Every box needs more than max(k, k_max) + 1 points. */
 typedef struct {
+    int clip_lanes;      /* lanes per seed of the fast clip kernel: 8 or 16         */
     int clip_slots;      /* vertex slots per seed of the fast clip kernel: 48 or 64 */
     int knn_wave_min_k;  /* rows at least this long use the wave-per-query KNN      */
     int blocks_per_cu;   /* persistent grid size of the KNN and clip kernels         */
@@ -287,7 +302,7 @@ int pg_batch_set_tuning(pg_batch* batch, const pg_batch_tuning* t);
  * the model must outlive its use.  Results do not change. */
 int pg_batch_set_kmodel(pg_batch* batch, const pg_kmodel* model);
 int pg_batch_get_tuning(const pg_batch* batch, pg_batch_tuning* t);
-/* Time every tuning candidate (clip slots x KNN kernel choice x grid size)
+/* Time every tuning candidate (clip shape x KNN kernel choice x grid size)
  * on `reps` steps from the current state, restore the state and keep the
  * fastest.  Results never depend on the tuning. */
 int pg_batch_autotune(pg_ctx* ctx, pg_batch* batch, int reps, pg_batch_tuning* best, double* best_ms);

@@ -18,6 +18,17 @@ def _ptr(a):
     return C.c_void_p(a.ctypes.data) if a is not None else None
 
 
+def set_clip_config(lanes: int = 16, slots: int = 64):
+    """Shape of the fast clip kernel for the following builds (timing only)."""
+    check(lib().pg_set_clip_config(lanes, slots))
+
+
+def get_clip_config():
+    lanes, slots = C.c_int(), C.c_int()
+    check(lib().pg_get_clip_config(C.byref(lanes), C.byref(slots)))
+    return lanes.value, slots.value
+
+
 def device_count() -> int:
     n = C.c_int(0)
     check(lib().pg_device_count(C.byref(n)))
@@ -269,6 +280,18 @@ class HydroState:
         self.free()
 
 
+MOVE = {"static": 0, "fluid": 1, "regularized": 2}
+
+
+def move_mode(move_mesh):
+    """PG_MOVE_* from False/True (static / with the fluid) or a name:
+    "static", "fluid", "regularized" (fluid velocity plus a drift towards the
+    cell centroid that keeps cells round)."""
+    if isinstance(move_mesh, str):
+        return MOVE[move_mesh]
+    return int(bool(move_mesh))
+
+
 class Simulation:
     """Device-resident moving-mesh loop (main.cpp:50-63 kept in HBM).
 
@@ -284,7 +307,7 @@ class Simulation:
         self.ctx, self.n, self.dim = ctx, pts.shape[0], pts.shape[1]
         p = SimParams()
         p.mesh = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max)
-        p.gamma, p.cfl, p.flux_mode, p.move_mesh = gamma, cfl, FLUX[flux], int(move_mesh)
+        p.gamma, p.cfl, p.flux_mode, p.move_mesh = gamma, cfl, FLUX[flux], move_mode(move_mesh)
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, vel, E)]
         h = C.c_void_p()
         check(lib().pg_sim_create(ctx.h, C.byref(p), _ptr(pts), _ptr(arrs[0]), _ptr(arrs[1]),
@@ -378,7 +401,7 @@ class Batch:
         assert pts.shape[0] == self.n
         p = SimParams()
         p.mesh = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max)
-        p.gamma, p.cfl, p.flux_mode, p.move_mesh = gamma, cfl, FLUX[flux], int(move_mesh)
+        p.gamma, p.cfl, p.flux_mode, p.move_mesh = gamma, cfl, FLUX[flux], move_mode(move_mesh)
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, vel, E)]
         h = C.c_void_p()
         check(lib().pg_batch_create(ctx.h, C.byref(p), len(self.sizes), _ptr(self.sizes), _ptr(pts),
@@ -398,9 +421,9 @@ class Batch:
         check(lib().pg_batch_get_tuning(self.h, C.byref(t)))
         return {name: getattr(t, name) for name, _ in t._fields_}
 
-    def set_tuning(self, clip_slots=64, knn_wave_min_k=33, blocks_per_cu=8, use_graph=True):
+    def set_tuning(self, clip_slots=64, knn_wave_min_k=33, blocks_per_cu=8, use_graph=True, clip_lanes=16):
         from ._native import BatchTuning
-        t = BatchTuning(clip_slots, knn_wave_min_k, blocks_per_cu, int(use_graph))
+        t = BatchTuning(clip_lanes, clip_slots, knn_wave_min_k, blocks_per_cu, int(use_graph))
         check(lib().pg_batch_set_tuning(self.h, C.byref(t)))
 
     def set_kmodel(self, km: "KModel | None"):

@@ -458,6 +458,21 @@ extern "C" {
 const char* pg_version(void) { return "pgpu 0.1 (gfx950)"; }
 const char* pg_last_error(void) { return g_err.c_str(); }
 
+int pg_set_clip_config(int lanes, int slots) {
+    return guarded([&] {
+        argchk(pg::valid_clip_config(lanes, slots), "clip lanes must be 8 or 16, slots 48 or 64");
+        pg::set_clip_config(lanes, slots);
+    });
+}
+
+int pg_get_clip_config(int* lanes, int* slots) {
+    return guarded([&] {
+        argchk(lanes && slots, "null argument");
+        *lanes = pg::clip_lanes();
+        *slots = pg::clip_slots();
+    });
+}
+
 int pg_device_count(int* count) {
     return guarded([&] {
         argchk(count != nullptr, "null count");
@@ -847,7 +862,7 @@ struct pg_sim {
     const pg_kmodel* km = nullptr;
     pg_mesh mesh;
     int64_t n = 0;
-    DBuf x, mass, mom, en, rho, vel, E, dt, part;
+    DBuf x, mass, mom, en, rho, vel, E, dt, part, wgen;
     Marks mk;
     pg_sim_times last{};
     ~pg_sim() {
@@ -881,14 +896,24 @@ void sim_hydro(pg_ctx* c, pg_sim* S, double dt_max, bool moving) {
     unsigned long long* d = S->dt.get<unsigned long long>(1);
     const double init = 1e100;
     HIPCHK(hipMemcpyAsync(d, &init, sizeof(double), hipMemcpyHostToDevice, s));
+    const double* w = nullptr;  // steered generator velocities (regularised motion)
+    if (moving && S->prm.move_mesh == PG_MOVE_REGULARIZED) {
+        double* wg = S->wgen.get<double>((size_t)S->n * dim);
+        pg::launch_gen_velocity(dim, S->n, S->x.as<double>(), m.o2s.as<int32_t>(), m.vol_u.as<double>(),
+                                m.ccen.as<double>(), m.n_seeds, sc, S->prm.gamma, S->rho.as<double>(),
+                                S->vel.as<double>(), S->E.as<double>(), wg, s);
+        w = wg;
+    }
+    pg::CflMesh cm;
+    if (w) cm = {m.count.as<uint8_t>(), m.nbr.as<int32_t>(), m.g.sorted.as<double>(), m.n_seeds, sc};
     pg::launch_dt_cfl(dim, S->n, m.o2s.as<int32_t>(), m.vol_u.as<double>(), S->prm.gamma, S->prm.cfl,
-                      S->rho.as<double>(), S->vel.as<double>(), S->E.as<double>(), d, s);
+                      S->rho.as<double>(), S->vel.as<double>(), S->E.as<double>(), d, s, w, cm);
     pg::launch_hll_ale(dim, m.n_seeds, m.n_hydro, m.n_seeds, m.n_seeds * MAXP, m.g.perm.as<uint32_t>(),
                        m.count.as<uint8_t>(), m.hid.as<int32_t>(), m.nbr.as<int32_t>(), m.nrm.as<double>(),
                        m.area_u.as<double>(), m.cen.as<double>(), m.g.sorted.as<double>(), sc, S->prm.gamma,
-                       (const double*)d, dt_max, S->prm.flux_mode, moving ? 1 : 0, S->view(), s);
+                       (const double*)d, dt_max, S->prm.flux_mode, moving ? 1 : 0, S->view(), s, w);
     if (moving)
-        pg::launch_drift(dim, S->n, S->x.as<double>(), S->vel.as<double>(), (const double*)d, dt_max,
+        pg::launch_drift(dim, S->n, S->x.as<double>(), w ? w : S->vel.as<double>(), (const double*)d, dt_max,
                          S->prm.mesh.boundary == PG_BOUNDARY_PERIODIC ? 1 : 0, s);
     HIPCHK(hipGetLastError());
 }
@@ -920,6 +945,7 @@ int pg_sim_create(pg_ctx* c, const pg_sim_params* prm, const double* pts, const 
         argchk(mp.boundary != PG_BOUNDARY_PERIODIC || (mp.buff > 0 && mp.buff < 0.5), "buff must be in (0, 0.5)");
         argchk(prm->gamma > 1.0 && prm->cfl > 0.0, "gamma > 1 and cfl > 0 required");
         argchk(prm->flux_mode == PG_FLUX_REFERENCE || prm->flux_mode == PG_FLUX_PHYSICAL, "unknown flux mode");
+        argchk(prm->move_mesh >= PG_MOVE_STATIC && prm->move_mesh <= PG_MOVE_REGULARIZED, "unknown move_mesh mode");
         argchk(n >= 2 && n < (int64_t)1 << 29, "n out of range");
         pg_sim* S = new pg_sim();
         try {
@@ -1289,14 +1315,14 @@ struct pg_batch {
     std::vector<int64_t> hoff_h;
     std::vector<int> ksched, ksched_learned;
     // tuning (timing only; results do not depend on it)
-    int slots = 64, wave_min_k = 33, bpc = 8;
+    int lanes = 16, slots = 64, wave_min_k = 20, bpc = 8;
     bool use_graph = true;
     const pg_kmodel* km = nullptr;  // learned k (nullptr: fixed k)
     // static
     DBuf hoff, hbox, x, xs;
     DBuf feat, kcls, klists, kcnt;  // learned k
     // conserved state and primitives (hydro order)
-    DBuf mass, mom, en, rho, vel, E, dt_box;
+    DBuf mass, mom, en, rho, vel, E, dt_box, wgen;
     DBuf mass_s, mom_s, en_s;  // autotuner save area
     // per step
     DBuf gcnt, goff, gtot, soff, N, coff, stat, spts, sbox, orig_h;
@@ -1370,7 +1396,7 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
     };
     auto clip = [&](const uint32_t* q, const int64_t* nq, int k, int big) {
         pg::launch_bvoronoi(dim, B->sorted.as<double>(), q, nq, cap, B->knn.as<uint32_t>(), k, p.max_p, p.max_t, big,
-                            B->slots, B->faces(), B->bpc, s);
+                            B->lanes, B->slots, B->faces(), B->bpc, s);
     };
     auto collect = [&](const uint32_t* L, const int64_t* nL, int code, uint32_t* out, int64_t* nout) {
         pg::launch_bcollect(nL, cap, L, B->status.as<int32_t>(), code, out, B->flags.as<int32_t>(),
@@ -1429,15 +1455,26 @@ void batch_hydro(pg_batch* B, hipStream_t s) {
     const int dim = B->dim;
     const bool move = B->prm.move_mesh != 0;
     pg::launch_prim_from_q(dim, B->view(), B->o2s.as<int32_t>(), B->vol_u.as<double>(), s);
+    const double* w = nullptr;  // steered generator velocities (regularised motion)
+    if (move && B->prm.move_mesh == PG_MOVE_REGULARIZED) {
+        double* wg = B->wgen.as<double>();
+        pg::launch_gen_velocity(dim, B->n_h, B->x.as<double>(), B->o2s.as<int32_t>(), B->vol_u.as<double>(),
+                                B->ccen.as<double>(), B->cap, B->scale(), B->prm.gamma, B->rho.as<double>(),
+                                B->vel.as<double>(), B->E.as<double>(), wg, s);
+        w = wg;
+    }
     pg::launch_bdt(dim, B->n_h, B->nbox, B->hbox.as<int32_t>(), B->o2s.as<int32_t>(), B->vol_u.as<double>(),
                    B->prm.gamma, B->prm.cfl, B->rho.as<double>(), B->vel.as<double>(), B->E.as<double>(),
-                   B->dt_box.as<double>(), s);
+                   B->dt_box.as<double>(), s, w,
+                   w ? pg::CflMesh{B->count.as<uint8_t>(), B->nbr.as<int32_t>(), B->sorted.as<double>(), B->cap,
+                                   B->scale()}
+                     : pg::CflMesh{});
     pg::launch_bhll(dim, B->n_dev(), B->cap, B->cap, B->cap * MAXP, B->hsorted.as<int32_t>(), B->hbox.as<int32_t>(),
                     B->count.as<uint8_t>(), B->hid.as<int32_t>(), B->nbr.as<int32_t>(), B->nrm.as<double>(),
                     B->area_u.as<double>(), B->cen.as<double>(), B->sorted.as<double>(), B->scale(), B->prm.gamma,
-                    B->dt_box.as<double>(), B->prm.flux_mode, move ? 1 : 0, B->view(), s);
+                    B->dt_box.as<double>(), B->prm.flux_mode, move ? 1 : 0, B->view(), s, w);
     if (move)
-        pg::launch_bdrift(dim, B->n_h, B->hbox.as<int32_t>(), B->x.as<double>(), B->vel.as<double>(),
+        pg::launch_bdrift(dim, B->n_h, B->hbox.as<int32_t>(), B->x.as<double>(), w ? w : B->vel.as<double>(),
                           B->dt_box.as<double>(), B->periodic ? 1 : 0, s);
 }
 
@@ -1530,6 +1567,7 @@ int pg_batch_create(pg_ctx* c, const pg_sim_params* prm, int nbox, const int64_t
         argchk(mp.boundary != PG_BOUNDARY_PERIODIC || (mp.buff > 0 && mp.buff < 0.5), "buff must be in (0, 0.5)");
         argchk(prm->gamma > 1.0 && prm->cfl > 0.0, "gamma > 1 and cfl > 0 required");
         argchk(prm->flux_mode == PG_FLUX_REFERENCE || prm->flux_mode == PG_FLUX_PHYSICAL, "unknown flux mode");
+        argchk(prm->move_mesh >= PG_MOVE_STATIC && prm->move_mesh <= PG_MOVE_REGULARIZED, "unknown move_mesh mode");
         argchk(nbox >= 1 && nbox <= 4096, "nbox must be in [1, 4096]");
         pg_batch* B = new pg_batch();
         try {
@@ -1582,6 +1620,7 @@ int pg_batch_create(pg_ctx* c, const pg_sim_params* prm, int nbox, const int64_t
             HIPCHK(hipMemcpyAsync(B->x.get<double>(n * d), pts, sizeof(double) * n * d, hipMemcpyHostToDevice, s));
             HIPCHK(hipMemcpyAsync(B->rho.get<double>(n), rho, sizeof(double) * n, hipMemcpyHostToDevice, s));
             HIPCHK(hipMemcpyAsync(B->vel.get<double>(n * d), vel, sizeof(double) * n * d, hipMemcpyHostToDevice, s));
+            if (B->prm.move_mesh == PG_MOVE_REGULARIZED) B->wgen.get<double>(n * d);
             HIPCHK(hipMemcpyAsync(B->E.get<double>(n), E, sizeof(double) * n, hipMemcpyHostToDevice, s));
             B->xs.get<double>(n * d);
             B->mass.get<double>(n);
@@ -1638,6 +1677,7 @@ int pg_batch_create(pg_ctx* c, const pg_sim_params* prm, int nbox, const int64_t
             B->kcnt.get<int64_t>(pg::LK_NCLASS);
             HIPCHK(hipEventCreate(&B->e0));
             HIPCHK(hipEventCreate(&B->e1));
+            B->lanes = pg::clip_lanes();
             B->slots = pg::clip_slots();
             batch_rebuild(B, s);
             HIPCHK(hipGetLastError());
@@ -1676,10 +1716,11 @@ int pg_batch_set_kmodel(pg_batch* B, const pg_kmodel* km) {
 int pg_batch_set_tuning(pg_batch* B, const pg_batch_tuning* t) {
     return guarded([&] {
         argchk(B && t, "null argument");
-        argchk(t->clip_slots == 48 || t->clip_slots == 64, "clip_slots must be 48 or 64");
+        argchk(pg::valid_clip_config(t->clip_lanes, t->clip_slots), "clip_lanes must be 8 or 16, clip_slots 48 or 64");
         argchk(t->knn_wave_min_k >= 1 && t->blocks_per_cu >= 1 && t->blocks_per_cu <= 64, "bad tuning");
-        const bool changed = B->slots != t->clip_slots || B->wave_min_k != t->knn_wave_min_k ||
+        const bool changed = B->lanes != t->clip_lanes || B->slots != t->clip_slots || B->wave_min_k != t->knn_wave_min_k ||
                              B->bpc != t->blocks_per_cu || B->use_graph != (t->use_graph != 0);
+        B->lanes = t->clip_lanes;
         B->slots = t->clip_slots;
         B->wave_min_k = t->knn_wave_min_k;
         B->bpc = t->blocks_per_cu;
@@ -1694,6 +1735,7 @@ int pg_batch_set_tuning(pg_batch* B, const pg_batch_tuning* t) {
 int pg_batch_get_tuning(const pg_batch* B, pg_batch_tuning* t) {
     return guarded([&] {
         argchk(B && t, "null argument");
+        t->clip_lanes = B->lanes;
         t->clip_slots = B->slots;
         t->knn_wave_min_k = B->wave_min_k;
         t->blocks_per_cu = B->bpc;
@@ -1707,16 +1749,16 @@ int pg_batch_autotune(pg_ctx* c, pg_batch* B, int reps, pg_batch_tuning* best, d
         argchk(B != nullptr && reps >= 1 && reps <= 16, "bad argument");
         hipStream_t s = c->stream;
         batch_copy_state(B, true, s);
-        const pg_batch_tuning start{B->slots, B->wave_min_k, B->bpc, B->use_graph ? 1 : 0};
+        const pg_batch_tuning start{B->lanes, B->slots, B->wave_min_k, B->bpc, B->use_graph ? 1 : 0};
         pg_batch_tuning winner = start;
         double wms = 1e300;
-        const int slot_c[2] = {64, 48};
+        const int shape_c[4][2] = {{16, 64}, {16, 48}, {8, 64}, {8, 48}};
         const int wave_c[2] = {33, 1};
-        const int bpc_c[3] = {4, 8, 16};
-        for (int a = 0; a < 2; a++)
+        const int bpc_c[2] = {8, 16};
+        for (int a = 0; a < 4; a++)
             for (int w = 0; w < 2; w++)
-                for (int q = 0; q < 3; q++) {
-                    const pg_batch_tuning t{slot_c[a], wave_c[w], bpc_c[q], 1};
+                for (int q = 0; q < 2; q++) {
+                    const pg_batch_tuning t{shape_c[a][0], shape_c[a][1], wave_c[w], bpc_c[q], 1};
                     if (pg_batch_set_tuning(B, &t) != PG_OK) throw Error{PG_ERR_ARG, g_err};
                     double tot = 0;
                     for (int r = 0; r < reps; r++) {

@@ -10,6 +10,7 @@
 // clip, restart schedule, CFL step), so its results are bit-identical to
 // running it alone.  No counterpart in the reference (one box per run).
 #include <algorithm>
+#include <type_traits>
 
 #include "common.h"
 #include "ghosts.h"
@@ -277,7 +278,8 @@ template <int DIM>
 __global__ void __launch_bounds__(TB)
     k_bdt(int64_t n_h, const int32_t* __restrict__ hbox, const int32_t* __restrict__ o2s,
           const double* __restrict__ vol_u, double gam, double cfl, const double* __restrict__ rho,
-          const double* __restrict__ vel, const double* __restrict__ E, unsigned long long* dt_box) {
+          const double* __restrict__ vel, const double* __restrict__ E, unsigned long long* dt_box,
+          const double* __restrict__ wgen, CflMesh cm) {
     const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
     double dti = 1e100;
     int b = -1;
@@ -291,15 +293,10 @@ __global__ void __launch_bounds__(TB)
         for (int d = 0; d < DIM; d++) s.v[d] = vel[(size_t)DIM * i + d];
         const double P = pressure<DIM>(gam, s);
         const double c = sqrt(gam * P / s.rho);
-        const double V = vol_u[o2s[i]];
-        double R, va;
-        if (DIM == 2) {
-            R = sqrt(V / M_PI);
-            va = sqrt(s.v[0] * s.v[0] + s.v[1] * s.v[1]);
-        } else {
-            R = cbrt(3.0 * V / (4.0 * M_PI));
-            va = sqrt(s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]);
-        }
+        const int32_t si = o2s[i];
+        const double R = cfl_length<DIM>(vol_u[si], si, cm);
+        double va = sqrt(s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]);
+        if (wgen) va = fmax(va, rel_speed<DIM>(s.v, wgen + (size_t)DIM * i));
         dti = cfl * (R / (c + va));
     }
     const int b0 = __shfl(b, 0, 64);
@@ -321,7 +318,7 @@ __global__ void __launch_bounds__(TB)
            const int32_t* __restrict__ hbox, const uint8_t* __restrict__ count, const int32_t* __restrict__ hid,
            const int32_t* __restrict__ nbr, const double* __restrict__ nrm, const double* __restrict__ area_u,
            const double* __restrict__ cen, const double* __restrict__ sorted, double scale, double gam,
-           const double* __restrict__ dt_box, int flux_mode, int move, SimView sv) {
+           const double* __restrict__ dt_box, int flux_mode, int move, SimView sv, const double* __restrict__ wgen) {
     const int64_t n = *n_dev;
     for (int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x; s < n; s += (int64_t)gridDim.x * TB) {
         const int32_t i = hsorted[s];
@@ -329,7 +326,7 @@ __global__ void __launch_bounds__(TB)
         const double dt = dt_box[hbox[i]];
         double fm, fp[3], fe;
         ale_cell_flux<DIM>(s, (uint32_t)i, stride, nplane, count, hid, nbr, nrm, area_u, cen, sorted, scale, gam,
-                           flux_mode, move, sv, fm, fp, fe);
+                           flux_mode, move, sv, wgen, fm, fp, fe);
         sv.mass[i] -= dt * fm;
 #pragma unroll
         for (int d = 0; d < DIM; d++) sv.mom[(size_t)DIM * i + d] -= dt * fp[d];
@@ -495,17 +492,18 @@ void launch_bknn(int dim, const double* sorted, const uint32_t* perm, const int3
 }
 
 void launch_bvoronoi(int dim, const double* sorted, const uint32_t* seeds, const int64_t* nq_dev, int64_t cap,
-                     const uint32_t* knn, int k, int max_p, int max_t, int big, int slots, FaceArrays fa,
+                     const uint32_t* knn, int k, int max_p, int max_t, int big, int lanes, int slots, FaceArrays fa,
                      int blocks_per_cu, hipStream_t s) {
-    if (dim == 2) {
-        if (big) bclip<2, 16, 8>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
-        else if (slots == 48) bclip<2, 16, 3>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
-        else bclip<2, 16, 4>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
-    } else {
-        if (big) bclip<3, 16, 8>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
-        else if (slots == 48) bclip<3, 16, 3>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
-        else bclip<3, 16, 4>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
-    }
+    auto go = [&](auto dimc) {
+        constexpr int D = decltype(dimc)::value;
+        if (big) bclip<D, 16, 8>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
+        else if (lanes == 8 && slots == 48) bclip<D, 8, 6>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
+        else if (lanes == 8) bclip<D, 8, 8>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
+        else if (slots == 48) bclip<D, 16, 3>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
+        else bclip<D, 16, 4>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
+    };
+    if (dim == 2) go(std::integral_constant<int, 2>{});
+    else go(std::integral_constant<int, 3>{});
 }
 
 void launch_bcollect(const int64_t* nq_dev, int64_t cap, const uint32_t* seeds, const int32_t* status, int code,
@@ -528,25 +526,27 @@ void launch_bfinalize(const int64_t* n_dev, int64_t cap, int64_t stride, const u
 
 void launch_bdt(int dim, int64_t n_h, int nbox, const int32_t* hbox, const int32_t* o2s, const double* vol_u,
                 double gam, double cfl, const double* rho, const double* vel, const double* E, double* dt_box,
-                hipStream_t s) {
+                hipStream_t s, const double* wgen, CflMesh cm) {
     k_fill<double><<<nblk(nbox), TB, 0, s>>>(dt_box, nbox, 1e100);
     if (dim == 2)
-        k_bdt<2><<<nblk(n_h), TB, 0, s>>>(n_h, hbox, o2s, vol_u, gam, cfl, rho, vel, E, (unsigned long long*)dt_box);
+        k_bdt<2><<<nblk(n_h), TB, 0, s>>>(n_h, hbox, o2s, vol_u, gam, cfl, rho, vel, E, (unsigned long long*)dt_box,
+                                          wgen, cm);
     else
-        k_bdt<3><<<nblk(n_h), TB, 0, s>>>(n_h, hbox, o2s, vol_u, gam, cfl, rho, vel, E, (unsigned long long*)dt_box);
+        k_bdt<3><<<nblk(n_h), TB, 0, s>>>(n_h, hbox, o2s, vol_u, gam, cfl, rho, vel, E, (unsigned long long*)dt_box,
+                                          wgen, cm);
 }
 
 void launch_bhll(int dim, const int64_t* n_dev, int64_t cap, int64_t stride, int64_t nplane, const int32_t* hsorted,
                  const int32_t* hbox, const uint8_t* count, const int32_t* hid, const int32_t* nbr, const double* nrm,
                  const double* area_u, const double* cen, const double* sorted, double scale, double gam,
-                 const double* dt_box, int flux_mode, int move, SimView sv, hipStream_t s) {
+                 const double* dt_box, int flux_mode, int move, SimView sv, hipStream_t s, const double* wgen) {
     const unsigned gb = std::min<unsigned>(nblk(cap), 4096);
     if (dim == 2)
         k_bhll<2><<<gb, TB, 0, s>>>(n_dev, stride, nplane, hsorted, hbox, count, hid, nbr, nrm, area_u, cen, sorted,
-                                    scale, gam, dt_box, flux_mode, move, sv);
+                                    scale, gam, dt_box, flux_mode, move, sv, wgen);
     else
         k_bhll<3><<<gb, TB, 0, s>>>(n_dev, stride, nplane, hsorted, hbox, count, hid, nbr, nrm, area_u, cen, sorted,
-                                    scale, gam, dt_box, flux_mode, move, sv);
+                                    scale, gam, dt_box, flux_mode, move, sv, wgen);
 }
 
 void launch_bdrift(int dim, int64_t n_h, const int32_t* hbox, double* x, const double* vel, const double* dt_box,

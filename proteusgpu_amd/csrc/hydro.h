@@ -18,12 +18,27 @@ struct SimView {
     double* E;    // energy per volume
 };
 
+// Mesh data for the CFL length of a moving mesh.  count == nullptr: the
+// reference's length (radius of the cell volume) alone.
+struct CflMesh {
+    const uint8_t* count = nullptr;  // faces per sorted cell
+    const int32_t* nbr = nullptr;    // sorted neighbour id per face (face-major, stride)
+    const double* sorted = nullptr;  // sorted generators (scaled box)
+    int64_t stride = 0;
+    double scale = 1.0;
+};
+
 void launch_q_from_prim(int dim, SimView sv, const int32_t* o2s, const double* vol_u, hipStream_t s);
 void launch_prim_from_q(int dim, SimView sv, const int32_t* o2s, const double* vol_u, hipStream_t s);
 void launch_hll_ale(int dim, int64_t n, int64_t n_hydro, int64_t stride, int64_t nplane, const uint32_t* perm,
                     const uint8_t* count, const int32_t* hid, const int32_t* nbr, const double* nrm,
                     const double* area_u, const double* cen, const double* sorted, double scale, double gam,
-                    const double* dt_dev, double dt_max, int flux_mode, int move, SimView sv, hipStream_t s);
+                    const double* dt_dev, double dt_max, int flux_mode, int move, SimView sv, hipStream_t s,
+                    const double* wgen = nullptr);
+// Steered generator velocities (regularised mesh motion), hydro order.
+void launch_gen_velocity(int dim, int64_t n_h, const double* x, const int32_t* o2s, const double* vol_u,
+                         const double* ccen, int64_t stride, double scale, double gam, const double* rho,
+                         const double* vel, const double* E, double* w, hipStream_t s);
 void launch_drift(int dim, int64_t n, double* x, const double* vel, const double* dt_dev, double dt_max,
                   int periodic, hipStream_t s);
 int64_t totals_blocks(int64_t n);
@@ -43,7 +58,7 @@ void launch_hll_static(int dim, int64_t n, int64_t n_hydro, int64_t stride, int6
 
 void launch_dt_cfl(int dim, int64_t n_hydro, const int32_t* o2s, const double* vol_u, double gam,
                    double cfl, const double* rho, const double* vel, const double* E,
-                   unsigned long long* out, hipStream_t s);
+                   unsigned long long* out, hipStream_t s, const double* wgen = nullptr, CflMesh cm = {});
 
 void launch_counts_orig(int64_t n, const uint32_t* perm, const uint8_t* count, int32_t* co,
                         hipStream_t s);
@@ -67,11 +82,12 @@ void launch_bfinalize(const int64_t* n_dev, int64_t cap, int64_t stride, const u
                       int32_t* hsorted, int32_t* bad, hipStream_t s);
 void launch_bdt(int dim, int64_t n_h, int nbox, const int32_t* hbox, const int32_t* o2s, const double* vol_u,
                 double gam, double cfl, const double* rho, const double* vel, const double* E, double* dt_box,
-                hipStream_t s);
+                hipStream_t s, const double* wgen = nullptr, CflMesh cm = {});
 void launch_bhll(int dim, const int64_t* n_dev, int64_t cap, int64_t stride, int64_t nplane, const int32_t* hsorted,
                  const int32_t* hbox, const uint8_t* count, const int32_t* hid, const int32_t* nbr, const double* nrm,
                  const double* area_u, const double* cen, const double* sorted, double scale, double gam,
-                 const double* dt_box, int flux_mode, int move, SimView sv, hipStream_t s);
+                 const double* dt_box, int flux_mode, int move, SimView sv, hipStream_t s,
+                 const double* wgen = nullptr);
 void launch_bdrift(int dim, int64_t n_h, const int32_t* hbox, double* x, const double* vel, const double* dt_box,
                    int periodic, hipStream_t s);
 void launch_btotals(int dim, int nbox, const int64_t* hoff, SimView sv, double* out, hipStream_t s);

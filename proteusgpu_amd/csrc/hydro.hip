@@ -109,7 +109,7 @@ template <int DIM>
 __global__ void __launch_bounds__(TB)
     k_dt_cfl(int64_t n_hydro, const int32_t* __restrict__ o2s, const double* __restrict__ vol_u,
              double gam, double cfl, const double* __restrict__ rho, const double* __restrict__ vel,
-             const double* __restrict__ E, unsigned long long* out) {
+             const double* __restrict__ E, unsigned long long* out, const double* __restrict__ wgen, CflMesh cm) {
     int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
     double dti = 1e100;
     if (i < n_hydro) {
@@ -121,15 +121,10 @@ __global__ void __launch_bounds__(TB)
         for (int d = 0; d < DIM; d++) s.v[d] = vel[(size_t)DIM * i + d];
         const double P = pressure<DIM>(gam, s);
         const double c = sqrt(gam * P / s.rho);
-        const double V = vol_u[o2s[i]];
-        double R, va;
-        if (DIM == 2) {
-            R = sqrt(V / M_PI);
-            va = sqrt(s.v[0] * s.v[0] + s.v[1] * s.v[1]);
-        } else {
-            R = cbrt(3.0 * V / (4.0 * M_PI));
-            va = sqrt(s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]);
-        }
+        const int32_t si = o2s[i];
+        const double R = cfl_length<DIM>(vol_u[si], si, cm);
+        double va = sqrt(s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]);
+        if (wgen) va = fmax(va, rel_speed<DIM>(s.v, wgen + (size_t)DIM * i));
         dti = cfl * (R / (c + va));
     }
     // wave min, then one atomic per wave (non-negative doubles order as bits)
@@ -184,7 +179,7 @@ __global__ void __launch_bounds__(TB)
               const int32_t* __restrict__ nbr, const double* __restrict__ nrm,
               const double* __restrict__ area_u, const double* __restrict__ cen,
               const double* __restrict__ sorted, double scale, double gam, const double* dt_dev,
-              double dt_max, int flux_mode, int move, SimView sv) {
+              double dt_max, int flux_mode, int move, SimView sv, const double* __restrict__ wgen) {
     int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x;
     if (s >= n) return;
     const uint32_t i = perm[s];
@@ -193,11 +188,54 @@ __global__ void __launch_bounds__(TB)
     if (dt_max > 0 && dt > dt_max) dt = dt_max;
     double fm, fp[3], fe;
     ale_cell_flux<DIM>(s, i, stride, nplane, count, hid, nbr, nrm, area_u, cen, sorted, scale, gam, flux_mode,
-                       move, sv, fm, fp, fe);
+                       move, sv, wgen, fm, fp, fe);
     sv.mass[i] -= dt * fm;
 #pragma unroll
     for (int d = 0; d < DIM; d++) sv.mom[(size_t)DIM * i + d] -= dt * fp[d];
     sv.en[i] -= dt * fe;
+}
+
+// Steered generator velocity (mesh regularisation, Springel 2010, MNRAS 401,
+// 791, sec. 4.2): w = v + chi f (c - r) / d, with c the cell centroid, d =
+// |c - r|, R the cell radius of the cell volume, c_s the sound speed and
+// f = 0 for d < 0.9 eta R, c_s (d - 0.9 eta R) / (0.2 eta R) up to
+// 1.1 eta R, c_s beyond (eta = 0.25, chi = 1).  Generators drift towards
+// their centroids once cells grow lopsided, which keeps them round.  No
+// reference counterpart.
+template <int DIM>
+__global__ void __launch_bounds__(TB)
+    k_gen_velocity(int64_t n_h, const double* __restrict__ x, const int32_t* __restrict__ o2s,
+                   const double* __restrict__ vol_u, const double* __restrict__ ccen, int64_t stride, double scale,
+                   double gam, const double* __restrict__ rho, const double* __restrict__ vel,
+                   const double* __restrict__ E, double* __restrict__ w) {
+    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i >= n_h) return;
+    const int32_t s = o2s[i];
+    const double V = vol_u[s];
+    const double R = DIM == 2 ? sqrt(V / M_PI) : cbrt(3.0 * V / (4.0 * M_PI));
+    Prim st;
+    st.rho = rho[i];
+    st.E = E[i];
+    st.v[2] = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) st.v[d] = vel[(size_t)DIM * i + d];
+    const double P = pressure<DIM>(gam, st);
+    const double cs = (P > 0.0 && st.rho > 0.0) ? sqrt(gam * P / st.rho) : 0.0;
+    double dc[3] = {0, 0, 0}, dist = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        const double c = (ccen[(int64_t)d * stride + s] - 0.5) * scale + 0.5;
+        dc[d] = c - x[(size_t)DIM * i + d];
+        dist += dc[d] * dc[d];
+    }
+    dist = sqrt(dist);
+    const double eta = 0.25, chi = 1.0;
+    const double lo = 0.9 * eta * R, hi = 1.1 * eta * R;
+    double f = 0.0;
+    if (dist > hi) f = cs;
+    else if (dist > lo) f = cs * (dist - lo) / (hi - lo);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) w[(size_t)DIM * i + d] = st.v[d] + (dist > 0.0 ? chi * f * dc[d] / dist : 0.0);
 }
 
 // Generators move with the fluid: x += v dt, wrapped (periodic) or mirrored
@@ -341,12 +379,20 @@ void launch_hll_static(int dim, int64_t n, int64_t n_hydro, int64_t stride, int6
 
 void launch_dt_cfl(int dim, int64_t n_hydro, const int32_t* o2s, const double* vol_u, double gam,
                    double cfl, const double* rho, const double* vel, const double* E,
-                   unsigned long long* out, hipStream_t s) {
+                   unsigned long long* out, hipStream_t s, const double* wgen, CflMesh cm) {
     if (n_hydro <= 0) return;
     if (dim == 2)
-        k_dt_cfl<2><<<nblk(n_hydro), TB, 0, s>>>(n_hydro, o2s, vol_u, gam, cfl, rho, vel, E, out);
+        k_dt_cfl<2><<<nblk(n_hydro), TB, 0, s>>>(n_hydro, o2s, vol_u, gam, cfl, rho, vel, E, out, wgen, cm);
     else
-        k_dt_cfl<3><<<nblk(n_hydro), TB, 0, s>>>(n_hydro, o2s, vol_u, gam, cfl, rho, vel, E, out);
+        k_dt_cfl<3><<<nblk(n_hydro), TB, 0, s>>>(n_hydro, o2s, vol_u, gam, cfl, rho, vel, E, out, wgen, cm);
+}
+
+void launch_gen_velocity(int dim, int64_t n_h, const double* x, const int32_t* o2s, const double* vol_u,
+                         const double* ccen, int64_t stride, double scale, double gam, const double* rho,
+                         const double* vel, const double* E, double* w, hipStream_t s) {
+    if (n_h <= 0) return;
+    if (dim == 2) k_gen_velocity<2><<<nblk(n_h), TB, 0, s>>>(n_h, x, o2s, vol_u, ccen, stride, scale, gam, rho, vel, E, w);
+    else k_gen_velocity<3><<<nblk(n_h), TB, 0, s>>>(n_h, x, o2s, vol_u, ccen, stride, scale, gam, rho, vel, E, w);
 }
 
 void launch_q_from_prim(int dim, SimView sv, const int32_t* o2s, const double* vol_u, hipStream_t s) {
@@ -364,14 +410,15 @@ void launch_prim_from_q(int dim, SimView sv, const int32_t* o2s, const double* v
 void launch_hll_ale(int dim, int64_t n, int64_t n_hydro, int64_t stride, int64_t nplane, const uint32_t* perm,
                     const uint8_t* count, const int32_t* hid, const int32_t* nbr, const double* nrm,
                     const double* area_u, const double* cen, const double* sorted, double scale, double gam,
-                    const double* dt_dev, double dt_max, int flux_mode, int move, SimView sv, hipStream_t s) {
+                    const double* dt_dev, double dt_max, int flux_mode, int move, SimView sv, hipStream_t s,
+                    const double* wgen) {
     if (n <= 0) return;
     if (dim == 2)
         k_hll_ale<2><<<nblk(n), TB, 0, s>>>(n, n_hydro, stride, nplane, perm, count, hid, nbr, nrm, area_u, cen,
-                                            sorted, scale, gam, dt_dev, dt_max, flux_mode, move, sv);
+                                            sorted, scale, gam, dt_dev, dt_max, flux_mode, move, sv, wgen);
     else
         k_hll_ale<3><<<nblk(n), TB, 0, s>>>(n, n_hydro, stride, nplane, perm, count, hid, nbr, nrm, area_u, cen,
-                                            sorted, scale, gam, dt_dev, dt_max, flux_mode, move, sv);
+                                            sorted, scale, gam, dt_dev, dt_max, flux_mode, move, sv, wgen);
 }
 
 void launch_drift(int dim, int64_t n, double* x, const double* vel, const double* dt_dev, double dt_max,

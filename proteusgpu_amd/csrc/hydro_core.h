@@ -118,6 +118,41 @@ PG_DEV Prim hll(double gam, Prim L, Prim R, const Geom& g, bool wall) {
     return F;
 }
 
+// CFL length of sorted cell s with volume V: the radius of the volume
+// (finite_volume_solver.cpp:146-150); with regularised mesh motion also at
+// most the distance to the nearest face neighbour, so that slivers (two
+// generators close together: small width, large radius) limit the step as
+// well.  Round cells have a nearest neighbour beyond R (2D hexagons 1.9 R,
+// 3D cubes 1.6 R), so the limit acts on slivers only.
+template <int DIM>
+PG_DEV double cfl_length(double V, int32_t s, const CflMesh& cm) {
+    double R = DIM == 2 ? sqrt(V / M_PI) : cbrt(3.0 * V / (4.0 * M_PI));
+    if (cm.count == nullptr) return R;
+    const int c = cm.count[s];
+    double dmin = 1e300;
+    for (int f = 0; f < c; f++) {
+        const int32_t js = cm.nbr[(int64_t)f * cm.stride + s];
+        if (js < 0) continue;
+        double dd = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const double t = cm.sorted[(size_t)DIM * js + d] - cm.sorted[(size_t)DIM * s + d];
+            dd += t * t;
+        }
+        dmin = fmin(dmin, dd);
+    }
+    return fmin(R, sqrt(dmin) * cm.scale);
+}
+
+// |v - w| (signal speed of the fluid relative to a moving generator)
+template <int DIM>
+PG_DEV double rel_speed(const double* v, const double* w) {
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) s += (v[d] - w[d]) * (v[d] - w[d]);
+    return sqrt(s);
+}
+
 // Fluxes through the faces of sorted cell s (hydro cell i), integrated
 // over the face areas; see k_hll_ale.
 template <int DIM>
@@ -125,8 +160,8 @@ PG_DEV void ale_cell_flux(int64_t s, uint32_t i, int64_t stride, int64_t nplane,
                           const int32_t* __restrict__ hid, const int32_t* __restrict__ nbr,
                           const double* __restrict__ nrm, const double* __restrict__ area_u,
                           const double* __restrict__ cen, const double* __restrict__ sorted, double scale,
-                          double gam, int flux_mode, int move, const SimView& sv, double& fm, double (&fp)[3],
-                          double& fe) {
+                          double gam, int flux_mode, int move, const SimView& sv, const double* __restrict__ wgen,
+                          double& fm, double (&fp)[3], double& fe) {
     Prim si;
     si.rho = sv.rho[i];
     si.E = sv.E[i];
@@ -136,6 +171,11 @@ PG_DEV void ale_cell_flux(int64_t s, uint32_t i, int64_t stride, int64_t nplane,
     double ri[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < DIM; d++) ri[d] = (sorted[(size_t)DIM * s + d] - 0.5) * scale + 0.5;
+    // generator velocities: the fluid velocity, or the steered one (wgen)
+    auto gvel = [&](int32_t h, int d) { return wgen ? wgen[(size_t)DIM * h + d] : sv.vel[(size_t)DIM * h + d]; };
+    double gi[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; d++) gi[d] = gvel((int32_t)i, d);
     fm = 0.0;
     fp[0] = fp[1] = fp[2] = 0.0;
     fe = 0.0;
@@ -163,10 +203,10 @@ PG_DEV void ale_cell_flux(int64_t s, uint32_t i, int64_t stride, int64_t nplane,
                     mid[d] = 0.5 * (ri[d] + rj[d]);
                     dd += dr[d] * dr[d];
                     const double fc = (cen[(int64_t)d * nplane + at] - 0.5) * scale + 0.5;
-                    proj += (si.v[d] - sj.v[d]) * (fc - mid[d]);
+                    proj += (gi[d] - gvel(j, d)) * (fc - mid[d]);
                 }
 #pragma unroll
-                for (int d = 0; d < DIM; d++) w[d] = 0.5 * (si.v[d] + sj.v[d]) + proj * dr[d] / dd;
+                for (int d = 0; d < DIM; d++) w[d] = 0.5 * (gi[d] + gvel(j, d)) + proj * dr[d] / dd;
             }
         }
         // total energy per volume is frame dependent: boost it with the velocity

@@ -66,8 +66,11 @@ void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* se
                     const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa,
                     hipStream_t s, bool restart = false);
 
-// Vertex slots per seed of the fast clip kernel (48 or 64).
-void set_clip_slots(int slots);
+// Shape of the fast clip kernel: lanes per seed (8 or 16) and vertex slots
+// per seed (48 or 64).
+bool valid_clip_config(int lanes, int slots);
+void set_clip_config(int lanes, int slots);
+int clip_lanes();
 int clip_slots();
 // Collect the sorted ids whose status is not SUCCESS (order: ascending id).
 // nfail (device int32) receives the count; tmp as for scan.
@@ -142,7 +145,7 @@ void launch_bknn(int dim, const double* sorted, const uint32_t* perm, const int3
                  const int64_t* nq_dev, int64_t cap, int k, uint32_t* out, int wave_min_k, int blocks_per_cu,
                  hipStream_t s);
 void launch_bvoronoi(int dim, const double* sorted, const uint32_t* seeds, const int64_t* nq_dev, int64_t cap,
-                     const uint32_t* knn, int k, int max_p, int max_t, int big, int slots, FaceArrays fa,
+                     const uint32_t* knn, int k, int max_p, int max_t, int big, int lanes, int slots, FaceArrays fa,
                      int blocks_per_cu, hipStream_t s);
 void launch_bcollect(const int64_t* nq_dev, int64_t cap, const uint32_t* seeds, const int32_t* status, int code,
                      uint32_t* list, int32_t* flags, int32_t* offs, int32_t* tmp, int32_t* total, int64_t* nout,

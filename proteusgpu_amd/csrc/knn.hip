@@ -83,7 +83,7 @@ void launch_knn(int dim, const double* sorted, const uint32_t* perm, int64_t n, 
     // threshold can be moved with PGPU_KNN_WAVE_MIN_K for tuning runs.
     static const int wave_min_k = [] {
         const char* e = getenv("PGPU_KNN_WAVE_MIN_K");
-        return e ? atoi(e) : 33;
+        return e ? atoi(e) : 20;
     }();
     if (k >= wave_min_k) {
         if (dim == 2) dispatch_wave<2>(sorted, perm, N, starts, counts, queries, nq, k, out, s);

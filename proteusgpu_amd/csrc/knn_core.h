@@ -6,6 +6,30 @@
 
 namespace pg {
 
+// Lower bound of the squared distance from a query to any point stored in
+// grid cell (ii, jj, kk).  P = the query's coordinates times N, the same
+// product grid_cell() floors, so a point of cell i has P_q in [i, i + 1)
+// per axis; boundary cells are open towards the outside (points beyond
+// [0, 1) are clamped into them).  A slack of 1e-9 cell widths and a factor
+// 1 - 1e-12 keep the bound below every candidate's rounded distance, so a
+// cell whose bound exceeds the current k-th distance holds no candidate
+// that could enter the row: skipping it leaves the row unchanged (the
+// reference's ring rule, knn.cpp:272, still decides when the walk stops).
+template <int DIM>
+PG_DEV double cell_lower_d2(const double (&P)[3], int ii, int jj, int kk, int N) {
+    const int idx[3] = {ii, jj, kk};
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        const double lo = idx[d] == 0 ? -1e300 : (double)idx[d];
+        const double hi = idx[d] == N - 1 ? 1e300 : (double)(idx[d] + 1);
+        const double g = fmax(fmax(lo - P[d], P[d] - hi) - 1e-9, 0.0);
+        s += g * g;
+    }
+    const double n = (double)N;
+    return s / (n * n) * (1.0 - 1e-12);
+}
+
 // One query of the per-thread heap search; the heap lives in LDS at
 // hd/hi[entry * BLOCK + threadIdx.x].  Grid cells of this query's box are
 // cbase + i + j N + k N^2.
@@ -66,7 +90,11 @@ PG_DEV void knn_heap_query(const double* __restrict__ sorted, const uint32_t* __
             HI(j) = id;
         }
     };
+    double P[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; d++) P[d] = p[d] * (double)N;
     auto scan_cell = [&](int ii, int jj, int kk) {
+        if (size == k && cell_lower_d2<DIM>(P, ii, jj, kk, N) > HD(0)) return;
         const int64_t cc = cbase + ii + jj * N + (int64_t)kk * N * N;
         const int32_t b = starts[cc], e = b + counts[cc];
         for (int32_t s = b; s < e; s++) {
@@ -134,12 +162,14 @@ PG_DEV void knn_heap_query(const double* __restrict__ sorted, const uint32_t* __
 
 // ------------------------------------------------------------------------
 // Wave-per-query variant.  The 64 lanes of a wave cooperate on one query:
-// the k best (dist^2, original id, sorted id) are kept sorted in registers,
-// slot-major (rank r at slot r / 64 of lane r % 64, M = ceil(k/64) slots).
-// Ring cells are enumerated 64 at a time, their point ranges prefix-summed
-// across the wave, and candidates processed 64 at a time: a candidate is
-// kept if it beats the current k-th entry (broadcast), and every kept
-// candidate is inserted by one cross-lane shift of the ranks above it.
+// the best 64 M (dist^2, original id, sorted id) are kept sorted in
+// registers, slot-major (rank r at slot r / 64 of lane r % 64, M =
+// ceil(k/64) slots).  Ring cells are enumerated 64 at a time, their point
+// ranges prefix-summed across the wave, and candidates processed 64 at a
+// time: those that beat the current k-th entry (broadcast) are sorted
+// across the wave by a bitonic network and merged into the kept list, one
+// 64-entry slot at a time (lower half of slot + reversed batch stays, the
+// upper half carries into the next slot).
 // Same pruning rule and same total order as k_knn, no LDS heap, full waves:
 // used for long rows (certificate restarts), where the per-thread heap would
 // leave the CUs nearly empty.
@@ -150,6 +180,33 @@ struct Cand {
 
 PG_DEV bool cand_less(double da, uint32_t oa, double db, uint32_t ob) {
     return da < db || (da == db && oa < ob);
+}
+
+// Compare-exchange with lane l ^ j: keep the smaller (keep_min) or the
+// larger of the pair under cand_less.
+PG_DEV void wave_cx(double& d, uint32_t& o, uint32_t& s, int j, bool keep_min) {
+    const double pd = __shfl_xor(d, j, 64);
+    const uint32_t po = __shfl_xor(o, j, 64), ps = __shfl_xor(s, j, 64);
+    const bool other_less = cand_less(pd, po, d, o);
+    if (other_less == keep_min) {
+        d = pd;
+        o = po;
+        s = ps;
+    }
+}
+
+// Bitonic sort of one entry per lane, ascending with the lane index.
+PG_DEV void wave_sort64(double& d, uint32_t& o, uint32_t& s, int l) {
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) wave_cx(d, o, s, j, ((l & j) == 0) == ((l & k) == 0 || k == 64));
+}
+
+// Sort a bitonic sequence held one entry per lane (ascending).
+PG_DEV void wave_merge64(double& d, uint32_t& o, uint32_t& s, int l) {
+#pragma unroll
+    for (int j = 32; j > 0; j >>= 1) wave_cx(d, o, s, j, (l & j) == 0);
 }
 
 // One query by the 64 lanes of a one-wave block; c_start / c_pfx are the
@@ -164,6 +221,9 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
     double p[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < DIM; d++) p[d] = sorted[(size_t)DIM * q + d];
+    double P[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; d++) P[d] = p[d] * (double)N;
     const int c = grid_cell<DIM>(p, N);
     const int ci = c % N, cj = (c / N) % N, ck = DIM == 3 ? c / (N * N) : 0;
 
@@ -193,12 +253,20 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
             // this lane's cell of the cube, if it lies on the ring and in the grid
             const int t = base + l;
             int cnt = 0, st = 0;
+            // cells wholly beyond the current k-th distance are skipped
+            double prune = 1e300;
+            if (size == k) {
+#pragma unroll
+                for (int j = 0; j < M; j++)
+                    if (j == kj) prune = __shfl(kd[j], kl, 64);
+            }
             if (t < ncube) {
                 const int di = t % side - ring, dj = (t / side) % side - ring;
                 const int dk = DIM == 3 ? t / (side * side) - ring : 0;
                 const int cheb = max(abs(di), max(abs(dj), abs(dk)));
                 const int ii = ci + di, jj = cj + dj, kk = ck + dk;
-                if (cheb == ring && ii >= 0 && jj >= 0 && kk >= 0 && ii < N && jj < N && (DIM == 2 || kk < N)) {
+                if (cheb == ring && ii >= 0 && jj >= 0 && kk >= 0 && ii < N && jj < N && (DIM == 2 || kk < N) &&
+                    cell_lower_d2<DIM>(P, ii, jj, kk, N) <= prune) {
                     const int64_t cc = cbase + ii + jj * N + (int64_t)kk * N * N;
                     st = starts[cc];
                     cnt = counts[cc];
@@ -250,61 +318,36 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
                         }
                     take = take && cand_less(dd, co, kth, kto);
                 }
-                uint64_t surv = (uint64_t)__ballot(take);
-                while (surv) {
-                    const int src = __ffsll((unsigned long long)surv) - 1;
-                    surv &= surv - 1;
-                    const double xd = __shfl(dd, src, 64);
-                    const uint32_t xo = __shfl(co, src, 64), xs = __shfl(cs, src, 64);
-                    if (size == k) {
-                        // it may no longer beat the k-th entry after earlier inserts
-                        double kth = 0;
-                        uint32_t kto = 0;
-#pragma unroll
-                        for (int jj = 0; jj < M; jj++)
-                            if (jj == kj) {
-                                kth = __shfl(kd[jj], kl, 64);
-                                kto = __shfl(ko[jj], kl, 64);
-                            }
-                        if (!cand_less(xd, xo, kth, kto)) continue;
-                    }
-                    // insertion rank = number of entries below the candidate
-                    int pos = 0;
-#pragma unroll
-                    for (int jj = 0; jj < M; jj++) {
-                        const bool below = (jj * 64 + l < size) && cand_less(kd[jj], ko[jj], xd, xo);
-                        pos += __popcll((unsigned long long)__ballot(below));
-                    }
-                    // shift ranks >= pos up by one (slot-major), insert at pos
-                    double carry_d = 0;
-                    uint32_t carry_o = 0, carry_s = 0;
-#pragma unroll
-                    for (int jj = 0; jj < M; jj++) {
-                        const double up_d = __shfl_up(kd[jj], 1, 64);
-                        const uint32_t up_o = __shfl_up(ko[jj], 1, 64), up_s = __shfl_up(ks[jj], 1, 64);
-                        const double top_d = __shfl(kd[jj], 63, 64);
-                        const uint32_t top_o = __shfl(ko[jj], 63, 64), top_s = __shfl(ks[jj], 63, 64);
-                        const int r = jj * 64 + l;
-                        double nd = kd[jj];
-                        uint32_t no = ko[jj], ns = ks[jj];
-                        if (r == pos) {
-                            nd = xd;
-                            no = xo;
-                            ns = xs;
-                        } else if (r > pos) {
-                            nd = l > 0 ? up_d : carry_d;
-                            no = l > 0 ? up_o : carry_o;
-                            ns = l > 0 ? up_s : carry_s;
-                        }
-                        carry_d = top_d;
-                        carry_o = top_o;
-                        carry_s = top_s;
-                        kd[jj] = nd;
-                        ko[jj] = no;
-                        ks[jj] = ns;
-                    }
-                    if (size < k) size++;
+                const uint64_t surv = (uint64_t)__ballot(take);
+                if (surv == 0ull) continue;
+                // merge the batch into the kept list: sort the 64 candidates
+                // across the wave (bitonic), then, slot by slot, the lower
+                // half of (slot, batch) stays and the upper half carries on
+                if (!take) {
+                    dd = __builtin_inf();
+                    co = 0xffffffffu;
+                    cs = 0xffffffffu;
                 }
+                wave_sort64(dd, co, cs, l);
+#pragma unroll
+                for (int jj = 0; jj < M; jj++) {
+                    const double rd = __shfl(dd, 63 - l, 64);
+                    const uint32_t ro = __shfl(co, 63 - l, 64), rs = __shfl(cs, 63 - l, 64);
+                    const bool mine = cand_less(kd[jj], ko[jj], rd, ro);
+                    double hd = mine ? rd : kd[jj];
+                    uint32_t ho = mine ? ro : ko[jj], hs = mine ? rs : ks[jj];
+                    if (!mine) {
+                        kd[jj] = rd;
+                        ko[jj] = ro;
+                        ks[jj] = rs;
+                    }
+                    wave_merge64(kd[jj], ko[jj], ks[jj], l);
+                    wave_merge64(hd, ho, hs, l);
+                    dd = hd;
+                    co = ho;
+                    cs = hs;
+                }
+                size = min(k, size + __popcll((unsigned long long)surv));
             }
             __syncthreads();
         }

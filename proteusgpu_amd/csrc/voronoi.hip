@@ -1,4 +1,5 @@
 // Kernel entry points for the Voronoi clip and the restart bookkeeping.
+#include <cstdio>
 #include <cstdlib>
 
 #include "voronoi_kernel.h"
@@ -68,32 +69,49 @@ __global__ void __launch_bounds__(TB) k_compact(int64_t nq, const uint32_t* seed
 // Vertex slots of the fast kernel per seed: 16 lanes x 4 (64 slots) or
 // 16 lanes x 3 (48 slots; less work per clip, more seeds overflow into the
 // 128-slot kernel).  Chosen by the autotuner or PGPU_CLIP_SLOTS.
-static int g_clip_slots = [] {
-    const char* e = getenv("PGPU_CLIP_SLOTS");
-    return (e && atoi(e) == 48) ? 3 : 4;
+// Fast-kernel shape: lanes per seed x vertex slots per lane.  16 x 4 (64
+// slots, 4 seeds per wave), 16 x 3 (48 slots; less work per clip, a few
+// more seeds overflow into the 128-slot kernel), 8 x 6 and 8 x 8 (8 seeds
+// per wave: the group-wide work of a clip is shared by twice as many seeds,
+// at twice the LDS per wave).  Chosen by the autotuner or PGPU_CLIP_CFG
+// ("16x4", "16x3", "8x6", "8x8").
+static int g_clip_lanes = 16, g_clip_slots = 64;
+static const bool g_clip_env = [] {
+    const char* e = getenv("PGPU_CLIP_CFG");
+    int l = 0, sl = 0;
+    if (e && sscanf(e, "%dx%d", &l, &sl) == 2) set_clip_config(l, l * sl);
+    const char* e2 = getenv("PGPU_CLIP_SLOTS");
+    if (e2 && atoi(e2) == 48) set_clip_config(16, 48);
+    return true;
 }();
-void set_clip_slots(int slots) { g_clip_slots = slots == 48 ? 3 : 4; }
-int clip_slots() { return g_clip_slots * 16; }
+bool valid_clip_config(int lanes, int slots) {
+    return (lanes == 16 || lanes == 8) && (slots == 48 || slots == 64);
+}
+void set_clip_config(int lanes, int slots) {
+    if (!valid_clip_config(lanes, slots)) return;
+    g_clip_lanes = lanes;
+    g_clip_slots = slots;
+}
+int clip_lanes() { return g_clip_lanes; }
+int clip_slots() { return g_clip_slots; }
+
+template <int DIM>
+void launch_dim(const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq, const uint32_t* knn, int k,
+                int max_p, int max_t, int big, FaceArrays fa, hipStream_t s, bool restart, int lanes, int slots) {
+    if (big) run<DIM, 16, 8>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, true, s);
+    else if (lanes == 8 && slots == 48) run<DIM, 8, 6>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
+    else if (lanes == 8) run<DIM, 8, 8>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
+    else if (slots == 48) run<DIM, 16, 3>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
+    else run<DIM, 16, 4>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
+}
 
 void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq,
                     const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa,
                     hipStream_t s, bool restart) {
     if (nq <= 0) return;
-    if (dim == 2) {
-        if (big)
-            run<2, 16, 8>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, true, s);
-        else if (g_clip_slots == 3)
-            run<2, 16, 3>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
-        else
-            run<2, 16, 4>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
-    } else {
-        if (big)
-            run<3, 16, 8>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, true, s);
-        else if (g_clip_slots == 3)
-            run<3, 16, 3>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
-        else
-            run<3, 16, 4>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
-    }
+    (void)g_clip_env;
+    if (dim == 2) launch_dim<2>(sorted, n, seeds, nq, knn, k, max_p, max_t, big, fa, s, restart, g_clip_lanes, g_clip_slots);
+    else launch_dim<3>(sorted, n, seeds, nq, knn, k, max_p, max_t, big, fa, s, restart, g_clip_lanes, g_clip_slots);
 }
 
 void launch_collect_status(int64_t nq, const uint32_t* seeds, const int32_t* status, int code,

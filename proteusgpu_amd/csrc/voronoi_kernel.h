@@ -65,7 +65,7 @@ struct SeedSmem {
     uint8_t newslot[NS];    // slot of the j-th new vertex
     uint8_t by_start[MAXP]; // 3D: boundary edge starting at a plane
     uint8_t by_end[MAXP];   // 3D: boundary edge ending at a plane
-    uint32_t first[MAXP];   // extraction: lowest vertex slot on each plane
+    uint32_t first[MAXP];   // extraction: vertex of each plane with the lowest plane key
     uint32_t nbi[32];       // next chunk of the neighbour row (prefetched)
     double nbp[32][3];      // and the neighbours' positions
 };
@@ -78,6 +78,19 @@ PG_DEV float f2f_up(double d) {
     return f;
 }
 PG_DEV int tri_p(uint32_t t, int e) { return (int)((t >> (8 * e)) & 0xffu); }
+
+// Slot-independent identity of a vertex: its planes, sorted, packed in 24
+// bits (16 in 2D).
+template <int DIM>
+PG_DEV uint32_t vertex_key(uint32_t t) {
+    uint32_t a = t & 0xffu, b = (t >> 8) & 0xffu;
+    if (DIM == 2) return a < b ? (a << 8) | b : (b << 8) | a;
+    uint32_t c = (t >> 16) & 0xffu, x;
+    if (a > b) { x = a; a = b; b = x; }
+    if (b > c) { x = b; b = c; c = x; }
+    if (a > b) { x = a; a = b; b = x; }
+    return (a << 16) | (b << 8) | c;
+}
 
 template <int NW>
 PG_DEV bool mask_has(const uint64_t (&m)[NW], int gs) {
@@ -629,8 +642,12 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             if (!((R.live >> s) & 1u)) continue;
             const int gs = s * G + gl;
             const uint32_t t = sm.tri[gs];
+            // a face's cycle starts at its vertex with the smallest sorted
+            // plane tuple: the same vertex whatever slot it occupies, so the
+            // fan sums round identically for every kernel shape
+            const uint32_t key = vertex_key<D>(t);
 #pragma unroll
-            for (int ee = 0; ee < D; ee++) atomicMin(&sm.first[tri_p(t, ee)], (uint32_t)gs);
+            for (int ee = 0; ee < D; ee++) atomicMin(&sm.first[tri_p(t, ee)], (key << 8) | (uint32_t)gs);
         }
     }
     block_sync();
@@ -645,8 +662,9 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
         }
     }
     block_sync();
-    // faces: walk each plane's vertex cycle through the adjacency
-    double vol = 0.0, cxs = 0.0, cys = 0.0, czs = 0.0;
+    // faces: walk each plane's vertex cycle through the adjacency; per plane
+    // the volume and centroid moments of its fan
+    double pvol[C::PPL], pcx[C::PPL], pcy[C::PPL], pcz[C::PPL];
     bool isface[C::PPL];
     double farea[C::PPL], fcx[C::PPL], fcy[C::PPL], fcz[C::PPL];
     const double* vpos = sm.vpos;
@@ -655,9 +673,11 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
         const int p = gl + G * i;
         isface[i] = false;
         farea[i] = fcx[i] = fcy[i] = fcz[i] = 0.0;
+        pvol[i] = pcx[i] = pcy[i] = pcz[i] = 0.0;
+        double vol = 0.0, cxs = 0.0, cys = 0.0, czs = 0.0;
         if (!(ext && p < nb_v)) continue;
-        const uint32_t v0 = sm.first[p];
-        if (v0 == 0xffffffffu) continue;
+        if (sm.first[p] == 0xffffffffu) continue;
+        const uint32_t v0 = sm.first[p] & 0xffu;
         const double x0 = vpos[3 * v0], y0 = vpos[3 * v0 + 1], z0 = vpos[3 * v0 + 2];
         if (D == 2) {
             const uint32_t t0 = sm.tri[v0];
@@ -673,6 +693,9 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             cxs += ta * (sx + x0 + x1) / 3.0;
             cys += ta * (sy + y0 + y1) / 3.0;
             isface[i] = true;
+            pvol[i] = vol;
+            pcx[i] = cxs;
+            pcy[i] = cys;
         } else {
             int cnt = 0;
             uint32_t cv = v0;
@@ -714,6 +737,10 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
                 const int ee = tri_p(t, 0) == p ? 0 : (tri_p(t, 1) == p ? 1 : 2);
                 cv = sm.adj[ee][cv];
             } while (cv != v0 && cnt <= NS);
+            pvol[i] = vol;
+            pcx[i] = cxs;
+            pcy[i] = cys;
+            pcz[i] = czs;
             if (cnt >= 3) {
                 isface[i] = true;
                 farea[i] = area;
@@ -732,14 +759,28 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
         uint64_t b = (wave_ballot(isface[i]) >> shift) & gmask;
         fm |= b << (G * i);
     }
-    // reduce cell volume and centroid sums over the group
+    // cell volume and centroid moments: a pairwise sum over the plane index,
+    // lowest bit first (lanes hold bits below log2 G, the planes of a lane
+    // the bits above), i.e. the same tree for every kernel shape
 #pragma unroll
-    for (int d = 1; d < G; d <<= 1) {
-        vol += shfl_xor(vol, d, G);
-        cxs += shfl_xor(cxs, d, G);
-        cys += shfl_xor(cys, d, G);
-        czs += shfl_xor(czs, d, G);
-    }
+    for (int i = 0; i < C::PPL; i++)
+#pragma unroll
+        for (int d = 1; d < G; d <<= 1) {
+            pvol[i] += shfl_xor(pvol[i], d, G);
+            pcx[i] += shfl_xor(pcx[i], d, G);
+            pcy[i] += shfl_xor(pcy[i], d, G);
+            pcz[i] += shfl_xor(pcz[i], d, G);
+        }
+#pragma unroll
+    for (int w = 1; w < C::PPL; w <<= 1)
+#pragma unroll
+        for (int i = 0; i < C::PPL; i += 2 * w) {
+            pvol[i] += pvol[i + w];
+            pcx[i] += pcx[i + w];
+            pcy[i] += pcy[i + w];
+            pcz[i] += pcz[i + w];
+        }
+    const double vol = pvol[0], cxs = pcx[0], cys = pcy[0], czs = pcz[0];
     if (valid) {
         const int64_t st = fa.stride;
 #pragma unroll

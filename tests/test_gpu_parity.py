@@ -126,3 +126,17 @@ def test_reflective_box(ctx):
         st.step(gamma, st.dt_cfl(gamma, 0.3), "physical")
     rg, _, _ = st.copy_out()
     assert abs(np.sum(rg * g["volumes"]) - mass0) < 1e-12 * mass0
+
+
+@pytest.mark.parametrize("lanes,slots", [(16, 48), (8, 64), (8, 48)])
+def test_clip_shapes_equal_oracle(ctx, lanes, slots):
+    """Every shape of the fast clip kernel gives the oracle's mesh."""
+    print(f"clip shape {lanes}x{slots}", flush=True)
+    pts = uniform_points(13000, 3, 9)
+    api.set_clip_config(lanes, slots)
+    try:
+        m = api.Mesh(ctx, pts, boundary="none", k=48, restart=False)
+        g = m.copy_out()
+    finally:
+        api.set_clip_config(16, 64)
+    _cmp_mesh(g, O.Mesh(pts, "none", k=48, restart=False))
