@@ -7,7 +7,9 @@
 // cfl (0.4), gamma (5/3), buff (0.04), boundary (periodic|reflective),
 // move_mesh (false: the mesh is built once as in the reference; true: the
 // generators move with the fluid and the mesh is rebuilt every step, all
-// resident in HBM through pg_sim_*).  Learned k (resident runs): learned_k
+// resident in HBM through pg_sim_*), regularize_mesh (false; true with
+// move_mesh: generators also drift towards their cell centroids, which keeps
+// cells round on long runs).  Learned k (resident runs): learned_k
 // (false), mlp_weights (float32 parameter file to load; when it does not
 // exist the classifier is trained on the GPU on the first mesh and, with
 // mlp_weights_out, saved), mlp_train_steps (400).
@@ -36,7 +38,7 @@ void banner() {
 #endif
 }
 
-int run_resident(int n, double t_end, double cfl, int boundary, bool move) {
+int run_resident(int n, double t_end, double cfl, int boundary, bool move, bool regularize) {
     pg_sim_params sp;
     sp.mesh = pgpu_host::mesh_params(boundary, _K_);
     sp.mesh.restart = 1;
@@ -47,7 +49,7 @@ int run_resident(int n, double t_end, double cfl, int boundary, bool move) {
 #else
     sp.flux_mode = move ? PG_FLUX_PHYSICAL : PG_FLUX_REFERENCE;
 #endif
-    sp.move_mesh = move ? 1 : 0;
+    sp.move_mesh = !move ? PG_MOVE_STATIC : (regularize ? PG_MOVE_REGULARIZED : PG_MOVE_FLUID);
     pg_sim* sim = nullptr;
     pgpu_host::check(pg_sim_create(pgpu_host::ctx(), &sp, icData.seedpos.data(), icData.rho.data(),
                                    icData.vel.data(), icData.Energy.data(), n, &sim),
@@ -132,7 +134,8 @@ int main(int argc, char* argv[]) {
         const int boundary = bnd == "reflective" ? PG_BOUNDARY_REFLECTIVE : PG_BOUNDARY_PERIODIC;
         const bool move = input.getBool("move_mesh", false);
         const int n = (int)icData.seedpos_dims[0];
-        if (move || input.getBool("resident", false)) return run_resident(n, t_end, cfl, boundary, move);
+        if (move || input.getBool("resident", false))
+            return run_resident(n, t_end, cfl, boundary, move, input.getBool("regularize_mesh", false));
 
         // the reference's loop (main.cpp:38-84)
         primvars* primvar = hydro::init(n);
