@@ -189,6 +189,14 @@ def main():
     plain = api.Simulation(ctx, pts, rho, vel, E, **kw)
     ms_plain, st_plain = run_steps(ctx, plain, a.warmup, a.steps)
     sched_plain = schedule_ms(ctx, plain)
+    # the same with restarts that clip again from the first neighbour
+    phase("fixed k, host-driven step, restarts from scratch")
+    api.set_clip_resume(False)
+    scratch = api.Simulation(ctx, pts, rho, vel, E, **kw)
+    ms_scratch, st_scratch = run_steps(ctx, scratch, a.warmup, a.steps)
+    api.set_clip_resume(True)
+    state_scratch = scratch.get()
+    scratch.free()
 
     # 2. learned k: the classifier is trained on the GPU, in this run, on the
     #    first mesh of the workload (certified k of every seed as labels)
@@ -229,7 +237,7 @@ def main():
     for _ in range(a.warmup + a.steps):
         chk.step()
     ref = chk.get()
-    for got in (bfix.get(), blearn.get()):
+    for got in (bfix.get(), blearn.get(), state_scratch):
         same = same and all(np.array_equal(x, y) for x, y in zip(got, ref))
     chk.free()
 
@@ -269,12 +277,17 @@ def main():
                    "n_cells": a.n, "k_fixed": a.k},
         "roofline": roof, "cpu_baseline": cpu,
         "learned_k_speedup": round(ms_plain / best, 4),
+        "learned_k_speedup_vs_scratch_restarts": round(ms_scratch / best, 4),
         "results_identical": same,
         "plain_fixed_k": {"ms_per_step": round(ms_plain, 3), "cells_per_s": round(a.n / (ms_plain * 1e-3), 1),
                           "stages_ms": {k: round(v, 3) for k, v in st_plain.items() if k.endswith("_ms")},
                           "restarted_seeds_per_step": st_plain["n_restarted"],
                           "row_entries_per_step": st_plain["row_entries"],
                           "schedule_1rebuild_10hll_ms": round(sched_plain, 3)},
+        "plain_fixed_k_scratch_restarts": {
+            "ms_per_step": round(ms_scratch, 3), "cells_per_s": round(a.n / (ms_scratch * 1e-3), 1),
+            "stages_ms": {k: round(v, 3) for k, v in st_scratch.items() if k.endswith("_ms")},
+            "restarted_seeds_per_step": st_scratch["n_restarted"]},
         "learned_k_host_step": {"ms_per_step": round(ms_lsim, 3),
                                 "stages_ms": {k: round(v, 3) for k, v in st_lsim.items() if k.endswith("_ms")},
                                 "restarted_seeds_per_step": st_lsim["n_restarted"],

@@ -90,6 +90,15 @@ int pg_knn_free(pg_knn** knn);
 int pg_set_clip_config(int lanes, int slots);
 int pg_get_clip_config(int* lanes, int* slots);
 
+/* Certificate restarts (no reference counterpart; results never depend on
+ * it) for every following build and batch creation of this process: 1 =
+ * an uncertified seed's cell is saved (3328 bytes per seed of device memory
+ * while restarts are enabled) and the restart at a longer row continues it;
+ * 0 = the restart clips again from the first neighbour.  Default 1, or
+ * PGPU_CLIP_RESUME. */
+int pg_set_clip_resume(int on);
+int pg_get_clip_resume(int* on);
+
 /* ------------------------------------------------------------------ Voronoi */
 typedef struct {
     int dim;       /* 2 or 3 (Config.sh dim_2D / dim_3D)                          */

@@ -90,6 +90,8 @@ def lib():
         "pg_device_count": ([C.POINTER(C.c_int)], i32),
         "pg_set_clip_config": ([i32, i32], i32),
         "pg_get_clip_config": ([C.POINTER(C.c_int), C.POINTER(C.c_int)], i32),
+        "pg_set_clip_resume": ([i32], i32),
+        "pg_get_clip_resume": ([C.POINTER(C.c_int)], i32),
         "pg_ctx_create": ([i32, pp], i32), "pg_ctx_destroy": ([vp], i32), "pg_ctx_sync": ([vp], i32),
         "pg_stream_copy_probe": ([vp, i64, i32, C.POINTER(C.c_double)], i32),
         "pg_knn_init": ([vp, i32, vp, i64, pp], i32), "pg_knn_solve": ([vp, vp, i32], i32),
@@ -150,7 +152,7 @@ def check(rc):
 
 def exported_symbols():
     """Names of the C ABI entry points this binding knows (for the export test)."""
-    return ["pg_version", "pg_last_error", "pg_device_count", "pg_set_clip_config", "pg_get_clip_config", "pg_ctx_create", "pg_ctx_destroy",
+    return ["pg_version", "pg_last_error", "pg_device_count", "pg_set_clip_config", "pg_get_clip_config", "pg_set_clip_resume", "pg_get_clip_resume", "pg_ctx_create", "pg_ctx_destroy",
             "pg_ctx_sync", "pg_stream_copy_probe", "pg_knn_init", "pg_knn_solve", "pg_knn_info", "pg_knn_get_points",
             "pg_knn_get_knearest", "pg_knn_get_permutation", "pg_knn_free", "pg_mesh_params_default",
             "pg_mesh_compute", "pg_mesh_info", "pg_mesh_copy_out", "pg_mesh_free", "pg_state_create",

@@ -29,6 +29,18 @@ def get_clip_config():
     return lanes.value, slots.value
 
 
+def set_clip_resume(on: bool = True):
+    """Certificate restarts continue the saved cell (True) or clip again from
+    the first neighbour (False), for the following builds; same results."""
+    check(lib().pg_set_clip_resume(int(bool(on))))
+
+
+def get_clip_resume() -> bool:
+    on = C.c_int()
+    check(lib().pg_get_clip_resume(C.byref(on)))
+    return bool(on.value)
+
+
 def device_count() -> int:
     n = C.c_int(0)
     check(lib().pg_device_count(C.byref(n)))

@@ -140,6 +140,8 @@ struct pg_mesh {
     DBuf feat, kcls, klist, kcnt;  // learned k: features, bucket per seed, seeds by bucket, bucket sizes
     // faces (padded, face-slot major) and cells, sorted order
     DBuf nbr, area, nrm, cen, count, vol, ccen, status, kcert;
+    DBuf cstate, resume;  // saved cells of uncertified seeds (restarts resume them)
+    bool use_resume = false;
     // hydro tables
     DBuf hid, area_u, vol_u, o2s, bad;
     bool finalized = false;
@@ -157,6 +159,10 @@ struct pg_mesh {
         fa.ccen = ccen.as<double>();
         fa.status = status.as<int32_t>();
         fa.kcert = kcert.as<int32_t>();
+        if (use_resume) {
+            fa.cstate = cstate.as<uint8_t>();
+            fa.resume = resume.as<int32_t>();
+        }
         return fa;
     }
     double scale() const { return prm.boundary == PG_BOUNDARY_PERIODIC ? 1. + (2 * prm.buff) : 1.0; }
@@ -322,6 +328,11 @@ void mesh_build(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool on_dev
     m->ccen.get<double>(3 * ns);
     m->status.get<int32_t>(ns);
     m->kcert.get<int32_t>(ns);
+    m->use_resume = p.restart && pg::clip_resume();
+    if (m->use_resume) {
+        m->cstate.get<uint8_t>(ns * pg::CLIP_STATE_BYTES);
+        m->resume.get<int32_t>(ns);
+    }
     m->finalized = false;
 
     m->info = pg_mesh_info_t{};
@@ -470,6 +481,17 @@ int pg_get_clip_config(int* lanes, int* slots) {
         argchk(lanes && slots, "null argument");
         *lanes = pg::clip_lanes();
         *slots = pg::clip_slots();
+    });
+}
+
+int pg_set_clip_resume(int on) {
+    return guarded([&] { pg::set_clip_resume(on != 0); });
+}
+
+int pg_get_clip_resume(int* on) {
+    return guarded([&] {
+        argchk(on, "null argument");
+        *on = pg::clip_resume() ? 1 : 0;
     });
 }
 
@@ -1329,6 +1351,7 @@ struct pg_batch {
     DBuf cell, counts, starts, fill, tmp, total, perm, sorted, sorted_box, big;
     DBuf knn, lists, ncount, flags, offs;
     DBuf nbr, area, nrm, cen, count, vol, ccen, status, kcert;
+    DBuf cstate, resume;  // saved cells of uncertified seeds (restarts resume them)
     DBuf hid, area_u, vol_u, o2s, hsorted, tot;
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1354,6 +1377,10 @@ struct pg_batch {
         fa.ccen = ccen.as<double>();
         fa.status = status.as<int32_t>();
         fa.kcert = kcert.as<int32_t>();
+        if (resume.p) {
+            fa.cstate = cstate.as<uint8_t>();
+            fa.resume = resume.as<int32_t>();
+        }
         return fa;
     }
     double scale() const { return periodic ? 1. + (2 * prm.mesh.buff) : 1.0; }
@@ -1665,6 +1692,10 @@ int pg_batch_create(pg_ctx* c, const pg_sim_params* prm, int nbox, const int64_t
             B->ccen.get<double>(3 * cs);
             B->status.get<int32_t>(cs);
             B->kcert.get<int32_t>(cs);
+            if (B->prm.mesh.restart && pg::clip_resume()) {
+                B->cstate.get<uint8_t>(cs * pg::CLIP_STATE_BYTES);
+                B->resume.get<int32_t>(cs);
+            }
             B->hid.get<int32_t>(cs * MAXP);
             B->area_u.get<double>(cs * MAXP);
             B->vol_u.get<double>(cs);

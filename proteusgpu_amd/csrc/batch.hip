@@ -185,12 +185,13 @@ __global__ void __launch_bounds__(64)
                 const uint32_t* __restrict__ queries, const int64_t* __restrict__ nq_dev, int k,
                 uint32_t* __restrict__ out) {
     __shared__ int32_t c_start[64], c_pfx[65];
+    __shared__ KnnPend pend;
     const int64_t nq = *nq_dev;
     for (int64_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
         const uint32_t q = queries ? queries[qi] : (uint32_t)qi;
         const int b = sbox[q];
         knn_wave_query<DIM, M>(sorted, perm, N[b], coff[b], starts, counts, q, k, out + (size_t)qi * k, c_start,
-                               c_pfx);
+                               c_pfx, &pend);
         __syncthreads();
     }
 }

@@ -56,7 +56,16 @@ struct FaceArrays {
     double* ccen;      // cell centroid, 3 planes of stride doubles
     int32_t* status;   // PG status per cell
     int32_t* kcert;    // 1 + index of the certifying neighbour, 0 if none
+    // Certificate restarts resume where the previous pass stopped: the fast
+    // kernel saves an uncertified cell (planes, vertex planes, adjacency;
+    // CLIP_STATE_BYTES per sorted seed) and sets resume[s] = shape tag | row
+    // length, else 0.  nullptr: nothing is saved or resumed.
+    uint8_t* cstate = nullptr;
+    int32_t* resume = nullptr;
 };
+
+// Saved clip state of one seed (64 planes, up to 64 vertex slots).
+constexpr int64_t CLIP_STATE_BYTES = 3328;
 
 // Clip + certify + extract the seeds listed in `seeds` (sorted ids; nullptr
 // = all n) with their neighbour rows knn[row q] (row q belongs to seeds[q]).
@@ -72,6 +81,9 @@ bool valid_clip_config(int lanes, int slots);
 void set_clip_config(int lanes, int slots);
 int clip_lanes();
 int clip_slots();
+// Certificate restarts resume saved cells (true) or clip from scratch.
+bool clip_resume();
+void set_clip_resume(bool on);
 // Collect the sorted ids whose status is not SUCCESS (order: ascending id).
 // nfail (device int32) receives the count; tmp as for scan.
 void launch_collect_failed(int64_t n, const int32_t* status, uint32_t* list, int32_t* flags,

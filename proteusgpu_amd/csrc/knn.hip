@@ -37,10 +37,11 @@ __global__ void __launch_bounds__(64)
                const int32_t* __restrict__ starts, const int32_t* __restrict__ counts,
                const uint32_t* __restrict__ queries, int64_t nq, int k, uint32_t* __restrict__ out) {
     __shared__ int32_t c_start[64], c_pfx[65];
+    __shared__ KnnPend pend;
     const int64_t qi = blockIdx.x;
     if (qi >= nq) return;  // whole block
     const uint32_t q = queries ? queries[qi] : (uint32_t)qi;
-    knn_wave_query<DIM, M>(sorted, perm, N, 0, starts, counts, q, k, out + (size_t)qi * k, c_start, c_pfx);
+    knn_wave_query<DIM, M>(sorted, perm, N, 0, starts, counts, q, k, out + (size_t)qi * k, c_start, c_pfx, &pend);
 }
 
 template <int DIM, int M>

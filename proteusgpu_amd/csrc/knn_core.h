@@ -209,14 +209,49 @@ PG_DEV void wave_merge64(double& d, uint32_t& o, uint32_t& s, int l) {
     for (int j = 32; j > 0; j >>= 1) wave_cx(d, o, s, j, (l & j) == 0);
 }
 
+// Candidates that beat the k-th kept entry wait in LDS until 64 have
+// gathered; only then are they sorted and merged into the kept list.
+struct KnnPend {
+    double d[128];
+    uint32_t o[128], s[128];
+};
+
+// Merge 64 candidates (one per lane, unsorted, +inf padded) into the kept
+// list (slot-major, ascending): sort them across the wave (bitonic), then,
+// slot by slot, the lower half of (slot, batch) stays and the upper half
+// carries on.
+template <int M>
+PG_DEV void wave_merge_batch(double (&kd)[M], uint32_t (&ko)[M], uint32_t (&ks)[M], double dd, uint32_t co,
+                             uint32_t cs, int l) {
+    wave_sort64(dd, co, cs, l);
+#pragma unroll
+    for (int jj = 0; jj < M; jj++) {
+        const double rd = __shfl(dd, 63 - l, 64);
+        const uint32_t ro = __shfl(co, 63 - l, 64), rs = __shfl(cs, 63 - l, 64);
+        const bool mine = cand_less(kd[jj], ko[jj], rd, ro);
+        double hd = mine ? rd : kd[jj];
+        uint32_t ho = mine ? ro : ko[jj], hs = mine ? rs : ks[jj];
+        if (!mine) {
+            kd[jj] = rd;
+            ko[jj] = ro;
+            ks[jj] = rs;
+        }
+        wave_merge64(kd[jj], ko[jj], ks[jj], l);
+        wave_merge64(hd, ho, hs, l);
+        dd = hd;
+        co = ho;
+        cs = hs;
+    }
+}
+
 // One query by the 64 lanes of a one-wave block; c_start / c_pfx are the
-// block's LDS tables (64 and 65 entries).  Grid cells of the query's box are
-// cbase + i + j N + k N^2.
+// block's LDS tables (64 and 65 entries), pend its candidate buffer.  Grid
+// cells of the query's box are cbase + i + j N + k N^2.
 template <int DIM, int M>
 PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __restrict__ perm, int N,
                            int64_t cbase, const int32_t* __restrict__ starts,
                            const int32_t* __restrict__ counts, uint32_t q, int k, uint32_t* __restrict__ o,
-                           int32_t* c_start, int32_t* c_pfx) {
+                           int32_t* c_start, int32_t* c_pfx, KnnPend* pend) {
     const int l = threadIdx.x;
     double p[3] = {0, 0, 0};
 #pragma unroll
@@ -235,10 +270,27 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
         ko[j] = 0xffffffffu;
         ks[j] = 0xffffffffu;
     }
-    int size = 0;
+    int size = 0, npend = 0;
     const int kl = (k - 1) & 63, kj = (k - 1) >> 6;  // lane and slot of rank k-1
+    // merge the first n (<= 64) waiting candidates into the kept list
+    auto flush = [&](int n) {
+        __syncthreads();
+        double dd = __builtin_inf();
+        uint32_t co = 0xffffffffu, cs = 0xffffffffu;
+        if (l < n) {
+            dd = pend->d[l];
+            co = pend->o[l];
+            cs = pend->s[l];
+        }
+        wave_merge_batch<M>(kd, ko, ks, dd, co, cs, l);
+        size = min(k, size + n);
+    };
 
     for (int ring = 0; ring < N; ring++) {
+        if (npend > 0) {  // the ring test below needs the exact k-th entry
+            flush(npend);
+            npend = 0;
+        }
         if (ring >= 1 && size == k) {
             double kth = 0;
 #pragma unroll
@@ -320,38 +372,39 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
                 }
                 const uint64_t surv = (uint64_t)__ballot(take);
                 if (surv == 0ull) continue;
-                // merge the batch into the kept list: sort the 64 candidates
-                // across the wave (bitonic), then, slot by slot, the lower
-                // half of (slot, batch) stays and the upper half carries on
-                if (!take) {
-                    dd = __builtin_inf();
-                    co = 0xffffffffu;
-                    cs = 0xffffffffu;
+                // survivors join the LDS buffer in lane order (npend < 64,
+                // at most 64 join: positions stay below 128)
+                if (take) {
+                    const int at = npend + __popcll((unsigned long long)(surv & ((1ull << l) - 1ull)));
+                    pend->d[at] = dd;
+                    pend->o[at] = co;
+                    pend->s[at] = cs;
                 }
-                wave_sort64(dd, co, cs, l);
-#pragma unroll
-                for (int jj = 0; jj < M; jj++) {
-                    const double rd = __shfl(dd, 63 - l, 64);
-                    const uint32_t ro = __shfl(co, 63 - l, 64), rs = __shfl(cs, 63 - l, 64);
-                    const bool mine = cand_less(kd[jj], ko[jj], rd, ro);
-                    double hd = mine ? rd : kd[jj];
-                    uint32_t ho = mine ? ro : ko[jj], hs = mine ? rs : ks[jj];
-                    if (!mine) {
-                        kd[jj] = rd;
-                        ko[jj] = ro;
-                        ks[jj] = rs;
+                npend += __popcll((unsigned long long)surv);
+                if (npend >= 64) {
+                    flush(64);
+                    // move the rest (fewer than 64) to the front
+                    double rd = 0;
+                    uint32_t ro = 0, rs = 0;
+                    const int rest = npend - 64;
+                    if (l < rest) {
+                        rd = pend->d[64 + l];
+                        ro = pend->o[64 + l];
+                        rs = pend->s[64 + l];
                     }
-                    wave_merge64(kd[jj], ko[jj], ks[jj], l);
-                    wave_merge64(hd, ho, hs, l);
-                    dd = hd;
-                    co = ho;
-                    cs = hs;
+                    __syncthreads();
+                    if (l < rest) {
+                        pend->d[l] = rd;
+                        pend->o[l] = ro;
+                        pend->s[l] = rs;
+                    }
+                    npend = rest;
                 }
-                size = min(k, size + __popcll((unsigned long long)surv));
             }
             __syncthreads();
         }
     }
+    if (npend > 0) flush(npend);
 #pragma unroll
     for (int jj = 0; jj < M; jj++) {
         const int r = jj * 64 + l;

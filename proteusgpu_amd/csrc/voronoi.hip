@@ -95,6 +95,15 @@ void set_clip_config(int lanes, int slots) {
 int clip_lanes() { return g_clip_lanes; }
 int clip_slots() { return g_clip_slots; }
 
+// Certificate restarts resume the saved cell (default) or clip again from
+// the first neighbour (PGPU_CLIP_RESUME=0); the results are the same.
+static bool g_clip_resume = [] {
+    const char* e = getenv("PGPU_CLIP_RESUME");
+    return !(e && atoi(e) == 0);
+}();
+bool clip_resume() { return g_clip_resume; }
+void set_clip_resume(bool on) { g_clip_resume = on; }
+
 template <int DIM>
 void launch_dim(const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq, const uint32_t* knn, int k,
                 int max_p, int max_t, int big, FaceArrays fa, hipStream_t s, bool restart, int lanes, int slots) {

@@ -203,6 +203,19 @@ PG_DEV void vertex_numerics(const Plane& a, const Plane& b, const Plane& c, doub
     d2 = dx * dx + dy * dy + dz * dz;
 }
 
+// Saved clip state (CLIP_STATE_BYTES per seed): planes, their bounds and
+// neighbour ids, then per vertex slot its planes (0xffffffff: empty slot)
+// and adjacency, then the plane and live-vertex counts.
+struct ClipState {
+    Plane plane[64];
+    float pn1[64], pmx[64];
+    int32_t pvid[64];
+    uint32_t tri[64];
+    uint8_t adj[3][64];
+    int32_t nb_v, nlive;
+};
+static_assert(sizeof(ClipState) <= CLIP_STATE_BYTES, "clip state layout");
+
 template <int DIM, int G, int S>
 __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, const uint32_t* __restrict__ seeds,
                               int64_t nq, const uint32_t* __restrict__ knn, int k, int max_p,
@@ -227,8 +240,38 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
     const double sz = D == 3 ? pts[(size_t)D * sid + 2] : 0.0;
 
     // ---------------------------------------------------------------- init
+    // A seed whose previous pass (same shape, shorter row) stopped
+    // uncertified resumes from its saved cell at neighbour v0: clipping the
+    // same planes in the same order is deterministic, so the cell equals a
+    // pass over the longer row from scratch.
+    const int32_t shape_tag = (G << 24) | (S << 16);
+    int v0 = 0;
+    if (valid && fa.resume != nullptr && NS <= 64) {
+        const int32_t r = fa.resume[sid];
+        if (r > 0 && (r & 0x7fff0000) == shape_tag && (r & 0xffff) < k) v0 = r & 0xffff;
+    }
+    const ClipState* cst = v0 ? reinterpret_cast<const ClipState*>(fa.cstate + (size_t)sid * CLIP_STATE_BYTES)
+                              : nullptr;
+    int nb_v0 = 2 * D, nlive0 = D == 3 ? 8 : 4;
+    if (v0) {
+        nb_v0 = cst->nb_v;
+        nlive0 = cst->nlive;
+        for (int p = gl; p < nb_v0; p += G) {
+            sm.plane[p] = cst->plane[p];
+            sm.pn1[p] = cst->pn1[p];
+            sm.pmx[p] = cst->pmx[p];
+            sm.pvid[p] = cst->pvid[p];
+        }
+#pragma unroll
+        for (int i = 0; i < S; i++) {
+            const int gs = i * G + gl;
+            sm.tri[gs] = cst->tri[gs];
+#pragma unroll
+            for (int e = 0; e < D; e++) sm.adj[e][gs] = cst->adj[e][gs];
+        }
+    }
     // Box planes (cell.cpp:96-136): eps = 1e-14 margin.
-    if (gl < 2 * D) {
+    if (!v0 && gl < 2 * D) {
         const double eps = 1e-14, lo = -eps, hi = 1.0 + eps;
         Plane p;
         switch (gl) {
@@ -245,7 +288,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
     }
     // Initial vertices (cell.cpp:139-157).
     constexpr int NV0 = D == 3 ? 8 : 4;
-    if (gl < NV0) {
+    if (!v0 && gl < NV0) {
         uint32_t t;
         if (D == 3) {
             const uint32_t T3[8] = {2 | 5 << 8 | 0 << 16, 5 | 3 << 8 | 0 << 16, 1 | 5 << 8 | 2 << 16,
@@ -266,8 +309,27 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
         R.px[s] = R.py[s] = R.pz[s] = R.pw[s] = R.d2[s] = 0.0;
         R.q[s] = R.m[s] = 0.f;
     }
+    // resumed cell: vertex numerics from the saved planes, with the same
+    // arithmetic (and operand order) as when the vertices were made
+    if (v0) {
+#pragma unroll
+        for (int i = 0; i < S; i++) {
+            const uint32_t t = sm.tri[i * G + gl];
+            if (t == 0xffffffffu) continue;
+            const Plane& a = sm.plane[tri_p(t, 0)];
+            const Plane& b = sm.plane[tri_p(t, 1)];
+            const Plane& c = sm.plane[D == 3 ? tri_p(t, 2) : tri_p(t, 1)];
+            double px, py, pz, pw, d2;
+            vertex_numerics<D>(a, b, c, sx, sy, sz, px, py, pz, pw, d2);
+            float q = D == 3 ? sm.pn1[tri_p(t, 0)] * sm.pn1[tri_p(t, 1)] * sm.pn1[tri_p(t, 2)]
+                             : sm.pn1[tri_p(t, 0)] * sm.pn1[tri_p(t, 1)];
+            float m = fmaxf(sm.pmx[tri_p(t, 0)], sm.pmx[tri_p(t, 1)]);
+            if (D == 3) m = fmaxf(m, sm.pmx[tri_p(t, 2)]);
+            slot_write<D, S>(R, i, px, py, pz, pw, d2, q * 1.000001f, m);
+        }
+    }
     // NV0 <= G: initial vertex j lives in slot 0 of lane j.
-    if (gl < NV0) {
+    if (!v0 && gl < NV0) {
         uint32_t t = sm.tri[gl];
         const Plane& a = sm.plane[tri_p(t, 0)];
         const Plane& b = sm.plane[tri_p(t, 1)];
@@ -303,22 +365,29 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
     }
     block_sync();
 
-    int nb_v = 2 * D;
-    int nlive = NV0;
+    int nb_v = nb_v0;
+    int nlive = nlive0;
     int status = SUCCESS;
     int kc = 0;
     bool done = !valid;
     double R2 = 0.0;
 
     // --------------------------------------------------------- clip loop
-    for (int v = 0; v < k; v++) {
+    // Seeds of a wave may start at different neighbours (resumed cells):
+    // the wave walks v from the smallest start, a group joins at its own.
+    int vstart = v0;
+#pragma unroll
+    for (int d = G; d < 64; d <<= 1) vstart = min(vstart, __shfl_xor(vstart, d, 64));
+    int cbase = -NB_CHUNK;  // first neighbour of the prefetched chunk
+    for (int v = vstart; v < k; v++) {
         if (wave_ballot(!done) == 0ull) break;
-        const bool act = !done;
+        const bool act = !done && v >= v0;
         // the group prefetches the next NB_CHUNK neighbours and their
         // positions into LDS, so the clip loop issues no dependent global loads
-        if ((v & (NB_CHUNK - 1)) == 0) {
+        if (v >= cbase + NB_CHUNK) {
+            cbase = v;
             for (int j = gl; j < NB_CHUNK; j += G) {
-                if (act && v + j < k) {
+                if (!done && v + j < k) {
                     const uint32_t zz = row[v + j];
                     sm.nbi[j] = zz;
 #pragma unroll
@@ -334,7 +403,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
         bool plane_ok = false;
         int cur = 0;
         if (act) {
-            const int jj = v & (NB_CHUNK - 1);
+            const int jj = v - cbase;
             z = sm.nbi[jj];
             bx = sm.nbp[jj][0];
             by = sm.nbp[jj][1];
@@ -601,6 +670,9 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
 
     // final certificate with the k-th neighbour (voronoi.cpp:145-147); a row
     // holding every other seed certifies the cell by exhaustion
+    // (a slot overflow keeps its code: the 128-slot kernel redoes the seed
+    // at the same row length and applies this test itself)
+    const int loop_status = status;
     if (valid) {
         const uint32_t zl = row[k - 1];
         const double dx = pts[(size_t)D * zl] - sx, dy = pts[(size_t)D * zl + 1] - sy;
@@ -608,10 +680,37 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
         if (!(dx * dx + dy * dy + dz * dz > 4 * R2)) {
             if ((int64_t)k >= n_total - 1 && status == SUCCESS)
                 kc = k;
-            else
+            else if (status != SLOT_OVERFLOW)
                 status = SECURITY_RADIUS_NOT_REACHED;
         }
         if (status != SUCCESS) kc = 0;
+    }
+
+    // save an uncertified cell that ran through its whole row, for a resume
+    // at a longer row
+    if (fa.resume != nullptr && NS <= 64) {
+        const bool save = valid && loop_status == SUCCESS && status == SECURITY_RADIUS_NOT_REACHED;
+        if (save) {
+            ClipState* out = reinterpret_cast<ClipState*>(fa.cstate + (size_t)sid * CLIP_STATE_BYTES);
+            for (int p = gl; p < nb_v; p += G) {
+                out->plane[p] = sm.plane[p];
+                out->pn1[p] = sm.pn1[p];
+                out->pmx[p] = sm.pmx[p];
+                out->pvid[p] = sm.pvid[p];
+            }
+#pragma unroll
+            for (int i = 0; i < S; i++) {
+                const int gs = i * G + gl;
+                out->tri[gs] = ((R.live >> i) & 1u) ? sm.tri[gs] : 0xffffffffu;
+#pragma unroll
+                for (int e = 0; e < D; e++) out->adj[e][gs] = sm.adj[e][gs];
+            }
+            if (gl == 0) {
+                out->nb_v = nb_v;
+                out->nlive = nlive;
+            }
+        }
+        if (valid && gl == 0) fa.resume[sid] = save ? (shape_tag | k) : 0;
     }
 
     // ---------------------------------------------------------- extraction

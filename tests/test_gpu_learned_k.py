@@ -14,8 +14,11 @@ Bar:
   bit (statuses, certified k, faces, normals; geometry within 1e-12), also
   for a deliberately bad model that predicts k = 16 everywhere;
 * trained on one point set and applied to another, the model cuts the
-  restarts of the fixed-k build by more than 3x and its build time by more
-  than 10% (median of three builds).
+  restarts of the fixed-k build by more than 3x; with restarts that clip
+  again from the first neighbour, the captured batched step is faster than
+  the fixed-k one.  (With restarts that resume the saved cell, the default,
+  a restart costs only its longer KNN row and the two steps take about the
+  same time on uniform seeds; the test prints both.)
 """
 import numpy as np
 import pytest
@@ -167,16 +170,24 @@ def test_learned_k_cuts_work(ctx):
     assert learned.n_restarted < 0.35 * plain.n_restarted
     rho, vel, E = sod_state(pts, 1.4)
     kw = dict(boundary="reflective", k=32, gamma=1.4, cfl=0.3, flux="physical", move_mesh=True)
-    res, ms = [], []
-    for model in (None, km):
-        B = api.Batch(ctx, np.array([len(pts)]), pts, rho, vel, E, **kw)
-        if model is not None:
-            B.set_kmodel(model)
-        t = [B.step() for _ in range(6)]
-        ms.append(float(np.median(t[1:])))
-        res.append(B.get())
-        B.free()
-    print(f"batched step: fixed k {ms[0]:.3f} ms, learned k {ms[1]:.3f} ms", flush=True)
-    for a, b in zip(res[0], res[1]):
-        assert np.array_equal(a, b)
-    assert ms[1] < 0.97 * ms[0]
+    res, ms = [], {}
+    try:
+        for resume in (False, True):
+            api.set_clip_resume(resume)
+            for model in (None, km):
+                B = api.Batch(ctx, np.array([len(pts)]), pts, rho, vel, E, **kw)
+                if model is not None:
+                    B.set_kmodel(model)
+                t = [B.step() for _ in range(6)]
+                ms[(resume, model is not None)] = float(np.median(t[1:]))
+                res.append(B.get())
+                B.free()
+    finally:
+        api.set_clip_resume(True)
+    print(f"batched step, restarts from scratch: fixed k {ms[(False, False)]:.3f} ms, learned k "
+          f"{ms[(False, True)]:.3f} ms; resumed restarts: fixed k {ms[(True, False)]:.3f} ms, learned k "
+          f"{ms[(True, True)]:.3f} ms", flush=True)
+    for r in res[1:]:
+        for a, b in zip(res[0], r):
+            assert np.array_equal(a, b)
+    assert ms[(False, True)] < 0.97 * ms[(False, False)]

@@ -140,3 +140,25 @@ def test_clip_shapes_equal_oracle(ctx, lanes, slots):
     finally:
         api.set_clip_config(16, 64)
     _cmp_mesh(g, O.Mesh(pts, "none", k=48, restart=False))
+
+
+@pytest.mark.parametrize("dim,n,k,lanes,slots", [(3, 20000, 16, 16, 64), (3, 20000, 24, 8, 48), (2, 10000, 8, 16, 48)])
+def test_restart_resume_equals_scratch(ctx, dim, n, k, lanes, slots):
+    """Certificate restarts that continue the saved cell give the mesh of
+    restarts that clip from the first neighbour, and the oracle's."""
+    print(f"resume dim={dim} n={n} k={k} shape {lanes}x{slots}", flush=True)
+    pts = uniform_points(n, dim, 12)
+    api.set_clip_config(lanes, slots)
+    try:
+        assert api.get_clip_resume()
+        a = api.Mesh(ctx, pts, boundary="periodic", buff=0.12, k=k, restart=True)
+        api.set_clip_resume(False)
+        b = api.Mesh(ctx, pts, boundary="periodic", buff=0.12, k=k, restart=True)
+    finally:
+        api.set_clip_resume(True)
+        api.set_clip_config(16, 64)
+    assert a.n_restarted > 0.5 * a.n_seeds and a.n_restarted == b.n_restarted
+    ga, gb = a.copy_out(), b.copy_out()
+    for key in ga:
+        assert np.array_equal(ga[key], gb[key]), key
+    _cmp_mesh(ga, O.Mesh(pts, "periodic", buff=0.12, k=k, restart=True))
