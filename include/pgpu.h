@@ -290,7 +290,8 @@ typedef struct {
     int clip_lanes;      /* lanes per seed of the fast clip kernel: 8 or 16         */
     int clip_slots;      /* vertex slots per seed of the fast clip kernel: 48 or 64 */
     int knn_wave_min_k;  /* rows at least this long use the wave-per-query KNN      */
-    int blocks_per_cu;   /* persistent grid size of the KNN and clip kernels         */
+    int blocks_per_cu;   /* resident blocks per CU of the persistent KNN and clip
+                          * kernels, at most their occupancy (default 64: all)    */
     int use_graph;       /* 1: capture the step in a hipGraph, 0: plain launches     */
 } pg_batch_tuning;
 

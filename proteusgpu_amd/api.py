@@ -433,7 +433,7 @@ class Batch:
         check(lib().pg_batch_get_tuning(self.h, C.byref(t)))
         return {name: getattr(t, name) for name, _ in t._fields_}
 
-    def set_tuning(self, clip_slots=64, knn_wave_min_k=33, blocks_per_cu=8, use_graph=True, clip_lanes=16):
+    def set_tuning(self, clip_slots=64, knn_wave_min_k=33, blocks_per_cu=64, use_graph=True, clip_lanes=16):
         from ._native import BatchTuning
         t = BatchTuning(clip_lanes, clip_slots, knn_wave_min_k, blocks_per_cu, int(use_graph))
         check(lib().pg_batch_set_tuning(self.h, C.byref(t)))

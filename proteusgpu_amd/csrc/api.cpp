@@ -1337,7 +1337,7 @@ struct pg_batch {
     std::vector<int64_t> hoff_h;
     std::vector<int> ksched, ksched_learned;
     // tuning (timing only; results do not depend on it)
-    int lanes = 16, slots = 64, wave_min_k = 20, bpc = 8;
+    int lanes = 16, slots = 64, wave_min_k = 20, bpc = 64;
     bool use_graph = true;
     const pg_kmodel* km = nullptr;  // learned k (nullptr: fixed k)
     // static
@@ -1352,6 +1352,7 @@ struct pg_batch {
     DBuf knn, lists, ncount, flags, offs;
     DBuf nbr, area, nrm, cen, count, vol, ccen, status, kcert;
     DBuf cstate, resume;  // saved cells of uncertified seeds (restarts resume them)
+    DBuf ticket;          // work-queue counter of the persistent kernels
     DBuf hid, area_u, vol_u, o2s, hsorted, tot;
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1419,11 +1420,12 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
     auto knn = [&](const uint32_t* q, const int64_t* nq, int k) {
         pg::launch_bknn(dim, B->sorted.as<double>(), B->perm.as<uint32_t>(), B->sorted_box.as<int32_t>(),
                         B->N.as<int32_t>(), B->coff.as<int64_t>(), B->starts.as<int32_t>(), B->counts.as<int32_t>(),
-                        q, nq, cap, k, B->knn.as<uint32_t>(), B->wave_min_k, B->bpc, s);
+                        q, nq, cap, k, B->knn.as<uint32_t>(), B->wave_min_k, B->bpc,
+                        B->ticket.as<unsigned long long>(), s);
     };
     auto clip = [&](const uint32_t* q, const int64_t* nq, int k, int big) {
         pg::launch_bvoronoi(dim, B->sorted.as<double>(), q, nq, cap, B->knn.as<uint32_t>(), k, p.max_p, p.max_t, big,
-                            B->lanes, B->slots, B->faces(), B->bpc, s);
+                            B->lanes, B->slots, B->faces(), B->bpc, B->ticket.as<unsigned long long>(), s);
     };
     auto collect = [&](const uint32_t* L, const int64_t* nL, int code, uint32_t* out, int64_t* nout) {
         pg::launch_bcollect(nL, cap, L, B->status.as<int32_t>(), code, out, B->flags.as<int32_t>(),
@@ -1692,6 +1694,7 @@ int pg_batch_create(pg_ctx* c, const pg_sim_params* prm, int nbox, const int64_t
             B->ccen.get<double>(3 * cs);
             B->status.get<int32_t>(cs);
             B->kcert.get<int32_t>(cs);
+            B->ticket.get<unsigned long long>(1);
             if (B->prm.mesh.restart && pg::clip_resume()) {
                 B->cstate.get<uint8_t>(cs * pg::CLIP_STATE_BYTES);
                 B->resume.get<int32_t>(cs);
@@ -1785,7 +1788,7 @@ int pg_batch_autotune(pg_ctx* c, pg_batch* B, int reps, pg_batch_tuning* best, d
         double wms = 1e300;
         const int shape_c[4][2] = {{16, 64}, {16, 48}, {8, 64}, {8, 48}};
         const int wave_c[2] = {33, 1};
-        const int bpc_c[2] = {8, 16};
+        const int bpc_c[2] = {8, 64};
         for (int a = 0; a < 4; a++)
             for (int w = 0; w < 2; w++)
                 for (int q = 0; q < 2; q++) {

@@ -159,6 +159,45 @@ __global__ void __launch_bounds__(TB) k_fill(T* a, int64_t n, T v) {
     for (int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TB) a[i] = v;
 }
 
+// Work queue of the one-wave persistent kernels: blocks take the next
+// chunk of items from a counter (cleared before the launch), so seeds of
+// uneven cost balance over the resident blocks; chunks keep the atomics on
+// the one counter few.
+PG_DEV int64_t next_ticket(unsigned long long* ticket, int chunk) {
+    unsigned long long t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(ticket, (unsigned long long)chunk);
+    return (int64_t)__shfl(t, 0, 64);
+}
+// Items per ticket: up to `most`, fewer for short lists (each block should
+// see several chunks, so that the last ones balance).
+PG_DEV int ticket_chunk(int64_t items, int most) {
+    const int64_t c = items / (4 * (int64_t)gridDim.x);
+    return (int)(c < 1 ? 1 : (c > most ? most : c));
+}
+constexpr int KNN_CHUNK = 8;   // queries per ticket, at most
+constexpr int CLIP_CHUNK = 2;  // blocks of seeds per ticket, at most
+
+// Resident one-wave blocks of a kernel on the device (its occupancy per CU,
+// capped at blocks_per_cu, times the CUs).
+int device_cus() {
+    static const int ncu = [] {
+        int dev = 0, n = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+        return n > 0 ? n : 256;
+    }();
+    return ncu;
+}
+template <class K>
+int occupancy(K kernel) {
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, 64, 0) != hipSuccess || occ <= 0) occ = 1;
+    return occ;
+}
+unsigned resident_blocks(int occ, int blocks_per_cu) {
+    return (unsigned)(device_cus() * std::max(1, std::min(occ, blocks_per_cu)));
+}
+
 // ------------------------------------------------------------------ KNN
 template <int DIM, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
@@ -183,32 +222,45 @@ __global__ void __launch_bounds__(64)
                 const int32_t* __restrict__ sbox, const int32_t* __restrict__ N, const int64_t* __restrict__ coff,
                 const int32_t* __restrict__ starts, const int32_t* __restrict__ counts,
                 const uint32_t* __restrict__ queries, const int64_t* __restrict__ nq_dev, int k,
-                uint32_t* __restrict__ out) {
+                uint32_t* __restrict__ out, unsigned long long* ticket) {
     __shared__ int32_t c_start[64], c_pfx[65];
     __shared__ KnnPend pend;
     const int64_t nq = *nq_dev;
-    for (int64_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
-        const uint32_t q = queries ? queries[qi] : (uint32_t)qi;
-        const int b = sbox[q];
-        knn_wave_query<DIM, M>(sorted, perm, N[b], coff[b], starts, counts, q, k, out + (size_t)qi * k, c_start,
-                               c_pfx, &pend);
-        __syncthreads();
+    const int chunk = ticket_chunk(nq, KNN_CHUNK);
+    for (;;) {
+        const int64_t q0 = next_ticket(ticket, chunk);
+        if (q0 >= nq) break;
+        const int64_t q1 = q0 + chunk < nq ? q0 + chunk : nq;
+        for (int64_t qi = q0; qi < q1; qi++) {
+            const uint32_t q = queries ? queries[qi] : (uint32_t)qi;
+            const int b = sbox[q];
+            knn_wave_query<DIM, M>(sorted, perm, N[b], coff[b], starts, counts, q, k, out + (size_t)qi * k,
+                                   c_start, c_pfx, &pend);
+            __syncthreads();
+        }
     }
 }
 
 // ------------------------------------------------------------------ clip
 template <int DIM, int G, int S>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, (VoroCfg<DIM, G, S>::WAVES))
     k_bvoronoi(const double* __restrict__ pts, const uint32_t* __restrict__ seeds, const int64_t* __restrict__ nq_dev,
-               const uint32_t* __restrict__ knn, int k, int max_p, int max_t, FaceArrays fa) {
+               const uint32_t* __restrict__ knn, int k, int max_p, int max_t, FaceArrays fa,
+               unsigned long long* ticket) {
     using C = VoroCfg<DIM, G, S>;
     __shared__ SeedSmem<C::NS, C::MAXP> sm[C::SPW];
     const int64_t nq = *nq_dev;
     const int64_t nvb = (nq + C::SPW - 1) / C::SPW;
-    for (int64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
-        // no box holds k + 1 seeds or fewer: the exhaustive-row rule never applies
-        voronoi_group<DIM, G, S>(pts, INT64_MAX, seeds, nq, knn, k, max_p, max_t, fa, sm, vb);
-        __syncthreads();
+    const int chunk = ticket_chunk(nvb, CLIP_CHUNK);
+    for (;;) {
+        const int64_t b0 = next_ticket(ticket, chunk);
+        if (b0 >= nvb) break;
+        const int64_t b1 = b0 + chunk < nvb ? b0 + chunk : nvb;
+        for (int64_t vb = b0; vb < b1; vb++) {
+            // no box holds k + 1 seeds or fewer: the exhaustive-row rule never applies
+            voronoi_group<DIM, G, S>(pts, INT64_MAX, seeds, nq, knn, k, max_p, max_t, fa, sm, vb);
+            __syncthreads();
+        }
     }
 }
 
@@ -450,21 +502,24 @@ void bknn_heap(const double* sorted, const uint32_t* perm, const int32_t* sbox, 
 template <int DIM, int M>
 void bknn_wave(const double* sorted, const uint32_t* perm, const int32_t* sbox, const int32_t* N,
                const int64_t* coff, const int32_t* starts, const int32_t* counts, const uint32_t* q,
-               const int64_t* nq_dev, int64_t cap, int k, uint32_t* out, int blocks_per_cu, hipStream_t s) {
-    const unsigned nb = (unsigned)std::min<int64_t>(cap, 256LL * 8 * blocks_per_cu);
+               const int64_t* nq_dev, int64_t cap, int k, uint32_t* out, int blocks_per_cu,
+               unsigned long long* ticket, hipStream_t s) {
+    static const int occ = occupancy(k_bknn_wave<DIM, M>);
+    const unsigned nb = (unsigned)std::min<int64_t>(cap, resident_blocks(occ, 8 * blocks_per_cu));
+    k_fill<unsigned long long><<<1, TB, 0, s>>>(ticket, 1, 0ull);
     k_bknn_wave<DIM, M><<<std::max(nb, 1u), 64, 0, s>>>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, k,
-                                                        out);
+                                                        out, ticket);
 }
 template <int DIM>
 void bknn_dim(const double* sorted, const uint32_t* perm, const int32_t* sbox, const int32_t* N,
               const int64_t* coff, const int32_t* starts, const int32_t* counts, const uint32_t* q,
               const int64_t* nq_dev, int64_t cap, int k, uint32_t* out, int wave_min_k, int blocks_per_cu,
-              hipStream_t s) {
+              unsigned long long* ticket, hipStream_t s) {
     if (k >= wave_min_k) {
-        if (k <= 64) bknn_wave<DIM, 1>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, s);
-        else if (k <= 128) bknn_wave<DIM, 2>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, s);
-        else if (k <= 192) bknn_wave<DIM, 3>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, s);
-        else bknn_wave<DIM, 4>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, s);
+        if (k <= 64) bknn_wave<DIM, 1>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, ticket, s);
+        else if (k <= 128) bknn_wave<DIM, 2>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, ticket, s);
+        else if (k <= 192) bknn_wave<DIM, 3>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, ticket, s);
+        else bknn_wave<DIM, 4>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, ticket, s);
     } else if (k <= 208) {
         bknn_heap<DIM, 64>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, s);
     } else {
@@ -473,35 +528,38 @@ void bknn_dim(const double* sorted, const uint32_t* perm, const int32_t* sbox, c
 }
 template <int DIM, int G, int S>
 void bclip(const double* pts, const uint32_t* seeds, const int64_t* nq_dev, int64_t cap, const uint32_t* knn, int k,
-           int max_p, int max_t, FaceArrays fa, int blocks_per_cu, hipStream_t s) {
+           int max_p, int max_t, FaceArrays fa, int blocks_per_cu, unsigned long long* ticket, hipStream_t s) {
     using C = VoroCfg<DIM, G, S>;
-    const unsigned nb = (unsigned)std::min<int64_t>((cap + C::SPW - 1) / C::SPW, 256LL * blocks_per_cu);
-    k_bvoronoi<DIM, G, S><<<std::max(nb, 1u), 64, 0, s>>>(pts, seeds, nq_dev, knn, k, max_p, max_t, fa);
+    static const int occ = occupancy(k_bvoronoi<DIM, G, S>);
+    const unsigned nb =
+        (unsigned)std::min<int64_t>((cap + C::SPW - 1) / C::SPW, resident_blocks(occ, blocks_per_cu));
+    k_fill<unsigned long long><<<1, TB, 0, s>>>(ticket, 1, 0ull);
+    k_bvoronoi<DIM, G, S><<<std::max(nb, 1u), 64, 0, s>>>(pts, seeds, nq_dev, knn, k, max_p, max_t, fa, ticket);
 }
 }  // namespace
 
 void launch_bknn(int dim, const double* sorted, const uint32_t* perm, const int32_t* sbox, const int32_t* N,
                  const int64_t* coff, const int32_t* starts, const int32_t* counts, const uint32_t* queries,
                  const int64_t* nq_dev, int64_t cap, int k, uint32_t* out, int wave_min_k, int blocks_per_cu,
-                 hipStream_t s) {
+                 unsigned long long* ticket, hipStream_t s) {
     if (dim == 2)
         bknn_dim<2>(sorted, perm, sbox, N, coff, starts, counts, queries, nq_dev, cap, k, out, wave_min_k,
-                    blocks_per_cu, s);
+                    blocks_per_cu, ticket, s);
     else
         bknn_dim<3>(sorted, perm, sbox, N, coff, starts, counts, queries, nq_dev, cap, k, out, wave_min_k,
-                    blocks_per_cu, s);
+                    blocks_per_cu, ticket, s);
 }
 
 void launch_bvoronoi(int dim, const double* sorted, const uint32_t* seeds, const int64_t* nq_dev, int64_t cap,
                      const uint32_t* knn, int k, int max_p, int max_t, int big, int lanes, int slots, FaceArrays fa,
-                     int blocks_per_cu, hipStream_t s) {
+                     int blocks_per_cu, unsigned long long* ticket, hipStream_t s) {
     auto go = [&](auto dimc) {
         constexpr int D = decltype(dimc)::value;
-        if (big) bclip<D, 16, 8>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
-        else if (lanes == 8 && slots == 48) bclip<D, 8, 6>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
-        else if (lanes == 8) bclip<D, 8, 8>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
-        else if (slots == 48) bclip<D, 16, 3>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
-        else bclip<D, 16, 4>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, s);
+        if (big) bclip<D, 16, 8>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, ticket, s);
+        else if (lanes == 8 && slots == 48) bclip<D, 8, 6>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, ticket, s);
+        else if (lanes == 8) bclip<D, 8, 8>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, ticket, s);
+        else if (slots == 48) bclip<D, 16, 3>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, ticket, s);
+        else bclip<D, 16, 4>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, ticket, s);
     };
     if (dim == 2) go(std::integral_constant<int, 2>{});
     else go(std::integral_constant<int, 3>{});

@@ -64,6 +64,9 @@ struct FaceArrays {
     int32_t* resume = nullptr;
 };
 
+// Face slots per cell in the padded face arrays (the reference's _MAX_P_).
+constexpr int FACE_SLOTS = 64;
+
 // Saved clip state of one seed (64 planes, up to 64 vertex slots).
 constexpr int64_t CLIP_STATE_BYTES = 3328;
 
@@ -155,10 +158,12 @@ void launch_bgrid(int dim, const double* pts, const int64_t* n_dev, int64_t cap,
 void launch_bknn(int dim, const double* sorted, const uint32_t* perm, const int32_t* sbox, const int32_t* N,
                  const int64_t* coff, const int32_t* starts, const int32_t* counts, const uint32_t* queries,
                  const int64_t* nq_dev, int64_t cap, int k, uint32_t* out, int wave_min_k, int blocks_per_cu,
-                 hipStream_t s);
+                 unsigned long long* ticket, hipStream_t s);
+// ticket: a device counter, the work queue of the persistent kernels
+// (cleared by the launcher, in stream order)
 void launch_bvoronoi(int dim, const double* sorted, const uint32_t* seeds, const int64_t* nq_dev, int64_t cap,
                      const uint32_t* knn, int k, int max_p, int max_t, int big, int lanes, int slots, FaceArrays fa,
-                     int blocks_per_cu, hipStream_t s);
+                     int blocks_per_cu, unsigned long long* ticket, hipStream_t s);
 void launch_bcollect(const int64_t* nq_dev, int64_t cap, const uint32_t* seeds, const int32_t* status, int code,
                      uint32_t* list, int32_t* flags, int32_t* offs, int32_t* tmp, int32_t* total, int64_t* nout,
                      hipStream_t s);

@@ -11,7 +11,7 @@ namespace {
 // wave.  Large kernel: 16 lanes x 8 slots = 128 slots, for seeds whose cell
 // outgrows 64 vertices before max_t (rare).
 template <int DIM, int G, int S>
-__global__ void __launch_bounds__(64) k_voronoi(const double* __restrict__ pts, int64_t n_total,
+__global__ void __launch_bounds__(64, (VoroCfg<DIM, G, S>::WAVES)) k_voronoi(const double* __restrict__ pts, int64_t n_total,
                                                 const uint32_t* __restrict__ seeds, int64_t nq,
                                                 const uint32_t* __restrict__ knn, int k, int max_p,
                                                 int max_t, FaceArrays fa) {
@@ -23,7 +23,7 @@ __global__ void __launch_bounds__(64) k_voronoi(const double* __restrict__ pts, 
 // The same kernel under its own name for certificate restarts, so that
 // profiles separate the first pass from the restarts.
 template <int DIM, int G, int S>
-__global__ void __launch_bounds__(64) k_voronoi_restart(const double* __restrict__ pts, int64_t n_total,
+__global__ void __launch_bounds__(64, (VoroCfg<DIM, G, S>::WAVES)) k_voronoi_restart(const double* __restrict__ pts, int64_t n_total,
                                                         const uint32_t* __restrict__ seeds, int64_t nq,
                                                         const uint32_t* __restrict__ knn, int k, int max_p,
                                                         int max_t, FaceArrays fa) {

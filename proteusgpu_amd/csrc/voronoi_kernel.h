@@ -39,16 +39,26 @@ struct Plane {
     double x, y, z, w;
 };
 
+// Fast kernels (at most 64 vertex slots) hold 48 planes per seed, which
+// keeps a seed's LDS below 3.3 KB (three waves per SIMD); a seed that needs
+// more planes before max_p, or more vertex slots, is redone by the 128-slot
+// kernel, which holds the full 64 planes.
 template <int DIM, int G, int S>
 struct VoroCfg {
     static constexpr int NS = G * S;               // vertex slots per seed
     static constexpr int SPW = 64 / G;             // seeds per wave (= block)
-    static constexpr int MAXP = 64;                // plane slots per seed
+    static constexpr int MAXP = NS > 64 ? 64 : 48; // plane slots per seed
     static constexpr int NW = (NS + 63) / 64;      // 64-bit words of a slot mask
     static constexpr int PPL = MAXP / G;           // planes per lane at extraction
+    // waves per SIMD the register budget is sized for: the 16-lane fast
+    // kernels fit three in LDS (under 3.4 KB per seed)
+    static constexpr int WAVES = (G == 16 && NS <= 64) ? 3 : 1;
     static_assert(NS <= 128, "slot ids are bytes below 255");
     static_assert(G <= 64 && 64 % G == 0, "groups tile the wave");
+    static_assert(MAXP % G == 0, "planes tile the group");
 };
+
+constexpr int NB_CHUNK = 12;
 
 template <int NS, int MAXP>
 struct SeedSmem {
@@ -61,15 +71,18 @@ struct SeedSmem {
     int32_t pvid[MAXP];     // neighbour (sorted id) of each plane, -1 = box
     uint32_t tri[NS];       // planes of each vertex: a | b << 8 | c << 16
     uint8_t adj[3][NS];     // neighbour slot across edge/line e
-    uint32_t edge[NS];      // boundary edges of the current clip
-    uint8_t newslot[NS];    // slot of the j-th new vertex
-    uint8_t by_start[MAXP]; // 3D: boundary edge starting at a plane
-    uint8_t by_end[MAXP];   // 3D: boundary edge ending at a plane
-    uint32_t first[MAXP];   // extraction: vertex of each plane with the lowest plane key
-    uint32_t nbi[32];       // next chunk of the neighbour row (prefetched)
-    double nbp[32][3];      // and the neighbours' positions
+    union {
+        struct {
+            uint32_t edge[NS];        // boundary edges of the current clip
+            uint8_t newslot[NS];      // slot of the j-th new vertex
+            uint8_t by_start[MAXP];   // 3D: boundary edge starting at a plane
+            uint8_t by_end[MAXP];     // 3D: boundary edge ending at a plane
+            uint32_t nbi[NB_CHUNK];   // next chunk of the neighbour row (prefetched)
+            double nbp[NB_CHUNK][3];  // and the neighbours' positions
+        };
+        uint32_t first[MAXP];  // extraction: vertex of each plane with the lowest plane key
+    };
 };
-constexpr int NB_CHUNK = 32;
 
 // Round a non-negative double up to float (next float above on inexact).
 PG_DEV float f2f_up(double d) {
@@ -232,7 +245,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
     const int64_t qi = vblock * C::SPW + grp;
     const bool valid = qi < nq;
     SeedSmem<NS, MAXP>& sm = smem[grp];
-    if (max_p > MAXP) max_p = MAXP;
+    if (max_p > FACE_SLOTS) max_p = FACE_SLOTS;
 
     const uint32_t sid = valid ? (seeds ? seeds[qi] : (uint32_t)qi) : 0u;
     const uint32_t* row = knn + (valid ? (size_t)qi * (size_t)k : 0);
@@ -410,6 +423,8 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             bz = D == 3 ? sm.nbp[jj][2] : 0.0;
             if (nb_v >= max_p) {
                 status = VERTEX_OVERFLOW;
+            } else if (nb_v >= MAXP) {
+                status = SLOT_OVERFLOW;  // more planes than this kernel holds
             } else {
                 double dx = sx - bx, dy = sy - by, dz = sz - bz;
                 double ax = sx + bx, ay = sy + by, az = sz + bz;
@@ -873,7 +888,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
 #pragma unroll
     for (int w = 1; w < C::PPL; w <<= 1)
 #pragma unroll
-        for (int i = 0; i < C::PPL; i += 2 * w) {
+        for (int i = 0; i + w < C::PPL; i += 2 * w) {  // absent planes add nothing
             pvol[i] += pvol[i + w];
             pcx[i] += pcx[i + w];
             pcy[i] += pcy[i + w];
@@ -890,7 +905,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             const int64_t at = (int64_t)fi * st + sid;
             fa.nbr[at] = pv[i];
             fa.area[at] = farea[i];
-            const int64_t plane_stride = st * MAXP;
+            const int64_t plane_stride = st * FACE_SLOTS;
             fa.nrm[at] = nx[i];
             fa.nrm[plane_stride + at] = ny[i];
             fa.nrm[2 * plane_stride + at] = nz[i];
