@@ -154,6 +154,15 @@ typedef struct {
 } pg_vmesh_view;
 
 int pg_mesh_copy_out(pg_ctx* ctx, const pg_mesh* mesh, pg_vmesh_view* out);
+/* The KNN problem of a fixed-k mesh build, as knn::get_points /
+ * get_knearest / get_permutation (knn.cpp:437-455) return it for the KNN
+ * HDF5 dump (knn.cpp:457-473, output.cpp:305-411): the build's points
+ * (ghosts included, scaled into the unit box for periodic meshes) in sorted
+ * order (n_seeds*dim), the first pass's neighbour rows of length k in
+ * sorted-index space (n_seeds*k) and the permutation (n_seeds).  Any pointer
+ * may be NULL.  PG_ERR_ARG for a mesh built with learned k (its rows have
+ * per-seed lengths). */
+int pg_mesh_get_knn(pg_ctx* ctx, const pg_mesh* mesh, double* points, uint32_t* nearest, uint32_t* permutation);
 /* voronoi::free_vmesh (voronoi.h:65). */
 int pg_mesh_free(pg_mesh** mesh);
 

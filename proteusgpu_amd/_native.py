@@ -102,6 +102,7 @@ def lib():
         "pg_mesh_compute": ([vp, C.POINTER(MeshParams), vp, i64, pp], i32),
         "pg_mesh_info": ([vp, C.POINTER(MeshInfo)], i32),
         "pg_mesh_copy_out": ([vp, vp, C.POINTER(VMeshView)], i32), "pg_mesh_free": ([pp], i32),
+        "pg_mesh_get_knn": ([vp, vp, vp, vp, vp], i32),
         "pg_state_create": ([vp, vp, vp, vp, vp, pp], i32),
         "pg_dt_cfl": ([vp, vp, vp, dbl, dbl, C.POINTER(C.c_double)], i32),
         "pg_hydro_step": ([vp, vp, vp, dbl, dbl, i32], i32),
@@ -152,7 +153,7 @@ def check(rc):
 
 def exported_symbols():
     """Names of the C ABI entry points this binding knows (for the export test)."""
-    return ["pg_version", "pg_last_error", "pg_device_count", "pg_set_clip_config", "pg_get_clip_config", "pg_set_clip_resume", "pg_get_clip_resume", "pg_ctx_create", "pg_ctx_destroy",
+    return ["pg_version", "pg_last_error", "pg_device_count", "pg_set_clip_config", "pg_get_clip_config", "pg_set_clip_resume", "pg_get_clip_resume", "pg_mesh_get_knn", "pg_ctx_create", "pg_ctx_destroy",
             "pg_ctx_sync", "pg_stream_copy_probe", "pg_knn_init", "pg_knn_solve", "pg_knn_info", "pg_knn_get_points",
             "pg_knn_get_knearest", "pg_knn_get_permutation", "pg_knn_free", "pg_mesh_params_default",
             "pg_mesh_compute", "pg_mesh_info", "pg_mesh_copy_out", "pg_mesh_free", "pg_state_create",

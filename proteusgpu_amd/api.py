@@ -204,7 +204,7 @@ class Mesh:
     def __init__(self, ctx: Context, pts: np.ndarray, boundary="periodic", buff=0.04, k=32, max_p=64,
                  max_t=96, restart=True, k_max=256, kmodel: KModel | None = None):
         pts = np.ascontiguousarray(pts, dtype=np.float64)
-        self.ctx, self.dim = ctx, pts.shape[1]
+        self.ctx, self.dim, self.k = ctx, pts.shape[1], k
         prm = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max)
         h = C.c_void_p()
         km = kmodel.h if kmodel is not None else None
@@ -227,6 +227,18 @@ class Mesh:
         out = np.empty((self.n_seeds, KMODEL_NFEAT), dtype=np.float32)
         check(lib().pg_mesh_features(self.ctx.h, self.h, _ptr(out)))
         return out
+
+    def knn(self):
+        """The build's KNN problem as knn::get_points / get_knearest /
+        get_permutation return it (fixed-k builds): sorted points (scaled
+        into the unit box for periodic meshes), rows of length k in
+        sorted-index space, sorted -> input permutation."""
+        n = self.n_seeds
+        pts = np.empty((n, self.dim))
+        rows = np.empty((n, self.k), np.uint32)
+        perm = np.empty(n, np.uint32)
+        check(lib().pg_mesh_get_knn(self.ctx.h, self.h, _ptr(pts), _ptr(rows), _ptr(perm)))
+        return pts, rows, perm
 
     def copy_out(self):
         """Reference VMesh arrays in original order (dict of numpy arrays)."""

@@ -142,6 +142,7 @@ struct pg_mesh {
     DBuf nbr, area, nrm, cen, count, vol, ccen, status, kcert;
     DBuf cstate, resume;  // saved cells of uncertified seeds (restarts resume them)
     bool use_resume = false;
+    bool learned_rows = false;  // first pass with per-seed row lengths (knn rows not kept)
     // hydro tables
     DBuf hid, area_u, vol_u, o2s, bad;
     bool finalized = false;
@@ -336,6 +337,7 @@ void mesh_build(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool on_dev
     m->finalized = false;
 
     m->info = pg_mesh_info_t{};
+    m->learned_rows = km != nullptr;
     int k_used = p.k;
     if (!km) {
         // first pass: every seed at the configured k
@@ -707,6 +709,22 @@ int pg_mesh_info(const pg_mesh* m, pg_mesh_info_t* info) {
     return guarded([&] {
         argchk(m && info, "null argument");
         *info = m->info;
+    });
+}
+
+int pg_mesh_get_knn(pg_ctx* c, const pg_mesh* m, double* points, uint32_t* nearest, uint32_t* permutation) {
+    return guarded([&] {
+        use(c);
+        argchk(m != nullptr, "null mesh");
+        argchk(!m->learned_rows, "the mesh was built with learned k: its rows have per-seed lengths");
+        hipStream_t s = c->stream;
+        const size_t ns = (size_t)m->n_seeds;
+        if (points)
+            HIPCHK(hipMemcpyAsync(points, m->g.sorted.p, sizeof(double) * ns * m->prm.dim, hipMemcpyDeviceToHost, s));
+        if (nearest)
+            HIPCHK(hipMemcpyAsync(nearest, m->knn.p, sizeof(uint32_t) * ns * m->prm.k, hipMemcpyDeviceToHost, s));
+        if (permutation) HIPCHK(hipMemcpyAsync(permutation, m->g.perm.p, sizeof(uint32_t) * ns, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
     });
 }
 

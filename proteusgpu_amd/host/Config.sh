@@ -8,6 +8,9 @@ dim_3D
 # HDF5 for IC and snapshot files
 USE_HDF5
 
+# Dump each mesh build's KNN problem to output_knn_file (HDF5)
+#WRITE_KNN_OUTPUT
+
 # Recompute seeds that fail the security-radius certificate at a larger k
 # (comment out for the reference's behaviour: such cells stay failed)
 CERT_RESTART

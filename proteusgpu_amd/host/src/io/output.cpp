@@ -90,4 +90,27 @@ bool OutputHandler::writeMeshFile(const std::string& filename, const VMesh* m, c
     std::cout << "OUTPUT: wrote " << path << std::endl;
     return true;
 }
+
+bool OutputHandler::writeKNNFile(const std::string& filename, const double* points, const unsigned int* nearest,
+                                 const unsigned int* permutation, int num_points, int k) {
+    std::string path = outputDirectory + filename;  // output.cpp:312 joins without a separator
+    hid_t f = H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    if (f < 0) {
+        std::cerr << "OUTPUT: Error! Could not create file: " << path << std::endl;
+        return false;
+    }
+    hid_t h = H5Gcreate(f, "header", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    attr(h, "dimension", H5T_NATIVE_INT, (int)DIMENSION);
+    attr(h, "num_points", H5T_NATIVE_INT, num_points);
+    attr(h, "k", H5T_NATIVE_INT, k);
+    H5Gclose(h);
+    hid_t g = H5Gcreate(f, "knn", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    write(g, "points", H5T_NATIVE_DOUBLE, points, (hsize_t)num_points, DIMENSION);
+    write(g, "nearest", H5T_NATIVE_UINT, nearest, (hsize_t)num_points, (hsize_t)k);
+    write(g, "permutation", H5T_NATIVE_UINT, permutation, (hsize_t)num_points);
+    H5Gclose(g);
+    H5Fclose(f);
+    std::cout << "OUTPUT: wrote " << path << std::endl;
+    return true;
+}
 #endif

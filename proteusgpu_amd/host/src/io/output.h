@@ -18,6 +18,11 @@ class OutputHandler {
     std::string getOutputDirectory() const { return outputDirectory; }
 #ifdef USE_HDF5
     bool writeMeshFile(const std::string& filename, const VMesh* mesh, const primvars* primvar, int n_hydro);
+    // KNN dump (output.cpp:305-411): /header@{dimension,num_points,k},
+    // /knn/{points[num_points][d], nearest[num_points][k], permutation[num_points]},
+    // the first num_points rows of the sorted problem.
+    bool writeKNNFile(const std::string& filename, const double* points, const unsigned int* nearest,
+                      const unsigned int* permutation, int num_points, int k);
 #endif
    private:
     std::string outputDirectory;

@@ -1,12 +1,34 @@
 #include "voronoi.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <iostream>
 #include <vector>
 
+#if defined(USE_HDF5) && defined(WRITE_KNN_OUTPUT)
+#include "../io/input.h"
+#include "../io/output.h"
+#endif
+
 namespace voronoi {
 
 namespace {
+
+#if defined(USE_HDF5) && defined(WRITE_KNN_OUTPUT)
+// knn::write_knn_output (knn.cpp:457-473) for the build's own KNN problem:
+// the first icData.seedpos_dims[0] rows of the sorted points, neighbour rows
+// and permutation.
+void write_knn_dump(pg_mesh* dm, const pg_mesh_info_t& info) {
+    const size_t ns = (size_t)info.n_seeds;
+    std::vector<double> pts(ns * DIMENSION);
+    std::vector<unsigned int> rows(ns * (size_t)_K_), perm(ns);
+    pgpu_host::check(pg_mesh_get_knn(pgpu_host::ctx(), dm, pts.data(), rows.data(), perm.data()), "knn dump");
+    const int num_points = (int)std::min<size_t>(ns, (size_t)icData.seedpos_dims[0]);
+    if (!output.writeKNNFile(input.getParameter("output_knn_file"), pts.data(), rows.data(), perm.data(),
+                             num_points, _K_))
+        std::exit(EXIT_FAILURE);
+}
+#endif
 
 VMesh* build(POINT_TYPE* pts, int64_t n, int boundary) {
     std::cout << "VORONOI: Computing Voronoi mesh..." << std::endl;
@@ -20,6 +42,9 @@ VMesh* build(POINT_TYPE* pts, int64_t n, int boundary) {
     pgpu_host::check(rc, "voronoi::compute_mesh");
     pg_mesh_info_t info;
     pg_mesh_info(dm, &info);
+#if defined(USE_HDF5) && defined(WRITE_KNN_OUTPUT)
+    write_knn_dump(dm, info);  // voronoi.cpp:42-44
+#endif
     const size_t ns = (size_t)info.n_seeds, nf = (size_t)info.num_faces;
     VMesh* m = (VMesh*)std::calloc(1, sizeof(VMesh));
     m->n_seeds = ns;

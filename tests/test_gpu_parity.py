@@ -162,3 +162,20 @@ def test_restart_resume_equals_scratch(ctx, dim, n, k, lanes, slots):
     for key in ga:
         assert np.array_equal(ga[key], gb[key]), key
     _cmp_mesh(ga, O.Mesh(pts, "periodic", buff=0.12, k=k, restart=True))
+
+
+@pytest.mark.parametrize("boundary", ["none", "periodic"])
+def test_mesh_knn_problem_equals_oracle(ctx, boundary):
+    """pg_mesh_get_knn (the KNN dump of a mesh build): sorted points, rows and
+    permutation equal the oracle's knn problem of the same build."""
+    pts = uniform_points(13000, 3, 13)
+    m = api.Mesh(ctx, pts, boundary=boundary, k=32, restart=True)
+    P, rows, perm = m.knn()
+    ref = O.Mesh(pts, boundary, k=32, restart=False)
+    assert np.array_equal(P, ref.get("sorted_points"))
+    assert np.array_equal(rows, ref.get("knearest"))
+    assert np.array_equal(perm, ref.get("permutation"))
+    km = api.KModel(ctx)
+    from proteusgpu_amd._native import PgError
+    with pytest.raises(PgError):
+        api.Mesh(ctx, pts, boundary=boundary, k=32, kmodel=km).knn()
