@@ -383,7 +383,12 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
     int status = SUCCESS;
     int kc = 0;
     bool done = !valid;
-    double R2 = 0.0;
+    double R2 = 0.0;  // largest squared seed-vertex distance of the cell
+#pragma unroll
+    for (int s = 0; s < S; s++)
+        if ((R.live >> s) & 1u) R2 = smax(R.d2[s], R2);
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) R2 = smax(shfl_xor(R2, d, G), R2);
 
     // --------------------------------------------------------- clip loop
     // Seeds of a wave may start at different neighbours (resumed cells):
@@ -555,25 +560,50 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
         // -- consistency and capacity
         bool ok = doclip;
         bool bad = false;
+        int nxt = gl, lab = 0;  // 3D: successor edge and orbit label of edge gl
         if (doclip) {
             if (D == 3) {
+                // every boundary edge owns its start and end plane, and the
+                // edge continuing it (the one starting at its end plane)
+                // exists: the successor map is a permutation of the Bt edges
                 for (int jj = gl; jj < Bt; jj += G) {
                     uint32_t ed = sm.edge[jj];
                     int a = ed & 0xff, b = (ed >> 8) & 0xff;
                     if (sm.by_start[a] != jj || sm.by_end[b] != jj) bad = true;
+                    const int nx = sm.by_start[b];
+                    if (nx >= Bt || (int)(sm.edge[nx] & 0xffu) != b) bad = true;
                 }
-                if (gl == 0 && !bad) {
-                    // the boundary must be one cycle through all Bt edges
-                    int jj = 0, steps = 0;
-                    do {
-                        int b = (sm.edge[jj] >> 8) & 0xff;
-                        jj = sm.by_start[b];
-                        steps++;
-                    } while (jj != 0 && steps <= Bt);
-                    if (steps != Bt) bad = true;
+                if (Bt <= G && gl < Bt) {
+                    lab = gl;
+                    if (!bad) nxt = sm.by_start[(sm.edge[gl] >> 8) & 0xffu];
                 }
             } else {
                 bad = (Bt != 2);
+            }
+        }
+        if (D == 3) {
+            // ... and one cycle through all Bt edges: the smallest edge index
+            // on each orbit, by pointer jumping over the group's lanes (all
+            // groups of the wave take part; lanes without an edge point to
+            // themselves), must be 0 everywhere
+            if (nxt >= G) nxt = gl;  // only after a failed check above
+#pragma unroll
+            for (int r = 1; r < 16 && r < G; r <<= 1) {
+                const int l2 = shfl(lab, nxt, G);
+                nxt = shfl(nxt, nxt, G);
+                lab = min(lab, l2);
+            }
+            if (doclip) {
+                if (Bt <= G) {
+                    if (lab != 0) bad = true;
+                } else if (gl == 0 && !bad) {
+                    int jj = 0, steps = 0;
+                    do {
+                        jj = sm.by_start[(sm.edge[jj] >> 8) & 0xffu];
+                        steps++;
+                    } while (jj != 0 && jj < Bt && steps <= Bt);
+                    if (steps != Bt || jj != 0) bad = true;
+                }
             }
         }
         bad = ((wave_ballot(bad) >> shift) & gmask) != 0ull;
@@ -664,13 +694,17 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             nlive += Bt;
         }
         block_sync();
-        // -- certificate (cell.cpp:423-438) with the just-clipped neighbour
-        double mx = 0.0;
+        // -- certificate (cell.cpp:423-438) with the just-clipped neighbour;
+        //    the largest seed-vertex distance only changes with a clip
+        double mx = R2;
+        if (wave_ballot(doclip) != 0ull) {
+            mx = 0.0;
 #pragma unroll
-        for (int s = 0; s < S; s++)
-            if ((R.live >> s) & 1u) mx = smax(R.d2[s], mx);
+            for (int s = 0; s < S; s++)
+                if ((R.live >> s) & 1u) mx = smax(R.d2[s], mx);
 #pragma unroll
-        for (int d = 1; d < G; d <<= 1) mx = smax(shfl_xor(mx, d, G), mx);
+            for (int d = 1; d < G; d <<= 1) mx = smax(shfl_xor(mx, d, G), mx);
+        }
         if (act) {
             R2 = mx;
             const double dx = bx - sx, dy = by - sy, dz = bz - sz;
