@@ -502,18 +502,22 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
                 doclip = true;
             }
         }
-        // -- cavity boundary: edges of removed vertices whose neighbour is kept
-        int mycnt = 0;
+        // -- cavity boundary: edges of removed vertices whose neighbour is
+        //    kept; bit 3 s + e of bm = edge e of my slot s (a lane walks only
+        //    its removed slots and then only its boundary edges)
+        uint32_t bm = 0;
         if (doclip) {
-#pragma unroll
-            for (int s = 0; s < S; s++) {
-                if (!((rem >> s) & 1u)) continue;
+            uint32_t rr = rem;
+            while (rr) {
+                const int s = __builtin_ctz(rr);
+                rr &= rr - 1u;
                 const int gs = s * G + gl;
 #pragma unroll
                 for (int ee = 0; ee < D; ee++)
-                    if (!mask_has<NW>(Rm, sm.adj[ee][gs])) mycnt++;
+                    if (!mask_has<NW>(Rm, sm.adj[ee][gs])) bm |= 1u << (3 * s + ee);
             }
         }
+        const int mycnt = __builtin_popcount(bm);
         int incl = mycnt;
 #pragma unroll
         for (int d = 1; d < G; d <<= 1) {
@@ -523,30 +527,29 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
         const int Bt = shfl(incl, G - 1, G);
         int j = incl - mycnt;
         if (doclip) {
-#pragma unroll
-            for (int s = 0; s < S; s++) {
-                if (!((rem >> s) & 1u)) continue;
+            // in (slot, edge) order, as counted
+            uint32_t b2 = bm;
+            while (b2) {
+                const int bit = __builtin_ctz(b2);
+                b2 &= b2 - 1u;
+                if (j >= NS) {  // more boundary edges than slots: overflow below
+                    j++;
+                    continue;
+                }
+                const int s = bit / 3, ee = bit - 3 * s;
                 const int gs = s * G + gl;
                 const uint32_t t = sm.tri[gs];
-#pragma unroll
-                for (int ee = 0; ee < D; ee++) {
-                    const int K = sm.adj[ee][gs];
-                    if (mask_has<NW>(Rm, K)) continue;
-                    if (j >= NS) {  // more boundary edges than slots: overflow below
-                        j++;
-                        continue;
-                    }
-                    if (D == 3) {
-                        const int a = tri_p(t, ee), b = tri_p(t, (ee + 1) % 3);
-                        sm.edge[j] = (uint32_t)a | (uint32_t)b << 8 | (uint32_t)K << 16;
-                        sm.by_start[a] = (uint8_t)j;
-                        sm.by_end[b] = (uint8_t)j;
-                    } else {
-                        const int L = tri_p(t, ee);
-                        sm.edge[j] = (uint32_t)L | (uint32_t)K << 16;
-                    }
-                    j++;
+                const int K = sm.adj[ee][gs];
+                if (D == 3) {
+                    const int a = tri_p(t, ee), b = tri_p(t, ee == 2 ? 0 : ee + 1);
+                    sm.edge[j] = (uint32_t)a | (uint32_t)b << 8 | (uint32_t)K << 16;
+                    sm.by_start[a] = (uint8_t)j;
+                    sm.by_end[b] = (uint8_t)j;
+                } else {
+                    const int L = tri_p(t, ee);
+                    sm.edge[j] = (uint32_t)L | (uint32_t)K << 16;
                 }
+                j++;
             }
         }
         // live mask of the group (after removal)
