@@ -162,58 +162,75 @@ PG_DEV void knn_heap_query(const double* __restrict__ sorted, const uint32_t* __
 
 // ------------------------------------------------------------------------
 // Wave-per-query variant.  The 64 lanes of a wave cooperate on one query:
-// the best 64 M (dist^2, original id, sorted id) are kept sorted in
-// registers, slot-major (rank r at slot r / 64 of lane r % 64, M =
-// ceil(k/64) slots).  Ring cells are enumerated 64 at a time, their point
-// ranges prefix-summed across the wave, and candidates processed 64 at a
-// time: those that beat the current k-th entry (broadcast) are sorted
-// across the wave by a bitonic network and merged into the kept list, one
-// 64-entry slot at a time (lower half of slot + reversed batch stays, the
-// upper half carries into the next slot).
+// the best 64 M (dist^2, sorted id) are kept sorted in registers,
+// slot-major (rank r at slot r / 64 of lane r % 64, M = ceil(k/64) slots).
+// Ring cells are enumerated 64 at a time, their point ranges prefix-summed
+// across the wave, and candidates processed 64 at a time: those that beat
+// the current k-th entry (broadcast) gather in LDS; every 64 of them are
+// sorted across the wave by a bitonic network and merged into the kept
+// list, one 64-entry slot at a time (lower half of slot + reversed batch
+// stays, the upper half carries into the next slot).
 // Same pruning rule and same total order as k_knn, no LDS heap, full waves:
-// used for long rows (certificate restarts), where the per-thread heap would
-// leave the CUs nearly empty.
-struct Cand {
-    double d;
-    uint32_t o, s;  // original id (ties), sorted id (output)
-};
+// used for rows of 20 and more, where the per-thread heap would leave the
+// CUs nearly empty.
 
 PG_DEV bool cand_less(double da, uint32_t oa, double db, uint32_t ob) {
     return da < db || (da == db && oa < ob);
 }
 
+// The wave kernel's entries are (FP64 d^2, sorted id); the order is
+// (d^2, original id), and the original id (perm of the sorted id) is read
+// only for equal distances.  Padding entries are (+inf, 0xffffffff).
+PG_DEV uint32_t orig_id(const uint32_t* __restrict__ perm, uint32_t s) {
+    return s == 0xffffffffu ? 0xffffffffu : perm[s];
+}
+PG_DEV bool ent_less(const uint32_t* __restrict__ perm, double da, uint32_t sa, double db, uint32_t sb) {
+    if (da != db) return da < db;
+    return orig_id(perm, sa) < orig_id(perm, sb);
+}
+
 // Compare-exchange with lane l ^ j: keep the smaller (keep_min) or the
-// larger of the pair under cand_less.
-PG_DEV void wave_cx(double& d, uint32_t& o, uint32_t& s, int j, bool keep_min) {
+// larger of the pair under ent_less.
+PG_DEV void wave_cx(const uint32_t* __restrict__ perm, double& d, uint32_t& s, int j, bool keep_min) {
     const double pd = __shfl_xor(d, j, 64);
-    const uint32_t po = __shfl_xor(o, j, 64), ps = __shfl_xor(s, j, 64);
-    const bool other_less = cand_less(pd, po, d, o);
+    const uint32_t ps = __shfl_xor(s, j, 64);
+    const bool other_less = ent_less(perm, pd, ps, d, s);
     if (other_less == keep_min) {
         d = pd;
-        o = po;
         s = ps;
     }
 }
 
+// t / d for 0 <= t < 2^22, 1 <= d, rd = 1/d rounded: the float quotient is
+// within one of the exact one (relative error below 2^-22 on a value below
+// 2^22), and one step corrects it.
+PG_DEV int idiv_small(int t, int d, float rd) {
+    int q = (int)((float)t * rd);
+    const int r = t - q * d;
+    if (r < 0) q--;
+    else if (r >= d) q++;
+    return q;
+}
+
 // Bitonic sort of one entry per lane, ascending with the lane index.
-PG_DEV void wave_sort64(double& d, uint32_t& o, uint32_t& s, int l) {
+PG_DEV void wave_sort64(const uint32_t* __restrict__ perm, double& d, uint32_t& s, int l) {
 #pragma unroll
     for (int k = 2; k <= 64; k <<= 1)
 #pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) wave_cx(d, o, s, j, ((l & j) == 0) == ((l & k) == 0 || k == 64));
+        for (int j = k >> 1; j > 0; j >>= 1) wave_cx(perm, d, s, j, ((l & j) == 0) == ((l & k) == 0 || k == 64));
 }
 
 // Sort a bitonic sequence held one entry per lane (ascending).
-PG_DEV void wave_merge64(double& d, uint32_t& o, uint32_t& s, int l) {
+PG_DEV void wave_merge64(const uint32_t* __restrict__ perm, double& d, uint32_t& s, int l) {
 #pragma unroll
-    for (int j = 32; j > 0; j >>= 1) wave_cx(d, o, s, j, (l & j) == 0);
+    for (int j = 32; j > 0; j >>= 1) wave_cx(perm, d, s, j, (l & j) == 0);
 }
 
 // Candidates that beat the k-th kept entry wait in LDS until 64 have
 // gathered; only then are they sorted and merged into the kept list.
 struct KnnPend {
     double d[128];
-    uint32_t o[128], s[128];
+    uint32_t s[128];
 };
 
 // Merge 64 candidates (one per lane, unsorted, +inf padded) into the kept
@@ -221,25 +238,23 @@ struct KnnPend {
 // slot by slot, the lower half of (slot, batch) stays and the upper half
 // carries on.
 template <int M>
-PG_DEV void wave_merge_batch(double (&kd)[M], uint32_t (&ko)[M], uint32_t (&ks)[M], double dd, uint32_t co,
+PG_DEV void wave_merge_batch(const uint32_t* __restrict__ perm, double (&kd)[M], uint32_t (&ks)[M], double dd,
                              uint32_t cs, int l) {
-    wave_sort64(dd, co, cs, l);
+    wave_sort64(perm, dd, cs, l);
 #pragma unroll
     for (int jj = 0; jj < M; jj++) {
         const double rd = __shfl(dd, 63 - l, 64);
-        const uint32_t ro = __shfl(co, 63 - l, 64), rs = __shfl(cs, 63 - l, 64);
-        const bool mine = cand_less(kd[jj], ko[jj], rd, ro);
+        const uint32_t rs = __shfl(cs, 63 - l, 64);
+        const bool mine = ent_less(perm, kd[jj], ks[jj], rd, rs);
         double hd = mine ? rd : kd[jj];
-        uint32_t ho = mine ? ro : ko[jj], hs = mine ? rs : ks[jj];
+        uint32_t hs = mine ? rs : ks[jj];
         if (!mine) {
             kd[jj] = rd;
-            ko[jj] = ro;
             ks[jj] = rs;
         }
-        wave_merge64(kd[jj], ko[jj], ks[jj], l);
-        wave_merge64(hd, ho, hs, l);
+        wave_merge64(perm, kd[jj], ks[jj], l);
+        wave_merge64(perm, hd, hs, l);
         dd = hd;
-        co = ho;
         cs = hs;
     }
 }
@@ -263,11 +278,10 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
     const int ci = c % N, cj = (c / N) % N, ck = DIM == 3 ? c / (N * N) : 0;
 
     double kd[M];
-    uint32_t ko[M], ks[M];
+    uint32_t ks[M];
 #pragma unroll
     for (int j = 0; j < M; j++) {
         kd[j] = __builtin_inf();
-        ko[j] = 0xffffffffu;
         ks[j] = 0xffffffffu;
     }
     int size = 0, npend = 0;
@@ -276,13 +290,12 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
     auto flush = [&](int n) {
         __syncthreads();
         double dd = __builtin_inf();
-        uint32_t co = 0xffffffffu, cs = 0xffffffffu;
+        uint32_t cs = 0xffffffffu;
         if (l < n) {
             dd = pend->d[l];
-            co = pend->o[l];
             cs = pend->s[l];
         }
-        wave_merge_batch<M>(kd, ko, ks, dd, co, cs, l);
+        wave_merge_batch<M>(perm, kd, ks, dd, cs, l);
         size = min(k, size + n);
     };
 
@@ -301,6 +314,7 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
         }
         const int side = 2 * ring + 1;
         const int ncube = DIM == 3 ? side * side * side : side * side;
+        const float rside = 1.0f / (float)side;
         for (int base = 0; base < ncube; base += 64) {
             // this lane's cell of the cube, if it lies on the ring and in the grid
             const int t = base + l;
@@ -313,8 +327,17 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
                     if (j == kj) prune = __shfl(kd[j], kl, 64);
             }
             if (t < ncube) {
-                const int di = t % side - ring, dj = (t / side) % side - ring;
-                const int dk = DIM == 3 ? t / (side * side) - ring : 0;
+                // t = (dk + ring) side^2 + (dj + ring) side + (di + ring)
+                int q1, q2;
+                if (ncube < (1 << 22)) {  // float quotient, one correction step
+                    q1 = idiv_small(t, side, rside);
+                    q2 = idiv_small(q1, side, rside);
+                } else {
+                    q1 = t / side;
+                    q2 = q1 / side;
+                }
+                const int di = t - q1 * side - ring, dj = q1 - q2 * side - ring;
+                const int dk = DIM == 3 ? q2 - ring : 0;
                 const int cheb = max(abs(di), max(abs(dj), abs(dk)));
                 const int ii = ci + di, jj = cj + dj, kk = ck + dk;
                 if (cheb == ring && ii >= 0 && jj >= 0 && kk >= 0 && ii < N && jj < N && (DIM == 2 || kk < N) &&
@@ -340,7 +363,7 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
                 const int j = cb + l;
                 bool take = false;
                 double dd = 0;
-                uint32_t co = 0, cs = 0;
+                uint32_t cs = 0;
                 if (j < total) {
                     // cell holding candidate j: largest a with c_pfx[a] <= j
                     int lo = 0, hi = 63;
@@ -355,20 +378,19 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
 #pragma unroll
                         for (int d = 0; d < DIM; d++) pc[d] = sorted[(size_t)DIM * cs + d];
                         dd = dist2<DIM>(p, pc);
-                        co = perm[cs];
                         take = true;
                     }
                 }
                 if (size == k) {
                     double kth = 0;
-                    uint32_t kto = 0;
+                    uint32_t kts = 0;
 #pragma unroll
                     for (int jj = 0; jj < M; jj++)
                         if (jj == kj) {
                             kth = __shfl(kd[jj], kl, 64);
-                            kto = __shfl(ko[jj], kl, 64);
+                            kts = __shfl(ks[jj], kl, 64);
                         }
-                    take = take && cand_less(dd, co, kth, kto);
+                    take = take && ent_less(perm, dd, cs, kth, kts);
                 }
                 const uint64_t surv = (uint64_t)__ballot(take);
                 if (surv == 0ull) continue;
@@ -377,7 +399,6 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
                 if (take) {
                     const int at = npend + __popcll((unsigned long long)(surv & ((1ull << l) - 1ull)));
                     pend->d[at] = dd;
-                    pend->o[at] = co;
                     pend->s[at] = cs;
                 }
                 npend += __popcll((unsigned long long)surv);
@@ -385,17 +406,15 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
                     flush(64);
                     // move the rest (fewer than 64) to the front
                     double rd = 0;
-                    uint32_t ro = 0, rs = 0;
+                    uint32_t rs = 0;
                     const int rest = npend - 64;
                     if (l < rest) {
                         rd = pend->d[64 + l];
-                        ro = pend->o[64 + l];
                         rs = pend->s[64 + l];
                     }
                     __syncthreads();
                     if (l < rest) {
                         pend->d[l] = rd;
-                        pend->o[l] = ro;
                         pend->s[l] = rs;
                     }
                     npend = rest;
