@@ -140,7 +140,7 @@ struct pg_mesh {
     DBuf feat, kcls, klist, kcnt;  // learned k: features, bucket per seed, seeds by bucket, bucket sizes
     // faces (padded, face-slot major) and cells, sorted order
     DBuf nbr, area, nrm, cen, count, vol, ccen, status, kcert;
-    DBuf cstate, resume;  // saved cells of uncertified seeds (restarts resume them)
+    DBuf cstate, resume, lastnb;  // saved cells of uncertified seeds (restarts resume them)
     bool use_resume = false;
     bool learned_rows = false;  // first pass with per-seed row lengths (knn rows not kept)
     // hydro tables
@@ -163,6 +163,7 @@ struct pg_mesh {
         if (use_resume) {
             fa.cstate = cstate.as<uint8_t>();
             fa.resume = resume.as<int32_t>();
+            fa.lastnb = lastnb.as<uint32_t>();
         }
         return fa;
     }
@@ -204,10 +205,16 @@ int64_t collect_list(pg_mesh* m, const uint32_t* seeds, int64_t nq, int code, ui
     return read_i32(c, s);
 }
 
+// The shape tag under which the fast clip kernel saves and resumes cells.
+int32_t clip_shape_tag(int lanes, int slots) { return (lanes << 24) | ((slots / lanes) << 16); }
+
 void run_list(pg_mesh* m, const uint32_t* list, int64_t nq, int k, int big, hipStream_t s) {
     uint32_t* rows = m->rows.get<uint32_t>((size_t)nq * k);
+    pg::KnnTail tail;  // seeds that resume need only the rest of their row
+    if (m->use_resume && !big)
+        tail = {m->resume.as<int32_t>(), m->lastnb.as<uint32_t>(), clip_shape_tag(pg::clip_lanes(), pg::clip_slots())};
     pg::launch_knn(m->prm.dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), m->n_seeds, m->g.N,
-                   m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), list, nq, k, rows, s);
+                   m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), list, nq, k, rows, s, tail);
     pg::launch_voronoi(m->prm.dim, m->g.sorted.as<double>(), m->n_seeds, list, nq, rows, k,
                        m->prm.max_p, m->prm.max_t, big, m->faces(), s, true);
     HIPCHK(hipGetLastError());
@@ -332,7 +339,10 @@ void mesh_build(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool on_dev
     m->use_resume = p.restart && pg::clip_resume();
     if (m->use_resume) {
         m->cstate.get<uint8_t>(ns * pg::CLIP_STATE_BYTES);
-        m->resume.get<int32_t>(ns);
+        m->lastnb.get<uint32_t>(ns);
+        // saved cells belong to this build only: no pass may resume a cell
+        // of an earlier build (or of recycled memory)
+        pg::launch_fill_i32(m->resume.get<int32_t>(ns), (int64_t)ns, 0, s);
     }
     m->finalized = false;
 
@@ -1369,7 +1379,7 @@ struct pg_batch {
     DBuf cell, counts, starts, fill, tmp, total, perm, sorted, sorted_box, big;
     DBuf knn, lists, ncount, flags, offs;
     DBuf nbr, area, nrm, cen, count, vol, ccen, status, kcert;
-    DBuf cstate, resume;  // saved cells of uncertified seeds (restarts resume them)
+    DBuf cstate, resume, lastnb;  // saved cells of uncertified seeds (restarts resume them)
     DBuf ticket;          // work-queue counter of the persistent kernels
     DBuf hid, area_u, vol_u, o2s, hsorted, tot;
     hipGraphExec_t exec = nullptr;
@@ -1399,6 +1409,7 @@ struct pg_batch {
         if (resume.p) {
             fa.cstate = cstate.as<uint8_t>();
             fa.resume = resume.as<int32_t>();
+            fa.lastnb = lastnb.as<uint32_t>();
         }
         return fa;
     }
@@ -1418,6 +1429,8 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
     const pg_mesh_params& p = B->prm.mesh;
     const int64_t cap = B->cap;
     pg::launch_fill_i32(B->st(), 4, 0, s);
+    // saved cells belong to this rebuild only
+    if (B->resume.p) pg::launch_fill_i32(B->resume.as<int32_t>(), cap, 0, s);
     const int32_t* goff = nullptr;
     if (B->periodic) {
         pg::launch_ghost_count(dim, B->x.as<double>(), B->n_h, p.buff, B->gcnt.as<int32_t>(), s);
@@ -1435,11 +1448,14 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
                      B->coff.as<int64_t>(), B->cell.as<int32_t>(), B->counts.as<int32_t>(), B->starts.as<int32_t>(),
                      B->fill.as<int32_t>(), B->tmp.as<int32_t>(), B->total.as<int32_t>(), B->perm.as<uint32_t>(),
                      B->sorted.as<double>(), B->sorted_box.as<int32_t>(), B->big.as<int32_t>(), s);
-    auto knn = [&](const uint32_t* q, const int64_t* nq, int k) {
+    auto knn = [&](const uint32_t* q, const int64_t* nq, int k, bool big) {
+        pg::KnnTail tail;  // seeds that resume need only the rest of their row
+        if (B->resume.p && !big)
+            tail = {B->resume.as<int32_t>(), B->lastnb.as<uint32_t>(), clip_shape_tag(B->lanes, B->slots)};
         pg::launch_bknn(dim, B->sorted.as<double>(), B->perm.as<uint32_t>(), B->sorted_box.as<int32_t>(),
                         B->N.as<int32_t>(), B->coff.as<int64_t>(), B->starts.as<int32_t>(), B->counts.as<int32_t>(),
                         q, nq, cap, k, B->knn.as<uint32_t>(), B->wave_min_k, B->bpc,
-                        B->ticket.as<unsigned long long>(), s);
+                        B->ticket.as<unsigned long long>(), s, tail);
     };
     auto clip = [&](const uint32_t* q, const int64_t* nq, int k, int big) {
         pg::launch_bvoronoi(dim, B->sorted.as<double>(), q, nq, cap, B->knn.as<uint32_t>(), k, p.max_p, p.max_t, big,
@@ -1454,7 +1470,7 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
     const std::vector<int>& sched = B->km ? B->ksched_learned : B->ksched;
     int k = sched[0];
     if (!B->km) {
-        knn(nullptr, n_dev, k);
+        knn(nullptr, n_dev, k, false);
         clip(nullptr, n_dev, k, 0);
     } else {
         float* feat = B->feat.as<float>();
@@ -1467,7 +1483,7 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
                            B->total.as<int32_t>(), cap, B->klists.as<uint32_t>(), B->kcnt.as<int64_t>(), s);
         for (int c = 0; c < pg::LK_NCLASS; c++) {
             const uint32_t* lc = B->klists.as<uint32_t>() + (size_t)c * cap;
-            knn(lc, B->kcnt.as<int64_t>() + c, pg::lk_bucket(c));
+            knn(lc, B->kcnt.as<int64_t>() + c, pg::lk_bucket(c), false);
             clip(lc, B->kcnt.as<int64_t>() + c, pg::lk_bucket(c), 0);
         }
     }
@@ -1477,13 +1493,13 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
     const int64_t* nL = n_dev;
     for (size_t r = 1; r <= sched.size(); r++) {
         collect(L, nL, pg::SLOT_OVERFLOW, lst[2], cnt[2]);
-        knn(lst[2], cnt[2], k);
+        knn(lst[2], cnt[2], k, true);
         clip(lst[2], cnt[2], k, 1);
         if (r == sched.size() || !p.restart) break;
         const int d = (L == lst[0]) ? 1 : 0;
         collect(L, nL, PG_SECURITY_RADIUS_NOT_REACHED, lst[d], cnt[d]);
         k = sched[r];
-        knn(lst[d], cnt[d], k);
+        knn(lst[d], cnt[d], k, false);
         clip(lst[d], cnt[d], k, 0);
         L = lst[d];
         nL = cnt[d];
@@ -1715,6 +1731,7 @@ int pg_batch_create(pg_ctx* c, const pg_sim_params* prm, int nbox, const int64_t
             B->ticket.get<unsigned long long>(1);
             if (B->prm.mesh.restart && pg::clip_resume()) {
                 B->cstate.get<uint8_t>(cs * pg::CLIP_STATE_BYTES);
+                B->lastnb.get<uint32_t>(cs);
                 B->resume.get<int32_t>(cs);
             }
             B->hid.get<int32_t>(cs * MAXP);

@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(64)
                 const int32_t* __restrict__ sbox, const int32_t* __restrict__ N, const int64_t* __restrict__ coff,
                 const int32_t* __restrict__ starts, const int32_t* __restrict__ counts,
                 const uint32_t* __restrict__ queries, const int64_t* __restrict__ nq_dev, int k,
-                uint32_t* __restrict__ out, unsigned long long* ticket) {
+                uint32_t* __restrict__ out, unsigned long long* ticket, KnnTail tail) {
     __shared__ int32_t c_start[64], c_pfx[65];
     __shared__ KnnPend pend;
     const int64_t nq = *nq_dev;
@@ -234,8 +234,11 @@ __global__ void __launch_bounds__(64)
         for (int64_t qi = q0; qi < q1; qi++) {
             const uint32_t q = queries ? queries[qi] : (uint32_t)qi;
             const int b = sbox[q];
+            int kp = 0;
+            uint32_t lo = 0;
+            tail_of(tail, q, k, kp, lo);
             knn_wave_query<DIM, M>(sorted, perm, N[b], coff[b], starts, counts, q, k, out + (size_t)qi * k,
-                                   c_start, c_pfx, &pend);
+                                   c_start, c_pfx, &pend, kp, lo);
             __syncthreads();
         }
     }
@@ -503,23 +506,23 @@ template <int DIM, int M>
 void bknn_wave(const double* sorted, const uint32_t* perm, const int32_t* sbox, const int32_t* N,
                const int64_t* coff, const int32_t* starts, const int32_t* counts, const uint32_t* q,
                const int64_t* nq_dev, int64_t cap, int k, uint32_t* out, int blocks_per_cu,
-               unsigned long long* ticket, hipStream_t s) {
+               unsigned long long* ticket, hipStream_t s, KnnTail tail) {
     static const int occ = occupancy(k_bknn_wave<DIM, M>);
     const unsigned nb = (unsigned)std::min<int64_t>(cap, resident_blocks(occ, 8 * blocks_per_cu));
     k_fill<unsigned long long><<<1, TB, 0, s>>>(ticket, 1, 0ull);
     k_bknn_wave<DIM, M><<<std::max(nb, 1u), 64, 0, s>>>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, k,
-                                                        out, ticket);
+                                                        out, ticket, tail);
 }
 template <int DIM>
 void bknn_dim(const double* sorted, const uint32_t* perm, const int32_t* sbox, const int32_t* N,
               const int64_t* coff, const int32_t* starts, const int32_t* counts, const uint32_t* q,
               const int64_t* nq_dev, int64_t cap, int k, uint32_t* out, int wave_min_k, int blocks_per_cu,
-              unsigned long long* ticket, hipStream_t s) {
+              unsigned long long* ticket, hipStream_t s, KnnTail tail) {
     if (k >= wave_min_k) {
-        if (k <= 64) bknn_wave<DIM, 1>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, ticket, s);
-        else if (k <= 128) bknn_wave<DIM, 2>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, ticket, s);
-        else if (k <= 192) bknn_wave<DIM, 3>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, ticket, s);
-        else bknn_wave<DIM, 4>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, ticket, s);
+        if (k <= 64) bknn_wave<DIM, 1>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, ticket, s, tail);
+        else if (k <= 128) bknn_wave<DIM, 2>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, ticket, s, tail);
+        else if (k <= 192) bknn_wave<DIM, 3>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, ticket, s, tail);
+        else bknn_wave<DIM, 4>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, ticket, s, tail);
     } else if (k <= 208) {
         bknn_heap<DIM, 64>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, cap, k, out, blocks_per_cu, s);
     } else {
@@ -541,13 +544,13 @@ void bclip(const double* pts, const uint32_t* seeds, const int64_t* nq_dev, int6
 void launch_bknn(int dim, const double* sorted, const uint32_t* perm, const int32_t* sbox, const int32_t* N,
                  const int64_t* coff, const int32_t* starts, const int32_t* counts, const uint32_t* queries,
                  const int64_t* nq_dev, int64_t cap, int k, uint32_t* out, int wave_min_k, int blocks_per_cu,
-                 unsigned long long* ticket, hipStream_t s) {
+                 unsigned long long* ticket, hipStream_t s, KnnTail tail) {
     if (dim == 2)
         bknn_dim<2>(sorted, perm, sbox, N, coff, starts, counts, queries, nq_dev, cap, k, out, wave_min_k,
-                    blocks_per_cu, ticket, s);
+                    blocks_per_cu, ticket, s, tail);
     else
         bknn_dim<3>(sorted, perm, sbox, N, coff, starts, counts, queries, nq_dev, cap, k, out, wave_min_k,
-                    blocks_per_cu, ticket, s);
+                    blocks_per_cu, ticket, s, tail);
 }
 
 void launch_bvoronoi(int dim, const double* sorted, const uint32_t* seeds, const int64_t* nq_dev, int64_t cap,

@@ -37,11 +37,21 @@ void launch_gather_points(int dim, int64_t n, const uint32_t* perm, const double
                           double* sorted, hipStream_t s);
 
 // ----------------------------------------------------------------- knn.hip
+// A KNN launch for seeds whose clip will resume: their rows need only the
+// neighbours after the saved row length (resume[s] & 0xffff, the last of
+// them lastnb[s]); others get full rows.  tag: the shape tag the clip
+// kernel resumes under.  resume == nullptr: full rows for every seed.
+struct KnnTail {
+    const int32_t* resume = nullptr;
+    const uint32_t* lastnb = nullptr;
+    int32_t tag = 0;
+};
+
 // k nearest neighbours of queries (sorted ids; nullptr = all n points).
 // Rows ascending by (FP64 dist^2, original index); out row stride k.
 void launch_knn(int dim, const double* sorted, const uint32_t* perm, int64_t n, int N,
                 const int32_t* starts, const int32_t* counts, const uint32_t* queries, int64_t nq,
-                int k, uint32_t* out, hipStream_t s);
+                int k, uint32_t* out, hipStream_t s, KnnTail tail = {});
 
 // ------------------------------------------------------------- voronoi.hip
 struct FaceArrays {
@@ -62,7 +72,9 @@ struct FaceArrays {
     // length, else 0.  nullptr: nothing is saved or resumed.
     uint8_t* cstate = nullptr;
     int32_t* resume = nullptr;
+    uint32_t* lastnb = nullptr;  // saved seeds: the last neighbour of the row (sorted id)
 };
+
 
 // Face slots per cell in the padded face arrays (the reference's _MAX_P_).
 constexpr int FACE_SLOTS = 64;
@@ -158,7 +170,7 @@ void launch_bgrid(int dim, const double* pts, const int64_t* n_dev, int64_t cap,
 void launch_bknn(int dim, const double* sorted, const uint32_t* perm, const int32_t* sbox, const int32_t* N,
                  const int64_t* coff, const int32_t* starts, const int32_t* counts, const uint32_t* queries,
                  const int64_t* nq_dev, int64_t cap, int k, uint32_t* out, int wave_min_k, int blocks_per_cu,
-                 unsigned long long* ticket, hipStream_t s);
+                 unsigned long long* ticket, hipStream_t s, KnnTail tail = {});
 // ticket: a device counter, the work queue of the persistent kernels
 // (cleared by the launcher, in stream order)
 void launch_bvoronoi(int dim, const double* sorted, const uint32_t* seeds, const int64_t* nq_dev, int64_t cap,

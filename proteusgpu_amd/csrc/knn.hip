@@ -35,30 +35,35 @@ template <int DIM, int M>
 __global__ void __launch_bounds__(64)
     k_knn_wave(const double* __restrict__ sorted, const uint32_t* __restrict__ perm, int N,
                const int32_t* __restrict__ starts, const int32_t* __restrict__ counts,
-               const uint32_t* __restrict__ queries, int64_t nq, int k, uint32_t* __restrict__ out) {
+               const uint32_t* __restrict__ queries, int64_t nq, int k, uint32_t* __restrict__ out,
+               KnnTail tail) {
     __shared__ int32_t c_start[64], c_pfx[65];
     __shared__ KnnPend pend;
     const int64_t qi = blockIdx.x;
     if (qi >= nq) return;  // whole block
     const uint32_t q = queries ? queries[qi] : (uint32_t)qi;
-    knn_wave_query<DIM, M>(sorted, perm, N, 0, starts, counts, q, k, out + (size_t)qi * k, c_start, c_pfx, &pend);
+    int kp = 0;
+    uint32_t lo = 0;
+    tail_of(tail, q, k, kp, lo);
+    knn_wave_query<DIM, M>(sorted, perm, N, 0, starts, counts, q, k, out + (size_t)qi * k, c_start, c_pfx, &pend,
+                           kp, lo);
 }
 
 template <int DIM, int M>
 void run_wave(const double* sorted, const uint32_t* perm, int N, const int32_t* starts,
               const int32_t* counts, const uint32_t* queries, int64_t nq, int k, uint32_t* out,
-              hipStream_t s) {
-    k_knn_wave<DIM, M><<<(unsigned)nq, 64, 0, s>>>(sorted, perm, N, starts, counts, queries, nq, k, out);
+              KnnTail tail, hipStream_t s) {
+    k_knn_wave<DIM, M><<<(unsigned)nq, 64, 0, s>>>(sorted, perm, N, starts, counts, queries, nq, k, out, tail);
 }
 
 template <int DIM>
 void dispatch_wave(const double* sorted, const uint32_t* perm, int N, const int32_t* starts,
                    const int32_t* counts, const uint32_t* queries, int64_t nq, int k, uint32_t* out,
-                   hipStream_t s) {
-    if (k <= 64) run_wave<DIM, 1>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
-    else if (k <= 128) run_wave<DIM, 2>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
-    else if (k <= 192) run_wave<DIM, 3>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
-    else run_wave<DIM, 4>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
+                   KnnTail tail, hipStream_t s) {
+    if (k <= 64) run_wave<DIM, 1>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
+    else if (k <= 128) run_wave<DIM, 2>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
+    else if (k <= 192) run_wave<DIM, 3>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
+    else run_wave<DIM, 4>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
 }
 
 template <int DIM, int BLOCK>
@@ -77,7 +82,7 @@ void run(const double* sorted, const uint32_t* perm, int N, const int32_t* start
 
 void launch_knn(int dim, const double* sorted, const uint32_t* perm, int64_t n, int N,
                 const int32_t* starts, const int32_t* counts, const uint32_t* queries, int64_t nq,
-                int k, uint32_t* out, hipStream_t s) {
+                int k, uint32_t* out, hipStream_t s, KnnTail tail) {
     (void)n;
     if (nq <= 0) return;
     // Long rows (and short query lists) go to the wave-per-query kernel; the
@@ -87,8 +92,8 @@ void launch_knn(int dim, const double* sorted, const uint32_t* perm, int64_t n, 
         return e ? atoi(e) : 20;
     }();
     if (k >= wave_min_k) {
-        if (dim == 2) dispatch_wave<2>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
-        else dispatch_wave<3>(sorted, perm, N, starts, counts, queries, nq, k, out, s);
+        if (dim == 2) dispatch_wave<2>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
+        else dispatch_wave<3>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
         return;
     }
     // LDS per block = 12 B * k * BLOCK.  gfx950 gives a workgroup up to

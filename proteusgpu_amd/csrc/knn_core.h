@@ -3,6 +3,7 @@
 #pragma once
 
 #include "common.h"
+#include "kernels.h"
 
 namespace pg {
 
@@ -259,18 +260,42 @@ PG_DEV void wave_merge_batch(const uint32_t* __restrict__ perm, double (&kd)[M],
     }
 }
 
+// Rows of a resumed seed start after its saved row length (KnnTail).
+PG_DEV void tail_of(const KnnTail& t, uint32_t q, int k, int& kp, uint32_t& lo) {
+    kp = 0;
+    lo = 0;
+    if (t.resume == nullptr) return;
+    const int32_t r = t.resume[q];
+    if (r > 0 && (r & 0x7fff0000) == t.tag && (r & 0xffff) < k) {
+        kp = r & 0xffff;
+        lo = t.lastnb[q];
+    }
+}
+
 // One query by the 64 lanes of a one-wave block; c_start / c_pfx are the
 // block's LDS tables (64 and 65 entries), pend its candidate buffer.  Grid
-// cells of the query's box are cbase + i + j N + k N^2.
+// cells of the query's box are cbase + i + j N + k N^2.  kp > 0: the first
+// kp neighbours are known, lo_s (sorted id) the last of them; only
+// neighbours kp..k-1 are found (candidates up to lo_s in the order are
+// skipped) and written to o[kp..k).
 template <int DIM, int M>
 PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __restrict__ perm, int N,
                            int64_t cbase, const int32_t* __restrict__ starts,
                            const int32_t* __restrict__ counts, uint32_t q, int k, uint32_t* __restrict__ o,
-                           int32_t* c_start, int32_t* c_pfx, KnnPend* pend) {
+                           int32_t* c_start, int32_t* c_pfx, KnnPend* pend, int kp = 0, uint32_t lo_s = 0) {
     const int l = threadIdx.x;
     double p[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < DIM; d++) p[d] = sorted[(size_t)DIM * q + d];
+    double lo_d = -1.0;  // kp = 0: no lower bound
+    if (kp > 0) {
+        double pc[3] = {0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < DIM; d++) pc[d] = sorted[(size_t)DIM * lo_s + d];
+        lo_d = dist2<DIM>(p, pc);
+        o += kp;
+        k -= kp;
+    }
     double P[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < DIM; d++) P[d] = p[d] * (double)N;
@@ -378,7 +403,7 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
 #pragma unroll
                         for (int d = 0; d < DIM; d++) pc[d] = sorted[(size_t)DIM * cs + d];
                         dd = dist2<DIM>(p, pc);
-                        take = true;
+                        take = kp == 0 || ent_less(perm, lo_d, lo_s, dd, cs);
                     }
                 }
                 if (size == k) {

@@ -405,7 +405,9 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
         if (v >= cbase + NB_CHUNK) {
             cbase = v;
             for (int j = gl; j < NB_CHUNK; j += G) {
-                if (!done && v + j < k) {
+                // a resumed group reads only its row from v0 on: the entries
+                // before are not written by a KNN launch that resumes too
+                if (!done && v + j < k && v + j >= v0) {
                     const uint32_t zz = row[v + j];
                     sm.nbi[j] = zz;
 #pragma unroll
@@ -725,8 +727,8 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
     // (a slot overflow keeps its code: the 128-slot kernel redoes the seed
     // at the same row length and applies this test itself)
     const int loop_status = status;
+    const uint32_t zl = valid ? row[k - 1] : 0u;
     if (valid) {
-        const uint32_t zl = row[k - 1];
         const double dx = pts[(size_t)D * zl] - sx, dy = pts[(size_t)D * zl + 1] - sy;
         const double dz = D == 3 ? pts[(size_t)D * zl + 2] - sz : 0.0;
         if (!(dx * dx + dy * dy + dz * dz > 4 * R2)) {
@@ -762,7 +764,10 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
                 out->nlive = nlive;
             }
         }
-        if (valid && gl == 0) fa.resume[sid] = save ? (shape_tag | k) : 0;
+        if (valid && gl == 0) {
+            fa.resume[sid] = save ? (shape_tag | k) : 0;
+            if (save) fa.lastnb[sid] = zl;
+        }
     }
 
     // ---------------------------------------------------------- extraction
