@@ -71,7 +71,9 @@ PG_DEV void kfeatures_one(const double* __restrict__ sorted, int64_t s, int64_t 
     const int ci = c % N, cj = (c / N) % N, ck = DIM == 3 ? c / (N * N) : 0;
     const double ncell = (double)N * N * (DIM == 3 ? N : 1);
     const double lam = (double)n / ncell;
-    double best[8];
+    double best[8];  // the 8 smallest squared distances, ascending (+inf: none)
+#pragma unroll
+    for (int j = 0; j < 8; j++) best[j] = __builtin_inf();
     int nb = 0;
     double csum[4] = {0, 0, 0, 0};
     int cin[4] = {0, 0, 0, 0};
@@ -105,14 +107,12 @@ PG_DEV void kfeatures_one(const double* __restrict__ sorted, int64_t s, int64_t 
 #pragma unroll
                     for (int d = 0; d < DIM; d++) pq[d] = sorted[(size_t)DIM * q + d];
                     const double d2 = dist2<DIM>(p, pq);
-                    if (nb < 8 || d2 < best[7]) {
-                        int j = nb < 8 ? nb++ : 7;
-                        while (j > 0 && best[j - 1] > d2) {
-                            best[j] = best[j - 1];
-                            j--;
-                        }
-                        best[j] = d2;
-                    }
+                    nb++;
+                    // insertion without branches: new best[j] =
+                    // max(best[j-1], min(best[j], d2)), from the top down
+#pragma unroll
+                    for (int j = 7; j > 0; j--) best[j] = fmax(best[j - 1], fmin(best[j], d2));
+                    best[0] = fmin(best[0], d2);
                 }
             }
     const int cube = DIM == 3 ? 27 : 9;
