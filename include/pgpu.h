@@ -257,8 +257,11 @@ int pg_kmodel_get_params(pg_ctx* ctx, const pg_kmodel* model, float* params);
  * + r m (m clip steps: the certified k, or k when the certificate fails);
  * a rejected seed repeats at 2k (then 4k, ...).  The bucket whose chain has
  * the least expected cost under the predicted distribution wins.
- * r = clip cost per step / KNN cost per row entry (default 2.8, measured on
- * MI355X at 1M 3D seeds). */
+ * r = clip cost per step / KNN cost per row entry.  Restarts that resume
+ * the saved cell waste fewer clip steps than the model charges, so the
+ * default r = 1.0 is set by measurement rather than the ratio itself (1M
+ * uniform 3D seeds on MI355X, learned-k step: r = 2.8 26.1 ms, 1.5 25.4 ms,
+ * 0.8 25.5 ms, 0.3 25.3 ms). */
 int pg_kmodel_set_cost(pg_kmodel* model, double r);
 /* Host features (n x 16) -> bucket index per row (and the 6 logits per row
  * when logits != NULL). */

@@ -172,7 +172,7 @@ struct pg_mesh {
 
 struct pg_kmodel {
     int device = 0;
-    float a0 = 2.8f;  // clip cost per step over KNN cost per row entry
+    float a0 = 1.0f;  // cost model: clip cost per step over KNN cost per row entry
     int64_t step = 0;  // Adam steps taken
     DBuf P, m, v, part, ploss, loss, grad, X, y, cls, logits;
 };
