@@ -990,7 +990,7 @@ void kfeatures(int dim, const double* sorted, int64_t n, int N, const int32_t* c
                 }
             }
     const int cube = dim == 3 ? 27 : 9;
-    for (int m = 0; m < 4; m++) f[m] = cin[m] ? (float)(csum[m] / cin[m] / lam) : 0.0f;
+    for (int m = 0; m < 4; m++) f[m] = cin[m] ? (float)std::log2(1.0 + csum[m] / cin[m] / lam) : 0.0f;
     f[4] = (float)((double)inside / cube);
     const int cidx[3] = {ci, cj, ck};
     for (int d = 0; d < 3; d++) {
@@ -1011,8 +1011,8 @@ void kfeatures(int dim, const double* sorted, int64_t n, int N, const int32_t* c
     double wall = 1e300;
     for (int d = 0; d < dim; d++) wall = std::min(wall, std::min(p[d], 1.0 - p[d]));
     f[12] = (float)(std::min(std::max(wall, 0.0) * N, 3.0) / 3.0);
-    f[13] = nmax >= 0 ? (float)((double)(nmax - nmin) / lam) : 0.0f;
-    f[14] = (float)(std::sqrt(gx * gx + gy * gy + gz * gz) / (cube * lam));
+    f[13] = nmax >= 0 ? (float)std::log2(1.0 + (double)(nmax - nmin) / lam) : 0.0f;
+    f[14] = (float)std::log2(1.0 + std::sqrt(gx * gx + gy * gy + gz * gz) / (cube * lam));
     f[15] = (float)(r8 / r4);
 }
 

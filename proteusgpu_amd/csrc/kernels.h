@@ -113,16 +113,18 @@ constexpr int LK_NFEAT = 16, LK_NCLASS = 6;
 constexpr int LK_NPARAM = 16 * 64 + 64 + 64 * 64 + 64 + 64 * 6 + 6;
 int lk_bucket(int c);  // 16, 24, 32, 48, 64, 96
 // Features of every seed (sorted space) into out[16 * (out_index ? out_index[s] : s)]:
-//   f0..f3  mean count per in-grid cell of the own cell / face / edge /
-//           corner neighbours of the 3^dim cube, over the mean occupancy
+//   f0..f3  log2(1 + x), x = mean count per in-grid cell of the own cell /
+//           face / edge / corner neighbours of the 3^dim cube, over the mean
+//           occupancy (the logarithm keeps clustered inputs, x up to ~1e3,
+//           in the range the bfloat16 network trains on)
 //   f4      in-grid fraction of the cube
 //   f5..f7  |2u - 1|, u = position inside the own cell per axis (0 in 2D z)
 //   f8,f9   distance to the 4th / 8th nearest point of the cube, in cell
 //           widths (2 if absent);  f10 the nearest;  f15 = f9 / f8
 //   f11     log2(1 + cube density / mean density)
 //   f12     min(distance to the unit box faces in cell widths, 3) / 3
-//   f13     (max - min neighbour count) / mean occupancy
-//   f14     |sum of count * offset| / (3^dim * mean occupancy)
+//   f13     log2(1 + (max - min neighbour count) / mean occupancy)
+//   f14     log2(1 + |sum of count * offset| / (3^dim * mean occupancy))
 void launch_kfeatures(int dim, const double* sorted, int64_t n, int N, const int32_t* counts,
                       const int32_t* starts, const uint32_t* out_index, float* out, hipStream_t s);
 // cls[s] = bucket with the least expected cost of its restart chain (a pass

@@ -117,7 +117,7 @@ PG_DEV void kfeatures_one(const double* __restrict__ sorted, int64_t s, int64_t 
             }
     const int cube = DIM == 3 ? 27 : 9;
 #pragma unroll
-    for (int m = 0; m < 4; m++) f[m] = cin[m] ? (float)(csum[m] / cin[m] / lam) : 0.0f;
+    for (int m = 0; m < 4; m++) f[m] = cin[m] ? (float)log2(1.0 + csum[m] / cin[m] / lam) : 0.0f;
     f[4] = (float)((double)inside / cube);
     const int cidx[3] = {ci, cj, ck};
 #pragma unroll
@@ -140,8 +140,8 @@ PG_DEV void kfeatures_one(const double* __restrict__ sorted, int64_t s, int64_t 
 #pragma unroll
     for (int d = 0; d < DIM; d++) wall = fmin(wall, fmin(p[d], 1.0 - p[d]));
     f[12] = (float)(fmin(fmax(wall, 0.0) * N, 3.0) / 3.0);
-    f[13] = nmax >= 0 ? (float)((double)(nmax - nmin) / lam) : 0.0f;
-    f[14] = (float)(sqrt(gx * gx + gy * gy + gz * gz) / (cube * lam));
+    f[13] = nmax >= 0 ? (float)log2(1.0 + (double)(nmax - nmin) / lam) : 0.0f;
+    f[14] = (float)log2(1.0 + sqrt(gx * gx + gy * gy + gz * gz) / (cube * lam));
     f[15] = (float)(r8 / r4);
 }
 

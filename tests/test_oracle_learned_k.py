@@ -127,9 +127,9 @@ def test_features_on_a_lattice_closed_forms(dim):
     assert np.all(F[inner, 4] == 1.0)
     corner = np.all((cells == 0), axis=1)
     np.testing.assert_allclose(F[corner, 4], 2 ** dim / 3 ** dim, rtol=1e-7)
-    # f0 = own cell count / lam, exactly
+    # f0 = log2(1 + own cell count / lam)
     own = cnt[(cells[:, 0] + cells[:, 1] * N + (cells[:, 2] * N * N if dim == 3 else 0))]
-    np.testing.assert_allclose(F[:, 0], (own / lam).astype(np.float32), rtol=1e-7)
+    np.testing.assert_allclose(F[:, 0], np.log2(1.0 + own / lam).astype(np.float32), rtol=1e-6)
     # nearest lattice neighbour is at spacing a for every point
     a = 1.0 / m
     np.testing.assert_allclose(F[:, 10], a * N, rtol=1e-6)
@@ -162,7 +162,7 @@ def test_feature_distances_against_ckdtree(dim, n):
     cid = lambda v: v[0] + v[1] * N + (v[2] * N * N if dim == 3 else 0)
     face_mean = np.mean([cnt[cid(c + o)] for o in offs])
     lam = n / N ** dim
-    assert F[ok][0, 1] == np.float32(face_mean / lam)
+    np.testing.assert_allclose(F[ok][0, 1], np.float32(np.log2(1.0 + face_mean / lam)), rtol=1e-6)
 
 
 def test_labels_are_the_minimal_certified_k():
