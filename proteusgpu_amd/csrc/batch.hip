@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(BLOCK)
 }
 
 template <int DIM, int M>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, (M <= 2 ? 6 : 5))
     k_bknn_wave(const double* __restrict__ sorted, const uint32_t* __restrict__ perm,
                 const int32_t* __restrict__ sbox, const int32_t* __restrict__ N, const int64_t* __restrict__ coff,
                 const int32_t* __restrict__ starts, const int32_t* __restrict__ counts,

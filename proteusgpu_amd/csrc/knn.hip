@@ -32,7 +32,7 @@ __global__ void __launch_bounds__(BLOCK)
 }
 
 template <int DIM, int M>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, (M <= 2 ? 6 : 5))
     k_knn_wave(const double* __restrict__ sorted, const uint32_t* __restrict__ perm, int N,
                const int32_t* __restrict__ starts, const int32_t* __restrict__ counts,
                const uint32_t* __restrict__ queries, int64_t nq, int k, uint32_t* __restrict__ out,
