@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--probe-bytes", type=int, default=1 << 30,
                     help="size of the streaming-copy probe (measured HBM bandwidth)")
     ap.add_argument("--train-steps", type=int, default=400, help="Adam steps of the k classifier")
+    ap.add_argument("--no-boxes", action="store_true", help="skip the 256-box Kelvin-Helmholtz batch timing")
     ap.add_argument("--only", choices=["plain", "learned", "batched"], default=None,
                     help="run one variant only (for profiling): the fixed-k host step, the learned-k host "
                          "step, or the batched fixed- and learned-k steps")
@@ -228,6 +229,25 @@ def main():
     tune_learn, _ = blearn.autotune(reps=1)
     ms_blearn, gpu_blearn = run_batch(ctx, blearn, a.warmup, a.steps)
 
+    # 256 uneven 2D Kelvin-Helmholtz boxes in one captured step (batched
+    # boxes, reported beside the headline; not part of `value`)
+    boxes = None
+    if not a.no_boxes:
+        phase("256 Kelvin-Helmholtz boxes, batched")
+        from proteusgpu_amd.generators import batch_points, box_sizes, kelvin_helmholtz_state
+        bs = box_sizes(256, 1000, 65536, 11)
+        bp = batch_points(bs, 2, 500)
+        br, bv, bE = kelvin_helmholtz_state(bp)
+        kh = api.Batch(ctx, bs, bp, br, bv, bE, boundary="periodic", buff=0.1, k=16, gamma=5.0 / 3.0, cfl=0.3,
+                       flux="physical", move_mesh="regularized")
+        tune_kh, _ = kh.autotune(reps=1)
+        ms_kh, gpu_kh = run_batch(ctx, kh, a.warmup, a.steps)
+        boxes = {"workload": "256 periodic 2D boxes, sizes log-uniform in [1000, 65536], Kelvin-Helmholtz, "
+                             "regularised moving mesh, k = 16", "cells": int(bs.sum()),
+                 "ms_per_step": round(ms_kh, 3), "gpu_ms_per_step": round(gpu_kh, 3),
+                 "cells_per_s": round(float(bs.sum()) / (ms_kh * 1e-3), 1), "tuning": tune_kh}
+        kh.free()
+
     # every run took the same steps from the same state: identical results
     # (the batched runs did warmup + steps; a fresh host-driven run is
     # brought to the same step count)
@@ -297,6 +317,7 @@ def main():
                             "tuning": tune_fix},
         "batched_learned_k": {"ms_per_step": round(ms_blearn, 3), "gpu_ms_per_step": round(gpu_blearn, 3),
                               "tuning": tune_learn},
+        "batched_boxes_kh256": boxes,
         "training": {"ms": round(train_ms, 2), "adam_steps": a.train_steps, "batch": 4096,
                      "final_loss": round(loss, 4)},
         "path_roofline": {"bytes_per_cell": PATH_BYTES_PER_CELL, "achieved_GBs": round(path_gbs, 2),
