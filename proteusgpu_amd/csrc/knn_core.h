@@ -325,7 +325,10 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
     };
 
     for (int ring = 0; ring < N; ring++) {
-        if (npend > 0) {  // the ring test below needs the exact k-th entry
+        // the ring test below (and the cell pruning of the ring) need the
+        // exact k-th entry; before ring 1 the test cannot pass (its bound is
+        // 0), so the few candidates of the own cell wait
+        if (npend > 0 && ring >= 2) {
             flush(npend);
             npend = 0;
         }
