@@ -327,8 +327,9 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
     for (int ring = 0; ring < N; ring++) {
         // the ring test below (and the cell pruning of the ring) need the
         // exact k-th entry; before ring 1 the test cannot pass (its bound is
-        // 0), so the few candidates of the own cell wait
-        if (npend > 0 && ring >= 2) {
+        // 0), and while fewer than k candidates are known there is no k-th
+        // entry: the waiting candidates stay in LDS
+        if (npend > 0 && ring >= 2 && size + npend >= k) {
             flush(npend);
             npend = 0;
         }
