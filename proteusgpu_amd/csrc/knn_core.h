@@ -254,9 +254,11 @@ PG_DEV void wave_merge_batch(const uint32_t* __restrict__ perm, double (&kd)[M],
             ks[jj] = rs;
         }
         wave_merge64(perm, kd[jj], ks[jj], l);
-        wave_merge64(perm, hd, hs, l);
-        dd = hd;
-        cs = hs;
+        if (jj + 1 < M) {  // the upper half carries into the next slot
+            wave_merge64(perm, hd, hs, l);
+            dd = hd;
+            cs = hs;
+        }
     }
 }
 
