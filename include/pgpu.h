@@ -109,6 +109,20 @@ typedef struct {
     int max_t;     /* vertex capacity per cell (Config.sh _MAX_T_, <= 128)       */
     int restart;   /* nonzero: recompute uncertified seeds at larger k           */
     int k_max;     /* largest k a restart may use (default 256)                  */
+    /* Degeneracy fallback (no reference counterpart; the reference leaves
+     * such cells failed, voronoi.cpp:104-110).  0 (default): reference
+     * behaviour.  > 0: when a host-driven build (pg_mesh_compute*, pg_sim_*)
+     * leaves cells with PG_NEEDS_EXACT_PREDICATES or PG_INCONSISTENT_BOUNDARY
+     * (degenerate inputs such as lattices, whose cospherical neighbours give
+     * zero predicates), every generator is displaced by a deterministic
+     * offset of at most perturb x n^(-1/dim) per axis and the mesh is built
+     * again, with ten times the amplitude and a new offset pattern per retry
+     * (three builds at most).  The mesh is then the exact Voronoi mesh of
+     * the displaced generators: its seeds are the displaced points, and the
+     * zero-area faces of the degenerate mesh become faces of area
+     * O(amplitude x spacing).  pg_mesh_info_t.displacement reports the
+     * amplitude used.  Batches (pg_batch_*) do not apply it.               */
+    double perturb;
 } pg_mesh_params;
 
 void pg_mesh_params_default(pg_mesh_params* p);
@@ -124,6 +138,11 @@ typedef struct {
     int64_t row_entries;/* neighbour-row entries computed (sum of rows x k over   *
                          * the first pass and every restart)                     */
     int64_t n_class[6]; /* learned build: seeds predicted per k bucket            */
+    int64_t n_degenerate; /* cells the undisplaced build left with                *
+                           * PG_NEEDS_EXACT_PREDICATES or PG_INCONSISTENT_BOUNDARY *
+                           * (counted when the degeneracy fallback is on)         */
+    double displacement;  /* amplitude of the displacement the mesh was built     *
+                           * with (0: the generators as given)                    */
 } pg_mesh_info_t;
 
 /* voronoi::compute_mesh (voronoi.h:68, voronoi.cpp:17-65) for boundary NONE,

@@ -97,6 +97,39 @@ def kfeatures(sorted_pts, N):
     return out
 
 
+# Degeneracy fallback of a mesh build (no reference counterpart): the
+# generators of retry t are displaced with the SplitMix64 stream seeded
+# DISPLACE_SALT * (t + 1); coordinate d of point i uses counter i * dim + d.
+DISPLACE_SALT = 0xD1B54A32D192ED03
+
+
+def splitmix64(seed, counters):
+    """SplitMix64 output at the given counters (uint64 array): mix(seed + (c + 1) * golden)."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed % 2 ** 64) + (np.asarray(counters, dtype=np.uint64) + np.uint64(1)) * \
+            np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def displace(pts, amp, retry=0, periodic=False):
+    """The fallback's displaced generators: x + amp * (2u - 1) per coordinate,
+    u = top 53 bits / 2^53; wrapped into [0, 1) (periodic) or clamped."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    n, dim = pts.shape
+    z = splitmix64(DISPLACE_SALT * (retry + 1), np.arange(n * dim, dtype=np.uint64))
+    u = (z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    t = 2.0 * u - 1.0
+    x = pts.ravel() + amp * t
+    if periodic:
+        x = np.where(x < 0.0, x + 1.0, x)
+        x = np.where(x >= 1.0, x - 1.0, x)
+    else:
+        x = np.minimum(np.maximum(x, 0.0), np.nextafter(1.0, 0.0))
+    return x.reshape(n, dim)
+
+
 def hll(dim, gamma, UL, UR, n, wall=False):
     """HLL flux (riemann.cpp:7-45) of states (rho, vx, vy, vz, E) across normal n."""
     out = np.zeros(5)

@@ -122,11 +122,12 @@ class KnnProblem:
 
 
 def mesh_params(dim=3, boundary="periodic", buff=0.04, k=32, max_p=64, max_t=96, restart=True,
-                k_max=256) -> MeshParams:
+                k_max=256, perturb=0.0) -> MeshParams:
     p = MeshParams()
     lib().pg_mesh_params_default(C.byref(p))
     p.dim, p.boundary, p.buff, p.k = dim, BOUNDARY[boundary], buff, k
     p.max_p, p.max_t, p.restart, p.k_max = max_p, max_t, int(restart), k_max
+    p.perturb = perturb
     return p
 
 
@@ -199,13 +200,16 @@ class KModel:
 
 class Mesh:
     """voronoi::compute_mesh / compute_periodic_mesh and the VMesh copy-out.
-    kmodel: a trained KModel picks each seed's first k (learned k)."""
+    kmodel: a trained KModel picks each seed's first k (learned k).
+    perturb > 0: degeneracy fallback (builds that leave cells with
+    degenerate predicates are repeated on deterministically displaced
+    generators; `displacement` reports the amplitude used)."""
 
     def __init__(self, ctx: Context, pts: np.ndarray, boundary="periodic", buff=0.04, k=32, max_p=64,
-                 max_t=96, restart=True, k_max=256, kmodel: KModel | None = None):
+                 max_t=96, restart=True, k_max=256, kmodel: KModel | None = None, perturb=0.0):
         pts = np.ascontiguousarray(pts, dtype=np.float64)
         self.ctx, self.dim, self.k = ctx, pts.shape[1], k
-        prm = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max)
+        prm = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max, perturb)
         h = C.c_void_p()
         km = kmodel.h if kmodel is not None else None
         check(lib().pg_mesh_compute_k(ctx.h, C.byref(prm), km, _ptr(pts), pts.shape[0], C.byref(h)))
@@ -221,6 +225,7 @@ class Mesh:
             info.n_failed, info.n_restarted, info.N_grid, info.k_used)
         self.row_entries = info.row_entries
         self.n_class = np.array(info.n_class[:], dtype=np.int64)
+        self.n_degenerate, self.displacement = info.n_degenerate, info.displacement
 
     def features(self):
         """Learned-k features of every seed, original order (n_seeds, 16)."""
@@ -325,12 +330,12 @@ class Simulation:
 
     def __init__(self, ctx: Context, pts, rho, vel, E, boundary="reflective", buff=0.04, k=32,
                  gamma=5.0 / 3.0, cfl=0.4, flux="physical", move_mesh=True, max_p=64, max_t=96,
-                 restart=True, k_max=256):
+                 restart=True, k_max=256, perturb=0.0):
         from ._native import SimParams
         pts = np.ascontiguousarray(pts, dtype=np.float64)
         self.ctx, self.n, self.dim = ctx, pts.shape[0], pts.shape[1]
         p = SimParams()
-        p.mesh = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max)
+        p.mesh = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max, perturb)
         p.gamma, p.cfl, p.flux_mode, p.move_mesh = gamma, cfl, FLUX[flux], move_mode(move_mesh)
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, vel, E)]
         h = C.c_void_p()

@@ -81,6 +81,8 @@ int ngrid(int dim, int64_t n) { return pg::ngrid_of(dim, n); }
 
 const int KSCHED[] = {16, 24, 32, 48, 64, 96, 128, 192, 256};
 constexpr int MAXP = 64;
+// offset pattern of the degeneracy fallback's t-th retry: salt (t + 1) x this
+constexpr uint64_t DISPLACE_SALT = 0xD1B54A32D192ED03ull;
 
 }  // namespace
 
@@ -135,7 +137,7 @@ struct pg_mesh {
     pg_mesh_params prm{};
     int64_t n_in = 0, n_seeds = 0, n_hydro = 0;
     Grid g;
-    DBuf in_pts, gcnt, goff, gsrc, gtot;
+    DBuf in_pts, disp_pts, gcnt, goff, gsrc, gtot;
     DBuf knn, rows, list, list2, list3, flags, offs, cnt1;
     DBuf feat, kcls, klist, kcnt;  // learned k: features, bucket per seed, seeds by bucket, bucket sizes
     // faces (padded, face-slot major) and cells, sorted order
@@ -282,8 +284,10 @@ struct Marks {
 };
 enum { MK_START, MK_GHOSTS, MK_GRID, MK_PREDICT, MK_KNN, MK_CLIP, MK_RESTART, MK_FINAL, MK_HYDRO, MK_END, MK_N };
 
-void mesh_build(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool on_device = false,
-                Marks* mk = nullptr, const pg_kmodel* km = nullptr) {
+// One build of the mesh of pts (device memory when on_device).  count:
+// also learn VMesh.num_faces (builds whose mesh is finalized learn it there).
+void mesh_build_once(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool on_device, bool count, Marks* mk,
+                     const pg_kmodel* km) {
     hipStream_t s = c->stream;
     const pg_mesh_params& p = m->prm;
     const int dim = p.dim;
@@ -421,7 +425,40 @@ void mesh_build(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool on_dev
     m->info.N_grid = m->g.N;
     m->info.k_used = k_used;
     m->info.n_failed = collect_list(m, nullptr, nseeds, -1, m->list.get<uint32_t>(ns), s);
-    if (!on_device) count_faces(m, s);
+    if (count) count_faces(m, s);
+}
+
+// Cells of the last build whose predicates were degenerate.
+int64_t degenerate_cells(pg_mesh* m, hipStream_t s) {
+    uint32_t* tmp = m->list.get<uint32_t>((size_t)m->n_seeds);
+    return collect_list(m, nullptr, m->n_seeds, PG_NEEDS_EXACT_PREDICATES, tmp, s) +
+           collect_list(m, nullptr, m->n_seeds, PG_INCONSISTENT_BOUNDARY, tmp, s);
+}
+
+// A mesh build with the degeneracy fallback of pg_mesh_params.perturb:
+// builds that leave degenerate cells are repeated on displaced generators
+// (amplitude perturb x n^(-1/dim), times ten per retry, three builds at most).
+void mesh_build(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool on_device = false,
+                Marks* mk = nullptr, const pg_kmodel* km = nullptr) {
+    mesh_build_once(c, m, pts, n, on_device, !on_device, mk, km);
+    if (m->prm.perturb <= 0 || m->info.n_failed == 0) return;
+    hipStream_t s = c->stream;
+    const int64_t ndeg = degenerate_cells(m, s);
+    if (ndeg == 0) return;
+    const int dim = m->prm.dim;
+    // the generators as given, on the device (a host input was copied there)
+    const double* src = on_device ? pts : m->in_pts.as<double>();
+    double* dsp = m->disp_pts.get<double>((size_t)n * dim);
+    double amp = m->prm.perturb * std::pow((double)n, -1.0 / dim);
+    for (int t = 0; t < 2; t++, amp *= 10.0) {
+        pg::launch_displace(dim, src, n, amp, DISPLACE_SALT * (uint64_t)(t + 1),
+                            m->prm.boundary == PG_BOUNDARY_PERIODIC ? 1 : 0, dsp, s);
+        HIPCHK(hipGetLastError());
+        mesh_build_once(c, m, dsp, n, true, !on_device, mk, km);
+        m->info.displacement = amp;
+        if (m->info.n_failed == 0 || degenerate_cells(m, s) == 0) break;
+    }
+    m->info.n_degenerate = ndeg;
 }
 
 // Total faces over all cells (VMesh.num_faces).
@@ -682,6 +719,7 @@ void pg_mesh_params_default(pg_mesh_params* p) {
     p->max_t = 96;
     p->restart = 1;
     p->k_max = 256;
+    p->perturb = 0.0;
 }
 
 int pg_mesh_compute_k(pg_ctx* c, const pg_mesh_params* prm, const pg_kmodel* km, const double* pts, int64_t n,
@@ -697,6 +735,7 @@ int pg_mesh_compute_k(pg_ctx* c, const pg_mesh_params* prm, const pg_kmodel* km,
         argchk(prm->max_t >= 8 && prm->max_t <= 128, "max_t must be in [8, 128]");
         argchk(prm->boundary != PG_BOUNDARY_PERIODIC || (prm->buff > 0 && prm->buff < 0.5),
                "buff must be in (0, 0.5)");
+        argchk(prm->perturb >= 0.0 && prm->perturb <= 1e-3, "perturb must be in [0, 1e-3]");
         argchk(n >= 2 && n < (int64_t)1 << 29, "n out of range");
         pg_mesh* m = new pg_mesh();
         try {
@@ -996,6 +1035,7 @@ int pg_sim_create(pg_ctx* c, const pg_sim_params* prm, const double* pts, const 
         argchk(prm->gamma > 1.0 && prm->cfl > 0.0, "gamma > 1 and cfl > 0 required");
         argchk(prm->flux_mode == PG_FLUX_REFERENCE || prm->flux_mode == PG_FLUX_PHYSICAL, "unknown flux mode");
         argchk(prm->move_mesh >= PG_MOVE_STATIC && prm->move_mesh <= PG_MOVE_REGULARIZED, "unknown move_mesh mode");
+        argchk(mp.perturb >= 0.0 && mp.perturb <= 1e-3, "perturb must be in [0, 1e-3]");
         argchk(n >= 2 && n < (int64_t)1 << 29, "n out of range");
         pg_sim* S = new pg_sim();
         try {
@@ -1624,6 +1664,7 @@ int pg_batch_create(pg_ctx* c, const pg_sim_params* prm, int nbox, const int64_t
         argchk(mp.dim == 2 || mp.dim == 3, "dim must be 2 or 3");
         argchk(mp.boundary == PG_BOUNDARY_PERIODIC || mp.boundary == PG_BOUNDARY_REFLECTIVE,
                "a batch needs periodic or reflective boxes");
+        argchk(mp.perturb == 0.0, "batches do not apply the degeneracy fallback (perturb must be 0)");
         argchk(mp.k >= 1 && mp.k <= 256 && mp.k_max <= 256, "k and k_max must be in [1, 256]");
         argchk(mp.max_p >= 2 * mp.dim + 1 && mp.max_p <= MAXP, "max_p must be in [2*dim+1, 64]");
         argchk(mp.max_t >= 8 && mp.max_t <= 128, "max_t must be in [8, 128]");

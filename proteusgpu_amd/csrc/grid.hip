@@ -163,6 +163,34 @@ __global__ void __launch_bounds__(TB) k_gather_points(int64_t n, const uint32_t*
     for (int d = 0; d < DIM; d++) sorted[DIM * s + d] = pts[DIM * (int64_t)o + d];
 }
 
+// Deterministic displacement (the degeneracy fallback of a mesh build):
+// coordinate d of point i moves by amp * (2u - 1), u = the top 53 bits of
+// SplitMix64 at counter salt + (i * DIM + d + 1) * 0x9E3779B97F4A7C15, over
+// 2^53.  A coordinate that leaves [0, 1) wraps (periodic) or is clamped.
+template <int DIM>
+__global__ void __launch_bounds__(TB) k_displace(const double* in, int64_t n, double amp, uint64_t salt,
+                                                 int periodic, double* out) {
+    int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        uint64_t z = salt + (uint64_t)(i * DIM + d + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const double u = (double)(z >> 11) * 0x1p-53;
+        const double t = 2.0 * u - 1.0;
+        double x = in[DIM * i + d] + amp * t;
+        if (periodic) {
+            if (x < 0.0) x += 1.0;
+            if (x >= 1.0) x -= 1.0;
+        } else {
+            x = fmin(fmax(x, 0.0), 0x1.fffffffffffffp-1);
+        }
+        out[DIM * i + d] = x;
+    }
+}
+
 inline unsigned nblk(int64_t n) { return (unsigned)((n + TB - 1) / TB); }
 
 __global__ void k_fill_one(int32_t* p) {
@@ -210,6 +238,15 @@ void launch_cell_sort(int64_t ncell, const int32_t* starts, const int32_t* count
     k_fill_one<<<1, 64, 0, s>>>(nbig);
     k_cell_sort<<<nblk(ncell), TB, 0, s>>>(ncell, starts, counts, idx, big, nbig);
     k_cell_sort_big<<<(unsigned)std::min<int64_t>(ncell, 1024), TB, 0, s>>>(big, nbig, starts, counts, idx);
+}
+
+void launch_displace(int dim, const double* in, int64_t n, double amp, uint64_t salt, int periodic, double* out,
+                     hipStream_t s) {
+    if (n <= 0) return;
+    if (dim == 2)
+        k_displace<2><<<nblk(n), TB, 0, s>>>(in, n, amp, salt, periodic, out);
+    else
+        k_displace<3><<<nblk(n), TB, 0, s>>>(in, n, amp, salt, periodic, out);
 }
 
 void launch_gather_points(int dim, int64_t n, const uint32_t* perm, const double* pts,

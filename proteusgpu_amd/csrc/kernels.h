@@ -32,6 +32,11 @@ void launch_cell_scatter(int64_t n, const int32_t* cell, const int32_t* starts, 
 // int32) and nbig (one int32) are scratch for ranges longer than 32.
 void launch_cell_sort(int64_t ncell, const int32_t* starts, const int32_t* counts, uint32_t* idx,
                       int32_t* big, int32_t* nbig, hipStream_t s);
+// out = in displaced by at most amp per axis, deterministically from each
+// point's index and salt (the degeneracy fallback of a mesh build); periodic:
+// coordinates wrap into [0, 1), otherwise they are clamped into it.
+void launch_displace(int dim, const double* in, int64_t n, double amp, uint64_t salt, int periodic, double* out,
+                     hipStream_t s);
 // sorted[s] = pts[perm[s]]
 void launch_gather_points(int dim, int64_t n, const uint32_t* perm, const double* pts,
                           double* sorted, hipStream_t s);

@@ -4,6 +4,7 @@
 
 double buff = (1. / 100.) * 4;  // allvars.cpp:13
 double _gamma_ = 5. / 3.;       // allvars.cpp:14
+double perturb = 0.0;           // param.txt perturb (degenerate inputs such as lattices)
 
 namespace pgpu_host {
 
@@ -26,6 +27,7 @@ pg_mesh_params mesh_params(int boundary, int k) {
     p.dim = DIMENSION;
     p.boundary = boundary;
     p.buff = buff;
+    p.perturb = perturb;
     p.k = k;
     p.max_p = _MAX_P_;
     p.max_t = _MAX_T_;

@@ -68,6 +68,7 @@ struct prim {
 
 extern double buff;     // ghost layer width (allvars.cpp:13)
 extern double _gamma_;  // adiabatic index (allvars.cpp:14)
+extern double perturb;  // degeneracy fallback amplitude (pg_mesh_params.perturb; 0 = off)
 
 namespace pgpu_host {
 // Process-wide context on GPU 0, created on first use.  Fatal errors print

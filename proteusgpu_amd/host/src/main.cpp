@@ -12,7 +12,9 @@
 // cells round on long runs).  Learned k (resident runs): learned_k
 // (false), mlp_weights (float32 parameter file to load; when it does not
 // exist the classifier is trained on the GPU on the first mesh and, with
-// mlp_weights_out, saved), mlp_train_steps (400).
+// mlp_weights_out, saved), mlp_train_steps (400).  perturb (0): amplitude
+// of the degeneracy fallback, in units of the mean spacing (1e-6 suits
+// lattice ICs, whose cells the reference leaves failed).
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -130,6 +132,7 @@ int main(int argc, char* argv[]) {
         const double cfl = input.getDouble("cfl", 0.4);
         _gamma_ = input.getDouble("gamma", 5. / 3.);
         buff = input.getDouble("buff", buff);
+        perturb = input.getDouble("perturb", 0.0);
         const std::string bnd = input.getString("boundary", "periodic");
         const int boundary = bnd == "reflective" ? PG_BOUNDARY_REFLECTIVE : PG_BOUNDARY_PERIODIC;
         const bool move = input.getBool("move_mesh", false);

@@ -237,3 +237,58 @@ def test_two_cell_known_answer():
     the central flux and pushes it the other way."""
     assert two_cell_momentum(1) > 0
     assert two_cell_momentum(0) < 0
+
+
+# ------------------------------------------------------ degeneracy fallback
+def test_displacement_known_answers_and_bounds():
+    # SplitMix64 seeded with 0: its published first three outputs
+    assert [int(v) for v in O.splitmix64(0, [0, 1, 2])] == \
+        [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    pts = uniform_points(2000, 3, 9)
+    amp = 1e-3
+    for periodic in (False, True):
+        d = O.displace(pts, amp, 0, periodic)
+        assert np.all((d >= 0) & (d < 1))
+        sig = d - pts
+        sig -= np.round(sig)  # a wrapped coordinate moved by about one box length
+        off = np.abs(sig)
+        assert off.max() <= amp * (1 + 1e-12) and off.max() > 0.99 * amp
+        # offsets are uniform on [-amp, amp]: mean ~ 0, mean |.| ~ amp / 2
+        assert abs(np.mean(sig)) < 0.05 * amp
+        assert abs(np.mean(off) - amp / 2) < 0.05 * amp
+    assert np.array_equal(O.displace(pts, amp, 1), O.displace(pts, amp, 1))
+    assert np.mean(O.displace(pts, amp, 0) == O.displace(pts, amp, 1)) < 1e-3
+    # coordinates pushed out of the box wrap (periodic) or are clamped
+    edge = np.array([[0.0, 0.5, np.nextafter(1.0, 0.0)]] * 64)
+    w, c = O.displace(edge, 0.1, 0, True), O.displace(edge, 0.1, 0, False)
+    assert np.all((w >= 0) & (w < 1)) and np.any(w[:, 0] > 0.5) and np.any(w[:, 2] < 0.5)
+    assert np.any(c[:, 0] == 0.0) and np.any(c[:, 2] == np.nextafter(1.0, 0.0))
+
+
+def square_lattice(dim, m):
+    g = (np.arange(m) + 0.5) / m
+    return np.stack([a.ravel() for a in np.meshgrid(*([g] * dim), indexing="ij")], 1)
+
+
+@pytest.mark.parametrize("dim,m,k", [(2, 40, 16), (3, 22, 32)])
+def test_displaced_lattice_is_the_cubic_lattice(dim, m, k):
+    """An exact cubic lattice fails (cospherical neighbours: zero predicates,
+    cell.cpp:187-217); displaced by 1e-6 of the spacing it certifies, and its
+    cells are the lattice's cubes to O(1e-6): volume h^dim, 2 dim faces of
+    h^(dim-1), the other faces of area O(1e-6 h^(dim-1))."""
+    pts = square_lattice(dim, m)
+    h = 1.0 / m
+    exact = O.Mesh(pts, "reflective", k=k, restart=True)
+    assert np.sum(exact.get("status") == 5) > 0.9 * len(pts)
+    d = O.displace(pts, 1e-6 * h, 0, False)
+    mesh = O.Mesh(d, "reflective", k=k, restart=True)
+    assert np.all(mesh.get("status") == 4)
+    vol, fc, fp, area = mesh.get("volumes"), mesh.get("face_counts"), mesh.get("face_ptr"), mesh.get("face_area")
+    np.testing.assert_allclose(vol, h ** dim, rtol=1e-5)
+    assert abs(vol.sum() - 1.0) < 1e-12
+    for i in range(0, len(pts), 7):
+        a = np.sort(area[fp[i]:fp[i] + fc[i]])[::-1]
+        big = a[a > 0.5 * h ** (dim - 1)]
+        assert len(big) == 2 * dim
+        np.testing.assert_allclose(big, h ** (dim - 1), rtol=1e-5)
+        assert np.all(a[2 * dim:] < 1e-4 * h ** (dim - 1))
