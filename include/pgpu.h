@@ -152,6 +152,20 @@ int pg_mesh_compute(pg_ctx* ctx, const pg_mesh_params* params, const double* pts
                     pg_mesh** out);
 int pg_mesh_info(const pg_mesh* mesh, pg_mesh_info_t* info);
 
+/* voronoi::compute_cells (voronoi.h:69, voronoi.cpp:67-111): the cells of a
+ * solved KNN problem (pg_knn_solve), clipped with its rows of length k and
+ * no certificate restarts (boundary NONE, the box [0,1]^dim).  The mesh is
+ * indexed by the problem's sorted ids, as compute_cells leaves it before
+ * unpermute_vmesh: pg_mesh_copy_out returns cells in sorted order with
+ * neighbour ids in sorted-index space.  status (may be NULL): the per-seed
+ * status (PG_* codes, n entries, sorted order), as compute_cells' stat. */
+int pg_voronoi_compute(pg_ctx* ctx, const pg_knn* knn, int max_p, int max_t, pg_mesh** out, int32_t* status);
+/* voronoi::unpermute_vmesh (voronoi.h:77, voronoi.cpp:212-333) for a mesh of
+ * pg_voronoi_compute: later copy-outs return cells in input order with
+ * original neighbour ids (the reorder runs on the device at copy-out).
+ * PG_ERR_ARG for a mesh already in input order. */
+int pg_mesh_unpermute(pg_ctx* ctx, pg_mesh* mesh);
+
 /* The reference VMesh (voronoi.h:22-41) after unpermute_vmesh
  * (voronoi.cpp:212-333) and, for periodic meshes, the unscale of
  * periodic_mesh.cpp:170-190.  Caller allocates every array (sizes from
@@ -198,6 +212,16 @@ int pg_dt_cfl(pg_ctx* ctx, const pg_mesh* mesh, const pg_state* st, double gamma
 int pg_hydro_step(pg_ctx* ctx, const pg_mesh* mesh, pg_state* st, double gamma, double dt,
                   int flux_mode);
 int pg_state_copy_out(pg_ctx* ctx, const pg_state* st, double* rho, double* vel, double* E);
+/* hydro::riemann_hll (riemann.h:24, riemann.cpp:7-43) for n independent
+ * faces: left and right states (rho, v[dim], E; E total energy per volume)
+ * and the stored face normal (3 per face, unit (s_i - s_j) as VMesh keeps
+ * it) in host arrays; wall (may be NULL): nonzero marks a reflective wall,
+ * whose right state is the mirrored left one.  flux (n*(dim+2)): mass,
+ * momentum[dim] and energy flux per unit area in the lab frame.
+ * PG_FLUX_REFERENCE takes the stored normal as the face's +x axis, as the
+ * reference does; PG_FLUX_PHYSICAL its negation (the outward normal). */
+int pg_hll_flux(pg_ctx* ctx, int dim, double gamma, int64_t n, const double* left, const double* right,
+                const double* normal, const uint8_t* wall, int flux_mode, double* flux);
 /* hydro::free_prim (finite_volume_solver.h:21). */
 int pg_state_free(pg_state** st);
 

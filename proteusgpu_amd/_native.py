@@ -109,6 +109,9 @@ def lib():
         "pg_dt_cfl": ([vp, vp, vp, dbl, dbl, C.POINTER(C.c_double)], i32),
         "pg_hydro_step": ([vp, vp, vp, dbl, dbl, i32], i32),
         "pg_state_copy_out": ([vp, vp, vp, vp, vp], i32), "pg_state_free": ([pp], i32),
+        "pg_hll_flux": ([vp, i32, dbl, i64, vp, vp, vp, vp, i32, vp], i32),
+        "pg_voronoi_compute": ([vp, vp, i32, i32, pp, vp], i32),
+        "pg_mesh_unpermute": ([vp, vp], i32),
         "pg_sim_create": ([vp, C.POINTER(SimParams), vp, vp, vp, vp, i64, pp], i32),
         "pg_sim_step": ([vp, vp, dbl, C.POINTER(C.c_double)], i32),
         "pg_sim_hydro_steps": ([vp, vp, i32, dbl, C.POINTER(C.c_double)], i32),
@@ -156,6 +159,7 @@ def check(rc):
 def exported_symbols():
     """Names of the C ABI entry points this binding knows (for the export test)."""
     return ["pg_version", "pg_last_error", "pg_device_count", "pg_set_clip_config", "pg_get_clip_config", "pg_set_clip_resume", "pg_get_clip_resume", "pg_mesh_get_knn", "pg_ctx_create", "pg_ctx_destroy",
+            "pg_hll_flux", "pg_voronoi_compute", "pg_mesh_unpermute",
             "pg_ctx_sync", "pg_stream_copy_probe", "pg_knn_init", "pg_knn_solve", "pg_knn_info", "pg_knn_get_points",
             "pg_knn_get_knearest", "pg_knn_get_permutation", "pg_knn_free", "pg_mesh_params_default",
             "pg_mesh_compute", "pg_mesh_info", "pg_mesh_copy_out", "pg_mesh_free", "pg_state_create",

@@ -216,6 +216,25 @@ class Mesh:
         self.h = h
         self._load_info()
 
+    @classmethod
+    def from_knn(cls, ctx: Context, knn: "KnnProblem", max_p=64, max_t=96):
+        """voronoi::compute_cells: the cells of a solved KNN problem (no
+        restarts, box [0,1]^dim), in the problem's sorted order.  Returns
+        (mesh, status)."""
+        m = cls.__new__(cls)
+        m.ctx, m.dim, m.k = ctx, knn.dim, knn.k
+        status = np.empty(knn.n, np.int32)
+        h = C.c_void_p()
+        check(lib().pg_voronoi_compute(ctx.h, knn.h, max_p, max_t, C.byref(h), _ptr(status)))
+        m.h = h
+        m._load_info()
+        return m, status
+
+    def unpermute(self):
+        """voronoi::unpermute_vmesh for a mesh of from_knn: later copy-outs
+        are in input order."""
+        check(lib().pg_mesh_unpermute(self.ctx.h, self.h))
+
     def _load_info(self):
         info = MeshInfo()
         check(lib().pg_mesh_info(self.h, C.byref(info)))
@@ -270,6 +289,21 @@ class Mesh:
 
     def __del__(self):
         self.free()
+
+
+def hll_flux(ctx: Context, gamma, left, right, normal, wall=None, flux="reference"):
+    """hydro::riemann_hll for many faces: states (n, dim+2) as (rho, v, E),
+    stored normals (n, 3), optional wall flags; fluxes (n, dim+2)."""
+    left = np.ascontiguousarray(left, dtype=np.float64)
+    right = np.ascontiguousarray(right, dtype=np.float64)
+    normal = np.ascontiguousarray(normal, dtype=np.float64)
+    n, w = left.shape
+    assert right.shape == (n, w) and normal.shape == (n, 3)
+    wl = None if wall is None else np.ascontiguousarray(wall, dtype=np.uint8)
+    out = np.empty((n, w))
+    check(lib().pg_hll_flux(ctx.h, w - 2, gamma, n, _ptr(left), _ptr(right), _ptr(normal), _ptr(wl),
+                            FLUX[flux], _ptr(out)))
+    return out
 
 
 class HydroState:

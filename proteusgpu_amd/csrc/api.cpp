@@ -145,6 +145,12 @@ struct pg_mesh {
     DBuf cstate, resume, lastnb;  // saved cells of uncertified seeds (restarts resume them)
     bool use_resume = false;
     bool learned_rows = false;  // first pass with per-seed row lengths (knn rows not kept)
+    // cells of a KNN problem (pg_voronoi_compute): outputs stay in the
+    // problem's sorted order (iperm = identity), while g.perm keeps the
+    // problem's permutation for the KNN tie order
+    bool sorted_space = false;
+    DBuf iperm;
+    uint32_t* out_order() { return sorted_space ? iperm.as<uint32_t>() : g.perm.as<uint32_t>(); }
     // hydro tables
     DBuf hid, area_u, vol_u, o2s, bad;
     bool finalized = false;
@@ -284,6 +290,19 @@ struct Marks {
 };
 enum { MK_START, MK_GHOSTS, MK_GRID, MK_PREDICT, MK_KNN, MK_CLIP, MK_RESTART, MK_FINAL, MK_HYDRO, MK_END, MK_N };
 
+// Face and cell outputs of ns seeds.
+void alloc_cells(pg_mesh* m, size_t ns) {
+    m->nbr.get<int32_t>(ns * MAXP);
+    m->area.get<double>(ns * MAXP);
+    m->nrm.get<double>(3 * ns * MAXP);
+    m->cen.get<double>(3 * ns * MAXP);
+    m->count.get<uint8_t>(ns);
+    m->vol.get<double>(ns);
+    m->ccen.get<double>(3 * ns);
+    m->status.get<int32_t>(ns);
+    m->kcert.get<int32_t>(ns);
+}
+
 // One build of the mesh of pts (device memory when on_device).  count:
 // also learn VMesh.num_faces (builds whose mesh is finalized learn it there).
 void mesh_build_once(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool on_device, bool count, Marks* mk,
@@ -329,17 +348,8 @@ void mesh_build_once(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool o
     m->g.build(s);
     if (mk) mk->mark(MK_GRID, s);
 
-    // face and cell outputs
     const size_t ns = (size_t)nseeds;
-    m->nbr.get<int32_t>(ns * MAXP);
-    m->area.get<double>(ns * MAXP);
-    m->nrm.get<double>(3 * ns * MAXP);
-    m->cen.get<double>(3 * ns * MAXP);
-    m->count.get<uint8_t>(ns);
-    m->vol.get<double>(ns);
-    m->ccen.get<double>(3 * ns);
-    m->status.get<int32_t>(ns);
-    m->kcert.get<int32_t>(ns);
+    alloc_cells(m, ns);
     m->use_resume = p.restart && pg::clip_resume();
     if (m->use_resume) {
         m->cstate.get<uint8_t>(ns * pg::CLIP_STATE_BYTES);
@@ -468,7 +478,7 @@ void count_faces(pg_mesh* m, hipStream_t s) {
     int32_t* co = m->flags.get<int32_t>(ns);
     int32_t* po = m->offs.get<int32_t>(ns);
     int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(std::max<int64_t>(nseeds, m->g.ncell)));
-    pg::launch_counts_orig(nseeds, m->g.perm.as<uint32_t>(), m->count.as<uint8_t>(), co, s);
+    pg::launch_counts_orig(nseeds, m->out_order(), m->count.as<uint8_t>(), co, s);
     pg::scan_exclusive_i32(co, po, nseeds, tp, m->cnt1.get<int32_t>(1), s);
     m->info.num_faces = read_i32(m->cnt1.as<int32_t>(), s);
 }
@@ -483,7 +493,7 @@ void mesh_finalize(pg_ctx* c, pg_mesh* m) {
     int32_t* bad = m->bad.get<int32_t>(2);
     HIPCHK(hipMemsetAsync(bad, 0, 2 * sizeof(int32_t), s));
     const int64_t* gsrc = m->prm.boundary == PG_BOUNDARY_PERIODIC ? m->gsrc.as<int64_t>() : nullptr;
-    pg::launch_finalize(m->n_seeds, m->n_hydro, m->n_seeds, m->g.perm.as<uint32_t>(), gsrc,
+    pg::launch_finalize(m->n_seeds, m->n_hydro, m->n_seeds, m->out_order(), gsrc,
                         m->count.as<uint8_t>(), m->nbr.as<int32_t>(), m->area.as<double>(),
                         m->vol.as<double>(), as, vs, m->hid.get<int32_t>(ns * MAXP),
                         m->area_u.get<double>(ns * MAXP), m->vol_u.get<double>(ns),
@@ -494,7 +504,7 @@ void mesh_finalize(pg_ctx* c, pg_mesh* m) {
     int32_t* co = m->flags.get<int32_t>(ns);
     int32_t* po = m->offs.get<int32_t>(ns);
     int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(std::max<int64_t>(m->n_seeds, m->g.ncell)));
-    pg::launch_counts_orig(m->n_seeds, m->g.perm.as<uint32_t>(), m->count.as<uint8_t>(), co, s);
+    pg::launch_counts_orig(m->n_seeds, m->out_order(), m->count.as<uint8_t>(), co, s);
     pg::scan_exclusive_i32(co, po, m->n_seeds, tp, m->cnt1.get<int32_t>(1), s);
     HIPCHK(hipGetLastError());
     int32_t hb[2] = {0, 0}, nf = 0;
@@ -754,6 +764,89 @@ int pg_mesh_compute(pg_ctx* c, const pg_mesh_params* prm, const double* pts, int
     return pg_mesh_compute_k(c, prm, nullptr, pts, n, out);
 }
 
+int pg_voronoi_compute(pg_ctx* c, const pg_knn* kc, int max_p, int max_t, pg_mesh** out, int32_t* status) {
+    return guarded([&] {
+        use(c);
+        argchk(kc && out, "null argument");
+        pg_knn* kp = const_cast<pg_knn*>(kc);
+        argchk(kp->k >= 1, "the KNN problem is not solved (pg_knn_solve)");
+        const int dim = kp->g.dim;
+        argchk(max_p >= 2 * dim + 1 && max_p <= MAXP, "max_p must be in [2*dim+1, 64]");
+        argchk(max_t >= 8 && max_t <= 128, "max_t must be in [8, 128]");
+        hipStream_t s = c->stream;
+        pg_mesh* m = new pg_mesh();
+        try {
+            pg_mesh_params_default(&m->prm);
+            m->prm.dim = dim;
+            m->prm.boundary = PG_BOUNDARY_NONE;
+            m->prm.k = m->prm.k_max = kp->k;
+            m->prm.max_p = max_p;
+            m->prm.max_t = max_t;
+            m->prm.restart = 0;  // compute_cells has no restarts (voronoi.cpp:104-110)
+            const int64_t n = kp->g.n;
+            const size_t nn = (size_t)n;
+            m->n_in = m->n_seeds = n;
+            m->n_hydro = 0;
+            // the KNN problem's grid and rows; the mesh's outputs are indexed
+            // by the problem's sorted ids, as compute_cells leaves them
+            // before unpermute_vmesh
+            Grid& g = m->g;
+            g.dim = dim;
+            g.n = n;
+            g.N = kp->g.N;
+            g.ncell = kp->g.ncell;
+            HIPCHK(hipMemcpyAsync(g.sorted.get<double>(nn * dim), kp->g.sorted.p, sizeof(double) * nn * dim,
+                                  hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(g.counts.get<int32_t>(g.ncell), kp->g.counts.p, sizeof(int32_t) * g.ncell,
+                                  hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(g.starts.get<int32_t>(g.ncell), kp->g.starts.p, sizeof(int32_t) * g.ncell,
+                                  hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(g.perm.get<uint32_t>(nn), kp->g.perm.p, sizeof(uint32_t) * nn,
+                                  hipMemcpyDeviceToDevice, s));
+            std::vector<uint32_t> iota(nn);
+            for (size_t i = 0; i < nn; i++) iota[i] = (uint32_t)i;
+            HIPCHK(hipMemcpyAsync(m->iperm.get<uint32_t>(nn), iota.data(), sizeof(uint32_t) * nn,
+                                  hipMemcpyHostToDevice, s));
+            m->sorted_space = true;
+            const int k = kp->k;
+            uint32_t* rows = m->knn.get<uint32_t>(nn * k);
+            HIPCHK(hipMemcpyAsync(rows, kp->knn.p, sizeof(uint32_t) * nn * k, hipMemcpyDeviceToDevice, s));
+            alloc_cells(m, nn);
+            m->use_resume = false;
+            m->finalized = false;
+            m->info = pg_mesh_info_t{};
+            pg::launch_voronoi(dim, g.sorted.as<double>(), n, nullptr, n, rows, k, max_p, max_t, 0, m->faces(), s);
+            HIPCHK(hipGetLastError());
+            m->info.row_entries = n * k;
+            // seeds that outgrew the fast kernel's slots: the 128-slot kernel
+            m->info.k_used = restart_chain(m, nullptr, n, k, s);
+            m->info.n_seeds = n;
+            m->info.N_grid = g.N;
+            m->info.n_failed = collect_list(m, nullptr, n, -1, m->list.get<uint32_t>(nn), s);
+            count_faces(m, s);
+            if (status) {
+                HIPCHK(hipMemcpyAsync(status, m->status.p, sizeof(int32_t) * nn, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+            }
+            // iota must outlive the copy above
+            HIPCHK(hipStreamSynchronize(s));
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int pg_mesh_unpermute(pg_ctx* c, pg_mesh* m) {
+    return guarded([&] {
+        use(c);
+        argchk(m != nullptr, "null mesh");
+        argchk(m->sorted_space, "the mesh is already in input order");
+        m->sorted_space = false;
+    });
+}
+
 int pg_mesh_info(const pg_mesh* m, pg_mesh_info_t* info) {
     return guarded([&] {
         argchk(m && info, "null argument");
@@ -792,7 +885,7 @@ int pg_mesh_copy_out(pg_ctx* c, const pg_mesh* mc, pg_vmesh_view* o) {
         int32_t* co = m->flags.get<int32_t>(n);
         int32_t* po = m->offs.get<int32_t>(n);
         int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(std::max<int64_t>(n, m->g.ncell)));
-        pg::launch_counts_orig(n, m->g.perm.as<uint32_t>(), m->count.as<uint8_t>(), co, s);
+        pg::launch_counts_orig(n, m->out_order(), m->count.as<uint8_t>(), co, s);
         pg::scan_exclusive_i32(co, po, n, tp, m->cnt1.get<int32_t>(1), s);
         // the face arrays below are sized by the caller from pg_mesh_info:
         // refuse a mesh whose face count differs from the one reported
@@ -807,7 +900,7 @@ int pg_mesh_copy_out(pg_ctx* c, const pg_mesh* mc, pg_vmesh_view* o) {
         double* dnrm = onrm.get<double>(3 * F);
         double* dar = oarea.get<double>(F);
         double* dcen = ocen.get<double>(3 * F);
-        pg::launch_pack_faces(n, n, n * MAXP, dim, m->g.perm.as<uint32_t>(), m->count.as<uint8_t>(), po,
+        pg::launch_pack_faces(n, n, n * MAXP, dim, m->out_order(), m->count.as<uint8_t>(), po,
                               m->nbr.as<int32_t>(), m->nrm.as<double>(), m->area.as<double>(),
                               m->cen.as<double>(), as, sc, dnbr, dnrm, dar, dcen, s);
         double* dse = oseeds.get<double>(3 * n);
@@ -815,7 +908,7 @@ int pg_mesh_copy_out(pg_ctx* c, const pg_mesh* mc, pg_vmesh_view* o) {
         int32_t* dst = ost.get<int32_t>(n);
         int32_t* dkc = okc.get<int32_t>(n);
         double* dcc = occ.get<double>(3 * n);
-        pg::launch_cells_orig(n, n, dim, m->g.perm.as<uint32_t>(), m->g.sorted.as<double>(),
+        pg::launch_cells_orig(n, n, dim, m->out_order(), m->g.sorted.as<double>(),
                               m->status.as<int32_t>(), m->kcert.as<int32_t>(), m->vol.as<double>(),
                               m->ccen.as<double>(), sc, vs, dse, dvo, dst, dkc, dcc, s);
         HIPCHK(hipGetLastError());
@@ -911,7 +1004,7 @@ int pg_hydro_step(pg_ctx* c, const pg_mesh* mc, pg_state* st, double gamma, doub
         mesh_finalize(c, m);
         const int b = st->cur, nb = 1 - b;
         pg::launch_hll_static(st->dim, m->n_seeds, m->n_hydro, m->n_seeds, m->n_seeds * MAXP,
-                              m->g.perm.as<uint32_t>(), m->count.as<uint8_t>(), m->hid.as<int32_t>(),
+                              m->out_order(), m->count.as<uint8_t>(), m->hid.as<int32_t>(),
                               m->nrm.as<double>(), m->area_u.as<double>(), m->vol_u.as<double>(), gamma, dt,
                               flux_mode, st->rho[b].as<double>(), st->vel[b].as<double>(),
                               st->E[b].as<double>(), st->rho[nb].as<double>(), st->vel[nb].as<double>(),
@@ -932,6 +1025,35 @@ int pg_state_copy_out(pg_ctx* c, const pg_state* st, double* rho, double* vel, d
         if (vel) HIPCHK(hipMemcpyAsync(vel, st->vel[b].p, sizeof(double) * n * st->dim, hipMemcpyDeviceToHost, c->stream));
         if (E) HIPCHK(hipMemcpyAsync(E, st->E[b].p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pg_hll_flux(pg_ctx* c, int dim, double gamma, int64_t n, const double* left, const double* right,
+                const double* normal, const uint8_t* wall, int flux_mode, double* flux) {
+    return guarded([&] {
+        use(c);
+        argchk(dim == 2 || dim == 3, "dim must be 2 or 3");
+        argchk(gamma > 1.0, "gamma > 1 required");
+        argchk(n >= 0 && n < (int64_t)1 << 28, "n out of range");
+        argchk(flux_mode == PG_FLUX_REFERENCE || flux_mode == PG_FLUX_PHYSICAL, "unknown flux mode");
+        if (n == 0) return;
+        argchk(left && right && normal && flux, "null argument");
+        hipStream_t s = c->stream;
+        const size_t w = (size_t)dim + 2, nn = (size_t)n;
+        DBuf dl, dr, dn, dw, df;
+        HIPCHK(hipMemcpyAsync(dl.get<double>(nn * w), left, sizeof(double) * nn * w, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(dr.get<double>(nn * w), right, sizeof(double) * nn * w, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(dn.get<double>(nn * 3), normal, sizeof(double) * nn * 3, hipMemcpyHostToDevice, s));
+        const uint8_t* dwp = nullptr;
+        if (wall) {
+            HIPCHK(hipMemcpyAsync(dw.get<uint8_t>(nn), wall, nn, hipMemcpyHostToDevice, s));
+            dwp = dw.as<uint8_t>();
+        }
+        pg::launch_hll_faces(dim, n, gamma, dl.as<double>(), dr.as<double>(), dn.as<double>(), dwp, flux_mode,
+                             df.get<double>(nn * w), s);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(flux, df.as<double>(), sizeof(double) * nn * w, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
     });
 }
 
@@ -997,7 +1119,7 @@ void sim_hydro(pg_ctx* c, pg_sim* S, double dt_max, bool moving) {
     if (w) cm = {m.count.as<uint8_t>(), m.nbr.as<int32_t>(), m.g.sorted.as<double>(), m.n_seeds, sc};
     pg::launch_dt_cfl(dim, S->n, m.o2s.as<int32_t>(), m.vol_u.as<double>(), S->prm.gamma, S->prm.cfl,
                       S->rho.as<double>(), S->vel.as<double>(), S->E.as<double>(), d, s, w, cm);
-    pg::launch_hll_ale(dim, m.n_seeds, m.n_hydro, m.n_seeds, m.n_seeds * MAXP, m.g.perm.as<uint32_t>(),
+    pg::launch_hll_ale(dim, m.n_seeds, m.n_hydro, m.n_seeds, m.n_seeds * MAXP, m.out_order(),
                        m.count.as<uint8_t>(), m.hid.as<int32_t>(), m.nbr.as<int32_t>(), m.nrm.as<double>(),
                        m.area_u.as<double>(), m.cen.as<double>(), m.g.sorted.as<double>(), sc, S->prm.gamma,
                        (const double*)d, dt_max, S->prm.flux_mode, moving ? 1 : 0, S->view(), s, w);

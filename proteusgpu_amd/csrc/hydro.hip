@@ -104,6 +104,39 @@ __global__ void __launch_bounds__(TB)
     E2[i] = si.E - frac * tot.E;
 }
 
+// riemann.cpp:7-43 for n independent faces: left / right states (rho,
+// v[DIM], E per face), the stored normal of each face (3 per face), optional
+// wall flags (the right state is then the mirrored left one), fluxes out
+// (mass, momentum[DIM], energy per face).  sgn = -1 takes the outward
+// orientation (flux mode 1), exactly as the cell kernels do.
+template <int DIM>
+__global__ void __launch_bounds__(TB)
+    k_hll_faces(int64_t n, double gam, const double* __restrict__ L, const double* __restrict__ R,
+                const double* __restrict__ nrm, const uint8_t* __restrict__ wall, double sgn,
+                double* __restrict__ F) {
+    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    constexpr int W = DIM + 2;
+    Prim a, b;
+    a.v[2] = b.v[2] = 0.0;
+    a.rho = L[W * i];
+    b.rho = R[W * i];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        a.v[d] = L[W * i + 1 + d];
+        b.v[d] = R[W * i + 1 + d];
+    }
+    a.E = L[W * i + DIM + 1];
+    b.E = R[W * i + DIM + 1];
+    const bool w = wall != nullptr && wall[i] != 0;
+    const Geom g = geom_of(sgn * nrm[3 * i], sgn * nrm[3 * i + 1], sgn * nrm[3 * i + 2]);
+    const Prim f = hll<DIM>(gam, a, w ? a : b, g, w);
+    F[W * i] = f.rho;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) F[W * i + 1 + d] = f.v[d];
+    F[W * i + DIM + 1] = f.E;
+}
+
 // finite_volume_solver.cpp:151-186: per-cell CFL time step, min over cells.
 template <int DIM>
 __global__ void __launch_bounds__(TB)
@@ -419,6 +452,14 @@ void launch_hll_ale(int dim, int64_t n, int64_t n_hydro, int64_t stride, int64_t
     else
         k_hll_ale<3><<<nblk(n), TB, 0, s>>>(n, n_hydro, stride, nplane, perm, count, hid, nbr, nrm, area_u, cen,
                                             sorted, scale, gam, dt_dev, dt_max, flux_mode, move, sv, wgen);
+}
+
+void launch_hll_faces(int dim, int64_t n, double gam, const double* L, const double* R, const double* nrm,
+                      const uint8_t* wall, int flux_mode, double* F, hipStream_t s) {
+    if (n <= 0) return;
+    const double sgn = flux_mode == 1 ? -1.0 : 1.0;
+    if (dim == 2) k_hll_faces<2><<<nblk(n), TB, 0, s>>>(n, gam, L, R, nrm, wall, sgn, F);
+    else k_hll_faces<3><<<nblk(n), TB, 0, s>>>(n, gam, L, R, nrm, wall, sgn, F);
 }
 
 void launch_drift(int dim, int64_t n, double* x, const double* vel, const double* dt_dev, double dt_max,

@@ -30,39 +30,29 @@ void write_knn_dump(pg_mesh* dm, const pg_mesh_info_t& info) {
 }
 #endif
 
-VMesh* build(POINT_TYPE* pts, int64_t n, int boundary) {
-    std::cout << "VORONOI: Computing Voronoi mesh..." << std::endl;
-    pg_mesh_params prm = pgpu_host::mesh_params(boundary, _K_);
-    pg_mesh* dm = nullptr;
-    int rc = pg_mesh_compute(pgpu_host::ctx(), &prm, (const double*)pts, n, &dm);
-    if (rc == PG_ERR_GRID) {  // knn.cpp:111-115
-        std::cerr << "KNN: We don't support meshes with less than approx 12700 cells (3D)." << std::endl;
-        std::exit(EXIT_FAILURE);
-    }
-    pgpu_host::check(rc, "voronoi::compute_mesh");
+// (Re)fill the host arrays of m from the device mesh dm (copy-out in the
+// mesh's current order), sized to the actual cell and face counts.
+void fill_vmesh(VMesh* m, pg_mesh* dm) {
     pg_mesh_info_t info;
-    pg_mesh_info(dm, &info);
-#if defined(USE_HDF5) && defined(WRITE_KNN_OUTPUT)
-    write_knn_dump(dm, info);  // voronoi.cpp:42-44
-#endif
+    pgpu_host::check(pg_mesh_info(dm, &info), "pg_mesh_info");
     const size_t ns = (size_t)info.n_seeds, nf = (size_t)info.num_faces;
-    VMesh* m = (VMesh*)std::calloc(1, sizeof(VMesh));
     m->n_seeds = ns;
     m->num_faces = nf;
     m->n_hydro = (hsize_t)info.n_hydro;
     m->dev = dm;
-    m->cell_ids = (hsize_t*)std::malloc(sizeof(hsize_t) * ns);
-    m->seeds = (double3*)std::malloc(sizeof(double3) * ns);
-    m->volumes = (double*)std::malloc(sizeof(double) * ns);
-    m->face_counts = (hsize_t*)std::malloc(sizeof(hsize_t) * ns);
-    m->face_ptr = (hsize_t*)std::malloc(sizeof(hsize_t) * ns);
-    m->neighbor_cell = (int*)std::malloc(sizeof(int) * (nf ? nf : 1));
-    m->face_normal = (double3*)std::malloc(sizeof(double3) * (nf ? nf : 1));
-    m->face_area = (double*)std::malloc(sizeof(double) * (nf ? nf : 1));
-    m->status = (int*)std::malloc(sizeof(int) * ns);
-    m->kcert = (int*)std::malloc(sizeof(int) * ns);
+    m->cell_ids = (hsize_t*)std::realloc(m->cell_ids, sizeof(hsize_t) * ns);
+    m->seeds = (double3*)std::realloc(m->seeds, sizeof(double3) * ns);
+    m->volumes = (double*)std::realloc(m->volumes, sizeof(double) * ns);
+    m->face_counts = (hsize_t*)std::realloc(m->face_counts, sizeof(hsize_t) * ns);
+    m->face_ptr = (hsize_t*)std::realloc(m->face_ptr, sizeof(hsize_t) * ns);
+    m->neighbor_cell = (int*)std::realloc(m->neighbor_cell, sizeof(int) * (nf ? nf : 1));
+    m->face_normal = (double3*)std::realloc(m->face_normal, sizeof(double3) * (nf ? nf : 1));
+    m->face_area = (double*)std::realloc(m->face_area, sizeof(double) * (nf ? nf : 1));
+    m->status = (int*)std::realloc(m->status, sizeof(int) * ns);
+    m->kcert = (int*)std::realloc(m->kcert, sizeof(int) * ns);
     const size_t ng = ns - (size_t)info.n_hydro;
-    m->ghost_ids = (boundary == PG_BOUNDARY_PERIODIC && ng) ? (hsize_t*)std::malloc(sizeof(hsize_t) * ng) : nullptr;
+    std::free(m->ghost_ids);
+    m->ghost_ids = (info.n_hydro && ng) ? (hsize_t*)std::malloc(sizeof(hsize_t) * ng) : nullptr;
     pg_vmesh_view v = {};
     v.cell_ids = (int64_t*)m->cell_ids;
     v.seeds = (double*)m->seeds;
@@ -76,8 +66,27 @@ VMesh* build(POINT_TYPE* pts, int64_t n, int boundary) {
     v.status = m->status;
     v.kcert = m->kcert;
     pgpu_host::check(pg_mesh_copy_out(pgpu_host::ctx(), dm, &v), "voronoi copy-out");
+}
+
+VMesh* build(POINT_TYPE* pts, int64_t n, int boundary) {
+    std::cout << "VORONOI: Computing Voronoi mesh..." << std::endl;
+    pg_mesh_params prm = pgpu_host::mesh_params(boundary, _K_);
+    pg_mesh* dm = nullptr;
+    int rc = pg_mesh_compute(pgpu_host::ctx(), &prm, (const double*)pts, n, &dm);
+    if (rc == PG_ERR_GRID) {  // knn.cpp:111-115
+        std::cerr << "KNN: We don't support meshes with less than approx 12700 cells (3D)." << std::endl;
+        std::exit(EXIT_FAILURE);
+    }
+    pgpu_host::check(rc, "voronoi::compute_mesh");
+#if defined(USE_HDF5) && defined(WRITE_KNN_OUTPUT)
+    pg_mesh_info_t info;
+    pg_mesh_info(dm, &info);
+    write_knn_dump(dm, info);  // voronoi.cpp:42-44
+#endif
+    VMesh* m = (VMesh*)std::calloc(1, sizeof(VMesh));
+    fill_vmesh(m, dm);
     // voronoi.cpp:189-196: report cells that failed
-    for (size_t i = 0; i < ns; i++)
+    for (size_t i = 0; i < (size_t)m->n_seeds; i++)
         if (m->status[i] != success)
             std::cout << "VORONOI: cell " << i << " failed with status: " << m->status[i] << std::endl;
     return m;
@@ -94,6 +103,55 @@ VMesh* compute_periodic_mesh(POINT_TYPE* pts_data, hsize_t num_points) {
 
 VMesh* compute_reflective_mesh(POINT_TYPE* pts_data, hsize_t num_points) {
     return build(pts_data, (int64_t)num_points, PG_BOUNDARY_REFLECTIVE);
+}
+
+VMesh* allocate_vmesh(hsize_t n_seeds, hsize_t initial_face_capacity) {
+    VMesh* m = (VMesh*)std::calloc(1, sizeof(VMesh));
+    m->n_seeds = n_seeds;
+    m->cell_ids = (hsize_t*)std::malloc(sizeof(hsize_t) * (n_seeds ? n_seeds : 1));
+    m->seeds = (double3*)std::malloc(sizeof(double3) * (n_seeds ? n_seeds : 1));
+    m->volumes = (double*)std::malloc(sizeof(double) * (n_seeds ? n_seeds : 1));
+    m->face_counts = (hsize_t*)std::calloc(n_seeds ? n_seeds : 1, sizeof(hsize_t));
+    m->face_ptr = (hsize_t*)std::calloc(n_seeds ? n_seeds : 1, sizeof(hsize_t));
+    m->status = (int*)std::malloc(sizeof(int) * (n_seeds ? n_seeds : 1));
+    m->kcert = (int*)std::malloc(sizeof(int) * (n_seeds ? n_seeds : 1));
+    const hsize_t fc = initial_face_capacity ? initial_face_capacity : 1;
+    m->neighbor_cell = (int*)std::malloc(sizeof(int) * fc);
+    m->face_normal = (double3*)std::malloc(sizeof(double3) * fc);
+    m->face_area = (double*)std::malloc(sizeof(double) * fc);
+    return m;
+}
+
+void compute_cells(int N_seedpts, knn_problem* knn, std::vector<Status>& stat, VMesh* mesh) {
+    if (!knn || !mesh || N_seedpts != knn->len_pts) {
+        std::cerr << "VORONOI: compute_cells needs the solved KNN problem of its seeds" << std::endl;
+        std::exit(EXIT_FAILURE);
+    }
+    std::vector<int32_t> st((size_t)N_seedpts);
+    pg_mesh* dm = nullptr;
+    pgpu_host::check(pg_voronoi_compute(pgpu_host::ctx(), knn->dev, _MAX_P_, _MAX_T_, &dm, st.data()),
+                     "voronoi::compute_cells");
+    if (mesh->dev) pg_mesh_free(&mesh->dev);
+    fill_vmesh(mesh, dm);
+    stat.resize((size_t)N_seedpts);
+    for (int i = 0; i < N_seedpts; i++) {
+        stat[i] = (Status)st[i];
+        if (stat[i] != success)  // voronoi.cpp:104-110
+            std::cout << "VORONOI: cell " << i << " failed with status: " << stat[i] << std::endl;
+    }
+}
+
+void unpermute_vmesh(VMesh* mesh, const unsigned int* sorted_to_original) {
+    const size_t n = (size_t)mesh->n_seeds;
+    std::vector<uint32_t> perm(n);
+    if (!mesh->dev ||
+        pg_mesh_get_knn(pgpu_host::ctx(), mesh->dev, nullptr, nullptr, perm.data()) != PG_OK ||
+        !std::equal(perm.begin(), perm.end(), sorted_to_original)) {
+        std::cerr << "VORONOI: unpermute_vmesh needs a mesh of compute_cells and its KNN permutation" << std::endl;
+        std::exit(EXIT_FAILURE);
+    }
+    pgpu_host::check(pg_mesh_unpermute(pgpu_host::ctx(), mesh->dev), "voronoi::unpermute_vmesh");
+    fill_vmesh(mesh, mesh->dev);
 }
 
 void free_vmesh(VMesh* m) {

@@ -5,7 +5,10 @@
 #ifndef PGPU_HOST_VORONOI_H
 #define PGPU_HOST_VORONOI_H
 
+#include <vector>
+
 #include "../global/allvars.h"
+#include "../knn/knn.h"
 
 struct VMesh {
     hsize_t* cell_ids;      // cell ids
@@ -28,11 +31,20 @@ struct VMesh {
 
 namespace voronoi {
 
+// voronoi.h:64: cell arrays for n_seeds and face arrays of the given
+// capacity (compute_cells resizes them to the mesh it computes)
+VMesh* allocate_vmesh(hsize_t n_seeds, hsize_t initial_face_capacity);
 void free_vmesh(VMesh* mesh);                                          // voronoi.h:65
 VMesh* compute_mesh(POINT_TYPE* pts_data, int num_points);             // voronoi.h:68
 VMesh* compute_periodic_mesh(POINT_TYPE* pts_data, hsize_t num_points);  // periodic_mesh.h:22
 // no reference counterpart: the unit box as reflective walls
 VMesh* compute_reflective_mesh(POINT_TYPE* pts_data, hsize_t num_points);
+// voronoi.h:69: the cells of a solved KNN problem (no restarts), in its
+// sorted order with sorted neighbour ids; stat per sorted seed
+void compute_cells(int N_seedpts, knn_problem* knn, std::vector<Status>& stat, VMesh* mesh);
+// voronoi.h:77: cells of compute_cells back to input order (sorted_to_original
+// must be the KNN problem's permutation; the reorder runs on the device)
+void unpermute_vmesh(VMesh* mesh, const unsigned int* sorted_to_original);
 
 }  // namespace voronoi
 

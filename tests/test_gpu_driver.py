@@ -78,3 +78,13 @@ def test_driver_snapshot_and_knn_dump(ctx, tmp_path):
     assert np.array_equal(h5(mesh_f, "/hydro/rho", np.float64, tmp), rho)
     assert np.array_equal(h5(mesh_f, "/hydro/vel", np.float64, tmp).reshape(n, 3), vel)
     assert np.array_equal(h5(mesh_f, "/hydro/Energy", np.float64, tmp), E)
+
+
+def test_reference_shaped_entry_points():
+    """voronoi::allocate_vmesh / compute_cells / unpermute_vmesh and
+    hydro::riemann_hll through the C++ drop-in layer (src/check/shim_check.cpp)."""
+    exe = os.path.join(HOST, "shim_check")
+    assert os.path.exists(exe), "shim_check not built (make -C proteusgpu_amd/host)"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "SHIM OK" in r.stdout
