@@ -38,47 +38,35 @@ void attr(hid_t g, const char* name, hid_t type, T v) {
 }
 }  // namespace
 
-bool OutputHandler::writeMeshFile(const std::string& filename, const VMesh* m, const primvars* p, int n_hydro) {
+bool OutputHandler::writeMeshFile(const std::string& filename, const MeshCellData& md, const primvars* p,
+                                  int n_hydro) {
     std::string path = outputDirectory + "/" + filename;
     hid_t f = H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
     if (f < 0) {
         std::cerr << "OUTPUT: Error! Could not create file: " << path << std::endl;
         return false;
     }
-    const int n = (int)m->n_hydro;  // vmesh_to_meshdata writes n_hydro cells (voronoi.cpp:341)
+    const int n = md.header.n;
     hid_t h = H5Gcreate(f, "header", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
-    attr(h, "dimension", H5T_NATIVE_INT, (int)DIMENSION);
-    attr(h, "extent", H5T_NATIVE_DOUBLE, 1.0);
+    attr(h, "dimension", H5T_NATIVE_INT, md.header.dimension);
+    attr(h, "extent", H5T_NATIVE_DOUBLE, md.header.extent);
     attr(h, "n", H5T_NATIVE_INT, n);
-    attr(h, "k", H5T_NATIVE_INT, (int)_K_);
-    attr(h, "nmax", H5T_NATIVE_INT, (int)_MAX_P_);
-    attr(h, "seed", H5T_NATIVE_INT, 0);
-    hbool_t store = 0;
+    attr(h, "k", H5T_NATIVE_INT, md.header.k);
+    attr(h, "nmax", H5T_NATIVE_INT, md.header.nmax);
+    attr(h, "seed", H5T_NATIVE_INT, md.header.seed);
+    hbool_t store = md.header.store_edge_coords ? 1 : 0;
     attr(h, "store_edge_coords", H5T_NATIVE_HBOOL, store);
     H5Gclose(h);
     hid_t cells = H5Gcreate(f, "cells", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
-    std::vector<int> ids(n), fc(n);
-    std::vector<double> seeds((size_t)n * DIMENSION);
-    for (int i = 0; i < n; i++) {
-        ids[i] = (int)m->cell_ids[i];
-        fc[i] = (int)m->face_counts[i];
-        const double* s = (const double*)&m->seeds[i];
-        for (int d = 0; d < DIMENSION; d++) seeds[(size_t)i * DIMENSION + d] = s[d];
-    }
-    write(cells, "cell_ids", H5T_NATIVE_INT, ids.data(), n);
-    write(cells, "seeds", H5T_NATIVE_DOUBLE, seeds.data(), n, DIMENSION);
-    write(cells, "volumes", H5T_NATIVE_DOUBLE, m->volumes, n);
-    write(cells, "face_counts", H5T_NATIVE_INT, fc.data(), n);
+    write(cells, "cell_ids", H5T_NATIVE_INT, md.cell_ids.data(), n);
+    write(cells, "seeds", H5T_NATIVE_DOUBLE, md.seeds.data(), n, DIMENSION);
+    write(cells, "volumes", H5T_NATIVE_DOUBLE, md.volumes.data(), n);
+    write(cells, "face_counts", H5T_NATIVE_INT, md.face_counts.data(), n);
     hid_t faces = H5Gcreate(cells, "faces", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
-    const hsize_t nf = m->num_faces;
-    std::vector<double> nrm(nf * DIMENSION);
-    for (hsize_t i = 0; i < nf; i++) {
-        const double* v = (const double*)&m->face_normal[i];
-        for (int d = 0; d < DIMENSION; d++) nrm[i * DIMENSION + d] = v[d];
-    }
-    write(faces, "neighbor_cell", H5T_NATIVE_INT, m->neighbor_cell, nf);
-    write(faces, "normal", H5T_NATIVE_DOUBLE, nrm.data(), nf, DIMENSION);
-    write(faces, "area", H5T_NATIVE_DOUBLE, m->face_area, nf);
+    const hsize_t nf = md.faces.area.size();
+    write(faces, "neighbor_cell", H5T_NATIVE_INT, md.faces.neighbor_cell.data(), nf);
+    write(faces, "normal", H5T_NATIVE_DOUBLE, md.faces.normal.data(), nf, DIMENSION);
+    write(faces, "area", H5T_NATIVE_DOUBLE, md.faces.area.data(), nf);
     H5Gclose(faces);
     H5Gclose(cells);
     hid_t hy = H5Gcreate(f, "hydro", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
@@ -89,6 +77,12 @@ bool OutputHandler::writeMeshFile(const std::string& filename, const VMesh* m, c
     H5Fclose(f);
     std::cout << "OUTPUT: wrote " << path << std::endl;
     return true;
+}
+
+bool OutputHandler::writeMeshFile(const std::string& filename, const VMesh* m, const primvars* p, int n_hydro) {
+    MeshCellData md;
+    voronoi::vmesh_to_meshdata(const_cast<VMesh*>(m), md);
+    return writeMeshFile(filename, md, p, n_hydro);
 }
 
 bool OutputHandler::writeKNNFile(const std::string& filename, const double* points, const unsigned int* nearest,

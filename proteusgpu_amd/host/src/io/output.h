@@ -10,6 +10,7 @@
 
 #include "../global/allvars.h"
 #include "../voronoi/voronoi.h"
+#include "meshdata.h"
 
 class OutputHandler {
    public:
@@ -17,6 +18,10 @@ class OutputHandler {
     bool initialize();
     std::string getOutputDirectory() const { return outputDirectory; }
 #ifdef USE_HDF5
+    // output.h:74: the snapshot of vmesh_to_meshdata's result
+    bool writeMeshFile(const std::string& filename, const MeshCellData& meshData, const primvars* primvar,
+                       int n_hydro);
+    // the same file straight from a VMesh (vmesh_to_meshdata + the above)
     bool writeMeshFile(const std::string& filename, const VMesh* mesh, const primvars* primvar, int n_hydro);
     // KNN dump (output.cpp:305-411): /header@{dimension,num_points,k},
     // /knn/{points[num_points][d], nearest[num_points][k], permutation[num_points]},

@@ -154,6 +154,34 @@ void unpermute_vmesh(VMesh* mesh, const unsigned int* sorted_to_original) {
     fill_vmesh(mesh, mesh->dev);
 }
 
+void vmesh_to_meshdata(VMesh* mesh, MeshCellData& md) {
+    const int n = (int)mesh->n_hydro;
+    md.header = MeshHeader{};
+    md.header.n = n;
+    md.header.k = _K_;
+    md.header.nmax = _MAX_P_;
+    md.seeds_dims = {(hsize_t)n, (hsize_t)DIMENSION};
+    md.cell_ids.resize(n);
+    md.seeds.resize((size_t)n * DIMENSION);
+    md.volumes.assign(mesh->volumes, mesh->volumes + n);
+    md.face_counts.resize(n);
+    for (int i = 0; i < n; i++) {
+        md.cell_ids[i] = (int)mesh->cell_ids[i];
+        md.face_counts[i] = (int)mesh->face_counts[i];
+        const double* s = (const double*)&mesh->seeds[i];
+        for (int d = 0; d < DIMENSION; d++) md.seeds[(size_t)i * DIMENSION + d] = s[d];
+    }
+    const hsize_t nf = mesh->num_faces;
+    md.faces.neighbor_cell.assign(mesh->neighbor_cell, mesh->neighbor_cell + nf);
+    md.faces.area.assign(mesh->face_area, mesh->face_area + nf);
+    md.faces.normal_dims = {nf, (hsize_t)DIMENSION};
+    md.faces.normal.resize(nf * DIMENSION);
+    for (hsize_t f = 0; f < nf; f++) {
+        const double* v = (const double*)&mesh->face_normal[f];
+        for (int d = 0; d < DIMENSION; d++) md.faces.normal[f * DIMENSION + d] = v[d];
+    }
+}
+
 void free_vmesh(VMesh* m) {
     if (!m) return;
     std::free(m->cell_ids);

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../global/allvars.h"
+#include "../io/meshdata.h"
 #include "../knn/knn.h"
 
 struct VMesh {
@@ -45,6 +46,10 @@ void compute_cells(int N_seedpts, knn_problem* knn, std::vector<Status>& stat, V
 // voronoi.h:77: cells of compute_cells back to input order (sorted_to_original
 // must be the KNN problem's permutation; the reorder runs on the device)
 void unpermute_vmesh(VMesh* mesh, const unsigned int* sorted_to_original);
+// voronoi.h:81, voronoi.cpp:336-421: the snapshot form of a mesh (header
+// n = n_hydro cells, k = _K_, nmax = _MAX_P_, extent 1, seed 0; cell arrays
+// of the n_hydro cells, face arrays of every cell)
+void vmesh_to_meshdata(VMesh* mesh, MeshCellData& meshData);
 
 }  // namespace voronoi
 
