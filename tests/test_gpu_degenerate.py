@@ -101,3 +101,33 @@ def test_moving_mesh_from_an_exact_lattice(ctx):
     np.testing.assert_allclose(t1[-1], t0[-1], rtol=1e-13)  # energy
     _, r, _, _ = sim.get()
     assert np.all(np.isfinite(r)) and np.all(r > 0)
+
+
+@pytest.mark.parametrize("dim,n,k", [(2, 4000, 16), (3, 13000, 32)])
+def test_duplicate_generators(ctx, dim, n, k):
+    """Duplicate generators give a zero bisector (REF-Q10): both copies stay
+    uncertified (the final certificate overwrites their needs_exact_predicates,
+    voronoi.cpp:145-147) and the cells around them, which see the pair's two
+    identical planes, fail with needs_exact_predicates, all as in the
+    reference; the fallback separates the pair and every cell certifies,
+    equal to the oracle's displaced mesh."""
+    from proteusgpu_amd.generators import uniform_points
+    print(f"duplicates dim={dim} n={n}", flush=True)
+    pts = uniform_points(n, dim, 41)
+    pts[5], pts[100], pts[n // 2] = pts[17], pts[101], pts[n // 2 + 3]
+    g = api.Mesh(ctx, pts, boundary="reflective", k=k, restart=True)
+    ref = O.Mesh(pts, "reflective", k=k, restart=True)
+    out = g.copy_out()
+    _cmp_mesh(out, ref, 1e-15 if dim == 2 else 0.0)
+    st = out["status"]
+    for a, b in ((5, 17), (100, 101), (n // 2, n // 2 + 3)):
+        assert st[a] != 4 and st[b] != 4
+    assert np.sum(st == 5) > 0
+    f = api.Mesh(ctx, pts, boundary="reflective", k=k, restart=True, perturb=1e-6)
+    assert f.n_failed == 0 and f.n_degenerate == np.sum(st == 5) + np.sum(st == 2)
+    d = O.displace(pts, f.displacement, 0, False)
+    # geometry rtol 1e-8: the separated pair lies ~1e-6 h apart, so the
+    # vertices on its bisector are ~1e6 times worse conditioned than usual and
+    # the last-ulp differences of evaluation order between the two sides grow
+    # accordingly (largest seen: 9.5e-10 in one of 13000 3D volumes)
+    _cmp_mesh(f.copy_out(), O.Mesh(d, "reflective", k=k, restart=True), 1e-15 if dim == 2 else 0.0, rtol=1e-8)
