@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -150,6 +151,11 @@ struct pg_mesh {
     // problem's permutation for the KNN tie order
     bool sorted_space = false;
     DBuf iperm;
+    // learned builds: previous certified k per input id orders the bucket
+    // lists (nullptr: ascending seed id)
+    const int32_t* order_prev = nullptr;
+    int64_t order_prev_n = 0;
+    DBuf okeys, ovals, otemp;
     uint32_t* out_order() { return sorted_space ? iperm.as<uint32_t>() : g.perm.as<uint32_t>(); }
     // hydro tables
     DBuf hid, area_u, vol_u, o2s, bad;
@@ -290,6 +296,16 @@ struct Marks {
 };
 enum { MK_START, MK_GHOSTS, MK_GRID, MK_PREDICT, MK_KNN, MK_CLIP, MK_RESTART, MK_FINAL, MK_HYDRO, MK_END, MK_N };
 
+// Learned builds order each bucket's seeds by their previous certified k
+// (PGPU_LK_ORDER=0 keeps ascending seed ids).
+bool lk_order_on() {
+    static const bool on = [] {
+        const char* e = std::getenv("PGPU_LK_ORDER");
+        return !(e && std::atoi(e) == 0);
+    }();
+    return on;
+}
+
 // Face and cell outputs of ns seeds.
 void alloc_cells(pg_mesh* m, size_t ns) {
     m->nbr.get<int32_t>(ns * MAXP);
@@ -399,6 +415,15 @@ void mesh_build_once(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool o
             HIPCHK(hipStreamSynchronize(s));
         }
         uint32_t* kl = m->klist.as<uint32_t>();
+        if (m->order_prev && lk_order_on()) {
+            const int64_t hmax = *std::max_element(hc, hc + pg::LK_NCLASS);
+            const size_t tb = pg::lk_order_temp_bytes(hmax);
+            for (int c = 0; c < pg::LK_NCLASS; c++)
+                if (hc[c] > 1)
+                    HIPCHK(pg::lk_order_list(kl + (size_t)c * ns, hc[c], m->g.perm.as<uint32_t>(), m->order_prev,
+                                             m->order_prev_n, m->okeys.get<uint8_t>(2 * (size_t)hmax),
+                                             m->ovals.get<uint32_t>((size_t)hmax), m->otemp.get<uint8_t>(tb), tb, s));
+        }
         int bk[pg::LK_NCLASS];
         int64_t roff[pg::LK_NCLASS + 1] = {0};
         for (int c = 0; c < pg::LK_NCLASS; c++) {
@@ -1074,6 +1099,8 @@ struct pg_sim {
     pg_mesh mesh;
     int64_t n = 0;
     DBuf x, mass, mom, en, rho, vel, E, dt, part, wgen;
+    DBuf kc_prev;  // certified k of the last rebuild, by point id
+    bool have_prev = false;
     Marks mk;
     pg_sim_times last{};
     ~pg_sim() {
@@ -1089,11 +1116,18 @@ struct pg_sim {
 namespace {
 
 void sim_rebuild(pg_ctx* c, pg_sim* S, Marks* mk) {
+    // a learned build clips seeds in the order of their last certified k
+    S->mesh.order_prev = (S->km && S->have_prev) ? S->kc_prev.as<int32_t>() : nullptr;
+    S->mesh.order_prev_n = S->n;
     mesh_build(c, &S->mesh, S->x.as<double>(), S->n, true, mk, S->km);
     if (S->mesh.info.n_failed)
         throw Error{PG_ERR_CELLS, std::to_string(S->mesh.info.n_failed) + " cells left uncertified"};
     S->mesh.finalized = false;
     mesh_finalize(c, &S->mesh);
+    pg::launch_keep_kcert(S->mesh.n_seeds, S->mesh.g.perm.as<uint32_t>(), S->mesh.kcert.as<int32_t>(), S->n,
+                          S->kc_prev.get<int32_t>((size_t)S->n), c->stream);
+    HIPCHK(hipGetLastError());
+    S->have_prev = true;
     if (mk) mk->mark(MK_FINAL, c->stream);
 }
 

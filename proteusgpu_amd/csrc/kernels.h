@@ -158,6 +158,17 @@ int64_t lk_bucket_blocks(int64_t n_cap);
 void launch_kbucket(const int32_t* cls, const int64_t* n_dev, int64_t n_cap, int32_t* cnt, int32_t* off,
                     int32_t* tmp, int32_t* total, int64_t list_cap, uint32_t* lists, int64_t* counts,
                     hipStream_t s);
+// Order of a bucket's seed list for the clip: stable by the previous
+// certified k of each seed (prev, by input id; ids >= prev_n key 0), so that
+// the seeds a wave clips together certify after similar numbers of steps.
+// keys: 2n bytes, vals: n, temp: lk_order_temp_bytes(n).  Results never
+// depend on the order.
+size_t lk_order_temp_bytes(int64_t n);
+hipError_t lk_order_list(uint32_t* list, int64_t n, const uint32_t* perm, const int32_t* prev, int64_t prev_n,
+                         uint8_t* keys, uint32_t* vals, void* temp, size_t temp_bytes, hipStream_t s);
+// prev[perm[s]] = kcert[s] for every seed s whose input id is below n_in.
+void launch_keep_kcert(int64_t n, const uint32_t* perm, const int32_t* kcert, int64_t n_in, int32_t* prev,
+                       hipStream_t s);
 // Training label (bucket index) of every seed from its certified k.
 void launch_klabels(const int32_t* kcert, int64_t n, int32_t* y, hipStream_t s);
 // -------------------------------------------------------------- batch.hip

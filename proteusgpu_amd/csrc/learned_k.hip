@@ -15,6 +15,8 @@
 // over seeds and go through a 16x16 transpose in LDS.
 #include <algorithm>
 
+#include <hipcub/hipcub.hpp>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -634,6 +636,51 @@ void launch_kbucket(const int32_t* cls, const int64_t* n_dev, int64_t n_cap, int
     scan_exclusive_i32(cnt, off, (int64_t)NC * nblk, tmp, total, s);
     k_bucket_scatter<<<nblk, BTB, 0, s>>>(cls, n_dev, n_cap, nblk, off, list_cap, lists);
     k_bucket_counts<<<1, 64, 0, s>>>(off, total, nblk, counts);
+}
+
+namespace {
+// key of list entry i: the previous certified k of its seed (by input id),
+// capped at 255; seeds without one (ghosts, new ids) key 0
+__global__ void k_order_keys(const uint32_t* __restrict__ list, int64_t n, const uint32_t* __restrict__ perm,
+                             const int32_t* __restrict__ prev, int64_t prev_n, uint8_t* __restrict__ key) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t o = perm[list[i]];
+    key[i] = o < (uint64_t)prev_n ? (uint8_t)min(max(prev[o], 0), 255) : (uint8_t)0;
+}
+
+// prev[perm[s]] = kcert[s] for the seeds whose input id is below n_in
+__global__ void k_keep_kcert(int64_t n, const uint32_t* __restrict__ perm, const int32_t* __restrict__ kcert,
+                             int64_t n_in, int32_t* __restrict__ prev) {
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t o = perm[s];
+    if (o < (uint64_t)n_in) prev[o] = kcert[s];
+}
+}  // namespace
+
+size_t lk_order_temp_bytes(int64_t n) {
+    size_t bytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint8_t*)nullptr, (uint8_t*)nullptr,
+                                           (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 8) != hipSuccess)
+        return 0;
+    return bytes;
+}
+
+hipError_t lk_order_list(uint32_t* list, int64_t n, const uint32_t* perm, const int32_t* prev, int64_t prev_n,
+                         uint8_t* keys, uint32_t* vals, void* temp, size_t temp_bytes, hipStream_t s) {
+    if (n <= 1) return hipSuccess;
+    k_order_keys<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(list, n, perm, prev, prev_n, keys);
+    size_t bytes = temp_bytes;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys, keys + n, list, vals, (int)n, 0, 8, s);
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(list, vals, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
+}
+
+void launch_keep_kcert(int64_t n, const uint32_t* perm, const int32_t* kcert, int64_t n_in, int32_t* prev,
+                       hipStream_t s) {
+    if (n <= 0) return;
+    k_keep_kcert<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, perm, kcert, n_in, prev);
 }
 
 int lk_bucket(int c) {
