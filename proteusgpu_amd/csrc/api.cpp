@@ -155,7 +155,7 @@ struct pg_mesh {
     // lists (nullptr: ascending seed id)
     const int32_t* order_prev = nullptr;
     int64_t order_prev_n = 0;
-    DBuf okeys, ovals, otemp;
+    DBuf okeys, ovals, ocnt, ooff, otmp, ototal;
     uint32_t* out_order() { return sorted_space ? iperm.as<uint32_t>() : g.perm.as<uint32_t>(); }
     // hydro tables
     DBuf hid, area_u, vol_u, o2s, bad;
@@ -407,23 +407,27 @@ void mesh_build_once(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool o
             const int64_t nbk = pg::lk_bucket_blocks(nseeds) * pg::LK_NCLASS;
             int32_t* tp = m->g.tmp.get<int32_t>(pg::scan_tmp_elems(std::max({nseeds, m->g.ncell, nbk})));
             int64_t* dcnt = m->kcnt.get<int64_t>(pg::LK_NCLASS);
+            // seeds in the order of their previous certified k, so that the
+            // seeds a wave clips together certify after similar numbers of
+            // steps (each bucket keeps this order)
+            const uint32_t* order = nullptr;
+            if (m->order_prev && lk_order_on()) {
+                const int64_t tab = pg::lk_order_table(nseeds);
+                uint32_t* ord = m->ovals.get<uint32_t>(ns);
+                pg::launch_order_list(nullptr, nullptr, nseeds, m->g.perm.as<uint32_t>(), nullptr, m->order_prev,
+                                      m->order_prev_n, m->okeys.get<uint8_t>(ns), m->ocnt.get<int32_t>(tab),
+                                      m->ooff.get<int32_t>(tab), m->otmp.get<int32_t>(pg::scan_tmp_elems(tab)),
+                                      m->ototal.get<int32_t>(1), ord, s);
+                order = ord;
+            }
             pg::launch_kbucket(cls, nullptr, nseeds, m->flags.get<int32_t>(std::max<size_t>(ns, nbk)),
                                m->offs.get<int32_t>(std::max<size_t>(ns, nbk)), tp, m->cnt1.get<int32_t>(1), ns,
-                               m->klist.get<uint32_t>(ns * pg::LK_NCLASS), dcnt, s);
+                               m->klist.get<uint32_t>(ns * pg::LK_NCLASS), dcnt, s, order);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(hc, dcnt, sizeof(hc), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
         }
         uint32_t* kl = m->klist.as<uint32_t>();
-        if (m->order_prev && lk_order_on()) {
-            const int64_t hmax = *std::max_element(hc, hc + pg::LK_NCLASS);
-            const size_t tb = pg::lk_order_temp_bytes(hmax);
-            for (int c = 0; c < pg::LK_NCLASS; c++)
-                if (hc[c] > 1)
-                    HIPCHK(pg::lk_order_list(kl + (size_t)c * ns, hc[c], m->g.perm.as<uint32_t>(), m->order_prev,
-                                             m->order_prev_n, m->okeys.get<uint8_t>(2 * (size_t)hmax),
-                                             m->ovals.get<uint32_t>((size_t)hmax), m->otemp.get<uint8_t>(tb), tb, s));
-        }
         int bk[pg::LK_NCLASS];
         int64_t roff[pg::LK_NCLASS + 1] = {0};
         for (int c = 0; c < pg::LK_NCLASS; c++) {
@@ -1124,8 +1128,8 @@ void sim_rebuild(pg_ctx* c, pg_sim* S, Marks* mk) {
         throw Error{PG_ERR_CELLS, std::to_string(S->mesh.info.n_failed) + " cells left uncertified"};
     S->mesh.finalized = false;
     mesh_finalize(c, &S->mesh);
-    pg::launch_keep_kcert(S->mesh.n_seeds, S->mesh.g.perm.as<uint32_t>(), S->mesh.kcert.as<int32_t>(), S->n,
-                          S->kc_prev.get<int32_t>((size_t)S->n), c->stream);
+    pg::launch_keep_kcert(nullptr, S->mesh.n_seeds, S->mesh.g.perm.as<uint32_t>(), nullptr,
+                          S->mesh.kcert.as<int32_t>(), S->n, S->kc_prev.get<int32_t>((size_t)S->n), c->stream);
     HIPCHK(hipGetLastError());
     S->have_prev = true;
     if (mk) mk->mark(MK_FINAL, c->stream);
@@ -1567,6 +1571,7 @@ struct pg_batch {
     // static
     DBuf hoff, hbox, x, xs;
     DBuf feat, kcls, klists, kcnt;  // learned k
+    DBuf kc_prev, okeys, ovals, ocnt, ooff, otmp, ototal;  // bucket order: last certified k per hydro cell
     // conserved state and primitives (hydro order)
     DBuf mass, mom, en, rho, vel, E, dt_box, wgen;
     DBuf mass_s, mom_s, en_s;  // autotuner save area
@@ -1675,8 +1680,18 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
                               B->soff.as<int64_t>(), B->N.as<int32_t>(), B->coff.as<int64_t>(), B->counts.as<int32_t>(),
                               B->starts.as<int32_t>(), feat, s);
         pg::launch_kpredict(feat, cap, B->km->P.as<float>(), B->km->a0, nullptr, cls, nullptr, s, n_dev);
+        // seeds in the order of their certified k of the last rebuild (all
+        // zero before the first: ascending ids), kept inside every bucket
+        const uint32_t* order = nullptr;
+        if (lk_order_on()) {
+            pg::launch_order_list(nullptr, n_dev, cap, B->perm.as<uint32_t>(), B->orig_h.as<int32_t>(),
+                                  B->kc_prev.as<int32_t>(), B->n_h, B->okeys.as<uint8_t>(), B->ocnt.as<int32_t>(),
+                                  B->ooff.as<int32_t>(), B->otmp.as<int32_t>(), B->ototal.as<int32_t>(),
+                                  B->ovals.as<uint32_t>(), s);
+            order = B->ovals.as<uint32_t>();
+        }
         pg::launch_kbucket(cls, n_dev, cap, B->flags.as<int32_t>(), B->offs.as<int32_t>(), B->tmp.as<int32_t>(),
-                           B->total.as<int32_t>(), cap, B->klists.as<uint32_t>(), B->kcnt.as<int64_t>(), s);
+                           B->total.as<int32_t>(), cap, B->klists.as<uint32_t>(), B->kcnt.as<int64_t>(), s, order);
         for (int c = 0; c < pg::LK_NCLASS; c++) {
             const uint32_t* lc = B->klists.as<uint32_t>() + (size_t)c * cap;
             knn(lc, B->kcnt.as<int64_t>() + c, pg::lk_bucket(c), false);
@@ -1707,6 +1722,9 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
                          B->hid.as<int32_t>(), B->area_u.as<double>(), B->vol_u.as<double>(), B->o2s.as<int32_t>(),
                          B->hsorted.as<int32_t>(), B->st() + 1, s);
     pg::launch_bcount_failed(n_dev, cap, B->status.as<int32_t>(), B->st() + 3, s);
+    if (B->km && lk_order_on())
+        pg::launch_keep_kcert(n_dev, cap, B->perm.as<uint32_t>(), B->orig_h.as<int32_t>(), B->kcert.as<int32_t>(),
+                              B->n_h, B->kc_prev.as<int32_t>(), s);
 }
 
 // primitives, per-box CFL step, ALE HLL update, drift
@@ -1941,6 +1959,16 @@ int pg_batch_create(pg_ctx* c, const pg_sim_params* prm, int nbox, const int64_t
             B->kcls.get<int32_t>(cs);
             B->klists.get<uint32_t>(cs * pg::LK_NCLASS);
             B->kcnt.get<int64_t>(pg::LK_NCLASS);
+            {  // bucket ordering by the last certified k (learned steps)
+                const int64_t tab = pg::lk_order_table(cs);
+                B->okeys.get<uint8_t>(cs);
+                B->ovals.get<uint32_t>(cs);
+                B->ocnt.get<int32_t>(tab);
+                B->ooff.get<int32_t>(tab);
+                B->otmp.get<int32_t>(pg::scan_tmp_elems(tab));
+                B->ototal.get<int32_t>(1);
+                pg::launch_fill_i32(B->kc_prev.get<int32_t>(n), n, 0, s);
+            }
             HIPCHK(hipEventCreate(&B->e0));
             HIPCHK(hipEventCreate(&B->e1));
             B->lanes = pg::clip_lanes();

@@ -151,24 +151,31 @@ void launch_kgrad(const float* X, const int32_t* y, int64_t n, int64_t B, uint64
 void launch_kadam(const float* part, const float* part_loss, int nblk, float* params, float* m, float* v,
                   float* grad_out, float* loss_out, int64_t step, float lr, hipStream_t s);
 // Stable split of seeds 0..n-1 (n = *n_dev when n_dev != nullptr, else
-// n_cap) by class: class c into lists + c * list_cap (ascending seed id),
-// its size into counts[c] (device int64).  cnt, off: NC * lk_bucket_blocks
-// int32; tmp as for scan of that size.
+// n_cap) by class: class c into lists + c * list_cap (ascending seed id, or
+// in the sequence order[0..n) when order != nullptr), its size into
+// counts[c] (device int64).  cnt, off: NC * lk_bucket_blocks int32; tmp as
+// for scan of that size.
 int64_t lk_bucket_blocks(int64_t n_cap);
 void launch_kbucket(const int32_t* cls, const int64_t* n_dev, int64_t n_cap, int32_t* cnt, int32_t* off,
                     int32_t* tmp, int32_t* total, int64_t list_cap, uint32_t* lists, int64_t* counts,
-                    hipStream_t s);
-// Order of a bucket's seed list for the clip: stable by the previous
-// certified k of each seed (prev, by input id; ids >= prev_n key 0), so that
-// the seeds a wave clips together certify after similar numbers of steps.
-// keys: 2n bytes, vals: n, temp: lk_order_temp_bytes(n).  Results never
-// depend on the order.
-size_t lk_order_temp_bytes(int64_t n);
-hipError_t lk_order_list(uint32_t* list, int64_t n, const uint32_t* perm, const int32_t* prev, int64_t prev_n,
-                         uint8_t* keys, uint32_t* vals, void* temp, size_t temp_bytes, hipStream_t s);
-// prev[perm[s]] = kcert[s] for every seed s whose input id is below n_in.
-void launch_keep_kcert(int64_t n, const uint32_t* perm, const int32_t* kcert, int64_t n_in, int32_t* prev,
-                       hipStream_t s);
+                    hipStream_t s, const uint32_t* order = nullptr);
+// Clip order of the seeds of a learned build: a stable sort of list[0..n)
+// (list == nullptr: the seeds 0..n-1; n = *n_dev when n_dev != nullptr,
+// else n_cap) by the previous certified k of each seed's point (o =
+// perm[s], through id_map when given; prev[o] for o in [0, prev_n), else
+// key 0), so that the seeds a wave clips together certify after similar
+// numbers of steps.  Results never depend on the order.  The sorted ids go
+// to out (and back into list when list != nullptr).  keys: n_cap bytes;
+// cnt, off: lk_order_table(n_cap) int32; tmp as for a scan of that size;
+// out: n_cap.  Graph capturable.
+int64_t lk_order_table(int64_t n_cap);
+void launch_order_list(uint32_t* list, const int64_t* n_dev, int64_t n_cap, const uint32_t* perm,
+                       const int32_t* id_map, const int32_t* prev, int64_t prev_n, uint8_t* keys, int32_t* cnt,
+                       int32_t* off, int32_t* tmp, int32_t* total, uint32_t* out, hipStream_t s);
+// prev[o] = kcert[s] for the sorted seeds s (as above) whose point o lies in
+// [0, prev_n).
+void launch_keep_kcert(const int64_t* n_dev, int64_t n_cap, const uint32_t* perm, const int32_t* id_map,
+                       const int32_t* kcert, int64_t prev_n, int32_t* prev, hipStream_t s);
 // Training label (bucket index) of every seed from its certified k.
 void launch_klabels(const int32_t* kcert, int64_t n, int32_t* y, hipStream_t s);
 // -------------------------------------------------------------- batch.hip

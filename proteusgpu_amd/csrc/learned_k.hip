@@ -15,8 +15,6 @@
 // over seeds and go through a 16x16 transpose in LDS.
 #include <algorithm>
 
-#include <hipcub/hipcub.hpp>
-
 #include "common.h"
 #include "kernels.h"
 
@@ -570,12 +568,12 @@ constexpr int BTB = 256;
 PG_DEV int64_t seed_count(const int64_t* n_dev, int64_t n) { return n_dev ? *n_dev : n; }
 
 __global__ void __launch_bounds__(BTB)
-    k_bucket_hist(const int32_t* __restrict__ cls, const int64_t* __restrict__ n_dev, int64_t n_host, int nblk,
-                  int32_t* __restrict__ cnt) {
+    k_bucket_hist(const int32_t* __restrict__ cls, const uint32_t* __restrict__ order, const int64_t* __restrict__ n_dev,
+                  int64_t n_host, int nblk, int32_t* __restrict__ cnt) {
     __shared__ int32_t wc[BTB / 64][NC];
     const int64_t n = seed_count(n_dev, n_host);
     const int64_t i = (int64_t)blockIdx.x * BTB + threadIdx.x;
-    const int c = i < n ? cls[i] : -1;
+    const int c = i < n ? cls[order ? order[i] : (uint32_t)i] : -1;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < NC; k++) {
@@ -592,12 +590,14 @@ __global__ void __launch_bounds__(BTB)
 }
 
 __global__ void __launch_bounds__(BTB)
-    k_bucket_scatter(const int32_t* __restrict__ cls, const int64_t* __restrict__ n_dev, int64_t n_host, int nblk,
-                     const int32_t* __restrict__ off, int64_t list_cap, uint32_t* __restrict__ lists) {
+    k_bucket_scatter(const int32_t* __restrict__ cls, const uint32_t* __restrict__ order,
+                     const int64_t* __restrict__ n_dev, int64_t n_host, int nblk, const int32_t* __restrict__ off,
+                     int64_t list_cap, uint32_t* __restrict__ lists) {
     __shared__ int32_t wc[BTB / 64][NC];
     const int64_t n = seed_count(n_dev, n_host);
     const int64_t i = (int64_t)blockIdx.x * BTB + threadIdx.x;
-    const int c = i < n ? cls[i] : -1;
+    const uint32_t seed = order && i < n ? order[i] : (uint32_t)i;
+    const int c = i < n ? cls[seed] : -1;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     int rank = 0;
 #pragma unroll
@@ -610,7 +610,7 @@ __global__ void __launch_bounds__(BTB)
     if (c < 0) return;
     for (int ww = 0; ww < w; ww++) rank += wc[ww][c];
     const int64_t pos = (int64_t)off[c * nblk + blockIdx.x] - off[c * nblk] + rank;
-    lists[(int64_t)c * list_cap + pos] = (uint32_t)i;
+    lists[(int64_t)c * list_cap + pos] = seed;
 }
 
 // counts[c] = seeds of class c (int64, device)
@@ -629,58 +629,114 @@ int64_t lk_bucket_blocks(int64_t n_cap) { return (n_cap + BTB - 1) / BTB; }
 
 void launch_kbucket(const int32_t* cls, const int64_t* n_dev, int64_t n_cap, int32_t* cnt, int32_t* off,
                     int32_t* tmp, int32_t* total, int64_t list_cap, uint32_t* lists, int64_t* counts,
-                    hipStream_t s) {
+                    hipStream_t s, const uint32_t* order) {
     const int nblk = (int)lk_bucket_blocks(n_cap);
     if (nblk <= 0) return;
-    k_bucket_hist<<<nblk, BTB, 0, s>>>(cls, n_dev, n_cap, nblk, cnt);
+    k_bucket_hist<<<nblk, BTB, 0, s>>>(cls, order, n_dev, n_cap, nblk, cnt);
     scan_exclusive_i32(cnt, off, (int64_t)NC * nblk, tmp, total, s);
-    k_bucket_scatter<<<nblk, BTB, 0, s>>>(cls, n_dev, n_cap, nblk, off, list_cap, lists);
+    k_bucket_scatter<<<nblk, BTB, 0, s>>>(cls, order, n_dev, n_cap, nblk, off, list_cap, lists);
     k_bucket_counts<<<1, 64, 0, s>>>(off, total, nblk, counts);
 }
 
 namespace {
-// key of list entry i: the previous certified k of its seed (by input id),
-// capped at 255; seeds without one (ghosts, new ids) key 0
-__global__ void k_order_keys(const uint32_t* __restrict__ list, int64_t n, const uint32_t* __restrict__ perm,
-                             const int32_t* __restrict__ prev, int64_t prev_n, uint8_t* __restrict__ key) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t o = perm[list[i]];
-    key[i] = o < (uint64_t)prev_n ? (uint8_t)min(max(prev[o], 0), 255) : (uint8_t)0;
+constexpr int NKEY = 256;  // order key: min(previous certified k, 255)
+
+// key of sorted seed s: the previous certified k of its point (o = perm[s],
+// through id_map when given; o outside [0, prev_n) keys 0)
+PG_DEV int order_key(uint32_t s, const uint32_t* __restrict__ perm, const int32_t* __restrict__ id_map,
+                     const int32_t* __restrict__ prev, int64_t prev_n) {
+    int64_t o = perm[s];
+    if (id_map) o = id_map[o];
+    if (o < 0 || o >= prev_n) return 0;
+    return min(max(prev[o], 0), 255);
 }
 
-// prev[perm[s]] = kcert[s] for the seeds whose input id is below n_in
-__global__ void k_keep_kcert(int64_t n, const uint32_t* __restrict__ perm, const int32_t* __restrict__ kcert,
-                             int64_t n_in, int32_t* __restrict__ prev) {
+__global__ void __launch_bounds__(BTB)
+    k_okey_hist(const uint32_t* __restrict__ list, const int64_t* __restrict__ n_dev, int64_t n_host,
+                const uint32_t* __restrict__ perm, const int32_t* __restrict__ id_map,
+                const int32_t* __restrict__ prev, int64_t prev_n, int nblk, uint8_t* __restrict__ keys,
+                int32_t* __restrict__ cnt) {
+    __shared__ int32_t h[NKEY];
+    for (int t = threadIdx.x; t < NKEY; t += BTB) h[t] = 0;
+    __syncthreads();
+    const int64_t n = seed_count(n_dev, n_host);
+    const int64_t i = (int64_t)blockIdx.x * BTB + threadIdx.x;
+    if (i < n) {
+        const int key = order_key(list ? list[i] : (uint32_t)i, perm, id_map, prev, prev_n);
+        keys[i] = (uint8_t)key;
+        atomicAdd(&h[key], 1);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NKEY; t += BTB) cnt[t * nblk + blockIdx.x] = h[t];
+}
+
+// stable: within a key, entries keep their list order (ranks by lane within
+// a wave, then by wave within the block, blocks by the key-major scan)
+__global__ void __launch_bounds__(BTB)
+    k_okey_scatter(const uint32_t* __restrict__ list, const int64_t* __restrict__ n_dev, int64_t n_host,
+                   const uint8_t* __restrict__ keys, int nblk, const int32_t* __restrict__ off,
+                   uint32_t* __restrict__ out) {
+    __shared__ int32_t wc[BTB / 64][NKEY];
+    for (int t = threadIdx.x; t < (BTB / 64) * NKEY; t += BTB) (&wc[0][0])[t] = 0;
+    __syncthreads();
+    const int64_t n = seed_count(n_dev, n_host);
+    const int64_t i = (int64_t)blockIdx.x * BTB + threadIdx.x;
+    const int key = i < n ? (int)keys[i] : -1;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    int rank = 0;
+    // peel the wave's distinct keys one at a time (todo is wave-uniform)
+    uint64_t todo = (uint64_t)__ballot(key >= 0);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const int lk = __shfl(key, leader, 64);
+        const uint64_t m = (uint64_t)__ballot(key == lk);
+        if (key == lk) rank = __popcll(m & ((1ull << l) - 1ull));
+        if (l == leader) wc[w][lk] = __popcll(m);
+        todo &= ~m;
+    }
+    __syncthreads();
+    if (key < 0) return;
+    for (int ww = 0; ww < w; ww++) rank += wc[ww][key];
+    out[off[key * nblk + blockIdx.x] + rank] = list ? list[i] : (uint32_t)i;
+}
+
+__global__ void __launch_bounds__(BTB)
+    k_ocopy(const uint32_t* __restrict__ src, const int64_t* __restrict__ n_dev, int64_t n_host,
+            uint32_t* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * BTB + threadIdx.x;
+    if (i < seed_count(n_dev, n_host)) dst[i] = src[i];
+}
+
+// prev[o] = kcert[s] for the sorted seeds s whose point o (perm[s], through
+// id_map when given) lies in [0, prev_n)
+__global__ void k_keep_kcert(const int64_t* __restrict__ n_dev, int64_t n_host, const uint32_t* __restrict__ perm,
+                             const int32_t* __restrict__ id_map, const int32_t* __restrict__ kcert, int64_t prev_n,
+                             int32_t* __restrict__ prev) {
     const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t o = perm[s];
-    if (o < (uint64_t)n_in) prev[o] = kcert[s];
+    if (s >= seed_count(n_dev, n_host)) return;
+    int64_t o = perm[s];
+    if (id_map) o = id_map[o];
+    if (o >= 0 && o < prev_n) prev[o] = kcert[s];
 }
 }  // namespace
 
-size_t lk_order_temp_bytes(int64_t n) {
-    size_t bytes = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint8_t*)nullptr, (uint8_t*)nullptr,
-                                           (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 8) != hipSuccess)
-        return 0;
-    return bytes;
+int64_t lk_order_table(int64_t n_cap) { return (int64_t)NKEY * lk_bucket_blocks(n_cap); }
+
+void launch_order_list(uint32_t* list, const int64_t* n_dev, int64_t n_cap, const uint32_t* perm,
+                       const int32_t* id_map, const int32_t* prev, int64_t prev_n, uint8_t* keys, int32_t* cnt,
+                       int32_t* off, int32_t* tmp, int32_t* total, uint32_t* out, hipStream_t s) {
+    const int nblk = (int)lk_bucket_blocks(n_cap);
+    if (nblk <= 0) return;
+    k_okey_hist<<<nblk, BTB, 0, s>>>(list, n_dev, n_cap, perm, id_map, prev, prev_n, nblk, keys, cnt);
+    scan_exclusive_i32(cnt, off, (int64_t)NKEY * nblk, tmp, total, s);
+    k_okey_scatter<<<nblk, BTB, 0, s>>>(list, n_dev, n_cap, keys, nblk, off, out);
+    if (list) k_ocopy<<<nblk, BTB, 0, s>>>(out, n_dev, n_cap, list);
 }
 
-hipError_t lk_order_list(uint32_t* list, int64_t n, const uint32_t* perm, const int32_t* prev, int64_t prev_n,
-                         uint8_t* keys, uint32_t* vals, void* temp, size_t temp_bytes, hipStream_t s) {
-    if (n <= 1) return hipSuccess;
-    k_order_keys<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(list, n, perm, prev, prev_n, keys);
-    size_t bytes = temp_bytes;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys, keys + n, list, vals, (int)n, 0, 8, s);
-    if (e != hipSuccess) return e;
-    return hipMemcpyAsync(list, vals, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
-}
-
-void launch_keep_kcert(int64_t n, const uint32_t* perm, const int32_t* kcert, int64_t n_in, int32_t* prev,
-                       hipStream_t s) {
-    if (n <= 0) return;
-    k_keep_kcert<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, perm, kcert, n_in, prev);
+void launch_keep_kcert(const int64_t* n_dev, int64_t n_cap, const uint32_t* perm, const int32_t* id_map,
+                       const int32_t* kcert, int64_t prev_n, int32_t* prev, hipStream_t s) {
+    if (n_cap <= 0) return;
+    k_keep_kcert<<<(unsigned)((n_cap + 255) / 256), 256, 0, s>>>(n_dev, n_cap, perm, id_map, kcert, prev_n, prev);
 }
 
 int lk_bucket(int c) {
