@@ -155,7 +155,7 @@ struct pg_mesh {
     // lists (nullptr: ascending seed id)
     const int32_t* order_prev = nullptr;
     int64_t order_prev_n = 0;
-    DBuf okeys, ovals, ocnt, ooff, otmp, ototal;
+    DBuf ovals;  // seeds in clip order
     uint32_t* out_order() { return sorted_space ? iperm.as<uint32_t>() : g.perm.as<uint32_t>(); }
     // hydro tables
     DBuf hid, area_u, vol_u, o2s, bad;
@@ -412,12 +412,9 @@ void mesh_build_once(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool o
             // steps (each bucket keeps this order)
             const uint32_t* order = nullptr;
             if (m->order_prev && lk_order_on()) {
-                const int64_t tab = pg::lk_order_table(nseeds);
                 uint32_t* ord = m->ovals.get<uint32_t>(ns);
                 pg::launch_order_list(nullptr, nullptr, nseeds, m->g.perm.as<uint32_t>(), nullptr, m->order_prev,
-                                      m->order_prev_n, m->okeys.get<uint8_t>(ns), m->ocnt.get<int32_t>(tab),
-                                      m->ooff.get<int32_t>(tab), m->otmp.get<int32_t>(pg::scan_tmp_elems(tab)),
-                                      m->ototal.get<int32_t>(1), ord, s);
+                                      m->order_prev_n, ord, s);
                 order = ord;
             }
             pg::launch_kbucket(cls, nullptr, nseeds, m->flags.get<int32_t>(std::max<size_t>(ns, nbk)),
@@ -1571,7 +1568,7 @@ struct pg_batch {
     // static
     DBuf hoff, hbox, x, xs;
     DBuf feat, kcls, klists, kcnt;  // learned k
-    DBuf kc_prev, okeys, ovals, ocnt, ooff, otmp, ototal;  // bucket order: last certified k per hydro cell
+    DBuf kc_prev, ovals;  // clip order: last certified k per hydro cell, seeds in that order
     // conserved state and primitives (hydro order)
     DBuf mass, mom, en, rho, vel, E, dt_box, wgen;
     DBuf mass_s, mom_s, en_s;  // autotuner save area
@@ -1685,9 +1682,7 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
         const uint32_t* order = nullptr;
         if (lk_order_on()) {
             pg::launch_order_list(nullptr, n_dev, cap, B->perm.as<uint32_t>(), B->orig_h.as<int32_t>(),
-                                  B->kc_prev.as<int32_t>(), B->n_h, B->okeys.as<uint8_t>(), B->ocnt.as<int32_t>(),
-                                  B->ooff.as<int32_t>(), B->otmp.as<int32_t>(), B->ototal.as<int32_t>(),
-                                  B->ovals.as<uint32_t>(), s);
+                                  B->kc_prev.as<int32_t>(), B->n_h, B->ovals.as<uint32_t>(), s);
             order = B->ovals.as<uint32_t>();
         }
         pg::launch_kbucket(cls, n_dev, cap, B->flags.as<int32_t>(), B->offs.as<int32_t>(), B->tmp.as<int32_t>(),
@@ -1959,16 +1954,9 @@ int pg_batch_create(pg_ctx* c, const pg_sim_params* prm, int nbox, const int64_t
             B->kcls.get<int32_t>(cs);
             B->klists.get<uint32_t>(cs * pg::LK_NCLASS);
             B->kcnt.get<int64_t>(pg::LK_NCLASS);
-            {  // bucket ordering by the last certified k (learned steps)
-                const int64_t tab = pg::lk_order_table(cs);
-                B->okeys.get<uint8_t>(cs);
-                B->ovals.get<uint32_t>(cs);
-                B->ocnt.get<int32_t>(tab);
-                B->ooff.get<int32_t>(tab);
-                B->otmp.get<int32_t>(pg::scan_tmp_elems(tab));
-                B->ototal.get<int32_t>(1);
-                pg::launch_fill_i32(B->kc_prev.get<int32_t>(n), n, 0, s);
-            }
+            // clip order of learned steps: last certified k per hydro cell
+            B->ovals.get<uint32_t>(cs);
+            pg::launch_fill_i32(B->kc_prev.get<int32_t>(n), n, 0, s);
             HIPCHK(hipEventCreate(&B->e0));
             HIPCHK(hipEventCreate(&B->e1));
             B->lanes = pg::clip_lanes();

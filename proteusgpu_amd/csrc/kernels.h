@@ -159,19 +159,16 @@ int64_t lk_bucket_blocks(int64_t n_cap);
 void launch_kbucket(const int32_t* cls, const int64_t* n_dev, int64_t n_cap, int32_t* cnt, int32_t* off,
                     int32_t* tmp, int32_t* total, int64_t list_cap, uint32_t* lists, int64_t* counts,
                     hipStream_t s, const uint32_t* order = nullptr);
-// Clip order of the seeds of a learned build: a stable sort of list[0..n)
-// (list == nullptr: the seeds 0..n-1; n = *n_dev when n_dev != nullptr,
-// else n_cap) by the previous certified k of each seed's point (o =
+// Clip order of the seeds of a learned build: list[0..n) (list == nullptr:
+// the seeds 0..n-1; n = *n_dev when n_dev != nullptr, else n_cap) into out,
+// sorted stably by the previous certified k of each seed's point (o =
 // perm[s], through id_map when given; prev[o] for o in [0, prev_n), else
-// key 0), so that the seeds a wave clips together certify after similar
-// numbers of steps.  Results never depend on the order.  The sorted ids go
-// to out (and back into list when list != nullptr).  keys: n_cap bytes;
-// cnt, off: lk_order_table(n_cap) int32; tmp as for a scan of that size;
-// out: n_cap.  Graph capturable.
-int64_t lk_order_table(int64_t n_cap);
-void launch_order_list(uint32_t* list, const int64_t* n_dev, int64_t n_cap, const uint32_t* perm,
-                       const int32_t* id_map, const int32_t* prev, int64_t prev_n, uint8_t* keys, int32_t* cnt,
-                       int32_t* off, int32_t* tmp, int32_t* total, uint32_t* out, hipStream_t s);
+// key 0) within tiles of 4096 consecutive positions, so that the seeds a
+// wave clips together certify after similar numbers of steps while a
+// launch still sweeps space tile by tile.  Results never depend on the
+// order.  out must not alias list.  Graph capturable.
+void launch_order_list(const uint32_t* list, const int64_t* n_dev, int64_t n_cap, const uint32_t* perm,
+                       const int32_t* id_map, const int32_t* prev, int64_t prev_n, uint32_t* out, hipStream_t s);
 // prev[o] = kcert[s] for the sorted seeds s (as above) whose point o lies in
 // [0, prev_n).
 void launch_keep_kcert(const int64_t* n_dev, int64_t n_cap, const uint32_t* perm, const int32_t* id_map,

@@ -651,60 +651,77 @@ PG_DEV int order_key(uint32_t s, const uint32_t* __restrict__ perm, const int32_
     return min(max(prev[o], 0), 255);
 }
 
-__global__ void __launch_bounds__(BTB)
-    k_okey_hist(const uint32_t* __restrict__ list, const int64_t* __restrict__ n_dev, int64_t n_host,
-                const uint32_t* __restrict__ perm, const int32_t* __restrict__ id_map,
-                const int32_t* __restrict__ prev, int64_t prev_n, int nblk, uint8_t* __restrict__ keys,
-                int32_t* __restrict__ cnt) {
-    __shared__ int32_t h[NKEY];
-    for (int t = threadIdx.x; t < NKEY; t += BTB) h[t] = 0;
-    __syncthreads();
-    const int64_t n = seed_count(n_dev, n_host);
-    const int64_t i = (int64_t)blockIdx.x * BTB + threadIdx.x;
-    if (i < n) {
-        const int key = order_key(list ? list[i] : (uint32_t)i, perm, id_map, prev, prev_n);
-        keys[i] = (uint8_t)key;
-        atomicAdd(&h[key], 1);
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < NKEY; t += BTB) cnt[t * nblk + blockIdx.x] = h[t];
-}
+constexpr int OTILE = 4096;  // list positions per tile of the clip order
 
-// stable: within a key, entries keep their list order (ranks by lane within
-// a wave, then by wave within the block, blocks by the key-major scan)
+// One tile of OTILE consecutive list positions per block, reordered in
+// place-of-tile (into out) by key, stably: key histogram in LDS, its
+// exclusive scan, then chunks of BTB entries ranked by ballots (lane order
+// within a wave, wave order within the chunk, chunk order within the tile).
 __global__ void __launch_bounds__(BTB)
-    k_okey_scatter(const uint32_t* __restrict__ list, const int64_t* __restrict__ n_dev, int64_t n_host,
-                   const uint8_t* __restrict__ keys, int nblk, const int32_t* __restrict__ off,
-                   uint32_t* __restrict__ out) {
+    k_order_tiles(const uint32_t* __restrict__ list, const int64_t* __restrict__ n_dev, int64_t n_host,
+                  const uint32_t* __restrict__ perm, const int32_t* __restrict__ id_map,
+                  const int32_t* __restrict__ prev, int64_t prev_n, uint32_t* __restrict__ out) {
+    static_assert(BTB == NKEY, "one key counter per thread");
+    __shared__ uint8_t key_s[OTILE];
+    __shared__ int32_t base[NKEY];
     __shared__ int32_t wc[BTB / 64][NKEY];
-    for (int t = threadIdx.x; t < (BTB / 64) * NKEY; t += BTB) (&wc[0][0])[t] = 0;
-    __syncthreads();
+    __shared__ int32_t wt[BTB / 64];
     const int64_t n = seed_count(n_dev, n_host);
-    const int64_t i = (int64_t)blockIdx.x * BTB + threadIdx.x;
-    const int key = i < n ? (int)keys[i] : -1;
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    int rank = 0;
-    // peel the wave's distinct keys one at a time (todo is wave-uniform)
-    uint64_t todo = (uint64_t)__ballot(key >= 0);
-    while (todo) {
-        const int leader = __ffsll((unsigned long long)todo) - 1;
-        const int lk = __shfl(key, leader, 64);
-        const uint64_t m = (uint64_t)__ballot(key == lk);
-        if (key == lk) rank = __popcll(m & ((1ull << l) - 1ull));
-        if (l == leader) wc[w][lk] = __popcll(m);
-        todo &= ~m;
+    const int64_t t0 = (int64_t)blockIdx.x * OTILE;
+    if (t0 >= n) return;  // whole block
+    const int len = (int)min<int64_t>(OTILE, n - t0);
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    base[tid] = 0;
+    __syncthreads();
+    for (int j = tid; j < len; j += BTB) {
+        const uint32_t e = list ? list[t0 + j] : (uint32_t)(t0 + j);
+        const int key = order_key(e, perm, id_map, prev, prev_n);
+        key_s[j] = (uint8_t)key;
+        atomicAdd(&base[key], 1);
     }
     __syncthreads();
-    if (key < 0) return;
-    for (int ww = 0; ww < w; ww++) rank += wc[ww][key];
-    out[off[key * nblk + blockIdx.x] + rank] = list ? list[i] : (uint32_t)i;
-}
-
-__global__ void __launch_bounds__(BTB)
-    k_ocopy(const uint32_t* __restrict__ src, const int64_t* __restrict__ n_dev, int64_t n_host,
-            uint32_t* __restrict__ dst) {
-    const int64_t i = (int64_t)blockIdx.x * BTB + threadIdx.x;
-    if (i < seed_count(n_dev, n_host)) dst[i] = src[i];
+    {  // exclusive scan of the NKEY counts, one per thread
+        const int v = base[tid];
+        int incl = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int u = __shfl_up(incl, d, 64);
+            if (l >= d) incl += u;
+        }
+        if (l == 63) wt[w] = incl;
+        __syncthreads();
+        for (int ww = 0; ww < w; ww++) incl += wt[ww];
+        base[tid] = incl - v;
+        __syncthreads();
+    }
+    for (int c0 = 0; c0 < len; c0 += BTB) {
+        const int j = c0 + tid;
+        const int key = j < len ? (int)key_s[j] : -1;
+        for (int t = tid; t < (BTB / 64) * NKEY; t += BTB) (&wc[0][0])[t] = 0;
+        __syncthreads();
+        int rank = 0;
+        // peel the wave's distinct keys one at a time (todo is wave-uniform)
+        uint64_t todo = (uint64_t)__ballot(key >= 0);
+        while (todo) {
+            const int leader = __ffsll((unsigned long long)todo) - 1;
+            const int lk = __shfl(key, leader, 64);
+            const uint64_t m = (uint64_t)__ballot(key == lk);
+            if (key == lk) rank = __popcll(m & ((1ull << l) - 1ull));
+            if (l == leader) wc[w][lk] = __popcll(m);
+            todo &= ~m;
+        }
+        __syncthreads();
+        if (key >= 0) {
+            for (int ww = 0; ww < w; ww++) rank += wc[ww][key];
+            out[t0 + base[key] + rank] = list ? list[t0 + j] : (uint32_t)(t0 + j);
+        }
+        __syncthreads();
+        int add = 0;
+#pragma unroll
+        for (int ww = 0; ww < BTB / 64; ww++) add += wc[ww][tid];
+        base[tid] += add;
+        __syncthreads();
+    }
 }
 
 // prev[o] = kcert[s] for the sorted seeds s whose point o (perm[s], through
@@ -720,17 +737,11 @@ __global__ void k_keep_kcert(const int64_t* __restrict__ n_dev, int64_t n_host, 
 }
 }  // namespace
 
-int64_t lk_order_table(int64_t n_cap) { return (int64_t)NKEY * lk_bucket_blocks(n_cap); }
-
-void launch_order_list(uint32_t* list, const int64_t* n_dev, int64_t n_cap, const uint32_t* perm,
-                       const int32_t* id_map, const int32_t* prev, int64_t prev_n, uint8_t* keys, int32_t* cnt,
-                       int32_t* off, int32_t* tmp, int32_t* total, uint32_t* out, hipStream_t s) {
-    const int nblk = (int)lk_bucket_blocks(n_cap);
-    if (nblk <= 0) return;
-    k_okey_hist<<<nblk, BTB, 0, s>>>(list, n_dev, n_cap, perm, id_map, prev, prev_n, nblk, keys, cnt);
-    scan_exclusive_i32(cnt, off, (int64_t)NKEY * nblk, tmp, total, s);
-    k_okey_scatter<<<nblk, BTB, 0, s>>>(list, n_dev, n_cap, keys, nblk, off, out);
-    if (list) k_ocopy<<<nblk, BTB, 0, s>>>(out, n_dev, n_cap, list);
+void launch_order_list(const uint32_t* list, const int64_t* n_dev, int64_t n_cap, const uint32_t* perm,
+                       const int32_t* id_map, const int32_t* prev, int64_t prev_n, uint32_t* out, hipStream_t s) {
+    if (n_cap <= 0) return;
+    const unsigned nb = (unsigned)((n_cap + OTILE - 1) / OTILE);
+    k_order_tiles<<<nb, BTB, 0, s>>>(list, n_dev, n_cap, perm, id_map, prev, prev_n, out);
 }
 
 void launch_keep_kcert(const int64_t* n_dev, int64_t n_cap, const uint32_t* perm, const int32_t* id_map,
