@@ -191,3 +191,31 @@ def test_learned_k_cuts_work(ctx):
         for a, b in zip(res[0], r):
             assert np.array_equal(a, b)
     assert ms[(False, True)] < 0.97 * ms[(False, False)]
+
+
+@pytest.mark.parametrize("dim,n,boundary", [(3, 30000, "reflective"), (2, 20000, "periodic")])
+def test_learned_simulation_equals_fixed_k(ctx, dim, n, boundary):
+    """Moving-mesh steps with learned k (from the second rebuild on, the
+    seeds are clipped in the order of their previous certified k) give the
+    fixed-k simulation bit for bit."""
+    from proteusgpu_amd.generators import sod_state
+    print(f"learned sim dim={dim} n={n} {boundary}", flush=True)
+    buff = 0.12 if boundary == "periodic" else 0.04
+    k = 32 if dim == 3 else 16
+    pts = uniform_points(n, dim, 17)
+    rho, vel, E = sod_state(pts, 1.4)
+    kw = dict(boundary=boundary, buff=buff, k=k, gamma=1.4, cfl=0.3, flux="physical", move_mesh=True)
+    km = api.KModel(ctx, seed=6)
+    km.train_mesh(api.Mesh(ctx, uniform_points(n, dim, 18), boundary=boundary, buff=buff, k=k), steps=200,
+                  batch=4096, lr=3e-3)
+    runs = []
+    for model in (None, km):
+        sim = api.Simulation(ctx, pts, rho, vel, E, **kw)
+        if model is not None:
+            sim.set_kmodel(model)
+        dts = [sim.step() for _ in range(4)]
+        runs.append((dts, sim.get()))
+        sim.free()
+    assert runs[0][0] == runs[1][0]
+    for a, b in zip(runs[0][1], runs[1][1]):
+        assert np.array_equal(a, b)
