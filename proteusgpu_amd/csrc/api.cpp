@@ -796,6 +796,9 @@ int pg_voronoi_compute(pg_ctx* c, const pg_knn* kc, int max_p, int max_t, pg_mes
         argchk(kc && out, "null argument");
         pg_knn* kp = const_cast<pg_knn*>(kc);
         argchk(kp->k >= 1, "the KNN problem is not solved (pg_knn_solve)");
+        // a row longer than the other points would hold missing entries
+        // (UINT_MAX); unreachable while pg_knn_init needs N_grid >= 16
+        argchk(kp->g.n - 1 >= kp->k, "k must be below the number of points");
         const int dim = kp->g.dim;
         argchk(max_p >= 2 * dim + 1 && max_p <= MAXP, "max_p must be in [2*dim+1, 64]");
         argchk(max_t >= 8 && max_t <= 128, "max_t must be in [8, 128]");

@@ -114,3 +114,4 @@ def test_unpermute_equals_compute_mesh(ctx):
         assert np.array_equal(a[key], b[key]), key
     with pytest.raises(RuntimeError):
         mesh.unpermute()
+
