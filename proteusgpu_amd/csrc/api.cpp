@@ -90,6 +90,7 @@ constexpr uint64_t DISPLACE_SALT = 0xD1B54A32D192ED03ull;
 struct pg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    DBuf hll_io[5];  // pg_hll_flux staging (left, right, normal, wall, flux), grow-only
 };
 
 namespace {
@@ -1069,7 +1070,7 @@ int pg_hll_flux(pg_ctx* c, int dim, double gamma, int64_t n, const double* left,
         argchk(left && right && normal && flux, "null argument");
         hipStream_t s = c->stream;
         const size_t w = (size_t)dim + 2, nn = (size_t)n;
-        DBuf dl, dr, dn, dw, df;
+        DBuf &dl = c->hll_io[0], &dr = c->hll_io[1], &dn = c->hll_io[2], &dw = c->hll_io[3], &df = c->hll_io[4];
         HIPCHK(hipMemcpyAsync(dl.get<double>(nn * w), left, sizeof(double) * nn * w, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(dr.get<double>(nn * w), right, sizeof(double) * nn * w, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(dn.get<double>(nn * 3), normal, sizeof(double) * nn * 3, hipMemcpyHostToDevice, s));
