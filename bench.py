@@ -38,6 +38,10 @@ CLIP_FIXED_BYTES_PER_CELL = 12 + 28 + 7.77 * 36
 # Whole step (grid 42.6 + KNN 140 + clip 448 + HLL 500 + drift 60) at k = 32.
 PATH_BYTES_PER_CELL = 1190.0
 HBM_PEAK_GBS = 8000.0
+# Single-thread time of the reference compiled from its own sources over the
+# oracle's time for the same work (2D 10k periodic mesh, k = 16, 20 HLL
+# steps; oracle/calibrate.py on an Intel Xeon @ 2.70GHz).
+REF_OVER_ORACLE = 1.426
 
 
 def parse():
@@ -47,7 +51,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--k", type=int, default=32)
-    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--cpu-sample", type=int, default=100_000, help="seeds of the one-thread CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe-bytes", type=int, default=1 << 30,
                     help="size of the streaming-copy probe (measured HBM bandwidth)")
@@ -73,23 +77,67 @@ def clip_traffic(launches):
     return None if per_step is None else round(per_step / launches, 1)
 
 
-def cpu_baseline(n_sample, k):
-    """The oracle (FP64 C++ restatement, OpenMP) on a bounded sample: one mesh
-    build with restarts + one CFL dt + one HLL step, same workload type."""
-    from oracle import oracle as O
-    from proteusgpu_amd.generators import sod_state, uniform_points
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    pts = uniform_points(n_sample, 3, 2024)
-    rho, vel, E = sod_state(pts, 1.4)
+
+def oracle_schedule(O, pts, rho, vel, E, k, gamma, cfl, hll_steps):
+    """The oracle (FP64 C++ restatement, OpenMP) on the 1 rebuild + n HLL
+    steps schedule: (stage seconds, final state, mesh)."""
     t0 = time.perf_counter()
     m = O.Mesh(pts, "reflective", k=k, restart=True)
-    dt = m.dt_cfl(1.4, 0.3, rho, vel, E)
-    m.hydro_step(1.4, dt, rho, vel, E, flux_mode=1)
-    el = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": n_sample / el, "unit": "cells/s", "cores": cores, "kind": "port",
-            "sample": f"oracle: 3D {n_sample} uniform seeds, reflective, k={k}+restarts, 1 rebuild + "
-                      f"1 HLL step ({el:.2f} s)"}
+    t_build = time.perf_counter() - t0
+    st = {f"{key}": v for key, v in m.times().items()}
+    st["build_s"] = t_build
+    r, v, e = rho.copy(), vel.copy(), E.copy()
+    t_dt = t_hll = 0.0
+    dts = []
+    for _ in range(hll_steps):
+        t1 = time.perf_counter()
+        dt = m.dt_cfl(gamma, cfl, r, v, e)
+        t2 = time.perf_counter()
+        m.hydro_step(gamma, dt, r, v, e, flux_mode=1)
+        t3 = time.perf_counter()
+        t_dt += t2 - t1
+        t_hll += t3 - t2
+        dts.append(dt)
+    st["dt_s"] = t_dt / max(hll_steps, 1)
+    st["hll_s"] = t_hll / max(hll_steps, 1)
+    return st, (r, v, e, dts), m
+
+
+def cpu_baseline(n, k, gamma, cfl, pts, rho, vel, E, sample):
+    """The oracle on the host cores: the full workload's rebuild + one HLL
+    step on all cores (one warm-up, median of 3, stage times), and a bounded
+    sample on one core.  cells/s = cells / (rebuild + dt + one HLL step)."""
+    from oracle import oracle as O
+    cores = O.get_threads()
+    oracle_schedule(O, pts[:sample], rho[:sample], vel[:sample], E[:sample], k, gamma, cfl, 1)  # warm-up
+    runs = []
+    for _ in range(3):
+        st, _, _ = oracle_schedule(O, pts, rho, vel, E, k, gamma, cfl, 1)
+        runs.append(st)
+    med = {key: float(np.median([r[key] for r in runs])) for key in runs[0]}
+    step_s = med["build_s"] + med["dt_s"] + med["hll_s"]
+    O.set_threads(1)
+    st1, _, _ = oracle_schedule(O, pts[:sample], rho[:sample], vel[:sample], E[:sample], k, gamma, cfl, 1)
+    O.set_threads(cores)
+    step1 = st1["build_s"] + st1["dt_s"] + st1["hll_s"]
+    return {"value": round(n / step_s, 1), "unit": "cells/s", "cores": cores, "kind": "port",
+            "cpu": cpu_model(),
+            "sample": f"oracle (FP64 restatement of the reference path): the full workload, 3D {n} uniform seeds, "
+                      f"reflective, k={k} + certificate restarts, one rebuild + dt + one HLL step, all {cores} "
+                      f"threads, median of 3 after a warm-up ({step_s:.2f} s); one thread on the first {sample} seeds",
+            "stages_s": {key: round(v, 4) for key, v in med.items()},
+            "one_core": {"cells_per_s": round(sample / step1, 1), "cells": sample,
+                         "stages_s": {key: round(v, 4) for key, v in st1.items()}},
+            "reference_over_oracle_1core": REF_OVER_ORACLE}
 
 
 def phase(name):
@@ -168,6 +216,28 @@ def profile_variant(a, ctx, api, pts, rho, vel, E, kw):
     ctx.close()
 
 
+def clip_valu(launches, avg_launch_ms):
+    """VALU issue share of the first clip pass: its measured VALU
+    instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU, committed in
+    profiles/) x 4 cycles per wave instruction over the SIMD-cycles of one
+    launch (1024 SIMDs x 2.4 GHz x the launch time measured here); None
+    without a summary."""
+    path = os.path.join(ROOT, "profiles", "k_voronoi_1m_valu.json")
+    if not os.path.exists(path) or launches <= 0:
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    per_launch = d["valu_insts_per_step"] / launches
+    simd_cycles = 1024 * 2.4e9 * avg_launch_ms * 1e-3
+    return {"insts_per_launch": round(per_launch, 1), "issue_share": round(4.0 * per_launch / simd_cycles, 4),
+            "insts_per_seed": round(d["valu_insts_per_step"] / d["seeds_per_step"], 1),
+            "source": "profiles/k_voronoi_1m_valu.json"}
+
+
+def stage_dict(st):
+    return {k: round(v, 3) for k, v in st.items() if k.endswith("_ms")}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -185,12 +255,24 @@ def main():
     if a.only:
         return profile_variant(a, ctx, api, pts, rho, vel, E, kw)
 
-    # 1. the plain method: fixed k=32 + certificate restarts, host-driven step
-    phase("fixed k, host-driven step")
-    plain = api.Simulation(ctx, pts, rho, vel, E, **kw)
-    ms_plain, st_plain = run_steps(ctx, plain, a.warmup, a.steps)
-    sched_plain = schedule_ms(ctx, plain)
-    # the same with restarts that clip again from the first neighbour
+    # 1. the plain method: fixed k + certificate restarts (resuming the saved
+    #    cell), host-driven step, at the configured k and at 48 and 64
+    fixed = {}
+    for kk in sorted({a.k, 48, 64}):
+        phase(f"fixed k={kk}, host-driven step")
+        sim = api.Simulation(ctx, pts, rho, vel, E, **dict(kw, k=kk))
+        ms, st = run_steps(ctx, sim, a.warmup, a.steps)
+        fixed[kk] = {"ms_per_step": round(ms, 3), "cells_per_s": round(a.n / (ms * 1e-3), 1),
+                     "stages_ms": stage_dict(st), "restarted_seeds_per_step": st["n_restarted"],
+                     "row_entries_per_step": st["row_entries"]}
+        if kk == a.k:
+            plain = sim
+            fixed[kk]["schedule_1rebuild_10hll_ms"] = round(schedule_ms(ctx, plain), 3)
+        else:
+            sim.free()
+    ms_plain = fixed[a.k]["ms_per_step"]
+    # the same with restarts that clip again from the first neighbour (the
+    # restart of the reference's semantics)
     phase("fixed k, host-driven step, restarts from scratch")
     api.set_clip_resume(False)
     scratch = api.Simulation(ctx, pts, rho, vel, E, **kw)
@@ -200,21 +282,25 @@ def main():
     scratch.free()
 
     # 2. learned k: the classifier is trained on the GPU, in this run, on the
-    #    first mesh of the workload (certified k of every seed as labels)
+    #    first mesh of a different point set (seed 43) of the same workload
+    #    (certified k of every seed as labels), then used on this one
     phase("training the k classifier")
     km = api.KModel(ctx, seed=1)
-    trainer = api.Simulation(ctx, pts, rho, vel, E, **kw)
+    tp = uniform_points(a.n, 3, 43)
+    tr, tv, tE = sod_state(tp, gamma)
+    trainer = api.Simulation(ctx, tp, tr, tv, tE, **kw)
     ctx.sync()
     t0 = time.perf_counter()
     loss = km.train_mesh(trainer, steps=a.train_steps, batch=4096, lr=3e-3)
     ctx.sync()
     train_ms = (time.perf_counter() - t0) * 1e3
-    # learned k in the host-driven step (stage times)
-    phase("learned k, host-driven step")
-    trainer.set_kmodel(km)
-    ms_lsim, st_lsim = run_steps(ctx, trainer, a.warmup, a.steps)
-    info_l = trainer.mesh_info()
     trainer.free()
+    phase("learned k, host-driven step")
+    lsim = api.Simulation(ctx, pts, rho, vel, E, **kw)
+    lsim.set_kmodel(km)
+    ms_lsim, st_lsim = run_steps(ctx, lsim, a.warmup, a.steps)
+    info_l = lsim.mesh_info()
+    lsim.free()
 
     # 3. the batched step: one box, the whole step in one hipGraph launch,
     #    fixed k and learned k
@@ -261,7 +347,7 @@ def main():
         same = same and all(np.array_equal(x, y) for x, y in zip(got, ref))
     chk.free()
 
-    # roofline of the dominant kernel: the first clip pass (k_voronoi<3,16,4>,
+    # roofline of the dominant kernel: the first clip pass (k_voronoi<3,16,S>,
     # one launch per k bucket) of the learned step
     n_seeds = info_l["n_seeds"]
     rows_first = float(sum(int(c) * b for c, b in zip(info_l["n_class"], (16, 24, 32, 48, 64, 96))))
@@ -270,20 +356,49 @@ def main():
     clip_ms = st_lsim["clip_ms"]
     achieved = clip_bytes / (clip_ms * 1e-3) / 1e9
     copy_gbs = ctx.stream_copy_probe(a.probe_bytes, 10)
+    avg_launch = clip_ms / max(launches, 1)
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": clip_traffic(launches),
-            "kernel": "k_voronoi<3,16,4> (first clip pass, one launch per k bucket)",
+            "kernel": "k_voronoi<3,16,S> (first clip pass of the learned step, one launch per k bucket)",
             "launches_per_step": launches, "bytes_per_launch": round(clip_bytes / max(launches, 1), 1),
-            "avg_launch_ms": round(clip_ms / max(launches, 1), 4),
-            "bytes_per_cell": round(clip_bytes / n_seeds, 1),
-            "measured_copy_GBs": round(copy_gbs, 1), "frac_of_measured_copy": round(achieved / copy_gbs, 5)}
+            "avg_launch_ms": round(avg_launch, 4), "bytes_per_cell": round(clip_bytes / n_seeds, 1),
+            "measured_copy_GBs": round(copy_gbs, 1), "frac_of_measured_copy": round(achieved / copy_gbs, 5),
+            "valu": clip_valu(launches, avg_launch)}
     best = ms_blearn
     path_gbs = PATH_BYTES_PER_CELL * a.n / (best * 1e-3) / 1e9
-    cpu = None
+
+    # 4. the FP64 oracle on the host: CPU baseline, and the state error of the
+    #    "1 rebuild + 10 HLL steps" schedule (static mesh after one build from
+    #    the initial state) against the GPU's
+    cpu, err = None, None
     if not a.no_cpu:
-        phase("cpu baseline")
-        cpu = cpu_baseline(a.cpu_sample, a.k)
+        phase("cpu baseline and state error vs the FP64 oracle")
+        from oracle import oracle as O
+        cpu = cpu_baseline(a.n, a.k, gamma, cfl, pts, rho, vel, E, min(a.cpu_sample, a.n))
+        m = api.Mesh(ctx, pts, boundary="reflective", k=a.k, restart=True)
+        hs = api.HydroState(ctx, m, rho, vel, E)
+        mo = O.Mesh(pts, "reflective", k=a.k, restart=True)
+        same_topo = bool(np.array_equal(m.copy_out()["neighbor_cell"], mo.get("neighbor_cell")))
+        dts_g = []
+        r_o, v_o, e_o = rho.copy(), vel.copy(), E.copy()
+        for _ in range(10):
+            dg = hs.dt_cfl(gamma, cfl)
+            do = mo.dt_cfl(gamma, cfl, r_o, v_o, e_o)
+            dts_g.append(abs(dg - do) / do)
+            hs.step(gamma, do, "physical")
+            mo.hydro_step(gamma, do, r_o, v_o, e_o, flux_mode=1)
+        r_g, v_g, e_g = hs.copy_out()
+        err = {"schedule": "one rebuild (fixed k + restarts) then 10 static HLL steps from the initial state, "
+                           "both at the oracle's dt", "topology_bit_exact": same_topo,
+               "rho": float(np.max(np.abs(r_g - r_o) / np.abs(r_o))),
+               "E": float(np.max(np.abs(e_g - e_o) / np.abs(e_o))),
+               "v": float(np.max(np.abs(v_g - v_o)) / np.max(np.abs(v_o))),
+               "dt": float(max(dts_g))}
+        hs.free()
+        m.free()
     tot = blearn.totals()[0]
+    best_fixed_k = min(fixed, key=lambda kk: fixed[kk]["ms_per_step"])
+    ms_best_fixed = fixed[best_fixed_k]["ms_per_step"]
     out = {
         "metric": "cells/s per Voronoi rebuild+HLL step, 1M 3D seeds; learned-k vs fixed-k speedup",
         "value": round(a.n / (best * 1e-3), 1), "unit": "cells/s", "n_gpus": 1, "steps": a.steps,
@@ -292,34 +407,33 @@ def main():
         "config": {"workload": f"3D {a.n} uniform generators, reflective box, Sod along x (gamma 1.4), "
                                f"CFL 0.3, moving mesh, one full Voronoi rebuild per step; batched step (one "
                                f"hipGraph launch) with k per seed from the learned classifier (trained in this "
-                               f"run on the first mesh) and certificate restarts; the plain fixed k={a.k} step "
-                               f"timed beside",
+                               f"run on the first mesh of another point set) and certificate restarts; "
+                               f"fixed-k steps at k = {', '.join(str(x) for x in sorted(fixed))} timed beside",
                    "n_cells": a.n, "k_fixed": a.k},
-        "roofline": roof, "cpu_baseline": cpu,
-        "learned_k_speedup": round(ms_plain / best, 4),
-        "learned_k_speedup_vs_scratch_restarts": round(ms_scratch / best, 4),
+        "roofline": roof, "cpu_baseline": cpu, "state_rel_err_vs_oracle": err,
+        "learned_k": {
+            "host_ms_per_step": round(ms_lsim, 3), "batched_ms_per_step": round(ms_blearn, 3),
+            "speedup_host_vs_fixed_k": round(ms_plain / ms_lsim, 4),
+            "speedup_batched_vs_fixed_k_batched": round(ms_bfix / ms_blearn, 4),
+            "best_fixed_k": best_fixed_k,
+            "learned_vs_best_fixed": round(ms_best_fixed / ms_lsim, 4),
+            "speedup_vs_scratch_restarts_host": round(ms_scratch / ms_lsim, 4),
+            "target_1_5x_vs_best_fixed_met": bool(ms_best_fixed / ms_lsim >= 1.5),
+            "stages_ms": stage_dict(st_lsim), "restarted_seeds_per_step": st_lsim["n_restarted"],
+            "row_entries_per_step": st_lsim["row_entries"],
+            "seeds_per_bucket": [int(c) for c in info_l["n_class"]],
+            "training": {"ms": round(train_ms, 2), "adam_steps": a.train_steps, "batch": 4096,
+                         "final_loss": round(loss, 4), "points": "seed 43 (evaluated on seed 42)"}},
         "results_identical": same,
-        "plain_fixed_k": {"ms_per_step": round(ms_plain, 3), "cells_per_s": round(a.n / (ms_plain * 1e-3), 1),
-                          "stages_ms": {k: round(v, 3) for k, v in st_plain.items() if k.endswith("_ms")},
-                          "restarted_seeds_per_step": st_plain["n_restarted"],
-                          "row_entries_per_step": st_plain["row_entries"],
-                          "schedule_1rebuild_10hll_ms": round(sched_plain, 3)},
-        "plain_fixed_k_scratch_restarts": {
+        "fixed_k_host": {str(kk): v for kk, v in fixed.items()},
+        "fixed_k_scratch_restarts_host": {
             "ms_per_step": round(ms_scratch, 3), "cells_per_s": round(a.n / (ms_scratch * 1e-3), 1),
-            "stages_ms": {k: round(v, 3) for k, v in st_scratch.items() if k.endswith("_ms")},
-            "restarted_seeds_per_step": st_scratch["n_restarted"]},
-        "learned_k_host_step": {"ms_per_step": round(ms_lsim, 3),
-                                "stages_ms": {k: round(v, 3) for k, v in st_lsim.items() if k.endswith("_ms")},
-                                "restarted_seeds_per_step": st_lsim["n_restarted"],
-                                "row_entries_per_step": st_lsim["row_entries"],
-                                "seeds_per_bucket": [int(c) for c in info_l["n_class"]]},
+            "stages_ms": stage_dict(st_scratch), "restarted_seeds_per_step": st_scratch["n_restarted"]},
         "batched_fixed_k": {"ms_per_step": round(ms_bfix, 3), "gpu_ms_per_step": round(gpu_bfix, 3),
                             "tuning": tune_fix},
         "batched_learned_k": {"ms_per_step": round(ms_blearn, 3), "gpu_ms_per_step": round(gpu_blearn, 3),
                               "tuning": tune_learn},
         "batched_boxes_kh256": boxes,
-        "training": {"ms": round(train_ms, 2), "adam_steps": a.train_steps, "batch": 4096,
-                     "final_loss": round(loss, 4)},
         "path_roofline": {"bytes_per_cell": PATH_BYTES_PER_CELL, "achieved_GBs": round(path_gbs, 2),
                           "frac": round(path_gbs / HBM_PEAK_GBS, 5)},
         "conserved_totals": [float(x) for x in tot],

@@ -26,9 +26,10 @@ extern "C" {
 #define PG_ERR_ARG (-1)      /* invalid argument                                  */
 #define PG_ERR_HIP (-2)      /* HIP runtime error (message in pg_last_error)       */
 #define PG_ERR_NOMEM (-3)    /* device or host allocation failed                  */
-#define PG_ERR_GRID (-4)     /* N_grid < 16 (knn.cpp:111-115 exits here)           */
+#define PG_ERR_GRID (-4)     /* N_grid < 16 (knn.cpp:32-36 exits here)             */
 #define PG_ERR_CELLS (-5)    /* cells left uncertified after every restart         */
 #define PG_ERR_NODEV (-6)    /* no usable GPU                                      */
+#define PG_ERR_FAILED (-7)   /* the object failed in an earlier call and is unusable */
 
 /* Per-cell status codes: same numbering as the reference (allvars.h:13-20). */
 #define PG_TRIANGLE_OVERFLOW 0
@@ -83,7 +84,8 @@ int pg_knn_get_permutation(pg_ctx* ctx, const pg_knn* knn, uint32_t* out); /* n 
 /* knn::knn_free (knn.h:54): frees and nulls the handle. */
 int pg_knn_free(pg_knn** knn);
 
-/* Shape of the fast clip kernel for every following build of this process
+/* Shape of the fast clip kernel for every following build of this process,
+ * read once at the start of each build
  * (no reference counterpart; results never depend on it): lanes per seed
  * (8 or 16) and vertex slots per seed (48 or 64; seeds that outgrow them
  * are recomputed by a 128-slot kernel).  Default 16 x 64, or PGPU_CLIP_CFG. */
@@ -152,7 +154,7 @@ int pg_mesh_compute(pg_ctx* ctx, const pg_mesh_params* params, const double* pts
                     pg_mesh** out);
 int pg_mesh_info(const pg_mesh* mesh, pg_mesh_info_t* info);
 
-/* voronoi::compute_cells (voronoi.h:69, voronoi.cpp:67-111): the cells of a
+/* voronoi::compute_cells (voronoi.h:69, voronoi.cpp:68-111): the cells of a
  * solved KNN problem (pg_knn_solve), clipped with its rows of length k and
  * no certificate restarts (boundary NONE, the box [0,1]^dim).  The mesh is
  * indexed by the problem's sorted ids, as compute_cells leaves it before
@@ -168,7 +170,7 @@ int pg_mesh_unpermute(pg_ctx* ctx, pg_mesh* mesh);
 
 /* The reference VMesh (voronoi.h:22-41) after unpermute_vmesh
  * (voronoi.cpp:212-333) and, for periodic meshes, the unscale of
- * periodic_mesh.cpp:170-190.  Caller allocates every array (sizes from
+ * periodic_mesh.cpp:146-166.  Caller allocates every array (sizes from
  * pg_mesh_info); any pointer may be NULL to skip that array. */
 typedef struct {
     int64_t* cell_ids;      /* n_seeds                                         */
@@ -188,7 +190,7 @@ typedef struct {
 
 int pg_mesh_copy_out(pg_ctx* ctx, const pg_mesh* mesh, pg_vmesh_view* out);
 /* The KNN problem of a fixed-k mesh build, as knn::get_points /
- * get_knearest / get_permutation (knn.cpp:437-455) return it for the KNN
+ * get_knearest / get_permutation (knn.cpp:439-455) return it for the KNN
  * HDF5 dump (knn.cpp:457-473, output.cpp:305-411): the build's points
  * (ghosts included, scaled into the unit box for periodic meshes) in sorted
  * order (n_seeds*dim), the first pass's neighbour rows of length k in
@@ -204,7 +206,7 @@ int pg_mesh_free(pg_mesh** mesh);
  * n_hydro cells; E is total energy per volume (allvars.h:203-207). */
 int pg_state_create(pg_ctx* ctx, const pg_mesh* mesh, const double* rho, const double* vel,
                     const double* E, pg_state** out);
-/* hydro::dt_CFL (finite_volume_solver.h:26, .cpp:151-186). */
+/* hydro::dt_CFL (finite_volume_solver.h:27, .cpp:151-186). */
 int pg_dt_cfl(pg_ctx* ctx, const pg_mesh* mesh, const pg_state* st, double gamma, double cfl,
               double* dt);
 /* hydro::hydro_step (finite_volume_solver.h:23, .cpp:43-123): one HLL
@@ -212,7 +214,7 @@ int pg_dt_cfl(pg_ctx* ctx, const pg_mesh* mesh, const pg_state* st, double gamma
 int pg_hydro_step(pg_ctx* ctx, const pg_mesh* mesh, pg_state* st, double gamma, double dt,
                   int flux_mode);
 int pg_state_copy_out(pg_ctx* ctx, const pg_state* st, double* rho, double* vel, double* E);
-/* hydro::riemann_hll (riemann.h:24, riemann.cpp:7-43) for n independent
+/* hydro::riemann_hll (riemann.h:24, riemann.cpp:7-45) for n independent
  * faces: left and right states (rho, v[dim], E; E total energy per volume)
  * and the stored face normal (3 per face, unit (s_i - s_j) as VMesh keeps
  * it) in host arrays; wall (may be NULL): nonzero marks a reflective wall,
@@ -265,7 +267,12 @@ typedef struct {
 
 int pg_sim_create(pg_ctx* ctx, const pg_sim_params* params, const double* pts, const double* rho,
                   const double* vel, const double* E, int64_t n, pg_sim** out);
-/* dt_max > 0 caps the CFL step (main.cpp:54-55 clamps to t_end). */
+/* dt_max > 0 caps the CFL step (main.cpp:54-55 clamps to t_end).  A step
+ * that fails (a rebuild that leaves cells uncertified, a periodic ghost layer
+ * thinner than the cells, a HIP error) returns its error after the state was
+ * already advanced and the generators moved; the simulation is then
+ * unusable: pg_sim_step, pg_sim_hydro_steps, pg_sim_get, pg_sim_totals and
+ * pg_sim_mesh return PG_ERR_FAILED without launching anything.  Free it. */
 int pg_sim_step(pg_ctx* ctx, pg_sim* sim, double dt_max, double* dt_used);
 /* nsteps static-mesh HLL updates on the current mesh (no rebuild). */
 int pg_sim_hydro_steps(pg_ctx* ctx, pg_sim* sim, int nsteps, double dt_max, double* t_advanced);
@@ -360,7 +367,9 @@ typedef struct {
 
 int pg_batch_create(pg_ctx* ctx, const pg_sim_params* params, int nbox, const int64_t* n_per_box,
                     const double* pts, const double* rho, const double* vel, const double* E, pg_batch** out);
-/* One step of every box; ms (optional) = GPU time of the step. */
+/* One step of every box; ms (optional) = GPU time of the step.  A failed
+ * step (or autotune) leaves the batch unusable as a failed pg_sim_step does:
+ * later steps, autotunes, gets, dts and totals return PG_ERR_FAILED. */
 int pg_batch_step(pg_ctx* ctx, pg_batch* batch, double* ms);
 int pg_batch_set_tuning(pg_batch* batch, const pg_batch_tuning* t);
 /* Learned k for the following steps (NULL: the fixed k of the parameters);

@@ -29,8 +29,11 @@
 //                                   divergence identity (no reference output).
 #include "oracle.h"
 
+#include <omp.h>
+
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdint>
@@ -59,7 +62,7 @@ inline double dot3(const P4& a, const P4& b) { return a.x * b.x + a.y * b.y + a.
 inline P4 cross3(const P4& a, const P4& b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x, 0};
 }
-// allvars.h:212-239 — the evaluation order of these determinants is part of
+// allvars.h:168-195 — the evaluation order of these determinants is part of
 // the bit pattern the reference produces.
 inline double d2(double a11, double a12, double a21, double a22) { return a11 * a22 - a12 * a21; }
 inline double d3(double a11, double a12, double a13, double a21, double a22, double a23, double a31,
@@ -87,7 +90,7 @@ inline double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
 const uint8_t END = 255;
 
 // ------------------------------------------------------------------ grid
-// knn.cpp:287-301
+// knn.cpp:208-222
 inline int cell_of(int dim, int N, const double* p) {
     int i = (int)std::floor(p[0] * (double)N);
     int j = (int)std::floor(p[1] * (double)N);
@@ -172,7 +175,7 @@ struct Cell {
         return false;
     }
 
-    // cell.cpp:212-226
+    // cell.cpp:158-172
     int new_halfplane(int vid) {
         if (nb_v >= max_p) {
             status = vertex_overflow;
@@ -188,7 +191,7 @@ struct Cell {
         return nb_v - 1;
     }
 
-    // cell.cpp:229-275
+    // cell.cpp:175-221
     bool conflict(int t, const P4& eqn) {
         const P4& p1 = plane[tp(t, 0)];
         const P4& p2 = plane[tp(t, 1)];
@@ -216,7 +219,7 @@ struct Cell {
         return det > 0.0;
     }
 
-    // cell.cpp:278-397
+    // cell.cpp:224-343
     void compute_boundary() {
         for (int i = 0; i < max_p; i++) bnext[i] = END;
         first_b = END;
@@ -280,7 +283,7 @@ struct Cell {
         }
     }
 
-    // cell.cpp:400-418
+    // cell.cpp:346-364
     void new_vertex(uint8_t i, uint8_t j, uint8_t k) {
         if (nb_t + 1 >= max_t) {
             status = triangle_overflow;
@@ -298,7 +301,7 @@ struct Cell {
         nb_t++;
     }
 
-    // cell.cpp:164-208
+    // cell.cpp:110-154
     void clip(int vid) {
         int cur = new_halfplane(vid);
         if (status == vertex_overflow) return;
@@ -330,7 +333,7 @@ struct Cell {
         } while (cir != first_b);
     }
 
-    // cell.cpp:442-465
+    // cell.cpp:388-411
     P4 vertex(int t, bool divide = true) {
         const P4& a = plane[tp(t, 0)];
         const P4& b = plane[tp(t, 1)];
@@ -352,7 +355,7 @@ struct Cell {
         return r;
     }
 
-    // cell.cpp:423-438
+    // cell.cpp:369-384
     bool radius_reached(const P4& last) {
         double vd = 0;
         for (int i = 0; i < nb_t; i++) {
@@ -365,7 +368,7 @@ struct Cell {
     }
 };
 
-// Extracted geometry of one cell, faces in plane order (cell.cpp:470-524).
+// Extracted geometry of one cell, faces in plane order (cell.cpp:416-470).
 struct CellOut {
     double volume = 0;
     double centroid[3] = {0, 0, 0};
@@ -373,7 +376,7 @@ struct CellOut {
     std::vector<double> normal, area, fcen;
 };
 
-// cell.cpp:529-552 — 2D area from atan2-sorted vertices.
+// cell.cpp:475-498 — 2D area from atan2-sorted vertices.
 double area2d(const std::vector<P4>& v, int nt) {
     double cx = 0, cy = 0;
     for (int i = 0; i < nt; i++) {
@@ -396,7 +399,7 @@ double area2d(const std::vector<P4>& v, int nt) {
     return std::fabs(a) / 2.0;
 }
 
-// cell.cpp:470-685 plus centroids (no reference counterpart: the cell
+// cell.cpp:416-669 plus centroids (no reference counterpart: the cell
 // centroid is the volume-weighted mean of the cone/triangle centroids of the
 // fan, the face centroid the area-weighted mean of the fan triangles).
 void extract(Cell& c, CellOut& out) {
@@ -415,7 +418,7 @@ void extract(Cell& c, CellOut& out) {
         if (dim == 2) {
             for (int i : idx) fv.push_back(vtx[i]);
         } else {
-            // cell.cpp:584-617 — order by adjacency
+            // cell.cpp:515-572 — order by adjacency
             std::vector<int> ord{idx[0]};
             std::vector<bool> used(idx.size(), false);
             used[0] = true;
@@ -519,9 +522,13 @@ void extract(Cell& c, CellOut& out) {
 // ------------------------------------------------------------------ mesh
 struct Mesh {
     int dim = 3, k = 0, N_grid = 0;
+    int boundary = 0;
+    double scale = 1.0;  // unscale factor of the sorted points (periodic: 1 + 2 buff)
     int64_t n_seeds = 0, n_hydro = 0;
     std::vector<double> sorted;
     std::vector<uint32_t> perm, knn;
+    std::vector<int64_t> o2s;  // original id -> sorted id
+    double times[4] = {0, 0, 0, 0};  // seconds: ghosts + grid, KNN, first clip pass, restarts
     // reference VMesh layout, original order
     std::vector<double> seeds, volumes, normal, area, ccen, fcen;
     std::vector<int64_t> fcount, fptr, ghost_ids;
@@ -556,14 +563,19 @@ void grid_sort(int dim, const double* pts, int64_t n, int N, double* sorted, uin
 }
 
 // knn.cpp:237-347: ring search, heap keyed by (dist^2, original index).
-void knn_search(int dim, const double* sorted, int64_t n, int N, const int32_t* cnt,
-                const int32_t* start, const uint32_t* perm, int k, uint32_t* out) {
+// Rows of the queries qlist[0..nq) (every point when qlist is null), row r
+// written at out + r * k.
+void knn_search_list(int dim, const double* sorted, int64_t n, int N, const int32_t* cnt,
+                     const int32_t* start, const uint32_t* perm, int k, const int64_t* qlist, int64_t nq,
+                     uint32_t* out) {
+    (void)n;  // the grid tables bound every access
 #pragma omp parallel
     {
         std::vector<double> hd(k);
         std::vector<uint32_t> hi(k), ho(k);
 #pragma omp for schedule(dynamic, 256)
-        for (int64_t q = 0; q < n; q++) {
+        for (int64_t r = 0; r < nq; r++) {
+            const int64_t q = qlist ? qlist[r] : r;
             const double* p = sorted + dim * q;
             int c = cell_of(dim, N, p);
             int ci = c % N, cj = (c / N) % N, ck = (dim == 3) ? c / (N * N) : 0;
@@ -633,9 +645,14 @@ void knn_search(int dim, const double* sorted, int64_t n, int N, const int32_t* 
             for (int i = 0; i < size; i++) ord[i] = i;
             std::sort(ord.begin(), ord.end(),
                       [&](int a, int b) { return less(hd[a], ho[a], hd[b], ho[b]); });
-            for (int i = 0; i < k; i++) out[q * k + i] = i < size ? hi[ord[i]] : UINT32_MAX;
+            for (int i = 0; i < k; i++) out[r * k + i] = i < size ? hi[ord[i]] : UINT32_MAX;
         }
     }
+}
+
+void knn_search(int dim, const double* sorted, int64_t n, int N, const int32_t* cnt,
+                const int32_t* start, const uint32_t* perm, int k, uint32_t* out) {
+    knn_search_list(dim, sorted, n, N, cnt, start, perm, k, nullptr, n, out);
 }
 
 int ngrid(int dim, int64_t n) {
@@ -708,9 +725,12 @@ int run_cell(int dim, int maxp, int maxt, const double* sorted, const uint32_t* 
     return kc;
 }
 
+double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
 Mesh* build(int dim, const double* in_pts, int64_t n, int boundary, double buff, int k, int maxp,
             int maxt, int restart) {
     Mesh* m = new Mesh();
+    double t0 = now();
     m->dim = dim;
     std::vector<double> pts;
     int64_t ntot = n;
@@ -743,24 +763,58 @@ Mesh* build(int dim, const double* in_pts, int64_t n, int boundary, double buff,
         if (kmax > ntot - 1) kmax = (int)(ntot - 1);
         if (kmax < k) kmax = k;
     }
-    m->k = kmax;
-    m->knn.resize((size_t)ntot * kmax);
-    knn_search(dim, m->sorted.data(), ntot, N, cnt.data(), start.data(), m->perm.data(), kmax,
-               m->knn.data());
+    m->k = k;
+    m->knn.resize((size_t)ntot * k);
+    double t1 = now();
+    knn_search(dim, m->sorted.data(), ntot, N, cnt.data(), start.data(), m->perm.data(), k, m->knn.data());
+    double t2 = now();
     std::vector<CellOut> cells(ntot);
     std::vector<int> st(ntot), kc(ntot);
+    // First pass: every seed with its row of length k (voronoi.cpp:115-156).
 #pragma omp parallel for schedule(dynamic, 64)
-    for (int64_t s = 0; s < ntot; s++) {
-        // Without restart the neighbour list is cut at k (reference semantics).
-        // With restart, certification at a prefix of the longer list gives the
-        // same cell as a restart with that longer list.
-        int kk = restart ? kmax : k;
-        std::vector<uint32_t> row(m->knn.begin() + s * kmax, m->knn.begin() + s * kmax + kk);
-        kc[s] = run_cell(dim, maxp, maxt, m->sorted.data(), row.data(), kk, (int)s, &cells[s], &st[s],
-                         kk >= ntot - 1);
+    for (int64_t s = 0; s < ntot; s++)
+        kc[s] = run_cell(dim, maxp, maxt, m->sorted.data(), m->knn.data() + (size_t)s * k, k, (int)s, &cells[s],
+                         &st[s], k >= ntot - 1);
+    double t3 = now();
+    // Restarts: a seed the first pass leaves uncertified is clipped again from
+    // the first neighbour with a row twice as long, up to kmax.  Rows are
+    // ordered by (dist^2, original id), so a longer row extends the shorter
+    // one and the restarted cell is the cell a fixed k of that length gives.
+    if (restart) {
+        std::vector<int64_t> todo;
+        for (int64_t s = 0; s < ntot; s++)
+            if (st[s] != success) todo.push_back(s);
+        std::vector<uint32_t> rows;
+        int kk = k;
+        while (!todo.empty() && kk < kmax) {
+            kk = std::min(kmax, 2 * kk);
+            const int64_t nq = (int64_t)todo.size();
+            rows.resize((size_t)nq * kk);
+            knn_search_list(dim, m->sorted.data(), ntot, N, cnt.data(), start.data(), m->perm.data(), kk,
+                            todo.data(), nq, rows.data());
+#pragma omp parallel for schedule(dynamic, 16)
+            for (int64_t r = 0; r < nq; r++) {
+                const int64_t s = todo[r];
+                cells[s] = CellOut();
+                kc[s] = run_cell(dim, maxp, maxt, m->sorted.data(), rows.data() + (size_t)r * kk, kk, (int)s,
+                                 &cells[s], &st[s], kk >= ntot - 1);
+            }
+            std::vector<int64_t> left;
+            for (int64_t s : todo)
+                if (st[s] != success) left.push_back(s);
+            todo.swap(left);
+        }
     }
-    // unpermute (voronoi.cpp:212-333) and, for periodic, unscale (periodic_mesh.cpp:170-190)
-    std::vector<int64_t> o2s(ntot);
+    double t4 = now();
+    m->times[0] = t1 - t0;
+    m->times[1] = t2 - t1;
+    m->times[2] = t3 - t2;
+    m->times[3] = t4 - t3;
+    // unpermute (voronoi.cpp:212-333) and, for periodic, unscale (periodic_mesh.cpp:146-166)
+    m->boundary = boundary;
+    m->scale = boundary == 0 ? 1. + (2 * buff) : 1.0;
+    std::vector<int64_t>& o2s = m->o2s;
+    o2s.resize(ntot);
     for (int64_t s = 0; s < ntot; s++) o2s[m->perm[s]] = s;
     m->seeds.assign(ntot * 3, 0.0);
     m->volumes.assign(ntot, 0.0);
@@ -820,7 +874,7 @@ struct Geom {
     double n[3], m[3], p[3];
 };
 
-// riemann.cpp:108-130
+// riemann.cpp:75-97
 Geom geom_of(const double* nr) {
     Geom g;
     double nn = std::sqrt(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
@@ -846,7 +900,7 @@ Geom geom_of(const double* nr) {
     return g;
 }
 
-// riemann.cpp:80-106
+// riemann.cpp:47-73
 void to_face(int dim, Prim& s, const Geom& g) {
     double vx = s.v[0], vy = s.v[1];
     if (dim == 2) {
@@ -871,12 +925,12 @@ void from_face(int dim, Prim& s, const Geom& g) {
         s.v[2] = vx * g.n[2] + vy * g.m[2] + vz * g.p[2];
     }
 }
-// riemann.cpp:151-157
+// riemann.cpp:118-124
 double pressure(int dim, double gam, const Prim& s) {
     if (dim == 2) return (gam - 1) * (s.E - (0.5 * s.rho * (s.v[0] * s.v[0] + s.v[1] * s.v[1])));
     return (gam - 1) * (s.E - (0.5 * s.rho * (s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2])));
 }
-// riemann.cpp:132-149
+// riemann.cpp:99-116
 Prim euler_flux(int dim, double gam, const Prim& s) {
     double P = pressure(dim, gam, s);
     Prim f;
@@ -917,6 +971,280 @@ Prim hll(int dim, double gam, Prim L, Prim R, const double* nrm, bool wall) {
     }
     from_face(dim, F, g);
     return F;
+}
+
+// ------------------------------------------------------------------ moving mesh
+// No reference counterpart: the reference's mesh is built once and never
+// moves (main.cpp:41-63).  Restated from the definitions of the moving-mesh
+// step (Springel 2010, MNRAS 401, 791, secs. 3.3 and 4.2).  Pinned by: the
+// static limit (move = 0 gives the face sums of the golden-pinned static
+// step bit for bit), Galilean invariance of the face-frame flux, a
+// face-velocity known answer (a rotating pair of generators), uniform flow
+// staying uniform on a moving mesh, and conservation.
+
+inline double unscale(double x, double scale) { return (x - 0.5) * scale + 0.5; }
+
+// Velocity of the face between generators r_i and r_j (velocities w_i, w_j)
+// whose centroid is fc: the mean generator velocity plus the velocity the
+// bisector plane gets at fc from the pair's relative motion (its rotation
+// about the midpoint of the pair).
+void face_velocity(int dim, const double* ri, const double* rj, const double* wi, const double* wj,
+                   const double* fc, double* w) {
+    double dr[3] = {0, 0, 0}, dd = 0.0, proj = 0.0;
+    for (int d = 0; d < dim; d++) {
+        dr[d] = rj[d] - ri[d];
+        const double mid = 0.5 * (ri[d] + rj[d]);
+        dd += dr[d] * dr[d];
+        proj += (wi[d] - wj[d]) * (fc[d] - mid);
+    }
+    for (int d = 0; d < 3; d++) w[d] = d < dim ? 0.5 * (wi[d] + wj[d]) + proj * dr[d] / dd : 0.0;
+}
+
+// Hydro index of face neighbour nb (-1: box plane, ghosts: their source).
+inline int64_t hydro_of(const Mesh& m, int64_t nb) {
+    if (nb < 0) return -1;
+    return nb < m.n_hydro ? nb : m.ghost_ids[nb - m.n_hydro];
+}
+
+// Position of original id o in box coordinates, from the build's (scaled)
+// sorted points.
+inline double gen_pos(const Mesh& m, int64_t o, int d) {
+    return unscale(m.sorted[(size_t)m.dim * m.o2s[o] + d], m.scale);
+}
+
+// Sum over the faces of hydro cell i of the area-integrated fluxes (mass,
+// momentum[3], energy: out[5]), finite_volume_solver.cpp:53-106 for the face
+// loop.  move = 0: the static flux of or_hydro_step, summed in the same
+// order.  move = 1: every face moves with face_velocity() of its generators'
+// velocities (wgen, or the fluid velocity when wgen is null; walls stay at
+// rest); both states are boosted into the face frame, where the total energy
+// per volume changes by rho (|v - w|^2 - |v|^2) / 2, the HLL problem is
+// solved there and its flux transformed back: mass unchanged, momentum
+// + F_m w, energy + F_p.w + |w|^2 F_m / 2.  Returns the number of box faces
+// met on a periodic mesh (a ghost layer thinner than the cell).
+int cell_flux(const Mesh& m, int64_t i, double gam, int flux_mode, int move, const double* rho, const double* vel,
+              const double* E, const double* wgen, double* out) {
+    const int dim = m.dim;
+    auto gvel = [&](int64_t h, int d) { return wgen ? wgen[dim * h + d] : vel[dim * h + d]; };
+    Prim si;
+    si.rho = rho[i];
+    si.E = E[i];
+    for (int d = 0; d < dim; d++) si.v[d] = vel[dim * i + d];
+    double ri[3] = {0, 0, 0}, wi[3] = {0, 0, 0};
+    for (int d = 0; d < dim; d++) {
+        ri[d] = gen_pos(m, i, d);
+        wi[d] = gvel(i, d);
+    }
+    double tot[5] = {0, 0, 0, 0, 0};
+    int bad = 0;
+    for (int64_t q = 0; q < m.fcount[i]; q++) {
+        const int64_t f = m.fptr[i] + q;
+        const int64_t nb = m.nbr[f];
+        const int64_t j = hydro_of(m, nb);
+        const bool wall = j < 0;
+        if (wall && m.boundary == 0) bad++;
+        Prim sj = si;
+        double w[3] = {0, 0, 0};
+        if (!wall) {
+            sj.rho = rho[j];
+            sj.E = E[j];
+            for (int d = 0; d < dim; d++) sj.v[d] = vel[dim * j + d];
+            if (move) {
+                double rj[3] = {0, 0, 0}, wj[3] = {0, 0, 0}, fc[3] = {0, 0, 0};
+                for (int d = 0; d < dim; d++) {
+                    rj[d] = gen_pos(m, nb, d);
+                    wj[d] = gvel(j, d);
+                    // face centroids: periodic meshes store them unscaled already
+                    fc[d] = m.boundary == 0 ? m.fcen[3 * f + d] : unscale(m.fcen[3 * f + d], 1.0);
+                }
+                face_velocity(dim, ri, rj, wi, wj, fc, w);
+            }
+        }
+        Prim L = si, R = sj;
+        if (move && !wall) {
+            double kl = 0, kr = 0, kl2 = 0, kr2 = 0;
+            for (int d = 0; d < dim; d++) {
+                kl += L.v[d] * L.v[d];
+                kr += R.v[d] * R.v[d];
+                L.v[d] -= w[d];
+                R.v[d] -= w[d];
+                kl2 += L.v[d] * L.v[d];
+                kr2 += R.v[d] * R.v[d];
+            }
+            L.E += 0.5 * L.rho * (kl2 - kl);
+            R.E += 0.5 * R.rho * (kr2 - kr);
+        }
+        double n3[3] = {m.normal[3 * f], m.normal[3 * f + 1], m.normal[3 * f + 2]};
+        if (flux_mode == 1) {
+            n3[0] = -n3[0];
+            n3[1] = -n3[1];
+            n3[2] = -n3[2];
+        }
+        const Prim F = hll(dim, gam, L, R, n3, wall);
+        double Fm = F.rho, Fp[3] = {F.v[0], F.v[1], F.v[2]}, Fe = F.E;
+        if (move && !wall) {
+            double wp = 0.0, ww = 0.0;
+            for (int d = 0; d < dim; d++) {
+                wp += Fp[d] * w[d];
+                ww += w[d] * w[d];
+            }
+            Fe += wp + 0.5 * ww * Fm;
+            for (int d = 0; d < dim; d++) Fp[d] += Fm * w[d];
+        }
+        const double a = m.area[f];
+        tot[0] += Fm * a;
+        for (int d = 0; d < dim; d++) tot[1 + d] += Fp[d] * a;
+        tot[4] += Fe * a;
+    }
+    for (int q = 0; q < 5; q++) out[q] = tot[q];
+    return bad;
+}
+
+// Steered generator velocity of hydro cell i (regularised motion, Springel
+// 2010 eq. 63 with eta = 0.25, chi = 1): w = v + chi f (c - r) / |c - r|
+// with c the cell centroid, r the generator, R the radius of the cell's
+// volume, f = 0 below 0.9 eta R, the sound speed above 1.1 eta R, linear in
+// between.
+void steered_velocity(const Mesh& m, int64_t i, double gam, const double* x, const double* rho, const double* vel,
+                      const double* E, double* w) {
+    const int dim = m.dim;
+    const double V = m.volumes[i];
+    const double R = dim == 2 ? std::sqrt(V / M_PI) : std::cbrt(3.0 * V / (4.0 * M_PI));
+    Prim st;
+    st.rho = rho[i];
+    st.E = E[i];
+    for (int d = 0; d < dim; d++) st.v[d] = vel[dim * i + d];
+    const double P = pressure(dim, gam, st);
+    const double cs = (P > 0.0 && st.rho > 0.0) ? std::sqrt(gam * P / st.rho) : 0.0;
+    double dc[3] = {0, 0, 0}, dist = 0.0;
+    for (int d = 0; d < dim; d++) {
+        const double c = m.boundary == 0 ? m.ccen[3 * i + d] : unscale(m.ccen[3 * i + d], 1.0);
+        dc[d] = c - x[dim * i + d];
+        dist += dc[d] * dc[d];
+    }
+    dist = std::sqrt(dist);
+    const double eta = 0.25, chi = 1.0;
+    const double lo = 0.9 * eta * R, hi = 1.1 * eta * R;
+    double f = 0.0;
+    if (dist > hi)
+        f = cs;
+    else if (dist > lo)
+        f = cs * (dist - lo) / (hi - lo);
+    for (int d = 0; d < dim; d++) w[dim * i + d] = st.v[d] + (dist > 0.0 ? chi * f * dc[d] / dist : 0.0);
+}
+
+// CFL step of hydro cell i (finite_volume_solver.cpp:155-183): cfl R / (c +
+// |v|).  With steered generators (wgen) the signal speed is the larger of
+// |v| and |v - w|, and R is capped by the distance to the nearest face
+// neighbour, which bounds the width of slivers.
+double cell_dt(const Mesh& m, int64_t i, double gam, double cfl, const double* rho, const double* vel,
+               const double* E, const double* wgen) {
+    const int dim = m.dim;
+    Prim s;
+    s.rho = rho[i];
+    s.E = E[i];
+    for (int d = 0; d < dim; d++) s.v[d] = vel[dim * i + d];
+    const double P = pressure(dim, gam, s);
+    const double c = std::sqrt(gam * P / s.rho);
+    const double V = m.volumes[i];
+    double R = dim == 2 ? std::sqrt(V / M_PI) : std::cbrt(3.0 * V / (4.0 * M_PI));
+    double va = std::sqrt(s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]);
+    if (wgen) {
+        const int64_t si = m.o2s[i];
+        double dmin = 1e300;
+        for (int64_t q = 0; q < m.fcount[i]; q++) {
+            const int64_t nb = m.nbr[m.fptr[i] + q];
+            if (nb < 0) continue;
+            const int64_t sj = m.o2s[nb];
+            double dd = 0.0;
+            for (int d = 0; d < dim; d++) {
+                const double t = m.sorted[(size_t)dim * sj + d] - m.sorted[(size_t)dim * si + d];
+                dd += t * t;
+            }
+            dmin = std::min(dmin, dd);
+        }
+        R = std::min(R, std::sqrt(dmin) * m.scale);
+        double rs = 0.0;
+        for (int d = 0; d < dim; d++) rs += (s.v[d] - wgen[dim * i + d]) * (s.v[d] - wgen[dim * i + d]);
+        va = std::max(va, std::sqrt(rs));
+    }
+    return cfl * (R / (c + va));
+}
+
+// A moving-mesh simulation: generators x, extensive FP64 state (mass,
+// momentum, total energy) per hydro cell, and the current mesh.
+struct Sim {
+    int dim = 3, boundary = 1, k = 32, flux_mode = 1, move = 1;
+    double buff = 0.04, gam = 1.4, cfl = 0.3;
+    int64_t n = 0;
+    std::vector<double> x, mass, mom, en, rho, vel, E, wgen;
+    Mesh* mesh = nullptr;
+    ~Sim() { delete mesh; }
+    bool rebuild() {
+        delete mesh;
+        mesh = build(dim, x.data(), n, boundary, buff, k, 64, 96, 1);
+        return mesh->n_failed == 0;
+    }
+    // primitives from the conserved state and the current volumes
+    void prim() {
+        for (int64_t i = 0; i < n; i++) {
+            const double V = mesh->volumes[i];
+            rho[i] = mass[i] / V;
+            for (int d = 0; d < dim; d++) vel[dim * i + d] = mom[dim * i + d] / mass[i];
+            E[i] = en[i] / V;
+        }
+    }
+};
+
+// One step: primitives, (steered generator velocities), CFL dt (capped at
+// dt_max > 0; a negative step from a bad state gives 0), the flux update of
+// the conserved state, the drift x += w dt (wrapped into the periodic box,
+// mirrored at reflective walls) and a full rebuild.  Returns the step, or
+// -1 when the rebuild leaves cells uncertified.
+double sim_step(Sim& S, double dt_max) {
+    const int dim = S.dim;
+    const Mesh& m = *S.mesh;
+    S.prim();
+    const bool moving = S.move != 0;
+    const double* w = nullptr;
+    if (moving && S.move == 2) {
+        S.wgen.assign((size_t)S.n * dim, 0.0);
+        for (int64_t i = 0; i < S.n; i++)
+            steered_velocity(m, i, S.gam, S.x.data(), S.rho.data(), S.vel.data(), S.E.data(), S.wgen.data());
+        w = S.wgen.data();
+    }
+    double dt = 1e100;
+    for (int64_t i = 0; i < S.n; i++)
+        dt = std::min(dt, cell_dt(m, i, S.gam, S.cfl, S.rho.data(), S.vel.data(), S.E.data(), w));
+    if (dt < 0.0) dt = 0.0;
+    if (dt_max > 0 && dt > dt_max) dt = dt_max;
+    std::vector<double> fl((size_t)S.n * 5);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < S.n; i++)
+        cell_flux(m, i, S.gam, S.flux_mode, moving ? 1 : 0, S.rho.data(), S.vel.data(), S.E.data(), w,
+                  fl.data() + 5 * i);
+    for (int64_t i = 0; i < S.n; i++) {
+        S.mass[i] -= dt * fl[5 * i];
+        for (int d = 0; d < dim; d++) S.mom[dim * i + d] -= dt * fl[5 * i + 1 + d];
+        S.en[i] -= dt * fl[5 * i + 4];
+    }
+    if (moving) {
+        const double* v = w ? w : S.vel.data();
+        for (int64_t i = 0; i < S.n; i++)
+            for (int d = 0; d < dim; d++) {
+                double p = S.x[dim * i + d] + v[dim * i + d] * dt;
+                if (S.boundary == 0) {
+                    p -= std::floor(p);
+                    if (p >= 1.0) p = 0.0;
+                } else {
+                    if (p < 0.0) p = -p;
+                    if (p >= 1.0) p = 2.0 - p;
+                    p = std::min(std::max(p, 0.0), 0.9999999999999999);
+                }
+                S.x[dim * i + d] = p;
+            }
+    }
+    return S.rebuild() ? dt : -1.0;
 }
 
 // ------------------------------------------------------------------ learned k
@@ -1056,6 +1384,14 @@ void* or_mesh_build(int dim, const double* pts, int64_t n, int boundary, double 
 
 void or_mesh_free(void* h) { delete (Mesh*)h; }
 
+void or_mesh_times(void* h, double* t4) {
+    for (int i = 0; i < 4; i++) t4[i] = ((Mesh*)h)->times[i];
+}
+
+void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
+int or_get_threads(void) { return omp_get_max_threads(); }
+
 void or_mesh_sizes(void* h, int64_t* s) {
     Mesh* m = (Mesh*)h;
     s[0] = m->n_seeds;
@@ -1181,5 +1517,79 @@ void or_hll(int dim, double gam, const double* UL, const double* UR, const doubl
     for (int d = 0; d < 3; d++) out[1 + d] = F.v[d];
     out[4] = F.E;
 }
+
+void or_face_velocity(int dim, const double* ri, const double* rj, const double* wi, const double* wj,
+                      const double* fc, double* w) {
+    face_velocity(dim, ri, rj, wi, wj, fc, w);
+}
+
+int or_cell_fluxes(void* h, double gam, int flux_mode, int move, const double* rho, const double* vel,
+                   const double* E, const double* wgen, double* out) {
+    const Mesh* m = (const Mesh*)h;
+    int bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(dynamic, 256)
+    for (int64_t i = 0; i < m->n_hydro; i++) bad += cell_flux(*m, i, gam, flux_mode, move, rho, vel, E, wgen, out + 5 * i);
+    return bad;
+}
+
+void* or_sim_create(int dim, const double* pts, const double* rho, const double* vel, const double* E, int64_t n,
+                    int boundary, double buff, int k, double gamma, double cfl, int flux_mode, int move) {
+    if ((dim != 2 && dim != 3) || n < 2 || k < 1 || k > n - 1 || (boundary != 0 && boundary != 1) ||
+        move < 0 || move > 2 || (flux_mode != 0 && flux_mode != 1))
+        return nullptr;
+    Sim* S = new Sim();
+    S->dim = dim;
+    S->boundary = boundary;
+    S->k = k;
+    S->flux_mode = flux_mode;
+    S->move = move;
+    S->buff = buff;
+    S->gam = gamma;
+    S->cfl = cfl;
+    S->n = n;
+    S->x.assign(pts, pts + n * dim);
+    S->rho.assign(rho, rho + n);
+    S->vel.assign(vel, vel + n * dim);
+    S->E.assign(E, E + n);
+    S->mass.resize(n);
+    S->mom.resize(n * dim);
+    S->en.resize(n);
+    if (!S->rebuild()) {
+        delete S;
+        return nullptr;
+    }
+    for (int64_t i = 0; i < n; i++) {
+        const double V = S->mesh->volumes[i];
+        S->mass[i] = rho[i] * V;
+        for (int d = 0; d < dim; d++) S->mom[dim * i + d] = rho[i] * vel[dim * i + d] * V;
+        S->en[i] = E[i] * V;
+    }
+    return S;
+}
+
+double or_sim_step(void* h, double dt_max) { return sim_step(*(Sim*)h, dt_max); }
+
+void or_sim_get(void* h, double* pts, double* rho, double* vel, double* E) {
+    Sim* S = (Sim*)h;
+    S->prim();
+    if (pts) std::memcpy(pts, S->x.data(), sizeof(double) * S->x.size());
+    if (rho) std::memcpy(rho, S->rho.data(), sizeof(double) * S->n);
+    if (vel) std::memcpy(vel, S->vel.data(), sizeof(double) * S->vel.size());
+    if (E) std::memcpy(E, S->E.data(), sizeof(double) * S->n);
+}
+
+void or_sim_totals(void* h, double* tot) {
+    const Sim* S = (const Sim*)h;
+    for (int q = 0; q < 5; q++) tot[q] = 0.0;
+    for (int64_t i = 0; i < S->n; i++) {
+        tot[0] += S->mass[i];
+        for (int d = 0; d < S->dim; d++) tot[1 + d] += S->mom[S->dim * i + d];
+        tot[4] += S->en[i];
+    }
+}
+
+void* or_sim_mesh(void* h) { return ((Sim*)h)->mesh; }
+
+void or_sim_free(void* h) { delete (Sim*)h; }
 
 }  // extern "C"

@@ -18,7 +18,7 @@
 extern "C" {
 #endif
 
-/* knn.cpp:101 — grid resolution, float arithmetic as in the reference. */
+/* knn.cpp:22 — grid resolution, float arithmetic as in the reference. */
 int or_ngrid(int dim, int64_t n);
 
 /* periodic_mesh.cpp:63-126 — ghost points (point-major, region order of the
@@ -50,6 +50,12 @@ int or_kfeatures(int dim, const double* sorted, int64_t n, int N_grid, const int
 void* or_mesh_build(int dim, const double* pts, int64_t n, int boundary, double buff, int k,
                     int max_p, int max_t, int restart);
 void or_mesh_free(void* h);
+/* Build stage times in seconds: [0] ghosts + grid sort, [1] KNN rows, [2]
+ * first clip pass, [3] certificate restarts. */
+void or_mesh_times(void* h, double* t4);
+/* OpenMP threads of the following calls (for 1-core / all-core timings). */
+void or_set_threads(int n);
+int or_get_threads(void);
 
 /* sizes: [0]=n_seeds [1]=n_hydro [2]=num_faces [3]=n_failed [4]=N_grid [5]=k_used */
 void or_mesh_sizes(void* h, int64_t* sizes);
@@ -81,6 +87,34 @@ int or_hydro_step(void* h, double gamma, double dt, int flux_mode, double* rho, 
  * wall != 0: the right state is the mirror of the left one. */
 void or_hll(int dim, double gamma, const double* UL, const double* UR, const double* n, int wall,
             double* out);
+
+/* Moving mesh (no reference counterpart: the reference never moves its mesh,
+ * main.cpp:41-63).  Velocity w[3] of the face between generators ri, rj moving
+ * with wi, wj, at face centroid fc. */
+void or_face_velocity(int dim, const double* ri, const double* rj, const double* wi, const double* wj,
+                      const double* fc, double* w);
+/* Area-integrated fluxes (mass, momentum[3], energy) summed over the faces of
+ * every hydro cell, out[n_hydro*5].  move = 0: static faces (the sums of
+ * or_hydro_step); move = 1: faces moving with the generators (wgen, or the
+ * fluid velocity when wgen is NULL), flux solved in the face frame.  Returns
+ * the number of box faces met on a periodic mesh. */
+int or_cell_fluxes(void* mesh, double gamma, int flux_mode, int move, const double* rho, const double* vel,
+                   const double* E, const double* wgen, double* out);
+/* Moving-mesh simulation: conserved FP64 state, one step = primitives,
+ * (steered velocities), CFL dt, flux update, drift, rebuild (k with
+ * certificate restarts, max_p 64, max_t 96).  boundary 0 periodic, 1
+ * reflective; move 0 static, 1 with the fluid, 2 regularised.  NULL when the
+ * arguments are invalid or the first build leaves cells uncertified. */
+void* or_sim_create(int dim, const double* pts, const double* rho, const double* vel, const double* E, int64_t n,
+                    int boundary, double buff, int k, double gamma, double cfl, int flux_mode, int move);
+/* Returns the step taken (capped at dt_max > 0), -1 when the rebuild leaves
+ * cells uncertified. */
+double or_sim_step(void* sim, double dt_max);
+void or_sim_get(void* sim, double* pts, double* rho, double* vel, double* E);
+void or_sim_totals(void* sim, double* tot5);
+/* The current mesh (owned by the simulation). */
+void* or_sim_mesh(void* sim);
+void or_sim_free(void* sim);
 
 #ifdef __cplusplus
 }

@@ -43,6 +43,28 @@ def lib():
         L.or_hydro_step.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, dp, dp, dp]
         L.or_hll.argtypes = [C.c_int, C.c_double, dp, dp, dp, C.c_int, dp]
         L.or_hll.restype = None
+        L.or_mesh_times.argtypes = [C.c_void_p, dp]
+        L.or_mesh_times.restype = None
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_threads.restype = None
+        L.or_get_threads.argtypes = []
+        L.or_get_threads.restype = C.c_int
+        L.or_face_velocity.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp]
+        L.or_face_velocity.restype = None
+        L.or_cell_fluxes.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, dp, dp, dp, C.c_void_p, dp]
+        L.or_sim_create.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int64, C.c_int, C.c_double, C.c_int,
+                                    C.c_double, C.c_double, C.c_int, C.c_int]
+        L.or_sim_create.restype = C.c_void_p
+        L.or_sim_step.argtypes = [C.c_void_p, C.c_double]
+        L.or_sim_step.restype = C.c_double
+        L.or_sim_get.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.or_sim_get.restype = None
+        L.or_sim_totals.argtypes = [C.c_void_p, dp]
+        L.or_sim_totals.restype = None
+        L.or_sim_mesh.argtypes = [C.c_void_p]
+        L.or_sim_mesh.restype = C.c_void_p
+        L.or_sim_free.argtypes = [C.c_void_p]
+        L.or_sim_free.restype = None
         _LIB = L
     return _LIB
 
@@ -130,6 +152,24 @@ def displace(pts, amp, retry=0, periodic=False):
     return x.reshape(n, dim)
 
 
+def set_threads(n):
+    """OpenMP threads of the following oracle calls."""
+    lib().or_set_threads(int(n))
+
+
+def get_threads():
+    return lib().or_get_threads()
+
+
+def face_velocity(ri, rj, wi, wj, fc):
+    """Velocity of the face between generators ri, rj moving with wi, wj, at face centroid fc."""
+    dim = len(ri)
+    out = np.zeros(3)
+    f = [np.ascontiguousarray(a, dtype=np.float64) for a in (ri, rj, wi, wj, fc)]
+    lib().or_face_velocity(dim, *f, out)
+    return out[:dim]
+
+
 def hll(dim, gamma, UL, UR, n, wall=False):
     """HLL flux (riemann.cpp:7-45) of states (rho, vx, vy, vz, E) across normal n."""
     out = np.zeros(5)
@@ -147,22 +187,34 @@ class Mesh:
 
     BOUNDARY = {"periodic": 0, "reflective": 1, "none": 2}
 
-    def __init__(self, pts, boundary="periodic", buff=0.04, k=32, max_p=64, max_t=96, restart=False):
-        pts = np.ascontiguousarray(pts, dtype=np.float64)
-        n, dim = pts.shape
-        self.dim = dim
-        self.h = lib().or_mesh_build(dim, pts, n, self.BOUNDARY[boundary], buff, k, max_p, max_t,
-                                     int(restart))
-        if not self.h:
-            raise ValueError("oracle mesh build rejected its arguments")
+    def __init__(self, pts, boundary="periodic", buff=0.04, k=32, max_p=64, max_t=96, restart=False,
+                 _borrow=None):
+        self._owner = None
+        if _borrow is not None:
+            # a mesh owned by an oracle Sim (kept alive through _owner)
+            self._owner, self.h, self.dim = _borrow
+        else:
+            pts = np.ascontiguousarray(pts, dtype=np.float64)
+            n, dim = pts.shape
+            self.dim = dim
+            self.h = lib().or_mesh_build(dim, pts, n, self.BOUNDARY[boundary], buff, k, max_p, max_t,
+                                         int(restart))
+            if not self.h:
+                raise ValueError("oracle mesh build rejected its arguments")
         s = np.zeros(6, dtype=np.int64)
         lib().or_mesh_sizes(self.h, s)
         self.n_seeds, self.n_hydro, self.num_faces, self.n_failed, self.N_grid, self.k = map(int, s)
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self, "_owner", None) is None:
             lib().or_mesh_free(self.h)
-            self.h = None
+        self.h = None
+
+    def times(self):
+        """Build stage seconds: ghosts + grid sort, KNN, first clip pass, restarts."""
+        t = np.zeros(4)
+        lib().or_mesh_times(self.h, t)
+        return {"grid_s": t[0], "knn_s": t[1], "clip_s": t[2], "restart_s": t[3]}
 
     def get(self, name):
         n, F, d = self.n_seeds, self.num_faces, self.dim
@@ -185,3 +237,63 @@ class Mesh:
     def hydro_step(self, gamma, dt, rho, vel, E, flux_mode=0):
         """In-place static-mesh HLL step; returns the number of faceless hydro cells."""
         return lib().or_hydro_step(self.h, gamma, dt, flux_mode, rho, vel, E)
+
+    def cell_fluxes(self, gamma, rho, vel, E, flux_mode=1, move=0, wgen=None):
+        """Area-integrated face-flux sums of every hydro cell, (n_hydro, 5):
+        mass, momentum[3], energy; move=1: faces moving with wgen (or the
+        fluid velocity), flux solved in the face frame."""
+        out = np.zeros(self.n_hydro * 5)
+        w = None if wgen is None else np.ascontiguousarray(wgen, dtype=np.float64)
+        bad = lib().or_cell_fluxes(self.h, gamma, flux_mode, move, np.ascontiguousarray(rho, dtype=np.float64),
+                                   np.ascontiguousarray(vel, dtype=np.float64),
+                                   np.ascontiguousarray(E, dtype=np.float64), None if w is None else w.ctypes.data,
+                                   out)
+        if bad:
+            raise ValueError(f"{bad} box faces on a periodic mesh: the ghost layer is thinner than the cells")
+        return out.reshape(-1, 5)
+
+
+class Sim:
+    """Oracle moving-mesh simulation (FP64, extensive conserved state)."""
+
+    MOVE = {"static": 0, "fluid": 1, "regularized": 2}
+
+    def __init__(self, pts, rho, vel, E, boundary="reflective", buff=0.04, k=32, gamma=1.4, cfl=0.3,
+                 flux="physical", move="fluid"):
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        self.n, self.dim = pts.shape
+        self.h = lib().or_sim_create(self.dim, pts, np.ascontiguousarray(rho, dtype=np.float64),
+                                     np.ascontiguousarray(vel, dtype=np.float64),
+                                     np.ascontiguousarray(E, dtype=np.float64), self.n,
+                                     Mesh.BOUNDARY[boundary], buff, k, gamma, cfl,
+                                     1 if flux == "physical" else 0, self.MOVE[move])
+        if not self.h:
+            raise ValueError("oracle simulation rejected its arguments or its first mesh")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_sim_free(self.h)
+            self.h = None
+
+    def step(self, dt_max=0.0):
+        dt = lib().or_sim_step(self.h, dt_max)
+        if dt < 0:
+            raise RuntimeError("oracle rebuild left cells uncertified")
+        return dt
+
+    def get(self):
+        pts = np.empty((self.n, self.dim))
+        rho = np.empty(self.n)
+        vel = np.empty((self.n, self.dim))
+        E = np.empty(self.n)
+        lib().or_sim_get(self.h, pts, rho, vel, E)
+        return pts, rho, vel, E
+
+    def totals(self):
+        t = np.zeros(5)
+        lib().or_sim_totals(self.h, t)
+        return t
+
+    def mesh(self):
+        """The current mesh (valid until the next step)."""
+        return Mesh(None, _borrow=(self, lib().or_sim_mesh(self.h), self.dim))

@@ -12,10 +12,18 @@
 //   voronoi::compute_periodic_mesh             src/voronoi/periodic_mesh.h:22
 //   hydro::init / dt_CFL / hydro_step          src/hydro/finite_volume_solver.h:20-27
 //
-// Input file (little endian): int32 dim, int64 n, f64 pts[n*dim],
-//   then (mode "periodic" only) f64 rho[n], f64 vel[n*dim], f64 E[n].
-// Output: <outdir>/<name>.bin raw arrays plus <outdir>/index.txt lines
+//   InputHandler::loadParameters / readICFile  src/io/input.h:30-54
+//   OutputHandler::writeMeshFile / writeKNNFile src/io/output.h:74-77
+//   voronoi::vmesh_to_meshdata                 src/voronoi/voronoi.h:81
+//
+// Modes knn | mesh | periodic: input file (little endian) int32 dim, int64 n,
+//   f64 pts[n*dim], then (periodic only) f64 rho[n], f64 vel[n*dim], f64 E[n].
+//   Output: <outdir>/<name>.bin raw arrays plus <outdir>/index.txt lines
 //   "<name> <dtype> <count>".
+// Mode files <param.txt> <gamma> <cfl> <t_end> <buff>: the reference's own
+//   main loop (main.cpp:38-84) with its IC reader and file writers: reads
+//   ic_file, builds the periodic mesh (with WRITE_KNN_OUTPUT the build writes
+//   output_knn_file), steps to t_end and writes output_mesh_file.
 #include <cstdio>
 #include <cstdlib>
 #include <cstdint>
@@ -79,9 +87,45 @@ void dump_mesh(const VMesh* m) {
     }
 }
 
+#ifdef USE_HDF5
+int run_files(int argc, char** argv) {
+    if (argc < 7) {
+        std::fprintf(stderr, "usage: %s files <param.txt> <gamma> <cfl> <t_end> <buff>\n", argv[0]);
+        return 2;
+    }
+    input = InputHandler(argv[2]);
+    if (!input.loadParameters()) return 2;
+    if (!input.readICFile(input.getParameter("ic_file"), icData)) return 3;
+    _gamma_ = std::atof(argv[3]);
+    const double cfl = std::atof(argv[4]), t_end = std::atof(argv[5]);
+    buff = std::atof(argv[6]);
+    output = OutputHandler(input.getParameter("output_directory"));
+    if (!output.initialize()) return 2;
+    const int n = (int)icData.seedpos_dims[0];
+    primvars* pv = hydro::init(n);
+    VMesh* mesh = voronoi::compute_periodic_mesh((POINT_TYPE*)icData.seedpos.data(), (hsize_t)n);
+    double t = 0.0;
+    while (t < t_end) {
+        double dt = hydro::dt_CFL(cfl, mesh, pv);
+        if (t + dt > t_end) dt = t_end - t;
+        hydro::hydro_step(dt, mesh, pv);
+        t += dt;
+    }
+    MeshCellData md;
+    voronoi::vmesh_to_meshdata(mesh, md);
+    const bool ok = output.writeMeshFile(input.getParameter("output_mesh_file"), md, pv, n);
+    voronoi::free_vmesh(mesh);
+    hydro::free_prim(&pv);
+    return ok ? 0 : 3;
+}
+#endif
+
 }  // namespace
 
 int main(int argc, char** argv) {
+#ifdef USE_HDF5
+    if (argc > 1 && std::string(argv[1]) == "files") return run_files(argc, argv);
+#endif
     if (argc < 4) {
         std::fprintf(stderr, "usage: %s knn|mesh|periodic <input.bin> <outdir> [gamma cfl nsteps [buff]]\n", argv[0]);
         return 2;

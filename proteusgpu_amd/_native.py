@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libpgpu.so")
 
 PG_OK = 0
 ERRORS = {-1: "PG_ERR_ARG", -2: "PG_ERR_HIP", -3: "PG_ERR_NOMEM", -4: "PG_ERR_GRID",
-          -5: "PG_ERR_CELLS", -6: "PG_ERR_NODEV"}
+          -5: "PG_ERR_CELLS", -6: "PG_ERR_NODEV", -7: "PG_ERR_FAILED"}
 
 BOUNDARY = {"periodic": 0, "reflective": 1, "none": 2}
 FLUX = {"reference": 0, "physical": 1}
@@ -77,17 +77,11 @@ KMODEL_BUCKETS = (16, 24, 32, 48, 64, 96)
 _lib = None
 
 
-def lib():
-    """Load libpgpu.so once; raise if it is not built."""
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError(f"libpgpu.so is not built ({LIB_PATH}); run `make -C proteusgpu_amd/csrc`")
-    L = C.CDLL(LIB_PATH)
+def _signatures():
+    """ctypes argument and result types of every C ABI entry point."""
     vp, i64, i32, dbl = C.c_void_p, C.c_int64, C.c_int, C.c_double
     pp = C.POINTER(C.c_void_p)
-    sig = {
+    return {
         "pg_version": ([], C.c_char_p), "pg_last_error": ([], C.c_char_p),
         "pg_device_count": ([C.POINTER(C.c_int)], i32),
         "pg_set_clip_config": ([i32, i32], i32),
@@ -143,7 +137,17 @@ def lib():
         "pg_batch_info": ([vp, vp, C.POINTER(BatchInfo)], i32),
         "pg_batch_free": ([pp], i32),
     }
-    for name, (args, res) in sig.items():
+
+
+def lib():
+    """Load libpgpu.so once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libpgpu.so is not built ({LIB_PATH}); run `make -C proteusgpu_amd/csrc`")
+    L = C.CDLL(LIB_PATH)
+    for name, (args, res) in _signatures().items():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = res
@@ -158,16 +162,4 @@ def check(rc):
 
 def exported_symbols():
     """Names of the C ABI entry points this binding knows (for the export test)."""
-    return ["pg_version", "pg_last_error", "pg_device_count", "pg_set_clip_config", "pg_get_clip_config", "pg_set_clip_resume", "pg_get_clip_resume", "pg_mesh_get_knn", "pg_ctx_create", "pg_ctx_destroy",
-            "pg_hll_flux", "pg_voronoi_compute", "pg_mesh_unpermute",
-            "pg_ctx_sync", "pg_stream_copy_probe", "pg_knn_init", "pg_knn_solve", "pg_knn_info", "pg_knn_get_points",
-            "pg_knn_get_knearest", "pg_knn_get_permutation", "pg_knn_free", "pg_mesh_params_default",
-            "pg_mesh_compute", "pg_mesh_info", "pg_mesh_copy_out", "pg_mesh_free", "pg_state_create",
-            "pg_dt_cfl", "pg_hydro_step", "pg_state_copy_out", "pg_state_free", "pg_sim_create",
-            "pg_sim_step", "pg_sim_hydro_steps", "pg_sim_times_last", "pg_sim_mesh", "pg_sim_get",
-            "pg_sim_totals", "pg_sim_free", "pg_sim_set_kmodel", "pg_kmodel_create", "pg_kmodel_set_params",
-            "pg_kmodel_get_params", "pg_kmodel_set_cost", "pg_kmodel_predict", "pg_kmodel_grad",
-            "pg_kmodel_train_step", "pg_kmodel_train_mesh", "pg_kmodel_free", "pg_mesh_features",
-            "pg_mesh_compute_k", "pg_batch_create", "pg_batch_step", "pg_batch_set_tuning", "pg_batch_set_kmodel",
-            "pg_batch_get_tuning", "pg_batch_autotune", "pg_batch_get", "pg_batch_dt", "pg_batch_totals",
-            "pg_batch_info", "pg_batch_free"]
+    return sorted(_signatures())

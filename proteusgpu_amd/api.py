@@ -18,6 +18,18 @@ def _ptr(a):
     return C.c_void_p(a.ctypes.data) if a is not None else None
 
 
+def _state_arrays(rho, vel, E, n, dim):
+    """rho (n,), vel (n, dim), E (n,) as contiguous float64; ValueError on a
+    shape mismatch (the C side reads exactly n cells from each array)."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    if rho.shape != (n,) or E.shape != (n,) or vel.shape != (n, dim):
+        raise ValueError(f"state shapes rho {rho.shape}, vel {vel.shape}, E {E.shape} do not match "
+                         f"n = {n} cells in {dim}D: expected ({n},), ({n}, {dim}), ({n},)")
+    return rho, vel, E
+
+
 def set_clip_config(lanes: int = 16, slots: int = 64):
     """Shape of the fast clip kernel for the following builds (timing only)."""
     check(lib().pg_set_clip_config(lanes, slots))
@@ -297,9 +309,14 @@ def hll_flux(ctx: Context, gamma, left, right, normal, wall=None, flux="referenc
     left = np.ascontiguousarray(left, dtype=np.float64)
     right = np.ascontiguousarray(right, dtype=np.float64)
     normal = np.ascontiguousarray(normal, dtype=np.float64)
+    if left.ndim != 2 or left.shape[1] not in (4, 5):
+        raise ValueError(f"left states must be (n, dim + 2) with dim 2 or 3, got {left.shape}")
     n, w = left.shape
-    assert right.shape == (n, w) and normal.shape == (n, 3)
+    if right.shape != (n, w) or normal.shape != (n, 3):
+        raise ValueError(f"right {right.shape} and normal {normal.shape} must be ({n}, {w}) and ({n}, 3)")
     wl = None if wall is None else np.ascontiguousarray(wall, dtype=np.uint8)
+    if wl is not None and wl.shape != (n,):
+        raise ValueError(f"wall flags must be ({n},), got {wl.shape}")
     out = np.empty((n, w))
     check(lib().pg_hll_flux(ctx.h, w - 2, gamma, n, _ptr(left), _ptr(right), _ptr(normal), _ptr(wl),
                             FLUX[flux], _ptr(out)))
@@ -311,10 +328,8 @@ class HydroState:
 
     def __init__(self, ctx: Context, mesh: Mesh, rho, vel, E):
         self.ctx, self.mesh = ctx, mesh
-        rho = np.ascontiguousarray(rho, dtype=np.float64)
-        vel = np.ascontiguousarray(vel, dtype=np.float64)
-        E = np.ascontiguousarray(E, dtype=np.float64)
-        self.n, self.dim = rho.shape[0], mesh.dim
+        self.n, self.dim = mesh.n_hydro, mesh.dim
+        rho, vel, E = _state_arrays(rho, vel, E, self.n, self.dim)
         h = C.c_void_p()
         check(lib().pg_state_create(ctx.h, mesh.h, _ptr(rho), _ptr(vel), _ptr(E), C.byref(h)))
         self.h = h
@@ -371,7 +386,7 @@ class Simulation:
         p = SimParams()
         p.mesh = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max, perturb)
         p.gamma, p.cfl, p.flux_mode, p.move_mesh = gamma, cfl, FLUX[flux], move_mode(move_mesh)
-        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, vel, E)]
+        arrs = _state_arrays(rho, vel, E, self.n, self.dim)
         h = C.c_void_p()
         check(lib().pg_sim_create(ctx.h, C.byref(p), _ptr(pts), _ptr(arrs[0]), _ptr(arrs[1]),
                                   _ptr(arrs[2]), self.n, C.byref(h)))
@@ -461,11 +476,12 @@ class Batch:
         self.ctx, self.dim = ctx, pts.shape[1]
         self.sizes = np.ascontiguousarray(sizes, dtype=np.int64)
         self.n = int(self.sizes.sum())
-        assert pts.shape[0] == self.n
+        if pts.ndim != 2 or pts.shape[0] != self.n:
+            raise ValueError(f"points {pts.shape} do not match the {self.n} cells of the boxes")
         p = SimParams()
         p.mesh = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max)
         p.gamma, p.cfl, p.flux_mode, p.move_mesh = gamma, cfl, FLUX[flux], move_mode(move_mesh)
-        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, vel, E)]
+        arrs = _state_arrays(rho, vel, E, self.n, self.dim)
         h = C.c_void_p()
         check(lib().pg_batch_create(ctx.h, C.byref(p), len(self.sizes), _ptr(self.sizes), _ptr(pts),
                                     _ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]), C.byref(h)))

@@ -146,6 +146,7 @@ struct pg_mesh {
     DBuf nbr, area, nrm, cen, count, vol, ccen, status, kcert;
     DBuf cstate, resume, lastnb;  // saved cells of uncertified seeds (restarts resume them)
     bool use_resume = false;
+    int lanes = 16, slots = 64;  // fast clip kernel shape of the current build
     bool learned_rows = false;  // first pass with per-seed row lengths (knn rows not kept)
     // cells of a KNN problem (pg_voronoi_compute): outputs stay in the
     // problem's sorted order (iperm = identity), while g.perm keeps the
@@ -227,11 +228,11 @@ void run_list(pg_mesh* m, const uint32_t* list, int64_t nq, int k, int big, hipS
     uint32_t* rows = m->rows.get<uint32_t>((size_t)nq * k);
     pg::KnnTail tail;  // seeds that resume need only the rest of their row
     if (m->use_resume && !big)
-        tail = {m->resume.as<int32_t>(), m->lastnb.as<uint32_t>(), clip_shape_tag(pg::clip_lanes(), pg::clip_slots())};
+        tail = {m->resume.as<int32_t>(), m->lastnb.as<uint32_t>(), clip_shape_tag(m->lanes, m->slots)};
     pg::launch_knn(m->prm.dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), m->n_seeds, m->g.N,
                    m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), list, nq, k, rows, s, tail);
     pg::launch_voronoi(m->prm.dim, m->g.sorted.as<double>(), m->n_seeds, list, nq, rows, k,
-                       m->prm.max_p, m->prm.max_t, big, m->faces(), s, true);
+                       m->prm.max_p, m->prm.max_t, big, m->faces(), m->lanes, m->slots, s, true);
     HIPCHK(hipGetLastError());
     m->info.row_entries += nq * k;
 }
@@ -367,7 +368,10 @@ void mesh_build_once(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool o
 
     const size_t ns = (size_t)nseeds;
     alloc_cells(m, ns);
+    // the clip shape and resume switch are read once per build
     m->use_resume = p.restart && pg::clip_resume();
+    m->lanes = pg::clip_lanes();
+    m->slots = pg::clip_slots();
     if (m->use_resume) {
         m->cstate.get<uint8_t>(ns * pg::CLIP_STATE_BYTES);
         m->lastnb.get<uint32_t>(ns);
@@ -389,7 +393,7 @@ void mesh_build_once(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool o
                        m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), nullptr, nseeds, k, rows, s);
         if (mk) mk->mark(MK_KNN, s);
         pg::launch_voronoi(dim, m->g.sorted.as<double>(), nseeds, nullptr, nseeds, rows, k, p.max_p, p.max_t,
-                           0, m->faces(), s);
+                           0, m->faces(), m->lanes, m->slots, s);
         HIPCHK(hipGetLastError());
         m->info.row_entries += nseeds * k;
         if (mk) mk->mark(MK_CLIP, s);
@@ -444,7 +448,7 @@ void mesh_build_once(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool o
         for (int c = 0; c < pg::LK_NCLASS; c++)
             if (hc[c] > 0) {
                 pg::launch_voronoi(dim, m->g.sorted.as<double>(), nseeds, kl + (size_t)c * ns, hc[c], rows + roff[c],
-                                   bk[c], p.max_p, p.max_t, 0, m->faces(), s);
+                                   bk[c], p.max_p, p.max_t, 0, m->faces(), m->lanes, m->slots, s);
                 m->info.row_entries += hc[c] * bk[c];
             }
         HIPCHK(hipGetLastError());
@@ -843,9 +847,12 @@ int pg_voronoi_compute(pg_ctx* c, const pg_knn* kc, int max_p, int max_t, pg_mes
             HIPCHK(hipMemcpyAsync(rows, kp->knn.p, sizeof(uint32_t) * nn * k, hipMemcpyDeviceToDevice, s));
             alloc_cells(m, nn);
             m->use_resume = false;
+            m->lanes = pg::clip_lanes();
+            m->slots = pg::clip_slots();
             m->finalized = false;
             m->info = pg_mesh_info_t{};
-            pg::launch_voronoi(dim, g.sorted.as<double>(), n, nullptr, n, rows, k, max_p, max_t, 0, m->faces(), s);
+            pg::launch_voronoi(dim, g.sorted.as<double>(), n, nullptr, n, rows, k, max_p, max_t, 0, m->faces(),
+                               m->lanes, m->slots, s);
             HIPCHK(hipGetLastError());
             m->info.row_entries = n * k;
             // seeds that outgrew the fast kernel's slots: the 128-slot kernel
@@ -1097,6 +1104,36 @@ int pg_state_free(pg_state** st) {
 
 }  // extern "C"
 
+// ============================================== objects that fail mid-step
+// A simulation or batch step that throws leaves the state advanced and the
+// mesh tables possibly from two different builds: the object records the
+// error and refuses every later call that would launch on it.
+namespace {
+
+template <class T>
+void usable(const T* o, const char* what) {
+    argchk(o != nullptr, (std::string("null ") + what).c_str());
+    if (!o->failed.empty())
+        throw Error{PG_ERR_FAILED, std::string("the ") + what + " failed in an earlier step (" + o->failed +
+                                       ") and is unusable"};
+}
+
+// Run one advancing call; any error marks the object failed.
+template <class T, class F>
+void fail_guard(T* o, F&& f) {
+    try {
+        f();
+    } catch (const Error& e) {
+        o->failed = e.msg;
+        throw;
+    } catch (...) {
+        o->failed = "unknown error";
+        throw;
+    }
+}
+
+}  // namespace
+
 // ================================================================ simulation
 struct pg_sim {
     pg_sim_params prm{};
@@ -1106,6 +1143,9 @@ struct pg_sim {
     DBuf x, mass, mom, en, rho, vel, E, dt, part, wgen;
     DBuf kc_prev;  // certified k of the last rebuild, by point id
     bool have_prev = false;
+    // set when a step throws: the state was already advanced and the mesh
+    // tables may belong to two different builds, so nothing may run on it
+    std::string failed;
     Marks mk;
     pg_sim_times last{};
     ~pg_sim() {
@@ -1232,7 +1272,8 @@ int pg_sim_create(pg_ctx* c, const pg_sim_params* prm, const double* pts, const 
 int pg_sim_step(pg_ctx* c, pg_sim* S, double dt_max, double* dt_used) {
     return guarded([&] {
         use(c);
-        argchk(S != nullptr, "null sim");
+        usable(S, "simulation");
+        fail_guard(S, [&] {
         hipStream_t s = c->stream;
         Marks& mk = S->mk;
         mk.mark(MK_START, s);
@@ -1267,19 +1308,23 @@ int pg_sim_step(pg_ctx* c, pg_sim* S, double dt_max, double* dt_used) {
         t.k_used = S->mesh.info.k_used;
         double dt = sim_read_dt(c, S, dt_max);
         if (dt_used) *dt_used = dt;
+        });
     });
 }
 
 int pg_sim_hydro_steps(pg_ctx* c, pg_sim* S, int nsteps, double dt_max, double* t_adv) {
     return guarded([&] {
         use(c);
-        argchk(S != nullptr && nsteps >= 0, "bad argument");
-        double t = 0;
-        for (int i = 0; i < nsteps; i++) {
-            sim_hydro(c, S, dt_max, false);
-            t += sim_read_dt(c, S, dt_max);
-        }
-        if (t_adv) *t_adv = t;
+        usable(S, "simulation");
+        argchk(nsteps >= 0, "bad argument");
+        fail_guard(S, [&] {
+            double t = 0;
+            for (int i = 0; i < nsteps; i++) {
+                sim_hydro(c, S, dt_max, false);
+                t += sim_read_dt(c, S, dt_max);
+            }
+            if (t_adv) *t_adv = t;
+        });
     });
 }
 
@@ -1292,7 +1337,8 @@ int pg_sim_times_last(const pg_sim* S, pg_sim_times* t) {
 
 int pg_sim_mesh(const pg_sim* S, const pg_mesh** m) {
     return guarded([&] {
-        argchk(S && m, "null argument");
+        usable(S, "simulation");
+        argchk(m != nullptr, "null argument");
         *m = &S->mesh;
     });
 }
@@ -1300,7 +1346,7 @@ int pg_sim_mesh(const pg_sim* S, const pg_mesh** m) {
 int pg_sim_get(pg_ctx* c, const pg_sim* Sc, double* pts, double* rho, double* vel, double* E) {
     return guarded([&] {
         use(c);
-        argchk(Sc != nullptr, "null sim");
+        usable(Sc, "simulation");
         pg_sim* S = const_cast<pg_sim*>(Sc);
         hipStream_t s = c->stream;
         const int dim = S->prm.mesh.dim;
@@ -1318,7 +1364,8 @@ int pg_sim_get(pg_ctx* c, const pg_sim* Sc, double* pts, double* rho, double* ve
 int pg_sim_totals(pg_ctx* c, pg_sim* S, double* tot) {
     return guarded([&] {
         use(c);
-        argchk(S && tot, "null argument");
+        usable(S, "simulation");
+        argchk(tot != nullptr, "null argument");
         const int dim = S->prm.mesh.dim;
         const int64_t nb = pg::totals_blocks(S->n);
         double* part = S->part.get<double>((size_t)5 * nb);
@@ -1560,6 +1607,7 @@ int pg_mesh_features(pg_ctx* c, const pg_mesh* mc, float* out) {
 // ================================================================== batches
 struct pg_batch {
     pg_sim_params prm{};
+    std::string failed;  // error of a failed step (the batch is then unusable)
     int nbox = 0, dim = 3;
     bool periodic = true;
     int64_t n_h = 0, cap = 0, cell_cap = 0;
@@ -1982,8 +2030,8 @@ int pg_batch_create(pg_ctx* c, const pg_sim_params* prm, int nbox, const int64_t
 int pg_batch_step(pg_ctx* c, pg_batch* B, double* ms) {
     return guarded([&] {
         use(c);
-        argchk(B != nullptr, "null batch");
-        batch_step(c, B);
+        usable(B, "batch");
+        fail_guard(B, [&] { batch_step(c, B); });
         if (ms) *ms = B->last_ms;
     });
 }
@@ -2032,7 +2080,9 @@ int pg_batch_get_tuning(const pg_batch* B, pg_batch_tuning* t) {
 int pg_batch_autotune(pg_ctx* c, pg_batch* B, int reps, pg_batch_tuning* best, double* best_ms) {
     return guarded([&] {
         use(c);
-        argchk(B != nullptr && reps >= 1 && reps <= 16, "bad argument");
+        usable(B, "batch");
+        argchk(reps >= 1 && reps <= 16, "bad argument");
+        fail_guard(B, [&] {
         hipStream_t s = c->stream;
         batch_copy_state(B, true, s);
         const pg_batch_tuning start{B->lanes, B->slots, B->wave_min_k, B->bpc, B->use_graph ? 1 : 0};
@@ -2065,13 +2115,14 @@ int pg_batch_autotune(pg_ctx* c, pg_batch* B, int reps, pg_batch_tuning* best, d
         batch_check(c, B);
         if (best) *best = winner;
         if (best_ms) *best_ms = wms;
+        });
     });
 }
 
 int pg_batch_get(pg_ctx* c, pg_batch* B, double* pts, double* rho, double* vel, double* E) {
     return guarded([&] {
         use(c);
-        argchk(B != nullptr, "null batch");
+        usable(B, "batch");
         hipStream_t s = c->stream;
         const size_t n = (size_t)B->n_h, d = (size_t)B->dim;
         pg::launch_prim_from_q(B->dim, B->view(), B->o2s.as<int32_t>(), B->vol_u.as<double>(), s);
@@ -2087,7 +2138,8 @@ int pg_batch_get(pg_ctx* c, pg_batch* B, double* pts, double* rho, double* vel, 
 int pg_batch_dt(pg_ctx* c, const pg_batch* B, double* dt) {
     return guarded([&] {
         use(c);
-        argchk(B && dt, "null argument");
+        usable(B, "batch");
+        argchk(dt != nullptr, "null argument");
         HIPCHK(hipMemcpyAsync(dt, B->dt_box.p, sizeof(double) * B->nbox, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     });
@@ -2096,7 +2148,8 @@ int pg_batch_dt(pg_ctx* c, const pg_batch* B, double* dt) {
 int pg_batch_totals(pg_ctx* c, pg_batch* B, double* tot) {
     return guarded([&] {
         use(c);
-        argchk(B && tot, "null argument");
+        usable(B, "batch");
+        argchk(tot != nullptr, "null argument");
         pg::launch_btotals(B->dim, B->nbox, B->hoff.as<int64_t>(), B->view(), B->tot.as<double>(), c->stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(tot, B->tot.p, sizeof(double) * 5 * B->nbox, hipMemcpyDeviceToHost, c->stream));

@@ -29,7 +29,7 @@ enum : int {
     SLOT_OVERFLOW = 16,
 };
 
-// allvars.h:212-239.  Operand order and association are the reference's.
+// allvars.h:168-195.  Operand order and association are the reference's.
 PG_DEV double det2(double a11, double a12, double a21, double a22) { return a11 * a22 - a12 * a21; }
 PG_DEV double det3(double a11, double a12, double a13, double a21, double a22, double a23,
                    double a31, double a32, double a33) {
@@ -55,7 +55,7 @@ PG_DEV double det4(double a11, double a12, double a13, double a14, double a21, d
 PG_DEV double smin(double a, double b) { return (b < a) ? b : a; }
 PG_DEV double smax(double a, double b) { return (a < b) ? b : a; }
 
-// knn.cpp:287-301
+// knn.cpp:208-222
 template <int DIM>
 PG_DEV int grid_cell(const double* p, int N) {
     int i = (int)floor(p[0] * (double)N);

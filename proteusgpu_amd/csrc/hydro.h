@@ -35,7 +35,7 @@ void launch_hll_ale(int dim, int64_t n, int64_t n_hydro, int64_t stride, int64_t
                     const double* area_u, const double* cen, const double* sorted, double scale, double gam,
                     const double* dt_dev, double dt_max, int flux_mode, int move, SimView sv, hipStream_t s,
                     const double* wgen = nullptr);
-// HLL fluxes of n independent faces (riemann.cpp:7-43): states (rho,
+// HLL fluxes of n independent faces (riemann.cpp:7-45): states (rho,
 // v[dim], E) per face, stored normals (3 per face), optional wall flags.
 void launch_hll_faces(int dim, int64_t n, double gam, const double* L, const double* R, const double* nrm,
                       const uint8_t* wall, int flux_mode, double* F, hipStream_t s);

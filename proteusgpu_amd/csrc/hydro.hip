@@ -1,7 +1,7 @@
 // Hydro on the Voronoi mesh: mesh finalisation, the HLL update
-// (finite_volume_solver.cpp:43-148, riemann.cpp:7-157), the CFL reduce
+// (finite_volume_solver.cpp:43-148, riemann.cpp:7-124), the CFL reduce
 // (finite_volume_solver.cpp:151-186) and the reference-layout copy-out of
-// the mesh (voronoi.cpp:212-333 unpermute, periodic_mesh.cpp:170-190 unscale).
+// the mesh (voronoi.cpp:212-333 unpermute, periodic_mesh.cpp:146-166 unscale).
 //
 // Face data come from the clip kernel in face-slot-major padded storage
 // (entry (f, sorted cell s) at f*stride + s), so threads of a wave that handle
@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(TB)
     E2[i] = si.E - frac * tot.E;
 }
 
-// riemann.cpp:7-43 for n independent faces: left / right states (rho,
+// riemann.cpp:7-45 for n independent faces: left / right states (rho,
 // v[DIM], E per face), the stored normal of each face (3 per face), optional
 // wall flags (the right state is then the mirrored left one), fluxes out
 // (mass, momentum[DIM], energy per face).  sgn = -1 takes the outward
@@ -357,7 +357,7 @@ __global__ void __launch_bounds__(TB)
 }
 
 // Copy-out: per-cell arrays in original order (seeds unscaled as the
-// reference, periodic_mesh.cpp:179-186; failed cells keep zero seeds).
+// reference, periodic_mesh.cpp:155-160; failed cells keep zero seeds).
 __global__ void __launch_bounds__(TB)
     k_cells_orig(int64_t n, int64_t stride, int dim, const uint32_t* perm, const double* sorted,
                  const int32_t* status, const int32_t* kcert, const double* vol, const double* ccen,

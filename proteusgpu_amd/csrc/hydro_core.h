@@ -18,7 +18,7 @@ PG_DEV double pressure(double gam, const Prim& s) {
            (s.E - (0.5 * s.rho * (s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2])));
 }
 
-// riemann.cpp:108-130
+// riemann.cpp:75-97
 struct Geom {
     double n[3], m[3], p[3];
 };
@@ -119,7 +119,7 @@ PG_DEV Prim hll(double gam, Prim L, Prim R, const Geom& g, bool wall) {
 }
 
 // CFL length of sorted cell s with volume V: the radius of the volume
-// (finite_volume_solver.cpp:146-150); with regularised mesh motion also at
+// (finite_volume_solver.cpp:170-175); with regularised mesh motion also at
 // most the distance to the nearest face neighbour, so that slivers (two
 // generators close together: small width, large radius) limit the step as
 // well.  Round cells have a nearest neighbour beyond R (2D hexagons 1.9 R,

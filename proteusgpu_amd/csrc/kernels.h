@@ -91,12 +91,15 @@ constexpr int64_t CLIP_STATE_BYTES = 3328;
 // = all n) with their neighbour rows knn[row q] (row q belongs to seeds[q]).
 // big != 0 selects the 128-slot kernel; restart launches the fast kernel
 // under the name k_voronoi_restart (profiles tell the passes apart).
+// lanes x slots: the fast kernel's shape (valid_clip_config), taken once
+// per build by the caller so that every launch and the KNN tail tag of a
+// build agree.
 void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq,
-                    const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa,
+                    const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa, int lanes, int slots,
                     hipStream_t s, bool restart = false);
 
-// Shape of the fast clip kernel: lanes per seed (8 or 16) and vertex slots
-// per seed (48 or 64).
+// Process-wide default shape of the fast clip kernel for the following
+// builds: lanes per seed (8 or 16) and vertex slots per seed (48 or 64).
 bool valid_clip_config(int lanes, int slots);
 void set_clip_config(int lanes, int slots);
 int clip_lanes();

@@ -127,7 +127,7 @@ PG_DEV void knn_heap_query(const double* __restrict__ sorted, const uint32_t* __
             }
         }
     }
-    // heapsort to ascending order (knn.cpp:420-426)
+    // heapsort to ascending order (knn.cpp:341-347)
     for (int end = size - 1; end > 0; end--) {
         double d = HD(end);
         uint32_t id = HI(end);

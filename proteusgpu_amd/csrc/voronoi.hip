@@ -1,4 +1,5 @@
 // Kernel entry points for the Voronoi clip and the restart bookkeeping.
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 
@@ -75,7 +76,7 @@ __global__ void __launch_bounds__(TB) k_compact(int64_t nq, const uint32_t* seed
 // per wave: the group-wide work of a clip is shared by twice as many seeds,
 // at twice the LDS per wave).  Chosen by the autotuner or PGPU_CLIP_CFG
 // ("16x4", "16x3", "8x6", "8x8").
-static int g_clip_lanes = 16, g_clip_slots = 64;
+static std::atomic<int> g_clip_lanes{16}, g_clip_slots{64};
 static const bool g_clip_env = [] {
     const char* e = getenv("PGPU_CLIP_CFG");
     int l = 0, sl = 0;
@@ -97,10 +98,10 @@ int clip_slots() { return g_clip_slots; }
 
 // Certificate restarts resume the saved cell (default) or clip again from
 // the first neighbour (PGPU_CLIP_RESUME=0); the results are the same.
-static bool g_clip_resume = [] {
+static std::atomic<bool> g_clip_resume{[] {
     const char* e = getenv("PGPU_CLIP_RESUME");
     return !(e && atoi(e) == 0);
-}();
+}()};
 bool clip_resume() { return g_clip_resume; }
 void set_clip_resume(bool on) { g_clip_resume = on; }
 
@@ -115,12 +116,12 @@ void launch_dim(const double* sorted, int64_t n, const uint32_t* seeds, int64_t 
 }
 
 void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq,
-                    const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa,
+                    const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa, int lanes, int slots,
                     hipStream_t s, bool restart) {
     if (nq <= 0) return;
     (void)g_clip_env;
-    if (dim == 2) launch_dim<2>(sorted, n, seeds, nq, knn, k, max_p, max_t, big, fa, s, restart, g_clip_lanes, g_clip_slots);
-    else launch_dim<3>(sorted, n, seeds, nq, knn, k, max_p, max_t, big, fa, s, restart, g_clip_lanes, g_clip_slots);
+    if (dim == 2) launch_dim<2>(sorted, n, seeds, nq, knn, k, max_p, max_t, big, fa, s, restart, lanes, slots);
+    else launch_dim<3>(sorted, n, seeds, nq, knn, k, max_p, max_t, big, fa, s, restart, lanes, slots);
 }
 
 void launch_collect_status(int64_t nq, const uint32_t* seeds, const int32_t* status, int code,

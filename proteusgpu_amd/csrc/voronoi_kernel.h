@@ -122,7 +122,7 @@ PG_DEV int mask_rank(const uint64_t (&m)[NW], int gs) {
 }
 
 // Vertex of three planes (two lines in 2D) by the reference's Cramer rule
-// (cell.cpp:442-465), homogeneous part (X, Y, Z, W).
+// (cell.cpp:388-411), homogeneous part (X, Y, Z, W).
 template <int DIM>
 PG_DEV void cramer(const Plane& a, const Plane& b, const Plane& c, double& X, double& Y, double& Z,
                    double& W) {
@@ -139,7 +139,7 @@ PG_DEV void cramer(const Plane& a, const Plane& b, const Plane& c, double& X, do
     }
 }
 
-// The reference conflict predicate (cell.cpp:229-275), evaluated exactly.
+// The reference conflict predicate (cell.cpp:175-221), evaluated exactly.
 template <int DIM>
 PG_DEV bool conflict_exact(const Plane& p1, const Plane& p2, const Plane& p3, const Plane& e,
                            bool& flag) {
@@ -201,7 +201,7 @@ PG_DEV void slot_write(SlotRegs<S>& r, int s, double px, double py, double pz, d
 }
 
 // Build vertex numerics from its planes (reference arithmetic for position
-// and squared seed distance, cell.cpp:423-465).
+// and squared seed distance, cell.cpp:369-411).
 template <int DIM>
 PG_DEV void vertex_numerics(const Plane& a, const Plane& b, const Plane& c, double sx, double sy,
                             double sz, double& px, double& py, double& pz, double& pw,
@@ -283,7 +283,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             for (int e = 0; e < D; e++) sm.adj[e][gs] = cst->adj[e][gs];
         }
     }
-    // Box planes (cell.cpp:96-136): eps = 1e-14 margin.
+    // Box planes (cell.cpp:44-82): eps = 1e-14 margin.
     if (!v0 && gl < 2 * D) {
         const double eps = 1e-14, lo = -eps, hi = 1.0 + eps;
         Plane p;
@@ -299,7 +299,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
         plane_bounds<D>(p, sm.pn1[gl], sm.pmx[gl]);
         sm.pvid[gl] = -1;
     }
-    // Initial vertices (cell.cpp:139-157).
+    // Initial vertices (cell.cpp:84-103).
     constexpr int NV0 = D == 3 ? 8 : 4;
     if (!v0 && gl < NV0) {
         uint32_t t;
@@ -416,7 +416,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             }
             block_sync();
         }
-        // -- new half plane (cell.cpp:212-226)
+        // -- new half plane (cell.cpp:158-172)
         uint32_t z = 0;
         double bx = 0, by = 0, bz = 0;
         Plane e = {0, 0, 0, 0};
@@ -499,7 +499,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             if (gflag) {
                 status = NEEDS_EXACT_PREDICATES;
             } else if (nb_r == 0) {
-                nb_v--;  // the plane clipped nothing (cell.cpp:187-190)
+                nb_v--;  // the plane clipped nothing (cell.cpp:132-136)
             } else {
                 doclip = true;
             }
@@ -618,7 +618,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
                 status = INCONSISTENT_BOUNDARY;
                 ok = false;
             } else if (nl_after >= max_t) {
-                status = TRIANGLE_OVERFLOW;  // cell.cpp:401-404
+                status = TRIANGLE_OVERFLOW;  // cell.cpp:347-350
                 ok = false;
             } else if (nl_after > NS) {
                 status = SLOT_OVERFLOW;
@@ -643,7 +643,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             }
         }
         block_sync();
-        // -- new vertices (cell.cpp:346-364 orientation, :442-465 position)
+        // -- new vertices (cell.cpp:346-364 orientation, :388-411 position)
         if (ok) {
             float cn1, cmx;
             plane_bounds<D>(e, cn1, cmx);
@@ -699,7 +699,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             nlive += Bt;
         }
         block_sync();
-        // -- certificate (cell.cpp:423-438) with the just-clipped neighbour;
+        // -- certificate (cell.cpp:369-384) with the just-clipped neighbour;
         //    the largest seed-vertex distance only changes with a clip
         double mx = R2;
         if (wave_ballot(doclip) != 0ull) {
@@ -772,7 +772,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
 
     // ---------------------------------------------------------- extraction
     const bool ext = valid && status == SUCCESS;
-    // normals and neighbour ids of my planes (cell.cpp:629-633)
+    // normals and neighbour ids of my planes (cell.cpp:575-579, 646-669)
     double nx[C::PPL], ny[C::PPL], nz[C::PPL];
     int32_t pv[C::PPL];
 #pragma unroll

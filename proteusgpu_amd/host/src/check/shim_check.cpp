@@ -1,5 +1,6 @@
 // Check of the reference-shaped entry points that the driver itself does not
-// call: voronoi::allocate_vmesh + compute_cells + unpermute_vmesh must give
+// call: knn::verify on a solved problem (and its failure on a corrupted
+// row), voronoi::allocate_vmesh + compute_cells + unpermute_vmesh must give
 // the mesh of a fixed-k build without restarts (the reference's
 // compute_mesh, voronoi.cpp:17-65), and hydro::riemann_hll must be the
 // consistent, antisymmetric HLL flux.  Prints "SHIM OK" and returns 0 on
@@ -7,6 +8,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <utility>
 #include <vector>
 
 #include "global/allvars.h"
@@ -43,6 +45,17 @@ int main() {
     // compute_cells on a solved KNN problem, then back to input order
     knn_problem* kp = knn::init(pts.data(), n);
     knn::solve(kp);
+    // knn::verify (knn.cpp:350-422): every row against brute force
+    expect(knn::verify(kp), "knn::verify of the solved problem");
+    {
+        // a corrupted row must fail it
+        POINT_TYPE* sp = knn::get_points(kp);
+        unsigned int* rows = knn::get_knearest(kp);
+        std::swap(rows[0], rows[_K_ - 1]);
+        expect(!knn::verify_rows((const double*)sp, rows, n, _K_, 1e-8, 1), "verify rejects a swapped row");
+        std::free(sp);
+        std::free(rows);
+    }
     std::vector<Status> stat;
     VMesh* a = voronoi::allocate_vmesh(n, 16 * (hsize_t)n);
     voronoi::compute_cells(n, kp, stat, a);

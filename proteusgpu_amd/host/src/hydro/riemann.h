@@ -7,7 +7,7 @@
 
 namespace hydro {
 
-// riemann.cpp:7-43: the HLL flux across face j of cell i between the states
+// riemann.cpp:7-45: the HLL flux across face j of cell i between the states
 // st_l (cell i) and st_r (its neighbour), in the lab frame, per unit area:
 // rho = mass flux, v = momentum flux, E = energy flux.  Orientation as
 // hydro_step (the reference's unless PHYSICAL_FLUX).  One device call per

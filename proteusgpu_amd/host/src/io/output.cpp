@@ -19,8 +19,11 @@ bool OutputHandler::initialize() {
 
 #ifdef USE_HDF5
 namespace {
+// One dataset of n0 (x n1) elements; like the reference's writer
+// (output.cpp:95-226, 239-290), an empty or missing array writes no dataset.
 template <class T>
 void write(hid_t group, const char* name, hid_t type, const T* data, hsize_t n0, hsize_t n1 = 0) {
+    if (n0 == 0 || data == nullptr) return;
     hsize_t dims[2] = {n0, n1};
     hid_t sp = H5Screate_simple(n1 ? 2 : 1, dims, nullptr);
     hid_t ds = H5Dcreate(group, name, type, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
@@ -40,7 +43,7 @@ void attr(hid_t g, const char* name, hid_t type, T v) {
 
 bool OutputHandler::writeMeshFile(const std::string& filename, const MeshCellData& md, const primvars* p,
                                   int n_hydro) {
-    std::string path = outputDirectory + "/" + filename;
+    std::string path = outputDirectory + filename;  // output.cpp:29 joins without a separator
     hid_t f = H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
     if (f < 0) {
         std::cerr << "OUTPUT: Error! Could not create file: " << path << std::endl;
@@ -69,10 +72,13 @@ bool OutputHandler::writeMeshFile(const std::string& filename, const MeshCellDat
     write(faces, "area", H5T_NATIVE_DOUBLE, md.faces.area.data(), nf);
     H5Gclose(faces);
     H5Gclose(cells);
+    // output.cpp:236-293: the hydro group always, each array only when present
     hid_t hy = H5Gcreate(f, "hydro", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
-    write(hy, "rho", H5T_NATIVE_DOUBLE, p->rho, n_hydro);
-    write(hy, "vel", H5T_NATIVE_DOUBLE, (const double*)p->v, n_hydro, DIMENSION);
-    write(hy, "Energy", H5T_NATIVE_DOUBLE, p->E, n_hydro);
+    if (p) {
+        write(hy, "rho", H5T_NATIVE_DOUBLE, p->rho, n_hydro);
+        write(hy, "vel", H5T_NATIVE_DOUBLE, (const double*)p->v, n_hydro, DIMENSION);
+        write(hy, "Energy", H5T_NATIVE_DOUBLE, p->E, n_hydro);
+    }
     H5Gclose(hy);
     H5Fclose(f);
     std::cout << "OUTPUT: wrote " << path << std::endl;

@@ -15,7 +15,7 @@ knn_problem* init(POINT_TYPE* pts, int len_pts) {
     k->k = 0;
     k->dev = nullptr;
     int rc = pg_knn_init(pgpu_host::ctx(), DIMENSION, (const double*)pts, len_pts, &k->dev);
-    if (rc == PG_ERR_GRID) {  // knn.cpp:111-115
+    if (rc == PG_ERR_GRID) {  // knn.cpp:32-36
         std::cerr << "KNN: We don't support meshes with less than approx 12700 cells (3D)." << std::endl;
         std::exit(EXIT_FAILURE);
     }
@@ -57,17 +57,11 @@ unsigned int* get_permutation(knn_problem* k) {
 
 // knn.cpp:350-422: brute-force check of the distances of every list (a test
 // utility of the reference, run on the host over the copied results).
-bool verify(knn_problem* k, double tol, int max_report) {
-    if (!k || k->k == 0) {
-        std::cerr << "KNN VERIFY: null or unsolved knn problem." << std::endl;
-        return false;
-    }
-    const int n = k->len_pts, kk = k->k;
-    POINT_TYPE* pts = get_points(k);
-    unsigned int* idx = get_knearest(k);
+// pts: n points (sorted order), idx: n rows of k (sorted-index space).
+bool verify_rows(const double* pts, const unsigned int* idx, int n, int kk, double tol, int max_report) {
     auto d2 = [&](int a, int b) {
-        const double* p = (const double*)&pts[a];
-        const double* q = (const double*)&pts[b];
+        const double* p = pts + (size_t)DIMENSION * a;
+        const double* q = pts + (size_t)DIMENSION * b;
         double s = 0;
         for (int d = 0; d < DIMENSION; d++) s += (p[d] - q[d]) * (p[d] - q[d]);
         return s;
@@ -90,18 +84,30 @@ bool verify(knn_problem* k, double tol, int max_report) {
         bool ok = true;
         for (int m = 0; m < kk && ok; m++) {
             unsigned int id = idx[(size_t)i * kk + m];
-            ok = id != UINT_MAX && std::fabs(d2(i, (int)id) - best[m]) <= tol * (1.0 + best[m]);
+            ok = id != UINT_MAX && id < (unsigned int)n &&
+                 std::fabs(d2(i, (int)id) - best[m]) <= tol * (1.0 + best[m]);
         }
         if (!ok && ++bad <= max_report) std::cerr << "KNN VERIFY: mismatch at point " << i << std::endl;
     }
-    std::free(pts);
-    std::free(idx);
     if (bad) {
         std::cerr << "KNN VERIFY: failed " << bad << " / " << n << " points mismatch." << std::endl;
         return false;
     }
     std::cout << "KNN VERIFY: passed - all points match brute-force." << std::endl;
     return true;
+}
+
+bool verify(knn_problem* k, double tol, int max_report) {
+    if (!k || k->k == 0) {
+        std::cerr << "KNN VERIFY: null or unsolved knn problem." << std::endl;
+        return false;
+    }
+    POINT_TYPE* pts = get_points(k);
+    unsigned int* idx = get_knearest(k);
+    const bool ok = verify_rows((const double*)pts, idx, k->len_pts, k->k, tol, max_report);
+    std::free(pts);
+    std::free(idx);
+    return ok;
 }
 
 }  // namespace knn

@@ -8,7 +8,7 @@
 
 typedef struct {
     int len_pts;  // number of input points
-    int N_grid;   // grid resolution (knn.cpp:101)
+    int N_grid;   // grid resolution (knn.cpp:22)
     int k;        // neighbours per point of the last solve
     pg_knn* dev;  // device-side problem (grid, sorted points, lists)
 } knn_problem;
@@ -22,6 +22,9 @@ POINT_TYPE* get_points(knn_problem* knn);          // knn.h:55 (sorted order)
 unsigned int* get_knearest(knn_problem* knn);      // knn.h:56 (sorted-index space)
 unsigned int* get_permutation(knn_problem* knn);   // knn.h:57 (sorted -> input index)
 bool verify(knn_problem* knn, double tol = 1e-8, int max_report = 5);  // knn.h:60
+// The check of verify on host arrays: n points (sorted order) and their rows
+// of length k (sorted-index space), e.g. the KNN problem of a mesh build.
+bool verify_rows(const double* pts, const unsigned int* idx, int n, int k, double tol = 1e-8, int max_report = 5);
 
 }  // namespace knn
 

@@ -1,4 +1,4 @@
-// Driver with the reference's flow (src/main.cpp:33-84, begrun.cpp:14-45):
+// Driver with the reference's flow (src/main.cpp:32-84, begrun.cpp:14-45):
 // banner, param.txt, IC file, mesh, CFL-limited HLL steps to t_end, snapshot.
 //
 // param.txt keys of the reference: ic_file, output_directory,

@@ -9,6 +9,9 @@
 #include "../io/input.h"
 #include "../io/output.h"
 #endif
+#ifdef VERIFY
+#include "../knn/knn.h"
+#endif
 
 namespace voronoi {
 
@@ -73,11 +76,23 @@ VMesh* build(POINT_TYPE* pts, int64_t n, int boundary) {
     pg_mesh_params prm = pgpu_host::mesh_params(boundary, _K_);
     pg_mesh* dm = nullptr;
     int rc = pg_mesh_compute(pgpu_host::ctx(), &prm, (const double*)pts, n, &dm);
-    if (rc == PG_ERR_GRID) {  // knn.cpp:111-115
+    if (rc == PG_ERR_GRID) {  // knn.cpp:32-36
         std::cerr << "KNN: We don't support meshes with less than approx 12700 cells (3D)." << std::endl;
         std::exit(EXIT_FAILURE);
     }
     pgpu_host::check(rc, "voronoi::compute_mesh");
+#ifdef VERIFY
+    {
+        // voronoi.cpp:39-41: brute-force check of the build's first-pass rows
+        pg_mesh_info_t vi;
+        pg_mesh_info(dm, &vi);
+        const size_t ns = (size_t)vi.n_seeds;
+        std::vector<double> vp(ns * DIMENSION);
+        std::vector<unsigned int> vr(ns * (size_t)_K_);
+        pgpu_host::check(pg_mesh_get_knn(pgpu_host::ctx(), dm, vp.data(), vr.data(), nullptr), "knn rows");
+        if (!knn::verify_rows(vp.data(), vr.data(), (int)ns, _K_)) std::exit(EXIT_FAILURE);
+    }
+#endif
 #if defined(USE_HDF5) && defined(WRITE_KNN_OUTPUT)
     pg_mesh_info_t info;
     pg_mesh_info(dm, &info);
@@ -85,7 +100,7 @@ VMesh* build(POINT_TYPE* pts, int64_t n, int boundary) {
 #endif
     VMesh* m = (VMesh*)std::calloc(1, sizeof(VMesh));
     fill_vmesh(m, dm);
-    // voronoi.cpp:189-196: report cells that failed
+    // voronoi.cpp:104-110: report cells that failed
     for (size_t i = 0; i < (size_t)m->n_seeds; i++)
         if (m->status[i] != success)
             std::cout << "VORONOI: cell " << i << " failed with status: " << m->status[i] << std::endl;

@@ -9,9 +9,11 @@ and pinned by a SHA-256 of their bytes.  Large arrays are stored in full where
 they pin topology and as a leading slice plus a SHA-256 of the whole array
 otherwise.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            (every fixture)
+    python tests/golden/make_golden.py files      (only the file-layout fixture)
 """
 import hashlib
+import json
 import os
 import struct
 import subprocess
@@ -94,15 +96,82 @@ def periodic_case(name, dim, n, k, seed, buff, gamma, cfl, steps):
          face_area_sha=sha(o["face_area"]), dts=o["dts"], rho=o["rho"], vel=o["vel"], E=o["E"])
 
 
+H5DUMP = "/opt/conda/bin/h5dump"
+MAKE_IC = os.path.join(ROOT, "proteusgpu_amd", "host", "make_ic")
+
+
+def h5_header(path):
+    """h5dump -H of an HDF5 file (groups, datasets, dtypes, shapes and
+    attributes), without the line naming the file."""
+    r = subprocess.run([H5DUMP, "-H", path], capture_output=True, text=True, check=True)
+    return "\n".join(r.stdout.splitlines()[1:])
+
+
+def h5_attrs(path):
+    """h5dump -A of the /header group: its attributes with their values."""
+    r = subprocess.run([H5DUMP, "-A", "-g", "/header", path], capture_output=True, text=True, check=True)
+    return "\n".join(r.stdout.splitlines()[1:])
+
+
+def h5_sha(path, name):
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "x.bin")
+        subprocess.run([H5DUMP, "-d", name, "-b", "LE", "-o", out, path], check=True, capture_output=True)
+        return hashlib.sha256(open(out, "rb").read()).hexdigest()
+
+
+def files_case(name, periodic_golden):
+    """The reference's own main loop (IC reader, periodic mesh with its KNN
+    dump, CFL loop, snapshot writer) on an IC written by the project's
+    make_ic, run to the time the golden periodic case reaches in its steps.
+    Stores the structure of both files (h5dump -H) and SHA-256 of the
+    datasets a port must reproduce bit for bit."""
+    g = np.load(os.path.join(HERE, periodic_golden + ".npz"))
+    n, seed, gamma, cfl, buff = int(g["n"]), int(g["seed"]), float(g["gamma"]), float(g["cfl"]), float(g["buff"])
+    t_end = 0.0
+    for dt in g["dts"]:
+        t_end += float(dt)
+    with tempfile.TemporaryDirectory() as d:
+        ic = os.path.join(d, "IC.hdf5")
+        subprocess.run([MAKE_IC, ic, "3", str(n), str(seed), "sod", repr(gamma)], check=True, capture_output=True)
+        outdir = os.path.join(d, "out") + "/"
+        par = os.path.join(d, "param.txt")
+        with open(par, "w") as f:
+            f.write(f"ic_file = {ic}\noutput_directory = {outdir}\noutput_mesh_file = mesh.hdf5\n"
+                    f"output_knn_file = knn.hdf5\n")
+        r = subprocess.run([os.path.join(REF, "ref_files_3d_k128"), "files", par, repr(gamma), repr(cfl), repr(t_end),
+                            repr(buff)], capture_output=True, text=True, cwd=d)
+        if r.returncode != 0:
+            raise RuntimeError(f"ref_files failed: {r.returncode}\n{r.stderr[-2000:]}")
+        mesh, knn = os.path.join(outdir, "mesh.hdf5"), os.path.join(outdir, "knn.hdf5")
+        # the reference's snapshot state is the golden periodic state
+        for key, ds in (("rho", "/hydro/rho"), ("E", "/hydro/Energy"), ("vel", "/hydro/vel")):
+            assert h5_sha(mesh, ds) == sha(g[key]), key
+        out = {"n": n, "seed": seed, "gamma": gamma, "cfl": cfl, "buff": buff, "t_end": t_end, "k": 128,
+               "mesh_header": h5_header(mesh), "knn_header": h5_header(knn),
+               "mesh_attrs": h5_attrs(mesh), "knn_attrs": h5_attrs(knn),
+               "mesh_sha": {ds: h5_sha(mesh, ds) for ds in ("/cells/cell_ids", "/cells/face_counts",
+                                                              "/cells/faces/neighbor_cell")},
+               "knn_sha": {ds: h5_sha(knn, ds) for ds in ("/knn/points", "/knn/nearest", "/knn/permutation")}}
+    path = os.path.join(HERE, name + ".json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(f"{name}.json  {os.path.getsize(path) / 1024:.0f} KiB")
+
+
 def main():
     if not os.path.isdir(REF):
         sys.exit("oracle/_ref is not built (make -C oracle ref, where the reference is mounted)")
+    if sys.argv[1:] == ["files"]:
+        files_case("files_3d_n20000_k128", "periodic_3d_n20000_k128")
+        return
     knn_case("knn_2d_n2000_k16", 2, 2000, 16, 101)
     knn_case("knn_3d_n13000_k32", 3, 13000, 32, 102)
     mesh_case("mesh_2d_n2000_k32", 2, 2000, 32, 103, head_cells=2000)
     mesh_case("mesh_3d_n13000_k64", 3, 13000, 64, 104, head_cells=1500)
     periodic_case("periodic_2d_n10000_k48", 2, 10000, 48, 105, 0.04, 1.4, 0.3, 4)
     periodic_case("periodic_3d_n20000_k128", 3, 20000, 128, 106, 0.12, 1.4, 0.3, 3)
+    files_case("files_3d_n20000_k128", "periodic_3d_n20000_k128")
 
 
 if __name__ == "__main__":

@@ -1,9 +1,9 @@
 """The per-face and per-problem entry points of the drop-in boundary.
 
-pg_hll_flux is hydro::riemann_hll (riemann.cpp:7-43) for many faces; it
+pg_hll_flux is hydro::riemann_hll (riemann.cpp:7-45) for many faces; it
 runs the same device HLL as the cell kernels and must equal the oracle's
 restatement bit for bit.  pg_voronoi_compute is voronoi::compute_cells
-(voronoi.cpp:67-111): the cells of a solved KNN problem, no restarts, in the
+(voronoi.cpp:68-111): the cells of a solved KNN problem, no restarts, in the
 problem's sorted order (neighbour ids in sorted-index space); mapped through
 the problem's permutation it must equal the oracle's fixed-k mesh (statuses,
 topology and normals bit-exact, geometry rtol 1e-12).

@@ -3,7 +3,8 @@
 Bar: KNN lists, sorted points, permutation and Voronoi topology (face counts,
 neighbour ids, statuses, certified k) bit-exact; normals bit-exact (same
 arithmetic on the same planes); volumes and areas within rtol 1e-12 (the GPU
-sums fan triangles in its own vertex order); hydro states within rtol 1e-11
+sums fan triangles in its own vertex order); cell and face centroids within
+1e-13 absolute (coordinates of order one; 2D cell centroids 5e-11); hydro states within rtol 1e-11
 after several steps (geometry round-off propagated; dt within rtol 1e-13
 because cbrt is not pinned across libms).
 """
@@ -47,7 +48,7 @@ def test_knn_ties_lattice(ctx):
 
 
 def _cmp_mesh(g, ref, geom_rtol=1e-12):
-    # 2D volumes: the reference's shoelace (cell.cpp:529-552) works in absolute
+    # 2D volumes: the reference's shoelace (cell.cpp:475-498) works in absolute
     # coordinates, so its round-off is absolute (~u |x|^2), not relative.
     vol_atol = 1e-15 if ref.dim == 2 else 0.0
     assert np.array_equal(g["status"], ref.get("status"))
@@ -59,6 +60,15 @@ def _cmp_mesh(g, ref, geom_rtol=1e-12):
     np.testing.assert_allclose(g["volumes"], ref.get("volumes"), rtol=geom_rtol, atol=vol_atol)
     np.testing.assert_allclose(g["face_area"], ref.get("face_area"), rtol=geom_rtol, atol=1e-300)
     np.testing.assert_array_equal(g["seeds"], ref.get("seeds"))
+    # centroids (no reference counterpart; the moving-mesh face velocity and
+    # the regularised motion use them): certified cells, coordinates of order
+    # one, each summed over fan triangles in the kernel's own vertex order;
+    # 2D cell centroids share the absolute-coordinate cancellation of the 2D
+    # area (~u |x|^2 / A per coordinate)
+    ok = g["status"] == 4
+    c_atol = 5e-11 if ref.dim == 2 else 1e-13
+    np.testing.assert_allclose(g["cell_centroid"][ok], ref.get("cell_centroid")[ok], rtol=0, atol=c_atol)
+    np.testing.assert_allclose(g["face_centroid"], ref.get("face_centroid"), rtol=0, atol=1e-13)
 
 
 @pytest.mark.parametrize("dim,n,k,seed", [(2, 2000, 32, 4), (3, 13000, 64, 5), (3, 13000, 32, 6)])

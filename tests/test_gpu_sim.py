@@ -29,7 +29,7 @@ def test_static_sim_matches_oracle(ctx):
         dt_o = ref.dt_cfl(gamma, 0.3, r, v, e)
         dt_g = sim.step()
         # 2D cell areas of the reference carry ~1e-16 absolute round-off from
-        # its absolute-coordinate shoelace (cell.cpp:529-552): ~1e-12 relative
+        # its absolute-coordinate shoelace (cell.cpp:475-498): ~1e-12 relative
         # on cells of area 1e-4, which dt = CFL*sqrt(V/pi)/(c+|v|) inherits.
         assert abs(dt_g - dt_o) <= 1e-11 * dt_o
         ref.hydro_step(gamma, dt_g, r, v, e, flux_mode=1)
@@ -56,7 +56,7 @@ def test_galilean_invariance(ctx, dim, k):
     p2, r2, v2, e2 = sim.get()
     # The mass flux is exactly zero (identical states at rest in the face
     # frame); what remains is the round-off of the rebuilt cell volumes: the
-    # reference's planes use absolute coordinates (cell.cpp:218-222), so
+    # reference's planes use absolute coordinates (cell.cpp:164-168), so
     # vertices carry ~u*|x|^2/h error, ~1e-12 relative volume per rebuild.
     assert np.max(np.abs(r2 - 1.3)) < 1e-10
     assert np.max(np.abs(v2 - u)) < 1e-10
@@ -113,3 +113,29 @@ def test_zero_velocity_moving_step_equals_static(ctx, dim, n, k):
     assert np.array_equal(x_m, pts) and np.array_equal(x_s, pts)
     for a, b in ((r_s, r_m), (v_s, v_m), (e_s, e_m)):
         assert np.array_equal(a, b)
+
+
+def test_failed_step_makes_the_sim_unusable(ctx):
+    """A step whose rebuild fails leaves the simulation unusable: two
+    generators moving head-on meet exactly (velocities +-0.5, 2^-10 apart,
+    the step capped at 2^-10, below every cell's CFL step), the rebuild
+    reports the duplicate pair's cells uncertified (PG_ERR_CELLS), and every
+    later call that would launch on the simulation returns PG_ERR_FAILED."""
+    from proteusgpu_amd._native import PgError
+    n, gamma = 1000, 1.4
+    pts = uniform_points(n, 2, 71)
+    pts[0] = [0.25, 0.5]
+    pts[1] = [0.25 + 2.0 ** -10, 0.5]
+    rho = np.ones(n)
+    vel = np.zeros((n, 2))
+    vel[0], vel[1] = [0.5, 0.0], [-0.5, 0.0]
+    E = 1e-3 / (gamma - 1) + 0.5 * rho * np.sum(vel * vel, axis=1)
+    sim = api.Simulation(ctx, pts, rho, vel, E, boundary="reflective", k=16, gamma=gamma, cfl=0.3,
+                         flux="physical", move_mesh="fluid")
+    with pytest.raises(PgError) as e:
+        sim.step(dt_max=2.0 ** -10)
+    assert e.value.code == -5
+    for call in (lambda: sim.step(), lambda: sim.hydro_steps(1), sim.get, sim.totals, sim.mesh_handle):
+        with pytest.raises(PgError) as e:
+            call()
+        assert e.value.code == -7 and "earlier step" in str(e.value)

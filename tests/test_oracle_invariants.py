@@ -84,7 +84,7 @@ def bcc_lattice(m, off=0.0131, jitter=0.0):
 
 def test_bcc_exact_lattice_is_flagged():
     """The exact lattice has vertices on later bisector planes: the
-    reference's eps test flags those cells (cell.cpp:270-272)."""
+    reference's eps test flags those cells (cell.cpp:216-218)."""
     pts, a = bcc_lattice(20)
     m = O.Mesh(pts, "reflective", k=32, restart=True)
     st = m.get("status")
