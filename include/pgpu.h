@@ -84,6 +84,23 @@ int pg_knn_get_permutation(pg_ctx* ctx, const pg_knn* knn, uint32_t* out); /* n 
 /* knn::knn_free (knn.h:54): frees and nulls the handle. */
 int pg_knn_free(pg_knn** knn);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): pg_knn_init + pg_knn_solve on DEVICE arrays, with every
+ * scratch buffer (grid counts and starts, cell ids, scan partials) in a
+ * static device array of PG_NOALLOC_SCRATCH_BYTES that each GPU holds until
+ * the process ends.  pts (n*dim, device) in; sorted (n*dim), perm (n) and
+ * rows (n*k) device out, equal to pg_knn_get_points / get_permutation /
+ * get_knearest of the allocating path; N_grid (host, may be NULL).  About
+ * 9.2 bytes of scratch per point: n up to roughly 3.5M points; larger n
+ * gives PG_ERR_ARG before anything is launched.  Calls no hipMalloc, hipFree
+ * or memset; work is enqueued on the context's stream and the call returns
+ * without synchronising.  Every call on a GPU shares that GPU's array, so
+ * calls that overlap in time on one GPU (several contexts or threads) race. */
+#define PG_NOALLOC_SCRATCH_BYTES (32 * 1024 * 1024)
+int pg_knn_solve_noalloc(pg_ctx* ctx, int dim, const double* pts, int64_t n, int k, double* sorted,
+                         uint32_t* perm, uint32_t* rows, int* N_grid);
+
 /* Shape of the fast clip kernel for every following build of this process,
  * read once at the start of each build
  * (no reference counterpart; results never depend on it): lanes per seed

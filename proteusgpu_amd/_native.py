@@ -94,6 +94,7 @@ def _signatures():
         "pg_knn_info": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int)], i32),
         "pg_knn_get_points": ([vp, vp, vp], i32), "pg_knn_get_knearest": ([vp, vp, vp], i32),
         "pg_knn_get_permutation": ([vp, vp, vp], i32), "pg_knn_free": ([pp], i32),
+        "pg_knn_solve_noalloc": ([vp, i32, vp, i64, i32, vp, vp, vp, C.POINTER(C.c_int)], i32),
         "pg_mesh_params_default": ([C.POINTER(MeshParams)], None),
         "pg_mesh_compute": ([vp, C.POINTER(MeshParams), vp, i64, pp], i32),
         "pg_mesh_info": ([vp, C.POINTER(MeshInfo)], i32),

@@ -90,6 +90,7 @@ constexpr uint64_t DISPLACE_SALT = 0xD1B54A32D192ED03ull;
 struct pg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    void* scratch = nullptr;  // this GPU's static scratch array (allocation-free entry points)
     DBuf hll_io[5];  // pg_hll_flux staging (left, right, normal, wall, flux), grow-only
 };
 
@@ -610,7 +611,12 @@ int pg_ctx_create(int device, pg_ctx** out) {
         HIPCHK(hipSetDevice(device));
         pg_ctx* c = new pg_ctx();
         c->device = device;
-        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        hipError_t e = pg::noalloc_setup(device, &c->scratch);
+        if (e != hipSuccess) {
+            delete c;
+            hipchk(e, "hipGetSymbolAddress of the scratch array");
+        }
+        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e != hipSuccess) {
             delete c;
             hipchk(e, "hipStreamCreate");
@@ -699,6 +705,29 @@ int pg_knn_solve(pg_ctx* c, pg_knn* k, int kk) {
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
         k->k = kk;
+    });
+}
+
+int pg_knn_solve_noalloc(pg_ctx* c, int dim, const double* pts, int64_t n, int kk, double* sorted, uint32_t* perm,
+                         uint32_t* rows, int* N_grid) {
+    static_assert(PG_NOALLOC_SCRATCH_BYTES == pg::NOALLOC_SCRATCH_BYTES, "scratch size");
+    return guarded([&] {
+        use(c);
+        argchk(pts && sorted && perm && rows, "null argument");
+        argchk(dim == 2 || dim == 3, "dim must be 2 or 3");
+        argchk(n >= 2 && n < (int64_t)1 << 31, "n out of range");
+        argchk(kk >= 1 && kk <= 256, "k must be in [1, 256]");
+        const int N = ngrid(dim, n);
+        if (N < 16) throw Error{PG_ERR_GRID, "KNN: N_grid < 16 (the reference's minimum mesh size)"};
+        const int64_t need = pg::noalloc_knn_bytes(dim, n, N);
+        if (need > pg::NOALLOC_SCRATCH_BYTES)
+            throw Error{PG_ERR_ARG, "pg_knn_solve_noalloc: " + std::to_string(n) + " points need " +
+                                        std::to_string(need) + " bytes of scratch, more than the " +
+                                        std::to_string(pg::NOALLOC_SCRATCH_BYTES) + " reserved per GPU"};
+        argchk(c->scratch != nullptr, "context without scratch array");
+        pg::noalloc_knn(c->scratch, dim, pts, n, N, kk, sorted, perm, rows, c->stream);
+        HIPCHK(hipGetLastError());
+        if (N_grid) *N_grid = N;
     });
 }
 

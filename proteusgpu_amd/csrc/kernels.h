@@ -206,6 +206,21 @@ void launch_bcollect(const int64_t* nq_dev, int64_t cap, const uint32_t* seeds, 
                      hipStream_t s);
 void launch_bcount_failed(const int64_t* n_dev, int64_t cap, const int32_t* status, int32_t* nfail, hipStream_t s);
 void launch_fill_i32(int32_t* a, int64_t n, int32_t v, hipStream_t s);
+// ------------------------------------------------------------- noalloc.hip
+// Size of the static device array behind the allocation-free entry points
+// (one copy per GPU, held until the process ends).
+constexpr int64_t NOALLOC_SCRATCH_BYTES = (int64_t)32 << 20;
+// Device address of this GPU's array (the device must be current); looked
+// up once per device, then cached.
+hipError_t noalloc_setup(int device, void** ptr);
+// Scratch bytes the allocation-free KNN build needs for n points on an N^dim grid.
+int64_t noalloc_knn_bytes(int dim, int64_t n, int N);
+// knn::init + knn::solve on device points: sorted points, permutation and
+// rows of length k, scratch carved from `scratch` (noalloc_setup's address),
+// no allocation, no memset, no synchronisation.
+void noalloc_knn(void* scratch, int dim, const double* pts, int64_t n, int N, int k, double* sorted, uint32_t* perm,
+                 uint32_t* rows, hipStream_t s);
+
 // ---------------------------------------------------------------- probe.hip
 // Streaming device copy of `bytes` (multiple of 32).
 void launch_stream_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
