@@ -396,6 +396,46 @@ def main():
                "dt": float(max(dts_g))}
         hs.free()
         m.free()
+    # 5. config C1 beside the headline: 2D, 10,000 uniform generators,
+    #    reflective box, Sod, k = 16, 20 moving steps at CFL 0.3; GPU step
+    #    time, the oracle's on the host (all threads and one), cell by cell
+    #    difference after the 20 steps
+    c1 = None
+    if not a.no_cpu:
+        phase("config C1 (2D 10k Sod, 20 moving steps)")
+        from oracle import oracle as O
+        p1 = uniform_points(10000, 2, 1001)
+        r1, v1, e1 = sod_state(p1, gamma)
+        kw1 = dict(boundary="reflective", k=16, gamma=gamma, cfl=cfl, flux="physical", move_mesh="fluid")
+        g1 = api.Simulation(ctx, p1, r1, v1, e1, **kw1)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            g1.step()
+        ctx.sync()
+        g1_ms = (time.perf_counter() - t0) * 1e3 / 20
+        oms = {}
+        for threads in (O.get_threads(), 1):
+            cores = O.get_threads()
+            O.set_threads(threads)
+            o1 = O.Sim(p1, r1, v1, e1, boundary="reflective", k=16, gamma=gamma, cfl=cfl, flux="physical",
+                       move="fluid")
+            t0 = time.perf_counter()
+            for _ in range(20):
+                o1.step()
+            oms[threads] = (time.perf_counter() - t0) * 1e3 / 20
+            O.set_threads(cores)
+        xg, rg, vg, eg = g1.get()
+        xo, ro, vo, eo = o1.get()
+        c1 = {"gpu_ms_per_step": round(g1_ms, 3), "gpu_cells_per_s": round(1e4 / (g1_ms * 1e-3), 1),
+              "oracle_ms_per_step": {str(t): round(v, 2) for t, v in oms.items()},
+              "oracle_cells_per_s": {str(t): round(1e4 / (v * 1e-3), 1) for t, v in oms.items()},
+              "state_rel_err_vs_oracle": {"rho": float(np.max(np.abs(rg - ro) / ro)),
+                                          "E": float(np.max(np.abs(eg - eo) / eo)),
+                                          "x": float(np.max(np.abs(xg - xo)))},
+              "topology_bit_exact": bool(np.array_equal(g1.mesh_copy_out()["neighbor_cell"],
+                                                        o1.mesh().get("neighbor_cell")))}
+        g1.free()
     tot = blearn.totals()[0]
     best_fixed_k = min(fixed, key=lambda kk: fixed[kk]["ms_per_step"])
     ms_best_fixed = fixed[best_fixed_k]["ms_per_step"]
@@ -410,7 +450,7 @@ def main():
                                f"run on the first mesh of another point set) and certificate restarts; "
                                f"fixed-k steps at k = {', '.join(str(x) for x in sorted(fixed))} timed beside",
                    "n_cells": a.n, "k_fixed": a.k},
-        "roofline": roof, "cpu_baseline": cpu, "state_rel_err_vs_oracle": err,
+        "roofline": roof, "cpu_baseline": cpu, "state_rel_err_vs_oracle": err, "config_c1": c1,
         "learned_k": {
             "host_ms_per_step": round(ms_lsim, 3), "batched_ms_per_step": round(ms_blearn, 3),
             "speedup_host_vs_fixed_k": round(ms_plain / ms_lsim, 4),

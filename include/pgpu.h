@@ -127,7 +127,8 @@ typedef struct {
     int max_p;     /* plane capacity per cell (Config.sh _MAX_P_, <= 64)         */
     int max_t;     /* vertex capacity per cell (Config.sh _MAX_T_, <= 128)       */
     int restart;   /* nonzero: recompute uncertified seeds at larger k           */
-    int k_max;     /* largest k a restart may use (default 256)                  */
+    int k_max;     /* largest k a restart may use (default and limit 1024;      *
+                    * batches: at most 256)                                      */
     /* Degeneracy fallback (no reference counterpart; the reference leaves
      * such cells failed, voronoi.cpp:104-110).  0 (default): reference
      * behaviour.  > 0: when a host-driven build (pg_mesh_compute*, pg_sim_*)
@@ -142,6 +143,23 @@ typedef struct {
      * O(amplitude x spacing).  pg_mesh_info_t.displacement reports the
      * amplitude used.  Batches (pg_batch_*) do not apply it.               */
     double perturb;
+    /* Exact-predicate fallback (no reference counterpart; the reference sets
+     * needs_exact_predicates, cell.cpp:216-218, and leaves the cell failed,
+     * voronoi.cpp:104-110).  0 (default): reference behaviour.  1: cells a
+     * build leaves with PG_NEEDS_EXACT_PREDICATES are clipped again with the
+     * conflict test evaluated exactly from the generators (fixed-width
+     * integers) and exact ties broken by Simulation of Simplicity on power
+     * weights ranked by input index, so every cell of the perturbed diagram
+     * is certified and neighbouring cells agree face for face; the
+     * generators are not moved and the zero-area faces of the degenerate
+     * diagram appear as faces of zero area.  Cells whose filtered predicate
+     * never needs it are unchanged bit for bit.  Such runs certify a cell
+     * only when the certifying neighbour lies beyond twice the largest
+     * vertex distance by a relative margin of 1e-9 (a tied neighbour can cut
+     * a zero-area face).  Coordinates must lie in [2^-120, 2) or be zero
+     * for the exact test (else the cell keeps PG_NEEDS_EXACT_PREDICATES).
+     * Applied before perturb.  Batches do not apply it.                     */
+    int exact;
 } pg_mesh_params;
 
 void pg_mesh_params_default(pg_mesh_params* p);
@@ -162,6 +180,7 @@ typedef struct {
                            * (counted when the degeneracy fallback is on)         */
     double displacement;  /* amplitude of the displacement the mesh was built     *
                            * with (0: the generators as given)                    */
+    int64_t n_exact;      /* cells clipped again by the exact-predicate fallback */
 } pg_mesh_info_t;
 
 /* voronoi::compute_mesh (voronoi.h:68, voronoi.cpp:17-65) for boundary NONE,

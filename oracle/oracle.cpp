@@ -109,6 +109,234 @@ inline double dist2(int dim, const double* a, const double* b) {
     return dx * dx + dy * dy + dz * dz;
 }
 
+// ------------------------------------------------------------- exact sign
+// Degenerate conflict tests (the reference's needs_exact_predicates,
+// cell.cpp:216-218, which leaves the cell failed, voronoi.cpp:104-110),
+// decided exactly.  No reference counterpart; the rule (pinned against
+// rational arithmetic in tests/test_oracle_exact.py):
+//   sign of det[rows: the vertex's planes, then the new plane] over the
+//   planes computed without rounding from the generators: bisector of seed s
+//   and point b = (s - b, (|b|^2 - |s|^2) / 2), box plane = the initial
+//   cell's; ties broken by Simulation of Simplicity on power weights
+//   eps^rank(key): point x's weight adds (w_s - w_x) C_x / 2 to the
+//   determinant (C_x the cofactor of its w entry), the first non-zero
+//   coefficient in key order decides (the seed's: sum of the bisector
+//   cofactors; a bisector point's: -C_x).
+// Integers: every value is m 2^e; with E the smallest e involved, rows
+// scaled to (2 (S - B), |B|^2 - |S|^2) and (sg e_a, -sg C) (S, B, C: values
+// over 2^E) keep the sign of the determinant.  Determinants by the Leibniz
+// sum over permutations, sign-magnitude integers in base 2^32.
+struct BigZ {
+    int sg = 0;                 // -1, 0, +1
+    std::vector<uint32_t> mag;  // little endian, no leading zero limbs
+};
+
+void bz_trim(std::vector<uint32_t>& a) {
+    while (!a.empty() && a.back() == 0) a.pop_back();
+}
+int bz_cmp_mag(const std::vector<uint32_t>& a, const std::vector<uint32_t>& b) {
+    if (a.size() != b.size()) return a.size() < b.size() ? -1 : 1;
+    for (size_t i = a.size(); i-- > 0;)
+        if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+    return 0;
+}
+std::vector<uint32_t> bz_add_mag(const std::vector<uint32_t>& a, const std::vector<uint32_t>& b) {
+    std::vector<uint32_t> r(std::max(a.size(), b.size()) + 1, 0);
+    uint64_t c = 0;
+    for (size_t i = 0; i < r.size(); i++) {
+        uint64_t t = c + (i < a.size() ? a[i] : 0) + (i < b.size() ? b[i] : 0);
+        r[i] = (uint32_t)t;
+        c = t >> 32;
+    }
+    bz_trim(r);
+    return r;
+}
+std::vector<uint32_t> bz_sub_mag(const std::vector<uint32_t>& a, const std::vector<uint32_t>& b) {  // a >= b
+    std::vector<uint32_t> r(a.size(), 0);
+    int64_t br = 0;
+    for (size_t i = 0; i < a.size(); i++) {
+        int64_t t = (int64_t)a[i] - (i < b.size() ? b[i] : 0) - br;
+        br = t < 0;
+        r[i] = (uint32_t)(t + (br << 32));
+    }
+    bz_trim(r);
+    return r;
+}
+BigZ bz_add(const BigZ& a, const BigZ& b) {
+    if (a.sg == 0) return b;
+    if (b.sg == 0) return a;
+    BigZ r;
+    if (a.sg == b.sg) {
+        r.mag = bz_add_mag(a.mag, b.mag);
+        r.sg = a.sg;
+        return r;
+    }
+    const int c = bz_cmp_mag(a.mag, b.mag);
+    if (c == 0) return r;
+    r.mag = c > 0 ? bz_sub_mag(a.mag, b.mag) : bz_sub_mag(b.mag, a.mag);
+    r.sg = c > 0 ? a.sg : b.sg;
+    return r;
+}
+BigZ bz_neg(BigZ a) {
+    a.sg = -a.sg;
+    return a;
+}
+BigZ bz_mul(const BigZ& a, const BigZ& b) {
+    BigZ r;
+    if (a.sg == 0 || b.sg == 0) return r;
+    std::vector<uint64_t> acc(a.mag.size() + b.mag.size() + 1, 0);
+    std::vector<uint32_t> m(acc.size(), 0);
+    for (size_t i = 0; i < a.mag.size(); i++) {
+        uint64_t c = 0;
+        for (size_t j = 0; j < b.mag.size(); j++) {
+            uint64_t t = (uint64_t)a.mag[i] * b.mag[j] + m[i + j] + c;
+            m[i + j] = (uint32_t)t;
+            c = t >> 32;
+        }
+        for (size_t k = i + b.mag.size(); c; k++) {
+            uint64_t t = (uint64_t)m[k] + c;
+            m[k] = (uint32_t)t;
+            c = t >> 32;
+        }
+    }
+    bz_trim(m);
+    r.mag = m;
+    r.sg = a.sg * b.sg;
+    return r;
+}
+// m 2^sh for an integer |m| < 2^53 and sh >= 0
+BigZ bz_from(int64_t m, int sh) {
+    BigZ r;
+    if (m == 0) return r;
+    r.sg = m < 0 ? -1 : 1;
+    uint64_t a = (uint64_t)(m < 0 ? -m : m);
+    std::vector<uint32_t> v(sh / 32 + 4, 0);
+    const int q = sh / 32, b = sh % 32;
+    // a < 2^53 shifted by b < 32 bits spans at most three limbs
+    unsigned __int128 x = (unsigned __int128)a << b;
+    for (int i = 0; i < 3; i++) v[q + i] = (uint32_t)(x >> (32 * i));
+    bz_trim(v);
+    r.mag = v;
+    return r;
+}
+
+// Leibniz determinant of an n x n matrix (n <= 4), row major.
+BigZ bz_det(const std::vector<BigZ>& a, int n) {
+    int perm[4] = {0, 1, 2, 3};
+    BigZ det;
+    do {
+        // parity by counting inversions
+        int inv = 0;
+        for (int i = 0; i < n; i++)
+            for (int j = i + 1; j < n; j++) inv += perm[i] > perm[j];
+        BigZ t = a[0 * n + perm[0]];
+        for (int i = 1; i < n && t.sg != 0; i++) t = bz_mul(t, a[(size_t)i * n + perm[i]]);
+        det = bz_add(det, (inv & 1) ? bz_neg(t) : t);
+    } while (std::next_permutation(perm, perm + n));
+    return det;
+}
+
+// value = m 2^e (m = 0 for zero); false for non-finite or subnormal values
+bool split_exact(double v, int64_t& m, int& e) {
+    if (v == 0.0) {
+        m = 0;
+        e = 0;
+        return true;
+    }
+    if (!std::isfinite(v) || std::fabs(v) < DBL_MIN) return false;
+    int ex;
+    const double f = std::frexp(v, &ex);  // v = f 2^ex, 0.5 <= |f| < 1
+    m = (int64_t)std::ldexp(f, 53);
+    e = ex - 53;
+    return true;
+}
+
+// One row of the exact conflict determinant: box plane (axis, sign) or the
+// bisector of the seed and point p.
+struct ExRow {
+    int box;  // 0: bisector; +-(axis + 1): box plane x_axis >= lo (+) / <= hi (-)
+    double p[3];
+    uint32_t key;
+};
+
+// +1: conflict, -1: no conflict, 0: undecided (a subnormal or non-finite value)
+int exact_conflict(int dim, const double* s, uint32_t skey, const ExRow* rows) {
+    const int nr = dim + 1;
+    const double lo = -1e-14, hi = 1.0 + 1e-14;  // the initial cell's box
+    std::vector<double> vals(s, s + dim);
+    for (int k = 0; k < nr; k++) {
+        if (rows[k].box) vals.push_back(rows[k].box > 0 ? lo : hi);
+        else
+            for (int d = 0; d < dim; d++) vals.push_back(rows[k].p[d]);
+    }
+    int emin = INT_MAX;
+    for (double v : vals) {
+        int64_t m;
+        int e;
+        if (!split_exact(v, m, e)) return 0;
+        if (m != 0) emin = std::min(emin, e);
+    }
+    if (emin == INT_MAX) emin = 0;
+    auto I = [&](double v) {
+        int64_t m = 0;
+        int e = 0;
+        split_exact(v, m, e);
+        return m == 0 ? BigZ() : bz_from(m, e - emin);
+    };
+    BigZ S[3];
+    BigZ S2;
+    for (int d = 0; d < dim; d++) {
+        S[d] = I(s[d]);
+        S2 = bz_add(S2, bz_mul(S[d], S[d]));
+    }
+    // matrix rows (normal..., w)
+    std::vector<BigZ> M((size_t)nr * nr);
+    for (int k = 0; k < nr; k++) {
+        BigZ* r = &M[(size_t)k * nr];
+        if (rows[k].box) {
+            const int ax = std::abs(rows[k].box) - 1;
+            const int sg = rows[k].box > 0 ? 1 : -1;
+            for (int d = 0; d < dim; d++) r[d] = d == ax ? bz_from(sg, 0) : BigZ();
+            const BigZ c = I(rows[k].box > 0 ? lo : hi);
+            r[dim] = sg > 0 ? bz_neg(c) : c;
+        } else {
+            BigZ B2;
+            for (int d = 0; d < dim; d++) {
+                const BigZ b = I(rows[k].p[d]);
+                const BigZ df = bz_add(S[d], bz_neg(b));
+                r[d] = bz_add(df, df);
+                B2 = bz_add(B2, bz_mul(b, b));
+            }
+            r[dim] = bz_add(B2, bz_neg(S2));
+        }
+    }
+    const BigZ det = bz_det(M, nr);
+    if (det.sg != 0) return det.sg;
+    // cofactors of the w column
+    std::vector<BigZ> C(nr);
+    for (int k = 0; k < nr; k++) {
+        std::vector<BigZ> sub;
+        for (int i = 0; i < nr; i++)
+            if (i != k)
+                for (int j = 0; j < dim; j++) sub.push_back(M[(size_t)i * nr + j]);
+        const BigZ m = bz_det(sub, dim);
+        C[k] = ((k + dim) & 1) ? bz_neg(m) : m;
+    }
+    // candidates (key, coefficient sign) in key order
+    std::vector<std::pair<uint32_t, int>> cand;
+    BigZ cs;
+    for (int k = 0; k < nr; k++)
+        if (!rows[k].box) {
+            cs = bz_add(cs, C[k]);
+            cand.push_back({rows[k].key, -C[k].sg});
+        }
+    cand.push_back({skey, cs.sg});
+    std::sort(cand.begin(), cand.end());
+    for (auto& c : cand)
+        if (c.second != 0) return c.second;
+    return 0;
+}
+
 // ------------------------------------------------------------------ cell
 // cell.cpp:42-411: the convex cell as planes + vertices (plane triples).
 struct Cell {
@@ -123,10 +351,20 @@ struct Cell {
     std::vector<int> pvid;
     std::vector<uint8_t> tri;  // 3 bytes per vertex (third unused in 2D)
     std::vector<uint8_t> bnext;
+    // exact-predicate fallback.  xmode: a test is trusted in floating point
+    // only beyond XTOL times the product of its rows' magnitudes (pmw: the
+    // magnitude the rounding of w is relative to); below, it is decided by
+    // exact_conflict with the tie-break keys per sorted id (keys set) or it
+    // flags the cell (keys nullptr).
+    const uint32_t* keys = nullptr;
+    bool xmode = false;
+    std::vector<double> pmw;
 
-    Cell(int dim_, int maxp, int maxt, const double* p, int sid)
-        : dim(dim_), max_p(maxp), max_t(maxt), pts(p), seed_id(sid) {
+    Cell(int dim_, int maxp, int maxt, const double* p, int sid, const uint32_t* keys_ = nullptr,
+         bool xmode_ = false)
+        : dim(dim_), max_p(maxp), max_t(maxt), pts(p), seed_id(sid), keys(keys_), xmode(xmode_ || keys_) {
         plane.assign(max_p, P4{0, 0, 0, 0});
+        pmw.assign(max_p, 0.0);
         pvid.assign(max_p, -1);
         tri.assign(3 * (size_t)max_t, 0);
         bnext.assign(max_p, END);
@@ -140,6 +378,7 @@ struct Cell {
         plane[2] = {0.0, 1.0, 0.0, -lo};
         plane[3] = {0.0, -1.0, 0.0, hi};
         if (dim == 2) {
+            for (int b = 0; b < 4; b++) pmw[b] = std::fabs(plane[b].w);
             const uint8_t t0[4][2] = {{2, 0}, {1, 2}, {3, 1}, {0, 3}};
             for (int t = 0; t < 4; t++) set_tri(t, t0[t][0], t0[t][1], 0);
             nb_v = 4;
@@ -147,6 +386,7 @@ struct Cell {
         } else {
             plane[4] = {0.0, 0.0, 1.0, -lo};
             plane[5] = {0.0, 0.0, -1.0, hi};
+            for (int b = 0; b < 6; b++) pmw[b] = std::fabs(plane[b].w);
             const uint8_t t0[8][3] = {{2, 5, 0}, {5, 3, 0}, {1, 5, 2}, {5, 1, 3},
                                       {4, 2, 0}, {4, 0, 3}, {2, 4, 1}, {4, 3, 1}};
             for (int t = 0; t < 8; t++) set_tri(t, t0[t][0], t0[t][1], t0[t][2]);
@@ -186,13 +426,35 @@ struct Cell {
         P4 ave2 = add4(seed, B);
         double dot = dot3(ave2, dir);
         plane[nb_v] = {dir.x, dir.y, dir.z, -dot / 2.0};
+        pmw[nb_v] = 0.5 * (std::fabs(ave2.x * dir.x) + std::fabs(ave2.y * dir.y) + std::fabs(ave2.z * dir.z));
         pvid[nb_v] = vid;
         nb_v++;
         return nb_v - 1;
     }
 
+    // the degenerate test of vertex t against plane cur, decided exactly
+    int exact_test(int t, int cur) {
+        ExRow rows[4];
+        for (int r = 0; r <= dim; r++) {
+            const int pid = r < dim ? tp(t, r) : cur;
+            ExRow& x = rows[r];
+            x.p[0] = x.p[1] = x.p[2] = 0.0;
+            x.key = 0;
+            x.box = 0;
+            if (pvid[pid] < 0) {
+                x.box = (pid & 1) ? -(pid / 2 + 1) : (pid / 2 + 1);
+            } else {
+                const double* f = pts + (size_t)dim * pvid[pid];
+                for (int d = 0; d < dim; d++) x.p[d] = f[d];
+                x.key = keys[pvid[pid]];
+            }
+        }
+        const double* sp = pts + (size_t)dim * seed_id;
+        return exact_conflict(dim, sp, keys[seed_id], rows);
+    }
+
     // cell.cpp:175-221
-    bool conflict(int t, const P4& eqn) {
+    bool conflict(int t, const P4& eqn, int cur) {
         const P4& p1 = plane[tp(t, 0)];
         const P4& p2 = plane[tp(t, 1)];
         double det, eps;
@@ -215,7 +477,22 @@ struct Cell {
             double mm = std::max(std::max(mx, my), mz);
             eps *= (mm * mm);
         }
-        if (std::fabs(det) < eps) status = needs_exact_predicates;
+        double xb = 0.0;
+        if (xmode) {
+            xb = 1e-12;  // XTOL of the device code (exact.h)
+            for (int r = 0; r <= dim; r++) {
+                const int pid = r < dim ? tp(t, r) : cur;
+                const P4& q = plane[pid];
+                xb *= std::fabs(q.x) + std::fabs(q.y) + std::fabs(q.z) + pmw[pid];
+            }
+        }
+        if (std::fabs(det) < eps || std::fabs(det) < xb) {
+            if (keys) {
+                const int r = exact_test(t, cur);
+                if (r != 0) return r > 0;
+            }
+            status = needs_exact_predicates;
+        }
         return det > 0.0;
     }
 
@@ -309,7 +586,7 @@ struct Cell {
         nb_r = 0;
         int i = 0;
         while (i < nb_t) {
-            if (conflict(i, eqn)) {
+            if (conflict(i, eqn, cur)) {
                 nb_t--;
                 swap_tri(i, nb_t);
                 nb_r++;
@@ -355,8 +632,11 @@ struct Cell {
         return r;
     }
 
-    // cell.cpp:369-384
-    bool radius_reached(const P4& last) {
+    // cell.cpp:369-384; with the exact fallback on, only neighbours clearly
+    // beyond twice the largest vertex distance certify (relative margin
+    // 1e-9): a tied neighbour can still cut a zero-area face under the
+    // symbolic perturbation
+    bool radius_reached(const P4& last, double margin = -1.0) {
         double vd = 0;
         for (int i = 0; i < nb_t; i++) {
             P4 pc = vertex(i);
@@ -364,8 +644,12 @@ struct Cell {
             vd = dmax(dot3(df, df), vd);
         }
         P4 df = sub4(last, seed);
-        return dot3(df, df) > 4 * vd;
+        if (margin < 0) margin = keys ? 1.0 + 1e-9 : 1.0;
+        return dot3(df, df) > 4 * vd * margin;
     }
+    // exact mode without the exact test: a certificate within the margin of
+    // a tie flags the cell for the exact pass
+    bool near_tie(const P4& last) { return radius_reached(last, 1.0) && !radius_reached(last, 1.0 + 1e-9); }
 };
 
 // Extracted geometry of one cell, faces in plane order (cell.cpp:416-470).
@@ -464,9 +748,18 @@ void extract(Cell& c, CellOut& out) {
             cy += ta * (c.seed.y + fv[0].y + fv[1].y) / 3.0;
         } else {
             {
-                P4 e1 = sub4(fv[1], fv[0]);
-                P4 e2 = sub4(fv[2], fv[0]);
-                P4 cr = cross3(e1, e2);
+                // orientation from the sum of the fan's cross products (the
+                // reference takes the first triangle's, cell.cpp:515-572: the
+                // same decision for a convex face, but also defined when the
+                // first vertices coincide, as on the zero-area faces of the
+                // exact-predicate fallback)
+                P4 cr{0, 0, 0, 0};
+                for (size_t i = 1; i + 1 < fv.size(); i++) {
+                    const P4 t = cross3(sub4(fv[i], fv[0]), sub4(fv[i + 1], fv[0]));
+                    cr.x += t.x;
+                    cr.y += t.y;
+                    cr.z += t.z;
+                }
                 P4 cen{0, 0, 0, 0};
                 for (const P4& q : fv) {
                     cen.x += q.x;
@@ -701,18 +994,24 @@ int64_t periodic_ghosts(int dim, const double* pts, int64_t n, double buff, doub
 // A row that holds every other seed (k == n - 1) certifies the cell by
 // exhaustion; the reference would report security_radius_not_reached there.
 int run_cell(int dim, int maxp, int maxt, const double* sorted, const uint32_t* row, int k, int sid,
-             CellOut* out, int* status, bool exhaustive) {
-    Cell c(dim, maxp, maxt, sorted, sid);
+             CellOut* out, int* status, bool exhaustive, const uint32_t* keys, bool xmode) {
+    Cell c(dim, maxp, maxt, sorted, sid, keys, xmode);
+    const bool flag_ties = xmode && !keys;
     int kc = 0;
     for (int v = 0; v < k; v++) {
         uint32_t z = row[v];
         c.clip((int)z);
         if (c.radius_reached(c.point((int)z))) {
-            kc = v + 1;
+            if (flag_ties && c.status == success && c.near_tie(c.point((int)z)))
+                c.status = needs_exact_predicates;
+            else
+                kc = v + 1;
             break;
         }
         if (c.status != success) break;
     }
+    if (flag_ties && c.status == success && c.near_tie(c.point((int)row[k - 1])))
+        c.status = needs_exact_predicates;
     if (!c.radius_reached(c.point((int)row[k - 1]))) {
         if (exhaustive && c.status == success)
             kc = k;
@@ -727,8 +1026,10 @@ int run_cell(int dim, int maxp, int maxt, const double* sorted, const uint32_t* 
 
 double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
+// exact != 0: degenerate conflict tests are decided by exact_conflict, with
+// the input index (ghosts: n + ghost index) as tie-break key.
 Mesh* build(int dim, const double* in_pts, int64_t n, int boundary, double buff, int k, int maxp,
-            int maxt, int restart) {
+            int maxt, int restart, int k_max = 1024, int exact = 0) {
     Mesh* m = new Mesh();
     double t0 = now();
     m->dim = dim;
@@ -757,9 +1058,7 @@ Mesh* build(int dim, const double* in_pts, int64_t n, int boundary, double buff,
     grid_sort(dim, pts.data(), ntot, N, m->sorted.data(), m->perm.data(), cnt.data(), start.data());
     int kmax = k;
     if (restart) {
-        const int sched[] = {16, 24, 32, 48, 64, 96, 128, 192, 256};
-        for (int s : sched)
-            if (s > kmax) kmax = s;
+        kmax = k_max;
         if (kmax > ntot - 1) kmax = (int)(ntot - 1);
         if (kmax < k) kmax = k;
     }
@@ -774,18 +1073,42 @@ Mesh* build(int dim, const double* in_pts, int64_t n, int boundary, double buff,
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t s = 0; s < ntot; s++)
         kc[s] = run_cell(dim, maxp, maxt, m->sorted.data(), m->knn.data() + (size_t)s * k, k, (int)s, &cells[s],
-                         &st[s], k >= ntot - 1);
+                         &st[s], k >= ntot - 1, nullptr, exact != 0);
     double t3 = now();
     // Restarts: a seed the first pass leaves uncertified is clipped again from
     // the first neighbour with a row twice as long, up to kmax.  Rows are
     // ordered by (dist^2, original id), so a longer row extends the shorter
     // one and the restarted cell is the cell a fixed k of that length gives.
-    if (restart) {
+    // exact != 0: afterwards the cells left with needs_exact_predicates are
+    // clipped again in exact mode from the first neighbour at k, and
+    // restarted the same way while uncertified.
+    const uint32_t* keys = nullptr;
+    for (int pass = 0; pass < (exact ? 2 : 1); pass++) {
         std::vector<int64_t> todo;
-        for (int64_t s = 0; s < ntot; s++)
-            if (st[s] != success) todo.push_back(s);
         std::vector<uint32_t> rows;
         int kk = k;
+        if (pass == 1) {
+            keys = m->perm.data();
+            for (int64_t s = 0; s < ntot; s++)
+                if (st[s] == needs_exact_predicates) todo.push_back(s);
+            const int64_t nq = (int64_t)todo.size();
+            if (nq == 0) break;
+#pragma omp parallel for schedule(dynamic, 16)
+            for (int64_t r = 0; r < nq; r++) {
+                const int64_t s = todo[r];
+                cells[s] = CellOut();
+                kc[s] = run_cell(dim, maxp, maxt, m->sorted.data(), m->knn.data() + (size_t)s * k, k, (int)s,
+                                 &cells[s], &st[s], k >= ntot - 1, keys, true);
+            }
+            std::vector<int64_t> left;
+            for (int64_t s : todo)
+                if (st[s] != success) left.push_back(s);
+            todo.swap(left);
+        } else {
+            for (int64_t s = 0; s < ntot; s++)
+                if (st[s] != success) todo.push_back(s);
+        }
+        if (!restart) continue;
         while (!todo.empty() && kk < kmax) {
             kk = std::min(kmax, 2 * kk);
             const int64_t nq = (int64_t)todo.size();
@@ -797,7 +1120,7 @@ Mesh* build(int dim, const double* in_pts, int64_t n, int boundary, double buff,
                 const int64_t s = todo[r];
                 cells[s] = CellOut();
                 kc[s] = run_cell(dim, maxp, maxt, m->sorted.data(), rows.data() + (size_t)r * kk, kk, (int)s,
-                                 &cells[s], &st[s], kk >= ntot - 1);
+                                 &cells[s], &st[s], kk >= ntot - 1, keys, exact != 0);
             }
             std::vector<int64_t> left;
             for (int64_t s : todo)
@@ -1174,7 +1497,7 @@ double cell_dt(const Mesh& m, int64_t i, double gam, double cfl, const double* r
 // A moving-mesh simulation: generators x, extensive FP64 state (mass,
 // momentum, total energy) per hydro cell, and the current mesh.
 struct Sim {
-    int dim = 3, boundary = 1, k = 32, flux_mode = 1, move = 1;
+    int dim = 3, boundary = 1, k = 32, flux_mode = 1, move = 1, exact = 0;
     double buff = 0.04, gam = 1.4, cfl = 0.3;
     int64_t n = 0;
     std::vector<double> x, mass, mom, en, rho, vel, E, wgen;
@@ -1182,7 +1505,7 @@ struct Sim {
     ~Sim() { delete mesh; }
     bool rebuild() {
         delete mesh;
-        mesh = build(dim, x.data(), n, boundary, buff, k, 64, 96, 1);
+        mesh = build(dim, x.data(), n, boundary, buff, k, 64, 96, 1, 1024, exact);
         return mesh->n_failed == 0;
     }
     // primitives from the conserved state and the current volumes
@@ -1376,10 +1699,22 @@ int or_knn(int dim, const double* sorted, int64_t n, int N_grid, const int32_t* 
 }
 
 void* or_mesh_build(int dim, const double* pts, int64_t n, int boundary, double buff, int k,
-                    int max_p, int max_t, int restart) {
+                    int max_p, int max_t, int restart, int k_max, int exact) {
     if ((dim != 2 && dim != 3) || n < 2 || k < 1 || k > n - 1 || max_p > 255 || max_t > 255)
         return nullptr;
-    return build(dim, pts, n, boundary, buff, k, max_p, max_t, restart);
+    return build(dim, pts, n, boundary, buff, k, max_p, max_t, restart, k_max, exact);
+}
+
+int or_exact_conflict(int dim, const double* seed, uint32_t skey, const int* box, const double* pts,
+                      const uint32_t* keys) {
+    if (dim != 2 && dim != 3) return -2;
+    ExRow rows[4];
+    for (int r = 0; r <= dim; r++) {
+        rows[r].box = box[r];
+        rows[r].key = keys[r];
+        for (int d = 0; d < 3; d++) rows[r].p[d] = d < dim ? pts[3 * r + d] : 0.0;
+    }
+    return exact_conflict(dim, seed, skey, rows);
 }
 
 void or_mesh_free(void* h) { delete (Mesh*)h; }
@@ -1533,7 +1868,7 @@ int or_cell_fluxes(void* h, double gam, int flux_mode, int move, const double* r
 }
 
 void* or_sim_create(int dim, const double* pts, const double* rho, const double* vel, const double* E, int64_t n,
-                    int boundary, double buff, int k, double gamma, double cfl, int flux_mode, int move) {
+                    int boundary, double buff, int k, double gamma, double cfl, int flux_mode, int move, int exact) {
     if ((dim != 2 && dim != 3) || n < 2 || k < 1 || k > n - 1 || (boundary != 0 && boundary != 1) ||
         move < 0 || move > 2 || (flux_mode != 0 && flux_mode != 1))
         return nullptr;
@@ -1543,6 +1878,7 @@ void* or_sim_create(int dim, const double* pts, const double* rho, const double*
     S->k = k;
     S->flux_mode = flux_mode;
     S->move = move;
+    S->exact = exact;
     S->buff = buff;
     S->gam = gamma;
     S->cfl = cfl;

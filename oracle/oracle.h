@@ -44,11 +44,21 @@ int or_kfeatures(int dim, const double* sorted, int64_t n, int N_grid, const int
 /* Mesh build.  boundary: 0 = periodic ghosts (periodic_mesh.cpp:44), 1 = the
  * unit box as reflective walls (no ghosts, no scaling), 2 = plain
  * compute_mesh on the given points (voronoi.cpp:17).  k: neighbours per seed.
- * restart != 0: seeds that fail the certificate are recomputed from a longer
- * neighbour list (k schedule 16,24,32,48,64,96,128,192,256 above k).
+ * restart != 0: seeds that fail the certificate are recomputed from a
+ * neighbour list twice as long, up to k_max.  exact != 0: degenerate
+ * conflict tests (needs_exact_predicates) are decided by or_exact_conflict's
+ * rule, keys = input index (ghosts: n + ghost index).
  * Returns an opaque handle or NULL. */
 void* or_mesh_build(int dim, const double* pts, int64_t n, int boundary, double buff, int k,
-                    int max_p, int max_t, int restart);
+                    int max_p, int max_t, int restart, int k_max, int exact);
+/* The exact conflict test of the exact-predicate fallback (no reference
+ * counterpart): vertex of rows 0..dim-1 against row dim, for the seed at
+ * `seed` with tie-break key skey.  Row r: box[r] = 0 and point pts[3r..] with
+ * key keys[r] (bisector), or box[r] = +-(axis + 1) for the initial cell's
+ * box plane x_axis >= -1e-14 (+) / x_axis <= 1 + 1e-14 (-).  Returns +1
+ * (conflict), -1 (no conflict), 0 (a subnormal or non-finite value). */
+int or_exact_conflict(int dim, const double* seed, uint32_t skey, const int* box, const double* pts,
+                      const uint32_t* keys);
 void or_mesh_free(void* h);
 /* Build stage times in seconds: [0] ghosts + grid sort, [1] KNN rows, [2]
  * first clip pass, [3] certificate restarts. */
@@ -103,10 +113,11 @@ int or_cell_fluxes(void* mesh, double gamma, int flux_mode, int move, const doub
 /* Moving-mesh simulation: conserved FP64 state, one step = primitives,
  * (steered velocities), CFL dt, flux update, drift, rebuild (k with
  * certificate restarts, max_p 64, max_t 96).  boundary 0 periodic, 1
- * reflective; move 0 static, 1 with the fluid, 2 regularised.  NULL when the
- * arguments are invalid or the first build leaves cells uncertified. */
+ * reflective; move 0 static, 1 with the fluid, 2 regularised; exact as in
+ * or_mesh_build.  NULL when the arguments are invalid or the first build
+ * leaves cells uncertified. */
 void* or_sim_create(int dim, const double* pts, const double* rho, const double* vel, const double* E, int64_t n,
-                    int boundary, double buff, int k, double gamma, double cfl, int flux_mode, int move);
+                    int boundary, double buff, int k, double gamma, double cfl, int flux_mode, int move, int exact);
 /* Returns the step taken (capped at dt_max > 0), -1 when the rebuild leaves
  * cells uncertified. */
 double or_sim_step(void* sim, double dt_max);

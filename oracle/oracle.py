@@ -33,7 +33,9 @@ def lib():
         L.or_kfeatures.argtypes = [C.c_int, dp, C.c_int64, C.c_int, i32p, i32p,
                                    np.ctypeslib.ndpointer(dtype=np.float32, flags="C")]
         L.or_mesh_build.argtypes = [C.c_int, dp, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_int,
-                                    C.c_int, C.c_int]
+                                    C.c_int, C.c_int, C.c_int, C.c_int]
+        L.or_exact_conflict.argtypes = [C.c_int, dp, C.c_uint32, i32p, dp, u32p]
+        L.or_exact_conflict.restype = C.c_int
         L.or_mesh_build.restype = C.c_void_p
         L.or_mesh_free.argtypes = [C.c_void_p]
         L.or_mesh_sizes.argtypes = [C.c_void_p, i64p]
@@ -53,7 +55,7 @@ def lib():
         L.or_face_velocity.restype = None
         L.or_cell_fluxes.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, dp, dp, dp, C.c_void_p, dp]
         L.or_sim_create.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int64, C.c_int, C.c_double, C.c_int,
-                                    C.c_double, C.c_double, C.c_int, C.c_int]
+                                    C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
         L.or_sim_create.restype = C.c_void_p
         L.or_sim_step.argtypes = [C.c_void_p, C.c_double]
         L.or_sim_step.restype = C.c_double
@@ -182,13 +184,32 @@ _I8 = ("face_counts", "face_ptr", "ghost_ids")
 _I4 = ("neighbor_cell", "status", "kcert")
 
 
+def exact_conflict(seed, skey, rows):
+    """The exact-predicate fallback's conflict test (or_exact_conflict).
+    rows: dim + 1 entries, each ("box", axis, +1 | -1) or (point, key); the
+    first dim rows are the vertex's planes, the last the new plane.  Returns
+    +1 (conflict), -1 (no conflict), 0 (undecided)."""
+    seed = np.ascontiguousarray(seed, dtype=np.float64)
+    dim = len(seed)
+    box = np.zeros(dim + 1, dtype=np.int32)
+    pts = np.zeros((dim + 1) * 3, dtype=np.float64)
+    keys = np.zeros(dim + 1, dtype=np.uint32)
+    for r, row in enumerate(rows):
+        if isinstance(row[0], str):
+            box[r] = (row[1] + 1) * row[2]
+        else:
+            pts[3 * r:3 * r + dim] = row[0]
+            keys[r] = row[1]
+    return lib().or_exact_conflict(dim, seed, int(skey), box, pts, keys)
+
+
 class Mesh:
     """Oracle mesh in the reference VMesh layout (original order)."""
 
     BOUNDARY = {"periodic": 0, "reflective": 1, "none": 2}
 
     def __init__(self, pts, boundary="periodic", buff=0.04, k=32, max_p=64, max_t=96, restart=False,
-                 _borrow=None):
+                 k_max=1024, exact=False, _borrow=None):
         self._owner = None
         if _borrow is not None:
             # a mesh owned by an oracle Sim (kept alive through _owner)
@@ -198,7 +219,7 @@ class Mesh:
             n, dim = pts.shape
             self.dim = dim
             self.h = lib().or_mesh_build(dim, pts, n, self.BOUNDARY[boundary], buff, k, max_p, max_t,
-                                         int(restart))
+                                         int(restart), int(k_max), int(exact))
             if not self.h:
                 raise ValueError("oracle mesh build rejected its arguments")
         s = np.zeros(6, dtype=np.int64)
@@ -259,14 +280,14 @@ class Sim:
     MOVE = {"static": 0, "fluid": 1, "regularized": 2}
 
     def __init__(self, pts, rho, vel, E, boundary="reflective", buff=0.04, k=32, gamma=1.4, cfl=0.3,
-                 flux="physical", move="fluid"):
+                 flux="physical", move="fluid", exact=False):
         pts = np.ascontiguousarray(pts, dtype=np.float64)
         self.n, self.dim = pts.shape
         self.h = lib().or_sim_create(self.dim, pts, np.ascontiguousarray(rho, dtype=np.float64),
                                      np.ascontiguousarray(vel, dtype=np.float64),
                                      np.ascontiguousarray(E, dtype=np.float64), self.n,
                                      Mesh.BOUNDARY[boundary], buff, k, gamma, cfl,
-                                     1 if flux == "physical" else 0, self.MOVE[move])
+                                     1 if flux == "physical" else 0, self.MOVE[move], int(exact))
         if not self.h:
             raise ValueError("oracle simulation rejected its arguments or its first mesh")
 

@@ -29,14 +29,14 @@ class PgError(RuntimeError):
 class MeshParams(C.Structure):
     _fields_ = [("dim", C.c_int), ("boundary", C.c_int), ("buff", C.c_double), ("k", C.c_int),
                 ("max_p", C.c_int), ("max_t", C.c_int), ("restart", C.c_int), ("k_max", C.c_int),
-                ("perturb", C.c_double)]
+                ("perturb", C.c_double), ("exact", C.c_int)]
 
 
 class MeshInfo(C.Structure):
     _fields_ = [("n_seeds", C.c_int64), ("n_hydro", C.c_int64), ("num_faces", C.c_int64),
                 ("n_failed", C.c_int64), ("n_restarted", C.c_int64), ("N_grid", C.c_int),
                 ("k_used", C.c_int), ("row_entries", C.c_int64), ("n_class", C.c_int64 * 6),
-                ("n_degenerate", C.c_int64), ("displacement", C.c_double)]
+                ("n_degenerate", C.c_int64), ("displacement", C.c_double), ("n_exact", C.c_int64)]
 
 
 class VMeshView(C.Structure):

@@ -134,12 +134,13 @@ class KnnProblem:
 
 
 def mesh_params(dim=3, boundary="periodic", buff=0.04, k=32, max_p=64, max_t=96, restart=True,
-                k_max=256, perturb=0.0) -> MeshParams:
+                k_max=1024, perturb=0.0, exact=False) -> MeshParams:
     p = MeshParams()
     lib().pg_mesh_params_default(C.byref(p))
     p.dim, p.boundary, p.buff, p.k = dim, BOUNDARY[boundary], buff, k
     p.max_p, p.max_t, p.restart, p.k_max = max_p, max_t, int(restart), k_max
     p.perturb = perturb
+    p.exact = int(exact)
     return p
 
 
@@ -215,13 +216,15 @@ class Mesh:
     kmodel: a trained KModel picks each seed's first k (learned k).
     perturb > 0: degeneracy fallback (builds that leave cells with
     degenerate predicates are repeated on deterministically displaced
-    generators; `displacement` reports the amplitude used)."""
+    generators; `displacement` reports the amplitude used).
+    exact: cells with degenerate predicates are clipped again with the
+    exact predicate and symbolic tie-breaking (`n_exact` counts them)."""
 
     def __init__(self, ctx: Context, pts: np.ndarray, boundary="periodic", buff=0.04, k=32, max_p=64,
-                 max_t=96, restart=True, k_max=256, kmodel: KModel | None = None, perturb=0.0):
+                 max_t=96, restart=True, k_max=1024, kmodel: KModel | None = None, perturb=0.0, exact=False):
         pts = np.ascontiguousarray(pts, dtype=np.float64)
         self.ctx, self.dim, self.k = ctx, pts.shape[1], k
-        prm = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max, perturb)
+        prm = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max, perturb, exact)
         h = C.c_void_p()
         km = kmodel.h if kmodel is not None else None
         check(lib().pg_mesh_compute_k(ctx.h, C.byref(prm), km, _ptr(pts), pts.shape[0], C.byref(h)))
@@ -257,6 +260,7 @@ class Mesh:
         self.row_entries = info.row_entries
         self.n_class = np.array(info.n_class[:], dtype=np.int64)
         self.n_degenerate, self.displacement = info.n_degenerate, info.displacement
+        self.n_exact = info.n_exact
 
     def features(self):
         """Learned-k features of every seed, original order (n_seeds, 16)."""
@@ -379,12 +383,12 @@ class Simulation:
 
     def __init__(self, ctx: Context, pts, rho, vel, E, boundary="reflective", buff=0.04, k=32,
                  gamma=5.0 / 3.0, cfl=0.4, flux="physical", move_mesh=True, max_p=64, max_t=96,
-                 restart=True, k_max=256, perturb=0.0):
+                 restart=True, k_max=1024, perturb=0.0, exact=False):
         from ._native import SimParams
         pts = np.ascontiguousarray(pts, dtype=np.float64)
         self.ctx, self.n, self.dim = ctx, pts.shape[0], pts.shape[1]
         p = SimParams()
-        p.mesh = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max, perturb)
+        p.mesh = mesh_params(self.dim, boundary, buff, k, max_p, max_t, restart, k_max, perturb, exact)
         p.gamma, p.cfl, p.flux_mode, p.move_mesh = gamma, cfl, FLUX[flux], move_mode(move_mesh)
         arrs = _state_arrays(rho, vel, E, self.n, self.dim)
         h = C.c_void_p()

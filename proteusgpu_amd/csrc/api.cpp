@@ -80,7 +80,11 @@ struct DBuf {
 
 int ngrid(int dim, int64_t n) { return pg::ngrid_of(dim, n); }
 
-const int KSCHED[] = {16, 24, 32, 48, 64, 96, 128, 192, 256};
+// Row lengths of the first pass and of certificate restarts.  Rows beyond
+// 256 serve seeds beside dense clusters (their security radius holds the
+// whole cluster); the wave KNN keeps up to KNN_MAX_K entries.
+const int KSCHED[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024};
+constexpr int KNN_MAX_K = 1024;
 constexpr int MAXP = 64;
 // offset pattern of the degeneracy fallback's t-th retry: salt (t + 1) x this
 constexpr uint64_t DISPLACE_SALT = 0xD1B54A32D192ED03ull;
@@ -162,6 +166,11 @@ struct pg_mesh {
     uint32_t* out_order() { return sorted_space ? iperm.as<uint32_t>() : g.perm.as<uint32_t>(); }
     // hydro tables
     DBuf hid, area_u, vol_u, o2s, bad;
+    // owner-computes-once update: hydro code per sorted seed, twin code per
+    // face slot, and scratch for the sorted cell records and the per-slot
+    // face fluxes
+    DBuf hcode, nbr_cm, twin, solve, frec, fbuf;
+    bool want_cm = false;  // the clip keeps nbr_cm (meshes of simulations)
     bool finalized = false;
     pg_mesh_info_t info{};
 
@@ -182,6 +191,10 @@ struct pg_mesh {
             fa.resume = resume.as<int32_t>();
             fa.lastnb = lastnb.as<uint32_t>();
         }
+        // exact-predicate builds: every pass flags the tests that rounding
+        // could decide wrongly (the tie-break keys switch that mode on)
+        if (prm.exact) fa.key = g.perm.as<uint32_t>();
+        if (want_cm) fa.nbr_cm = nbr_cm.as<int32_t>();
         return fa;
     }
     double scale() const { return prm.boundary == PG_BOUNDARY_PERIODIC ? 1. + (2 * prm.buff) : 1.0; }
@@ -225,15 +238,24 @@ int64_t collect_list(pg_mesh* m, const uint32_t* seeds, int64_t nq, int code, ui
 // The shape tag under which the fast clip kernel saves and resumes cells.
 int32_t clip_shape_tag(int lanes, int slots) { return (lanes << 24) | ((slots / lanes) << 16); }
 
-void run_list(pg_mesh* m, const uint32_t* list, int64_t nq, int k, int big, hipStream_t s) {
+void run_list(pg_mesh* m, const uint32_t* list, int64_t nq, int k, int big, hipStream_t s, bool exact = false) {
     uint32_t* rows = m->rows.get<uint32_t>((size_t)nq * k);
     pg::KnnTail tail;  // seeds that resume need only the rest of their row
-    if (m->use_resume && !big)
+    if (m->use_resume && !big && !exact)
         tail = {m->resume.as<int32_t>(), m->lastnb.as<uint32_t>(), clip_shape_tag(m->lanes, m->slots)};
     pg::launch_knn(m->prm.dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), m->n_seeds, m->g.N,
                    m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), list, nq, k, rows, s, tail);
+    pg::FaceArrays fa = m->faces();
+    if (exact) {
+        // the exact kernel neither saves nor resumes cells; ties are broken
+        // by the input index
+        fa.cstate = nullptr;
+        fa.resume = nullptr;
+        fa.lastnb = nullptr;
+        fa.key = m->g.perm.as<uint32_t>();
+    }
     pg::launch_voronoi(m->prm.dim, m->g.sorted.as<double>(), m->n_seeds, list, nq, rows, k,
-                       m->prm.max_p, m->prm.max_t, big, m->faces(), m->lanes, m->slots, s, true);
+                       m->prm.max_p, m->prm.max_t, big, fa, m->lanes, m->slots, s, true, exact);
     HIPCHK(hipGetLastError());
     m->info.row_entries += nq * k;
 }
@@ -261,7 +283,7 @@ int restart_chain(pg_mesh* m, const uint32_t* cur, int64_t nq, int k, hipStream_
     const size_t ns = (size_t)m->n_seeds;
     uint32_t* buf[2] = {m->list.get<uint32_t>(ns), m->list2.get<uint32_t>(ns)};
     uint32_t* ov = m->list3.get<uint32_t>(ns);
-    const int cap = (int)std::min<int64_t>(std::max(p.k_max, k), m->n_seeds - 1);
+    const int cap = (int)std::min<int64_t>(std::min(std::max(p.k_max, k), KNN_MAX_K), m->n_seeds - 1);
     const uint32_t* L = cur;
     int64_t nL = nq;
     int kmax = k;
@@ -281,6 +303,38 @@ int restart_chain(pg_mesh* m, const uint32_t* cur, int64_t nq, int k, hipStream_
         run_list(m, dst, nr, k, 0, s);
         m->info.n_restarted += nr;
         kmax = std::max(kmax, k);
+        L = dst;
+        nL = nr;
+    }
+    return kmax;
+}
+
+// Exact-predicate fallback (pg_mesh_params.exact): seeds left with
+// needs_exact_predicates are clipped again by the exact kernel from the
+// first neighbour at the configured k, then at growing k while uncertified
+// (the schedule of restart_chain).  Returns the largest k used.
+int exact_chain(pg_mesh* m, hipStream_t s) {
+    const pg_mesh_params& p = m->prm;
+    const size_t ns = (size_t)m->n_seeds;
+    uint32_t* buf[2] = {m->list.get<uint32_t>(ns), m->list2.get<uint32_t>(ns)};
+    int64_t nL = collect_list(m, nullptr, m->n_seeds, PG_NEEDS_EXACT_PREDICATES, buf[0], s);
+    m->info.n_exact = nL;
+    if (nL == 0) return 0;
+    const int cap = (int)std::min<int64_t>(std::min(std::max(p.k_max, p.k), KNN_MAX_K), m->n_seeds - 1);
+    int k = (int)std::min<int64_t>(p.k, m->n_seeds - 1);
+    uint32_t* L = buf[0];
+    run_list(m, L, nL, k, 1, s, true);
+    int kmax = k;
+    while (p.restart) {
+        const int kn = next_k(k, cap);
+        if (kn < 0) break;
+        uint32_t* dst = (L == buf[0]) ? buf[1] : buf[0];
+        const int64_t nr = collect_list(m, L, nL, PG_SECURITY_RADIUS_NOT_REACHED, dst, s);
+        if (nr == 0) break;
+        k = kn;
+        run_list(m, dst, nr, k, 1, s, true);
+        m->info.n_restarted += nr;
+        kmax = k;
         L = dst;
         nL = nr;
     }
@@ -320,6 +374,7 @@ void alloc_cells(pg_mesh* m, size_t ns) {
     m->ccen.get<double>(3 * ns);
     m->status.get<int32_t>(ns);
     m->kcert.get<int32_t>(ns);
+    if (m->want_cm) m->nbr_cm.get<int32_t>(ns * MAXP);
 }
 
 // One build of the mesh of pts (device memory when on_device).  count:
@@ -461,6 +516,7 @@ void mesh_build_once(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool o
         k_used = std::max(restart_chain(m, nullptr, nseeds, std::min<int64_t>(64, nseeds - 1), s),
                           *std::max_element(bk, bk + pg::LK_NCLASS));
     }
+    if (p.exact) k_used = std::max(k_used, exact_chain(m, s));
     if (mk) mk->mark(MK_RESTART, s);
     m->info.n_seeds = nseeds;
     m->info.n_hydro = m->n_hydro;
@@ -529,7 +585,7 @@ void mesh_finalize(pg_ctx* c, pg_mesh* m) {
                         m->count.as<uint8_t>(), m->nbr.as<int32_t>(), m->area.as<double>(),
                         m->vol.as<double>(), as, vs, m->hid.get<int32_t>(ns * MAXP),
                         m->area_u.get<double>(ns * MAXP), m->vol_u.get<double>(ns),
-                        m->o2s.get<int32_t>(std::max<int64_t>(m->n_hydro, 1)), bad, s);
+                        m->o2s.get<int32_t>(std::max<int64_t>(m->n_hydro, 1)), m->hcode.get<int32_t>(ns), bad, s);
     HIPCHK(hipGetLastError());
     // faces over all cells (VMesh.num_faces), read with the error counters:
     // meshes built on the device (simulations) learn their face count here
@@ -788,8 +844,9 @@ void pg_mesh_params_default(pg_mesh_params* p) {
     p->max_p = 64;
     p->max_t = 96;
     p->restart = 1;
-    p->k_max = 256;
+    p->k_max = KNN_MAX_K;
     p->perturb = 0.0;
+    p->exact = 0;
 }
 
 int pg_mesh_compute_k(pg_ctx* c, const pg_mesh_params* prm, const pg_kmodel* km, const double* pts, int64_t n,
@@ -801,11 +858,13 @@ int pg_mesh_compute_k(pg_ctx* c, const pg_mesh_params* prm, const pg_kmodel* km,
         argchk(prm->dim == 2 || prm->dim == 3, "dim must be 2 or 3");
         argchk(prm->boundary >= 0 && prm->boundary <= 2, "unknown boundary");
         argchk(prm->k >= 1 && prm->k <= 256, "k must be in [1, 256]");
+        argchk(prm->k_max <= KNN_MAX_K, "k_max must be at most 1024");
         argchk(prm->max_p >= 2 * prm->dim + 1 && prm->max_p <= MAXP, "max_p must be in [2*dim+1, 64]");
         argchk(prm->max_t >= 8 && prm->max_t <= 128, "max_t must be in [8, 128]");
         argchk(prm->boundary != PG_BOUNDARY_PERIODIC || (prm->buff > 0 && prm->buff < 0.5),
                "buff must be in (0, 0.5)");
         argchk(prm->perturb >= 0.0 && prm->perturb <= 1e-3, "perturb must be in [0, 1e-3]");
+        argchk(prm->exact == 0 || prm->exact == 1, "exact must be 0 or 1");
         argchk(n >= 2 && n < (int64_t)1 << 29, "n out of range");
         pg_mesh* m = new pg_mesh();
         try {
@@ -1198,6 +1257,10 @@ void sim_rebuild(pg_ctx* c, pg_sim* S, Marks* mk) {
         throw Error{PG_ERR_CELLS, std::to_string(S->mesh.info.n_failed) + " cells left uncertified"};
     S->mesh.finalized = false;
     mesh_finalize(c, &S->mesh);
+    pg_mesh& m = S->mesh;
+    pg::launch_face_twins(nullptr, m.n_seeds, m.n_seeds, m.hcode.as<int32_t>(), m.count.as<uint8_t>(),
+                          m.nbr_cm.as<int32_t>(), m.twin.get<uint8_t>((size_t)m.n_seeds * MAXP),
+                          m.solve.get<uint64_t>((size_t)m.n_seeds), c->stream);
     pg::launch_keep_kcert(nullptr, S->mesh.n_seeds, S->mesh.g.perm.as<uint32_t>(), nullptr,
                           S->mesh.kcert.as<int32_t>(), S->n, S->kc_prev.get<int32_t>((size_t)S->n), c->stream);
     HIPCHK(hipGetLastError());
@@ -1227,10 +1290,31 @@ void sim_hydro(pg_ctx* c, pg_sim* S, double dt_max, bool moving) {
     if (w) cm = {m.count.as<uint8_t>(), m.nbr.as<int32_t>(), m.g.sorted.as<double>(), m.n_seeds, sc};
     pg::launch_dt_cfl(dim, S->n, m.o2s.as<int32_t>(), m.vol_u.as<double>(), S->prm.gamma, S->prm.cfl,
                       S->rho.as<double>(), S->vel.as<double>(), S->E.as<double>(), d, s, w, cm);
-    pg::launch_hll_ale(dim, m.n_seeds, m.n_hydro, m.n_seeds, m.n_seeds * MAXP, m.out_order(),
-                       m.count.as<uint8_t>(), m.hid.as<int32_t>(), m.nbr.as<int32_t>(), m.nrm.as<double>(),
-                       m.area_u.as<double>(), m.cen.as<double>(), m.g.sorted.as<double>(), sc, S->prm.gamma,
-                       (const double*)d, dt_max, S->prm.flux_mode, moving ? 1 : 0, S->view(), s, w);
+    const size_t nplane = (size_t)m.n_seeds * MAXP;
+    pg::OwnedUpdate u;
+    u.n = m.n_seeds;
+    u.stride = m.n_seeds;
+    u.nplane = (int64_t)nplane;
+    u.hcode = m.hcode.as<int32_t>();
+    u.count = m.count.as<uint8_t>();
+    u.twin = m.twin.as<uint8_t>();
+    u.solve = m.solve.as<uint64_t>();
+    u.nbr = m.nbr.as<int32_t>();
+    u.nrm = m.nrm.as<double>();
+    u.area_u = m.area_u.as<double>();
+    u.cen = m.cen.as<double>();
+    u.sorted = m.g.sorted.as<double>();
+    u.scale = sc;
+    u.gam = S->prm.gamma;
+    u.flux_mode = S->prm.flux_mode;
+    u.move = moving ? 1 : 0;
+    u.dt_dev = (const double*)d;
+    u.dt_max = dt_max;
+    u.sv = S->view();
+    u.wgen = w;
+    u.rec = m.frec.get<pg::CellRec>((size_t)m.n_seeds);
+    u.fbuf = m.fbuf.get<double>((size_t)(dim + 2) * nplane);
+    pg::launch_hll_owned(dim, u, s);
     if (moving)
         pg::launch_drift(dim, S->n, S->x.as<double>(), w ? w : S->vel.as<double>(), (const double*)d, dt_max,
                          S->prm.mesh.boundary == PG_BOUNDARY_PERIODIC ? 1 : 0, s);
@@ -1259,6 +1343,7 @@ int pg_sim_create(pg_ctx* c, const pg_sim_params* prm, const double* pts, const 
         argchk(mp.boundary == PG_BOUNDARY_PERIODIC || mp.boundary == PG_BOUNDARY_REFLECTIVE,
                "a simulation needs a periodic or reflective box");
         argchk(mp.k >= 1 && mp.k <= 256, "k must be in [1, 256]");
+        argchk(mp.k_max <= KNN_MAX_K, "k_max must be at most 1024");
         argchk(mp.max_p >= 2 * mp.dim + 1 && mp.max_p <= MAXP, "max_p must be in [2*dim+1, 64]");
         argchk(mp.max_t >= 8 && mp.max_t <= 128, "max_t must be in [8, 128]");
         argchk(mp.boundary != PG_BOUNDARY_PERIODIC || (mp.buff > 0 && mp.buff < 0.5), "buff must be in (0, 0.5)");
@@ -1266,12 +1351,14 @@ int pg_sim_create(pg_ctx* c, const pg_sim_params* prm, const double* pts, const 
         argchk(prm->flux_mode == PG_FLUX_REFERENCE || prm->flux_mode == PG_FLUX_PHYSICAL, "unknown flux mode");
         argchk(prm->move_mesh >= PG_MOVE_STATIC && prm->move_mesh <= PG_MOVE_REGULARIZED, "unknown move_mesh mode");
         argchk(mp.perturb >= 0.0 && mp.perturb <= 1e-3, "perturb must be in [0, 1e-3]");
+        argchk(mp.exact == 0 || mp.exact == 1, "exact must be 0 or 1");
         argchk(n >= 2 && n < (int64_t)1 << 29, "n out of range");
         pg_sim* S = new pg_sim();
         try {
             S->prm = *prm;
             S->mesh.prm = prm->mesh;
             if (S->mesh.prm.k_max < S->mesh.prm.k) S->mesh.prm.k_max = S->mesh.prm.k;
+            S->mesh.want_cm = true;  // face ownership of the moving-mesh update
             S->n = n;
             const int dim = mp.dim;
             const size_t nn = (size_t)n;
@@ -1661,6 +1748,7 @@ struct pg_batch {
     DBuf cstate, resume, lastnb;  // saved cells of uncertified seeds (restarts resume them)
     DBuf ticket;          // work-queue counter of the persistent kernels
     DBuf hid, area_u, vol_u, o2s, hsorted, tot;
+    DBuf hcode, nbr_cm, twin, solve, frec, fbuf;  // owner-computes-once update
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float last_ms = 0.f;
@@ -1690,6 +1778,7 @@ struct pg_batch {
             fa.resume = resume.as<int32_t>();
             fa.lastnb = lastnb.as<uint32_t>();
         }
+        fa.nbr_cm = nbr_cm.as<int32_t>();
         return fa;
     }
     double scale() const { return periodic ? 1. + (2 * prm.mesh.buff) : 1.0; }
@@ -1796,7 +1885,9 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
     pg::launch_bfinalize(n_dev, cap, cap, B->perm.as<uint32_t>(), B->orig_h.as<int32_t>(), B->count.as<uint8_t>(),
                          B->nbr.as<int32_t>(), B->area.as<double>(), B->vol.as<double>(), as, vs, B->periodic ? 1 : 0,
                          B->hid.as<int32_t>(), B->area_u.as<double>(), B->vol_u.as<double>(), B->o2s.as<int32_t>(),
-                         B->hsorted.as<int32_t>(), B->st() + 1, s);
+                         B->hsorted.as<int32_t>(), B->hcode.as<int32_t>(), B->st() + 1, s);
+    pg::launch_face_twins(n_dev, cap, cap, B->hcode.as<int32_t>(), B->count.as<uint8_t>(), B->nbr_cm.as<int32_t>(),
+                          B->twin.as<uint8_t>(), B->solve.as<uint64_t>(), s);
     pg::launch_bcount_failed(n_dev, cap, B->status.as<int32_t>(), B->st() + 3, s);
     if (B->km && lk_order_on())
         pg::launch_keep_kcert(n_dev, cap, B->perm.as<uint32_t>(), B->orig_h.as<int32_t>(), B->kcert.as<int32_t>(),
@@ -1822,10 +1913,31 @@ void batch_hydro(pg_batch* B, hipStream_t s) {
                    w ? pg::CflMesh{B->count.as<uint8_t>(), B->nbr.as<int32_t>(), B->sorted.as<double>(), B->cap,
                                    B->scale()}
                      : pg::CflMesh{});
-    pg::launch_bhll(dim, B->n_dev(), B->cap, B->cap, B->cap * MAXP, B->hsorted.as<int32_t>(), B->hbox.as<int32_t>(),
-                    B->count.as<uint8_t>(), B->hid.as<int32_t>(), B->nbr.as<int32_t>(), B->nrm.as<double>(),
-                    B->area_u.as<double>(), B->cen.as<double>(), B->sorted.as<double>(), B->scale(), B->prm.gamma,
-                    B->dt_box.as<double>(), B->prm.flux_mode, move ? 1 : 0, B->view(), s, w);
+    pg::OwnedUpdate u;
+    u.n_dev = B->n_dev();
+    u.n = B->cap;
+    u.stride = B->cap;
+    u.nplane = B->cap * MAXP;
+    u.hcode = B->hcode.as<int32_t>();
+    u.count = B->count.as<uint8_t>();
+    u.twin = B->twin.as<uint8_t>();
+    u.solve = B->solve.as<uint64_t>();
+    u.nbr = B->nbr.as<int32_t>();
+    u.nrm = B->nrm.as<double>();
+    u.area_u = B->area_u.as<double>();
+    u.cen = B->cen.as<double>();
+    u.sorted = B->sorted.as<double>();
+    u.scale = B->scale();
+    u.gam = B->prm.gamma;
+    u.flux_mode = B->prm.flux_mode;
+    u.move = move ? 1 : 0;
+    u.dt_box = B->dt_box.as<double>();
+    u.hbox = B->hbox.as<int32_t>();
+    u.sv = B->view();
+    u.wgen = w;
+    u.rec = B->frec.as<pg::CellRec>();
+    u.fbuf = B->fbuf.as<double>();
+    pg::launch_hll_owned(dim, u, s);
     if (move)
         pg::launch_bdrift(dim, B->n_h, B->hbox.as<int32_t>(), B->x.as<double>(), w ? w : B->vel.as<double>(),
                           B->dt_box.as<double>(), B->periodic ? 1 : 0, s);
@@ -1915,6 +2027,7 @@ int pg_batch_create(pg_ctx* c, const pg_sim_params* prm, int nbox, const int64_t
         argchk(mp.boundary == PG_BOUNDARY_PERIODIC || mp.boundary == PG_BOUNDARY_REFLECTIVE,
                "a batch needs periodic or reflective boxes");
         argchk(mp.perturb == 0.0, "batches do not apply the degeneracy fallback (perturb must be 0)");
+        argchk(mp.exact == 0, "batches do not apply the exact-predicate fallback (exact must be 0)");
         argchk(mp.k >= 1 && mp.k <= 256 && mp.k_max <= 256, "k and k_max must be in [1, 256]");
         argchk(mp.max_p >= 2 * mp.dim + 1 && mp.max_p <= MAXP, "max_p must be in [2*dim+1, 64]");
         argchk(mp.max_t >= 8 && mp.max_t <= 128, "max_t must be in [8, 128]");
@@ -2030,6 +2143,12 @@ int pg_batch_create(pg_ctx* c, const pg_sim_params* prm, int nbox, const int64_t
             B->vol_u.get<double>(cs);
             B->o2s.get<int32_t>(n);
             B->hsorted.get<int32_t>(cs);
+            B->hcode.get<int32_t>(cs);
+            B->nbr_cm.get<int32_t>(cs * MAXP);
+            B->twin.get<uint8_t>(cs * MAXP);
+            B->solve.get<uint64_t>(cs);
+            B->frec.get<pg::CellRec>(cs);
+            B->fbuf.get<double>((size_t)(mp.dim + 2) * cs * MAXP);
             B->tot.get<double>(5 * (size_t)nbox);
             B->feat.get<float>(cs * pg::LK_NFEAT);
             B->kcls.get<int32_t>(cs);

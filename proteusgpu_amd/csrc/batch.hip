@@ -301,11 +301,12 @@ __global__ void __launch_bounds__(TB)
                 const int32_t* __restrict__ orig_h, const uint8_t* __restrict__ count, const int32_t* __restrict__ nbr,
                 const double* __restrict__ area, const double* __restrict__ vol, double ascale, double vscale,
                 int periodic, int32_t* __restrict__ hid, double* __restrict__ area_u, double* __restrict__ vol_u,
-                int32_t* __restrict__ o2s, int32_t* __restrict__ hsorted, int32_t* bad) {
+                int32_t* __restrict__ o2s, int32_t* __restrict__ hsorted, int32_t* __restrict__ hcode, int32_t* bad) {
     const int64_t n = *n_dev;
     for (int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x; s < n; s += (int64_t)gridDim.x * TB) {
         const int32_t h = orig_h[perm[s]];
         hsorted[s] = h;
+        hcode[s] = h;  // ghosts: -1 - source, as orig_h
         if (h < 0) continue;
         o2s[h] = (int32_t)s;
         vol_u[s] = vscale * vol[s];
@@ -365,28 +366,6 @@ __global__ void __launch_bounds__(TB)
     if ((threadIdx.x & 63) == 0 && b0 >= 0) {
         if (m >= 0.0) atomic_min_pos_double(&dt_box[b0], m);
         else atomicMin(&dt_box[b0], 0ull);
-    }
-}
-
-template <int DIM>
-__global__ void __launch_bounds__(TB)
-    k_bhll(const int64_t* __restrict__ n_dev, int64_t stride, int64_t nplane, const int32_t* __restrict__ hsorted,
-           const int32_t* __restrict__ hbox, const uint8_t* __restrict__ count, const int32_t* __restrict__ hid,
-           const int32_t* __restrict__ nbr, const double* __restrict__ nrm, const double* __restrict__ area_u,
-           const double* __restrict__ cen, const double* __restrict__ sorted, double scale, double gam,
-           const double* __restrict__ dt_box, int flux_mode, int move, SimView sv, const double* __restrict__ wgen) {
-    const int64_t n = *n_dev;
-    for (int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x; s < n; s += (int64_t)gridDim.x * TB) {
-        const int32_t i = hsorted[s];
-        if (i < 0) continue;
-        const double dt = dt_box[hbox[i]];
-        double fm, fp[3], fe;
-        ale_cell_flux<DIM>(s, (uint32_t)i, stride, nplane, count, hid, nbr, nrm, area_u, cen, sorted, scale, gam,
-                           flux_mode, move, sv, wgen, fm, fp, fe);
-        sv.mass[i] -= dt * fm;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) sv.mom[(size_t)DIM * i + d] -= dt * fp[d];
-        sv.en[i] -= dt * fe;
     }
 }
 
@@ -580,10 +559,10 @@ void launch_bcollect(const int64_t* nq_dev, int64_t cap, const uint32_t* seeds, 
 void launch_bfinalize(const int64_t* n_dev, int64_t cap, int64_t stride, const uint32_t* perm, const int32_t* orig_h,
                       const uint8_t* count, const int32_t* nbr, const double* area, const double* vol, double ascale,
                       double vscale, int periodic, int32_t* hid, double* area_u, double* vol_u, int32_t* o2s,
-                      int32_t* hsorted, int32_t* bad, hipStream_t s) {
+                      int32_t* hsorted, int32_t* hcode, int32_t* bad, hipStream_t s) {
     k_bfinalize<<<std::min<unsigned>(nblk(cap), 2048), TB, 0, s>>>(n_dev, stride, perm, orig_h, count, nbr, area, vol,
                                                                    ascale, vscale, periodic, hid, area_u, vol_u, o2s,
-                                                                   hsorted, bad);
+                                                                   hsorted, hcode, bad);
 }
 
 void launch_bdt(int dim, int64_t n_h, int nbox, const int32_t* hbox, const int32_t* o2s, const double* vol_u,
@@ -596,19 +575,6 @@ void launch_bdt(int dim, int64_t n_h, int nbox, const int32_t* hbox, const int32
     else
         k_bdt<3><<<nblk(n_h), TB, 0, s>>>(n_h, hbox, o2s, vol_u, gam, cfl, rho, vel, E, (unsigned long long*)dt_box,
                                           wgen, cm);
-}
-
-void launch_bhll(int dim, const int64_t* n_dev, int64_t cap, int64_t stride, int64_t nplane, const int32_t* hsorted,
-                 const int32_t* hbox, const uint8_t* count, const int32_t* hid, const int32_t* nbr, const double* nrm,
-                 const double* area_u, const double* cen, const double* sorted, double scale, double gam,
-                 const double* dt_box, int flux_mode, int move, SimView sv, hipStream_t s, const double* wgen) {
-    const unsigned gb = std::min<unsigned>(nblk(cap), 4096);
-    if (dim == 2)
-        k_bhll<2><<<gb, TB, 0, s>>>(n_dev, stride, nplane, hsorted, hbox, count, hid, nbr, nrm, area_u, cen, sorted,
-                                    scale, gam, dt_box, flux_mode, move, sv, wgen);
-    else
-        k_bhll<3><<<gb, TB, 0, s>>>(n_dev, stride, nplane, hsorted, hbox, count, hid, nbr, nrm, area_u, cen, sorted,
-                                    scale, gam, dt_box, flux_mode, move, sv, wgen);
 }
 
 void launch_bdrift(int dim, int64_t n_h, const int32_t* hbox, double* x, const double* vel, const double* dt_box,

@@ -28,13 +28,43 @@ struct CflMesh {
     double scale = 1.0;
 };
 
+// Primitives and generator velocity of one sorted cell (64 B), read by the
+// face-flux pass of the owner-computes-once update.
+struct CellRec {
+    double rho, v[3], E, g[3];
+};
+
 void launch_q_from_prim(int dim, SimView sv, const int32_t* o2s, const double* vol_u, hipStream_t s);
 void launch_prim_from_q(int dim, SimView sv, const int32_t* o2s, const double* vol_u, hipStream_t s);
-void launch_hll_ale(int dim, int64_t n, int64_t n_hydro, int64_t stride, int64_t nplane, const uint32_t* perm,
-                    const uint8_t* count, const int32_t* hid, const int32_t* nbr, const double* nrm,
-                    const double* area_u, const double* cen, const double* sorted, double scale, double gam,
-                    const double* dt_dev, double dt_max, int flux_mode, int move, SimView sv, hipStream_t s,
-                    const double* wgen = nullptr);
+// Owner-computes-once moving-mesh update.  hcode per sorted seed: hydro
+// id, or -1 - (hydro id of the source) for a periodic ghost.  Seeds: n, or
+// *n_dev when n_dev is set (batches: n is then the capacity).
+// nbr_cm: cell-major neighbour lists (FACE_SLOTS per cell, from the clip
+// kernel); solve: per cell, the mask of the faces it solves.
+void launch_face_twins(const int64_t* n_dev, int64_t n, int64_t stride, const int32_t* hcode, const uint8_t* count,
+                       const int32_t* nbr_cm, uint8_t* twin, uint64_t* solve, hipStream_t s);
+struct OwnedUpdate {
+    const int64_t* n_dev = nullptr;
+    int64_t n = 0, stride = 0, nplane = 0;
+    const int32_t* hcode = nullptr;
+    const uint8_t* count = nullptr;
+    const uint8_t* twin = nullptr;  // launch_face_twins of the same mesh
+    const uint64_t* solve = nullptr;
+    const int32_t* nbr = nullptr;
+    const double *nrm = nullptr, *area_u = nullptr, *cen = nullptr, *sorted = nullptr;
+    double scale = 1.0, gam = 1.4;
+    int flux_mode = 1, move = 1;
+    // time step: *dt_dev capped at dt_max > 0, or per box dt_box[hbox[i]]
+    const double* dt_dev = nullptr;
+    double dt_max = 0.0;
+    const double* dt_box = nullptr;
+    const int32_t* hbox = nullptr;
+    SimView sv{};
+    const double* wgen = nullptr;  // steered generator velocities (nullptr: the fluid's)
+    CellRec* rec = nullptr;        // scratch: n records
+    double* fbuf = nullptr;        // scratch: (dim + 2) doubles per face slot (slot-major)
+};
+void launch_hll_owned(int dim, OwnedUpdate u, hipStream_t s);
 // HLL fluxes of n independent faces (riemann.cpp:7-45): states (rho,
 // v[dim], E) per face, stored normals (3 per face), optional wall flags.
 void launch_hll_faces(int dim, int64_t n, double gam, const double* L, const double* R, const double* nrm,
@@ -51,7 +81,7 @@ void launch_totals(int dim, SimView sv, double* part, hipStream_t s);
 void launch_finalize(int64_t n, int64_t n_hydro, int64_t stride, const uint32_t* perm,
                      const int64_t* gsrc, const uint8_t* count, const int32_t* nbr,
                      const double* area, const double* vol, double ascale, double vscale,
-                     int32_t* hid, double* area_u, double* vol_u, int32_t* o2s, int32_t* bad,
+                     int32_t* hid, double* area_u, double* vol_u, int32_t* o2s, int32_t* hcode, int32_t* bad,
                      hipStream_t s);
 
 void launch_hll_static(int dim, int64_t n, int64_t n_hydro, int64_t stride, int64_t nplane,
@@ -83,15 +113,10 @@ void launch_cells_orig(int64_t n, int64_t stride, int dim, const uint32_t* perm,
 void launch_bfinalize(const int64_t* n_dev, int64_t cap, int64_t stride, const uint32_t* perm, const int32_t* orig_h,
                       const uint8_t* count, const int32_t* nbr, const double* area, const double* vol, double ascale,
                       double vscale, int periodic, int32_t* hid, double* area_u, double* vol_u, int32_t* o2s,
-                      int32_t* hsorted, int32_t* bad, hipStream_t s);
+                      int32_t* hsorted, int32_t* hcode, int32_t* bad, hipStream_t s);
 void launch_bdt(int dim, int64_t n_h, int nbox, const int32_t* hbox, const int32_t* o2s, const double* vol_u,
                 double gam, double cfl, const double* rho, const double* vel, const double* E, double* dt_box,
                 hipStream_t s, const double* wgen = nullptr, CflMesh cm = {});
-void launch_bhll(int dim, const int64_t* n_dev, int64_t cap, int64_t stride, int64_t nplane, const int32_t* hsorted,
-                 const int32_t* hbox, const uint8_t* count, const int32_t* hid, const int32_t* nbr, const double* nrm,
-                 const double* area_u, const double* cen, const double* sorted, double scale, double gam,
-                 const double* dt_box, int flux_mode, int move, SimView sv, hipStream_t s,
-                 const double* wgen = nullptr);
 void launch_bdrift(int dim, int64_t n_h, const int32_t* hbox, double* x, const double* vel, const double* dt_box,
                    int periodic, hipStream_t s);
 void launch_btotals(int dim, int nbox, const int64_t* hoff, SimView sv, double* out, hipStream_t s);

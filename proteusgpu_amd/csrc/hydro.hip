@@ -6,9 +6,12 @@
 // Face data come from the clip kernel in face-slot-major padded storage
 // (entry (f, sorted cell s) at f*stride + s), so threads of a wave that handle
 // consecutive sorted cells read consecutive addresses for every face slot.
+#include <algorithm>
+
 #include "common.h"
 #include "hydro.h"
 #include "hydro_core.h"
+#include "kernels.h"
 
 namespace pg {
 namespace {
@@ -23,10 +26,12 @@ __global__ void __launch_bounds__(TB) k_finalize(int64_t n, int64_t n_hydro, int
                                                  const uint8_t* count, const int32_t* nbr,
                                                  const double* area, const double* vol, double ascale,
                                                  double vscale, int32_t* hid, double* area_u,
-                                                 double* vol_u, int32_t* o2s, int32_t* bad) {
+                                                 double* vol_u, int32_t* o2s, int32_t* hcode, int32_t* bad) {
     int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x;
     if (s >= n) return;
     const uint32_t o = perm[s];
+    // hydro id, or -1 - source for a periodic ghost (-1 without ghosts)
+    hcode[s] = o < n_hydro ? (int32_t)o : (gsrc ? -1 - (int32_t)gsrc[o - n_hydro] : -1);
     if (o >= n_hydro) return;
     o2s[o] = (int32_t)s;
     vol_u[s] = vscale * vol[s];
@@ -198,6 +203,7 @@ __global__ void __launch_bounds__(TB) k_prim_from_q(SimView sv, const int32_t* o
     sv.E[i] = sv.en[i] / V;
 }
 
+// ---------------------------------------------------------- owner-computes-once
 // Moving-mesh (ALE) HLL update of the conserved state.  The face moves with
 // w_f = (w_i + w_j)/2 plus the standard correction for the offset of the face
 // centroid from the generators' midpoint; the Riemann problem is solved in
@@ -205,27 +211,197 @@ __global__ void __launch_bounds__(TB) k_prim_from_q(SimView sv, const int32_t* o
 // + F_m w_f, energy + F_p.w_f + |w_f|^2 F_m / 2).  With move == 0 this is the
 // static flux of k_hll_static applied to extensive variables.  Walls do not
 // move (w_f = 0).  No reference counterpart: the reference never moves its mesh.
+//
+// The update solves each face between two hydro cells once (the reference,
+// finite_volume_solver.cpp:69-87, solves every face from both sides).
+// A face of sorted cells s < t is owned by s; t finds the same face in s's
+// list (twin) and subtracts s's flux instead of solving it again.  Faces to
+// walls and periodic ghosts, and a face that the other cell does not list,
+// are solved by the cell itself.  Three passes, none with atomics:
+//   k_cell_records  primitives + generator velocity per sorted seed (ghosts
+//                   take their source cell's), so that neighbour reads are
+//                   by sorted id and land near the cell's own
+//   k_face_flux     one thread per (face row r, cell): faces r, r+FR, ...
+//                   that the cell solves; writes flux x area per face slot
+//   k_face_sum      per cell: its own slots plus the negated twins, in face
+//                   order (a segmented sum over the cell's face list), and
+//                   the update of the conserved state
+constexpr int FR = 8;            // threads per cell in k_face_flux (face slot rows)
+constexpr uint8_t SOLVE = 0xFF;  // twin code: solved by this cell
+
+// Seeds of one update: n (or *n_dev, a device-side count, for batches);
+// hcode[s] = hydro id of sorted seed s, or -1 - (hydro id of its source) for
+// a periodic ghost.  Kernels walk the seeds grid-stride.
+PG_DEV int64_t seed_count(const int64_t* n_dev, int64_t n) { return n_dev ? *n_dev : n; }
+
+// One thread per cell: twin[at] = face index of face (s, f) in the owner's
+// list, or SOLVE; solve[s] = mask of the faces s solves.  The owner's list is
+// searched in its cell-major copy (nbr_cm, FACE_SLOTS per cell, written by
+// the clip kernel), 16 bytes per load.
+__global__ void __launch_bounds__(TB) k_face_twins(const int64_t* n_dev, int64_t n_host, int64_t stride,
+                                                   const int32_t* __restrict__ hcode,
+                                                   const uint8_t* __restrict__ count,
+                                                   const int32_t* __restrict__ nbr_cm, uint8_t* __restrict__ twin,
+                                                   uint64_t* __restrict__ solve) {
+    const int64_t n = seed_count(n_dev, n_host);
+    for (int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x; s < n; s += (int64_t)gridDim.x * TB) {
+        if (hcode[s] < 0) continue;
+        const int c = count[s];
+        const int4* ls = reinterpret_cast<const int4*>(nbr_cm + s * FACE_SLOTS);
+        uint64_t mask = 0;
+        for (int f0 = 0; f0 < c; f0 += 4) {
+            const int4 q = ls[f0 >> 2];
+            const int32_t tq[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int f = f0 + e;
+                if (f >= c) break;
+                const int32_t t = tq[e];
+                uint8_t code = SOLVE;
+                if (t >= 0 && t < s && hcode[t] >= 0) {
+                    const int ct = count[t];
+                    const int4* lt = reinterpret_cast<const int4*>(nbr_cm + (int64_t)t * FACE_SLOTS);
+                    for (int g0 = 0; g0 < ct && code == SOLVE; g0 += 4) {
+                        const int4 v = lt[g0 >> 2];
+                        const int32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                        for (int h = 0; h < 4; h++)
+                            if (g0 + h < ct && vv[h] == (int32_t)s && code == SOLVE) code = (uint8_t)(g0 + h);
+                    }
+                }
+                twin[(int64_t)f * stride + s] = code;
+                if (code == SOLVE) mask |= 1ull << f;
+            }
+        }
+        solve[s] = mask;
+    }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(TB) k_cell_records(const int64_t* n_dev, int64_t n_host,
+                                                     const int32_t* __restrict__ hcode, SimView sv,
+                                                     const double* __restrict__ wgen, CellRec* __restrict__ rec) {
+    const int64_t n = seed_count(n_dev, n_host);
+    const double* gv = wgen ? wgen : sv.vel;
+    for (int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x; s < n; s += (int64_t)gridDim.x * TB) {
+        const int32_t c = hcode[s];
+        const int64_t h = c >= 0 ? c : -1 - (int64_t)c;
+        CellRec r;
+        r.rho = sv.rho[h];
+        r.E = sv.E[h];
+        r.v[2] = r.g[2] = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            r.v[d] = sv.vel[(size_t)DIM * h + d];
+            r.g[d] = gv[(size_t)DIM * h + d];
+        }
+        rec[s] = r;
+    }
+}
+
 template <int DIM>
 __global__ void __launch_bounds__(TB)
-    k_hll_ale(int64_t n, int64_t n_hydro, int64_t stride, int64_t nplane, const uint32_t* __restrict__ perm,
-              const uint8_t* __restrict__ count, const int32_t* __restrict__ hid,
-              const int32_t* __restrict__ nbr, const double* __restrict__ nrm,
-              const double* __restrict__ area_u, const double* __restrict__ cen,
-              const double* __restrict__ sorted, double scale, double gam, const double* dt_dev,
-              double dt_max, int flux_mode, int move, SimView sv, const double* __restrict__ wgen) {
-    int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t i = perm[s];
-    if (i >= n_hydro) return;
-    double dt = *dt_dev;
-    if (dt_max > 0 && dt > dt_max) dt = dt_max;
-    double fm, fp[3], fe;
-    ale_cell_flux<DIM>(s, i, stride, nplane, count, hid, nbr, nrm, area_u, cen, sorted, scale, gam, flux_mode,
-                       move, sv, wgen, fm, fp, fe);
-    sv.mass[i] -= dt * fm;
+    k_face_flux(const int64_t* n_dev, int64_t n_host, int64_t stride, int64_t nplane, const int32_t* __restrict__ hcode,
+                const uint64_t* __restrict__ solve, const int32_t* __restrict__ nbr,
+                const double* __restrict__ nrm, const double* __restrict__ area_u, const double* __restrict__ cen,
+                const double* __restrict__ sorted, double scale, double gam, int flux_mode, int move,
+                const CellRec* __restrict__ rec, double* __restrict__ fbuf) {
+    const int64_t n = seed_count(n_dev, n_host);
+    const double sgn = flux_mode == 1 ? -1.0 : 1.0;
+    for (int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x; s < n; s += (int64_t)gridDim.x * TB) {
+        if (hcode[s] < 0) continue;
+        // face slots r, r + FR, ... (r = blockIdx.y) that this cell solves:
+        // the lanes of a wave read the same slot row of consecutive cells
+        const uint64_t m = solve[s] >> blockIdx.y;
+        if (!m) continue;
+        const CellRec own = rec[s];
+        Prim si;
+        si.rho = own.rho;
+        si.E = own.E;
+        double ri[3] = {0, 0, 0}, gi[3] = {0, 0, 0};
 #pragma unroll
-    for (int d = 0; d < DIM; d++) sv.mom[(size_t)DIM * i + d] -= dt * fp[d];
-    sv.en[i] -= dt * fe;
+        for (int d = 0; d < 3; d++) {
+            si.v[d] = own.v[d];
+            gi[d] = own.g[d];
+        }
+#pragma unroll
+        for (int d = 0; d < DIM; d++) ri[d] = (sorted[(size_t)DIM * s + d] - 0.5) * scale + 0.5;
+        for (int f = blockIdx.y; f < 64 && (m >> (f - blockIdx.y)); f += FR) {
+            if (!((m >> (f - blockIdx.y)) & 1ull)) continue;
+            const int64_t at = (int64_t)f * stride + s;
+            const int32_t t = nbr[at];
+            const bool wall = t < 0;
+            Prim sj = si;
+            double rj[3] = {0, 0, 0}, gj[3] = {0, 0, 0}, fc[3] = {0, 0, 0};
+            if (!wall) {
+                const CellRec o = rec[t];
+                sj.rho = o.rho;
+                sj.E = o.E;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    sj.v[d] = o.v[d];
+                    gj[d] = o.g[d];
+                }
+                if (move) {
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        rj[d] = (sorted[(size_t)DIM * t + d] - 0.5) * scale + 0.5;
+                        fc[d] = (cen[(int64_t)d * nplane + at] - 0.5) * scale + 0.5;
+                    }
+                }
+            }
+            double Fm, Fp[3], Fe;
+            ale_face<DIM>(gam, sgn, move, si, ri, gi, sj, rj, gj, wall, nrm[at], nrm[nplane + at],
+                          nrm[2 * nplane + at], fc, Fm, Fp, Fe);
+            const double a = area_u[at];
+            double* fo = fbuf + at * (DIM + 2);
+            fo[0] = Fm * a;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) fo[1 + d] = Fp[d] * a;
+            fo[1 + DIM] = Fe * a;
+        }
+    }
+}
+
+// dt: the device scalar (capped at dt_max > 0), or per box dt_box[hbox[i]]
+template <int DIM>
+__global__ void __launch_bounds__(TB)
+    k_face_sum(const int64_t* n_dev, int64_t n_host, int64_t stride, int64_t nplane, const int32_t* __restrict__ hcode,
+               const uint8_t* __restrict__ count, const uint8_t* __restrict__ twin, const int32_t* __restrict__ nbr,
+               const double* __restrict__ fbuf, const double* dt_dev, double dt_max, const double* __restrict__ dt_box,
+               const int32_t* __restrict__ hbox, SimView sv) {
+    const int64_t n = seed_count(n_dev, n_host);
+    double dt0 = 0.0;
+    if (dt_dev) {
+        dt0 = *dt_dev;
+        if (dt_max > 0 && dt0 > dt_max) dt0 = dt_max;
+    }
+    for (int64_t s = (int64_t)blockIdx.x * TB + threadIdx.x; s < n; s += (int64_t)gridDim.x * TB) {
+        const int32_t i = hcode[s];
+        if (i < 0) continue;
+        const double dt = dt_box ? dt_box[hbox[i]] : dt0;
+        double q[DIM + 2];
+#pragma unroll
+        for (int k = 0; k < DIM + 2; k++) q[k] = 0.0;
+        const int c = count[s];
+        for (int f = 0; f < c; f++) {
+            const int64_t at = (int64_t)f * stride + s;
+            const uint8_t tw = twin[at];
+            if (tw == SOLVE) {
+                const double* fo = fbuf + at * (DIM + 2);
+#pragma unroll
+                for (int k = 0; k < DIM + 2; k++) q[k] += fo[k];
+            } else {
+                const double* fo = fbuf + ((int64_t)tw * stride + nbr[at]) * (DIM + 2);
+#pragma unroll
+                for (int k = 0; k < DIM + 2; k++) q[k] -= fo[k];
+            }
+        }
+        sv.mass[i] -= dt * q[0];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) sv.mom[(size_t)DIM * i + d] -= dt * q[1 + d];
+        sv.en[i] -= dt * q[DIM + 1];
+    }
 }
 
 // Steered generator velocity (mesh regularisation, Springel 2010, MNRAS 401,
@@ -387,11 +563,11 @@ __global__ void __launch_bounds__(TB)
 void launch_finalize(int64_t n, int64_t n_hydro, int64_t stride, const uint32_t* perm,
                      const int64_t* gsrc, const uint8_t* count, const int32_t* nbr,
                      const double* area, const double* vol, double ascale, double vscale,
-                     int32_t* hid, double* area_u, double* vol_u, int32_t* o2s, int32_t* bad,
+                     int32_t* hid, double* area_u, double* vol_u, int32_t* o2s, int32_t* hcode, int32_t* bad,
                      hipStream_t s) {
     if (n <= 0) return;
     k_finalize<<<nblk(n), TB, 0, s>>>(n, n_hydro, stride, perm, gsrc, count, nbr, area, vol, ascale,
-                                      vscale, hid, area_u, vol_u, o2s, bad);
+                                      vscale, hid, area_u, vol_u, o2s, hcode, bad);
 }
 
 void launch_hll_static(int dim, int64_t n, int64_t n_hydro, int64_t stride, int64_t nplane,
@@ -440,18 +616,37 @@ void launch_prim_from_q(int dim, SimView sv, const int32_t* o2s, const double* v
     else k_prim_from_q<3><<<nblk(sv.n_hydro), TB, 0, s>>>(sv, o2s, vol_u);
 }
 
-void launch_hll_ale(int dim, int64_t n, int64_t n_hydro, int64_t stride, int64_t nplane, const uint32_t* perm,
-                    const uint8_t* count, const int32_t* hid, const int32_t* nbr, const double* nrm,
-                    const double* area_u, const double* cen, const double* sorted, double scale, double gam,
-                    const double* dt_dev, double dt_max, int flux_mode, int move, SimView sv, hipStream_t s,
-                    const double* wgen) {
+// grid: every seed once (host count), or a capped grid-stride grid over
+// the capacity (device count)
+static unsigned own_blocks(const int64_t* n_dev, int64_t n) {
+    return n_dev ? std::min<unsigned>(nblk(n), 4096) : nblk(n);
+}
+
+void launch_face_twins(const int64_t* n_dev, int64_t n, int64_t stride, const int32_t* hcode, const uint8_t* count,
+                       const int32_t* nbr_cm, uint8_t* twin, uint64_t* solve, hipStream_t s) {
     if (n <= 0) return;
-    if (dim == 2)
-        k_hll_ale<2><<<nblk(n), TB, 0, s>>>(n, n_hydro, stride, nplane, perm, count, hid, nbr, nrm, area_u, cen,
-                                            sorted, scale, gam, dt_dev, dt_max, flux_mode, move, sv, wgen);
-    else
-        k_hll_ale<3><<<nblk(n), TB, 0, s>>>(n, n_hydro, stride, nplane, perm, count, hid, nbr, nrm, area_u, cen,
-                                            sorted, scale, gam, dt_dev, dt_max, flux_mode, move, sv, wgen);
+    k_face_twins<<<own_blocks(n_dev, n), TB, 0, s>>>(n_dev, n, stride, hcode, count, nbr_cm, twin, solve);
+}
+
+void launch_hll_owned(int dim, OwnedUpdate u, hipStream_t s) {
+    if (u.n <= 0) return;
+    const unsigned gb = own_blocks(u.n_dev, u.n);
+    const dim3 gf(gb, FR);
+    if (dim == 2) {
+        k_cell_records<2><<<gb, TB, 0, s>>>(u.n_dev, u.n, u.hcode, u.sv, u.wgen, u.rec);
+        k_face_flux<2><<<gf, TB, 0, s>>>(u.n_dev, u.n, u.stride, u.nplane, u.hcode, u.solve, u.nbr, u.nrm,
+                                         u.area_u, u.cen, u.sorted, u.scale, u.gam, u.flux_mode, u.move, u.rec,
+                                         u.fbuf);
+        k_face_sum<2><<<gb, TB, 0, s>>>(u.n_dev, u.n, u.stride, u.nplane, u.hcode, u.count, u.twin, u.nbr, u.fbuf,
+                                        u.dt_dev, u.dt_max, u.dt_box, u.hbox, u.sv);
+    } else {
+        k_cell_records<3><<<gb, TB, 0, s>>>(u.n_dev, u.n, u.hcode, u.sv, u.wgen, u.rec);
+        k_face_flux<3><<<gf, TB, 0, s>>>(u.n_dev, u.n, u.stride, u.nplane, u.hcode, u.solve, u.nbr, u.nrm,
+                                         u.area_u, u.cen, u.sorted, u.scale, u.gam, u.flux_mode, u.move, u.rec,
+                                         u.fbuf);
+        k_face_sum<3><<<gb, TB, 0, s>>>(u.n_dev, u.n, u.stride, u.nplane, u.hcode, u.count, u.twin, u.nbr, u.fbuf,
+                                        u.dt_dev, u.dt_max, u.dt_box, u.hbox, u.sv);
+    }
 }
 
 void launch_hll_faces(int dim, int64_t n, double gam, const double* L, const double* R, const double* nrm,

@@ -153,97 +153,64 @@ PG_DEV double rel_speed(const double* v, const double* w) {
     return sqrt(s);
 }
 
-// Fluxes through the faces of sorted cell s (hydro cell i), integrated
-// over the face areas; see k_hll_ale.
+// Flux of one face in the lab frame, per unit area (see k_face_flux): own
+// state si at generator ri moving with gi, neighbour state sj at rj moving
+// with gj (ignored at walls), stored normal n and face centroid fc (both
+// unscaled for rj/ri/fc).  The face moves with w = (gi + gj)/2 plus the
+// correction for the offset of fc from the generators' midpoint; the
+// Riemann problem is solved in the face frame and transformed back.
 template <int DIM>
-PG_DEV void ale_cell_flux(int64_t s, uint32_t i, int64_t stride, int64_t nplane, const uint8_t* __restrict__ count,
-                          const int32_t* __restrict__ hid, const int32_t* __restrict__ nbr,
-                          const double* __restrict__ nrm, const double* __restrict__ area_u,
-                          const double* __restrict__ cen, const double* __restrict__ sorted, double scale,
-                          double gam, int flux_mode, int move, const SimView& sv, const double* __restrict__ wgen,
-                          double& fm, double (&fp)[3], double& fe) {
-    Prim si;
-    si.rho = sv.rho[i];
-    si.E = sv.E[i];
-    si.v[2] = 0.0;
+PG_DEV void ale_face(double gam, double sgn, int move, const Prim& si, const double (&ri)[3],
+                     const double (&gi)[3], const Prim& sj, const double (&rj)[3], const double (&gj)[3],
+                     bool wall, double n0, double n1, double n2, const double (&fc)[3], double& Fm,
+                     double (&Fp)[3], double& Fe) {
+    double w[3] = {0.0, 0.0, 0.0};
+    if (move && !wall) {
+        double dr[3], mid[3];
+        double dd = 0.0, proj = 0.0;
 #pragma unroll
-    for (int d = 0; d < DIM; d++) si.v[d] = sv.vel[(size_t)DIM * i + d];
-    double ri[3] = {0, 0, 0};
-#pragma unroll
-    for (int d = 0; d < DIM; d++) ri[d] = (sorted[(size_t)DIM * s + d] - 0.5) * scale + 0.5;
-    // generator velocities: the fluid velocity, or the steered one (wgen)
-    auto gvel = [&](int32_t h, int d) { return wgen ? wgen[(size_t)DIM * h + d] : sv.vel[(size_t)DIM * h + d]; };
-    double gi[3] = {0, 0, 0};
-#pragma unroll
-    for (int d = 0; d < DIM; d++) gi[d] = gvel((int32_t)i, d);
-    fm = 0.0;
-    fp[0] = fp[1] = fp[2] = 0.0;
-    fe = 0.0;
-    const int c = count[s];
-    const double sgn = flux_mode == 1 ? -1.0 : 1.0;
-    for (int f = 0; f < c; f++) {
-        const int64_t at = (int64_t)f * stride + s;
-        const int32_t j = hid[at];
-        const bool wall = j < 0;
-        Prim sj = si;
-        double w[3] = {0.0, 0.0, 0.0};
-        if (!wall) {
-            sj.rho = sv.rho[j];
-            sj.E = sv.E[j];
-#pragma unroll
-            for (int d = 0; d < DIM; d++) sj.v[d] = sv.vel[(size_t)DIM * j + d];
-            if (move) {
-                const int32_t js = nbr[at];
-                double rj[3] = {0, 0, 0}, dr[3], mid[3];
-                double dd = 0.0, proj = 0.0;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    rj[d] = (sorted[(size_t)DIM * js + d] - 0.5) * scale + 0.5;
-                    dr[d] = rj[d] - ri[d];
-                    mid[d] = 0.5 * (ri[d] + rj[d]);
-                    dd += dr[d] * dr[d];
-                    const double fc = (cen[(int64_t)d * nplane + at] - 0.5) * scale + 0.5;
-                    proj += (gi[d] - gvel(j, d)) * (fc - mid[d]);
-                }
-#pragma unroll
-                for (int d = 0; d < DIM; d++) w[d] = 0.5 * (gi[d] + gvel(j, d)) + proj * dr[d] / dd;
-            }
+        for (int d = 0; d < DIM; d++) {
+            dr[d] = rj[d] - ri[d];
+            mid[d] = 0.5 * (ri[d] + rj[d]);
+            dd += dr[d] * dr[d];
+            proj += (gi[d] - gj[d]) * (fc[d] - mid[d]);
         }
-        // total energy per volume is frame dependent: boost it with the velocity
-        Prim L = si, R = sj;
-        if (move && !wall) {
-            double ke_l = 0, ke_r = 0, ke_l2 = 0, ke_r2 = 0;
 #pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                ke_l += L.v[d] * L.v[d];
-                ke_r += R.v[d] * R.v[d];
-                L.v[d] -= w[d];
-                R.v[d] -= w[d];
-                ke_l2 += L.v[d] * L.v[d];
-                ke_r2 += R.v[d] * R.v[d];
-            }
-            L.E += 0.5 * L.rho * (ke_l2 - ke_l);
-            R.E += 0.5 * R.rho * (ke_r2 - ke_r);
+        for (int d = 0; d < DIM; d++) w[d] = 0.5 * (gi[d] + gj[d]) + proj * dr[d] / dd;
+    }
+    // total energy per volume is frame dependent: boost it with the velocity
+    Prim L = si, R = wall ? si : sj;
+    if (move && !wall) {
+        double ke_l = 0, ke_r = 0, ke_l2 = 0, ke_r2 = 0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            ke_l += L.v[d] * L.v[d];
+            ke_r += R.v[d] * R.v[d];
+            L.v[d] -= w[d];
+            R.v[d] -= w[d];
+            ke_l2 += L.v[d] * L.v[d];
+            ke_r2 += R.v[d] * R.v[d];
         }
-        const Geom g = geom_of(sgn * nrm[at], sgn * nrm[nplane + at], sgn * nrm[2 * nplane + at]);
-        const Prim F = hll<DIM>(gam, L, R, g, wall);
-        double Fm = F.rho, Fp[3] = {F.v[0], F.v[1], F.v[2]}, Fe = F.E;
-        if (move && !wall) {
-            double wp = 0.0, ww = 0.0;
+        L.E += 0.5 * L.rho * (ke_l2 - ke_l);
+        R.E += 0.5 * R.rho * (ke_r2 - ke_r);
+    }
+    const Geom g = geom_of(sgn * n0, sgn * n1, sgn * n2);
+    const Prim F = hll<DIM>(gam, L, R, g, wall);
+    Fm = F.rho;
+    Fp[0] = F.v[0];
+    Fp[1] = F.v[1];
+    Fp[2] = F.v[2];
+    Fe = F.E;
+    if (move && !wall) {
+        double wp = 0.0, ww = 0.0;
 #pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                wp += Fp[d] * w[d];
-                ww += w[d] * w[d];
-            }
-            Fe += wp + 0.5 * ww * Fm;
-#pragma unroll
-            for (int d = 0; d < DIM; d++) Fp[d] += Fm * w[d];
+        for (int d = 0; d < DIM; d++) {
+            wp += Fp[d] * w[d];
+            ww += w[d] * w[d];
         }
-        const double a = area_u[at];
-        fm += Fm * a;
+        Fe += wp + 0.5 * ww * Fm;
 #pragma unroll
-        for (int d = 0; d < DIM; d++) fp[d] += Fp[d] * a;
-        fe += Fe * a;
+        for (int d = 0; d < DIM; d++) Fp[d] += Fm * w[d];
     }
 }
 

@@ -78,6 +78,12 @@ struct FaceArrays {
     uint8_t* cstate = nullptr;
     int32_t* resume = nullptr;
     uint32_t* lastnb = nullptr;  // saved seeds: the last neighbour of the row (sorted id)
+    // exact-predicate runs: tie-break key per sorted id (the input index;
+    // nullptr: the sorted id itself)
+    const uint32_t* key = nullptr;
+    // optional cell-major copy of nbr (FACE_SLOTS per cell), for the face
+    // ownership of the hydro update
+    int32_t* nbr_cm = nullptr;
 };
 
 
@@ -93,10 +99,11 @@ constexpr int64_t CLIP_STATE_BYTES = 3328;
 // under the name k_voronoi_restart (profiles tell the passes apart).
 // lanes x slots: the fast kernel's shape (valid_clip_config), taken once
 // per build by the caller so that every launch and the KNN tail tag of a
-// build agree.
+// build agree.  exact: the exact-predicate kernel (exact.h; 128 slots, no
+// saved cells), for seeds whose filtered predicate was degenerate.
 void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq,
                     const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa, int lanes, int slots,
-                    hipStream_t s, bool restart = false);
+                    hipStream_t s, bool restart = false, bool exact = false);
 
 // Process-wide default shape of the fast clip kernel for the following
 // builds: lanes per seed (8 or 16) and vertex slots per seed (48 or 64).

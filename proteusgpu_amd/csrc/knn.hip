@@ -31,8 +31,13 @@ __global__ void __launch_bounds__(BLOCK)
     knn_heap_query<DIM, BLOCK>(sorted, perm, N, 0, starts, counts, q, k, out + (size_t)qi * k, hd, hi);
 }
 
+// waves per SIMD the register budget is sized for: 6 for rows up to 128,
+// 5 up to 256; the long restart rows (up to 1024) are rare
+template <int M>
+constexpr int knn_wave_occupancy() { return M <= 2 ? 6 : (M <= 4 ? 5 : 2); }
+
 template <int DIM, int M>
-__global__ void __launch_bounds__(64, (M <= 2 ? 6 : 5))
+__global__ void __launch_bounds__(64, knn_wave_occupancy<M>())
     k_knn_wave(const double* __restrict__ sorted, const uint32_t* __restrict__ perm, int N,
                const int32_t* __restrict__ starts, const int32_t* __restrict__ counts,
                const uint32_t* __restrict__ queries, int64_t nq, int k, uint32_t* __restrict__ out,
@@ -60,10 +65,14 @@ template <int DIM>
 void dispatch_wave(const double* sorted, const uint32_t* perm, int N, const int32_t* starts,
                    const int32_t* counts, const uint32_t* queries, int64_t nq, int k, uint32_t* out,
                    KnnTail tail, hipStream_t s) {
+    // the kept list holds 64 M entries; a resumed query keeps only its
+    // k - kp missing neighbours, but M is sized for the full row
     if (k <= 64) run_wave<DIM, 1>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
     else if (k <= 128) run_wave<DIM, 2>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
     else if (k <= 192) run_wave<DIM, 3>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
-    else run_wave<DIM, 4>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
+    else if (k <= 256) run_wave<DIM, 4>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
+    else if (k <= 512) run_wave<DIM, 8>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
+    else run_wave<DIM, 16>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
 }
 
 template <int DIM, int BLOCK>
@@ -91,7 +100,7 @@ void launch_knn(int dim, const double* sorted, const uint32_t* perm, int64_t n, 
         const char* e = getenv("PGPU_KNN_WAVE_MIN_K");
         return e ? atoi(e) : 20;
     }();
-    if (k >= wave_min_k) {
+    if (k >= wave_min_k || k > 256) {  // the LDS heap holds at most 256 entries per query
         if (dim == 2) dispatch_wave<2>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
         else dispatch_wave<3>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
         return;

@@ -33,6 +33,18 @@ __global__ void __launch_bounds__(64, (VoroCfg<DIM, G, S>::WAVES)) k_voronoi_res
     voronoi_group<DIM, G, S>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm, blockIdx.x);
 }
 
+// Cells whose filtered predicate was degenerate, again with the exact
+// predicate (128 vertex slots).
+template <int DIM>
+__global__ void __launch_bounds__(64, 1) k_voronoi_exact(const double* __restrict__ pts, int64_t n_total,
+                                                         const uint32_t* __restrict__ seeds, int64_t nq,
+                                                         const uint32_t* __restrict__ knn, int k, int max_p,
+                                                         int max_t, FaceArrays fa) {
+    using C = VoroCfg<DIM, 16, 8>;
+    __shared__ SeedSmem<C::NS, C::MAXP> sm[C::SPW];
+    voronoi_group<DIM, 16, 8, true>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm, blockIdx.x);
+}
+
 template <int DIM, int G, int S>
 void run(const double* pts, int64_t n_total, const uint32_t* seeds, int64_t nq, const uint32_t* knn, int k,
          int max_p, int max_t, FaceArrays fa, bool restart, hipStream_t s) {
@@ -107,8 +119,14 @@ void set_clip_resume(bool on) { g_clip_resume = on; }
 
 template <int DIM>
 void launch_dim(const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq, const uint32_t* knn, int k,
-                int max_p, int max_t, int big, FaceArrays fa, hipStream_t s, bool restart, int lanes, int slots) {
-    if (big) run<DIM, 16, 8>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, true, s);
+                int max_p, int max_t, int big, FaceArrays fa, hipStream_t s, bool restart, int lanes, int slots,
+                bool exact) {
+    if (exact) {
+        using C = VoroCfg<DIM, 16, 8>;
+        const int64_t nb = (nq + C::SPW - 1) / C::SPW;
+        k_voronoi_exact<DIM><<<dim3((unsigned)nb), dim3(64), 0, s>>>(sorted, n, seeds, nq, knn, k, max_p, max_t,
+                                                                      fa);
+    } else if (big) run<DIM, 16, 8>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, true, s);
     else if (lanes == 8 && slots == 48) run<DIM, 8, 6>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
     else if (lanes == 8) run<DIM, 8, 8>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
     else if (slots == 48) run<DIM, 16, 3>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, restart, s);
@@ -117,11 +135,13 @@ void launch_dim(const double* sorted, int64_t n, const uint32_t* seeds, int64_t 
 
 void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq,
                     const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa, int lanes, int slots,
-                    hipStream_t s, bool restart) {
+                    hipStream_t s, bool restart, bool exact) {
     if (nq <= 0) return;
     (void)g_clip_env;
-    if (dim == 2) launch_dim<2>(sorted, n, seeds, nq, knn, k, max_p, max_t, big, fa, s, restart, lanes, slots);
-    else launch_dim<3>(sorted, n, seeds, nq, knn, k, max_p, max_t, big, fa, s, restart, lanes, slots);
+    if (dim == 2)
+        launch_dim<2>(sorted, n, seeds, nq, knn, k, max_p, max_t, big, fa, s, restart, lanes, slots, exact);
+    else
+        launch_dim<3>(sorted, n, seeds, nq, knn, k, max_p, max_t, big, fa, s, restart, lanes, slots, exact);
 }
 
 void launch_collect_status(int64_t nq, const uint32_t* seeds, const int32_t* status, int code,

@@ -30,6 +30,7 @@
 #pragma once
 
 #include "common.h"
+#include "exact.h"
 #include "kernels.h"
 #include "simt.h"
 
@@ -142,7 +143,7 @@ PG_DEV void cramer(const Plane& a, const Plane& b, const Plane& c, double& X, do
 // The reference conflict predicate (cell.cpp:175-221), evaluated exactly.
 template <int DIM>
 PG_DEV bool conflict_exact(const Plane& p1, const Plane& p2, const Plane& p3, const Plane& e,
-                           bool& flag) {
+                           bool& flag, double xb = 0.0) {
     double det, eps;
     if (DIM == 2) {
         det = det3(p1.x, p2.x, e.x, p1.y, p2.y, e.y, p1.w, p2.w, e.w);
@@ -162,13 +163,15 @@ PG_DEV bool conflict_exact(const Plane& p1, const Plane& p2, const Plane& p3, co
         double mm = fmax(fmax(mx, my), mz);
         eps *= (mm * mm);
     }
-    if (fabs(det) < eps) flag = true;
+    if (fabs(det) < eps || fabs(det) < xb) flag = true;
     return det > 0.0;
 }
 
+// n1: |x| + |y| + |z| + wm rounded up to float, wm = |w| (or, in exact
+// mode, the magnitude that bounds the rounding error of w, see XTOL)
 template <int DIM>
-PG_DEV void plane_bounds(const Plane& p, float& n1, float& mx) {
-    n1 = f2f_up(fabs(p.x) + fabs(p.y) + fabs(p.z) + fabs(p.w));
+PG_DEV void plane_bounds(const Plane& p, double wm, float& n1, float& mx) {
+    n1 = f2f_up(fabs(p.x) + fabs(p.y) + fabs(p.z) + wm);
     double m = DIM == 2 ? fmax(fmax(fabs(p.x), fabs(p.y)), fabs(p.w))
                         : fmax(fmax(fabs(p.x), fabs(p.y)), fabs(p.z));
     mx = f2f_up(m);
@@ -229,7 +232,51 @@ struct ClipState {
 };
 static_assert(sizeof(ClipState) <= CLIP_STATE_BYTES, "clip state layout");
 
-template <int DIM, int G, int S>
+// Exact decision of a degenerate conflict test (exact.h) from the vertex's
+// plane triple tt, the new plane's point b (sorted id z) and the seed.
+template <int DIM, int NS, int MAXP>
+__device__ __noinline__ int exact_conflict(uint32_t tt, const SeedSmem<NS, MAXP>& sm, const double* pts,
+                                           uint32_t sid, const double* sp, uint32_t z, const double* bp,
+                                           const uint32_t* key) {
+    XRow rows[DIM + 1];
+    for (int r = 0; r < DIM; r++) {
+        const int pid = (int)tri_p(tt, r);
+        XRow& x = rows[r];
+        x.box = pid < 2 * DIM;
+        x.p[0] = x.p[1] = x.p[2] = 0.0;
+        x.key = 0;
+        x.axis = pid >> 1;
+        x.sg = (pid & 1) ? -1.0 : 1.0;
+        x.c = (pid & 1) ? 1.0 + 1e-14 : -1e-14;  // the box bounds of the initial cell
+        if (!x.box) {
+            const uint32_t q = (uint32_t)sm.pvid[pid];
+            for (int d = 0; d < DIM; d++) x.p[d] = pts[(size_t)DIM * q + d];
+            x.key = key ? key[q] : q;
+        }
+    }
+    XRow& e = rows[DIM];
+    e.box = false;
+    e.axis = 0;
+    e.sg = 1.0;
+    e.c = 0.0;
+    e.p[0] = bp[0];
+    e.p[1] = bp[1];
+    e.p[2] = bp[2];
+    e.key = key ? key[z] : z;
+    return conflict_sos<DIM>(sp, key ? key[sid] : sid, rows);
+}
+
+// Exact mode (fa.key set, or EXACT): a conflict test is trusted in floating
+// point only above XTOL times the product of its rows' 1-norms (w measured
+// by the magnitude its rounding is relative to), a valid bound on how far the
+// rounding of the planes and of the determinant can move it; below, EXACT
+// kernels decide it with exact_conflict and the others flag the cell
+// (needs_exact_predicates) for them.  The certificate of EXACT kernels keeps
+// a relative margin (CERT_TIE): a neighbour tied with twice the largest
+// vertex distance can still cut a zero-area face under the symbolic
+// perturbation, so only neighbours clearly beyond it certify the cell; the
+// other kernels flag a near tie.
+template <int DIM, int G, int S, bool EXACT = false>
 __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, const uint32_t* __restrict__ seeds,
                               int64_t nq, const uint32_t* __restrict__ knn, int k, int max_p,
                               int max_t, FaceArrays fa,
@@ -246,6 +293,10 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
     const bool valid = qi < nq;
     SeedSmem<NS, MAXP>& sm = smem[grp];
     if (max_p > FACE_SLOTS) max_p = FACE_SLOTS;
+    // exact mode (a build with the exact-predicate fallback): every test
+    // whose sign the rounding of the planes could flip is decided exactly
+    // (EXACT kernels) or flags the cell for them (the others)
+    const bool xmode = EXACT || fa.key != nullptr;
 
     const uint32_t sid = valid ? (seeds ? seeds[qi] : (uint32_t)qi) : 0u;
     const uint32_t* row = knn + (valid ? (size_t)qi * (size_t)k : 0);
@@ -296,7 +347,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             default: p = {0.0, 0.0, -1.0, hi}; break;
         }
         sm.plane[gl] = p;
-        plane_bounds<D>(p, sm.pn1[gl], sm.pmx[gl]);
+        plane_bounds<D>(p, fabs(p.w), sm.pn1[gl], sm.pmx[gl]);
         sm.pvid[gl] = -1;
     }
     // Initial vertices (cell.cpp:84-103).
@@ -420,6 +471,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
         uint32_t z = 0;
         double bx = 0, by = 0, bz = 0;
         Plane e = {0, 0, 0, 0};
+        double ewm = 0.0;  // magnitude of e.w for the bounds (see plane_bounds)
         bool plane_ok = false;
         int cur = 0;
         if (act) {
@@ -437,11 +489,12 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
                 double ax = sx + bx, ay = sy + by, az = sz + bz;
                 double dot = ax * dx + ay * dy + az * dz;
                 e = {dx, dy, dz, -dot / 2.0};
+                ewm = xmode ? 0.5 * (fabs(ax * dx) + fabs(ay * dy) + fabs(az * dz)) : fabs(e.w);
                 cur = nb_v;
                 plane_ok = true;
                 if (gl == 0) {
                     sm.plane[cur] = e;
-                    plane_bounds<D>(e, sm.pn1[cur], sm.pmx[cur]);
+                    plane_bounds<D>(e, ewm, sm.pn1[cur], sm.pmx[cur]);
                     sm.pvid[cur] = (int32_t)z;
                 }
                 nb_v++;
@@ -451,7 +504,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
         uint32_t rem = 0;
         bool flag = false;
         if (act && plane_ok) {
-            const double en1 = (fabs(e.x) + fabs(e.y) + fabs(e.z) + fabs(e.w)) * (1.0 + 1e-15);
+            const double en1 = (fabs(e.x) + fabs(e.y) + fabs(e.z) + ewm) * (1.0 + 1e-15);
             const double emx = D == 2 ? fmax(fmax(fabs(e.x), fabs(e.y)), fabs(e.w))
                                       : fmax(fmax(fabs(e.x), fabs(e.y)), fabs(e.z));
 #pragma unroll
@@ -465,7 +518,11 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
                 double M = fmax((double)R.m[s], emx);
                 double M2 = M * M;
                 double epsu = D == 3 ? 1.0000001e-14 * M2 * M2 * M : 1.0000001e-14 * M2 * M2;
-                double bnd = 5.4e-15 * en1 * (double)R.q[s] + epsu;
+                // exact mode: below XTOL en1 q the rounding of the planes
+                // themselves may flip the sign (the reference's eps does not
+                // bound it): such tests go to the exact predicate
+                const double xb = xmode ? XTOL * en1 * (double)R.q[s] : 0.0;
+                double bnd = 5.4e-15 * en1 * (double)R.q[s] + epsu + xb;
                 bool c;
                 if (fabs(t) > bnd) {
                     c = t > 0.0;
@@ -474,7 +531,18 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
                     const Plane p1 = sm.plane[tri_p(tt, 0)];
                     const Plane p2 = sm.plane[tri_p(tt, 1)];
                     const Plane p3 = sm.plane[D == 3 ? tri_p(tt, 2) : tri_p(tt, 1)];
-                    c = conflict_exact<D>(p1, p2, p3, e, flag);
+                    if (EXACT) {
+                        bool fl = false;
+                        c = conflict_exact<D>(p1, p2, p3, e, fl, xb);
+                        if (fl) {
+                            const double sp[3] = {sx, sy, sz}, bp[3] = {bx, by, bz};
+                            const int r = exact_conflict<D>(tt, sm, pts, sid, sp, z, bp, fa.key);
+                            if (r == 0) flag = true;  // outside the exact predicate's range
+                            else c = r > 0;
+                        }
+                    } else {
+                        c = conflict_exact<D>(p1, p2, p3, e, flag, xb);
+                    }
                 }
                 if (c) rem |= 1u << s;
             }
@@ -504,205 +572,212 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
                 doclip = true;
             }
         }
-        // -- cavity boundary: edges of removed vertices whose neighbour is
-        //    kept; bit 3 s + e of bm = edge e of my slot s (a lane walks only
-        //    its removed slots and then only its boundary edges)
-        uint32_t bm = 0;
-        if (doclip) {
-            uint32_t rr = rem;
-            while (rr) {
-                const int s = __builtin_ctz(rr);
-                rr &= rr - 1u;
-                const int gs = s * G + gl;
-#pragma unroll
-                for (int ee = 0; ee < D; ee++)
-                    if (!mask_has<NW>(Rm, sm.adj[ee][gs])) bm |= 1u << (3 * s + ee);
-            }
-        }
-        const int mycnt = __builtin_popcount(bm);
-        int incl = mycnt;
-#pragma unroll
-        for (int d = 1; d < G; d <<= 1) {
-            int t = shfl_up(incl, d, G);
-            if (gl >= d) incl += t;
-        }
-        const int Bt = shfl(incl, G - 1, G);
-        int j = incl - mycnt;
-        if (doclip) {
-            // in (slot, edge) order, as counted
-            uint32_t b2 = bm;
-            while (b2) {
-                const int bit = __builtin_ctz(b2);
-                b2 &= b2 - 1u;
-                if (j >= NS) {  // more boundary edges than slots: overflow below
-                    j++;
-                    continue;
-                }
-                const int s = bit / 3, ee = bit - 3 * s;
-                const int gs = s * G + gl;
-                const uint32_t t = sm.tri[gs];
-                const int K = sm.adj[ee][gs];
-                if (D == 3) {
-                    const int a = tri_p(t, ee), b = tri_p(t, ee == 2 ? 0 : ee + 1);
-                    sm.edge[j] = (uint32_t)a | (uint32_t)b << 8 | (uint32_t)K << 16;
-                    sm.by_start[a] = (uint8_t)j;
-                    sm.by_end[b] = (uint8_t)j;
-                } else {
-                    const int L = tri_p(t, ee);
-                    sm.edge[j] = (uint32_t)L | (uint32_t)K << 16;
-                }
-                j++;
-            }
-        }
-        // live mask of the group (after removal)
-#pragma unroll
-        for (int s = 0; s < S; s++) {
-            uint64_t b = (wave_ballot((R.live >> s) & 1u) >> shift) & gmask;
-            int pos = s * G;
-            Lm[pos >> 6] |= b << (pos & 63);
-        }
-        block_sync();
-        // -- consistency and capacity
-        bool ok = doclip;
-        bool bad = false;
-        int nxt = gl, lab = 0;  // 3D: successor edge and orbit label of edge gl
-        if (doclip) {
-            if (D == 3) {
-                // every boundary edge owns its start and end plane, and the
-                // edge continuing it (the one starting at its end plane)
-                // exists: the successor map is a permutation of the Bt edges
-                for (int jj = gl; jj < Bt; jj += G) {
-                    uint32_t ed = sm.edge[jj];
-                    int a = ed & 0xff, b = (ed >> 8) & 0xff;
-                    if (sm.by_start[a] != jj || sm.by_end[b] != jj) bad = true;
-                    const int nx = sm.by_start[b];
-                    if (nx >= Bt || (int)(sm.edge[nx] & 0xffu) != b) bad = true;
-                }
-                if (Bt <= G && gl < Bt) {
-                    lab = gl;
-                    if (!bad) nxt = sm.by_start[(sm.edge[gl] >> 8) & 0xffu];
-                }
-            } else {
-                bad = (Bt != 2);
-            }
-        }
-        if (D == 3) {
-            // ... and one cycle through all Bt edges: the smallest edge index
-            // on each orbit, by pointer jumping over the group's lanes (all
-            // groups of the wave take part; lanes without an edge point to
-            // themselves), must be 0 everywhere
-            if (nxt >= G) nxt = gl;  // only after a failed check above
-#pragma unroll
-            for (int r = 1; r < 16 && r < G; r <<= 1) {
-                const int l2 = shfl(lab, nxt, G);
-                nxt = shfl(nxt, nxt, G);
-                lab = min(lab, l2);
-            }
+        // Most clip steps clip nothing (about 30 % of a uniform 3D seed's
+        // steps clip, and in half of a wave's steps none of its seeds does):
+        // the cavity, consistency and new-vertex machinery runs only when a
+        // seed of the wave clips (a wave-uniform branch; a block is one wave).
+        const bool anyclip = wave_ballot(doclip) != 0ull;
+        if (anyclip) {
+            // -- cavity boundary: edges of removed vertices whose neighbour is
+            //    kept; bit 3 s + e of bm = edge e of my slot s (a lane walks only
+            //    its removed slots and then only its boundary edges)
+            uint32_t bm = 0;
             if (doclip) {
-                if (Bt <= G) {
-                    if (lab != 0) bad = true;
-                } else if (gl == 0 && !bad) {
-                    int jj = 0, steps = 0;
-                    do {
-                        jj = sm.by_start[(sm.edge[jj] >> 8) & 0xffu];
-                        steps++;
-                    } while (jj != 0 && jj < Bt && steps <= Bt);
-                    if (steps != Bt || jj != 0) bad = true;
+                uint32_t rr = rem;
+                while (rr) {
+                    const int s = __builtin_ctz(rr);
+                    rr &= rr - 1u;
+                    const int gs = s * G + gl;
+#pragma unroll
+                    for (int ee = 0; ee < D; ee++)
+                        if (!mask_has<NW>(Rm, sm.adj[ee][gs])) bm |= 1u << (3 * s + ee);
                 }
             }
-        }
-        bad = ((wave_ballot(bad) >> shift) & gmask) != 0ull;
-        if (doclip) {
-            const int nl_after = nlive - nb_r + Bt;
-            if (bad) {
-                status = INCONSISTENT_BOUNDARY;
-                ok = false;
-            } else if (nl_after >= max_t) {
-                status = TRIANGLE_OVERFLOW;  // cell.cpp:347-350
-                ok = false;
-            } else if (nl_after > NS) {
-                status = SLOT_OVERFLOW;
-                ok = false;
+            const int mycnt = __builtin_popcount(bm);
+            int incl = mycnt;
+#pragma unroll
+            for (int d = 1; d < G; d <<= 1) {
+                int t = shfl_up(incl, d, G);
+                if (gl >= d) incl += t;
             }
-            nlive -= nb_r;
-        }
-        // -- slots of the new vertices: the j-th boundary edge takes the
-        //    j-th free slot (slot-id order)
-        uint32_t asg = 0;
-        if (ok) {
+            const int Bt = shfl(incl, G - 1, G);
+            int j = incl - mycnt;
+            if (doclip) {
+                // in (slot, edge) order, as counted
+                uint32_t b2 = bm;
+                while (b2) {
+                    const int bit = __builtin_ctz(b2);
+                    b2 &= b2 - 1u;
+                    if (j >= NS) {  // more boundary edges than slots: overflow below
+                        j++;
+                        continue;
+                    }
+                    const int s = bit / 3, ee = bit - 3 * s;
+                    const int gs = s * G + gl;
+                    const uint32_t t = sm.tri[gs];
+                    const int K = sm.adj[ee][gs];
+                    if (D == 3) {
+                        const int a = tri_p(t, ee), b = tri_p(t, ee == 2 ? 0 : ee + 1);
+                        sm.edge[j] = (uint32_t)a | (uint32_t)b << 8 | (uint32_t)K << 16;
+                        sm.by_start[a] = (uint8_t)j;
+                        sm.by_end[b] = (uint8_t)j;
+                    } else {
+                        const int L = tri_p(t, ee);
+                        sm.edge[j] = (uint32_t)L | (uint32_t)K << 16;
+                    }
+                    j++;
+                }
+            }
+            // live mask of the group (after removal)
 #pragma unroll
             for (int s = 0; s < S; s++) {
-                const int gs = s * G + gl;
-                if ((R.live >> s) & 1u) continue;
-                // free-slot rank = gs - (live slots below gs)
-                int r = gs - mask_rank<NW>(Lm, gs);
-                if (r < Bt) {
-                    sm.newslot[r] = (uint8_t)gs;
-                    asg |= 1u << s;
-                }
+                uint64_t b = (wave_ballot((R.live >> s) & 1u) >> shift) & gmask;
+                int pos = s * G;
+                Lm[pos >> 6] |= b << (pos & 63);
             }
-        }
-        block_sync();
-        // -- new vertices (cell.cpp:346-364 orientation, :388-411 position)
-        if (ok) {
-            float cn1, cmx;
-            plane_bounds<D>(e, cn1, cmx);
-            while (asg) {
-                const int s = ffs64(asg);
-                asg &= asg - 1;
-                const int gs = s * G + gl;
-                const int r = gs - mask_rank<NW>(Lm, gs);
-                const uint32_t ed = sm.edge[r];
-                uint32_t t;
-                Plane pa, pb;
-                float q, m;
+            block_sync();
+            // -- consistency and capacity
+            bool ok = doclip;
+            bool bad = false;
+            int nxt = gl, lab = 0;  // 3D: successor edge and orbit label of edge gl
+            if (doclip) {
                 if (D == 3) {
-                    const int a = ed & 0xff, b = (ed >> 8) & 0xff, K = (ed >> 16) & 0xff;
-                    t = (uint32_t)cur | (uint32_t)a << 8 | (uint32_t)b << 16;
-                    pa = sm.plane[a];
-                    pb = sm.plane[b];
-                    q = cn1 * sm.pn1[a] * sm.pn1[b] * 1.000001f;
-                    m = fmaxf(cmx, fmaxf(sm.pmx[a], sm.pmx[b]));
-                    sm.tri[gs] = t;
-                    sm.adj[0][gs] = sm.newslot[sm.by_end[a]];
-                    sm.adj[1][gs] = (uint8_t)K;
-                    sm.adj[2][gs] = sm.newslot[sm.by_start[b]];
-                    const uint32_t tk = sm.tri[K];
-                    for (int f = 0; f < 3; f++)
-                        if (tri_p(tk, f) == b && tri_p(tk, (f + 1) % 3) == a) sm.adj[f][K] = (uint8_t)gs;
-                    double px, py, pz, pw, d2;
-                    vertex_numerics<D>(e, pa, pb, sx, sy, sz, px, py, pz, pw, d2);
-                    slot_write<D, S>(R, s, px, py, pz, pw, d2, q, m);
+                    // every boundary edge owns its start and end plane, and the
+                    // edge continuing it (the one starting at its end plane)
+                    // exists: the successor map is a permutation of the Bt edges
+                    for (int jj = gl; jj < Bt; jj += G) {
+                        uint32_t ed = sm.edge[jj];
+                        int a = ed & 0xff, b = (ed >> 8) & 0xff;
+                        if (sm.by_start[a] != jj || sm.by_end[b] != jj) bad = true;
+                        const int nx = sm.by_start[b];
+                        if (nx >= Bt || (int)(sm.edge[nx] & 0xffu) != b) bad = true;
+                    }
+                    if (Bt <= G && gl < Bt) {
+                        lab = gl;
+                        if (!bad) nxt = sm.by_start[(sm.edge[gl] >> 8) & 0xffu];
+                    }
                 } else {
-                    const int L = ed & 0xff, K = (ed >> 16) & 0xff;
-                    pa = sm.plane[L];
-                    const double rw = det2(e.x, e.y, pa.x, pa.y);
-                    const bool swap = rw > 0;
-                    t = swap ? ((uint32_t)L | (uint32_t)cur << 8) : ((uint32_t)cur | (uint32_t)L << 8);
-                    const uint8_t other = sm.newslot[1 - r];
-                    sm.tri[gs] = t;
-                    sm.adj[0][gs] = swap ? (uint8_t)K : other;
-                    sm.adj[1][gs] = swap ? other : (uint8_t)K;
-                    const uint32_t tk = sm.tri[K];
-                    for (int f = 0; f < 2; f++)
-                        if (tri_p(tk, f) == L) sm.adj[f][K] = (uint8_t)gs;
-                    q = cn1 * sm.pn1[L] * 1.000001f;
-                    m = fmaxf(cmx, sm.pmx[L]);
-                    double px, py, pz, pw, d2;
-                    if (swap)
-                        vertex_numerics<D>(pa, e, e, sx, sy, sz, px, py, pz, pw, d2);
-                    else
-                        vertex_numerics<D>(e, pa, pa, sx, sy, sz, px, py, pz, pw, d2);
-                    slot_write<D, S>(R, s, px, py, pz, pw, d2, q, m);
+                    bad = (Bt != 2);
                 }
             }
-            nlive += Bt;
+            if (D == 3) {
+                // ... and one cycle through all Bt edges: the smallest edge index
+                // on each orbit, by pointer jumping over the group's lanes (all
+                // groups of the wave take part; lanes without an edge point to
+                // themselves), must be 0 everywhere
+                if (nxt >= G) nxt = gl;  // only after a failed check above
+#pragma unroll
+                for (int r = 1; r < 16 && r < G; r <<= 1) {
+                    const int l2 = shfl(lab, nxt, G);
+                    nxt = shfl(nxt, nxt, G);
+                    lab = min(lab, l2);
+                }
+                if (doclip) {
+                    if (Bt <= G) {
+                        if (lab != 0) bad = true;
+                    } else if (gl == 0 && !bad) {
+                        int jj = 0, steps = 0;
+                        do {
+                            jj = sm.by_start[(sm.edge[jj] >> 8) & 0xffu];
+                            steps++;
+                        } while (jj != 0 && jj < Bt && steps <= Bt);
+                        if (steps != Bt || jj != 0) bad = true;
+                    }
+                }
+            }
+            bad = ((wave_ballot(bad) >> shift) & gmask) != 0ull;
+            if (doclip) {
+                const int nl_after = nlive - nb_r + Bt;
+                if (bad) {
+                    status = INCONSISTENT_BOUNDARY;
+                    ok = false;
+                } else if (nl_after >= max_t) {
+                    status = TRIANGLE_OVERFLOW;  // cell.cpp:347-350
+                    ok = false;
+                } else if (nl_after > NS) {
+                    status = SLOT_OVERFLOW;
+                    ok = false;
+                }
+                nlive -= nb_r;
+            }
+            // -- slots of the new vertices: the j-th boundary edge takes the
+            //    j-th free slot (slot-id order)
+            uint32_t asg = 0;
+            if (ok) {
+#pragma unroll
+                for (int s = 0; s < S; s++) {
+                    const int gs = s * G + gl;
+                    if ((R.live >> s) & 1u) continue;
+                    // free-slot rank = gs - (live slots below gs)
+                    int r = gs - mask_rank<NW>(Lm, gs);
+                    if (r < Bt) {
+                        sm.newslot[r] = (uint8_t)gs;
+                        asg |= 1u << s;
+                    }
+                }
+            }
+            block_sync();
+            // -- new vertices (cell.cpp:346-364 orientation, :388-411 position)
+            if (ok) {
+                float cn1, cmx;
+                plane_bounds<D>(e, ewm, cn1, cmx);
+                while (asg) {
+                    const int s = ffs64(asg);
+                    asg &= asg - 1;
+                    const int gs = s * G + gl;
+                    const int r = gs - mask_rank<NW>(Lm, gs);
+                    const uint32_t ed = sm.edge[r];
+                    uint32_t t;
+                    Plane pa, pb;
+                    float q, m;
+                    if (D == 3) {
+                        const int a = ed & 0xff, b = (ed >> 8) & 0xff, K = (ed >> 16) & 0xff;
+                        t = (uint32_t)cur | (uint32_t)a << 8 | (uint32_t)b << 16;
+                        pa = sm.plane[a];
+                        pb = sm.plane[b];
+                        q = cn1 * sm.pn1[a] * sm.pn1[b] * 1.000001f;
+                        m = fmaxf(cmx, fmaxf(sm.pmx[a], sm.pmx[b]));
+                        sm.tri[gs] = t;
+                        sm.adj[0][gs] = sm.newslot[sm.by_end[a]];
+                        sm.adj[1][gs] = (uint8_t)K;
+                        sm.adj[2][gs] = sm.newslot[sm.by_start[b]];
+                        const uint32_t tk = sm.tri[K];
+                        for (int f = 0; f < 3; f++)
+                            if (tri_p(tk, f) == b && tri_p(tk, (f + 1) % 3) == a) sm.adj[f][K] = (uint8_t)gs;
+                        double px, py, pz, pw, d2;
+                        vertex_numerics<D>(e, pa, pb, sx, sy, sz, px, py, pz, pw, d2);
+                        slot_write<D, S>(R, s, px, py, pz, pw, d2, q, m);
+                    } else {
+                        const int L = ed & 0xff, K = (ed >> 16) & 0xff;
+                        pa = sm.plane[L];
+                        const double rw = det2(e.x, e.y, pa.x, pa.y);
+                        const bool swap = rw > 0;
+                        t = swap ? ((uint32_t)L | (uint32_t)cur << 8) : ((uint32_t)cur | (uint32_t)L << 8);
+                        const uint8_t other = sm.newslot[1 - r];
+                        sm.tri[gs] = t;
+                        sm.adj[0][gs] = swap ? (uint8_t)K : other;
+                        sm.adj[1][gs] = swap ? other : (uint8_t)K;
+                        const uint32_t tk = sm.tri[K];
+                        for (int f = 0; f < 2; f++)
+                            if (tri_p(tk, f) == L) sm.adj[f][K] = (uint8_t)gs;
+                        q = cn1 * sm.pn1[L] * 1.000001f;
+                        m = fmaxf(cmx, sm.pmx[L]);
+                        double px, py, pz, pw, d2;
+                        if (swap)
+                            vertex_numerics<D>(pa, e, e, sx, sy, sz, px, py, pz, pw, d2);
+                        else
+                            vertex_numerics<D>(e, pa, pa, sx, sy, sz, px, py, pz, pw, d2);
+                        slot_write<D, S>(R, s, px, py, pz, pw, d2, q, m);
+                    }
+                }
+                nlive += Bt;
+            }
+            block_sync();
         }
-        block_sync();
         // -- certificate (cell.cpp:369-384) with the just-clipped neighbour;
         //    the largest seed-vertex distance only changes with a clip
         double mx = R2;
-        if (wave_ballot(doclip) != 0ull) {
+        if (anyclip) {
             mx = 0.0;
 #pragma unroll
             for (int s = 0; s < S; s++)
@@ -714,9 +789,13 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             R2 = mx;
             const double dx = bx - sx, dy = by - sy, dz = bz - sz;
             const double dz2 = dx * dx + dy * dy + dz * dz;
-            if (dz2 > 4 * R2) {
-                kc = v + 1;
-                done = true;
+            if (dz2 > 4 * R2 * (EXACT ? CERT_TIE : 1.0)) {
+                if (!EXACT && xmode && status == SUCCESS && !(dz2 > 4 * R2 * CERT_TIE)) {
+                    status = NEEDS_EXACT_PREDICATES;  // a near tie: the exact kernel decides
+                } else {
+                    kc = v + 1;
+                    done = true;
+                }
             }
             if (status != SUCCESS) done = true;
         }
@@ -731,7 +810,10 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
     if (valid) {
         const double dx = pts[(size_t)D * zl] - sx, dy = pts[(size_t)D * zl + 1] - sy;
         const double dz = D == 3 ? pts[(size_t)D * zl + 2] - sz : 0.0;
-        if (!(dx * dx + dy * dy + dz * dz > 4 * R2)) {
+        const double dl2 = dx * dx + dy * dy + dz * dz;
+        if (!EXACT && xmode && status == SUCCESS && dl2 > 4 * R2 && !(dl2 > 4 * R2 * CERT_TIE))
+            status = NEEDS_EXACT_PREDICATES;
+        if (!(dl2 > 4 * R2 * (EXACT ? CERT_TIE : 1.0))) {
             if ((int64_t)k >= n_total - 1 && status == SUCCESS)
                 kc = k;
             else if (status != SLOT_OVERFLOW)
@@ -946,6 +1028,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             const int fi = popc64(fm & ((1ull << p) - 1ull));
             const int64_t at = (int64_t)fi * st + sid;
             fa.nbr[at] = pv[i];
+            if (fa.nbr_cm) fa.nbr_cm[(int64_t)sid * FACE_SLOTS + fi] = pv[i];
             fa.area[at] = farea[i];
             const int64_t plane_stride = st * FACE_SLOTS;
             fa.nrm[at] = nx[i];

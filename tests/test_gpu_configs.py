@@ -3,8 +3,9 @@ properties (the oracle cannot run them in seconds):
 
 * clustered 3D (4M points, 64 Gaussians with sigma log-uniform in
   [0.002, 0.05] plus 20% uniform, periodic): the learned-k mesh equals the
-  fixed-k mesh array for array, statuses included; fewer than 2% of the
-  cells stay uncertified at k_max = 256 (the reference's status 3);
+  fixed-k mesh array for array, statuses included; certificate restarts
+  go past 256 neighbours, and fewer than 1% of the cells stay uncertified
+  at k_max = 1024 (the reference's status 3, or its capacity statuses);
 * 256 Kelvin-Helmholtz boxes (sizes log-uniform in [1000, 65536]), 50 batched
   steps: mass and energy of every box conserved, and the smallest boxes
   bit-identical to running them alone.
@@ -51,13 +52,19 @@ def test_clustered_4m_learned_equals_fixed(ctx):
           f"uncertified {learned.n_failed}", flush=True)
     for key in a:
         assert np.array_equal(a[key], b[key]), key
-    # Seeds beside the densest clusters need more than k_max = 256 neighbours
-    # (the cluster fills the whole row): they keep the reference's status 3,
-    # identically in both builds.  Every other cell is certified and stored.
+    # Seeds beside the densest clusters need more than k_max = 1024
+    # neighbours (the cluster fills the whole row): they keep the reference's
+    # status 3, identically in both builds; a few run into the reference's
+    # capacities on the way (_MAX_P_ = 64 planes, _MAX_T_ = 96 vertices:
+    # statuses 1 and 0).  Every other cell is certified and stored.
     st = a["status"]
-    assert np.all((st == 4) | (st == 3))
-    assert learned.n_failed == np.count_nonzero(st == 3) < 0.02 * len(st)
-    assert np.all(a["face_counts"][st == 3] == 0) and np.all(a["face_counts"][st == 4] > 0)
+    print(f"clustered: {np.count_nonzero(st == 3)} of {len(st)} seeds uncertified at k_max = 1024, "
+          f"{np.count_nonzero(st < 3)} over capacity, largest certified k {a['kcert'].max()}", flush=True)
+    assert np.all((st == 4) | (st == 3) | (st == 1) | (st == 0))
+    assert learned.n_failed == np.count_nonzero(st != 4) < 0.01 * len(st)
+    assert np.count_nonzero(st < 3) < 1e-4 * len(st)
+    assert a["kcert"].max() > 256
+    assert np.all(a["face_counts"][st != 4] == 0) and np.all(a["face_counts"][st == 4] > 0)
 
 
 def test_kh_256_boxes_50_steps(ctx):
