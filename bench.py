@@ -57,6 +57,8 @@ def parse():
                     help="size of the streaming-copy probe (measured HBM bandwidth)")
     ap.add_argument("--train-steps", type=int, default=400, help="Adam steps of the k classifier")
     ap.add_argument("--no-boxes", action="store_true", help="skip the 256-box Kelvin-Helmholtz batch timing")
+    ap.add_argument("--table", action="store_true",
+                    help="measure the five workload configurations instead (one JSON line each)")
     ap.add_argument("--only", choices=["plain", "learned", "batched"], default=None,
                     help="run one variant only (for profiling): the fixed-k host step, the learned-k host "
                          "step, or the batched fixed- and learned-k steps")
@@ -238,11 +240,200 @@ def stage_dict(st):
     return {k: round(v, 3) for k, v in st.items() if k.endswith("_ms")}
 
 
+def config_c1(ctx, api, gamma, cfl):
+    """Config C1: 2D, 10,000 uniform generators, reflective box, Sod, k = 16,
+    20 moving steps at CFL 0.3: GPU step time, the oracle's on the host (all
+    threads and one), cell by cell difference after the 20 steps."""
+    from oracle import oracle as O
+    from proteusgpu_amd.generators import sod_state, uniform_points
+    phase("config C1 (2D 10k Sod, 20 moving steps)")
+    p1 = uniform_points(10000, 2, 1001)
+    r1, v1, e1 = sod_state(p1, gamma)
+    kw1 = dict(boundary="reflective", k=16, gamma=gamma, cfl=cfl, flux="physical", move_mesh="fluid")
+    g1 = api.Simulation(ctx, p1, r1, v1, e1, **kw1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        g1.step()
+    ctx.sync()
+    g1_ms = (time.perf_counter() - t0) * 1e3 / 20
+    oms = {}
+    cores = O.get_threads()
+    for threads in (cores, 1):
+        O.set_threads(threads)
+        o1 = O.Sim(p1, r1, v1, e1, boundary="reflective", k=16, gamma=gamma, cfl=cfl, flux="physical",
+                   move="fluid")
+        t0 = time.perf_counter()
+        for _ in range(20):
+            o1.step()
+        oms[threads] = (time.perf_counter() - t0) * 1e3 / 20
+        O.set_threads(cores)
+    xg, rg, vg, eg = g1.get()
+    xo, ro, vo, eo = o1.get()
+    out = {"gpu_ms_per_step": round(g1_ms, 3), "gpu_cells_per_s": round(1e4 / (g1_ms * 1e-3), 1),
+           "oracle_ms_per_step": {str(t): round(v, 2) for t, v in oms.items()},
+           "oracle_cells_per_s": {str(t): round(1e4 / (v * 1e-3), 1) for t, v in oms.items()},
+           "state_rel_err_vs_oracle": {"rho": float(np.max(np.abs(rg - ro) / ro)),
+                                       "E": float(np.max(np.abs(eg - eo) / eo)),
+                                       "x": float(np.max(np.abs(xg - xo)))},
+           "topology_bit_exact": bool(np.array_equal(g1.mesh_copy_out()["neighbor_cell"],
+                                                     o1.mesh().get("neighbor_cell")))}
+    g1.free()
+    return out
+
+
+def oracle_steps(O, mk, steps, threads):
+    """Mean seconds per oracle step of the simulation mk() makes (its
+    creation, the first rebuild, is timed apart)."""
+    cores = O.get_threads()
+    O.set_threads(threads)
+    t0 = time.perf_counter()
+    sim = mk()
+    t_make = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    t_step = (time.perf_counter() - t0) / steps
+    O.set_threads(cores)
+    return t_make, t_step
+
+
+def table(a):
+    """The workload configurations, one JSON line each: GPU throughput of the
+    moving-mesh step (rebuild included), the oracle on this host's cores on
+    a bounded sample (named in each line), and the parity figures that fit
+    the oracle's time."""
+    from oracle import oracle as O
+    from proteusgpu_amd import api
+    from proteusgpu_amd.generators import (batch_points, box_sizes, gaussian_mixture, jittered_lattice,
+                                           kelvin_helmholtz_state, sod_state, uniform_points)
+    ctx = api.Context(0)
+    cores = O.get_threads()
+    gamma, cfl = 1.4, 0.3
+    emit = lambda d: print(json.dumps(d), flush=True)  # noqa: E731
+
+    c1 = config_c1(ctx, api, gamma, cfl)
+    emit({"config": "C1", "workload": "2D 10000 uniform seeds, reflective, Sod, k=16, 20 moving steps", **c1,
+          "cpu_cores": cores, "cpu": cpu_model()})
+
+    # C2: 1M 3D Sod, fixed k = 32: one rebuild + 10 static HLL steps
+    phase("C2: 1M 3D Sod, 1 rebuild + 10 HLL")
+    pts = uniform_points(1_000_000, 3, 42)
+    rho, vel, E = sod_state(pts, gamma)
+    sim = api.Simulation(ctx, pts, rho, vel, E, boundary="reflective", k=32, gamma=gamma, cfl=cfl,
+                         flux="physical", move_mesh=True)
+    sched = [schedule_ms(ctx, sim) for _ in range(3)]
+    sim.free()
+    st, _, _ = oracle_schedule(O, pts, rho, vel, E, 32, gamma, cfl, 10)
+    emit({"config": "C2", "workload": "3D 1000000 uniform seeds, reflective, Sod, k=32 + restarts, "
+                                      "one rebuild + 10 static HLL steps",
+          "gpu_ms": round(float(np.median(sched)), 3),
+          "oracle_s": round(st["build_s"] + 10 * (st["dt_s"] + st["hll_s"]), 3), "cpu_cores": cores,
+          "oracle_stages_s": {k: round(v, 4) for k, v in st.items()}})
+
+    # C3: 4M clustered, periodic, learned k vs fixed k: mesh builds (a
+    # simulation refuses meshes with uncertified cells, and cells beside the
+    # densest clusters stay uncertified at k_max = 1024)
+    phase("C3: 4M clustered, learned vs fixed k")
+    n3 = 4_000_000
+    p3 = gaussian_mixture(n3, 64, 0.002, 0.05, 0.2, 3)
+    km = api.KModel(ctx, seed=9)
+    sub = api.Mesh(ctx, p3[::10], boundary="periodic", k=32)
+    km.train_mesh(sub, steps=300, batch=4096, lr=3e-3)
+    sub.free()
+    res, fails = {}, {}
+    for name in ("fixed", "learned"):
+        ts = []
+        for _ in range(3):
+            ctx.sync()
+            t0 = time.perf_counter()
+            m3 = api.Mesh(ctx, p3, boundary="periodic", k=32, k_max=1024, kmodel=km if name == "learned" else None)
+            ctx.sync()
+            ts.append((time.perf_counter() - t0) * 1e3)
+            info = {"n_failed": m3.n_failed, "n_seeds": m3.n_seeds}
+            m3.free()
+        res[name] = float(np.median(ts))
+        fails[name] = info["n_failed"]
+    km.free()
+    sub_n = 400_000
+    ps = gaussian_mixture(sub_n, 64, 0.002, 0.05, 0.2, 3)
+    t0 = time.perf_counter()
+    om = O.Mesh(ps, "periodic", buff=0.04, k=32, restart=True, k_max=1024)
+    o_s = time.perf_counter() - t0
+    emit({"config": "C3", "workload": f"3D {n3} points (64 Gaussians, sigma in [0.002, 0.05], 20% uniform), "
+                                      "periodic, k=32 + restarts to 1024, one mesh build",
+          "gpu_ms_per_build": {k: round(v, 3) for k, v in res.items()},
+          "gpu_cells_per_s": {k: round(n3 / (v * 1e-3), 1) for k, v in res.items()},
+          "gpu_note": "host-driven mesh build from host points (copy to the device included), median of 3",
+          "learned_speedup": round(res["fixed"] / res["learned"], 4),
+          "uncertified": {k: int(v) for k, v in fails.items()}, "seeds": int(info["n_seeds"]),
+          "oracle_build_s": round(o_s, 3), "oracle_cells_per_s": round(sub_n / o_s, 1),
+          "oracle_sample": f"one mesh build of {sub_n} points of the same mixture, {cores} threads",
+          "oracle_uncertified": int(om.n_failed)})
+
+    # C4: 256 Kelvin-Helmholtz boxes, batched
+    phase("C4: 256 KH boxes, batched")
+    bs = box_sizes(256, 1000, 65536, 11)
+    bp = batch_points(bs, 2, 500)
+    br, bv, bE = kelvin_helmholtz_state(bp)
+    kw4 = dict(boundary="periodic", buff=0.1, k=16, gamma=5.0 / 3.0, cfl=0.3, flux="physical",
+               move_mesh="regularized")
+    kh = api.Batch(ctx, bs, bp, br, bv, bE, **kw4)
+    kh.autotune(reps=1)
+    ms4, gpu4 = run_batch(ctx, kh, 3, 20)
+    kh.free()
+    off = np.concatenate([[0], np.cumsum(bs)])
+    pick = np.random.default_rng(5).choice(len(bs), 16, replace=False)
+    o_cells = int(bs[pick].sum())
+    o_t = 0.0
+    for b in pick:
+        sl = slice(off[b], off[b + 1])
+        _, t_step = oracle_steps(O, lambda: O.Sim(bp[sl], br[sl], bv[sl], bE[sl], boundary="periodic", buff=0.1,
+                                                  k=16, gamma=5.0 / 3.0, cfl=0.3, move="regularized"), 3, cores)
+        o_t += t_step
+    emit({"config": "C4", "workload": f"256 periodic 2D boxes ({int(bs.sum())} cells, sizes log-uniform in "
+                                      "[1000, 65536]), Kelvin-Helmholtz, regularised moving mesh, k=16",
+          "gpu_ms_per_step": round(ms4, 3), "gpu_cells_per_s": round(float(bs.sum()) / (ms4 * 1e-3), 1),
+          "oracle_cells_per_s": round(o_cells / o_t, 1),
+          "oracle_sample": f"16 of the boxes ({o_cells} cells) one after another, 3 steps each, {cores} threads"})
+
+    # C5: 8M Sedov, learned k, batched step
+    phase("C5: 8M Sedov")
+    g5, m5 = 5.0 / 3.0, 200
+    p5 = jittered_lattice(m5, 3, 23)
+    n5 = len(p5)
+    d2 = np.sum((p5 - 0.5) ** 2, axis=1)
+    hot = np.argsort(d2, kind="stable")[:32]
+    r5 = np.ones(n5)
+    v5 = np.zeros((n5, 3))
+    e5 = np.full(n5, 1e-5 / (g5 - 1.0))
+    e5[hot] += (1.0 / 32.0) * m5 ** 3
+    kw5 = dict(boundary="reflective", k=32, gamma=g5, cfl=0.3, flux="physical", move_mesh="regularized")
+    B5 = api.Batch(ctx, np.array([n5]), p5, r5, v5, e5, **kw5)
+    ms5, _ = run_batch(ctx, B5, 2, 10)
+    B5.free()
+    m_s = 100
+    ps5 = jittered_lattice(m_s, 3, 23)
+    ns5 = len(ps5)
+    rs5, vs5 = np.ones(ns5), np.zeros((ns5, 3))
+    es5 = np.full(ns5, 1e-5 / (g5 - 1.0))
+    t_make, t_step = oracle_steps(O, lambda: O.Sim(ps5, rs5, vs5, es5, boundary="reflective", k=32, gamma=g5,
+                                                   cfl=0.3, move="regularized"), 1, cores)
+    emit({"config": "C5", "workload": f"3D {n5} jittered-lattice seeds, reflective, Sedov (32 hot cells), "
+                                      "regularised moving mesh, batched step, fixed k=32 + restarts",
+          "gpu_ms_per_step": round(ms5, 3), "gpu_cells_per_s": round(n5 / (ms5 * 1e-3), 1),
+          "oracle_cells_per_s": round(ns5 / t_step, 1),
+          "oracle_sample": f"100^3 jittered lattice, one moving step, {cores} threads"})
+    ctx.close()
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if a.table:
+        return table(a)
     from proteusgpu_amd import api
     from proteusgpu_amd.generators import sod_state, uniform_points
 
@@ -396,46 +587,9 @@ def main():
                "dt": float(max(dts_g))}
         hs.free()
         m.free()
-    # 5. config C1 beside the headline: 2D, 10,000 uniform generators,
-    #    reflective box, Sod, k = 16, 20 moving steps at CFL 0.3; GPU step
-    #    time, the oracle's on the host (all threads and one), cell by cell
-    #    difference after the 20 steps
-    c1 = None
-    if not a.no_cpu:
-        phase("config C1 (2D 10k Sod, 20 moving steps)")
-        from oracle import oracle as O
-        p1 = uniform_points(10000, 2, 1001)
-        r1, v1, e1 = sod_state(p1, gamma)
-        kw1 = dict(boundary="reflective", k=16, gamma=gamma, cfl=cfl, flux="physical", move_mesh="fluid")
-        g1 = api.Simulation(ctx, p1, r1, v1, e1, **kw1)
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(20):
-            g1.step()
-        ctx.sync()
-        g1_ms = (time.perf_counter() - t0) * 1e3 / 20
-        oms = {}
-        for threads in (O.get_threads(), 1):
-            cores = O.get_threads()
-            O.set_threads(threads)
-            o1 = O.Sim(p1, r1, v1, e1, boundary="reflective", k=16, gamma=gamma, cfl=cfl, flux="physical",
-                       move="fluid")
-            t0 = time.perf_counter()
-            for _ in range(20):
-                o1.step()
-            oms[threads] = (time.perf_counter() - t0) * 1e3 / 20
-            O.set_threads(cores)
-        xg, rg, vg, eg = g1.get()
-        xo, ro, vo, eo = o1.get()
-        c1 = {"gpu_ms_per_step": round(g1_ms, 3), "gpu_cells_per_s": round(1e4 / (g1_ms * 1e-3), 1),
-              "oracle_ms_per_step": {str(t): round(v, 2) for t, v in oms.items()},
-              "oracle_cells_per_s": {str(t): round(1e4 / (v * 1e-3), 1) for t, v in oms.items()},
-              "state_rel_err_vs_oracle": {"rho": float(np.max(np.abs(rg - ro) / ro)),
-                                          "E": float(np.max(np.abs(eg - eo) / eo)),
-                                          "x": float(np.max(np.abs(xg - xo)))},
-              "topology_bit_exact": bool(np.array_equal(g1.mesh_copy_out()["neighbor_cell"],
-                                                        o1.mesh().get("neighbor_cell")))}
-        g1.free()
+    # 5. config C1 beside the headline (its GPU step, the oracle's, cell by
+    #    cell difference)
+    c1 = None if a.no_cpu else config_c1(ctx, api, gamma, cfl)
     tot = blearn.totals()[0]
     best_fixed_k = min(fixed, key=lambda kk: fixed[kk]["ms_per_step"])
     ms_best_fixed = fixed[best_fixed_k]["ms_per_step"]
