@@ -126,7 +126,12 @@ typedef struct {
     int k;         /* neighbours per seed (Config.sh _K_)                        */
     int max_p;     /* plane capacity per cell (Config.sh _MAX_P_, <= 64)         */
     int max_t;     /* vertex capacity per cell (Config.sh _MAX_T_, <= 128)       */
-    int restart;   /* nonzero: recompute uncertified seeds at larger k           */
+    int restart;   /* nonzero: recompute uncertified seeds at larger k; seeds    *
+                    * still uncertified at k_max are finished by the vertex-ball *
+                    * completion (their first min(k_max, 64) neighbours, then   *
+                    * the points inside their vertices' empty-sphere balls      *
+                    * until every ball is empty; kcert = k_max + 1; host-driven *
+                    * builds only)                                               */
     int k_max;     /* largest k a restart may use (default and limit 1024;      *
                     * batches: at most 256)                                      */
     /* Degeneracy fallback (no reference counterpart; the reference leaves
@@ -181,6 +186,8 @@ typedef struct {
     double displacement;  /* amplitude of the displacement the mesh was built     *
                            * with (0: the generators as given)                    */
     int64_t n_exact;      /* cells clipped again by the exact-predicate fallback */
+    int64_t n_ball;       /* cells the vertex-ball completion finished (kcert =   *
+                           * k_max + 1 when it certified them)                    */
 } pg_mesh_info_t;
 
 /* voronoi::compute_mesh (voronoi.h:68, voronoi.cpp:17-65) for boundary NONE,

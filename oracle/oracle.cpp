@@ -32,6 +32,8 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <array>
+#include <numeric>
 #include <cfloat>
 #include <chrono>
 #include <climits>
@@ -827,6 +829,7 @@ struct Mesh {
     std::vector<int64_t> fcount, fptr, ghost_ids;
     std::vector<int32_t> nbr, status, kcert;
     int64_t n_failed = 0;
+    int64_t n_ball = 0;  // cells finished by the vertex-ball completion
 };
 
 // knn.cpp:16-117 + 119-204
@@ -1024,6 +1027,173 @@ int run_cell(int dim, int maxp, int maxt, const double* sorted, const uint32_t* 
     return kc;
 }
 
+// Vertex-ball completion (no reference counterpart).  A cell whose
+// certificate no neighbour row of k_max entries reaches (clustered input: a
+// sparse cell beside a dense cluster fills its whole row with cluster points
+// while its far vertices keep the security radius large) is finished by the
+// empty-sphere test instead: after the first kb neighbours of its row, while
+// the ball of some vertex v (centre v, radius |v - s|) strictly contains a
+// point that is not one of the cell's planes, the cell is clipped by the
+// nearest such point (smallest input index on ties) of the first vertex
+// whose ball holds one; when no ball does, every vertex is a vertex of the
+// Voronoi diagram and the cell is certified.  A ball found empty stays empty
+// while its vertex lives (the points do not change), so each vertex is
+// searched once.  Faces after the row follow their neighbours' (distance,
+// index) order, so a cell's faces keep the neighbour-row order.
+struct GridView {
+    int dim, N;
+    const double* sorted;
+    const int32_t* cnt;
+    const int32_t* start;
+    const uint32_t* perm;
+};
+
+// grid rings a ball search may visit (beyond: the completion gives up)
+const int BALL_RINGS = 32;
+
+// nearest point strictly inside the ball (v, r2); -1: none; -2: the search
+// would need more than BALL_RINGS rings
+int64_t ball_point(const GridView& g, const Cell& c, const P4& v, double r2) {
+    const int N = g.N, dim = g.dim;
+    const double vv[3] = {v.x, v.y, v.z};
+    int cc[3] = {0, 0, 0};
+    for (int d = 0; d < dim; d++) cc[d] = std::max(0, std::min(N - 1, (int)std::floor(vv[d] * N)));
+    double best = r2;
+    int64_t pick = -1;
+    const double h = 1.0 / N;
+    for (int ring = 0;; ring++) {
+        if (ring > 0) {
+            const double lb = (ring - 1) * h;
+            if (lb > 0 && lb * lb >= best) break;
+        }
+        if (ring > N) break;
+        if (ring > BALL_RINGS) return -2;
+        const int kz0 = dim == 3 ? -ring : 0, kz1 = dim == 3 ? ring : 0;
+        for (int dk = kz0; dk <= kz1; dk++)
+            for (int dj = -ring; dj <= ring; dj++)
+                for (int di = -ring; di <= ring; di++) {
+                    if (std::max(std::abs(di), std::max(std::abs(dj), std::abs(dk))) != ring) continue;
+                    const int ii = cc[0] + di, jj = cc[1] + dj, kk = cc[2] + dk;
+                    if (ii < 0 || jj < 0 || kk < 0 || ii >= N || jj >= N || (dim == 3 && kk >= N)) continue;
+                    // lower bound of the distance from v to the grid cell
+                    const int ix[3] = {ii, jj, kk};
+                    double lb2 = 0;
+                    for (int d = 0; d < dim; d++) {
+                        const double lo = ix[d] * h, hi = (ix[d] + 1) * h;
+                        const double e = vv[d] < lo ? lo - vv[d] : (vv[d] > hi ? vv[d] - hi : 0.0);
+                        lb2 += e * e;
+                    }
+                    if (lb2 >= best) continue;
+                    const int64_t cell = ii + (int64_t)jj * N + (int64_t)kk * N * N;
+                    for (int t = 0; t < g.cnt[cell]; t++) {
+                        const int64_t q = g.start[cell] + t;
+                        if (q == c.seed_id) continue;
+                        const double* pq = g.sorted + (size_t)dim * q;
+                        double d2 = 0;
+                        for (int d = 0; d < dim; d++) d2 += (pq[d] - vv[d]) * (pq[d] - vv[d]);
+                        // strictly inside; on ties the smaller input index
+                        if (d2 > best || (d2 == best && (pick < 0 || g.perm[q] > g.perm[pick]))) continue;
+                        bool plane = false;
+                        for (int pl = 0; pl < c.nb_v && !plane; pl++) plane = c.pvid[pl] == (int)q;
+                        if (plane) continue;
+                        best = d2;
+                        pick = q;
+                    }
+                }
+    }
+    return pick;
+}
+
+// The row's first kb neighbours, then vertex-ball completion; kcert on
+// success: kmark.
+int run_cell_ball(const GridView& g, int maxp, int maxt, const uint32_t* row, int kb, int sid, CellOut* out,
+                  int* status, int kmark) {
+    Cell c(g.dim, maxp, maxt, g.sorted, sid);
+    for (int v = 0; v < kb && c.status == success; v++) c.clip((int)row[v]);
+    const int first_ball = c.nb_v;
+    // per vertex (sorted plane triple): the nearest point inside its ball,
+    // or -1 when the ball is empty; valid while the vertex lives (a clip by p
+    // removes every vertex whose ball holds p)
+    std::vector<std::pair<std::array<uint8_t, 3>, int64_t>> known;
+    auto key3 = [&](int t) {
+        std::array<uint8_t, 3> a = {c.tp(t, 0), c.tp(t, 1), (uint8_t)(g.dim == 3 ? c.tp(t, 2) : 0)};
+        std::sort(a.begin(), a.end());
+        return a;
+    };
+    const double* sp = g.sorted + (size_t)g.dim * sid;
+    int iter = 0;
+    while (c.status == success) {
+        // the candidate nearest the seed ((distance, index) order) among the
+        // vertices' candidates: the completion follows the row's order
+        int64_t pick = -1;
+        double pd = 0;
+        bool far = false;
+        for (int t = 0; t < c.nb_t; t++) {
+            const auto kt = key3(t);
+            int64_t cand = -2;
+            for (auto& e : known)
+                if (e.first == kt) cand = e.second;
+            if (cand == -2) {
+                const P4 v = c.vertex(t);
+                const P4 df = sub4(v, c.seed);
+                cand = ball_point(g, c, v, dot3(df, df));
+                known.push_back({kt, cand});
+            }
+            if (cand == -2) far = true;
+            if (cand < 0) continue;
+            const double d2 = dist2(g.dim, g.sorted + (size_t)g.dim * cand, sp);
+            if (pick < 0 || d2 < pd || (d2 == pd && g.perm[cand] < g.perm[pick])) {
+                pick = cand;
+                pd = d2;
+            }
+        }
+        if (far) {
+            c.status = security_radius_not_reached;
+            break;
+        }
+        if (pick < 0) break;
+        if (++iter > 4 * maxp) {
+            c.status = security_radius_not_reached;
+            break;
+        }
+        c.clip((int)pick);
+    }
+    *status = c.status;
+    if (c.status != success) return 0;
+    if (out) {
+        extract(c, *out);
+        // faces of the planes added by the completion in (distance, index)
+        // order, after the row's
+        std::vector<int> idx(out->nbr.size());
+        std::iota(idx.begin(), idx.end(), 0);
+        auto dkey = [&](int f) {
+            const int q = out->nbr[f];
+            return std::make_pair(dist2(g.dim, g.sorted + (size_t)g.dim * q, sp), g.perm[q]);
+        };
+        // planes are stored in face order: faces from planes >= first_ball
+        // come after the others
+        int nrow = 0;
+        for (size_t f = 0; f < out->nbr.size(); f++) {
+            int pl = -1;
+            for (int q = 0; q < c.nb_v; q++)
+                if (c.pvid[q] == out->nbr[f]) pl = q;
+            if (pl < first_ball) nrow++;
+        }
+        std::sort(idx.begin() + nrow, idx.end(), [&](int a, int b) { return dkey(a) < dkey(b); });
+        CellOut o2 = *out;
+        for (size_t f = 0; f < idx.size(); f++) {
+            o2.nbr[f] = out->nbr[idx[f]];
+            o2.area[f] = out->area[idx[f]];
+            for (int d = 0; d < 3; d++) {
+                o2.normal[3 * f + d] = out->normal[3 * idx[f] + d];
+                o2.fcen[3 * f + d] = out->fcen[3 * idx[f] + d];
+            }
+        }
+        *out = o2;
+    }
+    return kmark;
+}
+
 double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
 // exact != 0: degenerate conflict tests are decided by exact_conflict, with
@@ -1126,6 +1296,28 @@ Mesh* build(int dim, const double* in_pts, int64_t n, int boundary, double buff,
             for (int64_t s : todo)
                 if (st[s] != success) left.push_back(s);
             todo.swap(left);
+        }
+        // seeds still uncertified at the largest row: vertex-ball completion
+        if (pass == 0) {
+            std::vector<int64_t> far;
+            for (int64_t s = 0; s < ntot; s++)
+                if (st[s] == security_radius_not_reached) far.push_back(s);
+            const int64_t nf = (int64_t)far.size();
+            if (nf > 0) {
+                const int kb = (int)std::min<int64_t>(std::min(kmax, 64), ntot - 1);
+                rows.resize((size_t)nf * kb);
+                knn_search_list(dim, m->sorted.data(), ntot, N, cnt.data(), start.data(), m->perm.data(), kb,
+                                far.data(), nf, rows.data());
+                const GridView gv{dim, N, m->sorted.data(), cnt.data(), start.data(), m->perm.data()};
+#pragma omp parallel for schedule(dynamic, 4)
+                for (int64_t r = 0; r < nf; r++) {
+                    const int64_t s = far[r];
+                    cells[s] = CellOut();
+                    kc[s] = run_cell_ball(gv, maxp, maxt, rows.data() + (size_t)r * kb, kb, (int)s, &cells[s],
+                                          &st[s], kmax + 1);
+                }
+                m->n_ball = nf;
+            }
         }
     }
     double t4 = now();

@@ -36,7 +36,8 @@ class MeshInfo(C.Structure):
     _fields_ = [("n_seeds", C.c_int64), ("n_hydro", C.c_int64), ("num_faces", C.c_int64),
                 ("n_failed", C.c_int64), ("n_restarted", C.c_int64), ("N_grid", C.c_int),
                 ("k_used", C.c_int), ("row_entries", C.c_int64), ("n_class", C.c_int64 * 6),
-                ("n_degenerate", C.c_int64), ("displacement", C.c_double), ("n_exact", C.c_int64)]
+                ("n_degenerate", C.c_int64), ("displacement", C.c_double), ("n_exact", C.c_int64),
+                ("n_ball", C.c_int64)]
 
 
 class VMeshView(C.Structure):

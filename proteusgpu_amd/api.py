@@ -261,6 +261,7 @@ class Mesh:
         self.n_class = np.array(info.n_class[:], dtype=np.int64)
         self.n_degenerate, self.displacement = info.n_degenerate, info.displacement
         self.n_exact = info.n_exact
+        self.n_ball = info.n_ball
 
     def features(self):
         """Learned-k features of every seed, original order (n_seeds, 16)."""

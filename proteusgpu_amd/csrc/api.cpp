@@ -309,6 +309,29 @@ int restart_chain(pg_mesh* m, const uint32_t* cur, int64_t nq, int k, hipStream_
     return kmax;
 }
 
+// Vertex-ball completion: seeds still uncertified after the restart chain
+// (its rows stop at the cap) are clipped by their first min(cap, 64)
+// neighbours and then by the points their vertex balls hold, until every
+// ball is empty (kcert = cap + 1).
+void ball_pass(pg_mesh* m, hipStream_t s) {
+    const pg_mesh_params& p = m->prm;
+    const size_t ns = (size_t)m->n_seeds;
+    uint32_t* L = m->list.get<uint32_t>(ns);
+    const int64_t nL = collect_list(m, nullptr, m->n_seeds, PG_SECURITY_RADIUS_NOT_REACHED, L, s);
+    m->info.n_ball = nL;
+    if (nL == 0) return;
+    const int cap = (int)std::min<int64_t>(std::min(std::max(p.k_max, p.k), KNN_MAX_K), m->n_seeds - 1);
+    const int kb = std::min(cap, 64);
+    uint32_t* rows = m->rows.get<uint32_t>((size_t)nL * kb);
+    pg::launch_knn(p.dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), m->n_seeds, m->g.N,
+                   m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), L, nL, kb, rows, s);
+    pg::launch_voronoi_ball(p.dim, m->g.sorted.as<double>(), m->n_seeds, L, nL, rows, kb, p.max_p, p.max_t,
+                            m->faces(), m->g.N, m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(),
+                            m->g.perm.as<uint32_t>(), cap + 1, s);
+    HIPCHK(hipGetLastError());
+    m->info.row_entries += nL * kb;
+}
+
 // Exact-predicate fallback (pg_mesh_params.exact): seeds left with
 // needs_exact_predicates are clipped again by the exact kernel from the
 // first neighbour at the configured k, then at growing k while uncertified
@@ -516,6 +539,7 @@ void mesh_build_once(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool o
         k_used = std::max(restart_chain(m, nullptr, nseeds, std::min<int64_t>(64, nseeds - 1), s),
                           *std::max_element(bk, bk + pg::LK_NCLASS));
     }
+    if (p.restart) ball_pass(m, s);
     if (p.exact) k_used = std::max(k_used, exact_chain(m, s));
     if (mk) mk->mark(MK_RESTART, s);
     m->info.n_seeds = nseeds;

@@ -105,6 +105,16 @@ void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* se
                     const uint32_t* knn, int k, int max_p, int max_t, int big, FaceArrays fa, int lanes, int slots,
                     hipStream_t s, bool restart = false, bool exact = false);
 
+// Vertex-ball completion (k_voronoi_ball): the seeds listed, uncertified at
+// the longest row, clipped by their rows of length k and then by the points
+// their vertex balls hold until every ball is empty (kcert = kmark); grid
+// tables of the sorted points (N per axis, starts, counts) and the input
+// index per sorted id.
+void launch_voronoi_ball(int dim, const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq,
+                         const uint32_t* knn, int k, int max_p, int max_t, FaceArrays fa, int N,
+                         const int32_t* starts, const int32_t* counts, const uint32_t* perm, int kmark,
+                         hipStream_t s);
+
 // Process-wide default shape of the fast clip kernel for the following
 // builds: lanes per seed (8 or 16) and vertex slots per seed (48 or 64).
 bool valid_clip_config(int lanes, int slots);

@@ -45,6 +45,17 @@ __global__ void __launch_bounds__(64, 1) k_voronoi_exact(const double* __restric
     voronoi_group<DIM, 16, 8, true>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm, blockIdx.x);
 }
 
+// Vertex-ball completion of seeds no row certifies (128 vertex slots).
+template <int DIM>
+__global__ void __launch_bounds__(64, 1) k_voronoi_ball(const double* __restrict__ pts, int64_t n_total,
+                                                        const uint32_t* __restrict__ seeds, int64_t nq,
+                                                        const uint32_t* __restrict__ knn, int k, int max_p,
+                                                        int max_t, FaceArrays fa, BallGrid bg) {
+    using C = VoroCfg<DIM, 16, 8>;
+    __shared__ SeedSmem<C::NS, C::MAXP> sm[C::SPW];
+    voronoi_group<DIM, 16, 8, false, true>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm, blockIdx.x, bg);
+}
+
 template <int DIM, int G, int S>
 void run(const double* pts, int64_t n_total, const uint32_t* seeds, int64_t nq, const uint32_t* knn, int k,
          int max_p, int max_t, FaceArrays fa, bool restart, hipStream_t s) {
@@ -142,6 +153,24 @@ void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* se
         launch_dim<2>(sorted, n, seeds, nq, knn, k, max_p, max_t, big, fa, s, restart, lanes, slots, exact);
     else
         launch_dim<3>(sorted, n, seeds, nq, knn, k, max_p, max_t, big, fa, s, restart, lanes, slots, exact);
+}
+
+void launch_voronoi_ball(int dim, const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq,
+                         const uint32_t* knn, int k, int max_p, int max_t, FaceArrays fa, int N,
+                         const int32_t* starts, const int32_t* counts, const uint32_t* perm, int kmark,
+                         hipStream_t s) {
+    if (nq <= 0) return;
+    BallGrid bg;
+    bg.N = N;
+    bg.starts = starts;
+    bg.counts = counts;
+    bg.perm = perm;
+    bg.kmark = kmark;
+    const int64_t nb = (nq + 3) / 4;  // 4 seeds per wave
+    if (dim == 2)
+        k_voronoi_ball<2><<<dim3((unsigned)nb), dim3(64), 0, s>>>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, bg);
+    else
+        k_voronoi_ball<3><<<dim3((unsigned)nb), dim3(64), 0, s>>>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, bg);
 }
 
 void launch_collect_status(int64_t nq, const uint32_t* seeds, const int32_t* status, int code,

@@ -266,6 +266,89 @@ __device__ __noinline__ int exact_conflict(uint32_t tt, const SeedSmem<NS, MAXP>
     return conflict_sos<DIM>(sp, key ? key[sid] : sid, rows);
 }
 
+// Vertex-ball completion (BALL kernels; no reference counterpart): a seed
+// whose certificate no row of k_max neighbours reaches is finished by the
+// empty-sphere test.  After the first kb neighbours of its row, while the
+// ball of some vertex v (centre v, radius |v - s|) strictly holds a point
+// that is not one of the cell's planes, the cell is clipped by the
+// candidate nearest the seed ((distance, input index) order) among the
+// vertices' candidates, each the point nearest its vertex (smallest input
+// index on ties); when no ball holds one, every vertex is a vertex of the
+// Voronoi diagram and the cell is certified (kcert = kmark).  A candidate
+// stays valid while its vertex lives: a clip by p removes every vertex whose
+// ball holds p.  A ball search visits grid rings out to BALL_RINGS; one that
+// would need more leaves the seed uncertified.
+constexpr int BALL_RINGS = 32;
+
+struct BallGrid {
+    int N = 0;                        // grid cells per axis of the sorted points
+    const int32_t* starts = nullptr;  // first sorted id per grid cell
+    const int32_t* counts = nullptr;  // points per grid cell
+    const uint32_t* perm = nullptr;   // input index per sorted id (ties)
+    int kmark = 0;                    // kcert of a cell the completion certifies
+};
+
+// The point nearest v strictly inside the ball (v, r2) that is neither the
+// seed nor a plane of the cell; -1: none; -2: beyond BALL_RINGS.
+template <int DIM>
+PG_DEV int32_t ball_search(const BallGrid& bg, const double* __restrict__ pts, const double (&v)[3], double r2,
+                           uint32_t sid, const int32_t* pvid, int nb_v) {
+    const int N = bg.N;
+    const double h = 1.0 / N;
+    int cc[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; d++) cc[d] = max(0, min(N - 1, (int)floor(v[d] * N)));
+    double best = r2;
+    int32_t pick = -1;
+    for (int ring = 0;; ring++) {
+        if (ring > 0) {
+            const double lb = (ring - 1) * h;
+            if (lb > 0 && lb * lb >= best) break;
+        }
+        if (ring > N) break;
+        if (ring > BALL_RINGS) return -2;
+        const int kr = DIM == 3 ? ring : 0;
+        for (int dk = -kr; dk <= kr; dk++)
+            for (int dj = -ring; dj <= ring; dj++) {
+                // the shell: full rows on its faces, two cells elsewhere
+                const bool full = (DIM == 3 && (dk == -ring || dk == ring)) || dj == -ring || dj == ring;
+                const int step = (full || ring == 0) ? 1 : 2 * ring;
+                for (int di = -ring; di <= ring; di += step) {
+                    const int ii = cc[0] + di, jj = cc[1] + dj, kk = cc[2] + dk;
+                    if (ii < 0 || jj < 0 || kk < 0 || ii >= N || jj >= N || (DIM == 3 && kk >= N)) continue;
+                    const int ix[3] = {ii, jj, kk};
+                    double lb2 = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        const double lo = ix[d] * h, hi = (ix[d] + 1) * h;
+                        const double e = v[d] < lo ? lo - v[d] : (v[d] > hi ? v[d] - hi : 0.0);
+                        lb2 += e * e;
+                    }
+                    if (lb2 >= best) continue;
+                    const int64_t cell = ii + (int64_t)jj * N + (int64_t)kk * N * N;
+                    const int32_t c0 = bg.starts[cell], cn = bg.counts[cell];
+                    for (int t = 0; t < cn; t++) {
+                        const int32_t q = c0 + t;
+                        if ((uint32_t)q == sid) continue;
+                        double d2 = 0.0;
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) {
+                            const double df = pts[(size_t)DIM * q + d] - v[d];
+                            d2 += df * df;
+                        }
+                        if (d2 > best || (d2 == best && (pick < 0 || bg.perm[q] > bg.perm[pick]))) continue;
+                        bool plane = false;
+                        for (int pl = 0; pl < nb_v && !plane; pl++) plane = pvid[pl] == q;
+                        if (plane) continue;
+                        best = d2;
+                        pick = q;
+                    }
+                }
+            }
+    }
+    return pick;
+}
+
 // Exact mode (fa.key set, or EXACT): a conflict test is trusted in floating
 // point only above XTOL times the product of its rows' 1-norms (w measured
 // by the magnitude its rounding is relative to), a valid bound on how far the
@@ -276,12 +359,14 @@ __device__ __noinline__ int exact_conflict(uint32_t tt, const SeedSmem<NS, MAXP>
 // vertex distance can still cut a zero-area face under the symbolic
 // perturbation, so only neighbours clearly beyond it certify the cell; the
 // other kernels flag a near tie.
-template <int DIM, int G, int S, bool EXACT = false>
+// BALL: the row is clipped whole, then the vertex-ball completion above
+// certifies the cell.
+template <int DIM, int G, int S, bool EXACT = false, bool BALL = false>
 __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, const uint32_t* __restrict__ seeds,
                               int64_t nq, const uint32_t* __restrict__ knn, int k, int max_p,
                               int max_t, FaceArrays fa,
                               SeedSmem<VoroCfg<DIM, G, S>::NS, VoroCfg<DIM, G, S>::MAXP>* smem,
-                              int64_t vblock) {
+                              int64_t vblock, BallGrid bg = {}) {
     using C = VoroCfg<DIM, G, S>;
     constexpr int NS = C::NS, NW = C::NW, MAXP = C::MAXP;
     constexpr int D = DIM;
@@ -441,6 +526,314 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
 #pragma unroll
     for (int d = 1; d < G; d <<= 1) R2 = smax(shfl_xor(R2, d, G), R2);
 
+    // One clip of the cell by the bisector of neighbour z at (bx, by, bz)
+    // (act: this group clips in this step; every lane of the wave calls it):
+    // new plane, conflict test of every live vertex, cavity, consistency and
+    // new vertices.  Returns whether any seed of the wave clipped.
+    auto clip_step = [&](bool act, uint32_t z, double bx, double by, double bz) -> bool {
+            // -- new half plane (cell.cpp:158-172)
+            Plane e = {0, 0, 0, 0};
+            double ewm = 0.0;  // magnitude of e.w for the bounds (see plane_bounds)
+            bool plane_ok = false;
+            int cur = 0;
+            if (act) {
+                if (nb_v >= max_p) {
+                    status = VERTEX_OVERFLOW;
+                } else if (nb_v >= MAXP) {
+                    status = SLOT_OVERFLOW;  // more planes than this kernel holds
+                } else {
+                    double dx = sx - bx, dy = sy - by, dz = sz - bz;
+                    double ax = sx + bx, ay = sy + by, az = sz + bz;
+                    double dot = ax * dx + ay * dy + az * dz;
+                    e = {dx, dy, dz, -dot / 2.0};
+                    ewm = xmode ? 0.5 * (fabs(ax * dx) + fabs(ay * dy) + fabs(az * dz)) : fabs(e.w);
+                    cur = nb_v;
+                    plane_ok = true;
+                    if (gl == 0) {
+                        sm.plane[cur] = e;
+                        plane_bounds<D>(e, ewm, sm.pn1[cur], sm.pmx[cur]);
+                        sm.pvid[cur] = (int32_t)z;
+                    }
+                    nb_v++;
+                }
+            }
+            // -- conflict test of every live vertex
+            uint32_t rem = 0;
+            bool flag = false;
+            if (act && plane_ok) {
+                const double en1 = (fabs(e.x) + fabs(e.y) + fabs(e.z) + ewm) * (1.0 + 1e-15);
+                const double emx = D == 2 ? fmax(fmax(fabs(e.x), fabs(e.y)), fabs(e.w))
+                                          : fmax(fmax(fabs(e.x), fabs(e.y)), fabs(e.z));
+    #pragma unroll
+                for (int s = 0; s < S; s++) {
+                    if (!((R.live >> s) & 1u)) continue;
+                    double t;
+                    if (D == 3)
+                        t = R.pw[s] * fma(e.x, R.px[s], fma(e.y, R.py[s], fma(e.z, R.pz[s], e.w)));
+                    else
+                        t = R.pw[s] * fma(e.x, R.px[s], fma(e.y, R.py[s], e.w));
+                    double M = fmax((double)R.m[s], emx);
+                    double M2 = M * M;
+                    double epsu = D == 3 ? 1.0000001e-14 * M2 * M2 * M : 1.0000001e-14 * M2 * M2;
+                    // exact mode: below XTOL en1 q the rounding of the planes
+                    // themselves may flip the sign (the reference's eps does not
+                    // bound it): such tests go to the exact predicate
+                    const double xb = xmode ? XTOL * en1 * (double)R.q[s] : 0.0;
+                    double bnd = 5.4e-15 * en1 * (double)R.q[s] + epsu + xb;
+                    bool c;
+                    if (fabs(t) > bnd) {
+                        c = t > 0.0;
+                    } else {
+                        uint32_t tt = sm.tri[s * G + gl];
+                        const Plane p1 = sm.plane[tri_p(tt, 0)];
+                        const Plane p2 = sm.plane[tri_p(tt, 1)];
+                        const Plane p3 = sm.plane[D == 3 ? tri_p(tt, 2) : tri_p(tt, 1)];
+                        if (EXACT) {
+                            bool fl = false;
+                            c = conflict_exact<D>(p1, p2, p3, e, fl, xb);
+                            if (fl) {
+                                const double sp[3] = {sx, sy, sz}, bp[3] = {bx, by, bz};
+                                const int r = exact_conflict<D>(tt, sm, pts, sid, sp, z, bp, fa.key);
+                                if (r == 0) flag = true;  // outside the exact predicate's range
+                                else c = r > 0;
+                            }
+                        } else {
+                            c = conflict_exact<D>(p1, p2, p3, e, flag, xb);
+                        }
+                    }
+                    if (c) rem |= 1u << s;
+                }
+            }
+            // -- group masks
+            uint64_t Rm[NW], Lm[NW];
+    #pragma unroll
+            for (int w = 0; w < NW; w++) Rm[w] = Lm[w] = 0ull;
+    #pragma unroll
+            for (int s = 0; s < S; s++) {
+                uint64_t b = (wave_ballot((rem >> s) & 1u) >> shift) & gmask;
+                int pos = s * G;
+                Rm[pos >> 6] |= b << (pos & 63);
+            }
+            const bool gflag = ((wave_ballot(flag) >> shift) & gmask) != 0ull;
+            int nb_r = 0;
+    #pragma unroll
+            for (int w = 0; w < NW; w++) nb_r += popc64(Rm[w]);
+            bool doclip = false;
+            if (act && plane_ok) {
+                R.live &= ~rem;  // removed vertices leave the cell (also on failure)
+                if (gflag) {
+                    status = NEEDS_EXACT_PREDICATES;
+                } else if (nb_r == 0) {
+                    nb_v--;  // the plane clipped nothing (cell.cpp:132-136)
+                } else {
+                    doclip = true;
+                }
+            }
+            // Most clip steps clip nothing (about 30 % of a uniform 3D seed's
+            // steps clip, and in half of a wave's steps none of its seeds does):
+            // the cavity, consistency and new-vertex machinery runs only when a
+            // seed of the wave clips (a wave-uniform branch; a block is one wave).
+            const bool anyclip = wave_ballot(doclip) != 0ull;
+            if (anyclip) {
+                // -- cavity boundary: edges of removed vertices whose neighbour is
+                //    kept; bit 3 s + e of bm = edge e of my slot s (a lane walks only
+                //    its removed slots and then only its boundary edges)
+                uint32_t bm = 0;
+                if (doclip) {
+                    uint32_t rr = rem;
+                    while (rr) {
+                        const int s = __builtin_ctz(rr);
+                        rr &= rr - 1u;
+                        const int gs = s * G + gl;
+    #pragma unroll
+                        for (int ee = 0; ee < D; ee++)
+                            if (!mask_has<NW>(Rm, sm.adj[ee][gs])) bm |= 1u << (3 * s + ee);
+                    }
+                }
+                const int mycnt = __builtin_popcount(bm);
+                int incl = mycnt;
+    #pragma unroll
+                for (int d = 1; d < G; d <<= 1) {
+                    int t = shfl_up(incl, d, G);
+                    if (gl >= d) incl += t;
+                }
+                const int Bt = shfl(incl, G - 1, G);
+                int j = incl - mycnt;
+                if (doclip) {
+                    // in (slot, edge) order, as counted
+                    uint32_t b2 = bm;
+                    while (b2) {
+                        const int bit = __builtin_ctz(b2);
+                        b2 &= b2 - 1u;
+                        if (j >= NS) {  // more boundary edges than slots: overflow below
+                            j++;
+                            continue;
+                        }
+                        const int s = bit / 3, ee = bit - 3 * s;
+                        const int gs = s * G + gl;
+                        const uint32_t t = sm.tri[gs];
+                        const int K = sm.adj[ee][gs];
+                        if (D == 3) {
+                            const int a = tri_p(t, ee), b = tri_p(t, ee == 2 ? 0 : ee + 1);
+                            sm.edge[j] = (uint32_t)a | (uint32_t)b << 8 | (uint32_t)K << 16;
+                            sm.by_start[a] = (uint8_t)j;
+                            sm.by_end[b] = (uint8_t)j;
+                        } else {
+                            const int L = tri_p(t, ee);
+                            sm.edge[j] = (uint32_t)L | (uint32_t)K << 16;
+                        }
+                        j++;
+                    }
+                }
+                // live mask of the group (after removal)
+    #pragma unroll
+                for (int s = 0; s < S; s++) {
+                    uint64_t b = (wave_ballot((R.live >> s) & 1u) >> shift) & gmask;
+                    int pos = s * G;
+                    Lm[pos >> 6] |= b << (pos & 63);
+                }
+                block_sync();
+                // -- consistency and capacity
+                bool ok = doclip;
+                bool bad = false;
+                int nxt = gl, lab = 0;  // 3D: successor edge and orbit label of edge gl
+                if (doclip) {
+                    if (D == 3) {
+                        // every boundary edge owns its start and end plane, and the
+                        // edge continuing it (the one starting at its end plane)
+                        // exists: the successor map is a permutation of the Bt edges
+                        for (int jj = gl; jj < Bt; jj += G) {
+                            uint32_t ed = sm.edge[jj];
+                            int a = ed & 0xff, b = (ed >> 8) & 0xff;
+                            if (sm.by_start[a] != jj || sm.by_end[b] != jj) bad = true;
+                            const int nx = sm.by_start[b];
+                            if (nx >= Bt || (int)(sm.edge[nx] & 0xffu) != b) bad = true;
+                        }
+                        if (Bt <= G && gl < Bt) {
+                            lab = gl;
+                            if (!bad) nxt = sm.by_start[(sm.edge[gl] >> 8) & 0xffu];
+                        }
+                    } else {
+                        bad = (Bt != 2);
+                    }
+                }
+                if (D == 3) {
+                    // ... and one cycle through all Bt edges: the smallest edge index
+                    // on each orbit, by pointer jumping over the group's lanes (all
+                    // groups of the wave take part; lanes without an edge point to
+                    // themselves), must be 0 everywhere
+                    if (nxt >= G) nxt = gl;  // only after a failed check above
+    #pragma unroll
+                    for (int r = 1; r < 16 && r < G; r <<= 1) {
+                        const int l2 = shfl(lab, nxt, G);
+                        nxt = shfl(nxt, nxt, G);
+                        lab = min(lab, l2);
+                    }
+                    if (doclip) {
+                        if (Bt <= G) {
+                            if (lab != 0) bad = true;
+                        } else if (gl == 0 && !bad) {
+                            int jj = 0, steps = 0;
+                            do {
+                                jj = sm.by_start[(sm.edge[jj] >> 8) & 0xffu];
+                                steps++;
+                            } while (jj != 0 && jj < Bt && steps <= Bt);
+                            if (steps != Bt || jj != 0) bad = true;
+                        }
+                    }
+                }
+                bad = ((wave_ballot(bad) >> shift) & gmask) != 0ull;
+                if (doclip) {
+                    const int nl_after = nlive - nb_r + Bt;
+                    if (bad) {
+                        status = INCONSISTENT_BOUNDARY;
+                        ok = false;
+                    } else if (nl_after >= max_t) {
+                        status = TRIANGLE_OVERFLOW;  // cell.cpp:347-350
+                        ok = false;
+                    } else if (nl_after > NS) {
+                        status = SLOT_OVERFLOW;
+                        ok = false;
+                    }
+                    nlive -= nb_r;
+                }
+                // -- slots of the new vertices: the j-th boundary edge takes the
+                //    j-th free slot (slot-id order)
+                uint32_t asg = 0;
+                if (ok) {
+    #pragma unroll
+                    for (int s = 0; s < S; s++) {
+                        const int gs = s * G + gl;
+                        if ((R.live >> s) & 1u) continue;
+                        // free-slot rank = gs - (live slots below gs)
+                        int r = gs - mask_rank<NW>(Lm, gs);
+                        if (r < Bt) {
+                            sm.newslot[r] = (uint8_t)gs;
+                            asg |= 1u << s;
+                        }
+                    }
+                }
+                block_sync();
+                // -- new vertices (cell.cpp:346-364 orientation, :388-411 position)
+                if (ok) {
+                    float cn1, cmx;
+                    plane_bounds<D>(e, ewm, cn1, cmx);
+                    while (asg) {
+                        const int s = ffs64(asg);
+                        asg &= asg - 1;
+                        const int gs = s * G + gl;
+                        const int r = gs - mask_rank<NW>(Lm, gs);
+                        const uint32_t ed = sm.edge[r];
+                        uint32_t t;
+                        Plane pa, pb;
+                        float q, m;
+                        if (D == 3) {
+                            const int a = ed & 0xff, b = (ed >> 8) & 0xff, K = (ed >> 16) & 0xff;
+                            t = (uint32_t)cur | (uint32_t)a << 8 | (uint32_t)b << 16;
+                            pa = sm.plane[a];
+                            pb = sm.plane[b];
+                            q = cn1 * sm.pn1[a] * sm.pn1[b] * 1.000001f;
+                            m = fmaxf(cmx, fmaxf(sm.pmx[a], sm.pmx[b]));
+                            sm.tri[gs] = t;
+                            sm.adj[0][gs] = sm.newslot[sm.by_end[a]];
+                            sm.adj[1][gs] = (uint8_t)K;
+                            sm.adj[2][gs] = sm.newslot[sm.by_start[b]];
+                            const uint32_t tk = sm.tri[K];
+                            for (int f = 0; f < 3; f++)
+                                if (tri_p(tk, f) == b && tri_p(tk, (f + 1) % 3) == a) sm.adj[f][K] = (uint8_t)gs;
+                            double px, py, pz, pw, d2;
+                            vertex_numerics<D>(e, pa, pb, sx, sy, sz, px, py, pz, pw, d2);
+                            slot_write<D, S>(R, s, px, py, pz, pw, d2, q, m);
+                        } else {
+                            const int L = ed & 0xff, K = (ed >> 16) & 0xff;
+                            pa = sm.plane[L];
+                            const double rw = det2(e.x, e.y, pa.x, pa.y);
+                            const bool swap = rw > 0;
+                            t = swap ? ((uint32_t)L | (uint32_t)cur << 8) : ((uint32_t)cur | (uint32_t)L << 8);
+                            const uint8_t other = sm.newslot[1 - r];
+                            sm.tri[gs] = t;
+                            sm.adj[0][gs] = swap ? (uint8_t)K : other;
+                            sm.adj[1][gs] = swap ? other : (uint8_t)K;
+                            const uint32_t tk = sm.tri[K];
+                            for (int f = 0; f < 2; f++)
+                                if (tri_p(tk, f) == L) sm.adj[f][K] = (uint8_t)gs;
+                            q = cn1 * sm.pn1[L] * 1.000001f;
+                            m = fmaxf(cmx, sm.pmx[L]);
+                            double px, py, pz, pw, d2;
+                            if (swap)
+                                vertex_numerics<D>(pa, e, e, sx, sy, sz, px, py, pz, pw, d2);
+                            else
+                                vertex_numerics<D>(e, pa, pa, sx, sy, sz, px, py, pz, pw, d2);
+                            slot_write<D, S>(R, s, px, py, pz, pw, d2, q, m);
+                        }
+                    }
+                    nlive += Bt;
+                }
+                block_sync();
+            }
+        return anyclip;
+    };
+
     // --------------------------------------------------------- clip loop
     // Seeds of a wave may start at different neighbours (resumed cells):
     // the wave walks v from the smallest start, a group joins at its own.
@@ -467,313 +860,17 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             }
             block_sync();
         }
-        // -- new half plane (cell.cpp:158-172)
+        // the step's neighbour: the prefetched row entry
         uint32_t z = 0;
         double bx = 0, by = 0, bz = 0;
-        Plane e = {0, 0, 0, 0};
-        double ewm = 0.0;  // magnitude of e.w for the bounds (see plane_bounds)
-        bool plane_ok = false;
-        int cur = 0;
         if (act) {
             const int jj = v - cbase;
             z = sm.nbi[jj];
             bx = sm.nbp[jj][0];
             by = sm.nbp[jj][1];
             bz = D == 3 ? sm.nbp[jj][2] : 0.0;
-            if (nb_v >= max_p) {
-                status = VERTEX_OVERFLOW;
-            } else if (nb_v >= MAXP) {
-                status = SLOT_OVERFLOW;  // more planes than this kernel holds
-            } else {
-                double dx = sx - bx, dy = sy - by, dz = sz - bz;
-                double ax = sx + bx, ay = sy + by, az = sz + bz;
-                double dot = ax * dx + ay * dy + az * dz;
-                e = {dx, dy, dz, -dot / 2.0};
-                ewm = xmode ? 0.5 * (fabs(ax * dx) + fabs(ay * dy) + fabs(az * dz)) : fabs(e.w);
-                cur = nb_v;
-                plane_ok = true;
-                if (gl == 0) {
-                    sm.plane[cur] = e;
-                    plane_bounds<D>(e, ewm, sm.pn1[cur], sm.pmx[cur]);
-                    sm.pvid[cur] = (int32_t)z;
-                }
-                nb_v++;
-            }
         }
-        // -- conflict test of every live vertex
-        uint32_t rem = 0;
-        bool flag = false;
-        if (act && plane_ok) {
-            const double en1 = (fabs(e.x) + fabs(e.y) + fabs(e.z) + ewm) * (1.0 + 1e-15);
-            const double emx = D == 2 ? fmax(fmax(fabs(e.x), fabs(e.y)), fabs(e.w))
-                                      : fmax(fmax(fabs(e.x), fabs(e.y)), fabs(e.z));
-#pragma unroll
-            for (int s = 0; s < S; s++) {
-                if (!((R.live >> s) & 1u)) continue;
-                double t;
-                if (D == 3)
-                    t = R.pw[s] * fma(e.x, R.px[s], fma(e.y, R.py[s], fma(e.z, R.pz[s], e.w)));
-                else
-                    t = R.pw[s] * fma(e.x, R.px[s], fma(e.y, R.py[s], e.w));
-                double M = fmax((double)R.m[s], emx);
-                double M2 = M * M;
-                double epsu = D == 3 ? 1.0000001e-14 * M2 * M2 * M : 1.0000001e-14 * M2 * M2;
-                // exact mode: below XTOL en1 q the rounding of the planes
-                // themselves may flip the sign (the reference's eps does not
-                // bound it): such tests go to the exact predicate
-                const double xb = xmode ? XTOL * en1 * (double)R.q[s] : 0.0;
-                double bnd = 5.4e-15 * en1 * (double)R.q[s] + epsu + xb;
-                bool c;
-                if (fabs(t) > bnd) {
-                    c = t > 0.0;
-                } else {
-                    uint32_t tt = sm.tri[s * G + gl];
-                    const Plane p1 = sm.plane[tri_p(tt, 0)];
-                    const Plane p2 = sm.plane[tri_p(tt, 1)];
-                    const Plane p3 = sm.plane[D == 3 ? tri_p(tt, 2) : tri_p(tt, 1)];
-                    if (EXACT) {
-                        bool fl = false;
-                        c = conflict_exact<D>(p1, p2, p3, e, fl, xb);
-                        if (fl) {
-                            const double sp[3] = {sx, sy, sz}, bp[3] = {bx, by, bz};
-                            const int r = exact_conflict<D>(tt, sm, pts, sid, sp, z, bp, fa.key);
-                            if (r == 0) flag = true;  // outside the exact predicate's range
-                            else c = r > 0;
-                        }
-                    } else {
-                        c = conflict_exact<D>(p1, p2, p3, e, flag, xb);
-                    }
-                }
-                if (c) rem |= 1u << s;
-            }
-        }
-        // -- group masks
-        uint64_t Rm[NW], Lm[NW];
-#pragma unroll
-        for (int w = 0; w < NW; w++) Rm[w] = Lm[w] = 0ull;
-#pragma unroll
-        for (int s = 0; s < S; s++) {
-            uint64_t b = (wave_ballot((rem >> s) & 1u) >> shift) & gmask;
-            int pos = s * G;
-            Rm[pos >> 6] |= b << (pos & 63);
-        }
-        const bool gflag = ((wave_ballot(flag) >> shift) & gmask) != 0ull;
-        int nb_r = 0;
-#pragma unroll
-        for (int w = 0; w < NW; w++) nb_r += popc64(Rm[w]);
-        bool doclip = false;
-        if (act && plane_ok) {
-            R.live &= ~rem;  // removed vertices leave the cell (also on failure)
-            if (gflag) {
-                status = NEEDS_EXACT_PREDICATES;
-            } else if (nb_r == 0) {
-                nb_v--;  // the plane clipped nothing (cell.cpp:132-136)
-            } else {
-                doclip = true;
-            }
-        }
-        // Most clip steps clip nothing (about 30 % of a uniform 3D seed's
-        // steps clip, and in half of a wave's steps none of its seeds does):
-        // the cavity, consistency and new-vertex machinery runs only when a
-        // seed of the wave clips (a wave-uniform branch; a block is one wave).
-        const bool anyclip = wave_ballot(doclip) != 0ull;
-        if (anyclip) {
-            // -- cavity boundary: edges of removed vertices whose neighbour is
-            //    kept; bit 3 s + e of bm = edge e of my slot s (a lane walks only
-            //    its removed slots and then only its boundary edges)
-            uint32_t bm = 0;
-            if (doclip) {
-                uint32_t rr = rem;
-                while (rr) {
-                    const int s = __builtin_ctz(rr);
-                    rr &= rr - 1u;
-                    const int gs = s * G + gl;
-#pragma unroll
-                    for (int ee = 0; ee < D; ee++)
-                        if (!mask_has<NW>(Rm, sm.adj[ee][gs])) bm |= 1u << (3 * s + ee);
-                }
-            }
-            const int mycnt = __builtin_popcount(bm);
-            int incl = mycnt;
-#pragma unroll
-            for (int d = 1; d < G; d <<= 1) {
-                int t = shfl_up(incl, d, G);
-                if (gl >= d) incl += t;
-            }
-            const int Bt = shfl(incl, G - 1, G);
-            int j = incl - mycnt;
-            if (doclip) {
-                // in (slot, edge) order, as counted
-                uint32_t b2 = bm;
-                while (b2) {
-                    const int bit = __builtin_ctz(b2);
-                    b2 &= b2 - 1u;
-                    if (j >= NS) {  // more boundary edges than slots: overflow below
-                        j++;
-                        continue;
-                    }
-                    const int s = bit / 3, ee = bit - 3 * s;
-                    const int gs = s * G + gl;
-                    const uint32_t t = sm.tri[gs];
-                    const int K = sm.adj[ee][gs];
-                    if (D == 3) {
-                        const int a = tri_p(t, ee), b = tri_p(t, ee == 2 ? 0 : ee + 1);
-                        sm.edge[j] = (uint32_t)a | (uint32_t)b << 8 | (uint32_t)K << 16;
-                        sm.by_start[a] = (uint8_t)j;
-                        sm.by_end[b] = (uint8_t)j;
-                    } else {
-                        const int L = tri_p(t, ee);
-                        sm.edge[j] = (uint32_t)L | (uint32_t)K << 16;
-                    }
-                    j++;
-                }
-            }
-            // live mask of the group (after removal)
-#pragma unroll
-            for (int s = 0; s < S; s++) {
-                uint64_t b = (wave_ballot((R.live >> s) & 1u) >> shift) & gmask;
-                int pos = s * G;
-                Lm[pos >> 6] |= b << (pos & 63);
-            }
-            block_sync();
-            // -- consistency and capacity
-            bool ok = doclip;
-            bool bad = false;
-            int nxt = gl, lab = 0;  // 3D: successor edge and orbit label of edge gl
-            if (doclip) {
-                if (D == 3) {
-                    // every boundary edge owns its start and end plane, and the
-                    // edge continuing it (the one starting at its end plane)
-                    // exists: the successor map is a permutation of the Bt edges
-                    for (int jj = gl; jj < Bt; jj += G) {
-                        uint32_t ed = sm.edge[jj];
-                        int a = ed & 0xff, b = (ed >> 8) & 0xff;
-                        if (sm.by_start[a] != jj || sm.by_end[b] != jj) bad = true;
-                        const int nx = sm.by_start[b];
-                        if (nx >= Bt || (int)(sm.edge[nx] & 0xffu) != b) bad = true;
-                    }
-                    if (Bt <= G && gl < Bt) {
-                        lab = gl;
-                        if (!bad) nxt = sm.by_start[(sm.edge[gl] >> 8) & 0xffu];
-                    }
-                } else {
-                    bad = (Bt != 2);
-                }
-            }
-            if (D == 3) {
-                // ... and one cycle through all Bt edges: the smallest edge index
-                // on each orbit, by pointer jumping over the group's lanes (all
-                // groups of the wave take part; lanes without an edge point to
-                // themselves), must be 0 everywhere
-                if (nxt >= G) nxt = gl;  // only after a failed check above
-#pragma unroll
-                for (int r = 1; r < 16 && r < G; r <<= 1) {
-                    const int l2 = shfl(lab, nxt, G);
-                    nxt = shfl(nxt, nxt, G);
-                    lab = min(lab, l2);
-                }
-                if (doclip) {
-                    if (Bt <= G) {
-                        if (lab != 0) bad = true;
-                    } else if (gl == 0 && !bad) {
-                        int jj = 0, steps = 0;
-                        do {
-                            jj = sm.by_start[(sm.edge[jj] >> 8) & 0xffu];
-                            steps++;
-                        } while (jj != 0 && jj < Bt && steps <= Bt);
-                        if (steps != Bt || jj != 0) bad = true;
-                    }
-                }
-            }
-            bad = ((wave_ballot(bad) >> shift) & gmask) != 0ull;
-            if (doclip) {
-                const int nl_after = nlive - nb_r + Bt;
-                if (bad) {
-                    status = INCONSISTENT_BOUNDARY;
-                    ok = false;
-                } else if (nl_after >= max_t) {
-                    status = TRIANGLE_OVERFLOW;  // cell.cpp:347-350
-                    ok = false;
-                } else if (nl_after > NS) {
-                    status = SLOT_OVERFLOW;
-                    ok = false;
-                }
-                nlive -= nb_r;
-            }
-            // -- slots of the new vertices: the j-th boundary edge takes the
-            //    j-th free slot (slot-id order)
-            uint32_t asg = 0;
-            if (ok) {
-#pragma unroll
-                for (int s = 0; s < S; s++) {
-                    const int gs = s * G + gl;
-                    if ((R.live >> s) & 1u) continue;
-                    // free-slot rank = gs - (live slots below gs)
-                    int r = gs - mask_rank<NW>(Lm, gs);
-                    if (r < Bt) {
-                        sm.newslot[r] = (uint8_t)gs;
-                        asg |= 1u << s;
-                    }
-                }
-            }
-            block_sync();
-            // -- new vertices (cell.cpp:346-364 orientation, :388-411 position)
-            if (ok) {
-                float cn1, cmx;
-                plane_bounds<D>(e, ewm, cn1, cmx);
-                while (asg) {
-                    const int s = ffs64(asg);
-                    asg &= asg - 1;
-                    const int gs = s * G + gl;
-                    const int r = gs - mask_rank<NW>(Lm, gs);
-                    const uint32_t ed = sm.edge[r];
-                    uint32_t t;
-                    Plane pa, pb;
-                    float q, m;
-                    if (D == 3) {
-                        const int a = ed & 0xff, b = (ed >> 8) & 0xff, K = (ed >> 16) & 0xff;
-                        t = (uint32_t)cur | (uint32_t)a << 8 | (uint32_t)b << 16;
-                        pa = sm.plane[a];
-                        pb = sm.plane[b];
-                        q = cn1 * sm.pn1[a] * sm.pn1[b] * 1.000001f;
-                        m = fmaxf(cmx, fmaxf(sm.pmx[a], sm.pmx[b]));
-                        sm.tri[gs] = t;
-                        sm.adj[0][gs] = sm.newslot[sm.by_end[a]];
-                        sm.adj[1][gs] = (uint8_t)K;
-                        sm.adj[2][gs] = sm.newslot[sm.by_start[b]];
-                        const uint32_t tk = sm.tri[K];
-                        for (int f = 0; f < 3; f++)
-                            if (tri_p(tk, f) == b && tri_p(tk, (f + 1) % 3) == a) sm.adj[f][K] = (uint8_t)gs;
-                        double px, py, pz, pw, d2;
-                        vertex_numerics<D>(e, pa, pb, sx, sy, sz, px, py, pz, pw, d2);
-                        slot_write<D, S>(R, s, px, py, pz, pw, d2, q, m);
-                    } else {
-                        const int L = ed & 0xff, K = (ed >> 16) & 0xff;
-                        pa = sm.plane[L];
-                        const double rw = det2(e.x, e.y, pa.x, pa.y);
-                        const bool swap = rw > 0;
-                        t = swap ? ((uint32_t)L | (uint32_t)cur << 8) : ((uint32_t)cur | (uint32_t)L << 8);
-                        const uint8_t other = sm.newslot[1 - r];
-                        sm.tri[gs] = t;
-                        sm.adj[0][gs] = swap ? (uint8_t)K : other;
-                        sm.adj[1][gs] = swap ? other : (uint8_t)K;
-                        const uint32_t tk = sm.tri[K];
-                        for (int f = 0; f < 2; f++)
-                            if (tri_p(tk, f) == L) sm.adj[f][K] = (uint8_t)gs;
-                        q = cn1 * sm.pn1[L] * 1.000001f;
-                        m = fmaxf(cmx, sm.pmx[L]);
-                        double px, py, pz, pw, d2;
-                        if (swap)
-                            vertex_numerics<D>(pa, e, e, sx, sy, sz, px, py, pz, pw, d2);
-                        else
-                            vertex_numerics<D>(e, pa, pa, sx, sy, sz, px, py, pz, pw, d2);
-                        slot_write<D, S>(R, s, px, py, pz, pw, d2, q, m);
-                    }
-                }
-                nlive += Bt;
-            }
-            block_sync();
-        }
+        const bool anyclip = clip_step(act, z, bx, by, bz);
         // -- certificate (cell.cpp:369-384) with the just-clipped neighbour;
         //    the largest seed-vertex distance only changes with a clip
         double mx = R2;
@@ -789,7 +886,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             R2 = mx;
             const double dx = bx - sx, dy = by - sy, dz = bz - sz;
             const double dz2 = dx * dx + dy * dy + dz * dz;
-            if (dz2 > 4 * R2 * (EXACT ? CERT_TIE : 1.0)) {
+            if (!BALL && dz2 > 4 * R2 * (EXACT ? CERT_TIE : 1.0)) {
                 if (!EXACT && xmode && status == SUCCESS && !(dz2 > 4 * R2 * CERT_TIE)) {
                     status = NEEDS_EXACT_PREDICATES;  // a near tie: the exact kernel decides
                 } else {
@@ -801,13 +898,89 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
         }
     }
 
+    // -- vertex-ball completion
+    const int first_ball = nb_v;
+    if (BALL) {
+        bool bdone = !valid || status != SUCCESS;
+        int32_t cand[S];
+        uint32_t ctri[S];
+        uint32_t known = 0;  // bit s: cand[s] belongs to the vertex now in slot s
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            cand[s] = -1;
+            ctri[s] = 0xffffffffu;
+        }
+        for (int iter = 0;; iter++) {
+            if (wave_ballot(!bdone) == 0ull) break;
+            // candidates of the live vertices not searched yet
+            double bd = 0.0;
+            int32_t bq = -1;
+            bool far = false;
+            if (!bdone) {
+#pragma unroll
+                for (int s = 0; s < S; s++) {
+                    if (!((R.live >> s) & 1u)) continue;
+                    const uint32_t t = sm.tri[s * G + gl];
+                    if (!((known >> s) & 1u) || ctri[s] != t) {
+                        const double vv[3] = {R.px[s], R.py[s], R.pz[s]};
+                        cand[s] = ball_search<D>(bg, pts, vv, R.d2[s], sid, sm.pvid, nb_v);
+                        ctri[s] = t;
+                        known |= 1u << s;
+                    }
+                    if (cand[s] == -2) far = true;
+                    if (cand[s] < 0) continue;
+                    const int32_t q = cand[s];
+                    const double dx = pts[(size_t)D * q] - sx, dy = pts[(size_t)D * q + 1] - sy;
+                    const double dz = D == 3 ? pts[(size_t)D * q + 2] - sz : 0.0;
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    if (bq < 0 || d2 < bd || (d2 == bd && bg.perm[q] < bg.perm[bq])) {
+                        bd = d2;
+                        bq = q;
+                    }
+                }
+            }
+            // the group's candidate nearest the seed
+#pragma unroll
+            for (int d = 1; d < G; d <<= 1) {
+                const double od = shfl_xor(bd, d, G);
+                const int32_t oq = shfl_xor(bq, d, G);
+                if (oq >= 0 && (bq < 0 || od < bd || (od == bd && bg.perm[oq] < bg.perm[bq]))) {
+                    bd = od;
+                    bq = oq;
+                }
+            }
+            far = ((wave_ballot(far) >> shift) & gmask) != 0ull;
+            bool act = false;
+            if (!bdone) {
+                if (far) {
+                    status = SECURITY_RADIUS_NOT_REACHED;  // a ball beyond BALL_RINGS: gives up
+                    bdone = true;
+                } else if (bq < 0) {
+                    kc = bg.kmark;  // every vertex ball is empty
+                    bdone = true;
+                } else if (iter >= 4 * max_p) {
+                    status = SECURITY_RADIUS_NOT_REACHED;
+                    bdone = true;
+                } else {
+                    act = true;
+                }
+            }
+            const uint32_t zb = act ? (uint32_t)bq : 0u;
+            const double bx = act ? pts[(size_t)D * zb] : 0.0, by = act ? pts[(size_t)D * zb + 1] : 0.0;
+            const double bz = (act && D == 3) ? pts[(size_t)D * zb + 2] : 0.0;
+            clip_step(act, zb, bx, by, bz);
+            if (act && status != SUCCESS) bdone = true;
+        }
+        if (valid && status != SUCCESS) kc = 0;
+    }
+
     // final certificate with the k-th neighbour (voronoi.cpp:145-147); a row
     // holding every other seed certifies the cell by exhaustion
     // (a slot overflow keeps its code: the 128-slot kernel redoes the seed
     // at the same row length and applies this test itself)
     const int loop_status = status;
     const uint32_t zl = valid ? row[k - 1] : 0u;
-    if (valid) {
+    if (valid && !BALL) {
         const double dx = pts[(size_t)D * zl] - sx, dy = pts[(size_t)D * zl + 1] - sy;
         const double dz = D == 3 ? pts[(size_t)D * zl + 2] - sz : 0.0;
         const double dl2 = dx * dx + dy * dy + dz * dz;
@@ -1025,7 +1198,20 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
         for (int i = 0; i < C::PPL; i++) {
             const int p = gl + G * i;
             if (!isface[i]) continue;
-            const int fi = popc64(fm & ((1ull << p) - 1ull));
+            int fi = popc64(fm & ((1ull << p) - 1ull));
+            if (BALL && p >= first_ball) {
+                // faces of the completion's planes in (distance, input
+                // index) order, after the row's
+                const uint32_t zp = (uint32_t)sm.pvid[p];
+                const double kp = dist2<D>(pts + (size_t)D * zp, pts + (size_t)D * sid);
+                fi = popc64(first_ball >= 64 ? fm : (fm & ((1ull << first_ball) - 1ull)));
+                for (int q = first_ball; q < nb_v; q++) {
+                    if (q == p || !((fm >> q) & 1ull)) continue;
+                    const uint32_t zq = (uint32_t)sm.pvid[q];
+                    const double kq = dist2<D>(pts + (size_t)D * zq, pts + (size_t)D * sid);
+                    if (kq < kp || (kq == kp && bg.perm[zq] < bg.perm[zp])) fi++;
+                }
+            }
             const int64_t at = (int64_t)fi * st + sid;
             fa.nbr[at] = pv[i];
             if (fa.nbr_cm) fa.nbr_cm[(int64_t)sid * FACE_SLOTS + fi] = pv[i];
