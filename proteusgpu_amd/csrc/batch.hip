@@ -166,7 +166,11 @@ __global__ void __launch_bounds__(TB) k_fill(T* a, int64_t n, T v) {
 PG_DEV int64_t next_ticket(unsigned long long* ticket, int chunk) {
     unsigned long long t = 0;
     if (threadIdx.x == 0) t = atomicAdd(ticket, (unsigned long long)chunk);
-    return (int64_t)__shfl(t, 0, 64);
+    // lane 0's value as a scalar: the compiler then knows the whole wave
+    // works on the same items (scalar loads, no per-lane copies of them)
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)t);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(t >> 32));
+    return (int64_t)((uint64_t)hi << 32 | lo);
 }
 // Items per ticket: up to `most`, fewer for short lists (each block should
 // see several chunks, so that the last ones balance).

@@ -4,6 +4,7 @@
 
 #include "common.h"
 #include "kernels.h"
+#include "simt.h"
 
 namespace pg {
 
@@ -166,14 +167,16 @@ PG_DEV void knn_heap_query(const double* __restrict__ sorted, const uint32_t* __
 // the best 64 M (dist^2, sorted id) are kept sorted in registers,
 // slot-major (rank r at slot r / 64 of lane r % 64, M = ceil(k/64) slots).
 // Ring cells are enumerated 64 at a time, their point ranges prefix-summed
-// across the wave, and candidates processed 64 at a time: those that beat
-// the current k-th entry (broadcast) gather in LDS; every 64 of them are
-// sorted across the wave by a bitonic network and merged into the kept
-// list, one 64-entry slot at a time (lower half of slot + reversed batch
-// stays, the upper half carries into the next slot).
-// Same pruning rule and same total order as k_knn, no LDS heap, full waves:
-// used for rows of 20 and more, where the per-thread heap would leave the
-// CUs nearly empty.
+// across the wave, and candidates processed 64 at a time: those inside the
+// query's ball (a radius guessed from the local density, so that one or
+// two sorts usually make the row) and ahead of the current k-th entry
+// (broadcast) gather in LDS; every 64 of them are sorted across the wave
+// by a bitonic network and merged into the kept list, one 64-entry slot at
+// a time (lower half of slot + reversed batch stays, the upper half carries
+// into the next slot).
+// Same rows and same total order as k_knn, no LDS heap, full waves: used
+// for rows of 20 and more, where the per-thread heap would leave the CUs
+// nearly empty.
 
 PG_DEV bool cand_less(double da, uint32_t oa, double db, uint32_t ob) {
     return da < db || (da == db && oa < ob);
@@ -190,16 +193,36 @@ PG_DEV bool ent_less(const uint32_t* __restrict__ perm, double da, uint32_t sa, 
     return orig_id(perm, sa) < orig_id(perm, sb);
 }
 
-// Compare-exchange with lane l ^ j: keep the smaller (keep_min) or the
-// larger of the pair under ent_less.
-PG_DEV void wave_cx(const uint32_t* __restrict__ perm, double& d, uint32_t& s, int j, bool keep_min) {
-    const double pd = __shfl_xor(d, j, 64);
-    const uint32_t ps = __shfl_xor(s, j, 64);
-    const bool other_less = ent_less(perm, pd, ps, d, s);
-    if (other_less == keep_min) {
-        d = pd;
-        s = ps;
+// Compare-exchange with lane l ^ X: the lane whose bit B is clear keeps the
+// smaller entry of the pair under ent_less, the other the larger.
+template <int X, int B>
+PG_DEV void wave_cx(const uint32_t* __restrict__ perm, double& d, uint32_t& s, int l) {
+    const double pd = lane_xor_d<X>(d);
+    const uint32_t ps = lane_xor<X>(s);
+    const bool take = ent_less(perm, pd, ps, d, s) == ((l & B) == 0);
+    d = take ? pd : d;
+    s = take ? ps : s;
+}
+
+// Half-cleaners J, J/2, .., 1: sorts a bitonic sequence held in blocks of
+// 2 J lanes (ascending with the lane index).
+template <int J>
+PG_DEV void wave_half_clean(const uint32_t* __restrict__ perm, double& d, uint32_t& s, int l) {
+    if constexpr (J >= 1) {
+        wave_cx<J, J>(perm, d, s, l);
+        wave_half_clean<J / 2>(perm, d, s, l);
     }
+}
+
+// Bitonic sort with one direction throughout: blocks of K lanes hold two
+// ascending halves; the mirror exchange (lane l with l ^ (K - 1)) leaves
+// two bitonic halves with the lower below the upper, and the half-cleaners
+// sort each.  K = 2, 4, .., 64.
+template <int K>
+PG_DEV void wave_sort_phase(const uint32_t* __restrict__ perm, double& d, uint32_t& s, int l) {
+    wave_cx<K - 1, K / 2>(perm, d, s, l);
+    wave_half_clean<K / 4>(perm, d, s, l);
+    if constexpr (K < 64) wave_sort_phase<2 * K>(perm, d, s, l);
 }
 
 // t / d for 0 <= t < 2^22, 1 <= d, rd = 1/d rounded: the float quotient is
@@ -215,16 +238,12 @@ PG_DEV int idiv_small(int t, int d, float rd) {
 
 // Bitonic sort of one entry per lane, ascending with the lane index.
 PG_DEV void wave_sort64(const uint32_t* __restrict__ perm, double& d, uint32_t& s, int l) {
-#pragma unroll
-    for (int k = 2; k <= 64; k <<= 1)
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) wave_cx(perm, d, s, j, ((l & j) == 0) == ((l & k) == 0 || k == 64));
+    wave_sort_phase<2>(perm, d, s, l);
 }
 
 // Sort a bitonic sequence held one entry per lane (ascending).
 PG_DEV void wave_merge64(const uint32_t* __restrict__ perm, double& d, uint32_t& s, int l) {
-#pragma unroll
-    for (int j = 32; j > 0; j >>= 1) wave_cx(perm, d, s, j, (l & j) == 0);
+    wave_half_clean<32>(perm, d, s, l);
 }
 
 // Candidates that beat the k-th kept entry wait in LDS until 64 have
@@ -244,8 +263,8 @@ PG_DEV void wave_merge_batch(const uint32_t* __restrict__ perm, double (&kd)[M],
     wave_sort64(perm, dd, cs, l);
 #pragma unroll
     for (int jj = 0; jj < M; jj++) {
-        const double rd = __shfl(dd, 63 - l, 64);
-        const uint32_t rs = __shfl(cs, 63 - l, 64);
+        const double rd = lane_xor_d<63>(dd);  // lane 63 - l
+        const uint32_t rs = lane_xor<63>(cs);
         const bool mine = ent_less(perm, kd[jj], ks[jj], rd, rs);
         double hd = mine ? rd : kd[jj];
         uint32_t hs = mine ? rs : ks[jj];
@@ -274,12 +293,60 @@ PG_DEV void tail_of(const KnnTail& t, uint32_t q, int k, int& kp, uint32_t& lo) 
     }
 }
 
+// Squared radius of the first pass of a wave query: the ball expected to
+// hold lam points at the density of the 3^DIM block of grid cells around
+// the query (cnt points, the query included, in nin in-grid cells of side
+// 1/N), widened where the unit box cuts it (the share of the cube
+// [p - R, p + R] inside the box stands for the ball's).  A heuristic: the
+// rows do not depend on it, only the work to find them.
+template <int DIM>
+PG_DEV double knn_guess_r2(const double (&p)[3], int N, int cnt, int nin, float lam) {
+    // float arithmetic and the hardware's log2 / exp2: a heuristic needs
+    // no more
+    const float h = 1.0f / (float)N;
+    const float vcell = DIM == 3 ? h * h * h : h * h;
+    const float rho = (float)max(cnt - 1, 1) / ((float)max(nin, 1) * vcell);
+    // ball volume lam / rho = (4 pi / 3) R^3 in 3D, pi R^2 in 2D
+    const float v = lam / rho * (DIM == 3 ? 0.2387324146f : 0.3183098862f);
+    float R = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(v) * (1.0f / DIM));
+    float f = 1.0f;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        const float x = (float)p[d];
+        f *= (fminf(x, R) + fminf(1.0f - x, R)) / (2.0f * R);
+    }
+    f = fmaxf(f, 1.0f / 64.0f);
+    R *= __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(f) * (-1.0f / DIM));
+    return (double)R * (double)R;
+}
+
 // One query by the 64 lanes of a one-wave block; c_start / c_pfx are the
 // block's LDS tables (64 and 65 entries), pend its candidate buffer.  Grid
 // cells of the query's box are cbase + i + j N + k N^2.  kp > 0: the first
 // kp neighbours are known, lo_s (sorted id) the last of them; only
 // neighbours kp..k-1 are found (candidates up to lo_s in the order are
 // skipped) and written to o[kp..k).
+//
+// The walk is the reference's (knn.cpp:237-347): rings of cells in
+// Chebyshev distance, stopped before ring r once the k-th distance is below
+// ((r - 1) / N)^2.  A pass also carries a ball of squared radius r2 (from
+// knn_guess_r2): only candidates with d^2 <= r2 are kept, cells whose lower
+// bound exceeds r2 are skipped, and the walk ends before the first ring r
+// with ((r - 1) / N)^2 > r2.  If at least k candidates lie in the ball,
+// their k first in the (d^2, original id) order are the reference's row:
+//  * every point the reference's row could hold within the ball was seen
+//    (pruned cells and rings hold none), and none outside it can rank
+//    before the ball's k-th;
+//  * the reference stops no later than the ring after the ball's last
+//    (its k-th distance there is at most the ball's k-th <= r2 <
+//    ((r - 1) / N)^2), and no earlier ring can stop it with a kept
+//    candidate beyond it, as long as every kept candidate of ring r has
+//    d^2 >= ((r - 1) / N)^2, the bound its ring gives.  A rounded distance
+//    below that bound (a point within rounding of a ring's edge) is checked
+//    for, and sends the query to a pass with no ball: the reference's walk
+//    itself.
+// Fewer than k candidates in the ball: a new pass from scratch with a wider
+// one (after three, none).
 template <int DIM, int M>
 PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __restrict__ perm, int N,
                            int64_t cbase, const int32_t* __restrict__ starts,
@@ -289,6 +356,7 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
     double p[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < DIM; d++) p[d] = sorted[(size_t)DIM * q + d];
+    const int kfull = k;
     double lo_d = -1.0;  // kp = 0: no lower bound
     if (kp > 0) {
         double pc[3] = {0, 0, 0};
@@ -306,14 +374,18 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
 
     double kd[M];
     uint32_t ks[M];
-#pragma unroll
-    for (int j = 0; j < M; j++) {
-        kd[j] = __builtin_inf();
-        ks[j] = 0xffffffffu;
-    }
     int size = 0, npend = 0;
     const int kl = (k - 1) & 63, kj = (k - 1) >> 6;  // lane and slot of rank k-1
-    // merge the first n (<= 64) waiting candidates into the kept list
+    auto kth_entry = [&](double& d, uint32_t& s) {
+#pragma unroll
+        for (int j = 0; j < M; j++)
+            if (j == kj) {
+                d = read_lane_d(kd[j], kl);
+                s = read_lane(ks[j], kl);
+            }
+    };
+    // merge the first n (<= 64) waiting candidates into the kept list; into
+    // an empty list the sorted batch is the list
     auto flush = [&](int n) {
         __syncthreads();
         double dd = __builtin_inf();
@@ -322,137 +394,184 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
             dd = pend->d[l];
             cs = pend->s[l];
         }
-        wave_merge_batch<M>(perm, kd, ks, dd, cs, l);
+        if (size == 0) {
+            wave_sort64(perm, dd, cs, l);
+            kd[0] = dd;
+            ks[0] = cs;
+        } else {
+            wave_merge_batch<M>(perm, kd, ks, dd, cs, l);
+        }
         size = min(k, size + n);
     };
 
-    for (int ring = 0; ring < N; ring++) {
-        // the ring test below (and the cell pruning of the ring) need the
-        // exact k-th entry; before ring 1 the test cannot pass (its bound is
-        // 0), and while fewer than k candidates are known there is no k-th
-        // entry: the waiting candidates stay in LDS
-        if (npend > 0 && ring >= 2 && size + npend >= k) {
-            flush(npend);
-            npend = 0;
+    // points the first ball should hold: k and a margin for the spread of
+    // the count (about 1% of passes come out short on uniform seeds)
+    const float lam = 1.25f * (float)kfull + 3.0f * __builtin_sqrtf((float)kfull) + 6.0f;
+    double r2;  // +inf: no ball (the reference's walk alone)
+    {
+        // the first ball from the density of the 3^DIM block of cells
+        // around the query
+        int cnt = 0;
+        bool in = false;
+        if (l < (DIM == 3 ? 27 : 9)) {
+            const int di = l % 3 - 1, dj = (l / 3) % 3 - 1, dk = DIM == 3 ? l / 9 - 1 : 0;
+            const int ii = ci + di, jj = cj + dj, kk = ck + dk;
+            in = ii >= 0 && jj >= 0 && kk >= 0 && ii < N && jj < N && (DIM == 2 || kk < N);
+            if (in) cnt = counts[cbase + ii + jj * N + (int64_t)kk * N * N];
         }
-        if (ring >= 1 && size == k) {
-            double kth = 0;
+        const int tot = (int)read_lane((uint32_t)wave_incl_sum(cnt), 63);
+        r2 = knn_guess_r2<DIM>(p, N, tot, __popcll((unsigned long long)__ballot(in)), lam);
+    }
+    for (int pass = 0;; pass++) {
 #pragma unroll
-            for (int j = 0; j < M; j++)
-                if (j == kj) kth = __shfl(kd[j], kl, 64);
+        for (int j = 0; j < M; j++) {
+            kd[j] = __builtin_inf();
+            ks[j] = 0xffffffffu;
+        }
+        size = 0;
+        npend = 0;
+        bool stop = false, odd_l = false;
+        // the first step takes the cells at Chebyshev distance 0 and 1
+        // together (the reference's ring-1 test cannot stop the walk: its
+        // bound is 0), then ring by ring
+        for (int ring = 0; ring < N; ring = ring == 0 ? 2 : ring + 1) {
             const double md = (double)(ring - 1) / (double)N;
-            if (kth < md * md) break;
-        }
-        const int side = 2 * ring + 1;
-        const int ncube = DIM == 3 ? side * side * side : side * side;
-        const float rside = 1.0f / (float)side;
-        for (int base = 0; base < ncube; base += 64) {
-            // this lane's cell of the cube, if it lies on the ring and in the grid
-            const int t = base + l;
-            int cnt = 0, st = 0;
-            // cells wholly beyond the current k-th distance are skipped
-            double prune = 1e300;
-            if (size == k) {
-#pragma unroll
-                for (int j = 0; j < M; j++)
-                    if (j == kj) prune = __shfl(kd[j], kl, 64);
-            }
-            if (t < ncube) {
-                // t = (dk + ring) side^2 + (dj + ring) side + (di + ring)
-                int q1, q2;
-                if (ncube < (1 << 22)) {  // float quotient, one correction step
-                    q1 = idiv_small(t, side, rside);
-                    q2 = idiv_small(q1, side, rside);
-                } else {
-                    q1 = t / side;
-                    q2 = q1 / side;
-                }
-                const int di = t - q1 * side - ring, dj = q1 - q2 * side - ring;
-                const int dk = DIM == 3 ? q2 - ring : 0;
-                const int cheb = max(abs(di), max(abs(dj), abs(dk)));
-                const int ii = ci + di, jj = cj + dj, kk = ck + dk;
-                if (cheb == ring && ii >= 0 && jj >= 0 && kk >= 0 && ii < N && jj < N && (DIM == 2 || kk < N) &&
-                    cell_lower_d2<DIM>(P, ii, jj, kk, N) <= prune) {
-                    const int64_t cc = cbase + ii + jj * N + (int64_t)kk * N * N;
-                    st = starts[cc];
-                    cnt = counts[cc];
-                }
-            }
-            int incl = cnt;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                int v = __shfl_up(incl, d, 64);
-                if (l >= d) incl += v;
-            }
-            const int total = __shfl(incl, 63, 64);
-            if (total == 0) continue;
-            c_start[l] = st;
-            c_pfx[l + 1] = incl;
-            if (l == 0) c_pfx[0] = 0;
-            __syncthreads();
-            for (int cb = 0; cb < total; cb += 64) {
-                const int j = cb + l;
-                bool take = false;
-                double dd = 0;
-                uint32_t cs = 0;
-                if (j < total) {
-                    // cell holding candidate j: largest a with c_pfx[a] <= j
-                    int lo = 0, hi = 63;
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (c_pfx[mid] <= j) lo = mid;
-                        else hi = mid - 1;
-                    }
-                    cs = (uint32_t)(c_start[lo] + (j - c_pfx[lo]));
-                    if (cs != q) {
-                        double pc[3] = {0, 0, 0};
-#pragma unroll
-                        for (int d = 0; d < DIM; d++) pc[d] = sorted[(size_t)DIM * cs + d];
-                        dd = dist2<DIM>(p, pc);
-                        take = kp == 0 || ent_less(perm, lo_d, lo_s, dd, cs);
-                    }
+            const double md2 = ring >= 2 ? md * md : 0.0;  // ring's distance bound
+            if (ring >= 2) {
+                // the ring test (and the cell pruning) need the exact k-th
+                // entry; while fewer than k candidates are known there is
+                // none: the waiting candidates stay in LDS
+                if (npend > 0 && size + npend >= k) {
+                    flush(npend);
+                    npend = 0;
                 }
                 if (size == k) {
-                    double kth = 0;
-                    uint32_t kts = 0;
-#pragma unroll
-                    for (int jj = 0; jj < M; jj++)
-                        if (jj == kj) {
-                            kth = __shfl(kd[jj], kl, 64);
-                            kts = __shfl(ks[jj], kl, 64);
-                        }
-                    take = take && ent_less(perm, dd, cs, kth, kts);
-                }
-                const uint64_t surv = (uint64_t)__ballot(take);
-                if (surv == 0ull) continue;
-                // survivors join the LDS buffer in lane order (npend < 64,
-                // at most 64 join: positions stay below 128)
-                if (take) {
-                    const int at = npend + __popcll((unsigned long long)(surv & ((1ull << l) - 1ull)));
-                    pend->d[at] = dd;
-                    pend->s[at] = cs;
-                }
-                npend += __popcll((unsigned long long)surv);
-                if (npend >= 64) {
-                    flush(64);
-                    // move the rest (fewer than 64) to the front
-                    double rd = 0;
-                    uint32_t rs = 0;
-                    const int rest = npend - 64;
-                    if (l < rest) {
-                        rd = pend->d[64 + l];
-                        rs = pend->s[64 + l];
+                    double kth;
+                    uint32_t kts;
+                    kth_entry(kth, kts);
+                    if (kth < md2) {
+                        stop = true;
+                        break;
                     }
-                    __syncthreads();
-                    if (l < rest) {
-                        pend->d[l] = rd;
-                        pend->s[l] = rs;
-                    }
-                    npend = rest;
                 }
+                if (md2 > r2) break;  // the ring lies beyond the ball
             }
-            __syncthreads();
+            const int reach = ring == 0 ? 1 : ring;  // cube half-side
+            const int side = 2 * reach + 1;
+            const int ncube = DIM == 3 ? side * side * side : side * side;
+            const float rside = 1.0f / (float)side;
+            for (int base = 0; base < ncube; base += 64) {
+                // this lane's cell of the cube, if it lies on the ring and in the grid
+                const int t = base + l;
+                int cnt = 0, st = 0;
+                // cells wholly beyond the ball or the current k-th distance are skipped
+                double prune = r2;
+                if (size == k) {
+                    double kth;
+                    uint32_t kts;
+                    kth_entry(kth, kts);
+                    prune = fmin(prune, kth);
+                }
+                if (t < ncube) {
+                    // t = (dk + reach) side^2 + (dj + reach) side + (di + reach)
+                    int q1, q2;
+                    if (ncube < (1 << 22)) {  // float quotient, one correction step
+                        q1 = idiv_small(t, side, rside);
+                        q2 = idiv_small(q1, side, rside);
+                    } else {
+                        q1 = t / side;
+                        q2 = q1 / side;
+                    }
+                    const int di = t - q1 * side - reach, dj = q1 - q2 * side - reach;
+                    const int dk = DIM == 3 ? q2 - reach : 0;
+                    const int cheb = max(abs(di), max(abs(dj), abs(dk)));
+                    const int ii = ci + di, jj = cj + dj, kk = ck + dk;
+                    const bool cell_in = (cheb == ring || ring == 0) && ii >= 0 && jj >= 0 && kk >= 0 && ii < N &&
+                                         jj < N && (DIM == 2 || kk < N);
+                    if (cell_in && cell_lower_d2<DIM>(P, ii, jj, kk, N) <= prune) {
+                        const int64_t cc = cbase + ii + jj * N + (int64_t)kk * N * N;
+                        st = starts[cc];
+                        cnt = counts[cc];
+                    }
+                }
+                const int incl = wave_incl_sum(cnt);
+                const int total = (int)read_lane((uint32_t)incl, 63);
+                if (total == 0) continue;
+                c_start[l] = st;
+                c_pfx[l + 1] = incl;
+                if (l == 0) c_pfx[0] = 0;
+                __syncthreads();
+                for (int cb = 0; cb < total; cb += 64) {
+                    const int j = cb + l;
+                    bool take = false;
+                    double dd = 0;
+                    uint32_t cs = 0;
+                    if (j < total) {
+                        // cell holding candidate j: largest a with c_pfx[a] <= j
+                        int lo = 0, hi = 63;
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (c_pfx[mid] <= j) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        cs = (uint32_t)(c_start[lo] + (j - c_pfx[lo]));
+                        if (cs != q) {
+                            double pc[3] = {0, 0, 0};
+#pragma unroll
+                            for (int d = 0; d < DIM; d++) pc[d] = sorted[(size_t)DIM * cs + d];
+                            dd = dist2<DIM>(p, pc);
+                            take = dd <= r2 && (kp == 0 || ent_less(perm, lo_d, lo_s, dd, cs));
+                        }
+                    }
+                    if (size == k) {
+                        double kth;
+                        uint32_t kts;
+                        kth_entry(kth, kts);
+                        take = take && ent_less(perm, dd, cs, kth, kts);
+                    }
+                    // a kept candidate nearer than its ring's bound (rounding
+                    // at a ring's edge): the ball's argument fails
+                    odd_l = odd_l || (take && dd < md2);
+                    const uint64_t surv = (uint64_t)__ballot(take);
+                    if (surv == 0ull) continue;
+                    // survivors join the LDS buffer in lane order (npend < 64,
+                    // at most 64 join: positions stay below 128)
+                    if (take) {
+                        const int at = npend + __popcll((unsigned long long)(surv & ((1ull << l) - 1ull)));
+                        pend->d[at] = dd;
+                        pend->s[at] = cs;
+                    }
+                    npend += __popcll((unsigned long long)surv);
+                    if (npend >= 64) {
+                        flush(64);
+                        // move the rest (fewer than 64) to the front
+                        double rd = 0;
+                        uint32_t rs = 0;
+                        const int rest = npend - 64;
+                        if (l < rest) {
+                            rd = pend->d[64 + l];
+                            rs = pend->s[64 + l];
+                        }
+                        __syncthreads();
+                        if (l < rest) {
+                            pend->d[l] = rd;
+                            pend->s[l] = rs;
+                        }
+                        npend = rest;
+                    }
+                }
+                __syncthreads();
+            }
         }
+        if (r2 < __builtin_inf() && __ballot(odd_l) != 0ull) {  // walk again, no ball
+            r2 = __builtin_inf();
+            continue;
+        }
+        if (stop || size + npend >= k || !(r2 < __builtin_inf())) break;
+        // fewer than k candidates in the ball: a wider one, from scratch
+        const float have = (float)(kp + size + npend) + 1.0f;
+        const float g = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(lam / have) * (2.0f / DIM));
+        r2 = pass >= 2 ? __builtin_inf() : r2 * (double)fminf(fmaxf(g, 1.5f), 16.0f);
     }
     if (npend > 0) flush(npend);
 #pragma unroll

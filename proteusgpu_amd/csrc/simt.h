@@ -24,6 +24,55 @@ __device__ __forceinline__ T shfl_up(T v, int d, int width) {
     return __shfl_up(v, d, width);
 }
 
+// Lane l's value of v at lane l ^ M, every lane of the wave active.  Inside
+// rows of 16 lanes the exchange is a DPP move (no LDS traffic): quad
+// permutes for M = 1, 2, 3, the half-row mirror for 7, the row mirror for
+// 15, a row rotation for 8, and 4 = 7 then 3.  M = 16 and 31 stay inside
+// 32 lanes (ds_swizzle, bit-mask mode); 32 and 63 go through ds_bpermute.
+template <int M>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v) {
+    const int x = (int)v;
+    if constexpr (M == 1) return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);
+    else if constexpr (M == 2) return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);
+    else if constexpr (M == 3) return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0x1B, 0xF, 0xF, false);
+    else if constexpr (M == 4) return lane_xor<3>(lane_xor<7>(v));
+    else if constexpr (M == 7) return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0x141, 0xF, 0xF, false);
+    else if constexpr (M == 8) return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);
+    else if constexpr (M == 15) return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0x140, 0xF, 0xF, false);
+    else if constexpr (M == 16) return (uint32_t)__builtin_amdgcn_ds_swizzle(x, 0x401F);
+    else if constexpr (M == 31) return (uint32_t)__builtin_amdgcn_ds_swizzle(x, 0x7C1F);
+    else {
+        static_assert(M == 32 || M == 63, "lane_xor: unsupported lane mask");
+        return (uint32_t)__builtin_amdgcn_ds_bpermute((lane_id() ^ M) << 2, x);
+    }
+}
+template <int M>
+__device__ __forceinline__ double lane_xor_d(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint64_t lo = lane_xor<M>((uint32_t)u), hi = lane_xor<M>((uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, lo | hi << 32);
+}
+// v at a lane index the whole wave agrees on (a scalar result)
+__device__ __forceinline__ uint32_t read_lane(uint32_t v, int lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+__device__ __forceinline__ double read_lane_d(double v, int lane) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint64_t lo = read_lane((uint32_t)u, lane), hi = read_lane((uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, lo | hi << 32);
+}
+// Inclusive prefix sum over the 64 lanes, every lane active: row shifts
+// inside rows of 16, then the row broadcasts of lanes 15 and 31.
+__device__ __forceinline__ int wave_incl_sum(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);  // row_bcast:15 into rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);  // row_bcast:31 into rows 2, 3
+    return x;
+}
+
 __device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
 // index of the lowest set bit, x != 0
 __device__ __forceinline__ int ffs64(uint64_t x) { return __ffsll((unsigned long long)x) - 1; }

@@ -13,7 +13,7 @@ import pytest
 
 from oracle import oracle as O
 from proteusgpu_amd import api
-from proteusgpu_amd.generators import sod_state, uniform_points
+from proteusgpu_amd.generators import gaussian_mixture, sod_state, uniform_points
 
 pytestmark = pytest.mark.gpu
 
@@ -34,6 +34,21 @@ def test_knn_bitexact(ctx, dim, n, k, seed):
     assert kp.N_grid == N
     assert np.array_equal(kp.get_points(), s)
     assert np.array_equal(kp.get_permutation(), perm)
+    assert np.array_equal(kp.get_knearest(), kn)
+
+
+@pytest.mark.parametrize("dim,n,k", [(3, 12000, 24), (3, 12000, 64), (3, 12000, 100), (2, 6000, 200),
+                                     (3, 13000, 256), (2, 1000, 250)])
+def test_knn_clustered_bitexact(ctx, dim, n, k):
+    """Clustered seeds (dense clumps, sparse space between them): the wave
+    query's first ball, guessed from the density around the query, is often
+    far too small or far too large there.  Rows bit-exact against the
+    oracle's ring search."""
+    print(f"knn clustered dim={dim} n={n} k={k}", flush=True)
+    pts = np.ascontiguousarray(gaussian_mixture(n, 8, 0.003, 0.03, 0.3, 40 + k)[:, :dim])
+    kp = api.KnnProblem(ctx, pts)
+    kp.solve(k)
+    _, _, _, kn = O.knn(pts, k)
     assert np.array_equal(kp.get_knearest(), kn)
 
 
