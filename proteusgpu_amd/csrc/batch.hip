@@ -179,7 +179,6 @@ PG_DEV int ticket_chunk(int64_t items, int most) {
     return (int)(c < 1 ? 1 : (c > most ? most : c));
 }
 constexpr int KNN_CHUNK = 8;   // queries per ticket, at most
-constexpr int CLIP_CHUNK = 2;  // blocks of seeds per ticket, at most
 
 // Resident one-wave blocks of a kernel on the device (its occupancy per CU,
 // capped at blocks_per_cu, times the CUs).
@@ -220,55 +219,81 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
+// The persistent KNN kernel below takes one struct of arguments and, for every
+// item, read it again from the kernel-argument segment through a pointer
+// the compiler must treat as new each time: loop-invariant arguments would
+// otherwise be hoisted out of the item loop and stay live across it, and
+// the item's own registers would spill.  The struct is the kernel's only
+// explicit argument, so it starts the segment.
+template <class A>
+PG_DEV const __attribute__((address_space(4))) A* fresh_args() {
+    const __attribute__((address_space(4))) A* p =
+        (const __attribute__((address_space(4))) A*)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
+struct BKnnArgs {
+    const double* sorted;
+    const uint32_t* perm;
+    const int32_t* sbox;
+    const int32_t* N;
+    const int64_t* coff;
+    const int32_t* starts;
+    const int32_t* counts;
+    const uint32_t* queries;
+    const int64_t* nq_dev;
+    uint32_t* out;
+    unsigned long long* ticket;
+    KnnTail tail;
+    int k;
+};
+
 template <int DIM, int M>
-__global__ void __launch_bounds__(64, (M <= 2 ? 6 : 5))
-    k_bknn_wave(const double* __restrict__ sorted, const uint32_t* __restrict__ perm,
-                const int32_t* __restrict__ sbox, const int32_t* __restrict__ N, const int64_t* __restrict__ coff,
-                const int32_t* __restrict__ starts, const int32_t* __restrict__ counts,
-                const uint32_t* __restrict__ queries, const int64_t* __restrict__ nq_dev, int k,
-                uint32_t* __restrict__ out, unsigned long long* ticket, KnnTail tail) {
+__global__ void __launch_bounds__(64, (M <= 2 ? 6 : 5)) k_bknn_wave(BKnnArgs args) {
     __shared__ int32_t c_start[64], c_pfx[65];
     __shared__ KnnPend pend;
-    const int64_t nq = *nq_dev;
+    const int64_t nq = *args.nq_dev;
     const int chunk = ticket_chunk(nq, KNN_CHUNK);
     for (;;) {
-        const int64_t q0 = next_ticket(ticket, chunk);
+        const int64_t q0 = next_ticket(fresh_args<BKnnArgs>()->ticket, chunk);
         if (q0 >= nq) break;
         const int64_t q1 = q0 + chunk < nq ? q0 + chunk : nq;
         for (int64_t qi = q0; qi < q1; qi++) {
-            const uint32_t q = queries ? queries[qi] : (uint32_t)qi;
-            const int b = sbox[q];
+            const auto* A = fresh_args<BKnnArgs>();
+            const uint32_t q = A->queries ? A->queries[qi] : (uint32_t)qi;
+            const int b = A->sbox[q];
+            const int k = A->k;
+            KnnTail tail;
+            tail.resume = A->tail.resume;
+            tail.lastnb = A->tail.lastnb;
+            tail.tag = A->tail.tag;
             int kp = 0;
             uint32_t lo = 0;
             tail_of(tail, q, k, kp, lo);
-            knn_wave_query<DIM, M>(sorted, perm, N[b], coff[b], starts, counts, q, k, out + (size_t)qi * k,
-                                   c_start, c_pfx, &pend, kp, lo);
+            knn_wave_query<DIM, M>(A->sorted, A->perm, A->N[b], A->coff[b], A->starts, A->counts, q, k,
+                                   A->out + (size_t)qi * k, c_start, c_pfx, &pend, kp, lo);
             __syncthreads();
         }
     }
 }
 
 // ------------------------------------------------------------------ clip
+// One block per group of SPW listed seeds, over the list's capacity: blocks
+// past the device-side count end at once.  (In a loop over groups, even
+// with fresh arguments, the compiler keeps values across the groups that
+// make the clip's registers spill.)
 template <int DIM, int G, int S>
 __global__ void __launch_bounds__(64, (VoroCfg<DIM, G, S>::WAVES))
     k_bvoronoi(const double* __restrict__ pts, const uint32_t* __restrict__ seeds, const int64_t* __restrict__ nq_dev,
-               const uint32_t* __restrict__ knn, int k, int max_p, int max_t, FaceArrays fa,
-               unsigned long long* ticket) {
+               const uint32_t* __restrict__ knn, int k, int max_p, int max_t, FaceArrays fa) {
     using C = VoroCfg<DIM, G, S>;
     __shared__ SeedSmem<C::NS, C::MAXP> sm[C::SPW];
     const int64_t nq = *nq_dev;
-    const int64_t nvb = (nq + C::SPW - 1) / C::SPW;
-    const int chunk = ticket_chunk(nvb, CLIP_CHUNK);
-    for (;;) {
-        const int64_t b0 = next_ticket(ticket, chunk);
-        if (b0 >= nvb) break;
-        const int64_t b1 = b0 + chunk < nvb ? b0 + chunk : nvb;
-        for (int64_t vb = b0; vb < b1; vb++) {
-            // no box holds k + 1 seeds or fewer: the exhaustive-row rule never applies
-            voronoi_group<DIM, G, S>(pts, INT64_MAX, seeds, nq, knn, k, max_p, max_t, fa, sm, vb);
-            __syncthreads();
-        }
-    }
+    const int64_t vb = blockIdx.x;
+    if (vb * C::SPW >= nq) return;  // whole block
+    // no box holds k + 1 seeds or fewer: the exhaustive-row rule never applies
+    voronoi_group<DIM, G, S>(pts, INT64_MAX, seeds, nq, knn, k, max_p, max_t, fa, sm, vb);
 }
 
 // status == code (code < 0: != SUCCESS) among the listed seeds
@@ -493,8 +518,8 @@ void bknn_wave(const double* sorted, const uint32_t* perm, const int32_t* sbox, 
     static const int occ = occupancy(k_bknn_wave<DIM, M>);
     const unsigned nb = (unsigned)std::min<int64_t>(cap, resident_blocks(occ, 8 * blocks_per_cu));
     k_fill<unsigned long long><<<1, TB, 0, s>>>(ticket, 1, 0ull);
-    k_bknn_wave<DIM, M><<<std::max(nb, 1u), 64, 0, s>>>(sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, k,
-                                                        out, ticket, tail);
+    const BKnnArgs args{sorted, perm, sbox, N, coff, starts, counts, q, nq_dev, out, ticket, tail, k};
+    k_bknn_wave<DIM, M><<<std::max(nb, 1u), 64, 0, s>>>(args);
 }
 template <int DIM>
 void bknn_dim(const double* sorted, const uint32_t* perm, const int32_t* sbox, const int32_t* N,
@@ -516,11 +541,10 @@ template <int DIM, int G, int S>
 void bclip(const double* pts, const uint32_t* seeds, const int64_t* nq_dev, int64_t cap, const uint32_t* knn, int k,
            int max_p, int max_t, FaceArrays fa, int blocks_per_cu, unsigned long long* ticket, hipStream_t s) {
     using C = VoroCfg<DIM, G, S>;
-    static const int occ = occupancy(k_bvoronoi<DIM, G, S>);
-    const unsigned nb =
-        (unsigned)std::min<int64_t>((cap + C::SPW - 1) / C::SPW, resident_blocks(occ, blocks_per_cu));
-    k_fill<unsigned long long><<<1, TB, 0, s>>>(ticket, 1, 0ull);
-    k_bvoronoi<DIM, G, S><<<std::max(nb, 1u), 64, 0, s>>>(pts, seeds, nq_dev, knn, k, max_p, max_t, fa, ticket);
+    (void)blocks_per_cu;
+    (void)ticket;
+    const unsigned nb = (unsigned)std::max<int64_t>((cap + C::SPW - 1) / C::SPW, 1);
+    k_bvoronoi<DIM, G, S><<<nb, 64, 0, s>>>(pts, seeds, nq_dev, knn, k, max_p, max_t, fa);
 }
 }  // namespace
 

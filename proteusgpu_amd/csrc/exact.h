@@ -190,8 +190,8 @@ PG_DEV int conflict_sos(const double* s, uint32_t skey, const XRow* rows) {
     int emin = 1 << 30, top = -(1 << 30);
     bool ok = true;
     auto scan = [&](double v) {
-        int64_t m;
-        int e;
+        int64_t m = 0;
+        int e = 0;
         ok = ok && split_double(v, m, e);
         if (m != 0) {
             emin = min(emin, e);

@@ -73,6 +73,50 @@ __device__ __forceinline__ int wave_incl_sum(int x) {
     return x;
 }
 
+// Butterfly over groups of G <= 16 lanes (DPP), every lane active: x is
+// combined with lane l ^ 1, then l ^ 2, .. l ^ G/2, as op(theirs, mine).
+template <int G, class F>
+__device__ __forceinline__ double group_butterfly_d(double x, F op) {
+    static_assert(G == 1 || G == 2 || G == 4 || G == 8 || G == 16, "group of 1..16 lanes");
+    if constexpr (G >= 2) x = op(lane_xor_d<1>(x), x);
+    if constexpr (G >= 4) x = op(lane_xor_d<2>(x), x);
+    if constexpr (G >= 8) x = op(lane_xor_d<4>(x), x);
+    if constexpr (G >= 16) x = op(lane_xor_d<8>(x), x);
+    return x;
+}
+// Inclusive prefix sum inside groups of G <= 16 lanes (gl = lane index in
+// its group), every lane active: row shifts by DPP, kept only where the
+// source lane lies in the same group.
+template <int G>
+__device__ __forceinline__ int group_incl_sum(int x, int gl) {
+    static_assert(G == 1 || G == 2 || G == 4 || G == 8 || G == 16, "group of 1..16 lanes");
+    if constexpr (G >= 2) {
+        const int t = __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);  // row_shr:1
+        if (gl >= 1) x += t;
+    }
+    if constexpr (G >= 4) {
+        const int t = __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);  // row_shr:2
+        if (gl >= 2) x += t;
+    }
+    if constexpr (G >= 8) {
+        const int t = __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);  // row_shr:4
+        if (gl >= 4) x += t;
+    }
+    if constexpr (G >= 16) {
+        const int t = __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);  // row_shr:8
+        if (gl >= 8) x += t;
+    }
+    return x;
+}
+// The value of the last lane of each group of G <= 16 lanes (ds_swizzle,
+// bit-mask mode: source lane (l & ~(G - 1)) | (G - 1) inside 32 lanes).
+template <int G>
+__device__ __forceinline__ int group_last(int x) {
+    static_assert(G == 1 || G == 2 || G == 4 || G == 8 || G == 16, "group of 1..16 lanes");
+    constexpr int and_mask = 0x1F & ~(G - 1), or_mask = G - 1;
+    return __builtin_amdgcn_ds_swizzle(x, and_mask | (or_mask << 5));
+}
+
 __device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
 // index of the lowest set bit, x != 0
 __device__ __forceinline__ int ffs64(uint64_t x) { return __ffsll((unsigned long long)x) - 1; }

@@ -523,8 +523,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
 #pragma unroll
     for (int s = 0; s < S; s++)
         if ((R.live >> s) & 1u) R2 = smax(R.d2[s], R2);
-#pragma unroll
-    for (int d = 1; d < G; d <<= 1) R2 = smax(shfl_xor(R2, d, G), R2);
+    R2 = group_butterfly_d<G>(R2, [](double a, double b) { return smax(a, b); });
 
     // One clip of the cell by the bisector of neighbour z at (bx, by, bz)
     // (act: this group clips in this step; every lane of the wave calls it):
@@ -651,13 +650,8 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
                     }
                 }
                 const int mycnt = __builtin_popcount(bm);
-                int incl = mycnt;
-    #pragma unroll
-                for (int d = 1; d < G; d <<= 1) {
-                    int t = shfl_up(incl, d, G);
-                    if (gl >= d) incl += t;
-                }
-                const int Bt = shfl(incl, G - 1, G);
+                const int incl = group_incl_sum<G>(mycnt, gl);
+                const int Bt = group_last<G>(incl);
                 int j = incl - mycnt;
                 if (doclip) {
                     // in (slot, edge) order, as counted
@@ -879,8 +873,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
 #pragma unroll
             for (int s = 0; s < S; s++)
                 if ((R.live >> s) & 1u) mx = smax(R.d2[s], mx);
-#pragma unroll
-            for (int d = 1; d < G; d <<= 1) mx = smax(shfl_xor(mx, d, G), mx);
+            mx = group_butterfly_d<G>(mx, [](double a, double b) { return smax(a, b); });
         }
         if (act) {
             R2 = mx;
@@ -1173,15 +1166,14 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
     // cell volume and centroid moments: a pairwise sum over the plane index,
     // lowest bit first (lanes hold bits below log2 G, the planes of a lane
     // the bits above), i.e. the same tree for every kernel shape
+    const auto add = [](double theirs, double mine) { return mine + theirs; };
 #pragma unroll
-    for (int i = 0; i < C::PPL; i++)
-#pragma unroll
-        for (int d = 1; d < G; d <<= 1) {
-            pvol[i] += shfl_xor(pvol[i], d, G);
-            pcx[i] += shfl_xor(pcx[i], d, G);
-            pcy[i] += shfl_xor(pcy[i], d, G);
-            pcz[i] += shfl_xor(pcz[i], d, G);
-        }
+    for (int i = 0; i < C::PPL; i++) {
+        pvol[i] = group_butterfly_d<G>(pvol[i], add);
+        pcx[i] = group_butterfly_d<G>(pcx[i], add);
+        pcy[i] = group_butterfly_d<G>(pcy[i], add);
+        pcz[i] = group_butterfly_d<G>(pcz[i], add);
+    }
 #pragma unroll
     for (int w = 1; w < C::PPL; w <<= 1)
 #pragma unroll
