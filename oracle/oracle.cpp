@@ -1051,8 +1051,29 @@ struct GridView {
 // grid rings a ball search may visit (beyond: the completion gives up)
 const int BALL_RINGS = 32;
 
-// nearest point strictly inside the ball (v, r2); -1: none; -2: the search
-// would need more than BALL_RINGS rings
+// Lower bound of the squared distance from v to any point of grid cell ix:
+// boundary cells are open outwards (points beyond the unit box are clamped
+// into them); a slack of 1e-9 cell widths per axis and a factor 1 - 1e-12
+// keep it below the rounded distance of every point of the cell, so
+// skipping a cell whose bound is at least the best distance so far never
+// skips a point that would be picked.
+double ball_cell_lb2(int dim, int N, const double* vv, const int* ix) {
+    const double h = 1.0 / N;
+    double s = 0.0;
+    for (int d = 0; d < dim; d++) {
+        const double lo = ix[d] == 0 ? -1e300 : ix[d] * h;
+        const double hi = ix[d] == N - 1 ? 1e300 : (ix[d] + 1) * h;
+        const double e = std::max(std::max(lo - vv[d], vv[d] - hi) - 1e-9 * h, 0.0);
+        s += e * e;
+    }
+    return s * (1.0 - 1e-12);
+}
+
+// nearest point strictly inside the ball (v, r2), (distance, input index)
+// order; -1: none; -2: the search would need more than BALL_RINGS rings.
+// With the bounds above, the point found is the argmin over all points, and
+// the ring at which the walk stops depends only on the points of the rings
+// before it: any visiting order within a ring gives the same answer.
 int64_t ball_point(const GridView& g, const Cell& c, const P4& v, double r2) {
     const int N = g.N, dim = g.dim;
     const double vv[3] = {v.x, v.y, v.z};
@@ -1062,9 +1083,10 @@ int64_t ball_point(const GridView& g, const Cell& c, const P4& v, double r2) {
     int64_t pick = -1;
     const double h = 1.0 / N;
     for (int ring = 0;; ring++) {
-        if (ring > 0) {
-            const double lb = (ring - 1) * h;
-            if (lb > 0 && lb * lb >= best) break;
+        if (ring > 1) {
+            // a point of ring r lies more than r - 1 cell widths away
+            const double lb = ((ring - 1) - 1e-9) * h;
+            if (lb * lb * (1.0 - 1e-12) >= best) break;
         }
         if (ring > N) break;
         if (ring > BALL_RINGS) return -2;
@@ -1075,15 +1097,8 @@ int64_t ball_point(const GridView& g, const Cell& c, const P4& v, double r2) {
                     if (std::max(std::abs(di), std::max(std::abs(dj), std::abs(dk))) != ring) continue;
                     const int ii = cc[0] + di, jj = cc[1] + dj, kk = cc[2] + dk;
                     if (ii < 0 || jj < 0 || kk < 0 || ii >= N || jj >= N || (dim == 3 && kk >= N)) continue;
-                    // lower bound of the distance from v to the grid cell
                     const int ix[3] = {ii, jj, kk};
-                    double lb2 = 0;
-                    for (int d = 0; d < dim; d++) {
-                        const double lo = ix[d] * h, hi = (ix[d] + 1) * h;
-                        const double e = vv[d] < lo ? lo - vv[d] : (vv[d] > hi ? vv[d] - hi : 0.0);
-                        lb2 += e * e;
-                    }
-                    if (lb2 >= best) continue;
+                    if (ball_cell_lb2(dim, N, vv, ix) >= best) continue;
                     const int64_t cell = ii + (int64_t)jj * N + (int64_t)kk * N * N;
                     for (int t = 0; t < g.cnt[cell]; t++) {
                         const int64_t q = g.start[cell] + t;

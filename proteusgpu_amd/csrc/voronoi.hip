@@ -53,6 +53,8 @@ __global__ void __launch_bounds__(64, 1) k_voronoi_ball(const double* __restrict
                                                         int max_t, FaceArrays fa, BallGrid bg) {
     using C = VoroCfg<DIM, 16, 8>;
     __shared__ SeedSmem<C::NS, C::MAXP> sm[C::SPW];
+    __shared__ int32_t scr[C::SPW * (2 * 16 + 1)];
+    bg.scr = scr;
     voronoi_group<DIM, 16, 8, false, true>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm, blockIdx.x, bg);
 }
 

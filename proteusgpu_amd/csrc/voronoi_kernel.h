@@ -286,13 +286,43 @@ struct BallGrid {
     const int32_t* counts = nullptr;  // points per grid cell
     const uint32_t* perm = nullptr;   // input index per sorted id (ties)
     int kmark = 0;                    // kcert of a cell the completion certifies
+    int32_t* scr = nullptr;           // LDS, 2 G + 1 ints per seed group (the group search's tables)
 };
 
-// The point nearest v strictly inside the ball (v, r2) that is neither the
-// seed nor a plane of the cell; -1: none; -2: beyond BALL_RINGS.
+// Lower bound of the squared distance from v to any point of grid cell ix:
+// boundary cells are open outwards (points beyond the unit box are clamped
+// into them); a slack of 1e-9 cell widths per axis and a factor 1 - 1e-12
+// keep it below the rounded distance of every point of the cell, so a cell
+// whose bound is at least the best distance so far holds no point that
+// would be picked.
 template <int DIM>
-PG_DEV int32_t ball_search(const BallGrid& bg, const double* __restrict__ pts, const double (&v)[3], double r2,
-                           uint32_t sid, const int32_t* pvid, int nb_v) {
+PG_DEV double ball_cell_lb2(int N, const double (&v)[3], const int (&ix)[3]) {
+    const double h = 1.0 / N;
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        const double lo = ix[d] == 0 ? -1e300 : ix[d] * h;
+        const double hi = ix[d] == N - 1 ? 1e300 : (ix[d] + 1) * h;
+        const double e = fmax(fmax(lo - v[d], v[d] - hi) - 1e-9 * h, 0.0);
+        s += e * e;
+    }
+    return s * (1.0 - 1e-12);
+}
+
+// The point nearest v strictly inside the ball (v, r2) that is neither the
+// seed nor a plane of the cell ((distance, input index) order); -1: none;
+// -2: beyond BALL_RINGS.  Found by the whole group: the cells of a grid
+// ring are taken G at a time, their points spread over the group's lanes
+// (prefix sum of the cell counts, a binary search per lane), and the
+// group's best is reduced at the end of every ring.  With the bounds of
+// ball_cell_lb2 the answer is the argmin over all points, and the ring at
+// which the walk stops depends only on the rings before it: the order the
+// points are visited in does not matter.  v, r2, sid: the same on every lane
+// of the group, every lane of which is active; cst / cpf: the group's LDS
+// tables (G and G + 1 entries).
+template <int DIM, int G>
+PG_DEV int32_t ball_search_group(const BallGrid& bg, const double* __restrict__ pts, const double (&v)[3], double r2,
+                                 uint32_t sid, const int32_t* pvid, int nb_v, int gl, int32_t* cst, int32_t* cpf) {
     const int N = bg.N;
     const double h = 1.0 / N;
     int cc[3] = {0, 0, 0};
@@ -301,50 +331,81 @@ PG_DEV int32_t ball_search(const BallGrid& bg, const double* __restrict__ pts, c
     double best = r2;
     int32_t pick = -1;
     for (int ring = 0;; ring++) {
-        if (ring > 0) {
-            const double lb = (ring - 1) * h;
-            if (lb > 0 && lb * lb >= best) break;
+        if (ring > 1) {
+            // a point of ring r lies more than r - 1 cell widths away
+            const double lb = ((ring - 1) - 1e-9) * h;
+            if (lb * lb * (1.0 - 1e-12) >= best) break;
         }
         if (ring > N) break;
         if (ring > BALL_RINGS) return -2;
-        const int kr = DIM == 3 ? ring : 0;
-        for (int dk = -kr; dk <= kr; dk++)
-            for (int dj = -ring; dj <= ring; dj++) {
-                // the shell: full rows on its faces, two cells elsewhere
-                const bool full = (DIM == 3 && (dk == -ring || dk == ring)) || dj == -ring || dj == ring;
-                const int step = (full || ring == 0) ? 1 : 2 * ring;
-                for (int di = -ring; di <= ring; di += step) {
-                    const int ii = cc[0] + di, jj = cc[1] + dj, kk = cc[2] + dk;
-                    if (ii < 0 || jj < 0 || kk < 0 || ii >= N || jj >= N || (DIM == 3 && kk >= N)) continue;
-                    const int ix[3] = {ii, jj, kk};
-                    double lb2 = 0.0;
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) {
-                        const double lo = ix[d] * h, hi = (ix[d] + 1) * h;
-                        const double e = v[d] < lo ? lo - v[d] : (v[d] > hi ? v[d] - hi : 0.0);
-                        lb2 += e * e;
-                    }
-                    if (lb2 >= best) continue;
-                    const int64_t cell = ii + (int64_t)jj * N + (int64_t)kk * N * N;
-                    const int32_t c0 = bg.starts[cell], cn = bg.counts[cell];
-                    for (int t = 0; t < cn; t++) {
-                        const int32_t q = c0 + t;
-                        if ((uint32_t)q == sid) continue;
-                        double d2 = 0.0;
-#pragma unroll
-                        for (int d = 0; d < DIM; d++) {
-                            const double df = pts[(size_t)DIM * q + d] - v[d];
-                            d2 += df * df;
-                        }
-                        if (d2 > best || (d2 == best && (pick < 0 || bg.perm[q] > bg.perm[pick]))) continue;
-                        bool plane = false;
-                        for (int pl = 0; pl < nb_v && !plane; pl++) plane = pvid[pl] == q;
-                        if (plane) continue;
-                        best = d2;
-                        pick = q;
-                    }
+        const int side = 2 * ring + 1;
+        const int ncube = DIM == 3 ? side * side * side : side * side;
+        double lbest = best;
+        int32_t lpick = pick;
+        for (int base = 0; base < ncube; base += G) {
+            const int t = base + gl;
+            int32_t st = 0, cnt = 0;
+            if (t < ncube) {
+                const int di = t % side - ring, dj = (t / side) % side - ring;
+                const int dk = DIM == 3 ? t / (side * side) - ring : 0;
+                const int cheb = max(abs(di), max(abs(dj), abs(dk)));
+                const int ix[3] = {cc[0] + di, cc[1] + dj, cc[2] + dk};
+                if (cheb == ring && ix[0] >= 0 && ix[1] >= 0 && ix[2] >= 0 && ix[0] < N && ix[1] < N &&
+                    (DIM == 2 || ix[2] < N) && ball_cell_lb2<DIM>(N, v, ix) < best) {
+                    const int64_t cell = ix[0] + (int64_t)ix[1] * N + (int64_t)ix[2] * N * N;
+                    st = bg.starts[cell];
+                    cnt = bg.counts[cell];
                 }
             }
+            const int incl = group_incl_sum<G>(cnt, gl);
+            const int total = group_last<G>(incl);
+            if (total == 0) continue;
+            cst[gl] = st;
+            cpf[gl + 1] = incl;
+            if (gl == 0) cpf[0] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int cb = 0; cb < total; cb += G) {
+                const int j = cb + gl;
+                if (j >= total) continue;
+                // cell holding point j: largest a with cpf[a] <= j
+                int lo = 0, hi = G - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (cpf[mid] <= j) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int32_t q = cst[lo] + (j - cpf[lo]);
+                if ((uint32_t)q == sid) continue;
+                double d2 = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    const double df = pts[(size_t)DIM * q + d] - v[d];
+                    d2 += df * df;
+                }
+                // strictly inside; on ties the smaller input index
+                if (d2 > lbest || (d2 == lbest && (lpick < 0 || bg.perm[q] > bg.perm[lpick]))) continue;
+                bool plane = false;
+                for (int pl = 0; pl < nb_v && !plane; pl++) plane = pvid[pl] == q;
+                if (plane) continue;
+                lbest = d2;
+                lpick = q;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+        // the group's best after this ring
+        best = lbest;
+        pick = lpick;
+#pragma unroll
+        for (int d = 1; d < G; d <<= 1) {
+            const double od = shfl_xor(best, d, G);
+            const int32_t oq = shfl_xor(pick, d, G);
+            if (oq >= 0 && (pick < 0 || od < best || (od == best && bg.perm[oq] < bg.perm[pick]))) {
+                best = od;
+                pick = oq;
+            }
+        }
     }
     return pick;
 }
@@ -909,17 +970,33 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
             double bd = 0.0;
             int32_t bq = -1;
             bool far = false;
+            // the group searches its vertices one at a time, slot by slot
+            // (the wave walks until every group has searched its own)
+            int32_t* const cst = bg.scr + grp * (2 * G + 1);
+            int32_t* const cpf = cst + G;
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const uint32_t t = sm.tri[s * G + gl];
+                const bool need = !bdone && ((R.live >> s) & 1u) && (!((known >> s) & 1u) || ctri[s] != t);
+                uint64_t m = (wave_ballot(need) >> shift) & gmask;
+                while (wave_ballot(m != 0ull) != 0ull) {
+                    if (m == 0ull) continue;
+                    const int src = ffs64(m);
+                    m &= m - 1ull;
+                    const double vv[3] = {shfl(R.px[s], src, G), shfl(R.py[s], src, G), shfl(R.pz[s], src, G)};
+                    const double r2v = shfl(R.d2[s], src, G);
+                    const int32_t cq = ball_search_group<D, G>(bg, pts, vv, r2v, sid, sm.pvid, nb_v, gl, cst, cpf);
+                    if (gl == src) cand[s] = cq;
+                }
+                if (need) {
+                    ctri[s] = t;
+                    known |= 1u << s;
+                }
+            }
             if (!bdone) {
 #pragma unroll
                 for (int s = 0; s < S; s++) {
                     if (!((R.live >> s) & 1u)) continue;
-                    const uint32_t t = sm.tri[s * G + gl];
-                    if (!((known >> s) & 1u) || ctri[s] != t) {
-                        const double vv[3] = {R.px[s], R.py[s], R.pz[s]};
-                        cand[s] = ball_search<D>(bg, pts, vv, R.d2[s], sid, sm.pvid, nb_v);
-                        ctri[s] = t;
-                        known |= 1u << s;
-                    }
                     if (cand[s] == -2) far = true;
                     if (cand[s] < 0) continue;
                     const int32_t q = cand[s];
