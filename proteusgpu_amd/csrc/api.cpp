@@ -1849,9 +1849,13 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
                         q, nq, cap, k, B->knn.as<uint32_t>(), B->wave_min_k, B->bpc,
                         B->ticket.as<unsigned long long>(), s, tail);
     };
-    auto clip = [&](const uint32_t* q, const int64_t* nq, int k, int big) {
+    // 3D first pass and learned buckets: one block per seed group (the 3D
+    // clip spills in a persistent loop); the short restart lists, and every
+    // 2D list (whose clip does not spill): persistent blocks
+    auto clip = [&](const uint32_t* q, const int64_t* nq, int k, int big, bool restart_list) {
         pg::launch_bvoronoi(dim, B->sorted.as<double>(), q, nq, cap, B->knn.as<uint32_t>(), k, p.max_p, p.max_t, big,
-                            B->lanes, B->slots, B->faces(), B->bpc, B->ticket.as<unsigned long long>(), s);
+                            B->lanes, B->slots, B->faces(), B->bpc, B->ticket.as<unsigned long long>(),
+                            restart_list || dim == 2, s);
     };
     auto collect = [&](const uint32_t* L, const int64_t* nL, int code, uint32_t* out, int64_t* nout) {
         pg::launch_bcollect(nL, cap, L, B->status.as<int32_t>(), code, out, B->flags.as<int32_t>(),
@@ -1863,7 +1867,7 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
     int k = sched[0];
     if (!B->km) {
         knn(nullptr, n_dev, k, false);
-        clip(nullptr, n_dev, k, 0);
+        clip(nullptr, n_dev, k, 0, false);
     } else {
         float* feat = B->feat.as<float>();
         int32_t* cls = B->kcls.as<int32_t>();
@@ -1884,7 +1888,7 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
         for (int c = 0; c < pg::LK_NCLASS; c++) {
             const uint32_t* lc = B->klists.as<uint32_t>() + (size_t)c * cap;
             knn(lc, B->kcnt.as<int64_t>() + c, pg::lk_bucket(c), false);
-            clip(lc, B->kcnt.as<int64_t>() + c, pg::lk_bucket(c), 0);
+            clip(lc, B->kcnt.as<int64_t>() + c, pg::lk_bucket(c), 0, false);
         }
     }
     uint32_t* lst[3] = {B->lists.as<uint32_t>(), B->lists.as<uint32_t>() + cap, B->lists.as<uint32_t>() + 2 * cap};
@@ -1894,13 +1898,13 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
     for (size_t r = 1; r <= sched.size(); r++) {
         collect(L, nL, pg::SLOT_OVERFLOW, lst[2], cnt[2]);
         knn(lst[2], cnt[2], k, true);
-        clip(lst[2], cnt[2], k, 1);
+        clip(lst[2], cnt[2], k, 1, true);
         if (r == sched.size() || !p.restart) break;
         const int d = (L == lst[0]) ? 1 : 0;
         collect(L, nL, PG_SECURITY_RADIUS_NOT_REACHED, lst[d], cnt[d]);
         k = sched[r];
         knn(lst[d], cnt[d], k, false);
-        clip(lst[d], cnt[d], k, 0);
+        clip(lst[d], cnt[d], k, 0, true);
         L = lst[d];
         nL = cnt[d];
     }

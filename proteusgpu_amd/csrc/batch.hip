@@ -179,6 +179,7 @@ PG_DEV int ticket_chunk(int64_t items, int most) {
     return (int)(c < 1 ? 1 : (c > most ? most : c));
 }
 constexpr int KNN_CHUNK = 8;   // queries per ticket, at most
+constexpr int CLIP_CHUNK = 2;  // seed groups per ticket, at most
 
 // Resident one-wave blocks of a kernel on the device (its occupancy per CU,
 // capped at blocks_per_cu, times the CUs).
@@ -279,10 +280,79 @@ __global__ void __launch_bounds__(64, (M <= 2 ? 6 : 5)) k_bknn_wave(BKnnArgs arg
 }
 
 // ------------------------------------------------------------------ clip
-// One block per group of SPW listed seeds, over the list's capacity: blocks
-// past the device-side count end at once.  (In a loop over groups, even
-// with fresh arguments, the compiler keeps values across the groups that
-// make the clip's registers spill.)
+// Two forms.  k_bvoronoi: one block per group of SPW listed seeds, over the
+// list's capacity; blocks past the device-side count end at once (each
+// costs a dispatch, so it suits lists that fill most of their capacity).
+// k_bvoronoi_loop: persistent blocks taking groups from a counter, the
+// arguments re-read per group (in a loop the compiler still keeps values
+// across groups, and the clip's registers spill: it suits the short lists
+// of the certificate restarts, where capacity-sized grids would be mostly
+// empty blocks).
+struct BClipArgs {
+    const double* pts;
+    const uint32_t* seeds;
+    const int64_t* nq_dev;
+    const uint32_t* knn;
+    unsigned long long* ticket;
+    FaceArrays fa;
+    int k, max_p, max_t;
+};
+
+// a FaceArrays out of the argument segment, field by field
+PG_DEV FaceArrays face_arrays_of(const __attribute__((address_space(4))) FaceArrays& a) {
+    FaceArrays f;
+    f.stride = a.stride;
+    f.nbr = a.nbr;
+    f.area = a.area;
+    f.nrm = a.nrm;
+    f.cen = a.cen;
+    f.count = a.count;
+    f.vol = a.vol;
+    f.ccen = a.ccen;
+    f.status = a.status;
+    f.kcert = a.kcert;
+    f.cstate = a.cstate;
+    f.resume = a.resume;
+    f.lastnb = a.lastnb;
+    f.key = a.key;
+    f.nbr_cm = a.nbr_cm;
+    return f;
+}
+
+// FRESH: the arguments are re-read per group (3D: fewer spills); else they
+// stay in registers across groups (2D, whose clip has room for them).
+template <int DIM, int G, int S, bool FRESH>
+__global__ void __launch_bounds__(64, (VoroCfg<DIM, G, S>::WAVES)) k_bvoronoi_loop(BClipArgs args) {
+    using C = VoroCfg<DIM, G, S>;
+    __shared__ SeedSmem<C::NS, C::MAXP> sm[C::SPW];
+    if constexpr (FRESH) {
+        for (;;) {
+            const auto* A0 = fresh_args<BClipArgs>();
+            const int64_t nvb = (*A0->nq_dev + C::SPW - 1) / C::SPW;
+            const int64_t vb = next_ticket(A0->ticket, 1);
+            if (vb >= nvb) break;
+            const auto* A = fresh_args<BClipArgs>();
+            voronoi_group<DIM, G, S>(A->pts, INT64_MAX, A->seeds, *A->nq_dev, A->knn, A->k, A->max_p, A->max_t,
+                                     face_arrays_of(A->fa), sm, vb);
+            __syncthreads();
+        }
+    } else {
+        const int64_t nq = *args.nq_dev;
+        const int64_t nvb = (nq + C::SPW - 1) / C::SPW;
+        const int chunk = ticket_chunk(nvb, CLIP_CHUNK);
+        for (;;) {
+            const int64_t b0 = next_ticket(args.ticket, chunk);
+            if (b0 >= nvb) break;
+            const int64_t b1 = b0 + chunk < nvb ? b0 + chunk : nvb;
+            for (int64_t vb = b0; vb < b1; vb++) {
+                voronoi_group<DIM, G, S>(args.pts, INT64_MAX, args.seeds, nq, args.knn, args.k, args.max_p,
+                                         args.max_t, args.fa, sm, vb);
+                __syncthreads();
+            }
+        }
+    }
+}
+
 template <int DIM, int G, int S>
 __global__ void __launch_bounds__(64, (VoroCfg<DIM, G, S>::WAVES))
     k_bvoronoi(const double* __restrict__ pts, const uint32_t* __restrict__ seeds, const int64_t* __restrict__ nq_dev,
@@ -539,10 +609,19 @@ void bknn_dim(const double* sorted, const uint32_t* perm, const int32_t* sbox, c
 }
 template <int DIM, int G, int S>
 void bclip(const double* pts, const uint32_t* seeds, const int64_t* nq_dev, int64_t cap, const uint32_t* knn, int k,
-           int max_p, int max_t, FaceArrays fa, int blocks_per_cu, unsigned long long* ticket, hipStream_t s) {
+           int max_p, int max_t, FaceArrays fa, int blocks_per_cu, unsigned long long* ticket, bool loop,
+           hipStream_t s) {
     using C = VoroCfg<DIM, G, S>;
-    (void)blocks_per_cu;
-    (void)ticket;
+    if (loop) {
+        constexpr bool FRESH = DIM == 3;
+        static const int occ = occupancy(k_bvoronoi_loop<DIM, G, S, FRESH>);
+        const unsigned nb =
+            (unsigned)std::min<int64_t>((cap + C::SPW - 1) / C::SPW, resident_blocks(occ, blocks_per_cu));
+        k_fill<unsigned long long><<<1, TB, 0, s>>>(ticket, 1, 0ull);
+        const BClipArgs args{pts, seeds, nq_dev, knn, ticket, fa, k, max_p, max_t};
+        k_bvoronoi_loop<DIM, G, S, FRESH><<<std::max(nb, 1u), 64, 0, s>>>(args);
+        return;
+    }
     const unsigned nb = (unsigned)std::max<int64_t>((cap + C::SPW - 1) / C::SPW, 1);
     k_bvoronoi<DIM, G, S><<<nb, 64, 0, s>>>(pts, seeds, nq_dev, knn, k, max_p, max_t, fa);
 }
@@ -562,14 +641,14 @@ void launch_bknn(int dim, const double* sorted, const uint32_t* perm, const int3
 
 void launch_bvoronoi(int dim, const double* sorted, const uint32_t* seeds, const int64_t* nq_dev, int64_t cap,
                      const uint32_t* knn, int k, int max_p, int max_t, int big, int lanes, int slots, FaceArrays fa,
-                     int blocks_per_cu, unsigned long long* ticket, hipStream_t s) {
+                     int blocks_per_cu, unsigned long long* ticket, bool loop, hipStream_t s) {
     auto go = [&](auto dimc) {
         constexpr int D = decltype(dimc)::value;
-        if (big) bclip<D, 16, 8>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, ticket, s);
-        else if (lanes == 8 && slots == 48) bclip<D, 8, 6>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, ticket, s);
-        else if (lanes == 8) bclip<D, 8, 8>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, ticket, s);
-        else if (slots == 48) bclip<D, 16, 3>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, ticket, s);
-        else bclip<D, 16, 4>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, ticket, s);
+        if (big) bclip<D, 16, 8>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, ticket, loop, s);
+        else if (lanes == 8 && slots == 48) bclip<D, 8, 6>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, ticket, loop, s);
+        else if (lanes == 8) bclip<D, 8, 8>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, ticket, loop, s);
+        else if (slots == 48) bclip<D, 16, 3>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, ticket, loop, s);
+        else bclip<D, 16, 4>(sorted, seeds, nq_dev, cap, knn, k, max_p, max_t, fa, blocks_per_cu, ticket, loop, s);
     };
     if (dim == 2) go(std::integral_constant<int, 2>{});
     else go(std::integral_constant<int, 3>{});

@@ -215,9 +215,11 @@ void launch_bknn(int dim, const double* sorted, const uint32_t* perm, const int3
                  unsigned long long* ticket, hipStream_t s, KnnTail tail = {});
 // ticket: a device counter, the work queue of the persistent kernels
 // (cleared by the launcher, in stream order)
+// loop: persistent blocks over the list (short lists); else one block per
+// group of seeds over the capacity (lists that fill most of it)
 void launch_bvoronoi(int dim, const double* sorted, const uint32_t* seeds, const int64_t* nq_dev, int64_t cap,
                      const uint32_t* knn, int k, int max_p, int max_t, int big, int lanes, int slots, FaceArrays fa,
-                     int blocks_per_cu, unsigned long long* ticket, hipStream_t s);
+                     int blocks_per_cu, unsigned long long* ticket, bool loop, hipStream_t s);
 void launch_bcollect(const int64_t* nq_dev, int64_t cap, const uint32_t* seeds, const int32_t* status, int code,
                      uint32_t* list, int32_t* flags, int32_t* offs, int32_t* tmp, int32_t* total, int64_t* nout,
                      hipStream_t s);
