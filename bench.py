@@ -208,8 +208,9 @@ def profile_variant(a, ctx, api, pts, rho, vel, E, kw):
             B = api.Batch(ctx, np.array([a.n]), pts, rho, vel, E, **kw)
             if model is not None:
                 B.set_kmodel(model)
+            tune, _ = B.autotune(reps=1)  # the configuration the full run times
             ms, gpu = run_batch(ctx, B, a.warmup, a.steps)
-            out[name] = {"ms_per_step": round(ms, 3), "gpu_ms_per_step": round(gpu, 3)}
+            out[name] = {"ms_per_step": round(ms, 3), "gpu_ms_per_step": round(gpu, 3), "tuning": tune}
             B.free()
     out["measured_copy_GBs"] = round(ctx.stream_copy_probe(a.probe_bytes, 10), 1)
     print(json.dumps(out))

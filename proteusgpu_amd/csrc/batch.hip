@@ -159,18 +159,25 @@ __global__ void __launch_bounds__(TB) k_fill(T* a, int64_t n, T v) {
     for (int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TB) a[i] = v;
 }
 
-// Work queue of the one-wave persistent kernels: blocks take the next
-// chunk of items from a counter (cleared before the launch), so seeds of
-// uneven cost balance over the resident blocks; chunks keep the atomics on
-// the one counter few.
-PG_DEV int64_t next_ticket(unsigned long long* ticket, int chunk) {
+// Work queue of the one-wave persistent kernels: a block's first chunk of
+// items is its own (chunk blockIdx.x), the later ones come from a counter
+// (cleared before the launch) counting past the grid's first chunks, so
+// seeds of uneven cost balance over the resident blocks.  Atomics on the
+// one counter serialize (about 10 ns each): the first chunks take none (a
+// block of an empty or short list leaves without one), and chunks keep the
+// rest few.  `first` starts true in every block.
+PG_DEV int64_t next_ticket(unsigned long long* ticket, int chunk, bool& first) {
+    if (first) {
+        first = false;
+        return (int64_t)blockIdx.x * chunk;
+    }
     unsigned long long t = 0;
     if (threadIdx.x == 0) t = atomicAdd(ticket, (unsigned long long)chunk);
     // lane 0's value as a scalar: the compiler then knows the whole wave
     // works on the same items (scalar loads, no per-lane copies of them)
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)t);
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(t >> 32));
-    return (int64_t)((uint64_t)hi << 32 | lo);
+    return (int64_t)gridDim.x * chunk + (int64_t)((uint64_t)hi << 32 | lo);
 }
 // Items per ticket: up to `most`, fewer for short lists (each block should
 // see several chunks, so that the last ones balance).
@@ -178,7 +185,7 @@ PG_DEV int ticket_chunk(int64_t items, int most) {
     const int64_t c = items / (4 * (int64_t)gridDim.x);
     return (int)(c < 1 ? 1 : (c > most ? most : c));
 }
-constexpr int KNN_CHUNK = 8;   // queries per ticket, at most
+constexpr int KNN_CHUNK = 16;  // queries per ticket, at most
 constexpr int CLIP_CHUNK = 2;  // seed groups per ticket, at most
 
 // Resident one-wave blocks of a kernel on the device (its occupancy per CU,
@@ -256,8 +263,9 @@ __global__ void __launch_bounds__(64, (M <= 2 ? 6 : 5)) k_bknn_wave(BKnnArgs arg
     __shared__ KnnPend pend;
     const int64_t nq = *args.nq_dev;
     const int chunk = ticket_chunk(nq, KNN_CHUNK);
+    bool first = true;
     for (;;) {
-        const int64_t q0 = next_ticket(fresh_args<BKnnArgs>()->ticket, chunk);
+        const int64_t q0 = next_ticket(fresh_args<BKnnArgs>()->ticket, chunk, first);
         if (q0 >= nq) break;
         const int64_t q1 = q0 + chunk < nq ? q0 + chunk : nq;
         for (int64_t qi = q0; qi < q1; qi++) {
@@ -326,10 +334,11 @@ __global__ void __launch_bounds__(64, (VoroCfg<DIM, G, S>::WAVES)) k_bvoronoi_lo
     using C = VoroCfg<DIM, G, S>;
     __shared__ SeedSmem<C::NS, C::MAXP> sm[C::SPW];
     if constexpr (FRESH) {
+        bool first = true;
         for (;;) {
             const auto* A0 = fresh_args<BClipArgs>();
             const int64_t nvb = (*A0->nq_dev + C::SPW - 1) / C::SPW;
-            const int64_t vb = next_ticket(A0->ticket, 1);
+            const int64_t vb = next_ticket(A0->ticket, 1, first);
             if (vb >= nvb) break;
             const auto* A = fresh_args<BClipArgs>();
             voronoi_group<DIM, G, S>(A->pts, INT64_MAX, A->seeds, *A->nq_dev, A->knn, A->k, A->max_p, A->max_t,
@@ -340,8 +349,9 @@ __global__ void __launch_bounds__(64, (VoroCfg<DIM, G, S>::WAVES)) k_bvoronoi_lo
         const int64_t nq = *args.nq_dev;
         const int64_t nvb = (nq + C::SPW - 1) / C::SPW;
         const int chunk = ticket_chunk(nvb, CLIP_CHUNK);
+        bool first = true;
         for (;;) {
-            const int64_t b0 = next_ticket(args.ticket, chunk);
+            const int64_t b0 = next_ticket(args.ticket, chunk, first);
             if (b0 >= nvb) break;
             const int64_t b1 = b0 + chunk < nvb ? b0 + chunk : nvb;
             for (int64_t vb = b0; vb < b1; vb++) {
