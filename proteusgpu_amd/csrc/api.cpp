@@ -112,6 +112,10 @@ struct Grid {
     int N = 0;
     int64_t ncell = 0;
     DBuf pts, sorted, perm, cell, counts, starts, fill, tmp, total, big;
+    // sub-grids of the crowded cells (pg::SubGrid), built with the grid
+    DBuf sub_m, sub_ptr, sub_start, sub_cnt, sub_idx, sub_alloc;
+    bool has_sub = false;
+    int32_t crowded = -1;  // cells over 32 points, read back by note_crowded (-1: not read)
 
     void build(hipStream_t s) {
         ncell = (int64_t)N * N * (dim == 3 ? N : 1);
@@ -128,7 +132,32 @@ struct Grid {
         int32_t* bg = big.get<int32_t>(ncell + 1);
         pg::launch_cell_sort(ncell, st, cnt, pm, bg + 1, bg, s);
         pg::launch_gather_points(dim, n, pm, pts.as<double>(), sorted.get<double>((size_t)n * dim), s);
+        // a sub-cell table holds at most n / 3 + 64 entries (about 3 points
+        // per sub-cell, at least 8 (2D: 9) sub-cells per cell of over 32)
+        const int64_t cap = n / 2 + 1024;
+        pg::launch_subgrid(dim, sorted.as<double>(), n, N, ncell, st, cnt, bg + 1, bg, sub_m.get<int32_t>(ncell),
+                           sub_ptr.get<int32_t>(ncell), sub_start.get<int32_t>(cap), sub_cnt.get<int32_t>(cap), cap,
+                           sub_idx.get<uint32_t>(n), sub_alloc.get<int32_t>(1), s);
+        has_sub = true;
+        crowded = -1;
         HIPCHK(hipGetLastError());
+    }
+    // Queue the copy of the crowded-cell count to crowded (read after the
+    // stream's next synchronisation).
+    void note_crowded(hipStream_t s) {
+        if (has_sub) HIPCHK(hipMemcpyAsync(&crowded, big.as<int32_t>(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    }
+    // The sub-grid for the searches: empty without crowded cells (the
+    // kernels without sub-grid code run), or when the count was not read.
+    pg::SubGrid sub() const {
+        pg::SubGrid g;
+        if (!has_sub || crowded <= 0) return g;
+        g.m = sub_m.as<int32_t>();
+        g.ptr = sub_ptr.as<int32_t>();
+        g.start = sub_start.as<int32_t>();
+        g.cnt = sub_cnt.as<int32_t>();
+        g.idx = sub_idx.as<uint32_t>();
+        return g;
     }
 };
 
@@ -244,7 +273,7 @@ void run_list(pg_mesh* m, const uint32_t* list, int64_t nq, int k, int big, hipS
     if (m->use_resume && !big && !exact)
         tail = {m->resume.as<int32_t>(), m->lastnb.as<uint32_t>(), clip_shape_tag(m->lanes, m->slots)};
     pg::launch_knn(m->prm.dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), m->n_seeds, m->g.N,
-                   m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), list, nq, k, rows, s, tail);
+                   m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), list, nq, k, rows, s, tail, m->g.sub());
     pg::FaceArrays fa = m->faces();
     if (exact) {
         // the exact kernel neither saves nor resumes cells; ties are broken
@@ -324,10 +353,10 @@ void ball_pass(pg_mesh* m, hipStream_t s) {
     const int kb = std::min(cap, 64);
     uint32_t* rows = m->rows.get<uint32_t>((size_t)nL * kb);
     pg::launch_knn(p.dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), m->n_seeds, m->g.N,
-                   m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), L, nL, kb, rows, s);
+                   m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), L, nL, kb, rows, s, {}, m->g.sub());
     pg::launch_voronoi_ball(p.dim, m->g.sorted.as<double>(), m->n_seeds, L, nL, rows, kb, p.max_p, p.max_t,
                             m->faces(), m->g.N, m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(),
-                            m->g.perm.as<uint32_t>(), cap + 1, s);
+                            m->g.perm.as<uint32_t>(), cap + 1, s, m->g.sub());
     HIPCHK(hipGetLastError());
     m->info.row_entries += nL * kb;
 }
@@ -466,10 +495,13 @@ void mesh_build_once(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool o
     if (!km) {
         // first pass: every seed at the configured k
         const int k = p.k;
+        m->g.note_crowded(s);  // the searches take the sub-grid only if some cell is crowded
+        HIPCHK(hipStreamSynchronize(s));
         if (mk) mk->mark(MK_PREDICT, s);
         uint32_t* rows = m->knn.get<uint32_t>(ns * k);
         pg::launch_knn(dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), nseeds, m->g.N,
-                       m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), nullptr, nseeds, k, rows, s);
+                       m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), nullptr, nseeds, k, rows, s, {},
+                       m->g.sub());
         if (mk) mk->mark(MK_KNN, s);
         pg::launch_voronoi(dim, m->g.sorted.as<double>(), nseeds, nullptr, nseeds, rows, k, p.max_p, p.max_t,
                            0, m->faces(), m->lanes, m->slots, s);
@@ -506,6 +538,7 @@ void mesh_build_once(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool o
                                m->klist.get<uint32_t>(ns * pg::LK_NCLASS), dcnt, s, order);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(hc, dcnt, sizeof(hc), hipMemcpyDeviceToHost, s));
+            m->g.note_crowded(s);
             HIPCHK(hipStreamSynchronize(s));
         }
         uint32_t* kl = m->klist.as<uint32_t>();
@@ -522,7 +555,7 @@ void mesh_build_once(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool o
             if (hc[c] > 0)
                 pg::launch_knn(dim, m->g.sorted.as<double>(), m->g.perm.as<uint32_t>(), nseeds, m->g.N,
                                m->g.starts.as<int32_t>(), m->g.counts.as<int32_t>(), kl + (size_t)c * ns, hc[c],
-                               bk[c], rows + roff[c], s);
+                               bk[c], rows + roff[c], s, {}, m->g.sub());
         if (mk) mk->mark(MK_KNN, s);
         for (int c = 0; c < pg::LK_NCLASS; c++)
             if (hc[c] > 0) {
@@ -764,6 +797,7 @@ int pg_knn_init(pg_ctx* c, int dim, const double* pts, int64_t n, pg_knn** out) 
             double* d = k->g.pts.get<double>((size_t)n * dim);
             HIPCHK(hipMemcpyAsync(d, pts, sizeof(double) * n * dim, hipMemcpyHostToDevice, c->stream));
             k->g.build(c->stream);
+            k->g.note_crowded(c->stream);
             HIPCHK(hipStreamSynchronize(c->stream));
         } catch (...) {
             delete k;
@@ -781,7 +815,7 @@ int pg_knn_solve(pg_ctx* c, pg_knn* k, int kk) {
         uint32_t* o = k->knn.get<uint32_t>((size_t)k->g.n * kk);
         pg::launch_knn(k->g.dim, k->g.sorted.as<double>(), k->g.perm.as<uint32_t>(), k->g.n, k->g.N,
                        k->g.starts.as<int32_t>(), k->g.counts.as<int32_t>(), nullptr, k->g.n, kk, o,
-                       c->stream);
+                       c->stream, {}, k->g.sub());
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
         k->k = kk;

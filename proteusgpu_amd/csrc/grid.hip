@@ -15,6 +15,7 @@
 #include "common.h"
 #include "ghosts.h"
 #include "kernels.h"
+#include "subgrid.h"
 
 namespace pg {
 namespace {
@@ -197,6 +198,86 @@ __global__ void k_fill_one(int32_t* p) {
     if (threadIdx.x == 0) *p = 0;
 }
 
+__global__ void __launch_bounds__(TB) k_fill_i32(int32_t* a, int64_t n, int32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TB) a[i] = v;
+}
+
+// One block per listed cell (grid-stride over the list of cells longer than
+// 32): the cell's sub-cell histogram in LDS, its exclusive scan (each thread
+// a contiguous run of sub-cells, then the runs' sums in order), the table
+// of (first position, count) per sub-cell from a range taken off *alloc,
+// and the scatter of the cell's sorted ids by sub-cell into idx.  The order
+// inside a sub-cell follows the LDS atomics (searches do not depend on it).
+// A cell whose table would not fit in cap keeps m = 0 (its searches scan
+// the whole cell).
+template <int DIM>
+__global__ void __launch_bounds__(TB) k_sub_build(const int32_t* __restrict__ big, const int32_t* __restrict__ nbig,
+                                                  const int32_t* __restrict__ starts,
+                                                  const int32_t* __restrict__ counts,
+                                                  const double* __restrict__ sorted, int N, int32_t* __restrict__ sm,
+                                                  int32_t* __restrict__ sptr, int32_t* __restrict__ sstart,
+                                                  int32_t* __restrict__ scnt, int64_t cap, uint32_t* __restrict__ idx,
+                                                  int32_t* alloc) {
+    __shared__ int32_t hist[4096];
+    __shared__ int32_t part[TB];
+    __shared__ int32_t base_s;
+    const int nb = *nbig;
+    const int tid = threadIdx.x;
+    for (int bi = blockIdx.x; bi < nb; bi += gridDim.x) {
+        const int c = big[bi];
+        const int64_t st = starts[c];
+        const int cnt = counts[c];
+        const int m = sub_split<DIM>(cnt);
+        const int nsub = DIM == 3 ? m * m * m : m * m;
+        const int ix[3] = {c % N, (c / N) % N, DIM == 3 ? c / (N * N) : 0};
+        for (int j = tid; j < nsub; j += TB) hist[j] = 0;
+        __syncthreads();
+        for (int i = tid; i < cnt; i += TB)
+            atomicAdd(&hist[sub_index<DIM>(sorted + (size_t)DIM * (st + i), ix, m, N)], 1);
+        __syncthreads();
+        const int per = (nsub + TB - 1) / TB;
+        const int q0 = min(nsub, tid * per), q1 = min(nsub, q0 + per);
+        int lsum = 0;
+        for (int q = q0; q < q1; q++) lsum += hist[q];
+        part[tid] = lsum;
+        __syncthreads();
+        if (tid == 0) {
+            int run = 0;
+            for (int t = 0; t < TB; t++) {
+                const int v = part[t];
+                part[t] = run;
+                run += v;
+            }
+            const int b = atomicAdd(alloc, nsub);
+            base_s = ((int64_t)b + nsub <= cap) ? b : -1;
+        }
+        __syncthreads();
+        const int base = base_s;
+        if (base >= 0) {
+            int run = part[tid];
+            for (int q = q0; q < q1; q++) {
+                const int h = hist[q];
+                sstart[base + q] = (int32_t)(st + run);
+                scnt[base + q] = h;
+                hist[q] = run;
+                run += h;
+            }
+            __syncthreads();
+            for (int i = tid; i < cnt; i += TB) {
+                const int64_t s = st + i;
+                const int j = sub_index<DIM>(sorted + (size_t)DIM * s, ix, m, N);
+                const int pos = atomicAdd(&hist[j], 1);
+                idx[st + pos] = (uint32_t)s;
+            }
+            if (tid == 0) {
+                sm[c] = m;
+                sptr[c] = base;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 void launch_ghost_count(int dim, const double* pts, int64_t n, double buff, int32_t* cnt,
@@ -238,6 +319,19 @@ void launch_cell_sort(int64_t ncell, const int32_t* starts, const int32_t* count
     k_fill_one<<<1, 64, 0, s>>>(nbig);
     k_cell_sort<<<nblk(ncell), TB, 0, s>>>(ncell, starts, counts, idx, big, nbig);
     k_cell_sort_big<<<(unsigned)std::min<int64_t>(ncell, 1024), TB, 0, s>>>(big, nbig, starts, counts, idx);
+}
+
+void launch_subgrid(int dim, const double* sorted, int64_t n, int N, int64_t ncell, const int32_t* starts,
+                    const int32_t* counts, const int32_t* big, const int32_t* nbig, int32_t* m, int32_t* ptr,
+                    int32_t* start, int32_t* cnt, int64_t cap, uint32_t* idx, int32_t* alloc, hipStream_t s) {
+    if (ncell <= 0 || n <= 0) return;
+    k_fill_i32<<<(unsigned)std::min<int64_t>(nblk(ncell), 4096), TB, 0, s>>>(m, ncell, 0);
+    k_fill_one<<<1, 64, 0, s>>>(alloc);
+    const unsigned nb = (unsigned)std::min<int64_t>(ncell, 1024);
+    if (dim == 2)
+        k_sub_build<2><<<nb, TB, 0, s>>>(big, nbig, starts, counts, sorted, N, m, ptr, start, cnt, cap, idx, alloc);
+    else
+        k_sub_build<3><<<nb, TB, 0, s>>>(big, nbig, starts, counts, sorted, N, m, ptr, start, cnt, cap, idx, alloc);
 }
 
 void launch_displace(int dim, const double* in, int64_t n, double amp, uint64_t salt, int periodic, double* out,

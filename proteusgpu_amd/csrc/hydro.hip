@@ -384,17 +384,32 @@ __global__ void __launch_bounds__(TB)
 #pragma unroll
         for (int k = 0; k < DIM + 2; k++) q[k] = 0.0;
         const int c = count[s];
-        for (int f = 0; f < c; f++) {
-            const int64_t at = (int64_t)f * stride + s;
-            const uint8_t tw = twin[at];
-            if (tw == SOLVE) {
-                const double* fo = fbuf + at * (DIM + 2);
+        // four faces' records are loaded together (independent loads in
+        // flight), then added in face order
+        for (int f0 = 0; f0 < c; f0 += 4) {
+            double v[4][DIM + 2];
+            bool own[4];
 #pragma unroll
-                for (int k = 0; k < DIM + 2; k++) q[k] += fo[k];
-            } else {
-                const double* fo = fbuf + ((int64_t)tw * stride + nbr[at]) * (DIM + 2);
+            for (int e = 0; e < 4; e++) {
+                own[e] = true;
 #pragma unroll
-                for (int k = 0; k < DIM + 2; k++) q[k] -= fo[k];
+                for (int k = 0; k < DIM + 2; k++) v[e][k] = 0.0;
+                if (f0 + e >= c) continue;
+                const int64_t at = (int64_t)(f0 + e) * stride + s;
+                const uint8_t tw = twin[at];
+                own[e] = tw == SOLVE;
+                const double* fo = fbuf + (own[e] ? at : (int64_t)tw * stride + nbr[at]) * (DIM + 2);
+#pragma unroll
+                for (int k = 0; k < DIM + 2; k++) v[e][k] = fo[k];
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                if (f0 + e >= c) break;
+#pragma unroll
+                for (int k = 0; k < DIM + 2; k++) {
+                    if (own[e]) q[k] += v[e][k];
+                    else q[k] -= v[e][k];
+                }
             }
         }
         sv.mass[i] -= dt * q[0];

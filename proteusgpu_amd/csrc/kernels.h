@@ -41,6 +41,30 @@ void launch_displace(int dim, const double* in, int64_t n, double amp, uint64_t 
 void launch_gather_points(int dim, int64_t n, const uint32_t* perm, const double* pts,
                           double* sorted, hipStream_t s);
 
+// Sub-grids of the crowded grid cells (clustered inputs): a cell holding
+// more than SUB_MIN points is split m ways per axis (m^dim sub-cells of about
+// 3 points), and its points (sorted ids starts[c] .. starts[c] + counts[c])
+// are listed again by sub-cell in idx[starts[c] ..].  m[c] = 0: no
+// sub-grid; else the cell's m^dim sub-cells are start/cnt[ptr[c] + j]
+// (first position in idx, count), j = a + b m (+ c m^2) for sub-cell
+// (a, b, c).  Searches use it only to skip parts of a cell their ball
+// misses; the sorted order, and with it every row, is unchanged.
+constexpr int SUB_MIN = 32;
+struct SubGrid {
+    const int32_t* m = nullptr;
+    const int32_t* ptr = nullptr;
+    const int32_t* start = nullptr;
+    const int32_t* cnt = nullptr;
+    const uint32_t* idx = nullptr;
+};
+// Build it after launch_cell_sort (whose list of cells longer than 32 it
+// takes) and launch_gather_points.  m, ptr: ncell int32; start, cnt: cap
+// int32 (cap >= n / 3 + 64 always suffices); idx: n uint32; alloc: one
+// int32 of scratch.
+void launch_subgrid(int dim, const double* sorted, int64_t n, int N, int64_t ncell, const int32_t* starts,
+                    const int32_t* counts, const int32_t* big, const int32_t* nbig, int32_t* m, int32_t* ptr,
+                    int32_t* start, int32_t* cnt, int64_t cap, uint32_t* idx, int32_t* alloc, hipStream_t s);
+
 // ----------------------------------------------------------------- knn.hip
 // A KNN launch for seeds whose clip will resume: their rows need only the
 // neighbours after the saved row length (resume[s] & 0xffff, the last of
@@ -56,7 +80,7 @@ struct KnnTail {
 // Rows ascending by (FP64 dist^2, original index); out row stride k.
 void launch_knn(int dim, const double* sorted, const uint32_t* perm, int64_t n, int N,
                 const int32_t* starts, const int32_t* counts, const uint32_t* queries, int64_t nq,
-                int k, uint32_t* out, hipStream_t s, KnnTail tail = {});
+                int k, uint32_t* out, hipStream_t s, KnnTail tail = {}, SubGrid sg = {});
 
 // ------------------------------------------------------------- voronoi.hip
 struct FaceArrays {
@@ -113,7 +137,7 @@ void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* se
 void launch_voronoi_ball(int dim, const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq,
                          const uint32_t* knn, int k, int max_p, int max_t, FaceArrays fa, int N,
                          const int32_t* starts, const int32_t* counts, const uint32_t* perm, int kmark,
-                         hipStream_t s);
+                         hipStream_t s, SubGrid sg = {});
 
 // Process-wide default shape of the fast clip kernel for the following
 // builds: lanes per seed (8 or 16) and vertex slots per seed (48 or 64).

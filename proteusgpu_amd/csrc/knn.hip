@@ -36,12 +36,14 @@ __global__ void __launch_bounds__(BLOCK)
 template <int M>
 constexpr int knn_wave_occupancy() { return M <= 2 ? 6 : (M <= 4 ? 5 : 2); }
 
-template <int DIM, int M>
+// SUB: the grid has crowded cells with sub-grids (a separate instance, so
+// that the uniform case keeps its registers)
+template <int DIM, int M, bool SUB>
 __global__ void __launch_bounds__(64, knn_wave_occupancy<M>())
     k_knn_wave(const double* __restrict__ sorted, const uint32_t* __restrict__ perm, int N,
                const int32_t* __restrict__ starts, const int32_t* __restrict__ counts,
                const uint32_t* __restrict__ queries, int64_t nq, int k, uint32_t* __restrict__ out,
-               KnnTail tail) {
+               KnnTail tail, SubGrid sg) {
     __shared__ int32_t c_start[64], c_pfx[65];
     __shared__ KnnPend pend;
     const int64_t qi = blockIdx.x;
@@ -50,29 +52,34 @@ __global__ void __launch_bounds__(64, knn_wave_occupancy<M>())
     int kp = 0;
     uint32_t lo = 0;
     tail_of(tail, q, k, kp, lo);
-    knn_wave_query<DIM, M>(sorted, perm, N, 0, starts, counts, q, k, out + (size_t)qi * k, c_start, c_pfx, &pend,
-                           kp, lo);
+    knn_wave_query<DIM, M, SUB>(sorted, perm, N, 0, starts, counts, q, k, out + (size_t)qi * k, c_start, c_pfx,
+                                &pend, kp, lo, sg);
 }
 
 template <int DIM, int M>
 void run_wave(const double* sorted, const uint32_t* perm, int N, const int32_t* starts,
               const int32_t* counts, const uint32_t* queries, int64_t nq, int k, uint32_t* out,
-              KnnTail tail, hipStream_t s) {
-    k_knn_wave<DIM, M><<<(unsigned)nq, 64, 0, s>>>(sorted, perm, N, starts, counts, queries, nq, k, out, tail);
+              KnnTail tail, SubGrid sg, hipStream_t s) {
+    if (sg.m != nullptr)
+        k_knn_wave<DIM, M, true><<<(unsigned)nq, 64, 0, s>>>(sorted, perm, N, starts, counts, queries, nq, k, out,
+                                                             tail, sg);
+    else
+        k_knn_wave<DIM, M, false><<<(unsigned)nq, 64, 0, s>>>(sorted, perm, N, starts, counts, queries, nq, k, out,
+                                                              tail, sg);
 }
 
 template <int DIM>
 void dispatch_wave(const double* sorted, const uint32_t* perm, int N, const int32_t* starts,
                    const int32_t* counts, const uint32_t* queries, int64_t nq, int k, uint32_t* out,
-                   KnnTail tail, hipStream_t s) {
+                   KnnTail tail, SubGrid sg, hipStream_t s) {
     // the kept list holds 64 M entries; a resumed query keeps only its
     // k - kp missing neighbours, but M is sized for the full row
-    if (k <= 64) run_wave<DIM, 1>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
-    else if (k <= 128) run_wave<DIM, 2>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
-    else if (k <= 192) run_wave<DIM, 3>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
-    else if (k <= 256) run_wave<DIM, 4>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
-    else if (k <= 512) run_wave<DIM, 8>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
-    else run_wave<DIM, 16>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
+    if (k <= 64) run_wave<DIM, 1>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, sg, s);
+    else if (k <= 128) run_wave<DIM, 2>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, sg, s);
+    else if (k <= 192) run_wave<DIM, 3>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, sg, s);
+    else if (k <= 256) run_wave<DIM, 4>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, sg, s);
+    else if (k <= 512) run_wave<DIM, 8>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, sg, s);
+    else run_wave<DIM, 16>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, sg, s);
 }
 
 template <int DIM, int BLOCK>
@@ -91,7 +98,7 @@ void run(const double* sorted, const uint32_t* perm, int N, const int32_t* start
 
 void launch_knn(int dim, const double* sorted, const uint32_t* perm, int64_t n, int N,
                 const int32_t* starts, const int32_t* counts, const uint32_t* queries, int64_t nq,
-                int k, uint32_t* out, hipStream_t s, KnnTail tail) {
+                int k, uint32_t* out, hipStream_t s, KnnTail tail, SubGrid sg) {
     (void)n;
     if (nq <= 0) return;
     // Long rows (and short query lists) go to the wave-per-query kernel; the
@@ -101,8 +108,8 @@ void launch_knn(int dim, const double* sorted, const uint32_t* perm, int64_t n, 
         return e ? atoi(e) : 20;
     }();
     if (k >= wave_min_k || k > 256) {  // the LDS heap holds at most 256 entries per query
-        if (dim == 2) dispatch_wave<2>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
-        else dispatch_wave<3>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, s);
+        if (dim == 2) dispatch_wave<2>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, sg, s);
+        else dispatch_wave<3>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, sg, s);
         return;
     }
     // LDS per block = 12 B * k * BLOCK.  gfx950 gives a workgroup up to

@@ -5,6 +5,7 @@
 #include "common.h"
 #include "kernels.h"
 #include "simt.h"
+#include "subgrid.h"
 
 namespace pg {
 
@@ -347,11 +348,12 @@ PG_DEV double knn_guess_r2(const double (&p)[3], int N, int cnt, int nin, float 
 //    itself.
 // Fewer than k candidates in the ball: a new pass from scratch with a wider
 // one (after three, none).
-template <int DIM, int M>
+template <int DIM, int M, bool SUB = false>
 PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __restrict__ perm, int N,
                            int64_t cbase, const int32_t* __restrict__ starts,
                            const int32_t* __restrict__ counts, uint32_t q, int k, uint32_t* __restrict__ o,
-                           int32_t* c_start, int32_t* c_pfx, KnnPend* pend, int kp = 0, uint32_t lo_s = 0) {
+                           int32_t* c_start, int32_t* c_pfx, KnnPend* pend, int kp = 0, uint32_t lo_s = 0,
+                           SubGrid sg = {}) {
     const int l = threadIdx.x;
     double p[3] = {0, 0, 0};
 #pragma unroll
@@ -422,6 +424,73 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
         const int tot = (int)read_lane((uint32_t)wave_incl_sum(cnt), 63);
         r2 = knn_guess_r2<DIM>(p, N, tot, __popcll((unsigned long long)__ballot(in)), lam);
     }
+    bool odd_l = false;  // a kept candidate below its ring's distance bound
+    // Candidates 0 .. total-1 of the cells (or sub-cells) in c_start / c_pfx
+    // (64 entries: first position, exclusive prefix of the counts), 64 at a
+    // time; positions are sorted ids, or indices into indir.  Those inside
+    // the ball and ahead of the current k-th entry join the LDS buffer.
+    auto scan = [&](int total, const uint32_t* __restrict__ indir, double md2) {
+        for (int cb = 0; cb < total; cb += 64) {
+            const int j = cb + l;
+            bool take = false;
+            double dd = 0;
+            uint32_t cs = 0;
+            if (j < total) {
+                // cell holding candidate j: largest a with c_pfx[a] <= j
+                int lo = 0, hi = 63;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (c_pfx[mid] <= j) lo = mid;
+                    else hi = mid - 1;
+                }
+                cs = (uint32_t)(c_start[lo] + (j - c_pfx[lo]));
+                if (indir != nullptr) cs = indir[cs];
+                if (cs != q) {
+                    double pc[3] = {0, 0, 0};
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) pc[d] = sorted[(size_t)DIM * cs + d];
+                    dd = dist2<DIM>(p, pc);
+                    take = dd <= r2 && (kp == 0 || ent_less(perm, lo_d, lo_s, dd, cs));
+                }
+            }
+            if (size == k) {
+                double kth;
+                uint32_t kts;
+                kth_entry(kth, kts);
+                take = take && ent_less(perm, dd, cs, kth, kts);
+            }
+            // a kept candidate nearer than its ring's bound (rounding at a
+            // ring's edge): the ball's argument fails
+            odd_l = odd_l || (take && dd < md2);
+            const uint64_t surv = (uint64_t)__ballot(take);
+            if (surv == 0ull) continue;
+            // survivors join the LDS buffer in lane order (npend < 64, at
+            // most 64 join: positions stay below 128)
+            if (take) {
+                const int at = npend + __popcll((unsigned long long)(surv & ((1ull << l) - 1ull)));
+                pend->d[at] = dd;
+                pend->s[at] = cs;
+            }
+            npend += __popcll((unsigned long long)surv);
+            if (npend >= 64) {
+                flush(64);
+                // move the rest (fewer than 64) to the front
+                double rd = 0;
+                uint32_t rs = 0;
+                const int rest = npend - 64;
+                if (l < rest) {
+                    rd = pend->d[64 + l];
+                    rs = pend->s[64 + l];
+                }
+                __syncthreads();
+                if (l < rest) {
+                    pend->d[l] = rd;
+                    pend->s[l] = rs;
+                }
+                npend = rest;
+            }
+        }
+    };
     for (int pass = 0;; pass++) {
 #pragma unroll
         for (int j = 0; j < M; j++) {
@@ -430,7 +499,8 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
         }
         size = 0;
         npend = 0;
-        bool stop = false, odd_l = false;
+        odd_l = false;
+        bool stop = false;
         // the first step takes the cells at Chebyshev distance 0 and 1
         // together (the reference's ring-1 test cannot stop the walk: its
         // bound is 0), then ring by ring
@@ -472,6 +542,8 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
                     kth_entry(kth, kts);
                     prune = fmin(prune, kth);
                 }
+                bool crowded = false;  // a cell with a sub-grid: taken sub-cell by sub-cell below
+                int ii = 0, jj = 0, kk = 0;
                 if (t < ncube) {
                     // t = (dk + reach) side^2 + (dj + reach) side + (di + reach)
                     int q1, q2;
@@ -485,82 +557,75 @@ PG_DEV void knn_wave_query(const double* __restrict__ sorted, const uint32_t* __
                     const int di = t - q1 * side - reach, dj = q1 - q2 * side - reach;
                     const int dk = DIM == 3 ? q2 - reach : 0;
                     const int cheb = max(abs(di), max(abs(dj), abs(dk)));
-                    const int ii = ci + di, jj = cj + dj, kk = ck + dk;
+                    ii = ci + di;
+                    jj = cj + dj;
+                    kk = ck + dk;
                     const bool cell_in = (cheb == ring || ring == 0) && ii >= 0 && jj >= 0 && kk >= 0 && ii < N &&
                                          jj < N && (DIM == 2 || kk < N);
                     if (cell_in && cell_lower_d2<DIM>(P, ii, jj, kk, N) <= prune) {
                         const int64_t cc = cbase + ii + jj * N + (int64_t)kk * N * N;
-                        st = starts[cc];
-                        cnt = counts[cc];
+                        if (SUB && sg.m[cc] > 0) {
+                            crowded = true;
+                        } else {
+                            st = starts[cc];
+                            cnt = counts[cc];
+                        }
                     }
                 }
                 const int incl = wave_incl_sum(cnt);
                 const int total = (int)read_lane((uint32_t)incl, 63);
-                if (total == 0) continue;
-                c_start[l] = st;
-                c_pfx[l + 1] = incl;
-                if (l == 0) c_pfx[0] = 0;
-                __syncthreads();
-                for (int cb = 0; cb < total; cb += 64) {
-                    const int j = cb + l;
-                    bool take = false;
-                    double dd = 0;
-                    uint32_t cs = 0;
-                    if (j < total) {
-                        // cell holding candidate j: largest a with c_pfx[a] <= j
-                        int lo = 0, hi = 63;
-                        while (lo < hi) {
-                            const int mid = (lo + hi + 1) >> 1;
-                            if (c_pfx[mid] <= j) lo = mid;
-                            else hi = mid - 1;
-                        }
-                        cs = (uint32_t)(c_start[lo] + (j - c_pfx[lo]));
-                        if (cs != q) {
-                            double pc[3] = {0, 0, 0};
-#pragma unroll
-                            for (int d = 0; d < DIM; d++) pc[d] = sorted[(size_t)DIM * cs + d];
-                            dd = dist2<DIM>(p, pc);
-                            take = dd <= r2 && (kp == 0 || ent_less(perm, lo_d, lo_s, dd, cs));
-                        }
-                    }
+                if (total > 0) {
+                    c_start[l] = st;
+                    c_pfx[l + 1] = incl;
+                    if (l == 0) c_pfx[0] = 0;
+                    __syncthreads();
+                    scan(total, nullptr, md2);
+                    __syncthreads();
+                }
+                // the crowded cells of this batch, one at a time: their
+                // sub-cells that the pruning distance can reach
+                uint64_t crowd = SUB ? (uint64_t)__ballot(crowded) : 0ull;
+                while (SUB && crowd != 0ull) {
+                    const int a = __builtin_ctzll(crowd);
+                    crowd &= crowd - 1ull;
+                    const int ix[3] = {(int)read_lane((uint32_t)ii, a), (int)read_lane((uint32_t)jj, a),
+                                       (int)read_lane((uint32_t)kk, a)};
+                    const int64_t cc = cbase + ix[0] + ix[1] * N + (int64_t)ix[2] * N * N;
+                    const int m = sg.m[cc];
+                    const int64_t sbase = sg.ptr[cc];
+                    double pd = r2;
                     if (size == k) {
                         double kth;
                         uint32_t kts;
                         kth_entry(kth, kts);
-                        take = take && ent_less(perm, dd, cs, kth, kts);
+                        pd = fmin(pd, kth);
                     }
-                    // a kept candidate nearer than its ring's bound (rounding
-                    // at a ring's edge): the ball's argument fails
-                    odd_l = odd_l || (take && dd < md2);
-                    const uint64_t surv = (uint64_t)__ballot(take);
-                    if (surv == 0ull) continue;
-                    // survivors join the LDS buffer in lane order (npend < 64,
-                    // at most 64 join: positions stay below 128)
-                    if (take) {
-                        const int at = npend + __popcll((unsigned long long)(surv & ((1ull << l) - 1ull)));
-                        pend->d[at] = dd;
-                        pend->s[at] = cs;
-                    }
-                    npend += __popcll((unsigned long long)surv);
-                    if (npend >= 64) {
-                        flush(64);
-                        // move the rest (fewer than 64) to the front
-                        double rd = 0;
-                        uint32_t rs = 0;
-                        const int rest = npend - 64;
-                        if (l < rest) {
-                            rd = pend->d[64 + l];
-                            rs = pend->s[64 + l];
+                    int lo[3], hi[3];
+                    sub_box<DIM>(P, ix, m, N, pd, lo, hi);
+                    const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+                    const int nbox = nx * ny * nz;
+                    for (int b0 = 0; b0 < nbox; b0 += 64) {
+                        const int u = b0 + l;
+                        int sst = 0, scnt = 0;
+                        if (u < nbox) {
+                            const int sx[3] = {lo[0] + u % nx, lo[1] + (u / nx) % ny, lo[2] + u / (nx * ny)};
+                            if (sub_lower_d2<DIM>(P, ix, sx, m, N) <= pd) {
+                                const int64_t j = sbase + sx[0] + (int64_t)m * (sx[1] + (int64_t)m * sx[2]);
+                                sst = sg.start[j];
+                                scnt = sg.cnt[j];
+                            }
                         }
+                        const int sincl = wave_incl_sum(scnt);
+                        const int stotal = (int)read_lane((uint32_t)sincl, 63);
+                        if (stotal == 0) continue;
+                        c_start[l] = sst;
+                        c_pfx[l + 1] = sincl;
+                        if (l == 0) c_pfx[0] = 0;
                         __syncthreads();
-                        if (l < rest) {
-                            pend->d[l] = rd;
-                            pend->s[l] = rs;
-                        }
-                        npend = rest;
+                        scan(stotal, sg.idx, md2);
+                        __syncthreads();
                     }
                 }
-                __syncthreads();
             }
         }
         if (r2 < __builtin_inf() && __ballot(odd_l) != 0ull) {  // walk again, no ball

@@ -160,7 +160,7 @@ void launch_voronoi(int dim, const double* sorted, int64_t n, const uint32_t* se
 void launch_voronoi_ball(int dim, const double* sorted, int64_t n, const uint32_t* seeds, int64_t nq,
                          const uint32_t* knn, int k, int max_p, int max_t, FaceArrays fa, int N,
                          const int32_t* starts, const int32_t* counts, const uint32_t* perm, int kmark,
-                         hipStream_t s) {
+                         hipStream_t s, SubGrid sg) {
     if (nq <= 0) return;
     BallGrid bg;
     bg.N = N;
@@ -168,6 +168,7 @@ void launch_voronoi_ball(int dim, const double* sorted, int64_t n, const uint32_
     bg.counts = counts;
     bg.perm = perm;
     bg.kmark = kmark;
+    bg.sg = sg;
     const int64_t nb = (nq + 3) / 4;  // 4 seeds per wave
     if (dim == 2)
         k_voronoi_ball<2><<<dim3((unsigned)nb), dim3(64), 0, s>>>(sorted, n, seeds, nq, knn, k, max_p, max_t, fa, bg);

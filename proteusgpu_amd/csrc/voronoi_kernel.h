@@ -33,6 +33,7 @@
 #include "exact.h"
 #include "kernels.h"
 #include "simt.h"
+#include "subgrid.h"
 
 namespace pg {
 
@@ -287,6 +288,7 @@ struct BallGrid {
     const uint32_t* perm = nullptr;   // input index per sorted id (ties)
     int kmark = 0;                    // kcert of a cell the completion certifies
     int32_t* scr = nullptr;           // LDS, 2 G + 1 ints per seed group (the group search's tables)
+    SubGrid sg;                       // sub-grids of the crowded cells (none: sg.m == nullptr)
 };
 
 // Lower bound of the squared distance from v to any point of grid cell ix:
@@ -317,12 +319,16 @@ PG_DEV double ball_cell_lb2(int N, const double (&v)[3], const int (&ix)[3]) {
 // group's best is reduced at the end of every ring.  With the bounds of
 // ball_cell_lb2 the answer is the argmin over all points, and the ring at
 // which the walk stops depends only on the rings before it: the order the
-// points are visited in does not matter.  v, r2, sid: the same on every lane
-// of the group, every lane of which is active; cst / cpf: the group's LDS
+// points are visited in does not matter.  A crowded cell (one with a
+// sub-grid) is taken sub-cell by sub-cell, only the sub-cells the ball of
+// the best so far reaches (sub_lower_d2 bounds them the same way).  v, r2,
+// sid: the same on every lane of the group, every lane of which is active;
+// shift / gmask: the group's lanes in a ballot; cst / cpf: the group's LDS
 // tables (G and G + 1 entries).
 template <int DIM, int G>
 PG_DEV int32_t ball_search_group(const BallGrid& bg, const double* __restrict__ pts, const double (&v)[3], double r2,
-                                 uint32_t sid, const int32_t* pvid, int nb_v, int gl, int32_t* cst, int32_t* cpf) {
+                                 uint32_t sid, const int32_t* pvid, int nb_v, int gl, int shift, uint64_t gmask,
+                                 int32_t* cst, int32_t* cpf) {
     const int N = bg.N;
     const double h = 1.0 / N;
     int cc[3] = {0, 0, 0};
@@ -342,27 +348,10 @@ PG_DEV int32_t ball_search_group(const BallGrid& bg, const double* __restrict__ 
         const int ncube = DIM == 3 ? side * side * side : side * side;
         double lbest = best;
         int32_t lpick = pick;
-        for (int base = 0; base < ncube; base += G) {
-            const int t = base + gl;
-            int32_t st = 0, cnt = 0;
-            if (t < ncube) {
-                const int di = t % side - ring, dj = (t / side) % side - ring;
-                const int dk = DIM == 3 ? t / (side * side) - ring : 0;
-                const int cheb = max(abs(di), max(abs(dj), abs(dk)));
-                const int ix[3] = {cc[0] + di, cc[1] + dj, cc[2] + dk};
-                if (cheb == ring && ix[0] >= 0 && ix[1] >= 0 && ix[2] >= 0 && ix[0] < N && ix[1] < N &&
-                    (DIM == 2 || ix[2] < N) && ball_cell_lb2<DIM>(N, v, ix) < best) {
-                    const int64_t cell = ix[0] + (int64_t)ix[1] * N + (int64_t)ix[2] * N * N;
-                    st = bg.starts[cell];
-                    cnt = bg.counts[cell];
-                }
-            }
-            const int incl = group_incl_sum<G>(cnt, gl);
-            const int total = group_last<G>(incl);
-            if (total == 0) continue;
-            cst[gl] = st;
-            cpf[gl + 1] = incl;
-            if (gl == 0) cpf[0] = 0;
+        // points 0 .. total-1 of the cells (or sub-cells) in cst / cpf
+        // (first position, exclusive prefix), G at a time; positions are
+        // sorted ids, or indices into indir
+        auto scan = [&](int total, const uint32_t* __restrict__ indir) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
             for (int cb = 0; cb < total; cb += G) {
@@ -375,7 +364,8 @@ PG_DEV int32_t ball_search_group(const BallGrid& bg, const double* __restrict__ 
                     if (cpf[mid] <= j) lo = mid;
                     else hi = mid - 1;
                 }
-                const int32_t q = cst[lo] + (j - cpf[lo]);
+                int32_t q = cst[lo] + (j - cpf[lo]);
+                if (indir != nullptr) q = (int32_t)indir[q];
                 if ((uint32_t)q == sid) continue;
                 double d2 = 0.0;
 #pragma unroll
@@ -393,6 +383,73 @@ PG_DEV int32_t ball_search_group(const BallGrid& bg, const double* __restrict__ 
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
+        };
+        for (int base = 0; base < ncube; base += G) {
+            const int t = base + gl;
+            int32_t st = 0, cnt = 0;
+            bool crowded = false;  // a cell with a sub-grid: taken sub-cell by sub-cell below
+            int ix[3] = {0, 0, 0};
+            if (t < ncube) {
+                const int di = t % side - ring, dj = (t / side) % side - ring;
+                const int dk = DIM == 3 ? t / (side * side) - ring : 0;
+                const int cheb = max(abs(di), max(abs(dj), abs(dk)));
+                ix[0] = cc[0] + di;
+                ix[1] = cc[1] + dj;
+                ix[2] = cc[2] + dk;
+                if (cheb == ring && ix[0] >= 0 && ix[1] >= 0 && ix[2] >= 0 && ix[0] < N && ix[1] < N &&
+                    (DIM == 2 || ix[2] < N) && ball_cell_lb2<DIM>(N, v, ix) < best) {
+                    const int64_t cell = ix[0] + (int64_t)ix[1] * N + (int64_t)ix[2] * N * N;
+                    if (bg.sg.m != nullptr && bg.sg.m[cell] > 0) {
+                        crowded = true;
+                    } else {
+                        st = bg.starts[cell];
+                        cnt = bg.counts[cell];
+                    }
+                }
+            }
+            const int incl = group_incl_sum<G>(cnt, gl);
+            const int total = group_last<G>(incl);
+            if (total > 0) {
+                cst[gl] = st;
+                cpf[gl + 1] = incl;
+                if (gl == 0) cpf[0] = 0;
+                scan(total, nullptr);
+            }
+            // the crowded cells of this batch, one at a time: their
+            // sub-cells that the ball of the ring-start best can reach
+            uint64_t crowd = (wave_ballot(crowded) >> shift) & gmask;
+            while (crowd != 0ull) {
+                const int a = ffs64(crowd);
+                crowd &= crowd - 1ull;
+                const int cx[3] = {shfl(ix[0], a, G), shfl(ix[1], a, G), shfl(ix[2], a, G)};
+                const int64_t cell = cx[0] + (int64_t)cx[1] * N + (int64_t)cx[2] * N * N;
+                const int m = bg.sg.m[cell];
+                const int64_t sbase = bg.sg.ptr[cell];
+                const double P[3] = {v[0] * N, v[1] * N, DIM == 3 ? v[2] * N : 0.0};
+                int lo[3], hi[3];
+                sub_box<DIM>(P, cx, m, N, best, lo, hi);
+                const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+                const int nbox = nx * ny * nz;
+                for (int b0 = 0; b0 < nbox; b0 += G) {
+                    const int u = b0 + gl;
+                    int32_t sst = 0, scnt = 0;
+                    if (u < nbox) {
+                        const int sx[3] = {lo[0] + u % nx, lo[1] + (u / nx) % ny, lo[2] + u / (nx * ny)};
+                        if (sub_lower_d2<DIM>(P, cx, sx, m, N) < best) {
+                            const int64_t j = sbase + sx[0] + (int64_t)m * (sx[1] + (int64_t)m * sx[2]);
+                            sst = bg.sg.start[j];
+                            scnt = bg.sg.cnt[j];
+                        }
+                    }
+                    const int sincl = group_incl_sum<G>(scnt, gl);
+                    const int stotal = group_last<G>(sincl);
+                    if (stotal == 0) continue;
+                    cst[gl] = sst;
+                    cpf[gl + 1] = sincl;
+                    if (gl == 0) cpf[0] = 0;
+                    scan(stotal, bg.sg.idx);
+                }
+            }
         }
         // the group's best after this ring
         best = lbest;
@@ -985,7 +1042,8 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
                     m &= m - 1ull;
                     const double vv[3] = {shfl(R.px[s], src, G), shfl(R.py[s], src, G), shfl(R.pz[s], src, G)};
                     const double r2v = shfl(R.d2[s], src, G);
-                    const int32_t cq = ball_search_group<D, G>(bg, pts, vv, r2v, sid, sm.pvid, nb_v, gl, cst, cpf);
+                    const int32_t cq =
+                        ball_search_group<D, G>(bg, pts, vv, r2v, sid, sm.pvid, nb_v, gl, shift, gmask, cst, cpf);
                     if (gl == src) cand[s] = cq;
                 }
                 if (need) {

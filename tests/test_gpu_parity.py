@@ -42,8 +42,9 @@ def test_knn_bitexact(ctx, dim, n, k, seed):
 def test_knn_clustered_bitexact(ctx, dim, n, k):
     """Clustered seeds (dense clumps, sparse space between them): the wave
     query's first ball, guessed from the density around the query, is often
-    far too small or far too large there.  Rows bit-exact against the
-    oracle's ring search."""
+    far too small or far too large there, and the clumps' grid cells hold
+    hundreds of points each (searched through their sub-grids).  Rows
+    bit-exact against the oracle's ring search."""
     print(f"knn clustered dim={dim} n={n} k={k}", flush=True)
     pts = np.ascontiguousarray(gaussian_mixture(n, 8, 0.003, 0.03, 0.3, 40 + k)[:, :dim])
     kp = api.KnnProblem(ctx, pts)
