@@ -55,6 +55,18 @@ PG_DEV double det4(double a11, double a12, double a13, double a14, double a21, d
 PG_DEV double smin(double a, double b) { return (b < a) ? b : a; }
 PG_DEV double smax(double a, double b) { return (a < b) ? b : a; }
 
+// Workgroups of a launch are dealt to the 8 XCDs round-robin (block b on
+// XCD b % 8), each XCD with its own L2.  Renumbering them so that XCD x takes
+// a contiguous share of the logical blocks keeps the work of neighbouring
+// blocks (neighbouring seeds, sharing neighbour points) in one L2.  A
+// bijection of [0, nb) for any nb; only locality depends on it.
+PG_DEV int64_t xcd_block(int64_t b, int64_t nb) {
+    constexpr int64_t X = 8;
+    const int64_t per = nb / X, rem = nb % X;
+    const int64_t x = b % X, i = b / X;
+    return x * per + (x < rem ? x : rem) + i;
+}
+
 // knn.cpp:208-222
 template <int DIM>
 PG_DEV int grid_cell(const double* p, int N) {

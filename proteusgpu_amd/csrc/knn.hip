@@ -46,7 +46,7 @@ __global__ void __launch_bounds__(64, knn_wave_occupancy<M>())
                KnnTail tail, SubGrid sg) {
     __shared__ int32_t c_start[64], c_pfx[65];
     __shared__ KnnPend pend;
-    const int64_t qi = blockIdx.x;
+    const int64_t qi = xcd_block(blockIdx.x, gridDim.x);
     if (qi >= nq) return;  // whole block
     const uint32_t q = queries ? queries[qi] : (uint32_t)qi;
     int kp = 0;
@@ -107,7 +107,9 @@ void launch_knn(int dim, const double* sorted, const uint32_t* perm, int64_t n, 
         const char* e = getenv("PGPU_KNN_WAVE_MIN_K");
         return e ? atoi(e) : 20;
     }();
-    if (k >= wave_min_k || k > 256) {  // the LDS heap holds at most 256 entries per query
+    // the LDS heap holds at most 256 entries per query, and only the wave
+    // kernel takes crowded cells through their sub-grids
+    if (k >= wave_min_k || k > 256 || sg.m != nullptr) {
         if (dim == 2) dispatch_wave<2>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, sg, s);
         else dispatch_wave<3>(sorted, perm, N, starts, counts, queries, nq, k, out, tail, sg, s);
         return;

@@ -651,25 +651,25 @@ PG_DEV int order_key(uint32_t s, const uint32_t* __restrict__ perm, const int32_
     return min(max(prev[o], 0), 255);
 }
 
-constexpr int OTILE = 4096;  // list positions per tile of the clip order
+constexpr int OTILE = 4096;  // list positions per tile of the clip order, at most (LDS)
 
-// One tile of OTILE consecutive list positions per block, reordered in
+// One tile of `tile` (at most OTILE) consecutive list positions per block, reordered in
 // place-of-tile (into out) by key, stably: key histogram in LDS, its
 // exclusive scan, then chunks of BTB entries ranked by ballots (lane order
 // within a wave, wave order within the chunk, chunk order within the tile).
 __global__ void __launch_bounds__(BTB)
     k_order_tiles(const uint32_t* __restrict__ list, const int64_t* __restrict__ n_dev, int64_t n_host,
                   const uint32_t* __restrict__ perm, const int32_t* __restrict__ id_map,
-                  const int32_t* __restrict__ prev, int64_t prev_n, uint32_t* __restrict__ out) {
+                  const int32_t* __restrict__ prev, int64_t prev_n, uint32_t* __restrict__ out, int tile) {
     static_assert(BTB == NKEY, "one key counter per thread");
     __shared__ uint8_t key_s[OTILE];
     __shared__ int32_t base[NKEY];
     __shared__ int32_t wc[BTB / 64][NKEY];
     __shared__ int32_t wt[BTB / 64];
     const int64_t n = seed_count(n_dev, n_host);
-    const int64_t t0 = (int64_t)blockIdx.x * OTILE;
+    const int64_t t0 = (int64_t)blockIdx.x * tile;
     if (t0 >= n) return;  // whole block
-    const int len = (int)min<int64_t>(OTILE, n - t0);
+    const int len = (int)min<int64_t>(tile, n - t0);
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
     base[tid] = 0;
     __syncthreads();
@@ -740,8 +740,17 @@ __global__ void k_keep_kcert(const int64_t* __restrict__ n_dev, int64_t n_host, 
 void launch_order_list(const uint32_t* list, const int64_t* n_dev, int64_t n_cap, const uint32_t* perm,
                        const int32_t* id_map, const int32_t* prev, int64_t prev_n, uint32_t* out, hipStream_t s) {
     if (n_cap <= 0) return;
-    const unsigned nb = (unsigned)((n_cap + OTILE - 1) / OTILE);
-    k_order_tiles<<<nb, BTB, 0, s>>>(list, n_dev, n_cap, perm, id_map, prev, prev_n, out);
+    // the tile can be shortened with PGPU_LK_OTILE for tuning runs
+    // (256 positions: the face records the clip writes then stay within a
+    // few hundred consecutive ids per wave, which halves its write traffic
+    // against 4096 at the same step time)
+    static const int tile = [] {
+        const char* e = getenv("PGPU_LK_OTILE");
+        const int t = e ? atoi(e) : 256;
+        return t < 64 ? 64 : (t > OTILE ? OTILE : t);
+    }();
+    const unsigned nb = (unsigned)((n_cap + tile - 1) / tile);
+    k_order_tiles<<<nb, BTB, 0, s>>>(list, n_dev, n_cap, perm, id_map, prev, prev_n, out, tile);
 }
 
 void launch_keep_kcert(const int64_t* n_dev, int64_t n_cap, const uint32_t* perm, const int32_t* id_map,

@@ -18,7 +18,8 @@ __global__ void __launch_bounds__(64, (VoroCfg<DIM, G, S>::WAVES)) k_voronoi(con
                                                 int max_t, FaceArrays fa) {
     using C = VoroCfg<DIM, G, S>;
     __shared__ SeedSmem<C::NS, C::MAXP> sm[C::SPW];
-    voronoi_group<DIM, G, S>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm, blockIdx.x);
+    voronoi_group<DIM, G, S>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm,
+                             xcd_block(blockIdx.x, gridDim.x));
 }
 
 // The same kernel under its own name for certificate restarts, so that
@@ -30,7 +31,8 @@ __global__ void __launch_bounds__(64, (VoroCfg<DIM, G, S>::WAVES)) k_voronoi_res
                                                         int max_t, FaceArrays fa) {
     using C = VoroCfg<DIM, G, S>;
     __shared__ SeedSmem<C::NS, C::MAXP> sm[C::SPW];
-    voronoi_group<DIM, G, S>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm, blockIdx.x);
+    voronoi_group<DIM, G, S>(pts, n_total, seeds, nq, knn, k, max_p, max_t, fa, sm,
+                             xcd_block(blockIdx.x, gridDim.x));
 }
 
 // Cells whose filtered predicate was degenerate, again with the exact

@@ -37,8 +37,8 @@ def test_knn_bitexact(ctx, dim, n, k, seed):
     assert np.array_equal(kp.get_knearest(), kn)
 
 
-@pytest.mark.parametrize("dim,n,k", [(3, 12000, 24), (3, 12000, 64), (3, 12000, 100), (2, 6000, 200),
-                                     (3, 13000, 256), (2, 1000, 250)])
+@pytest.mark.parametrize("dim,n,k", [(3, 12000, 8), (2, 6000, 16), (3, 12000, 24), (3, 12000, 64), (3, 12000, 100),
+                                     (2, 6000, 200), (3, 13000, 256), (2, 1000, 250)])
 def test_knn_clustered_bitexact(ctx, dim, n, k):
     """Clustered seeds (dense clumps, sparse space between them): the wave
     query's first ball, guessed from the density around the query, is often
