@@ -147,6 +147,18 @@ struct Grid {
     void note_crowded(hipStream_t s) {
         if (has_sub) HIPCHK(hipMemcpyAsync(&crowded, big.as<int32_t>(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
     }
+    // The sub-grid for kernels that look at it cell by cell (whatever the
+    // crowded count): empty only when it was not built.
+    pg::SubGrid sub_any() const {
+        pg::SubGrid g;
+        if (!has_sub) return g;
+        g.m = sub_m.as<int32_t>();
+        g.ptr = sub_ptr.as<int32_t>();
+        g.start = sub_start.as<int32_t>();
+        g.cnt = sub_cnt.as<int32_t>();
+        g.idx = sub_idx.as<uint32_t>();
+        return g;
+    }
     // The sub-grid for the searches: empty without crowded cells (the
     // kernels without sub-grid code run), or when the count was not read.
     pg::SubGrid sub() const {
@@ -515,7 +527,7 @@ void mesh_build_once(pg_ctx* c, pg_mesh* m, const double* pts, int64_t n, bool o
         float* feat = m->feat.get<float>(ns * pg::LK_NFEAT);
         int32_t* cls = m->kcls.get<int32_t>(ns);
         pg::launch_kfeatures(dim, m->g.sorted.as<double>(), nseeds, m->g.N, m->g.counts.as<int32_t>(),
-                             m->g.starts.as<int32_t>(), nullptr, feat, s);
+                             m->g.starts.as<int32_t>(), nullptr, feat, s, m->g.sub_any());
         pg::launch_kpredict(feat, nseeds, km->P.as<float>(), km->a0, nullptr, cls, nullptr, s);
         HIPCHK(hipGetLastError());
         int64_t hc[pg::LK_NCLASS];
@@ -1741,7 +1753,7 @@ int pg_kmodel_train_mesh(pg_ctx* c, pg_kmodel* km, const pg_mesh* mc, int steps,
         float* X = km->X.get<float>((size_t)n * pg::LK_NFEAT);
         int32_t* y = km->y.get<int32_t>((size_t)n);
         pg::launch_kfeatures(m->prm.dim, m->g.sorted.as<double>(), n, m->g.N, m->g.counts.as<int32_t>(),
-                             m->g.starts.as<int32_t>(), nullptr, X, c->stream);
+                             m->g.starts.as<int32_t>(), nullptr, X, c->stream, m->g.sub_any());
         pg::launch_klabels(m->kcert.as<int32_t>(), n, y, c->stream);
         HIPCHK(hipGetLastError());
         const uint64_t sd = seed ? seed : 1;
@@ -1769,7 +1781,7 @@ int pg_mesh_features(pg_ctx* c, const pg_mesh* mc, float* out) {
         DBuf tmp;
         float* d = tmp.get<float>((size_t)n * pg::LK_NFEAT);
         pg::launch_kfeatures(m->prm.dim, m->g.sorted.as<double>(), n, m->g.N, m->g.counts.as<int32_t>(),
-                             m->g.starts.as<int32_t>(), m->g.perm.as<uint32_t>(), d, c->stream);
+                             m->g.starts.as<int32_t>(), m->g.perm.as<uint32_t>(), d, c->stream, m->g.sub_any());
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(out, d, sizeof(float) * n * pg::LK_NFEAT, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));

@@ -174,8 +174,11 @@ int lk_bucket(int c);  // 16, 24, 32, 48, 64, 96
 //   f12     min(distance to the unit box faces in cell widths, 3) / 3
 //   f13     log2(1 + (max - min neighbour count) / mean occupancy)
 //   f14     log2(1 + |sum of count * offset| / (3^dim * mean occupancy))
+// sg: the grid's sub-grids (crowded cells are scanned through them; the
+// features are the same as a full scan's)
 void launch_kfeatures(int dim, const double* sorted, int64_t n, int N, const int32_t* counts,
-                      const int32_t* starts, const uint32_t* out_index, float* out, hipStream_t s);
+                      const int32_t* starts, const uint32_t* out_index, float* out, hipStream_t s,
+                      SubGrid sg = {});
 // cls[s] = bucket with the least expected cost of its restart chain (a pass
 // at row length k costs k + r m, m = clip steps, r = a0); logits
 // (n x 6, optional) receives the network outputs.

@@ -17,6 +17,7 @@
 
 #include "common.h"
 #include "kernels.h"
+#include "subgrid.h"
 
 namespace pg {
 namespace {
@@ -63,7 +64,8 @@ PG_DEV f4 zero4() { return f4{0.f, 0.f, 0.f, 0.f}; }
 // cbase + i + j N + k N^2; see the feature list in kernels.h.
 template <int DIM>
 PG_DEV void kfeatures_one(const double* __restrict__ sorted, int64_t s, int64_t n, int N, int64_t cbase,
-                          const int32_t* __restrict__ cnt, const int32_t* __restrict__ start, float (&f)[16]) {
+                          const int32_t* __restrict__ cnt, const int32_t* __restrict__ start, float (&f)[16],
+                          SubGrid sg = {}) {
     double p[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < DIM; d++) p[d] = sorted[(size_t)DIM * s + d];
@@ -74,7 +76,20 @@ PG_DEV void kfeatures_one(const double* __restrict__ sorted, int64_t s, int64_t 
     double best[8];  // the 8 smallest squared distances, ascending (+inf: none)
 #pragma unroll
     for (int j = 0; j < 8; j++) best[j] = __builtin_inf();
-    int nb = 0;
+    // keep d2 if it is among the 8 smallest so far: insertion without
+    // branches, new best[j] = max(best[j-1], min(best[j], d2)), top down
+    auto keep = [&](double d2) {
+#pragma unroll
+        for (int j = 7; j > 0; j--) best[j] = fmax(best[j - 1], fmin(best[j], d2));
+        best[0] = fmin(best[0], d2);
+    };
+    auto keep_point = [&](int64_t q) {
+        double pq[3] = {0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < DIM; d++) pq[d] = sorted[(size_t)DIM * q + d];
+        keep(dist2<DIM>(p, pq));
+    };
+    uint32_t crowded_cells = 0;  // bit: cube cell (di, dj, dk) has a sub-grid
     double csum[4] = {0, 0, 0, 0};
     int cin[4] = {0, 0, 0, 0};
     int inside = 0;
@@ -100,21 +115,66 @@ PG_DEV void kfeatures_one(const double* __restrict__ sorted, int64_t s, int64_t 
                     nmin = min(nmin, ct);
                     nmax = max(nmax, ct);
                 }
-                const int32_t b = start[cc];
-                for (int32_t q = b; q < b + ct; q++) {
-                    if (q == s) continue;
-                    double pq[3] = {0, 0, 0};
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) pq[d] = sorted[(size_t)DIM * q + d];
-                    const double d2 = dist2<DIM>(p, pq);
-                    nb++;
-                    // insertion without branches: new best[j] =
-                    // max(best[j-1], min(best[j], d2)), from the top down
-#pragma unroll
-                    for (int j = 7; j > 0; j--) best[j] = fmax(best[j - 1], fmin(best[j], d2));
-                    best[0] = fmin(best[0], d2);
+                if (sg.m != nullptr && sg.m[cc] > 0) {
+                    // a crowded cell: its sub-cells below, pruned by the 8th
+                    // distance (the same 8 smallest values as a full scan)
+                    crowded_cells |= 1u << ((dk + 1) * 9 + (dj + 1) * 3 + (di + 1));
+                    continue;
                 }
+                const int32_t b = start[cc];
+                for (int32_t q = b; q < b + ct; q++)
+                    if (q != s) keep_point(q);
             }
+    // crowded cells, the own one first: the sub-cells around p, then those
+    // of each cell whose lower bound is below the current 8th distance (a
+    // sub-cell whose bound is at least it holds no smaller distance)
+    if (crowded_cells != 0u) {
+        double P[3] = {0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < DIM; d++) P[d] = p[d] * (double)N;
+        const int own = 13;  // (0, 0, 0) in the cube's bit numbering (2D: dk = 0)
+        for (int pass = 0; pass < 2; pass++) {
+            uint32_t todo = crowded_cells;
+            while (todo != 0u) {
+                const int bit = (pass == 0 && ((todo >> own) & 1u)) ? own : __builtin_ctz(todo);
+                todo &= ~(1u << bit);
+                if (pass == 0 && bit != own) continue;
+                const int dk = DIM == 3 ? bit / 9 - 1 : 0, dj = (bit / 3) % 3 - 1, di = bit % 3 - 1;
+                const int ix[3] = {ci + di, cj + dj, ck + dk};
+                const int64_t cc = cbase + ix[0] + ix[1] * N + (int64_t)ix[2] * N * N;
+                const int m = sg.m[cc];
+                const int64_t base = sg.ptr[cc];
+                // pass 0: the own cell's sub-cells next to p (a first 8th
+                // distance); pass 1: every crowded cell, pruned
+                int lo[3], hi[3];
+                sub_box<DIM>(P, ix, m, N, pass == 0 ? 0.0 : best[7], lo, hi);
+                for (int c3 = lo[2]; c3 <= hi[2]; c3++)
+                    for (int b3 = lo[1]; b3 <= hi[1]; b3++)
+                        for (int a3 = lo[0]; a3 <= hi[0]; a3++) {
+                            const int sx[3] = {a3, b3, c3};
+                            const bool near = pass == 0;
+                            if (!near && sub_lower_d2<DIM>(P, ix, sx, m, N) >= best[7]) continue;
+                            if (pass == 1 && bit == own) {
+                                // already taken in pass 0: the sub-cells of
+                                // the pass-0 box
+                                int l0[3], h0[3];
+                                sub_box<DIM>(P, ix, m, N, 0.0, l0, h0);
+                                if (a3 >= l0[0] && a3 <= h0[0] && b3 >= l0[1] && b3 <= h0[1] && c3 >= l0[2] &&
+                                    c3 <= h0[2])
+                                    continue;
+                            }
+                            const int64_t j = base + a3 + (int64_t)m * (b3 + (int64_t)m * c3);
+                            const int32_t b = sg.start[j], e = b + sg.cnt[j];
+                            for (int32_t t = b; t < e; t++) {
+                                const uint32_t q = sg.idx[t];
+                                if ((int64_t)q != s) keep_point(q);
+                            }
+                        }
+            }
+        }
+    }
+    // points of the cube other than the seed (a full scan's count)
+    const int nb = (int)total - 1;
     const int cube = DIM == 3 ? 27 : 9;
 #pragma unroll
     for (int m = 0; m < 4; m++) f[m] = cin[m] ? (float)log2(1.0 + csum[m] / cin[m] / lam) : 0.0f;
@@ -156,11 +216,11 @@ template <int DIM>
 __global__ void __launch_bounds__(256)
     k_kfeatures(const double* __restrict__ sorted, int64_t n, int N, const int32_t* __restrict__ cnt,
                 const int32_t* __restrict__ start, const uint32_t* __restrict__ out_index,
-                float* __restrict__ out) {
+                float* __restrict__ out, SubGrid sg) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     float f[16];
-    kfeatures_one<DIM>(sorted, s, n, N, 0, cnt, start, f);
+    kfeatures_one<DIM>(sorted, s, n, N, 0, cnt, start, f, sg);
     store_features(out, out_index ? (int64_t)out_index[s] : s, f);
 }
 
@@ -765,11 +825,11 @@ int lk_bucket(int c) {
 }
 
 void launch_kfeatures(int dim, const double* sorted, int64_t n, int N, const int32_t* counts,
-                      const int32_t* starts, const uint32_t* out_index, float* out, hipStream_t s) {
+                      const int32_t* starts, const uint32_t* out_index, float* out, hipStream_t s, SubGrid sg) {
     if (n <= 0) return;
     const unsigned nb = (unsigned)((n + 255) / 256);
-    if (dim == 2) k_kfeatures<2><<<nb, 256, 0, s>>>(sorted, n, N, counts, starts, out_index, out);
-    else k_kfeatures<3><<<nb, 256, 0, s>>>(sorted, n, N, counts, starts, out_index, out);
+    if (dim == 2) k_kfeatures<2><<<nb, 256, 0, s>>>(sorted, n, N, counts, starts, out_index, out, sg);
+    else k_kfeatures<3><<<nb, 256, 0, s>>>(sorted, n, N, counts, starts, out_index, out, sg);
 }
 
 void launch_kpredict(const float* X, int64_t n, const float* params, float a0, const uint32_t* out_index,

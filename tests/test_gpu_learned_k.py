@@ -26,7 +26,7 @@ import pytest
 from oracle import learned_k as L
 from oracle import oracle as O
 from proteusgpu_amd import api
-from proteusgpu_amd.generators import uniform_points
+from proteusgpu_amd.generators import gaussian_mixture, uniform_points
 
 pytestmark = pytest.mark.gpu
 
@@ -36,10 +36,17 @@ def rand_params(seed, scale=0.3):
     return (rng.standard_normal(L.NPARAM) * scale).astype(np.float32)
 
 
-@pytest.mark.parametrize("dim,n,boundary", [(3, 20000, "reflective"), (2, 10000, "periodic")])
-def test_features_match_oracle(ctx, dim, n, boundary):
-    print(f"features dim={dim} n={n} {boundary}", flush=True)
-    pts = uniform_points(n, dim, 41)
+@pytest.mark.parametrize("dim,n,boundary,kind", [(3, 20000, "reflective", "uniform"), (2, 10000, "periodic", "uniform"),
+                                              (3, 20000, "reflective", "clustered"),
+                                              (2, 10000, "periodic", "clustered")])
+def test_features_match_oracle(ctx, dim, n, boundary, kind):
+    """Clustered: the crowded grid cells' points are scanned through their
+    sub-grids (pruned by the 8th distance), the oracle scans them all."""
+    print(f"features dim={dim} n={n} {boundary} {kind}", flush=True)
+    if kind == "uniform":
+        pts = uniform_points(n, dim, 41)
+    else:
+        pts = np.ascontiguousarray(gaussian_mixture(n, 8, 0.003, 0.03, 0.3, 43)[:, :dim])
     m = api.Mesh(ctx, pts, boundary=boundary, buff=0.12 if boundary == "periodic" else 0.04, k=32)
     Fg = m.features()
     ref = O.Mesh(pts, boundary, buff=0.12 if boundary == "periodic" else 0.04, k=32, restart=True)
