@@ -396,7 +396,8 @@ typedef struct {
     int clip_slots;      /* vertex slots per seed of the fast clip kernel: 48 or 64 */
     int knn_wave_min_k;  /* rows at least this long use the wave-per-query KNN      */
     int blocks_per_cu;   /* resident blocks per CU of the persistent KNN and clip
-                          * kernels, at most their occupancy (default 64: all)    */
+                          * kernels (KNN, 2D clip, restart clips), at most their
+                          * occupancy (default 64: all)                          */
     int use_graph;       /* 1: capture the step in a hipGraph, 0: plain launches     */
 } pg_batch_tuning;
 
