@@ -178,17 +178,40 @@ PG_DEV void plane_bounds(const Plane& p, double wm, float& n1, float& mx) {
     mx = f2f_up(m);
 }
 
-// Per-seed register state of one lane.
+// Per-seed register state of one lane.  Per slot: q bounds the product of
+// the vertex's plane 1-norms; e5 is the vertex's share of the reference's
+// eps, an upper bound (float, at least the smallest normal float) of
+// 1.0000001e-14 m^5 (m^4 in 2D), m the largest normal component among its
+// planes, made once with the vertex instead of at every conflict test.
 template <int S>
 struct SlotRegs {
     double px[S], py[S], pz[S], pw[S], d2[S];
-    float q[S], m[S];
+    float q[S], e5[S];
     uint32_t live;  // bit s = slot s holds a vertex
 };
+
+// 1.0000001e-14 M^5 (M^4 in 2D), the bound of the reference's conflict eps
+// (cell.cpp:175-221: 1e-14 times a product of five, or four, component
+// maxima each at most M), computed in double.  Rounding of the products is
+// monotone, so the bound of max(a, b) is the max of the bounds.
+template <int DIM>
+PG_DEV double eps_bound(double M) {
+    const double M2 = M * M;
+    return DIM == 3 ? 1.0000001e-14 * M2 * M2 * M : 1.0000001e-14 * M2 * M2;
+}
 
 template <int DIM, int S>
 PG_DEV void slot_write(SlotRegs<S>& r, int s, double px, double py, double pz, double pw, double d2,
                        float q, float m) {
+    // In float: the constant, 1.000001e-14 as a float (relative error below
+    // 6e-8), and at most four products, each within 2^-24 of the exact one,
+    // stay above 1.0000001e-14 m^5.  When that bound is a normal float every
+    // intermediate is normal too (m^k >= m^5 / 1e-14 for m < 1); a result
+    // below the smallest normal float, subnormal or flushed to zero, is
+    // raised to it by the integer max on its bit pattern (non-negative).
+    const float m2 = m * m;
+    const float mp = DIM == 3 ? m2 * m2 * m : m2 * m2;
+    const float e5 = __uint_as_float(max(__float_as_uint(mp * 1.000001e-14f), 0x00800000u));
 #pragma unroll
     for (int i = 0; i < S; i++) {
         if (i == s) {
@@ -198,7 +221,7 @@ PG_DEV void slot_write(SlotRegs<S>& r, int s, double px, double py, double pz, d
             r.pw[i] = pw;
             r.d2[i] = d2;
             r.q[i] = q;
-            r.m[i] = m;
+            r.e5[i] = e5;
         }
     }
     r.live |= 1u << s;
@@ -574,7 +597,7 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
 #pragma unroll
     for (int s = 0; s < S; s++) {
         R.px[s] = R.py[s] = R.pz[s] = R.pw[s] = R.d2[s] = 0.0;
-        R.q[s] = R.m[s] = 0.f;
+        R.q[s] = R.e5[s] = 0.f;
     }
     // resumed cell: vertex numerics from the saved planes, with the same
     // arithmetic (and operand order) as when the vertices were made
@@ -681,6 +704,13 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
                 const double en1 = (fabs(e.x) + fabs(e.y) + fabs(e.z) + ewm) * (1.0 + 1e-15);
                 const double emx = D == 2 ? fmax(fmax(fabs(e.x), fabs(e.y)), fabs(e.w))
                                           : fmax(fmax(fabs(e.x), fabs(e.y)), fabs(e.z));
+                // per plane: the eps bound of its own components, and the
+                // rounding bound's factor (exact mode: below XTOL en1 q the
+                // rounding of the planes themselves may flip the sign, which
+                // the reference's eps does not bound; such tests go to the
+                // exact predicate)
+                const uint32_t e_eps = __float_as_uint(fmaxf(f2f_up(eps_bound<D>(emx)), 1.17549435e-38f));
+                const double c1 = (5.4e-15 + (xmode ? XTOL : 0.0)) * en1;
     #pragma unroll
                 for (int s = 0; s < S; s++) {
                     if (!((R.live >> s) & 1u)) continue;
@@ -689,18 +719,15 @@ __device__ void voronoi_group(const double* __restrict__ pts, int64_t n_total, c
                         t = R.pw[s] * fma(e.x, R.px[s], fma(e.y, R.py[s], fma(e.z, R.pz[s], e.w)));
                     else
                         t = R.pw[s] * fma(e.x, R.px[s], fma(e.y, R.py[s], e.w));
-                    double M = fmax((double)R.m[s], emx);
-                    double M2 = M * M;
-                    double epsu = D == 3 ? 1.0000001e-14 * M2 * M2 * M : 1.0000001e-14 * M2 * M2;
-                    // exact mode: below XTOL en1 q the rounding of the planes
-                    // themselves may flip the sign (the reference's eps does not
-                    // bound it): such tests go to the exact predicate
-                    const double xb = xmode ? XTOL * en1 * (double)R.q[s] : 0.0;
-                    double bnd = 5.4e-15 * en1 * (double)R.q[s] + epsu + xb;
+                    // the larger of two non-negative floats by their bit
+                    // patterns (ordered like the values)
+                    const double epsu = (double)__uint_as_float(max(__float_as_uint(R.e5[s]), e_eps));
+                    const double bnd = fma(c1, (double)R.q[s], epsu);
                     bool c;
                     if (fabs(t) > bnd) {
                         c = t > 0.0;
                     } else {
+                        const double xb = xmode ? XTOL * en1 * (double)R.q[s] : 0.0;
                         uint32_t tt = sm.tri[s * G + gl];
                         const Plane p1 = sm.plane[tri_p(tt, 0)];
                         const Plane p2 = sm.plane[tri_p(tt, 1)];
