@@ -24,6 +24,9 @@ namespace {
 
 constexpr int TB = 256;
 inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + TB - 1) / TB); }
+inline unsigned nblk_dt(int64_t n) {
+    return (unsigned)std::max<int64_t>(1, (n + (int64_t)TB * DT_CELLS - 1) / ((int64_t)TB * DT_CELLS));
+}
 
 // ------------------------------------------------------------------ setup
 // Per box: ghosts, seed count, grid resolution (knn.cpp:22, the same
@@ -437,44 +440,57 @@ __global__ void __launch_bounds__(TB)
     }
 }
 
-// Per-box CFL step (finite_volume_solver.cpp:151-186 per box): a wave
-// reduces the lanes of its first lane's box, the others go straight to the
-// atomic (non-negative doubles order as their bits).
+// Per-box CFL step (finite_volume_solver.cpp:151-186 per box).  A wave
+// takes 64 x DT_CELLS consecutive cells; a lane keeps the minimum of its
+// current box (boxes hold consecutive cells) and reports a box it leaves;
+// the lanes holding lane 0's box reduce together, the others report alone,
+// and the waves of a block that end in one box report it once.
 template <int DIM>
 __global__ void __launch_bounds__(TB)
     k_bdt(int64_t n_h, const int32_t* __restrict__ hbox, const int32_t* __restrict__ o2s,
           const double* __restrict__ vol_u, double gam, double cfl, const double* __restrict__ rho,
           const double* __restrict__ vel, const double* __restrict__ E, unsigned long long* dt_box,
           const double* __restrict__ wgen, CflMesh cm) {
-    const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
-    double dti = 1e100;
-    int b = -1;
-    if (i < n_h) {
-        b = hbox[i];
-        Prim s;
-        s.rho = rho[i];
-        s.E = E[i];
-        s.v[2] = 0.0;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) s.v[d] = vel[(size_t)DIM * i + d];
-        const double P = pressure<DIM>(gam, s);
-        const double c = sqrt(gam * P / s.rho);
-        const int32_t si = o2s[i];
-        const double R = cfl_length<DIM>(vol_u[si], si, cm);
-        double va = sqrt(s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]);
-        if (wgen) va = fmax(va, rel_speed<DIM>(s.v, wgen + (size_t)DIM * i));
-        dti = cfl * (R / (c + va));
+    const int64_t base = ((int64_t)blockIdx.x * TB + (threadIdx.x & ~63u)) * DT_CELLS + (threadIdx.x & 63u);
+    double acc = 1e100;
+    int bacc = -1;
+    for (int it = 0; it < DT_CELLS; it++) {
+        const int64_t i = base + 64 * it;
+        if (i >= n_h) break;
+        const int b = hbox[i];
+        const double d = cell_cfl_dt<DIM>(i, o2s, vol_u, gam, cfl, rho, vel, E, wgen, cm);
+        if (b != bacc) {
+            if (bacc >= 0) report_dt(&dt_box[bacc], acc);
+            bacc = b;
+            acc = d;
+        } else {
+            acc = smin(d, acc);
+        }
     }
-    const int b0 = __shfl(b, 0, 64);
-    double m = (b == b0) ? dti : 1e100;
+    const int b0 = __shfl(bacc, 0, 64);
+    double m = (bacc == b0) ? acc : 1e100;
     for (int d = 32; d > 0; d >>= 1) m = smin(__shfl_xor(m, d, 64), m);
-    if (b >= 0 && b != b0) {
-        if (dti >= 0.0) atomic_min_pos_double(&dt_box[b], dti);
-        else atomicMin(&dt_box[b], 0ull);
+    if (bacc >= 0 && bacc != b0) report_dt(&dt_box[bacc], acc);
+    __shared__ double wm[TB / 64];
+    __shared__ int wb[TB / 64];
+    if ((threadIdx.x & 63) == 0) {
+        wm[threadIdx.x >> 6] = m;
+        wb[threadIdx.x >> 6] = b0;
     }
-    if ((threadIdx.x & 63) == 0 && b0 >= 0) {
-        if (m >= 0.0) atomic_min_pos_double(&dt_box[b0], m);
-        else atomicMin(&dt_box[b0], 0ull);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int cb = wb[0];
+        double cmin = wm[0];
+        for (int w = 1; w < TB / 64; w++) {
+            if (wb[w] == cb) {
+                cmin = smin(wm[w], cmin);
+            } else {
+                if (cb >= 0) report_dt(&dt_box[cb], cmin);
+                cb = wb[w];
+                cmin = wm[w];
+            }
+        }
+        if (cb >= 0) report_dt(&dt_box[cb], cmin);
     }
 }
 
@@ -687,10 +703,10 @@ void launch_bdt(int dim, int64_t n_h, int nbox, const int32_t* hbox, const int32
                 hipStream_t s, const double* wgen, CflMesh cm) {
     k_fill<double><<<nblk(nbox), TB, 0, s>>>(dt_box, nbox, 1e100);
     if (dim == 2)
-        k_bdt<2><<<nblk(n_h), TB, 0, s>>>(n_h, hbox, o2s, vol_u, gam, cfl, rho, vel, E, (unsigned long long*)dt_box,
+        k_bdt<2><<<nblk_dt(n_h), TB, 0, s>>>(n_h, hbox, o2s, vol_u, gam, cfl, rho, vel, E, (unsigned long long*)dt_box,
                                           wgen, cm);
     else
-        k_bdt<3><<<nblk(n_h), TB, 0, s>>>(n_h, hbox, o2s, vol_u, gam, cfl, rho, vel, E, (unsigned long long*)dt_box,
+        k_bdt<3><<<nblk_dt(n_h), TB, 0, s>>>(n_h, hbox, o2s, vol_u, gam, cfl, rho, vel, E, (unsigned long long*)dt_box,
                                           wgen, cm);
 }
 

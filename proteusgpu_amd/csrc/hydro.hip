@@ -18,6 +18,7 @@ namespace {
 
 constexpr int TB = 256;
 inline unsigned nblk(int64_t n) { return (unsigned)((n + TB - 1) / TB); }
+inline unsigned nblk_dt(int64_t n) { return (unsigned)((n + (int64_t)TB * DT_CELLS - 1) / ((int64_t)TB * DT_CELLS)); }
 
 // Per sorted cell: hydro id of every face's neighbour (-1 = box plane,
 // which is a reflective wall), unscaled area and cell volume.
@@ -148,28 +149,22 @@ __global__ void __launch_bounds__(TB)
     k_dt_cfl(int64_t n_hydro, const int32_t* __restrict__ o2s, const double* __restrict__ vol_u,
              double gam, double cfl, const double* __restrict__ rho, const double* __restrict__ vel,
              const double* __restrict__ E, unsigned long long* out, const double* __restrict__ wgen, CflMesh cm) {
-    int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
-    double dti = 1e100;
-    if (i < n_hydro) {
-        Prim s;
-        s.rho = rho[i];
-        s.E = E[i];
-        s.v[2] = 0.0;
+    const int64_t base = ((int64_t)blockIdx.x * TB + (threadIdx.x & ~63u)) * DT_CELLS + (threadIdx.x & 63u);
+    double m = 1e100;
 #pragma unroll
-        for (int d = 0; d < DIM; d++) s.v[d] = vel[(size_t)DIM * i + d];
-        const double P = pressure<DIM>(gam, s);
-        const double c = sqrt(gam * P / s.rho);
-        const int32_t si = o2s[i];
-        const double R = cfl_length<DIM>(vol_u[si], si, cm);
-        double va = sqrt(s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]);
-        if (wgen) va = fmax(va, rel_speed<DIM>(s.v, wgen + (size_t)DIM * i));
-        dti = cfl * (R / (c + va));
+    for (int it = 0; it < DT_CELLS; it++) {
+        const int64_t i = base + 64 * it;
+        if (i < n_hydro) m = smin(cell_cfl_dt<DIM>(i, o2s, vol_u, gam, cfl, rho, vel, E, wgen, cm), m);
     }
-    // wave min, then one atomic per wave (non-negative doubles order as bits)
-    for (int d = 32; d > 0; d >>= 1) dti = smin(__shfl_xor(dti, d, 64), dti);
-    if ((threadIdx.x & 63) == 0) {
-        if (dti >= 0.0) atomic_min_pos_double(out, dti);
-        else atomicMin(out, 0ull);  // a negative step (bad state) reports 0
+    // wave min, block min, one atomic per block
+    for (int d = 32; d > 0; d >>= 1) m = smin(__shfl_xor(m, d, 64), m);
+    __shared__ double wm[TB / 64];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int w = 1; w < TB / 64; w++) m = smin(wm[w], m);
+        report_dt(out, m);
     }
 }
 
@@ -606,9 +601,9 @@ void launch_dt_cfl(int dim, int64_t n_hydro, const int32_t* o2s, const double* v
                    unsigned long long* out, hipStream_t s, const double* wgen, CflMesh cm) {
     if (n_hydro <= 0) return;
     if (dim == 2)
-        k_dt_cfl<2><<<nblk(n_hydro), TB, 0, s>>>(n_hydro, o2s, vol_u, gam, cfl, rho, vel, E, out, wgen, cm);
+        k_dt_cfl<2><<<nblk_dt(n_hydro), TB, 0, s>>>(n_hydro, o2s, vol_u, gam, cfl, rho, vel, E, out, wgen, cm);
     else
-        k_dt_cfl<3><<<nblk(n_hydro), TB, 0, s>>>(n_hydro, o2s, vol_u, gam, cfl, rho, vel, E, out, wgen, cm);
+        k_dt_cfl<3><<<nblk_dt(n_hydro), TB, 0, s>>>(n_hydro, o2s, vol_u, gam, cfl, rho, vel, E, out, wgen, cm);
 }
 
 void launch_gen_velocity(int dim, int64_t n_h, const double* x, const int32_t* o2s, const double* vol_u,

@@ -153,6 +153,44 @@ PG_DEV double rel_speed(const double* v, const double* w) {
     return sqrt(s);
 }
 
+// CFL step of hydro cell i (finite_volume_solver.cpp:151-186): CFL number
+// times the cell's CFL length over its sound plus signal speed (with mesh
+// motion, the larger of |v| and |v - w|).  A bad state (negative or NaN
+// step) returns -1, which the minimum then reports as a zero step.
+template <int DIM>
+PG_DEV double cell_cfl_dt(int64_t i, const int32_t* __restrict__ o2s, const double* __restrict__ vol_u,
+                          double gam, double cfl, const double* __restrict__ rho,
+                          const double* __restrict__ vel, const double* __restrict__ E,
+                          const double* __restrict__ wgen, const CflMesh& cm) {
+    Prim s;
+    s.rho = rho[i];
+    s.E = E[i];
+    s.v[2] = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) s.v[d] = vel[(size_t)DIM * i + d];
+    const double P = pressure<DIM>(gam, s);
+    const double c = sqrt(gam * P / s.rho);
+    const int32_t si = o2s[i];
+    const double R = cfl_length<DIM>(vol_u[si], si, cm);
+    double va = sqrt(s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]);
+    if (wgen) va = fmax(va, rel_speed<DIM>(s.v, wgen + (size_t)DIM * i));
+    const double dti = cfl * (R / (c + va));
+    return dti >= 0.0 ? dti : -1.0;
+}
+
+// Minimum step into *out (non-negative doubles order as their bits; a
+// negative minimum, a bad state, reports 0).
+PG_DEV void report_dt(unsigned long long* out, double m) {
+    if (m >= 0.0) atomic_min_pos_double(out, m);
+    else atomicMin(out, 0ull);
+}
+
+// Consecutive cells per lane of the CFL reductions: a wave takes 64 x 4
+// consecutive cells (lane-coalesced loads), and a block of 256 lanes reports
+// once per 1024 cells, so a 1M-cell step makes about 1k atomic updates of
+// its minimum instead of one per wave.
+constexpr int DT_CELLS = 4;
+
 // Flux of one face in the lab frame, per unit area (see k_face_flux): own
 // state si at generator ri moving with gi, neighbour state sj at rj moving
 // with gj (ignored at walls), stored normal n and face centroid fc (both
