@@ -315,9 +315,20 @@ int next_k(int k, int cap) {
     return k < cap ? cap : -1;
 }
 
+// Row length of the 128-slot kernel's pass over overflowed seeds at chain
+// step k: k without restarts, else at least BIG_ROW (capped).  A longer row
+// extends a shorter one and the cell certifies at the same neighbour, so
+// the pass equals one at k, while a seed that would need the next steps of
+// the chain (the fast kernel overflows again at the same clip, then the
+// 128-slot kernel runs again) is done at once: the few overflowed seeds run
+// as one or two waves, latency rather than throughput.  Rows up to 256 stay
+// cheap in the wave KNN kernel.
+constexpr int BIG_ROW = 256;
+int big_row(int k, int cap, bool restart) { return restart ? std::max(k, std::min(cap, BIG_ROW)) : k; }
+
 // Seeds cur[0..nq) (nullptr: all) were just clipped with rows of length k.
 // Seeds that outgrew the fast kernel's vertex slots are recomputed by the
-// 128-slot kernel at the same k; uncertified seeds at the next k, until all
+// 128-slot kernel (at big_row); uncertified seeds at the next k, until all
 // pass or k reaches its cap.  Returns the largest k used.
 int restart_chain(pg_mesh* m, const uint32_t* cur, int64_t nq, int k, hipStream_t s) {
     const pg_mesh_params& p = m->prm;
@@ -331,8 +342,10 @@ int restart_chain(pg_mesh* m, const uint32_t* cur, int64_t nq, int k, hipStream_
     while (nL > 0) {
         const int64_t no = collect_list(m, L, nL, pg::SLOT_OVERFLOW, ov, s);
         if (no > 0) {
-            run_list(m, ov, no, k, 1, s);
+            const int kb = big_row(k, cap, p.restart != 0);
+            run_list(m, ov, no, kb, 1, s);
             m->info.n_restarted += no;
+            kmax = std::max(kmax, kb);
         }
         if (!p.restart) break;
         const int kn = next_k(k, cap);
@@ -1942,9 +1955,11 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
     const uint32_t* L = nullptr;
     const int64_t* nL = n_dev;
     for (size_t r = 1; r <= sched.size(); r++) {
+        // overflowed seeds: the 128-slot kernel at big_row (see restart_chain)
+        const int kbig = big_row(k, sched.back(), p.restart != 0);
         collect(L, nL, pg::SLOT_OVERFLOW, lst[2], cnt[2]);
-        knn(lst[2], cnt[2], k, true);
-        clip(lst[2], cnt[2], k, 1, true);
+        knn(lst[2], cnt[2], kbig, true);
+        clip(lst[2], cnt[2], kbig, 1, true);
         if (r == sched.size() || !p.restart) break;
         const int d = (L == lst[0]) ? 1 : 0;
         collect(L, nL, PG_SECURITY_RADIUS_NOT_REACHED, lst[d], cnt[d]);
