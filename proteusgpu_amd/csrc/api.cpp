@@ -326,30 +326,44 @@ int next_k(int k, int cap) {
 constexpr int BIG_ROW = 256;
 int big_row(int k, int cap, bool restart) { return restart ? std::max(k, std::min(cap, BIG_ROW)) : k; }
 
-// Seeds cur[0..nq) (nullptr: all) were just clipped with rows of length k.
-// Seeds that outgrew the fast kernel's vertex slots are recomputed by the
-// 128-slot kernel (at big_row); uncertified seeds at the next k, until all
-// pass or k reaches its cap.  Returns the largest k used.
+// Seeds cur[0..nq) (nullptr: all) were just clipped with rows of length k;
+// uncertified seeds go on at the next k, until all pass or k reaches its
+// cap.  Seeds that outgrow the fast kernel's vertex slots keep that status
+// through the chain, and the 128-slot kernel clips all of them once, at
+// big_row(k), when the chain's k would pass that row (or the chain ends);
+// after every longer step it runs again, and the next step's list is then
+// taken over all of cur, so that its uncertified seeds continue too.
+// Returns the largest k used.
 int restart_chain(pg_mesh* m, const uint32_t* cur, int64_t nq, int k, hipStream_t s) {
     const pg_mesh_params& p = m->prm;
     const size_t ns = (size_t)m->n_seeds;
     uint32_t* buf[2] = {m->list.get<uint32_t>(ns), m->list2.get<uint32_t>(ns)};
     uint32_t* ov = m->list3.get<uint32_t>(ns);
     const int cap = (int)std::min<int64_t>(std::min(std::max(p.k_max, k), KNN_MAX_K), m->n_seeds - 1);
+    const bool restart = p.restart != 0;
+    const int kb = big_row(k, cap, restart);
     const uint32_t* L = cur;
     int64_t nL = nq;
     int kmax = k;
-    while (nL > 0) {
-        const int64_t no = collect_list(m, L, nL, pg::SLOT_OVERFLOW, ov, s);
+    bool big_done = false;
+    auto big_pass = [&](int kk) -> int64_t {
+        const int64_t no = collect_list(m, cur, nq, pg::SLOT_OVERFLOW, ov, s);
         if (no > 0) {
-            const int kb = big_row(k, cap, p.restart != 0);
-            run_list(m, ov, no, kb, 1, s);
+            run_list(m, ov, no, kk, 1, s);
             m->info.n_restarted += no;
-            kmax = std::max(kmax, kb);
+            kmax = std::max(kmax, kk);
+            L = cur;
+            nL = nq;
         }
-        if (!p.restart) break;
+        return no;
+    };
+    while (restart && nL > 0) {
         const int kn = next_k(k, cap);
         if (kn < 0) break;
+        if (!big_done && kn > kb) {
+            big_pass(kb);
+            big_done = true;
+        }
         uint32_t* dst = (L == buf[0]) ? buf[1] : buf[0];
         const int64_t nr = collect_list(m, L, nL, PG_SECURITY_RADIUS_NOT_REACHED, dst, s);
         if (nr == 0) break;
@@ -359,6 +373,16 @@ int restart_chain(pg_mesh* m, const uint32_t* cur, int64_t nq, int k, hipStream_
         kmax = std::max(kmax, k);
         L = dst;
         nL = nr;
+        if (big_done) big_pass(k);
+    }
+    if (!big_done) {
+        const int64_t no = big_pass(kb);
+        // a 128-slot pass below the cap may leave seeds that longer rows
+        // certify: they continue in a chain from kb (the only uncertified
+        // seeds of cur at this point)
+        if (restart && no > 0 && kb < cap &&
+            collect_list(m, ov, no, PG_SECURITY_RADIUS_NOT_REACHED, buf[0], s) > 0)
+            kmax = std::max(kmax, restart_chain(m, cur, nq, kb, s));
     }
     return kmax;
 }
@@ -1954,13 +1978,24 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
     int64_t* cnt[3] = {B->ncount.as<int64_t>(), B->ncount.as<int64_t>() + 1, B->ncount.as<int64_t>() + 2};
     const uint32_t* L = nullptr;
     const int64_t* nL = n_dev;
-    for (size_t r = 1; r <= sched.size(); r++) {
-        // overflowed seeds: the 128-slot kernel at big_row (see restart_chain)
-        const int kbig = big_row(k, sched.back(), p.restart != 0);
-        collect(L, nL, pg::SLOT_OVERFLOW, lst[2], cnt[2]);
-        knn(lst[2], cnt[2], kbig, true);
-        clip(lst[2], cnt[2], kbig, 1, true);
-        if (r == sched.size() || !p.restart) break;
+    // overflowed seeds keep that status through the chain; the 128-slot
+    // kernel clips all of them once the chain's k would pass big_row, and
+    // after every longer step, whose list is then taken over the whole batch
+    // (see restart_chain)
+    const int kb = big_row(k, sched.back(), p.restart != 0);
+    bool big_done = false;
+    auto big_pass = [&](int kk) {
+        collect(nullptr, n_dev, pg::SLOT_OVERFLOW, lst[2], cnt[2]);
+        knn(lst[2], cnt[2], kk, true);
+        clip(lst[2], cnt[2], kk, 1, true);
+        L = nullptr;
+        nL = n_dev;
+    };
+    for (size_t r = 1; p.restart && r < sched.size(); r++) {
+        if (!big_done && sched[r] > kb) {
+            big_pass(kb);
+            big_done = true;
+        }
         const int d = (L == lst[0]) ? 1 : 0;
         collect(L, nL, PG_SECURITY_RADIUS_NOT_REACHED, lst[d], cnt[d]);
         k = sched[r];
@@ -1968,7 +2003,9 @@ void batch_rebuild(pg_batch* B, hipStream_t s) {
         clip(lst[d], cnt[d], k, 0, true);
         L = lst[d];
         nL = cnt[d];
+        if (big_done) big_pass(k);
     }
+    if (!big_done) big_pass(kb);
     const double sc = B->scale();
     const double as = dim == 2 ? sc : sc * sc, vs = dim == 2 ? sc * sc : sc * sc * sc;
     pg::launch_bfinalize(n_dev, cap, cap, B->perm.as<uint32_t>(), B->orig_h.as<int32_t>(), B->count.as<uint8_t>(),
